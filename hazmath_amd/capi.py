@@ -1,0 +1,261 @@
+"""ctypes mirror of include/hazmath_mi355x.h.
+
+The structures below have the same field order and C types as HAZmath's public
+types (include/sparse.h:25-45, include/vec.h:20-42, include/param.h:138-266), so
+the same Python objects can be handed to the MI355X library and to the CPU
+oracle.  Loading the library is explicit (`load_library`) and fails loudly when
+the built shared object is missing: there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+INT = C.c_int
+SHORT = C.c_short
+REAL = C.c_double
+MAX_AMG_LVL = 20
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "lib" / "libhazmath_mi355x.so"
+
+# Error codes (include/macro.h:38-75)
+SUCCESS = 0
+ERROR_AMG_SMOOTH_TYPE = -31
+ERROR_AMG_AGG_TYPE = -32
+ERROR_AMG_COARSEING = -33
+ERROR_SOLVER_TYPE = -40
+ERROR_SOLVER_STAG = -42
+ERROR_SOLVER_SOLSTAG = -43
+ERROR_SOLVER_TOLSMALL = -44
+ERROR_SOLVER_MAXIT = -48
+ERROR_DEVICE = -90
+
+# Enumerations (include/macro.h:126-221)
+SOLVER_CG = 1
+SOLVER_UMFPACK = 32
+STOP_REL_RES, STOP_REL_PRECRES, STOP_MOD_REL_RES = 1, 2, 3
+PREC_AMG = 2
+UA_AMG = 1
+VMB, HEC, HEM = 1, 4, 5
+V_CYCLE, W_CYCLE = 1, 2
+SMOOTHER_JACOBI, SMOOTHER_GS, SMOOTHER_L1DIAG = 1, 2, 10
+
+
+class dCSRmat(C.Structure):
+    _fields_ = [("row", INT), ("col", INT), ("nnz", INT),
+                ("IA", C.POINTER(INT)), ("JA", C.POINTER(INT)), ("val", C.POINTER(REAL))]
+
+
+class dvector(C.Structure):
+    _fields_ = [("row", INT), ("val", C.POINTER(REAL))]
+
+
+class linear_itsolver_param(C.Structure):
+    _fields_ = [("linear_itsolver_type", SHORT), ("linear_precond_type", SHORT),
+                ("linear_stop_type", SHORT), ("linear_maxit", INT), ("linear_tol", REAL),
+                ("linear_restart", INT), ("linear_print_level", SHORT),
+                ("HX_smooth_iter", SHORT), ("AL_scaling_param", REAL)]
+
+
+class AMG_param(C.Structure):
+    _fields_ = [("AMG_type", SHORT), ("print_level", SHORT), ("maxit", INT), ("tol", REAL),
+                ("max_levels", SHORT), ("coarse_dof", INT), ("cycle_type", SHORT),
+                ("smoother", SHORT), ("presmooth_iter", SHORT), ("postsmooth_iter", SHORT),
+                ("relaxation", REAL), ("polynomial_degree", SHORT), ("coarse_solver", SHORT),
+                ("coarse_scaling", SHORT), ("amli_degree", SHORT), ("amli_coef", C.POINTER(REAL)),
+                ("nl_amli_krylov_type", SHORT), ("aggregation_type", SHORT),
+                ("strong_coupled", REAL), ("max_aggregation", INT), ("smooth_filter", SHORT),
+                ("tentative_smooth", REAL), ("Schwarz_levels", INT), ("Schwarz_mmsize", INT),
+                ("Schwarz_maxlvl", INT), ("Schwarz_type", INT), ("Schwarz_blksolver", INT),
+                ("HAZDIR", C.c_char_p), ("Schwarz_on_blk", C.POINTER(INT)),
+                ("Schwarz_patch_type", C.POINTER(INT)), ("damping_param", REAL),
+                ("BSR_alpha", REAL), ("BSR_omega", REAL), ("fpwr", REAL),
+                ("smoother_function", C.c_void_p)]
+
+
+class hz_options(C.Structure):
+    _fields_ = [("jacobi_weight", REAL), ("hec_rng_source", INT), ("verbose_timing", INT)]
+
+
+class hz_solve_info(C.Structure):
+    _fields_ = [("iterations", INT), ("num_levels", INT),
+                ("level_rows", INT * MAX_AMG_LVL), ("level_nnz", INT * MAX_AMG_LVL),
+                ("setup_ms", REAL), ("solve_ms", REAL), ("total_ms", REAL),
+                ("final_relres", REAL), ("condest", REAL), ("relres_count", INT),
+                ("relres_hist", REAL * 4096)]
+
+
+class hz_hierarchy(C.Structure):
+    _fields_ = [("num_levels", INT), ("A", dCSRmat * MAX_AMG_LVL),
+                ("aggregates", C.POINTER(INT) * MAX_AMG_LVL),
+                ("num_aggregates", INT * MAX_AMG_LVL), ("theta", REAL * MAX_AMG_LVL)]
+
+
+class orc_trace(C.Structure):
+    _fields_ = [("num_levels", INT), ("level_rows", INT * MAX_AMG_LVL),
+                ("level_nnz", INT * MAX_AMG_LVL), ("theta", REAL * MAX_AMG_LVL),
+                ("iterations", INT), ("condest", REAL), ("final_relres", REAL),
+                ("relres_count", INT), ("relres_hist", REAL * 4096),
+                ("setup_s", REAL), ("solve_s", REAL)]
+
+
+# ---------------------------------------------------------------------------
+# numpy <-> struct helpers (the numpy arrays must outlive the structs)
+# ---------------------------------------------------------------------------
+class HostCSR:
+    """Keeps the numpy arrays of a CSR matrix alive next to its dCSRmat view."""
+
+    def __init__(self, ia, ja, val, ncol=None):
+        self.ia = np.ascontiguousarray(ia, dtype=np.int32)
+        self.ja = np.ascontiguousarray(ja, dtype=np.int32)
+        self.val = np.ascontiguousarray(val, dtype=np.float64)
+        n = self.ia.size - 1
+        self.mat = dCSRmat(n, n if ncol is None else ncol, int(self.ia[-1]),
+                           self.ia.ctypes.data_as(C.POINTER(INT)),
+                           self.ja.ctypes.data_as(C.POINTER(INT)),
+                           self.val.ctypes.data_as(C.POINTER(REAL)))
+
+    @property
+    def n(self):
+        return self.ia.size - 1
+
+    @property
+    def nnz(self):
+        return int(self.ia[-1])
+
+    def ptr(self):
+        return C.byref(self.mat)
+
+
+class HostVec:
+    def __init__(self, arr):
+        self.arr = np.ascontiguousarray(arr, dtype=np.float64)
+        self.vec = dvector(self.arr.size, self.arr.ctypes.data_as(C.POINTER(REAL)))
+
+    def ptr(self):
+        return C.byref(self.vec)
+
+
+def csr_to_numpy(m: dCSRmat):
+    n, nnz = m.row, m.nnz
+    ia = np.ctypeslib.as_array(m.IA, shape=(n + 1,)).copy() if n > 0 else np.zeros(1, np.int32)
+    ja = np.ctypeslib.as_array(m.JA, shape=(nnz,)).copy() if nnz > 0 else np.zeros(0, np.int32)
+    val = np.ctypeslib.as_array(m.val, shape=(nnz,)).copy() if nnz > 0 else np.zeros(0)
+    return ia, ja, val
+
+
+def hierarchy_to_numpy(h: hz_hierarchy):
+    """Returns a list of per-level dicts with A (ia, ja, val), aggregates, theta."""
+    out = []
+    for level in range(h.num_levels):
+        ia, ja, val = csr_to_numpy(h.A[level])
+        d = {"ia": ia, "ja": ja, "val": val, "theta": h.theta[level]}
+        if level < h.num_levels - 1 and bool(h.aggregates[level]):
+            d["agg"] = np.ctypeslib.as_array(h.aggregates[level], shape=(ia.size - 1,)).copy()
+            d["nagg"] = h.num_aggregates[level]
+        out.append(d)
+    return out
+
+
+def default_amg_param(lib=None) -> AMG_param:
+    p = AMG_param()
+    (lib or load_library()).param_amg_init(C.byref(p))
+    return p
+
+
+def default_itsolver_param(lib=None) -> linear_itsolver_param:
+    p = linear_itsolver_param()
+    (lib or load_library()).param_linear_solver_init(C.byref(p))
+    return p
+
+
+def solver_params(lib, *, tol=1e-8, maxit=500, aggregation=HEC, smoother=SMOOTHER_JACOBI,
+                  coarse_dof=100, coarse_solver=SOLVER_UMFPACK, max_levels=20, theta=0.0,
+                  coarse_scaling=0, cycle=V_CYCLE, print_level=0, max_aggregation=20,
+                  stop_type=STOP_REL_RES, amg_maxit=1):
+    """The UA-AMG-PCG settings of the reference's examples/solvers harness."""
+    it = linear_itsolver_param()
+    amg = AMG_param()
+    lib.param_linear_solver_init(C.byref(it))
+    lib.param_amg_init(C.byref(amg))
+    it.linear_itsolver_type = SOLVER_CG
+    it.linear_precond_type = PREC_AMG
+    it.linear_stop_type = stop_type
+    it.linear_maxit = maxit
+    it.linear_tol = tol
+    it.linear_print_level = print_level
+    amg.AMG_type = UA_AMG
+    amg.cycle_type = cycle
+    amg.max_levels = max_levels
+    amg.coarse_dof = coarse_dof
+    amg.coarse_solver = coarse_solver
+    amg.coarse_scaling = coarse_scaling
+    amg.smoother = smoother
+    amg.aggregation_type = aggregation
+    amg.strong_coupled = theta
+    amg.max_aggregation = max_aggregation
+    amg.maxit = amg_maxit
+    amg.print_level = print_level
+    return it, amg
+
+
+_LIB = None
+
+
+def _declare(lib):
+    P = C.POINTER
+    lib.linear_solver_dcsr_krylov_amg.argtypes = [P(dCSRmat), P(dvector), P(dvector),
+                                                  P(linear_itsolver_param), P(AMG_param)]
+    lib.linear_solver_dcsr_krylov_amg.restype = INT
+    lib.param_amg_init.argtypes = [P(AMG_param)]
+    lib.param_amg_init.restype = None
+    lib.param_linear_solver_init.argtypes = [P(linear_itsolver_param)]
+    lib.param_linear_solver_init.restype = None
+    lib.hz_get_options.argtypes = [P(hz_options)]
+    lib.hz_set_options.argtypes = [P(hz_options)]
+    lib.hz_get_last_info.argtypes = [P(hz_solve_info)]
+    lib.hz_krylov_amg_device.argtypes = [INT, INT, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_void_p, P(linear_itsolver_param),
+                                         P(AMG_param)]
+    lib.hz_krylov_amg_device.restype = INT
+    lib.hz_setup_hierarchy.argtypes = [P(dCSRmat), P(AMG_param), P(hz_hierarchy)]
+    lib.hz_setup_hierarchy.restype = INT
+    lib.hz_hierarchy_free.argtypes = [P(hz_hierarchy)]
+    lib.hz_dcsr_mxv.argtypes = [P(dCSRmat), P(REAL), P(REAL)]
+    lib.hz_dcsr_mxv.restype = INT
+    lib.hz_dcsr_residual.argtypes = [P(dCSRmat), P(REAL), P(REAL), P(REAL)]
+    lib.hz_dcsr_residual.restype = INT
+    lib.hz_smoother_sweep.argtypes = [P(dCSRmat), P(REAL), P(REAL), INT, REAL]
+    lib.hz_smoother_sweep.restype = INT
+    lib.hz_rap_agg.argtypes = [P(dCSRmat), P(INT), INT, P(dCSRmat)]
+    lib.hz_rap_agg.restype = INT
+    lib.hz_version.restype = C.c_char_p
+    lib.hz_device_count.restype = INT
+    lib.dcsr_free.argtypes = [P(dCSRmat)]
+
+
+def load_library(path: os.PathLike | None = None):
+    """Load libhazmath_mi355x.so (built in-tree by __graft_entry__.build())."""
+    global _LIB
+    if _LIB is not None and path is None:
+        return _LIB
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(f"{p} is not built; run __graft_entry__.build() first")
+    lib = C.CDLL(str(p), mode=C.RTLD_LOCAL)
+    _declare(lib)
+    if path is None:
+        _LIB = lib
+    return lib
+
+
+def dptr(arr: np.ndarray, ctype=REAL):
+    return arr.ctypes.data_as(C.POINTER(ctype))
+
+
+def libc():
+    return C.CDLL(None)

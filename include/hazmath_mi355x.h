@@ -1,0 +1,259 @@
+/*
+ * hazmath_mi355x.h -- public C ABI of the MI355X UA-AMG-PCG library.
+ *
+ * The types below are laid out field-for-field like HAZmath's public types so
+ * that a caller compiled against HAZmath can link against this library
+ * unchanged (same field names, types and order; INT=int, SHORT=short,
+ * REAL=double).  Every entry point cites the HAZmath function it replaces.
+ *
+ * The solver entry points take HOST arrays (HAZmath's contract): A, b, x live
+ * in host memory; the library uploads them, runs setup and PCG on the GPU, and
+ * writes the solution back into x->val.  A device-resident variant
+ * (hz_krylov_amg_device) takes device pointers for callers whose data already
+ * lives in HBM.
+ */
+#ifndef HAZMATH_MI355X_H
+#define HAZMATH_MI355X_H
+
+#include <stdio.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Scalar types: include/macro.h:102-111 (SHORT short, INT int, REAL double). */
+#ifndef SHORT
+#define SHORT short
+#endif
+#ifndef INT
+#define INT int
+#endif
+#ifndef REAL
+#define REAL double
+#endif
+
+/* Constants with the reference's values: include/macro.h. */
+#define HZ_SUCCESS                0
+#define HZ_ERROR_OPEN_FILE      -10
+#define HZ_ERROR_WRONG_FILE     -11
+#define HZ_ERROR_INPUT_PAR      -13
+#define HZ_ERROR_ALLOC_MEM      -20
+#define HZ_ERROR_DATA_STRUCTURE -21
+#define HZ_ERROR_AMG_SMOOTH_TYPE -31
+#define HZ_ERROR_AMG_AGG_TYPE   -32
+#define HZ_ERROR_AMG_COARSEING  -33
+#define HZ_ERROR_SOLVER_TYPE    -40
+#define HZ_ERROR_SOLVER_STAG    -42
+#define HZ_ERROR_SOLVER_SOLSTAG -43
+#define HZ_ERROR_SOLVER_TOLSMALL -44
+#define HZ_ERROR_SOLVER_MISC    -47
+#define HZ_ERROR_SOLVER_MAXIT   -48
+#define HZ_ERROR_DEVICE         -90   /* no counterpart: HIP runtime failure */
+
+/* include/macro.h:126-221 */
+#define HZ_SOLVER_DEFAULT   0
+#define HZ_SOLVER_CG        1
+#define HZ_SOLVER_UMFPACK  32
+#define HZ_STOP_REL_RES     1
+#define HZ_STOP_REL_PRECRES 2
+#define HZ_STOP_MOD_REL_RES 3
+#define HZ_PREC_AMG         2
+#define HZ_UA_AMG           1
+#define HZ_SA_AMG           2
+#define HZ_VMB              1
+#define HZ_HEC              4
+#define HZ_HEM              5
+#define HZ_V_CYCLE          1
+#define HZ_W_CYCLE          2
+#define HZ_SMOOTHER_JACOBI  1
+#define HZ_SMOOTHER_GS      2
+#define HZ_SMOOTHER_L1DIAG 10
+#define HZ_UNPT            -1
+#define HZ_MAX_AMG_LVL     20
+
+/* dCSRmat: include/sparse.h:25-45 (0-based CSR). */
+typedef struct dCSRmat {
+    INT row;
+    INT col;
+    INT nnz;
+    INT *IA;
+    INT *JA;
+    REAL *val;
+} dCSRmat;
+
+/* dvector / ivector: include/vec.h:20-42. */
+typedef struct dvector {
+    INT row;
+    REAL *val;
+} dvector;
+
+typedef struct ivector {
+    INT row;
+    INT *val;
+} ivector;
+
+/* linear_itsolver_param: include/param.h:138-154. */
+typedef struct {
+    SHORT linear_itsolver_type;
+    SHORT linear_precond_type;
+    SHORT linear_stop_type;
+    INT   linear_maxit;
+    REAL  linear_tol;
+    INT   linear_restart;
+    SHORT linear_print_level;
+    SHORT HX_smooth_iter;
+    REAL  AL_scaling_param;
+} linear_itsolver_param;
+
+/* AMG_param: include/param.h:163-266. */
+typedef struct {
+    SHORT AMG_type;
+    SHORT print_level;
+    INT   maxit;
+    REAL  tol;
+    SHORT max_levels;
+    INT   coarse_dof;
+    SHORT cycle_type;
+    SHORT smoother;
+    SHORT presmooth_iter;
+    SHORT postsmooth_iter;
+    REAL  relaxation;
+    SHORT polynomial_degree;
+    SHORT coarse_solver;
+    SHORT coarse_scaling;
+    SHORT amli_degree;
+    REAL *amli_coef;
+    SHORT nl_amli_krylov_type;
+    SHORT aggregation_type;
+    REAL  strong_coupled;
+    INT   max_aggregation;
+    SHORT smooth_filter;
+    REAL  tentative_smooth;
+    INT   Schwarz_levels;
+    INT   Schwarz_mmsize;
+    INT   Schwarz_maxlvl;
+    INT   Schwarz_type;
+    INT   Schwarz_blksolver;
+    char *HAZDIR;
+    INT  *Schwarz_on_blk;
+    INT  *Schwarz_patch_type;
+    REAL  damping_param;
+    REAL  BSR_alpha;
+    REAL  BSR_omega;
+    REAL  fpwr;
+    void *smoother_function;
+} AMG_param;
+
+/* ---------------------------------------------------------------------------
+ * Drop-in entry points (same prototypes as HAZmath).
+ * ------------------------------------------------------------------------- */
+
+/* Replaces linear_solver_dcsr_krylov_amg, src/solver/itsolver.c:846-931.
+ * Returns the PCG iteration count (>=0) or a negative HAZmath error code.
+ * x->val is overwritten with the solution; amgparam->strong_coupled is
+ * modified per level exactly like src/solver/amg_setup_ua.c:1325-1328; HEC
+ * aggregation draws from libc rand() like src/utilities/array.c:302-318. */
+INT linear_solver_dcsr_krylov_amg(dCSRmat *A, dvector *b, dvector *x,
+                                  linear_itsolver_param *itparam,
+                                  AMG_param *amgparam);
+
+/* Replaces param_amg_init, src/utilities/parameters.c:158-205. */
+void param_amg_init(AMG_param *amgparam);
+
+/* Replaces param_linear_solver_init, src/utilities/parameters.c:238-251. */
+void param_linear_solver_init(linear_itsolver_param *itsparam);
+
+/* Replace dcsr_create / dcsr_free, src/utilities/sparse.c:27-57, 573-592. */
+dCSRmat dcsr_create(const INT m, const INT n, const INT nnz);
+void dcsr_free(dCSRmat *A);
+
+/* Replace dvec_create / dvec_free / dvec_set, src/utilities/vec.c:68-79, 157-163, 280-295. */
+dvector dvec_create(const INT m);
+void dvec_free(dvector *u);
+void dvec_set(INT n, dvector *x, const REAL val);
+
+/* Replace dcoo_read_dcsr_p / dvector_read_p, src/utilities/io.c:2296-2346, 2427-2445
+ * ("m n nnz" header then "i j a_ij" lines, 0-based; row-stable COO->CSR like
+ * src/utilities/format.c:785-827). Returned pointers are freed with
+ * hz_free_dcsr_p / hz_free_dvector_p (or by free()-ing arrays then struct). */
+dCSRmat *dcoo_read_dcsr_p(FILE *fp);
+dvector *dvector_read_p(FILE *fp);
+
+/* ---------------------------------------------------------------------------
+ * Build-owned extensions (no HAZmath counterpart).
+ * ------------------------------------------------------------------------- */
+
+/* Side options that AMG_param cannot express without changing its layout.
+ * Process-global; set before calling the solver. */
+typedef struct {
+    REAL jacobi_weight;    /* damping w of SMOOTHER_JACOBI; the reference fixes w=1.0
+                              (src/solver/smoother.c:47).  1.0 = reference behaviour. */
+    INT  hec_rng_source;   /* 0: libc rand() drawn on the host (reference order);
+                              1: glibc-compatible stream generated on the GPU from the
+                                 current libc state (same values; state advanced after). */
+    INT  verbose_timing;   /* >0: print setup/solve wall times from hipEvents */
+} hz_options;
+
+void hz_get_options(hz_options *opt);
+void hz_set_options(const hz_options *opt);
+
+/* Per-solve statistics of the last call on this thread. */
+typedef struct {
+    INT    iterations;        /* return value of the PCG */
+    INT    num_levels;
+    INT    level_rows[HZ_MAX_AMG_LVL];
+    INT    level_nnz[HZ_MAX_AMG_LVL];
+    REAL   setup_ms;          /* GPU wall time of the AMG setup */
+    REAL   solve_ms;          /* GPU wall time of the PCG */
+    REAL   total_ms;          /* setup + solve (device-resident inputs) */
+    REAL   final_relres;
+    REAL   condest;
+    INT    relres_count;      /* entries valid in relres_hist (<= 4096) */
+    REAL   relres_hist[4096]; /* relres after each PCG iteration */
+} hz_solve_info;
+
+void hz_get_last_info(hz_solve_info *info);
+
+/* Device-resident drop-in: same semantics as linear_solver_dcsr_krylov_amg
+ * but IA/JA/val, b and x are device pointers (hipMalloc'd).  Timed region of
+ * the benchmark.  Returns iterations or a negative error code. */
+INT hz_krylov_amg_device(INT n, INT nnz, const INT *dIA, const INT *dJA,
+                         const REAL *dval, const REAL *db, REAL *dx,
+                         linear_itsolver_param *itparam, AMG_param *amgparam);
+
+/* Hierarchy export for parity checks: runs only the AMG setup of
+ * linear_solver_dcsr_krylov_amg on the GPU and copies level data to host
+ * buffers allocated with malloc (release with hz_hierarchy_free).  aggregates[l]
+ * has level_rows[l] entries (vertex -> aggregate id, -1 = UNPT). */
+typedef struct {
+    INT num_levels;
+    dCSRmat A[HZ_MAX_AMG_LVL];
+    INT *aggregates[HZ_MAX_AMG_LVL];
+    INT num_aggregates[HZ_MAX_AMG_LVL];
+    REAL theta[HZ_MAX_AMG_LVL];   /* strong_coupled in force when level l was aggregated */
+} hz_hierarchy;
+
+INT hz_setup_hierarchy(dCSRmat *A, AMG_param *amgparam, hz_hierarchy *out);
+void hz_hierarchy_free(hz_hierarchy *h);
+
+/* Single-kernel parity hooks on host arrays (upload, run one kernel, download).
+ * y = A x  (dcsr_mxv, src/utilities/sparse.c:1456-1590). */
+INT hz_dcsr_mxv(dCSRmat *A, const REAL *x, REAL *y);
+/* y = y - A x ... residual form of dcsr_aAxpy(-1.0), sparse.c:1693-1730. */
+INT hz_dcsr_residual(dCSRmat *A, const REAL *b, const REAL *x, REAL *w);
+/* One Jacobi (smoother=1, weight w) or L1DIAG (smoother=10) sweep in place,
+ * smoother.c:35-92 / 324-381. */
+INT hz_smoother_sweep(dCSRmat *A, const REAL *b, REAL *x, INT smoother, REAL w);
+/* Coarse product dcsr_rap_agg(R,A,P) with R=P^T built from aggregates
+ * (sparse.c:2557-2683, sparse.c:770-830). Result arrays are malloc'd. */
+INT hz_rap_agg(dCSRmat *A, const INT *aggregates, INT num_aggregates, dCSRmat *Ac);
+
+/* Runtime information. */
+const char *hz_version(void);
+INT hz_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HAZMATH_MI355X_H */

@@ -1,0 +1,65 @@
+/*
+ * hz_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * Serial CPU restatement of HAZmath's unsmoothed-aggregation AMG-preconditioned
+ * CG path.  Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg
+ * may load it; the product library never links or calls it.
+ *
+ * Every entry point mirrors one reference function (file:line in hz_oracle.c).
+ * Function names carry the prefix orc_ so the oracle can share a process with
+ * the product library (which exports the unprefixed HAZmath names).
+ */
+#ifndef HZ_ORACLE_H
+#define HZ_ORACLE_H
+
+#include "hazmath_mi355x.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_EXIT_HEC   (-1016)  /* reference calls exit(16) (amg_setup_ua.c:820-823) */
+#define ORC_EXIT_HEM   (-1017)  /* reference calls exit(17) (amg_setup_ua.c:1031-1033) */
+#define ORC_EXIT_PARAM (-1013)  /* reference calls check_error -> exit (message.c:191-282) */
+
+/* Trace of one solve (filled when non-NULL). */
+typedef struct {
+    INT  num_levels;
+    INT  level_rows[HZ_MAX_AMG_LVL];
+    INT  level_nnz[HZ_MAX_AMG_LVL];
+    REAL theta[HZ_MAX_AMG_LVL];
+    INT  iterations;
+    REAL condest;
+    REAL final_relres;
+    INT  relres_count;
+    REAL relres_hist[4096];
+    REAL setup_s;   /* CLOCK_MONOTONIC wall seconds */
+    REAL solve_s;
+} orc_trace;
+
+INT  orc_krylov_amg(dCSRmat *A, dvector *b, dvector *x,
+                    linear_itsolver_param *itparam, AMG_param *amgparam,
+                    orc_trace *trace);
+void orc_param_amg_init(AMG_param *p);
+void orc_param_linear_solver_init(linear_itsolver_param *p);
+
+/* Setup only, exporting the hierarchy (host arrays; free with orc_hierarchy_free). */
+INT  orc_setup_hierarchy(dCSRmat *A, AMG_param *amgparam, hz_hierarchy *out);
+void orc_hierarchy_free(hz_hierarchy *h);
+
+/* Kernel-level restatements. */
+void orc_dcsr_mxv(const dCSRmat *A, const REAL *x, REAL *y);
+void orc_dcsr_residual(const dCSRmat *A, const REAL *b, const REAL *x, REAL *w);
+void orc_smoother_sweep(const dCSRmat *A, const REAL *b, REAL *x, INT smoother, REAL w);
+INT  orc_rap_agg(const dCSRmat *A, const INT *agg, INT nagg, dCSRmat *Ac);
+void orc_strength(const dCSRmat *A, REAL theta, dCSRmat *N);
+INT  orc_aggregate(const dCSRmat *A, INT type, REAL theta, INT max_aggregation,
+                   INT *agg, INT *nagg);
+REAL orc_tridiag_condest(INT n, const REAL *alpha, const REAL *beta);
+/* Jacobi damping used by the V-cycle (reference: fixed 1.0, smoother.c:47). */
+void orc_set_jacobi_weight(REAL w);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
