@@ -1,0 +1,250 @@
+#!/usr/bin/env python3
+"""Benchmark: UA-AMG-preconditioned CG time to 1e-10 on the 3-D Kuhn P1 Poisson
+problem with 128^3 cells (n = 2,048,383, nnz = 30,340,171, FP64), one MI355X.
+
+A step is one call of the device-resident drop-in (hz_krylov_amg_device): AMG
+setup on the GPU plus PCG to relative residual 1e-10 from x0 = 0, inputs already
+resident in HBM.  Prints ONE JSON line (rank 0).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+from hazmath_amd import capi, problems  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, 8 TB/s
+
+AGG = {"hem": capi.HEM, "hec": capi.HEC, "vmb": capi.VMB}
+SMOOTHER = {"jacobi": capi.SMOOTHER_JACOBI, "l1": capi.SMOOTHER_L1DIAG}
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=5)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--m", type=int, default=128, help="cells per direction")
+    p.add_argument("--agg", choices=AGG, default="hem")
+    p.add_argument("--smoother", choices=SMOOTHER, default="l1")
+    p.add_argument("--jacobi-weight", type=float, default=1.0)
+    p.add_argument("--coarse-dof", type=int, default=200)
+    p.add_argument("--theta", type=float, default=0.04)
+    p.add_argument("--tol", type=float, default=1e-10)
+    p.add_argument("--kernel-reps", type=int, default=50)
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--cpu-sample-m", type=int, default=128,
+                   help="cells per direction of the CPU-baseline sample")
+    return p.parse_args()
+
+
+def set_params(lib, a):
+    it, amg = capi.solver_params(lib, tol=a.tol, maxit=1000, aggregation=AGG[a.agg],
+                                 smoother=SMOOTHER[a.smoother], coarse_dof=a.coarse_dof,
+                                 coarse_solver=capi.SOLVER_UMFPACK, max_levels=20, theta=a.theta)
+    return it, amg
+
+
+def cpu_baseline(a, ia, ja, va, b):
+    """Serial C restatement of the reference path (oracle/liboracle.so), 1 core."""
+    import oracle
+    orc = oracle.load()
+    orc.orc_set_jacobi_weight(a.jacobi_weight)
+    libc = C.CDLL(None)
+    it, amg = oracle.params(tol=a.tol, maxit=1000, aggregation=AGG[a.agg],
+                            smoother=SMOOTHER[a.smoother], coarse_dof=a.coarse_dof,
+                            coarse_solver=32, max_levels=20, theta=a.theta)
+    A = capi.HostCSR(ia, ja, va)
+    x = capi.HostVec(np.zeros(b.size))
+    bb = capi.HostVec(b)
+    tr = capi.orc_trace()
+    libc.srand(1)
+    devnull = os.open(os.devnull, os.O_WRONLY)
+    saved = os.dup(1)
+    sys.stdout.flush()
+    os.dup2(devnull, 1)
+    try:
+        t0 = time.perf_counter()
+        r = orc.orc_krylov_amg(A.ptr(), bb.ptr(), x.ptr(), C.byref(it), C.byref(amg), C.byref(tr))
+        t = time.perf_counter() - t0
+    finally:
+        os.dup2(saved, 1)
+        os.close(devnull)
+    return t, r, tr, x.arr
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py: this build targets one GPU; --gpus {a.gpus} is not supported "
+              "(the single-GPU throughput is the reported value)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    lib = capi.load_library()
+    if lib.hz_device_count() < 1 or not torch.cuda.is_available():
+        print("bench.py: no HIP device", file=sys.stderr)
+        sys.exit(1)
+    opts = capi.hz_options()
+    lib.hz_get_options(C.byref(opts))
+    opts.jacobi_weight = a.jacobi_weight
+    opts.return_on_fatal = 1
+    lib.hz_set_options(C.byref(opts))
+
+    t_gen = time.perf_counter()
+    ia, ja, va = problems.poisson3d_kuhn(a.m)
+    n, nnz = ia.size - 1, int(ia[-1])
+    b = problems.uniform_rhs(n, seed=1)
+    t_gen = time.perf_counter() - t_gen
+    print(f"[bench] generated n={n} nnz={nnz} in {t_gen:.1f}s", file=sys.stderr, flush=True)
+
+    dev = torch.device("cuda:0")
+    dia = torch.from_numpy(ia).to(dev)
+    dja = torch.from_numpy(ja).to(dev)
+    dva = torch.from_numpy(va).to(dev)
+    db = torch.from_numpy(b).to(dev)
+    dx = torch.zeros(n, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    libc = C.CDLL(None)
+
+    def step():
+        dx.zero_()
+        torch.cuda.synchronize()
+        it, amg = set_params(lib, a)
+        libc.srand(1)
+        r = lib.hz_krylov_amg_device(n, nnz, dia.data_ptr(), dja.data_ptr(), dva.data_ptr(),
+                                     db.data_ptr(), dx.data_ptr(), C.byref(it), C.byref(amg))
+        if r < 0:
+            raise RuntimeError(f"solver returned {r}")
+        return r
+
+    devnull = os.open(os.devnull, os.O_WRONLY)
+    saved = os.dup(1)
+
+    def quiet(fn):
+        sys.stdout.flush()
+        os.dup2(devnull, 1)
+        try:
+            return fn()
+        finally:
+            os.dup2(saved, 1)
+
+    for w in range(a.warmup):
+        r = quiet(step)
+        print(f"[bench] warmup {w}: iterations {r}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    infos = []
+    t0 = time.perf_counter()
+    for s in range(a.steps):
+        r = quiet(step)
+        info = capi.hz_solve_info()
+        lib.hz_get_last_info(C.byref(info))
+        infos.append((r, info.setup_ms, info.solve_ms, info.total_ms, info.num_levels,
+                      info.final_relres, list(info.level_rows[:info.num_levels]),
+                      list(info.level_nnz[:info.num_levels])))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = 1e3 * wall / a.steps
+    print(f"[bench] {a.steps} steps, {ms_per_step:.2f} ms/step", file=sys.stderr, flush=True)
+
+    # true relative residual of the last solution (FP64, on the CPU)
+    x = dx.cpu().numpy()
+    import scipy.sparse as sp
+    Asp = sp.csr_matrix((va, ja, ia), shape=(n, n))
+    true_rel = float(np.linalg.norm(b - Asp @ x) / np.linalg.norm(b))
+
+    # fine-level kernels: live HIP-event timing on the library's stream
+    kt = capi.hz_kernel_times()
+    rc = lib.hz_time_fine_kernels(n, nnz, dia.data_ptr(), dja.data_ptr(), dva.data_ptr(),
+                                  a.kernel_reps, C.byref(kt))
+    if rc != 0:
+        raise RuntimeError("hz_time_fine_kernels failed")
+    # algorithmic bytes: 12 nnz (val + col) + 4 (n+1) (row pointers) + 8 n (x) + 8 n (y)
+    bytes_spmv = 12 * nnz + 4 * (n + 1) + 16 * n
+    bytes_jacobi = 12 * nnz + 4 * (n + 1) + 24 * n
+    spmv_gbs = bytes_spmv / (kt.spmv_ms * 1e-3) / 1e9
+    jac_gbs = bytes_jacobi / (kt.jacobi_ms * 1e-3) / 1e9
+
+    iters = [i[0] for i in infos]
+    value = float(np.mean([i[3] for i in infos])) * 1e-3   # device time per solve (s)
+    cpu = None
+    if not a.no_cpu_baseline:
+        if a.cpu_sample_m == a.m:
+            ci, cj, cv, cb = ia, ja, va, b
+        else:
+            ci, cj, cv = problems.poisson3d_kuhn(a.cpu_sample_m)
+            cb = problems.uniform_rhs(ci.size - 1, seed=1)
+        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
+        t_cpu, r_cpu, tr, x_cpu = cpu_baseline(a, ci, cj, cv, cb)
+        print(f"[bench] cpu baseline {t_cpu:.2f}s, iterations {r_cpu}", file=sys.stderr, flush=True)
+        sample = (f"one full solve (setup + PCG to {a.tol:g}) of the same workload with "
+                  f"{a.cpu_sample_m}^3 cells, serial C restatement of the reference path, "
+                  f"{r_cpu} iterations")
+        scale = 1.0 if a.cpu_sample_m == a.m else (a.m / a.cpu_sample_m) ** 3
+        cpu = {"value": t_cpu * scale, "unit": "s", "cores": 1, "kind": "port", "sample": sample,
+               "iterations": int(r_cpu), "setup_s": tr.setup_s, "solve_s": tr.solve_s,
+               "host_cpu": os.uname().machine, "host_nproc": os.cpu_count()}
+        if a.cpu_sample_m == a.m:
+            cpu["x_rel_diff_vs_gpu"] = float(np.linalg.norm(x_cpu - x) / np.linalg.norm(x_cpu))
+
+    line = {
+        "metric": "AMG-PCG time-to-1e-10 rel. residual (s) + SpMV GB/s, 128³ P1; learned vs plain",
+        "value": value,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"3-D Kuhn P1 Poisson, {a.m}^3 cells, n={n}, nnz={nnz} (15-point rows, "
+                         f"zeros stored), seeded U[-1,1) RHS, x0=0, UA-AMG ({a.agg.upper()} "
+                         f"aggregation, theta={a.theta}, V(1,1) {a.smoother} w={a.jacobi_weight}, "
+                         f"coarse_dof={a.coarse_dof}, dense coarsest solve) PCG to {a.tol:g}, "
+                         "plain FP64 drop-in"),
+            "levels": infos[-1][4],
+            "level_rows": infos[-1][6],
+            "level_nnz": infos[-1][7],
+        },
+        "iterations": iters[-1],
+        "setup_ms": float(np.mean([i[1] for i in infos])),
+        "solve_ms": float(np.mean([i[2] for i in infos])),
+        "true_rel_residual": true_rel,
+        "spmv_gbs": spmv_gbs,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "SELL-64 SpMV y=Ax, fine level",
+            "achieved": spmv_gbs,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": spmv_gbs / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": bytes_spmv,
+            "avg_launch_ms": kt.spmv_ms,
+        },
+        "kernels": {
+            "spmv_ms": kt.spmv_ms, "spmv_dot_ms": kt.spmv_dot_ms, "residual_ms": kt.residual_ms,
+            "jacobi_ms": kt.jacobi_ms, "jacobi_gbs": jac_gbs, "sell_entries": kt.sell_entries,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -78,7 +78,8 @@ class AMG_param(C.Structure):
 
 
 class hz_options(C.Structure):
-    _fields_ = [("jacobi_weight", REAL), ("hec_rng_source", INT), ("verbose_timing", INT)]
+    _fields_ = [("jacobi_weight", REAL), ("hec_rng_source", INT), ("verbose_timing", INT),
+                ("return_on_fatal", INT)]
 
 
 class hz_solve_info(C.Structure):
@@ -93,6 +94,11 @@ class hz_hierarchy(C.Structure):
     _fields_ = [("num_levels", INT), ("A", dCSRmat * MAX_AMG_LVL),
                 ("aggregates", C.POINTER(INT) * MAX_AMG_LVL),
                 ("num_aggregates", INT * MAX_AMG_LVL), ("theta", REAL * MAX_AMG_LVL)]
+
+
+class hz_kernel_times(C.Structure):
+    _fields_ = [("reps", INT), ("spmv_ms", REAL), ("residual_ms", REAL), ("jacobi_ms", REAL),
+                ("spmv_dot_ms", REAL), ("sell_entries", C.c_longlong)]
 
 
 class orc_trace(C.Structure):
@@ -233,6 +239,9 @@ def _declare(lib):
     lib.hz_smoother_sweep.restype = INT
     lib.hz_rap_agg.argtypes = [P(dCSRmat), P(INT), INT, P(dCSRmat)]
     lib.hz_rap_agg.restype = INT
+    lib.hz_time_fine_kernels.argtypes = [INT, INT, C.c_void_p, C.c_void_p, C.c_void_p, INT,
+                                         P(hz_kernel_times)]
+    lib.hz_time_fine_kernels.restype = INT
     lib.hz_version.restype = C.c_char_p
     lib.hz_device_count.restype = INT
     lib.dcsr_free.argtypes = [P(dCSRmat)]

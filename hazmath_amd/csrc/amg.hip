@@ -1,0 +1,597 @@
+// amg.hip -- AMG setup loop, V/W cycle and PCG driver on the device.
+//
+// Control flow follows the reference line by line:
+//   setup  : amg_setup_unsmoothP_unsmoothR  (src/solver/amg_setup_ua.c:1228-1443)
+//   cycle  : mgcycle                        (src/solver/mgcycle.c:245-409)
+//   precond: precond_amg                    (src/solver/precond.c:164-183)
+//   PCG    : dcsr_pcg_w_cond_est            (src/solver/krylov.c:400-718)
+// Every vector operation runs on the GPU; the host only holds the scalar
+// control flow (one 256-byte device->host copy per PCG iteration).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "amg.hpp"
+
+namespace hz {
+
+// ---------------------------------------------------------------------------
+// Arena / stream
+// ---------------------------------------------------------------------------
+void *Arena::alloc(size_t bytes) {
+    bytes = (bytes + 255) & ~size_t(255);
+    if (bytes == 0) bytes = 256;
+    while (cur_ < chunks_.size()) {
+        Chunk &c = chunks_[cur_];
+        if (c.used + bytes <= c.size) {
+            void *p = c.base + c.used;
+            c.used += bytes;
+            return p;
+        }
+        ++cur_;
+    }
+    const size_t sz = std::max(bytes, size_t(256) << 20);
+    char *p = nullptr;
+    HZ_CHECK(hipMalloc(&p, sz));
+    chunks_.push_back({p, sz, bytes});
+    cur_ = chunks_.size() - 1;
+    return p;
+}
+
+void Arena::reset() {
+    for (auto &c : chunks_) c.used = 0;
+    cur_ = 0;
+}
+
+void Arena::release() {
+    for (auto &c : chunks_) (void)hipFree(c.base);
+    chunks_.clear();
+    cur_ = 0;
+}
+
+size_t Arena::bytes_in_use() const {
+    size_t s = 0;
+    for (auto &c : chunks_) s += c.used;
+    return s;
+}
+
+Arena &arena() {
+    static Arena a;
+    return a;
+}
+
+hipStream_t stream() {
+    static hipStream_t s = nullptr;
+    if (!s) HZ_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// HEC turn order: iarray_shuffle (src/utilities/array.c:302-318) with libc rand()
+// ---------------------------------------------------------------------------
+void hec_turn_order(int n, int *perm_dev, int) {
+    std::vector<int> perm(n);
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    if (n > 1) {
+        for (int i = 0; i < n - 1; ++i) {
+            const int j = i + rand() / (RAND_MAX / (n - i) + 1);
+            std::swap(perm[i], perm[j]);
+        }
+    }
+    HZ_CHECK(hipMemcpyAsync(perm_dev, perm.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice,
+                            stream()));
+    HZ_CHECK(hipStreamSynchronize(stream()));
+}
+
+// ---------------------------------------------------------------------------
+// Setup
+// ---------------------------------------------------------------------------
+int amg_setup(Hierarchy &H, const DevCSR &A0, AMG_param *param, bool) {
+    const short min_cdof = (short)std::max(param->coarse_dof, 1);
+    int max_levels = std::max<int>(1, param->max_levels);
+    if (max_levels > HZ_MAX_AMG_LVL) max_levels = HZ_MAX_AMG_LVL;
+    H.lv.assign(max_levels, Level{});
+    H.lv[0].A = A0;
+    sell_from_csr(H.lv[0].A, H.lv[0].S);
+    int *counts = arena().alloc_n<int>((size_t)A0.n + 1);
+    int lvl = 0;
+    const int type = param->aggregation_type;
+    while (H.lv[lvl].A.n > min_cdof && lvl < max_levels - 1) {
+        Level &L = H.lv[lvl];
+        const int n = L.A.n;
+        L.theta = param->strong_coupled;
+        if (type != HZ_VMB && type != HZ_HEC && type != HZ_HEM) {
+            std::printf("\n!!! ERROR HAZMATH DANGER: in function '%s' -- Unknown AMG aggregation type !!!\n\n",
+                        "amg_setup_unsmoothP_unsmoothR");
+            return -1013;
+        }
+        DevCSR N;
+        strength_graph(L.A, param->strong_coupled, N, counts);
+        L.agg = arena().alloc_n<int>((size_t)n + 1);
+        int nagg = 0, st = HZ_SUCCESS;
+        if (type == HZ_HEC) {
+            int *perm = arena().alloc_n<int>((size_t)n + 1);
+            hec_turn_order(n, perm, 0);
+            const int r = hec_aggregate(N, perm, L.agg);
+            if (r == -1016) {
+                std::fprintf(stderr, "\n\n%%%% ERROR Stop: negative index encountered in %s()\n\n",
+                             "aggregation_hec");
+                return -1016;
+            }
+            nagg = r;
+        } else {
+            st = aggregate_host(N, type, param->max_aggregation, L.agg, &nagg);
+            if (st == -1017) return -1017;
+        }
+        L.nagg = nagg;
+        // amg_setup_ua.c:1325-1328
+        if (nagg * 4 > n) param->strong_coupled /= 2;
+        else if (nagg * 1.25 < n) param->strong_coupled *= 2;
+        if (st < 0) break;
+        if (nagg < 1) break;
+        L.rptr = arena().alloc_n<int>((size_t)nagg + 1);
+        L.rmem = arena().alloc_n<int>((size_t)n + 1);
+        build_restriction(n, L.agg, nagg, L.rptr, L.rmem);
+        Level &C = H.lv[lvl + 1];
+        rap_agg(L.A, L.agg, nagg, L.rptr, L.rmem, C.A);
+        sell_from_csr(C.A, C.S);
+        ++lvl;
+    }
+    H.nl = lvl + 1;
+    H.lv[lvl].theta = param->strong_coupled;
+    Level &Cst = H.lv[lvl];
+    if (param->coarse_solver == HZ_SOLVER_UMFPACK) {
+        // amg_setup_ua.c:1390-1401: the coarsest matrix is replaced by its sorted transpose
+        DevCSR T;
+        csr_transpose(Cst.A, T);
+        Cst.A = T;
+        sell_from_csr(Cst.A, Cst.S);
+        Cst.Ainv = arena().alloc_n<double>((size_t)Cst.A.n * Cst.A.n + 1);
+        if (coarse_dense_setup(Cst.A, Cst.Ainv) != 0)
+            std::fprintf(stderr, "hazmath_mi355x: zero pivot in the coarsest L D L^T factorization\n");
+    } else {
+        Cst.cgwork = arena().alloc_n<double>((size_t)Cst.A.n * 4 + 4);
+    }
+    if (param->print_level >= 2) {
+        double gc = 0.0, oc = 0.0;
+        std::printf("-----------------------------------------------------------\n");
+        std::printf("  Level   Num of rows   Num of nonzeros   Avg. NNZ / row   \n");
+        std::printf("-----------------------------------------------------------\n");
+        for (int l = 0; l < H.nl; ++l) {
+            const double avg = (double)H.lv[l].A.nnz / H.lv[l].A.n;
+            std::printf("%5lld %13lld %17lld %14.2f\n", (long long)l, (long long)H.lv[l].A.n,
+                        (long long)H.lv[l].A.nnz, avg);
+            gc += H.lv[l].A.n;
+            oc += H.lv[l].A.nnz;
+        }
+        std::printf("-----------------------------------------------------------\n");
+        gc /= H.lv[0].A.n;
+        oc /= H.lv[0].A.nnz;
+        std::printf("  Grid complexity = %.3f  |", gc);
+        std::printf("  Operator complexity = %.3f\n", oc);
+        std::printf("-----------------------------------------------------------\n");
+    }
+    return HZ_SUCCESS;
+}
+
+void amg_prepare_cycle(Hierarchy &H, const AMG_param *param) {
+    // precond_amg rebuilds its parameters from param_amg_init + param_prec_to_amg
+    // (precond.c:173-174, parameters.c:649-675): tol is the default 1e-6.
+    CycleParams &cp = H.cp;
+    cp.cycle_type = param->cycle_type;
+    cp.smoother = param->smoother;
+    cp.presmooth = param->presmooth_iter;
+    cp.postsmooth = param->postsmooth_iter;
+    cp.coarse_solver = param->coarse_solver;
+    cp.coarse_scaling = param->coarse_scaling;
+    cp.coarse_tol = 1e-6 * 1e-2;
+    cp.maxit = param->maxit;
+    const int n0 = H.lv[0].A.n;
+    int maxgrid = std::max(8192, div_up(H.lv[0].S.nslices, 4) + 1);
+    H.ws.partials = arena().alloc_n<double>((size_t)maxgrid * ReduceWs::kSlots);
+    H.ws.tickets = arena().alloc_n<unsigned int>(64);
+    HZ_CHECK(hipMemsetAsync(H.ws.tickets, 0, 64 * sizeof(unsigned int), stream()));
+    H.sc = arena().alloc_n<PcgScalars>(1);
+    HZ_CHECK(hipMemsetAsync(H.sc, 0, sizeof(PcgScalars), stream()));
+    H.alpha_cs = arena().alloc_n<double>(1);
+    for (int l = 0; l < H.nl; ++l) {
+        Level &L = H.lv[l];
+        const size_t n = (size_t)L.A.n + 1;
+        if (l > 0) {
+            L.b = arena().alloc_n<double>(n);
+            L.x = arena().alloc_n<double>(n);
+        }
+        L.w = arena().alloc_n<double>(n);
+        if (l > 0) L.xt = arena().alloc_n<double>(n);
+        else H.x0buf = arena().alloc_n<double>(n);
+        if (l < H.nl - 1) {
+            L.dsm = arena().alloc_n<double>(n);
+            smoother_diag(L.S, cp.smoother, L.dsm);
+        }
+    }
+    (void)n0;
+}
+
+namespace {
+
+struct CycleState {
+    int num_lvl[HZ_MAX_AMG_LVL];
+    bool zero[HZ_MAX_AMG_LVL];        // x of the level is logically zero
+    bool first_done[HZ_MAX_AMG_LVL];  // zero-iterate sweep already fused into restriction
+};
+
+void presmooth(Hierarchy &H, int l, CycleState &cs) {
+    Level &L = H.lv[l];
+    const CycleParams &cp = H.cp;
+    int sweeps = cp.presmooth;
+    if (cs.zero[l]) {
+        if (sweeps == 0) {
+            fill(L.x, 0.0, L.A.n);
+        } else if (cs.first_done[l]) {
+            --sweeps;
+        } else {
+            smooth_from_zero(L.A.n, cp.smoother, cp.jacobi_w, L.dsm, L.b, L.x);
+            --sweeps;
+        }
+        cs.zero[l] = false;
+        cs.first_done[l] = false;
+    }
+    for (int s = 0; s < sweeps; ++s) {
+        smooth_sweep(L.S, cp.smoother, cp.jacobi_w, L.b, L.x, L.xt);
+        std::swap(L.x, L.xt);
+    }
+}
+
+void postsmooth(Hierarchy &H, int l) {
+    Level &L = H.lv[l];
+    const CycleParams &cp = H.cp;
+    for (int s = 0; s < cp.postsmooth; ++s) {
+        smooth_sweep(L.S, cp.smoother, cp.jacobi_w, L.b, L.x, L.xt);
+        std::swap(L.x, L.xt);
+    }
+}
+
+// mgcycle, mgcycle.c:245-409 (UA transfers, no Schwarz levels)
+void mgcycle(Hierarchy &H, CycleState &cs) {
+    const CycleParams &cp = H.cp;
+    const int nl = H.nl;
+    int l = 0;
+    double alpha = 1.0;
+    bool alpha_on_device = false;
+    for (int i = 0; i < HZ_MAX_AMG_LVL; ++i) cs.num_lvl[i] = 0;
+forward:
+    while (l < nl - 1) {
+        Level &L = H.lv[l];
+        Level &C = H.lv[l + 1];
+        cs.num_lvl[l]++;
+        presmooth(H, l, cs);
+        residual(L.S, L.b, L.x, L.w);
+        // the next level restarts from zero; fuse its first sweep into the restriction
+        const bool fuse = (l + 1 < nl - 1) && cp.presmooth > 0;
+        restrict_agg(L.nagg, L.rptr, L.rmem, L.w, C.b, cp.smoother, cp.jacobi_w,
+                     fuse ? C.dsm : nullptr, fuse ? C.x : nullptr);
+        ++l;
+        cs.zero[l] = true;
+        cs.first_done[l] = fuse;
+    }
+    {
+        Level &C = H.lv[nl - 1];
+        if (cp.coarse_solver == HZ_SOLVER_UMFPACK) {
+            coarse_dense_solve(C.A.n, C.Ainv, C.b, C.x);
+        } else {
+            fill(C.x, 0.0, C.A.n);
+            coarse_cg(C.S, C.b, C.x, cp.coarse_tol, 20 * C.A.n, C.cgwork);
+        }
+        cs.zero[nl - 1] = false;
+    }
+    while (l > 0) {
+        --l;
+        Level &L = H.lv[l];
+        Level &C = H.lv[l + 1];
+        if (cp.coarse_scaling == 1) {
+            coarse_scaling_alpha(C.S, C.x, C.b, H.sc, H.alpha_cs, H.ws);
+            alpha_on_device = true;
+        }
+        prolong_agg(L.A.n, L.agg, C.x, alpha, alpha_on_device ? H.alpha_cs : nullptr, L.x);
+        postsmooth(H, l);
+        if (cs.num_lvl[l] < cp.cycle_type) break;
+        cs.num_lvl[l] = 0;
+    }
+    if (l > 0) goto forward;
+}
+
+}  // namespace
+
+// precond_amg (precond.c:164-183): b0 = r, x0 = 0, maxit cycles, z = x0.
+void amg_precond(Hierarchy &H, const double *r, double *z) {
+    Level &L0 = H.lv[0];
+    const CycleParams &cp = H.cp;
+    const int n = L0.A.n;
+    L0.b = const_cast<double *>(r);
+    if (H.nl == 1) {
+        // a single level is solved directly by the coarsest solver
+        if (cp.coarse_solver == HZ_SOLVER_UMFPACK) {
+            for (int it = 0; it < cp.maxit; ++it) coarse_dense_solve(n, L0.Ainv, r, z);
+        } else {
+            fill(z, 0.0, n);
+            for (int it = 0; it < cp.maxit; ++it) coarse_cg(L0.S, r, z, cp.coarse_tol, 20 * n, L0.cgwork);
+        }
+        if (cp.maxit <= 0) fill(z, 0.0, n);
+        return;
+    }
+    // Arrange the ping-pong so that the last sweep of level 0 writes into z.
+    const int swaps = (cp.presmooth > 0 ? cp.presmooth - 1 : 0) + cp.postsmooth +
+                      (cp.maxit > 1 ? (cp.maxit - 1) * (cp.presmooth + cp.postsmooth) : 0);
+    double *scratch = H.x0buf;
+    if (swaps % 2 == 1) {
+        L0.x = scratch;
+        L0.xt = z;
+    } else {
+        L0.x = z;
+        L0.xt = scratch;
+    }
+    CycleState cs;
+    for (int i = 0; i < HZ_MAX_AMG_LVL; ++i) {
+        cs.zero[i] = false;
+        cs.first_done[i] = false;
+    }
+    cs.zero[0] = true;
+    for (int it = 0; it < cp.maxit; ++it) mgcycle(H, cs);
+    if (cp.maxit <= 0) fill(L0.x, 0.0, n);
+    if (L0.x != z) copy(z, L0.x, n);
+    L0.x = nullptr;
+    L0.xt = nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// Condition number estimate from the Lanczos tridiagonal (krylov.c:663-705):
+// extreme eigenvalues by Sturm bisection (stands in for LAPACK dsterf).
+// ---------------------------------------------------------------------------
+static int sturm_count(int n, const double *d, const double *e2, double x) {
+    int c = 0;
+    double q = d[0] - x;
+    if (q < 0) ++c;
+    for (int i = 1; i < n; ++i) {
+        if (q == 0.0) q = 1e-300;
+        q = d[i] - x - e2[i - 1] / q;
+        if (q < 0) ++c;
+    }
+    return c;
+}
+
+static double bisect(int n, const double *d, const double *e2, double lo, double hi, int k) {
+    for (int it = 0; it < 200; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (mid <= lo || mid >= hi) break;
+        if (sturm_count(n, d, e2, mid) > k) hi = mid;
+        else lo = mid;
+    }
+    return 0.5 * (lo + hi);
+}
+
+static double tridiag_condest(int m, const double *alpha, const double *beta) {
+    std::vector<double> d0(m), e(m, 0.0), e2(m, 0.0);
+    d0[0] = 1.0 / alpha[0];
+    for (int i = 0; i < m - 1; ++i) {
+        d0[i + 1] = 1.0 / alpha[i + 1] + beta[i] / alpha[i];
+        e[i] = std::sqrt(beta[i]) / alpha[i];
+        e2[i] = e[i] * e[i];
+    }
+    double lo = d0[0], hi = d0[0];
+    for (int i = 0; i < m; ++i) {
+        const double r = (i > 0 ? std::fabs(e[i - 1]) : 0.0) + (i < m - 1 ? std::fabs(e[i]) : 0.0);
+        lo = std::min(lo, d0[i] - r);
+        hi = std::max(hi, d0[i] + r);
+    }
+    const double pad = 1e-12 * (std::fabs(lo) + std::fabs(hi)) + 1e-300;
+    const double lmin = bisect(m, d0.data(), e2.data(), lo - pad, hi + pad, 0);
+    const double lmax = bisect(m, d0.data(), e2.data(), lo - pad, hi + pad, m - 1);
+    return lmax / lmin;
+}
+
+static void itinfo(int prt, int stop, int iter, double rel, double absr, double fac) {
+    if (prt < 2) return;
+    if (iter > 0) {
+        std::printf("%6lld | %13.6e   | %13.6e  | %10.4f\n", (long long)iter, rel, absr, fac);
+    } else {
+        std::printf("-----------------------------------------------------------\n");
+        if (stop == HZ_STOP_REL_RES) std::printf("It Num |   ||r||/||b||   |     ||r||      |  Conv. Factor\n");
+        else if (stop == HZ_STOP_REL_PRECRES) std::printf("It Num | ||r||_B/||b||_B |    ||r||_B     |  Conv. Factor\n");
+        else std::printf("It Num |   ||r||/||x||   |     ||r||      |  Conv. Factor\n");
+        std::printf("-----------------------------------------------------------\n");
+        std::printf("%6lld | %13.6e   | %13.6e  |     -.-- \n", (long long)iter, rel, absr);
+    }
+}
+
+namespace {
+PcgScalars *pinned_scalars() {
+    static PcgScalars *h = nullptr;
+    if (!h) HZ_CHECK(hipHostMalloc((void **)&h, sizeof(PcgScalars), hipHostMallocDefault));
+    return h;
+}
+
+void fetch(const Hierarchy &H, PcgScalars &out) {
+    PcgScalars *h = pinned_scalars();
+    HZ_CHECK(hipMemcpyAsync(h, H.sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, stream()));
+    HZ_CHECK(hipStreamSynchronize(stream()));
+    out = *h;
+}
+}  // namespace
+
+// dcsr_pcg_w_cond_est, krylov.c:400-718, with the AMG preconditioner.
+PcgResult pcg_solve(Hierarchy &H, const DevSELL &A, const double *b, double *u, double tol,
+                    int MaxIt, int stop, int prt) {
+    PcgResult res;
+    const int m = A.n;
+    const double maxdiff = tol * 1e-4;
+    int iter = 0, stag = 1, more_step = 1, restart_step = 1;
+    double absres0 = kBigReal, absres = kBigReal, relres = kBigReal;
+    double normu = kBigReal, normr0 = kBigReal;
+    double *p = arena().alloc_n<double>((size_t)m + 1);
+    double *z = arena().alloc_n<double>((size_t)m + 1);
+    double *r = arena().alloc_n<double>((size_t)m + 1);
+    double *t = arena().alloc_n<double>((size_t)m + 1);
+    const int histn = std::max(MaxIt, 1) + 1;
+    double *alpha_hist = arena().alloc_n<double>((size_t)histn);
+    double *beta_hist = arena().alloc_n<double>((size_t)histn);
+    int iter_condest = 0;
+    PcgScalars hs{};
+    double *sc_rr = &H.sc->rr, *sc_zr = &H.sc->zr;
+
+    auto measure = [&](double &absr) -> double {
+        // r = b - A u; then the stop-type residual
+        residual(A, b, u, r);
+        if (stop == HZ_STOP_REL_PRECRES) {
+            amg_precond(H, r, z);
+            dot(m, z, r, sc_zr, H.ws);
+            fetch(H, hs);
+            absr = std::sqrt(std::fabs(hs.zr));
+            return absr / normr0;
+        }
+        norm2sq(m, r, sc_rr, H.ws);
+        fetch(H, hs);
+        absr = std::sqrt(hs.rr);
+        return (stop == HZ_STOP_MOD_REL_RES) ? absr / normu : absr / normr0;
+    };
+
+    residual(A, b, u, r);
+    amg_precond(H, r, z);
+    switch (stop) {
+        case HZ_STOP_REL_RES:
+            norm2sq(m, r, sc_rr, H.ws);
+            fetch(H, hs);
+            absres0 = std::sqrt(hs.rr);
+            normr0 = std::max(kSmallReal, absres0);
+            relres = absres0 / normr0;
+            break;
+        case HZ_STOP_REL_PRECRES:
+            dot(m, r, z, sc_zr, H.ws);
+            fetch(H, hs);
+            absres0 = std::sqrt(hs.zr);
+            normr0 = std::max(kSmallReal, absres0);
+            relres = absres0 / normr0;
+            break;
+        case HZ_STOP_MOD_REL_RES:
+            norm2sq(m, r, sc_rr, H.ws);
+            fetch(H, hs);
+            absres0 = std::sqrt(hs.rr);
+            norm2sq(m, u, sc_rr, H.ws);
+            fetch(H, hs);
+            normu = std::max(kSmallReal, std::sqrt(hs.rr));
+            relres = absres0 / normu;
+            break;
+        default:
+            std::printf("### ERROR: Unrecognised stopping type for %s!\n", "dcsr_pcg_w_cond_est");
+            goto FINISHED;
+    }
+    if (relres < tol || absres0 < 1e-3 * tol) goto FINISHED;
+    itinfo(prt, stop, iter, relres, absres0, 0.0);
+    copy(p, z, m);
+    dot(m, z, r, sc_zr, H.ws);   // temp1 = (z, r) lives in sc->zr until the first update
+
+    while (iter++ < MaxIt) {
+        spmv(A, p, t, &H.sc->temp2, H.ws);
+        pcg_update(m, H.sc, p, t, u, r, alpha_hist, iter, H.ws);
+        fetch(H, hs);
+        const double alpha = hs.alpha;
+        if (stop == HZ_STOP_REL_PRECRES) {
+            amg_precond(H, r, z);
+            dot(m, z, r, sc_zr, H.ws);
+            fetch(H, hs);
+            absres = std::sqrt(std::fabs(hs.zr));
+            relres = absres / normr0;
+        } else {
+            absres = std::sqrt(hs.rr);
+            relres = (stop == HZ_STOP_MOD_REL_RES) ? absres / normu : absres / normr0;
+        }
+        res.relres_hist.push_back(relres);
+        itinfo(prt, stop, iter, relres, absres, absres / absres0);
+        // Check I
+        if (hs.uinf <= kSmallReal) {
+            if (prt > 1)
+                std::printf("### HAZMATH WARNING: Iteration stopped due to the solution is almost zero! %s : %lld\n",
+                            "dcsr_pcg_w_cond_est", 0LL);
+            iter = HZ_ERROR_SOLVER_SOLSTAG;
+            break;
+        }
+        // Check II
+        normu = std::sqrt(hs.uu);
+        const double reldiff = std::fabs(alpha) * std::sqrt(hs.pp) / normu;
+        if ((stag <= 20) & (reldiff < maxdiff)) {
+            if (prt >= 4) {
+                std::printf("||u-u'|| = %e and the comp. rel. res. = %e.\n", reldiff, relres);
+                std::printf("### HAZMATH WARNING: Iteration restarted due to stagnation! %s : %lld\n",
+                            "dcsr_pcg_w_cond_est", 0LL);
+            }
+            relres = measure(absres);
+            if (prt >= 4) std::printf("### HAZMATH WARNING: The actual relative residual = %e!\n", relres);
+            if (relres < tol) break;
+            if (stag >= 20) {
+                if (prt > 1)
+                    std::printf("### HAZMATH WARNING: Iteration stopped due to staggnation! %s : %lld\n",
+                                "dcsr_pcg_w_cond_est", 0LL);
+                iter = HZ_ERROR_SOLVER_STAG;
+                break;
+            }
+            if (stag == 1 && restart_step == 1) iter_condest = iter;
+            fill(p, 0.0, m);
+            ++stag;
+            ++restart_step;
+        }
+        // Check III
+        if (relres < tol) {
+            const double computed = relres;
+            relres = measure(absres);
+            if (relres < tol) break;
+            if (prt >= 4) {
+                std::printf("### HAZMATH WARNING: The computed relative residual = %e!\n", computed);
+                std::printf("### HAZMATH WARNING: The actual relative residual = %e!\n", relres);
+            }
+            if (more_step >= 20) {
+                if (prt > 1)
+                    std::printf("### HAZMATH WARNING: The tolerence might be too small! %s : %lld\n",
+                                "dcsr_pcg_w_cond_est", 0LL);
+                iter = HZ_ERROR_SOLVER_TOLSMALL;
+                break;
+            }
+            if (stag == 1 && restart_step == 1) iter_condest = iter;
+            fill(p, 0.0, m);
+            ++more_step;
+            ++restart_step;
+        }
+        absres0 = absres;
+        if (stop != HZ_STOP_REL_PRECRES) amg_precond(H, r, z);
+        dot(m, z, r, sc_zr, H.ws);
+        pcg_direction(m, H.sc, z, p, beta_hist, iter);
+    }
+    {
+        if (stag == 1 && restart_step == 1) iter_condest = iter;
+        if (iter_condest > MaxIt) iter_condest = MaxIt;
+        if (iter_condest < 1) {
+            std::printf("### HAZMATH WARNING: %s takes too few iterations for estimating condition number!!!\n",
+                        "dcsr_pcg_w_cond_est");
+        } else {
+            std::vector<double> ha(iter_condest), hb(iter_condest);
+            HZ_CHECK(hipMemcpyAsync(ha.data(), alpha_hist, sizeof(double) * iter_condest,
+                                    hipMemcpyDeviceToHost, stream()));
+            HZ_CHECK(hipMemcpyAsync(hb.data(), beta_hist, sizeof(double) * iter_condest,
+                                    hipMemcpyDeviceToHost, stream()));
+            HZ_CHECK(hipStreamSynchronize(stream()));
+            res.condest = tridiag_condest(iter_condest, ha.data(), hb.data());
+        }
+    }
+FINISHED:
+    if (prt > 0) {
+        if (iter > MaxIt)
+            std::printf("### HAZMATH WARNING: Max iter %lld reached with rel. resid. %e.\n", (long long)MaxIt,
+                        relres);
+        else if (iter >= 0)
+            std::printf("Num_iter(krylov.c) = %lld with relative residual %e.\n", (long long)iter, relres);
+    }
+    HZ_CHECK(hipStreamSynchronize(stream()));
+    res.relres = relres;
+    res.iter = (iter > MaxIt) ? HZ_ERROR_SOLVER_MAXIT : iter;
+    return res;
+}
+
+}  // namespace hz

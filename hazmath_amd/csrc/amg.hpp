@@ -1,0 +1,67 @@
+// amg.hpp -- device AMG hierarchy, V/W cycle and PCG driver.
+#pragma once
+
+#include <vector>
+
+#include "hz_internal.h"
+
+namespace hz {
+
+struct Level {
+    DevCSR A;                 // CSR (setup input; coarsest: sorted transpose for coarse_solver 32)
+    DevSELL S;                // SELL-64 copy used by every solve kernel
+    double *dsm = nullptr;    // smoother diagonal (Jacobi: last diagonal entry; L1DIAG: row l1 norm)
+    int *agg = nullptr;       // P: row i -> aggregate (UNPT = -1)
+    int nagg = 0;
+    int *rptr = nullptr;      // R = P^T rows
+    int *rmem = nullptr;
+    double *b = nullptr, *x = nullptr, *w = nullptr, *xt = nullptr;
+    double theta = 0.0;       // strong_coupled used to aggregate this level
+    double *Ainv = nullptr;   // coarsest, coarse_solver 32
+    double *cgwork = nullptr; // coarsest, coarse_solver 0
+};
+
+struct CycleParams {
+    int cycle_type = 1;
+    int smoother = 1;
+    int presmooth = 1, postsmooth = 1;
+    int coarse_solver = 0;
+    int coarse_scaling = 0;
+    double coarse_tol = 1e-8;   // param_amg_init tol 1e-6 times 1e-2 (mgcycle.c:255)
+    double jacobi_w = 1.0;
+    int maxit = 1;              // cycles per preconditioner application
+};
+
+struct Hierarchy {
+    std::vector<Level> lv;
+    int nl = 0;
+    CycleParams cp;
+    ReduceWs ws;
+    PcgScalars *sc = nullptr;
+    double *alpha_cs = nullptr;   // coarse scaling factor (device)
+    double *x0buf = nullptr;      // level-0 ping-pong partner of the preconditioner output
+};
+
+// Runs the reference's setup loop on the device.  A0 arrays are borrowed.
+// Returns SUCCESS, a negative HAZmath code, or a negative exit code
+// (-1016 / -1017 / -1013) for the reference's fatal paths.
+int amg_setup(Hierarchy &H, const DevCSR &A0, AMG_param *param, bool keep_for_export);
+void amg_prepare_cycle(Hierarchy &H, const AMG_param *param);
+// z = B(r): cp.maxit cycles from a zero iterate.
+void amg_precond(Hierarchy &H, const double *r, double *z);
+
+struct PcgResult {
+    int iter = 0;
+    double relres = 0.0;
+    double condest = 0.0;
+    std::vector<double> relres_hist;
+};
+
+// PCG with condition estimate, dcsr_pcg_w_cond_est (krylov.c:400-718).
+PcgResult pcg_solve(Hierarchy &H, const DevSELL &A, const double *b, double *u, double tol,
+                    int maxit, int stop_type, int prtlvl);
+
+// Host-staged sequential aggregation for VMB / HEM.
+int aggregate_host(const DevCSR &N, int type, int max_aggregation, int *agg_dev, int *nagg);
+
+}  // namespace hz

@@ -1,0 +1,533 @@
+// api.hip -- exported C ABI (include/hazmath_mi355x.h).
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "amg.hpp"
+
+using namespace hz;
+
+namespace {
+
+hz_options g_opt = {1.0, 0, 0, 0};
+hz_solve_info g_info;
+
+bool device_ready() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) {
+        std::fprintf(stderr, "hazmath_mi355x: no HIP device available\n");
+        return false;
+    }
+    return true;
+}
+
+template <class T>
+T *upload(const T *h, size_t n) {
+    T *d = arena().alloc_n<T>(n + 1);
+    if (n) HZ_CHECK(hipMemcpyAsync(d, h, sizeof(T) * n, hipMemcpyHostToDevice, stream()));
+    return d;
+}
+
+DevCSR upload_csr(const dCSRmat *A) {
+    DevCSR d;
+    d.n = A->row;
+    d.ncol = A->col;
+    d.nnz = A->IA[A->row];
+    d.ia = upload(A->IA, (size_t)A->row + 1);
+    d.ja = upload(A->JA, (size_t)d.nnz);
+    d.val = upload(A->val, (size_t)d.nnz);
+    return d;
+}
+
+void download_csr(const DevCSR &d, dCSRmat *A) {
+    A->row = d.n;
+    A->col = d.ncol;
+    A->nnz = d.nnz;
+    A->IA = (INT *)std::calloc((size_t)d.n + 1, sizeof(INT));
+    A->JA = (INT *)std::calloc((size_t)d.nnz + 1, sizeof(INT));
+    A->val = (REAL *)std::calloc((size_t)d.nnz + 1, sizeof(REAL));
+    HZ_CHECK(memcpy_sync(A->IA, d.ia, sizeof(int) * ((size_t)d.n + 1), hipMemcpyDeviceToHost));
+    if (d.nnz) {
+        HZ_CHECK(memcpy_sync(A->JA, d.ja, sizeof(int) * d.nnz, hipMemcpyDeviceToHost));
+        HZ_CHECK(memcpy_sync(A->val, d.val, sizeof(double) * d.nnz, hipMemcpyDeviceToHost));
+    }
+}
+
+int fatal(int code) {
+    // reference: exit(16) / exit(17) / check_error -> exit(status)
+    if (g_opt.return_on_fatal) return code;
+    std::fflush(stdout);
+    if (code == -1016) std::exit(16);
+    if (code == -1017) std::exit(17);
+    std::exit(HZ_ERROR_AMG_AGG_TYPE);
+}
+
+// Features of AMG_param this build implements on the GPU.
+int check_supported(const linear_itsolver_param *it, const AMG_param *amg) {
+    (void)it;
+    if (amg->smoother != HZ_SMOOTHER_JACOBI && amg->smoother != HZ_SMOOTHER_L1DIAG) {
+        std::printf("### ERROR: Wrong smoother type %lld!\n", (long long)amg->smoother);
+        return HZ_ERROR_AMG_SMOOTH_TYPE;
+    }
+    if (amg->AMG_type == HZ_SA_AMG) {
+        std::printf("### ERROR: smoothed aggregation is not available in this build\n");
+        return HZ_ERROR_INPUT_PAR;
+    }
+    if (amg->cycle_type != HZ_V_CYCLE && amg->cycle_type != HZ_W_CYCLE) {
+        std::printf("### ERROR: cycle type %lld is not available in this build\n", (long long)amg->cycle_type);
+        return HZ_ERROR_INPUT_PAR;
+    }
+    if (amg->Schwarz_levels > 0) {
+        std::printf("### ERROR: Schwarz smoothing is not available in this build\n");
+        return HZ_ERROR_INPUT_PAR;
+    }
+    return HZ_SUCCESS;
+}
+
+// linear_solver_dcsr_krylov_amg (itsolver.c:846-931) on device-resident data.
+INT core_solve(const DevCSR &A, const double *db, double *dx, linear_itsolver_param *it,
+               AMG_param *amg) {
+    std::memset(&g_info, 0, sizeof(g_info));
+    const short prt = it->linear_print_level;
+    hipEvent_t e0, e1, e2;
+    HZ_CHECK(hipEventCreate(&e0));
+    HZ_CHECK(hipEventCreate(&e1));
+    HZ_CHECK(hipEventCreate(&e2));
+    HZ_CHECK(hipEventRecord(e0, stream()));
+    if (prt > 0) std::printf("\n Calling UA AMG ...\n");
+    int st = check_supported(it, amg);
+    if (st < 0) return st;
+    Hierarchy H;
+    H.cp.jacobi_w = g_opt.jacobi_weight;
+    st = amg_setup(H, A, amg, false);
+    if (st <= -1000) return fatal(st);
+    if (st < 0) return st;
+    amg_prepare_cycle(H, amg);
+    H.cp.jacobi_w = g_opt.jacobi_weight;
+    HZ_CHECK(hipEventRecord(e1, stream()));
+    INT iter;
+    double condest = 0.0;
+    PcgResult pr;
+    if (it->linear_tol < kSmallReal)
+        std::printf("### HAZMATH WARNING: Convergence tolerance for iterative solver is too small!\n");
+    if (it->linear_maxit <= 0)
+        std::printf("### HAZMATH WARNING: Max number of iterations should be a POSITIVE integer!\n");
+    if (it->linear_itsolver_type != HZ_SOLVER_CG) {
+        std::printf("### ERROR: Unknown itertive solver type %lld!\n", (long long)it->linear_itsolver_type);
+        iter = HZ_ERROR_SOLVER_TYPE;
+    } else {
+        if (prt > 0) {
+            std::printf("**********************************************************\n");
+            std::printf(" --> using Conjugate Gradient Method:\n");
+        }
+        pr = pcg_solve(H, H.lv[0].S, db, dx, it->linear_tol, it->linear_maxit, it->linear_stop_type, prt);
+        iter = pr.iter;
+        condest = pr.condest;
+        std::printf("Estimated condition number = %13.6e\n", condest);
+    }
+    HZ_CHECK(hipEventRecord(e2, stream()));
+    HZ_CHECK(hipEventSynchronize(e2));
+    float ms_setup = 0.f, ms_solve = 0.f;
+    HZ_CHECK(hipEventElapsedTime(&ms_setup, e0, e1));
+    HZ_CHECK(hipEventElapsedTime(&ms_solve, e1, e2));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipEventDestroy(e2);
+    g_info.iterations = iter;
+    g_info.num_levels = H.nl;
+    for (int l = 0; l < H.nl && l < HZ_MAX_AMG_LVL; ++l) {
+        g_info.level_rows[l] = H.lv[l].A.n;
+        g_info.level_nnz[l] = H.lv[l].A.nnz;
+    }
+    g_info.setup_ms = ms_setup;
+    g_info.solve_ms = ms_solve;
+    g_info.total_ms = ms_setup + ms_solve;
+    g_info.final_relres = pr.relres;
+    g_info.condest = condest;
+    g_info.relres_count = (int)std::min<size_t>(pr.relres_hist.size(), 4096);
+    for (int i = 0; i < g_info.relres_count; ++i) g_info.relres_hist[i] = pr.relres_hist[i];
+    if (g_opt.verbose_timing)
+        std::printf("hazmath_mi355x: setup %.3f ms, solve %.3f ms, %d levels\n", ms_setup, ms_solve, H.nl);
+    std::fflush(stdout);
+    return iter;
+}
+
+}  // namespace
+
+extern "C" {
+
+// parameters.c:158-205
+void param_amg_init(AMG_param *p) {
+    std::memset(p, 0, sizeof(*p));
+    p->AMG_type = HZ_UA_AMG;
+    p->print_level = 0;
+    p->maxit = 1;
+    p->tol = 1e-6;
+    p->max_levels = 10;
+    p->coarse_dof = 200;
+    p->cycle_type = HZ_W_CYCLE;
+    p->smoother = HZ_SMOOTHER_GS;
+    p->presmooth_iter = 1;
+    p->postsmooth_iter = 1;
+    p->coarse_solver = HZ_SOLVER_DEFAULT;
+    p->relaxation = 1.0;
+    p->polynomial_degree = 2;
+    p->coarse_scaling = 0;
+    p->amli_degree = 2;
+    p->amli_coef = nullptr;
+    p->nl_amli_krylov_type = 4;
+    p->fpwr = 1.0;
+    p->aggregation_type = HZ_HEC;
+    p->strong_coupled = 0.04;
+    p->max_aggregation = 20;
+    p->tentative_smooth = 0.67;
+    p->smooth_filter = 1;
+    p->Schwarz_levels = 0;
+    p->Schwarz_mmsize = 200;
+    p->Schwarz_maxlvl = -1;
+    p->Schwarz_type = 1;
+    p->Schwarz_blksolver = HZ_SOLVER_UMFPACK;
+    p->HAZDIR = nullptr;
+    p->Schwarz_on_blk = nullptr;
+    p->Schwarz_patch_type = nullptr;
+    p->damping_param = 1.0;
+    p->BSR_alpha = -1000.;
+    p->BSR_omega = -1000.;
+}
+
+// parameters.c:238-251
+void param_linear_solver_init(linear_itsolver_param *p) {
+    std::memset(p, 0, sizeof(*p));
+    p->linear_print_level = 0;
+    p->linear_itsolver_type = 4;
+    p->linear_precond_type = 0;
+    p->linear_stop_type = HZ_STOP_REL_RES;
+    p->linear_maxit = 500;
+    p->linear_restart = 100;
+    p->linear_tol = 1e-6;
+    p->HX_smooth_iter = 1;
+}
+
+// sparse.c:27-57
+dCSRmat dcsr_create(const INT m, const INT n, const INT nnz) {
+    dCSRmat A;
+    A.IA = (m > 0) ? (INT *)std::calloc((size_t)m + 1, sizeof(INT)) : nullptr;
+    A.JA = (n > 0) ? (INT *)std::calloc((size_t)nnz, sizeof(INT)) : nullptr;
+    A.val = (nnz > 0) ? (REAL *)std::calloc((size_t)nnz, sizeof(REAL)) : nullptr;
+    A.row = m;
+    A.col = n;
+    A.nnz = nnz;
+    return A;
+}
+
+// sparse.c:573-592
+void dcsr_free(dCSRmat *A) {
+    if (!A) return;
+    std::free(A->IA);
+    std::free(A->JA);
+    std::free(A->val);
+    A->IA = nullptr;
+    A->JA = nullptr;
+    A->val = nullptr;
+}
+
+// vec.c:68-79
+dvector dvec_create(const INT m) {
+    dvector u;
+    u.row = m;
+    u.val = (m > 0) ? (REAL *)std::calloc((size_t)m, sizeof(REAL)) : nullptr;
+    return u;
+}
+
+// vec.c:157-163
+void dvec_free(dvector *u) {
+    if (!u) return;
+    std::free(u->val);
+    u->val = nullptr;
+    u->row = 0;
+}
+
+// vec.c:280-295
+void dvec_set(INT n, dvector *x, const REAL val) {
+    if (n > 0) x->row = n;
+    else n = x->row;
+    for (INT i = 0; i < n; ++i) x->val[i] = val;
+}
+
+// io.c:2296-2346 + format.c:785-827
+dCSRmat *dcoo_read_dcsr_p(FILE *fp) {
+    if (!fp) return nullptr;
+    char buf[512];
+    long pos;
+    while (true) {
+        pos = std::ftell(fp);
+        if (std::fscanf(fp, "%511s", buf) != 1) return nullptr;
+        if (buf[0] == '[' || buf[0] == '%' || buf[0] == '|') {
+            if (!std::fgets(buf, sizeof(buf), fp)) return nullptr;
+            continue;
+        }
+        break;
+    }
+    std::fseek(fp, pos, SEEK_SET);
+    long long m, n, nnz;
+    if (std::fscanf(fp, "%lld %lld %lld", &m, &n, &nnz) != 3) return nullptr;
+    std::vector<INT> ri(nnz), ci(nnz);
+    std::vector<REAL> v(nnz);
+    for (long long k = 0; k < nnz; ++k) {
+        long long i, j;
+        double a;
+        if (std::fscanf(fp, "%lld %lld %le", &i, &j, &a) != 3) return nullptr;
+        ri[k] = (INT)i;
+        ci[k] = (INT)j;
+        v[k] = a;
+    }
+    dCSRmat *A = (dCSRmat *)std::calloc(1, sizeof(dCSRmat));
+    *A = dcsr_create((INT)m, (INT)n, (INT)nnz);
+    std::vector<INT> next((size_t)m + 1, 0);
+    for (long long k = 0; k < nnz; ++k) next[ri[k] + 1]++;
+    A->IA[0] = 0;
+    for (long long i = 1; i <= m; ++i) {
+        A->IA[i] = A->IA[i - 1] + next[i];
+        next[i] = A->IA[i];
+    }
+    next[0] = 0;
+    for (long long k = 0; k < nnz; ++k) {
+        const INT r = ri[k];
+        const INT p = next[r]++;
+        A->JA[p] = ci[k];
+        A->val[p] = v[k];
+    }
+    return A;
+}
+
+// io.c:2427-2445
+dvector *dvector_read_p(FILE *fp) {
+    if (!fp) return nullptr;
+    long long n;
+    if (std::fscanf(fp, "%lld", &n) != 1) return nullptr;
+    dvector *b = (dvector *)std::calloc(1, sizeof(dvector));
+    *b = dvec_create((INT)n);
+    for (long long i = 0; i < n; ++i) {
+        double v;
+        if (std::fscanf(fp, "%lg", &v) != 1) break;
+        b->val[i] = v;
+    }
+    return b;
+}
+
+void hz_get_options(hz_options *opt) { *opt = g_opt; }
+void hz_set_options(const hz_options *opt) { g_opt = *opt; }
+void hz_get_last_info(hz_solve_info *info) { *info = g_info; }
+
+const char *hz_version(void) { return "hazmath_mi355x 0.1 (gfx950)"; }
+
+INT hz_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+INT hz_krylov_amg_device(INT n, INT nnz, const INT *dIA, const INT *dJA, const REAL *dval,
+                         const REAL *db, REAL *dx, linear_itsolver_param *itparam,
+                         AMG_param *amgparam) {
+    if (!device_ready()) return HZ_ERROR_DEVICE;
+    try {
+        arena().reset();
+        DevCSR A;
+        A.n = n;
+        A.ncol = n;
+        A.nnz = nnz;
+        A.ia = const_cast<int *>(dIA);
+        A.ja = const_cast<int *>(dJA);
+        A.val = const_cast<double *>(dval);
+        return core_solve(A, db, dx, itparam, amgparam);
+    } catch (const DeviceError &) {
+        return HZ_ERROR_DEVICE;
+    }
+}
+
+// itsolver.c:846-931
+INT linear_solver_dcsr_krylov_amg(dCSRmat *A, dvector *b, dvector *x, linear_itsolver_param *itparam,
+                                  AMG_param *amgparam) {
+    if (!device_ready()) return HZ_ERROR_DEVICE;
+    try {
+        arena().reset();
+        const DevCSR dA = upload_csr(A);
+        const double *db = upload(b->val, (size_t)A->row);
+        double *dx = upload(x->val, (size_t)A->row);
+        const INT st = core_solve(dA, db, dx, itparam, amgparam);
+        HZ_CHECK(hipMemcpyAsync(x->val, dx, sizeof(double) * (size_t)A->row, hipMemcpyDeviceToHost,
+                                stream()));
+        HZ_CHECK(hipStreamSynchronize(stream()));
+        return st;
+    } catch (const DeviceError &) {
+        return HZ_ERROR_DEVICE;
+    }
+}
+
+INT hz_setup_hierarchy(dCSRmat *A, AMG_param *amgparam, hz_hierarchy *out) {
+    std::memset(out, 0, sizeof(*out));
+    if (!device_ready()) return HZ_ERROR_DEVICE;
+    try {
+        arena().reset();
+        const DevCSR dA = upload_csr(A);
+        Hierarchy H;
+        int st = amg_setup(H, dA, amgparam, true);
+        if (st <= -1000) return fatal(st);
+        if (st < 0) return st;
+        out->num_levels = H.nl;
+        for (int l = 0; l < H.nl; ++l) {
+            download_csr(H.lv[l].A, &out->A[l]);
+            out->theta[l] = H.lv[l].theta;
+            if (l < H.nl - 1) {
+                out->num_aggregates[l] = H.lv[l].nagg;
+                out->aggregates[l] = (INT *)std::calloc((size_t)H.lv[l].A.n + 1, sizeof(INT));
+                HZ_CHECK(memcpy_sync(out->aggregates[l], H.lv[l].agg, sizeof(int) * H.lv[l].A.n,
+                                   hipMemcpyDeviceToHost));
+            }
+        }
+        return st;
+    } catch (const DeviceError &) {
+        return HZ_ERROR_DEVICE;
+    }
+}
+
+void hz_hierarchy_free(hz_hierarchy *h) {
+    for (int l = 0; l < HZ_MAX_AMG_LVL; ++l) {
+        dcsr_free(&h->A[l]);
+        std::free(h->aggregates[l]);
+        h->aggregates[l] = nullptr;
+    }
+    h->num_levels = 0;
+}
+
+INT hz_dcsr_mxv(dCSRmat *A, const REAL *x, REAL *y) {
+    if (!device_ready()) return HZ_ERROR_DEVICE;
+    try {
+        arena().reset();
+        DevCSR dA = upload_csr(A);
+        DevSELL S;
+        sell_from_csr(dA, S);
+        const double *dx = upload(x, (size_t)A->col);
+        double *dy = arena().alloc_n<double>((size_t)A->row + 1);
+        ReduceWs ws;
+        spmv(S, dx, dy, nullptr, ws);
+        HZ_CHECK(hipMemcpyAsync(y, dy, sizeof(double) * A->row, hipMemcpyDeviceToHost, stream()));
+        HZ_CHECK(hipStreamSynchronize(stream()));
+        return HZ_SUCCESS;
+    } catch (const DeviceError &) {
+        return HZ_ERROR_DEVICE;
+    }
+}
+
+INT hz_dcsr_residual(dCSRmat *A, const REAL *b, const REAL *x, REAL *w) {
+    if (!device_ready()) return HZ_ERROR_DEVICE;
+    try {
+        arena().reset();
+        DevCSR dA = upload_csr(A);
+        DevSELL S;
+        sell_from_csr(dA, S);
+        const double *db = upload(b, (size_t)A->row);
+        const double *dx = upload(x, (size_t)A->col);
+        double *dw = arena().alloc_n<double>((size_t)A->row + 1);
+        residual(S, db, dx, dw);
+        HZ_CHECK(hipMemcpyAsync(w, dw, sizeof(double) * A->row, hipMemcpyDeviceToHost, stream()));
+        HZ_CHECK(hipStreamSynchronize(stream()));
+        return HZ_SUCCESS;
+    } catch (const DeviceError &) {
+        return HZ_ERROR_DEVICE;
+    }
+}
+
+INT hz_smoother_sweep(dCSRmat *A, const REAL *b, REAL *x, INT smoother, REAL w) {
+    if (!device_ready()) return HZ_ERROR_DEVICE;
+    try {
+        arena().reset();
+        DevCSR dA = upload_csr(A);
+        DevSELL S;
+        sell_from_csr(dA, S);
+        const double *db = upload(b, (size_t)A->row);
+        double *dx = upload(x, (size_t)A->row);
+        double *dout = arena().alloc_n<double>((size_t)A->row + 1);
+        smooth_sweep(S, smoother, w, db, dx, dout);
+        HZ_CHECK(hipMemcpyAsync(x, dout, sizeof(double) * A->row, hipMemcpyDeviceToHost, stream()));
+        HZ_CHECK(hipStreamSynchronize(stream()));
+        return HZ_SUCCESS;
+    } catch (const DeviceError &) {
+        return HZ_ERROR_DEVICE;
+    }
+}
+
+INT hz_time_fine_kernels(INT n, INT nnz, const INT *dIA, const INT *dJA, const REAL *dval, INT reps,
+                         hz_kernel_times *out) {
+    std::memset(out, 0, sizeof(*out));
+    if (!device_ready()) return HZ_ERROR_DEVICE;
+    try {
+        arena().reset();
+        DevCSR A;
+        A.n = n;
+        A.ncol = n;
+        A.nnz = nnz;
+        A.ia = const_cast<int *>(dIA);
+        A.ja = const_cast<int *>(dJA);
+        A.val = const_cast<double *>(dval);
+        DevSELL S;
+        sell_from_csr(A, S);
+        double *x = arena().alloc_n<double>((size_t)n + 1);
+        double *y = arena().alloc_n<double>((size_t)n + 1);
+        double *b = arena().alloc_n<double>((size_t)n + 1);
+        double *dot_out = arena().alloc_n<double>(1);
+        fill(x, 1.0, n);
+        fill(b, 0.5, n);
+        ReduceWs ws;
+        ws.partials = arena().alloc_n<double>((size_t)std::max(8192, div_up(S.nslices, 4) + 1) * 4);
+        ws.tickets = arena().alloc_n<unsigned int>(64);
+        HZ_CHECK(hipMemsetAsync(ws.tickets, 0, 64 * sizeof(unsigned int), stream()));
+        hipEvent_t e0, e1;
+        HZ_CHECK(hipEventCreate(&e0));
+        HZ_CHECK(hipEventCreate(&e1));
+        auto time_it = [&](auto &&launch) -> double {
+            launch();   // warm
+            HZ_CHECK(hipEventRecord(e0, stream()));
+            for (int r = 0; r < reps; ++r) launch();
+            HZ_CHECK(hipEventRecord(e1, stream()));
+            HZ_CHECK(hipEventSynchronize(e1));
+            float ms = 0.f;
+            HZ_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            return (double)ms / reps;
+        };
+        out->reps = reps;
+        out->spmv_ms = time_it([&] { spmv(S, x, y, nullptr, ws); });
+        out->residual_ms = time_it([&] { residual(S, b, x, y); });
+        out->jacobi_ms = time_it([&] { smooth_sweep(S, HZ_SMOOTHER_JACOBI, 1.0, b, x, y); });
+        out->spmv_dot_ms = time_it([&] { spmv(S, x, y, dot_out, ws); });
+        out->sell_entries = S.nent;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return HZ_SUCCESS;
+    } catch (const DeviceError &) {
+        return HZ_ERROR_DEVICE;
+    }
+}
+
+INT hz_rap_agg(dCSRmat *A, const INT *aggregates, INT num_aggregates, dCSRmat *Ac) {
+    if (!device_ready()) return HZ_ERROR_DEVICE;
+    try {
+        arena().reset();
+        DevCSR dA = upload_csr(A);
+        int *dagg = upload(aggregates, (size_t)A->row);
+        int *rptr = arena().alloc_n<int>((size_t)num_aggregates + 1);
+        int *rmem = arena().alloc_n<int>((size_t)A->row + 1);
+        build_restriction(A->row, dagg, num_aggregates, rptr, rmem);
+        DevCSR dC;
+        rap_agg(dA, dagg, num_aggregates, rptr, rmem, dC);
+        download_csr(dC, Ac);
+        return HZ_SUCCESS;
+    } catch (const DeviceError &) {
+        return HZ_ERROR_DEVICE;
+    }
+}
+
+}  // extern "C"

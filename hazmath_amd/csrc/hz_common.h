@@ -1,0 +1,150 @@
+// hz_common.h -- shared host/device helpers of the MI355X UA-AMG-PCG library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "hazmath_mi355x.h"
+
+namespace hz {
+
+constexpr int kWave = 64;          // CDNA wavefront width
+constexpr int kSlice = 64;         // SELL slice height: one row per lane of a wavefront
+constexpr double kSmallReal = 1e-20;  // SMALLREAL, include/macro.h:24
+constexpr double kBigReal = 1e+20;    // BIGREAL, include/macro.h:23
+
+struct DeviceError {
+    hipError_t err;
+    const char *what;
+};
+
+#define HZ_CHECK(call)                                                                  \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            std::fprintf(stderr, "hazmath_mi355x: %s failed: %s (%s:%d)\n", #call,      \
+                         hipGetErrorString(e_), __FILE__, __LINE__);                    \
+            throw ::hz::DeviceError{e_, #call};                                         \
+        }                                                                               \
+    } while (0)
+
+#define HZ_LAUNCH_CHECK() HZ_CHECK(hipGetLastError())
+
+// Bump allocator over large hipMalloc'd chunks.  Chunks are kept for the life of
+// the process and reused by later solves, so a solve performs no hipMalloc once
+// the pool is warm.  Not thread-safe (the reference path is not re-entrant).
+class Arena {
+  public:
+    void *alloc(size_t bytes);
+    template <class T>
+    T *alloc_n(size_t n) { return static_cast<T *>(alloc(n * sizeof(T))); }
+    void reset();            // forget all allocations, keep chunks
+    void release();          // hipFree every chunk
+    size_t bytes_in_use() const;
+  private:
+    struct Chunk { char *base; size_t size; size_t used; };
+    std::vector<Chunk> chunks_;
+    size_t cur_ = 0;
+};
+
+Arena &arena();
+hipStream_t stream();
+
+// Every device<->host copy goes through the library's stream: a blocking
+// hipMemcpy runs on the null stream, which does not wait for work queued on a
+// non-blocking stream.
+inline hipError_t memcpy_sync(void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, stream());
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(stream());
+}
+
+inline int div_up(long a, long b) { return static_cast<int>((a + b - 1) / b); }
+
+// ---------------------------------------------------------------------------
+// Deterministic reductions: fixed tree inside a block, block partials stored by
+// block index, the last block to finish (atomic ticket) sums the partials in
+// index order.  The result depends only on the grid size, never on timing.
+// ---------------------------------------------------------------------------
+template <int BLOCK>
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < BLOCK / kWave; ++i) r += sh[i];
+    }
+    __syncthreads();
+    return r;  // valid in thread 0
+}
+
+template <int BLOCK>
+__device__ __forceinline__ double block_max(double v, double *sh) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));
+    const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int i = 0; i < BLOCK / kWave; ++i) r = fmax(r, sh[i]);
+    }
+    __syncthreads();
+    return r;
+}
+
+// Called by every thread of a block after thread 0 wrote its partial(s).
+// Returns true in the block that must finalize (all partials visible).
+__device__ __forceinline__ bool last_block_ticket(unsigned int *ticket) {
+    __shared__ bool is_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned int t = atomicAdd(ticket, 1u);
+        is_last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (is_last) __threadfence();
+    return is_last;
+}
+
+// Sum of partials[0..n) by one block in a fixed order (strided per thread,
+// then the block tree).
+template <int BLOCK>
+__device__ __forceinline__ double final_sum(const double *partials, int n, double *sh) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += BLOCK) v += partials[i];
+    return block_sum<BLOCK>(v, sh);
+}
+
+template <int BLOCK>
+__device__ __forceinline__ double final_max(const double *partials, int n, double *sh) {
+    double v = 0.0;
+    for (int i = threadIdx.x; i < n; i += BLOCK) v = fmax(v, partials[i]);
+    return block_max<BLOCK>(v, sh);
+}
+
+// XCD-aware block index: MI355X dispatches consecutive workgroups round-robin
+// over its 8 XCDs.  Remapping gives each XCD a contiguous range of row blocks,
+// so the stencil's neighbouring x entries stay in that XCD's L2.
+__device__ __forceinline__ int xcd_block_id() {
+    const int nxcd = 8;
+    const int b = blockIdx.x, g = gridDim.x;
+    const int per = g / nxcd, rem = g % nxcd;
+    const int xcd = b % nxcd, k = b / nxcd;
+    // blocks with xcd < rem get one extra slot
+    const int start = xcd * per + (xcd < rem ? xcd : rem);
+    return start + k;
+}
+
+}  // namespace hz

@@ -1,0 +1,114 @@
+// hz_internal.h -- device data structures and kernel entry points (host side).
+#pragma once
+
+#include "hz_common.h"
+
+namespace hz {
+
+// CSR on the device (0-based, any column order).  Same arrays as dCSRmat.
+struct DevCSR {
+    int n = 0, ncol = 0, nnz = 0;
+    int *ia = nullptr, *ja = nullptr;
+    double *val = nullptr;
+};
+
+// SELL-64: rows are grouped in slices of 64 consecutive rows; slice s stores
+// width_s columns-major entries (entry k of row 64s+lane at sptr[s]+64k+lane).
+// Row order inside each row is the CSR order, so per-row sums are formed in the
+// same order as the CSR loops of the reference.
+struct DevSELL {
+    int n = 0, nslices = 0;
+    long long nent = 0;        // padded entries
+    long long *sptr = nullptr; // nslices+1 offsets
+    int *len = nullptr;        // row lengths
+    int *col = nullptr;
+    double *val = nullptr;
+};
+
+// Device scalars of the PCG (one 256-byte block; host copies it back once per
+// iteration).
+struct PcgScalars {
+    double temp1;        // (z, r) of the previous step
+    double temp2;        // (A p, p) / (z, r)
+    double alpha, beta;
+    double rr;           // ||r||^2
+    double uinf;         // ||u||_inf
+    double uu;           // ||u||^2
+    double pp;           // ||p||^2
+    double zr;           // (z, r)
+    double xb, xax;      // coarse scaling: (x,b) and x^T A x
+    double pad[21];
+};
+
+// Reduction workspace (partials + tickets), sized for the largest grid used.
+struct ReduceWs {
+    double *partials = nullptr;   // kMaxBlocks * kSlots
+    unsigned int *tickets = nullptr;
+    static constexpr int kMaxBlocks = 8192;
+    static constexpr int kSlots = 4;
+};
+
+// ---- kernels_solve.hip ----------------------------------------------------
+void sell_from_csr(const DevCSR &A, DevSELL &S);
+// y = A x ; if dot_out: *dot_out = sum_i y_i * x_i (deterministic)
+void spmv(const DevSELL &S, const double *x, double *y, double *dot_out, const ReduceWs &ws);
+// w = b - A x
+void residual(const DevSELL &S, const double *b, const double *x, double *w);
+// One smoothing sweep out of place: xout = sweep(x).  smoother: 1 Jacobi (weight w), 10 L1DIAG.
+void smooth_sweep(const DevSELL &S, int smoother, double w, const double *b, const double *x,
+                  double *xout);
+// First sweep from a zero iterate: x = w*b/d (Jacobi) or b/d_l1 (L1DIAG); d precomputed.
+void smooth_from_zero(int n, int smoother, double w, const double *d, const double *b, double *x);
+// Smoother diagonal: Jacobi uses the last stored diagonal entry of each row
+// (smoother.c:57-60), L1DIAG uses the row's l1 norm (smoother.c:352-357).
+void smoother_diag(const DevSELL &S, int smoother, double *d);
+// bc = R w (member sums, ascending members);  optional fused zero-iterate sweep of the coarse level.
+void restrict_agg(int nc, const int *rptr, const int *rmem, const double *w, double *bc,
+                  int smoother, double wgt, const double *dc, double *xc);
+// x += alpha * P xc  (aggregate gather-add)
+void prolong_agg(int n, const int *agg, const double *xc, double alpha, const double *alpha_dev,
+                 double *x);
+// vector helpers
+void fill(double *x, double v, long n);
+void copy(double *dst, const double *src, long n);
+// PCG fused vector kernels (scalars live on the device)
+void pcg_update(int n, PcgScalars *s, const double *p, const double *t, double *u, double *r,
+                double *alpha_hist, int iter, const ReduceWs &ws);
+void dot(int n, const double *x, const double *y, double *out, const ReduceWs &ws);
+void norm2sq(int n, const double *x, double *out, const ReduceWs &ws);
+// beta = zr/temp1; temp1 = zr; p = z + beta p ; beta_hist[iter-1] = beta
+void pcg_direction(int n, PcgScalars *s, const double *z, double *p, double *beta_hist, int iter);
+// coarse scaling alpha = min((x,b)/(x^T A x), 2)
+void coarse_scaling_alpha(const DevSELL &S, const double *x, const double *b, PcgScalars *s,
+                          double *alpha_out, const ReduceWs &ws);
+
+// ---- kernels_setup.hip ----------------------------------------------------
+// Strength graph N (construct_strongly_coupled, amg_setup_ua.c:375-452).
+void strength_graph(const DevCSR &A, double theta, DevCSR &N, int *counts_ws);
+// Exclusive scan of n ints into out (n+1 entries, out[n] = total); returns total.
+long long exclusive_scan_int(const int *in, int *out, int n);
+// HEC aggregation (exact restatement of the sequential turn-order algorithm).
+// perm: turn order (perm[t] = vertex); returns number of aggregates or a negative code.
+int hec_aggregate(const DevCSR &N, const int *perm, int *agg);
+// R = P^T: rptr (nagg+1), rmem (members ascending).
+void build_restriction(int n, const int *agg, int nagg, int *rptr, int *rmem);
+// Galerkin product with unit R/P (dcsr_rap_agg semantics).  Allocates Ac arrays.
+void rap_agg(const DevCSR &A, const int *agg, int nagg, const int *rptr, const int *rmem,
+             DevCSR &Ac);
+// Transpose (sorted columns) of a device CSR (dcsr_trans semantics).
+void csr_transpose(const DevCSR &A, DevCSR &AT);
+
+// ---- kernels_dense.hip ----------------------------------------------------
+// Dense LDL^T factorization of the coarsest matrix and its explicit inverse.
+// Returns 0 on success.
+int coarse_dense_setup(const DevCSR &A, double *Ainv);
+// x = Ainv b
+void coarse_dense_solve(int n, const double *Ainv, const double *b, double *x);
+// Single-workgroup CG for coarse_solver = 0 (coarse_itsolver, mgcycle.c:34-53).
+void coarse_cg(const DevSELL &S, const double *b, double *x, double tol, int maxit, double *work);
+
+// ---- HEC turn order -------------------------------------------------------
+// Turn order of HEC from libc rand() (iarray_shuffle, array.c:302-318).
+void hec_turn_order(int n, int *perm_dev, int rng_source);
+
+}  // namespace hz

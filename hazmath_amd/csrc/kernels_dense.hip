@@ -1,0 +1,262 @@
+// kernels_dense.hip -- coarsest-level solvers on gfx950.
+//
+// coarse_solver = 32 (SOLVER_UMFPACK): the reference factors the coarsest
+// matrix once with its sparse U^T D U (src/solver/hazmath_direct.c:633-854)
+// and solves with it every cycle.  Here the coarsest matrix is factored once
+// as a dense L D L^T and its explicit inverse is formed; each cycle then costs
+// one dense matrix-vector product spread over many workgroups instead of two
+// latency-bound triangular sweeps.  Results equal the reference's to rounding.
+//
+// coarse_solver = 0: unpreconditioned CG to the cycle tolerance
+// (coarse_itsolver, src/solver/mgcycle.c:34-53 -> dcsr_pcg, krylov.c:122-372),
+// run inside one workgroup so the whole coarse solve is a single launch.
+#include "hz_internal.h"
+
+namespace hz {
+
+namespace {
+
+constexpr int kDenseBlock = 1024;
+
+__global__ void k_dense_from_csr(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                                 const double *__restrict__ val, double *__restrict__ M) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        for (int k = ia[i]; k < ia[i + 1]; ++k) M[(size_t)i * n + ja[k]] += val[k];
+}
+
+// Right-looking L D L^T in place (lower triangle holds L, diagonal holds D),
+// then X = L^{-1} and Ainv = X^T D^{-1} X.  One workgroup; n is the coarsest
+// size (small).  status[0] = 1 on a zero pivot.
+__global__ __launch_bounds__(kDenseBlock) void k_ldlt_inverse(int n, double *__restrict__ M,
+                                                              double *__restrict__ X,
+                                                              double *__restrict__ Ainv,
+                                                              int *__restrict__ status) {
+    const int tid = threadIdx.x;
+    for (int j = 0; j < n; ++j) {
+        const double dj = M[(size_t)j * n + j];
+        if (dj == 0.0) {
+            if (tid == 0) status[0] = 1;
+            return;
+        }
+        // column j of L
+        for (int i = j + 1 + tid; i < n; i += kDenseBlock) M[(size_t)i * n + j] = M[(size_t)i * n + j] / dj;
+        __syncthreads();
+        // trailing lower triangle: M[i][k] -= L[i][j] * dj * L[k][j], j < k <= i
+        const int m = n - j - 1;
+        const long tri = (long)m * (m + 1) / 2;
+        for (long t = tid; t < tri; t += kDenseBlock) {
+            // map t -> (i, k) with 0 <= k <= i < m
+            int i = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+            while ((long)(i + 1) * (i + 2) / 2 <= t) ++i;
+            while ((long)i * (i + 1) / 2 > t) --i;
+            const int k = (int)(t - (long)i * (i + 1) / 2);
+            const int gi = j + 1 + i, gk = j + 1 + k;
+            M[(size_t)gi * n + gk] -= M[(size_t)gi * n + j] * dj * M[(size_t)gk * n + j];
+        }
+        __syncthreads();
+    }
+    // X = L^{-1}: column c solves L x = e_c (unit lower triangular)
+    for (int c = tid; c < n; c += kDenseBlock) {
+        for (int i = 0; i < c; ++i) X[(size_t)i * n + c] = 0.0;
+        X[(size_t)c * n + c] = 1.0;
+        for (int i = c + 1; i < n; ++i) {
+            double s = 0.0;
+            for (int k = c; k < i; ++k) s += M[(size_t)i * n + k] * X[(size_t)k * n + c];
+            X[(size_t)i * n + c] = -s;
+        }
+    }
+    __syncthreads();
+    // Ainv[i][j] = sum_{k >= max(i,j)} X[k][i] X[k][j] / D[k]
+    const long nn = (long)n * n;
+    for (long t = tid; t < nn; t += kDenseBlock) {
+        const int i = (int)(t / n), jj = (int)(t % n);
+        const int k0 = i > jj ? i : jj;
+        double s = 0.0;
+        for (int k = k0; k < n; ++k) s += X[(size_t)k * n + i] * X[(size_t)k * n + jj] / M[(size_t)k * n + k];
+        Ainv[t] = s;
+    }
+}
+
+// x = Ainv b, one wavefront per row block of 4 rows.
+__global__ __launch_bounds__(256) void k_gemv(int n, const double *__restrict__ Ainv,
+                                              const double *__restrict__ b, double *__restrict__ x) {
+    const int row = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (row >= n) return;
+    double s = 0.0;
+    const double *a = Ainv + (size_t)row * n;
+    for (int k = lane; k < n; k += kWave) s += a[k] * b[k];
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, kWave);
+    if (lane == 0) x[row] = s;
+}
+
+// ---------------------------------------------------------------------------
+// One-workgroup CG without preconditioner, stop type 1 (||r||/||r0||), with the
+// reference's safeguards (dcsr_pcg, krylov.c:122-372).  x holds the initial
+// guess (zero in the cycle).  work: 4n doubles.
+// ---------------------------------------------------------------------------
+constexpr int kCgBlock = 1024;
+
+__device__ double cg_block_sum(double v, double *sh) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double r = 0.0;
+#pragma unroll
+    for (int i = 0; i < kCgBlock / kWave; ++i) r += sh[i];
+    return r;   // every thread gets the same value
+}
+
+__device__ double cg_block_max(double v, double *sh) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));
+    const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double r = 0.0;
+#pragma unroll
+    for (int i = 0; i < kCgBlock / kWave; ++i) r = fmax(r, sh[i]);
+    return r;
+}
+
+__device__ void cg_matvec(int n, int nslices, const long long *sptr, const int *col, const double *val,
+                          const double *x, double *y, double alpha_mode, const double *b) {
+    // alpha_mode 0: y = A x ; 1: y = b - A x
+    for (int s = threadIdx.x / kWave; s < nslices; s += kCgBlock / kWave) {
+        const int lane = threadIdx.x % kWave;
+        const int r = s * kSlice + lane;
+        const long long base = sptr[s];
+        const int wd = (int)((sptr[s + 1] - base) / kSlice);
+        double acc = 0.0;
+        for (int k = 0; k < wd; ++k) {
+            const int c = col[base + (long long)k * kSlice + lane];
+            if (c >= 0) acc = acc + val[base + (long long)k * kSlice + lane] * x[c];
+        }
+        if (r < n) y[r] = (alpha_mode == 0.0) ? acc : b[r] - acc;
+    }
+}
+
+__global__ __launch_bounds__(kCgBlock) void k_coarse_cg(int n, int nslices, const long long *__restrict__ sptr,
+                                                        const int *__restrict__ col,
+                                                        const double *__restrict__ val,
+                                                        const double *__restrict__ b,
+                                                        double *__restrict__ u, double tol, int maxit,
+                                                        double *__restrict__ work) {
+    __shared__ double sh[kCgBlock / kWave];
+    double *p = work, *r = work + n, *t = work + 2 * n;
+    const int tid = threadIdx.x;
+    const double maxdiff = tol * 1e-4;
+    int iter = 0, stag = 1, more_step = 1;
+    cg_matvec(n, nslices, sptr, col, val, u, r, 1.0, b);
+    __syncthreads();
+    double acc = 0.0;
+    for (int i = tid; i < n; i += kCgBlock) acc += r[i] * r[i];
+    double absres0 = sqrt(cg_block_sum(acc, sh));
+    const double normr0 = absres0 > kSmallReal ? absres0 : kSmallReal;
+    double relres = absres0 / normr0, absres = absres0;
+    if (relres < tol || absres0 < 1e-3 * tol) return;
+    for (int i = tid; i < n; i += kCgBlock) p[i] = r[i];
+    acc = 0.0;
+    for (int i = tid; i < n; i += kCgBlock) acc += r[i] * r[i];
+    double temp1 = cg_block_sum(acc, sh);
+    while (iter++ < maxit) {
+        __syncthreads();
+        cg_matvec(n, nslices, sptr, col, val, p, t, 0.0, nullptr);
+        __syncthreads();
+        acc = 0.0;
+        for (int i = tid; i < n; i += kCgBlock) acc += t[i] * p[i];
+        const double temp2 = cg_block_sum(acc, sh);
+        const double alpha = temp1 / temp2;
+        double rr = 0.0, ui = 0.0, uu = 0.0, pp = 0.0;
+        for (int i = tid; i < n; i += kCgBlock) {
+            const double ui_ = u[i] + alpha * p[i];
+            const double ri = r[i] + (-alpha) * t[i];
+            u[i] = ui_;
+            r[i] = ri;
+            rr += ri * ri;
+            ui = fmax(ui, fabs(ui_));
+            uu += ui_ * ui_;
+            pp += p[i] * p[i];
+        }
+        absres = sqrt(cg_block_sum(rr, sh));
+        relres = absres / normr0;
+        const double infnormu = cg_block_max(ui, sh);
+        if (infnormu <= kSmallReal) return;                       // Check I
+        const double normu = sqrt(cg_block_sum(uu, sh));
+        const double reldiff = fabs(alpha) * sqrt(cg_block_sum(pp, sh)) / normu;
+        if ((stag <= 20) & (reldiff < maxdiff)) {                  // Check II
+            __syncthreads();
+            cg_matvec(n, nslices, sptr, col, val, u, r, 1.0, b);
+            __syncthreads();
+            acc = 0.0;
+            for (int i = tid; i < n; i += kCgBlock) acc += r[i] * r[i];
+            absres = sqrt(cg_block_sum(acc, sh));
+            relres = absres / normr0;
+            if (relres < tol) return;
+            if (stag >= 20) return;
+            for (int i = tid; i < n; i += kCgBlock) p[i] = 0.0;
+            ++stag;
+        }
+        if (relres < tol) {                                        // Check III
+            __syncthreads();
+            cg_matvec(n, nslices, sptr, col, val, u, r, 1.0, b);
+            __syncthreads();
+            acc = 0.0;
+            for (int i = tid; i < n; i += kCgBlock) acc += r[i] * r[i];
+            absres = sqrt(cg_block_sum(acc, sh));
+            relres = absres / normr0;
+            if (relres < tol) return;
+            if (more_step >= 20) return;
+            for (int i = tid; i < n; i += kCgBlock) p[i] = 0.0;
+            ++more_step;
+        }
+        absres0 = absres;
+        __syncthreads();
+        acc = 0.0;
+        for (int i = tid; i < n; i += kCgBlock) acc += r[i] * r[i];
+        const double zr = cg_block_sum(acc, sh);
+        const double beta = zr / temp1;
+        temp1 = zr;
+        for (int i = tid; i < n; i += kCgBlock) p[i] = 1.0 * r[i] + beta * p[i];
+    }
+}
+
+}  // namespace
+
+int coarse_dense_setup(const DevCSR &A, double *Ainv) {
+    const int n = A.n;
+    hipStream_t st = stream();
+    double *M = arena().alloc_n<double>((size_t)n * n + 1);
+    double *X = arena().alloc_n<double>((size_t)n * n + 1);
+    int *status = arena().alloc_n<int>(1);
+    HZ_CHECK(hipMemsetAsync(M, 0, sizeof(double) * (size_t)n * n, st));
+    HZ_CHECK(hipMemsetAsync(status, 0, sizeof(int), st));
+    k_dense_from_csr<<<div_up(n, 256), 256, 0, st>>>(n, A.ia, A.ja, A.val, M);
+    HZ_LAUNCH_CHECK();
+    k_ldlt_inverse<<<1, kDenseBlock, 0, st>>>(n, M, X, Ainv, status);
+    HZ_LAUNCH_CHECK();
+    int hs = 0;
+    HZ_CHECK(hipMemcpyAsync(&hs, status, sizeof(int), hipMemcpyDeviceToHost, st));
+    HZ_CHECK(hipStreamSynchronize(st));
+    return hs;
+}
+
+void coarse_dense_solve(int n, const double *Ainv, const double *b, double *x) {
+    if (n == 0) return;
+    k_gemv<<<div_up(n, 4), 256, 0, stream()>>>(n, Ainv, b, x);
+    HZ_LAUNCH_CHECK();
+}
+
+void coarse_cg(const DevSELL &S, const double *b, double *x, double tol, int maxit, double *work) {
+    if (S.n == 0) return;
+    k_coarse_cg<<<1, kCgBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col, S.val, b, x, tol, maxit,
+                                              work);
+    HZ_LAUNCH_CHECK();
+}
+
+}  // namespace hz

@@ -1,0 +1,461 @@
+// kernels_setup.hip -- AMG setup kernels on gfx950: strength graph, HEC
+// aggregation, restriction lists and the Galerkin product with unit R/P.
+//
+// All results are bit-identical to the sequential reference
+// (src/solver/amg_setup_ua.c, src/utilities/sparse.c):
+//  * the strength graph keeps each row's CSR order;
+//  * HEC is resolved exactly from the turn order (see hec_aggregate);
+//  * R lists each aggregate's members in ascending fine index (stable sort);
+//  * the coarse row layout (diagonal first, then columns in first-discovery
+//    order) and every coarse value (a sum of fine entries in discovery order)
+//    are reproduced, so coarser levels aggregate identically.
+#include <climits>
+
+#include <cstring>
+
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include "hz_internal.h"
+
+namespace hz {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// ---------------------------------------------------------------------------
+// Strength graph: N = {j == i} U {a_ij < 0 and a_ij^2 >= theta^2 |a_ii a_jj|}
+// with a_ii the FIRST stored diagonal entry (dcsr_getdiag, sparse.c:2168-2189).
+// ---------------------------------------------------------------------------
+__global__ void k_first_diag(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                             const double *__restrict__ val, double *__restrict__ d) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double di = 0.0;
+        for (int k = ia[i]; k < ia[i + 1]; ++k)
+            if (ja[k] == i) { di = val[k]; break; }
+        d[i] = di;
+    }
+}
+
+template <bool kFill>
+__global__ void k_strength(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                           const double *__restrict__ val, const double *__restrict__ d, double th2,
+                           int *__restrict__ count, const int *__restrict__ nia,
+                           int *__restrict__ nja, double *__restrict__ nval) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double di = d[i];
+        int pos = kFill ? nia[i] : 0;
+        int cnt = 0;
+        for (int k = ia[i]; k < ia[i + 1]; ++k) {
+            const int j = ja[k];
+            const double a = val[k];
+            if (j == i || ((a * a) >= th2 * fabs(di * d[j]) && a < 0e0)) {
+                if (kFill) {
+                    nja[pos] = j;
+                    nval[pos] = a;
+                    ++pos;
+                }
+                ++cnt;
+            }
+        }
+        if (!kFill) count[i] = cnt;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// HEC (aggregation_hec, amg_setup_ua.c:752-845) resolved in parallel.
+//
+// In the sequential loop, vertex i (turn t(i)) with strongest neighbour j:
+//  - t(j) < t(i): i copies j's final value                       (FOLLOW j)
+//  - t(j) > t(i) and i is the first vertex (in turn order) whose target is j:
+//    i opens a new aggregate for {i, j}                          (CREATOR)
+//  - t(j) > t(i) otherwise: j still holds the id opened by that first
+//    vertex m0, and i takes it                                    (JOIN m0)
+// A vertex's own turn always overwrites its value, so FOLLOW chains end at
+// a CREATOR, a JOIN or an isolated vertex, and ids are the creators' ranks in
+// turn order.
+// ---------------------------------------------------------------------------
+enum : int { kStUnpt = 0, kStCreator = 1, kStJoin = 2, kStFollow = 3 };
+
+__global__ void k_hec_target(int n, const int *__restrict__ nia, const int *__restrict__ nja,
+                             const double *__restrict__ nval, int *__restrict__ target,
+                             int *__restrict__ err) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int b = nia[i], e = nia[i + 1];
+        if (e - b == 1) { target[i] = -1; continue; }
+        int best = -1;
+        double m = 0.0;
+        for (int k = b; k < e; ++k)
+            if (nja[k] != i && fabs(nval[k]) > m) { best = k; m = fabs(nval[k]); }
+        if (best < 0) {
+            m = -1e0;
+            for (int k = b; k < e; ++k)
+                if (nja[k] != i && fabs(nval[k]) > m) { best = k; m = fabs(nval[k]); }
+        }
+        if (best < 0) {
+            atomicExch(err, 1);
+            target[i] = -2;
+            continue;
+        }
+        target[i] = nja[best];
+    }
+}
+
+__global__ void k_turn_pos(int n, const int *__restrict__ perm, int *__restrict__ tpos) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x)
+        tpos[perm[t]] = t;
+}
+
+__global__ void k_fill_int(int *__restrict__ x, int v, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] = v;
+}
+
+__global__ void k_hec_first(int n, const int *__restrict__ target, const int *__restrict__ tpos,
+                            int *__restrict__ first) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int j = target[i];
+        if (j >= 0) atomicMin(&first[j], tpos[i]);
+    }
+}
+
+__global__ void k_hec_classify(int n, const int *__restrict__ target, const int *__restrict__ tpos,
+                               const int *__restrict__ first, int *__restrict__ state,
+                               int *__restrict__ cflag) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int j = target[i];
+        int st;
+        if (j < 0) st = kStUnpt;
+        else if (tpos[j] < tpos[i]) st = kStFollow;
+        else if (first[j] == tpos[i]) st = kStCreator;
+        else st = kStJoin;
+        state[i] = st;
+        cflag[tpos[i]] = (st == kStCreator) ? 1 : 0;
+    }
+}
+
+__global__ void k_hec_resolve(int n, const int *__restrict__ target, const int *__restrict__ tpos,
+                              const int *__restrict__ first, const int *__restrict__ state,
+                              const int *__restrict__ cid, int *__restrict__ agg) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int v = i;
+        while (state[v] == kStFollow) v = target[v];
+        const int st = state[v];
+        int a;
+        if (st == kStUnpt) a = HZ_UNPT;
+        else if (st == kStCreator) a = cid[tpos[v]];
+        else a = cid[first[target[v]]];
+        agg[i] = a;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// R = P^T
+// ---------------------------------------------------------------------------
+__global__ void k_agg_keys(int n, const int *__restrict__ agg, int nagg, unsigned int *__restrict__ keys,
+                           int *__restrict__ vals) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int a = agg[i];
+        keys[i] = (a > HZ_UNPT) ? (unsigned int)a : (unsigned int)nagg;
+        vals[i] = i;
+    }
+}
+
+__global__ void k_agg_count(int n, const int *__restrict__ agg, int *__restrict__ cnt) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int a = agg[i];
+        if (a > HZ_UNPT) atomicAdd(&cnt[a], 1);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Galerkin product with unit R/P (dcsr_rap_agg, sparse.c:2557-2683).
+// One wavefront per coarse row c.  The row's candidate sequence is the scan
+//   for member i1 of c (ascending), for k in A row i1 (stored order):
+//       (agg[A.ja[k]], A.val[k])       -- skipped when agg == UNPT
+// Output: slot 0 = (c, 0.0 + sum of candidates with column c, in order),
+// then one slot per other column in order of first appearance, whose value is
+// the in-order sum of its candidates starting from the first one.
+// ---------------------------------------------------------------------------
+constexpr int kRapCap = 512;    // candidates held in LDS per row (larger rows use global scratch)
+
+template <bool kFill, bool kGlobal>
+__global__ __launch_bounds__(kWave) void k_rap_row(
+    int nrows, const int *__restrict__ rowlist, const int *__restrict__ rptr,
+    const int *__restrict__ rmem, const int *__restrict__ aia, const int *__restrict__ aja,
+    const double *__restrict__ aval, const int *__restrict__ agg, int *__restrict__ rowcount,
+    int *__restrict__ rowlen, const int *__restrict__ cia, int *__restrict__ cja,
+    double *__restrict__ cval, int *__restrict__ gcol, double *__restrict__ gval,
+    const long long *__restrict__ goff) {
+    __shared__ int s_col[kGlobal ? 1 : kRapCap];
+    __shared__ double s_val[(kGlobal || !kFill) ? 1 : kRapCap];
+    const int lane = threadIdx.x;
+    const int idx = blockIdx.x;
+    if (idx >= nrows) return;
+    const int c = rowlist ? rowlist[idx] : idx;
+    int *col = kGlobal ? gcol + goff[idx] : s_col;
+    double *val = kGlobal ? gval + goff[idx] : s_val;
+    // 1) enumerate candidates
+    int L = 0;
+    const int q1 = rptr[c + 1];
+    for (int q = rptr[c]; q < q1; ++q) {
+        const int i1 = rmem[q];
+        const int b = aia[i1], len = aia[i1 + 1] - b;
+        for (int k = lane; k < len; k += kWave) {
+            const int p = L + k;
+            if (kGlobal || p < kRapCap) {
+                col[p] = agg[aja[b + k]];
+                if (kFill) val[p] = aval[b + k];
+            }
+        }
+        L += len;
+    }
+    if (!kGlobal && L > kRapCap) {   // handled by the global-memory variant
+        if (!kFill && lane == 0) { rowcount[c] = -1; rowlen[c] = L; }
+        return;
+    }
+    __syncthreads();
+    // 2) first appearances (excluding UNPT and the diagonal column c)
+    int nfirst_before = 0;   // running count of firsts in earlier chunks
+    for (int base = 0; base < L; base += kWave) {
+        const int p = base + lane;
+        bool first = false;
+        int cp = -1;
+        if (p < L) {
+            cp = col[p];
+            first = (cp > HZ_UNPT) && (cp != c);
+            for (int q = 0; q < p && first; ++q)
+                if (col[q] == cp) first = false;
+        }
+        const unsigned long long mask = __ballot(first);
+        if (kFill && first) {
+            const int rank = nfirst_before + __popcll(mask & ((1ull << lane) - 1ull));
+            double v = val[p];
+            for (int q = p + 1; q < L; ++q)
+                if (col[q] == cp) v = v + val[q];
+            const int pos = cia[c] + 1 + rank;
+            cja[pos] = cp;
+            cval[pos] = v;
+        }
+        nfirst_before += __popcll(mask);
+    }
+    if (lane == 0) {
+        if (kFill) {
+            double v = 0.0;
+            for (int q = 0; q < L; ++q)
+                if (col[q] == c) v = v + val[q];
+            cja[cia[c]] = c;
+            cval[cia[c]] = v;
+        } else {
+            rowcount[c] = 1 + nfirst_before;
+            if (!kGlobal) rowlen[c] = L;
+        }
+    }
+}
+
+__global__ void k_collect_overflow(int nc, const int *__restrict__ rowcount, int *__restrict__ list,
+                                   int *__restrict__ nlist) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x)
+        if (rowcount[c] < 0) list[atomicAdd(nlist, 1)] = c;
+}
+
+inline int grid_for(long n) {
+    long g = (n + kBlock - 1) / kBlock;
+    if (g > 4096) g = 4096;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+}  // namespace
+
+long long exclusive_scan_int(const int *in, int *out, int n) {
+    hipStream_t st = stream();
+    size_t tmp_bytes = 0;
+    // out has n+1 entries; scan n+1 inputs where the last input is treated as 0
+    int *in_ext = arena().alloc_n<int>((size_t)n + 1);
+    if (n > 0) HZ_CHECK(hipMemcpyAsync(in_ext, in, sizeof(int) * (size_t)n, hipMemcpyDeviceToDevice, st));
+    HZ_CHECK(hipMemsetAsync(in_ext + n, 0, sizeof(int), st));
+    HZ_CHECK(rocprim::exclusive_scan(nullptr, tmp_bytes, in_ext, out, 0, (size_t)n + 1,
+                                     rocprim::plus<int>(), st));
+    void *tmp = arena().alloc(tmp_bytes);
+    HZ_CHECK(rocprim::exclusive_scan(tmp, tmp_bytes, in_ext, out, 0, (size_t)n + 1,
+                                     rocprim::plus<int>(), st));
+    int total = 0;
+    HZ_CHECK(hipMemcpyAsync(&total, out + n, sizeof(int), hipMemcpyDeviceToHost, st));
+    HZ_CHECK(hipStreamSynchronize(st));
+    return total;
+}
+
+void strength_graph(const DevCSR &A, double theta, DevCSR &N, int *counts_ws) {
+    hipStream_t st = stream();
+    const int n = A.n;
+    double *d = arena().alloc_n<double>((size_t)n + 1);
+    k_first_diag<<<grid_for(n), kBlock, 0, st>>>(n, A.ia, A.ja, A.val, d);
+    HZ_LAUNCH_CHECK();
+    const double th2 = theta * theta;
+    k_strength<false><<<grid_for(n), kBlock, 0, st>>>(n, A.ia, A.ja, A.val, d, th2, counts_ws,
+                                                      nullptr, nullptr, nullptr);
+    HZ_LAUNCH_CHECK();
+    N.n = n;
+    N.ncol = A.ncol;
+    N.ia = arena().alloc_n<int>((size_t)n + 1);
+    N.nnz = (int)exclusive_scan_int(counts_ws, N.ia, n);
+    N.ja = arena().alloc_n<int>((size_t)N.nnz + 1);
+    N.val = arena().alloc_n<double>((size_t)N.nnz + 1);
+    k_strength<true><<<grid_for(n), kBlock, 0, st>>>(n, A.ia, A.ja, A.val, d, th2, nullptr, N.ia,
+                                                     N.ja, N.val);
+    HZ_LAUNCH_CHECK();
+}
+
+int hec_aggregate(const DevCSR &N, const int *perm, int *agg) {
+    hipStream_t st = stream();
+    const int n = N.n;
+    int *target = arena().alloc_n<int>((size_t)n + 1);
+    int *tpos = arena().alloc_n<int>((size_t)n + 1);
+    int *first = arena().alloc_n<int>((size_t)n + 1);
+    int *state = arena().alloc_n<int>((size_t)n + 1);
+    int *cflag = arena().alloc_n<int>((size_t)n + 1);
+    int *cid = arena().alloc_n<int>((size_t)n + 1);
+    int *err = arena().alloc_n<int>(1);
+    HZ_CHECK(hipMemsetAsync(err, 0, sizeof(int), st));
+    const int g = grid_for(n);
+    k_hec_target<<<g, kBlock, 0, st>>>(n, N.ia, N.ja, N.val, target, err);
+    k_turn_pos<<<g, kBlock, 0, st>>>(n, perm, tpos);
+    k_fill_int<<<g, kBlock, 0, st>>>(first, INT_MAX, n);
+    k_hec_first<<<g, kBlock, 0, st>>>(n, target, tpos, first);
+    k_hec_classify<<<g, kBlock, 0, st>>>(n, target, tpos, first, state, cflag);
+    HZ_LAUNCH_CHECK();
+    int herr = 0;
+    HZ_CHECK(hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, st));
+    const int nagg = (int)exclusive_scan_int(cflag, cid, n);   // synchronizes
+    if (herr) return -1016;
+    k_hec_resolve<<<g, kBlock, 0, st>>>(n, target, tpos, first, state, cid, agg);
+    HZ_LAUNCH_CHECK();
+    return nagg;
+}
+
+void build_restriction(int n, const int *agg, int nagg, int *rptr, int *rmem) {
+    hipStream_t st = stream();
+    int *cnt = arena().alloc_n<int>((size_t)nagg + 1);
+    HZ_CHECK(hipMemsetAsync(cnt, 0, sizeof(int) * ((size_t)nagg + 1), st));
+    k_agg_count<<<grid_for(n), kBlock, 0, st>>>(n, agg, cnt);
+    HZ_LAUNCH_CHECK();
+    exclusive_scan_int(cnt, rptr, nagg);
+    // stable radix sort of (aggregate, vertex) pairs: members come out ascending
+    unsigned int *keys = arena().alloc_n<unsigned int>((size_t)n + 1);
+    unsigned int *keys_out = arena().alloc_n<unsigned int>((size_t)n + 1);
+    int *vals = arena().alloc_n<int>((size_t)n + 1);
+    int *vals_out = arena().alloc_n<int>((size_t)n + 1);
+    k_agg_keys<<<grid_for(n), kBlock, 0, st>>>(n, agg, nagg, keys, vals);
+    HZ_LAUNCH_CHECK();
+    int bits = 1;
+    while ((1u << bits) <= (unsigned)nagg && bits < 32) ++bits;
+    size_t tmp_bytes = 0;
+    HZ_CHECK(rocprim::radix_sort_pairs(nullptr, tmp_bytes, keys, keys_out, vals, vals_out, (size_t)n,
+                                       0, bits, st));
+    void *tmp = arena().alloc(tmp_bytes);
+    HZ_CHECK(rocprim::radix_sort_pairs(tmp, tmp_bytes, keys, keys_out, vals, vals_out, (size_t)n, 0,
+                                       bits, st));
+    HZ_CHECK(hipMemcpyAsync(rmem, vals_out, sizeof(int) * (size_t)n, hipMemcpyDeviceToDevice, st));
+}
+
+void rap_agg(const DevCSR &A, const int *agg, int nagg, const int *rptr, const int *rmem,
+             DevCSR &Ac) {
+    hipStream_t st = stream();
+    int *rowcount = arena().alloc_n<int>((size_t)nagg + 1);
+    int *rowlen = arena().alloc_n<int>((size_t)nagg + 1);
+    if (nagg > 0) {
+        k_rap_row<false, false><<<nagg, kWave, 0, st>>>(nagg, nullptr, rptr, rmem, A.ia, A.ja, A.val,
+                                                        agg, rowcount, rowlen, nullptr, nullptr,
+                                                        nullptr, nullptr, nullptr, nullptr);
+        HZ_LAUNCH_CHECK();
+    }
+    // rows whose candidate list exceeds the LDS capacity
+    int *olist = arena().alloc_n<int>((size_t)nagg + 1);
+    int *nol = arena().alloc_n<int>(1);
+    HZ_CHECK(hipMemsetAsync(nol, 0, sizeof(int), st));
+    if (nagg > 0) {
+        k_collect_overflow<<<grid_for(nagg), kBlock, 0, st>>>(nagg, rowcount, olist, nol);
+        HZ_LAUNCH_CHECK();
+    }
+    int hnol = 0;
+    HZ_CHECK(hipMemcpyAsync(&hnol, nol, sizeof(int), hipMemcpyDeviceToHost, st));
+    HZ_CHECK(hipStreamSynchronize(st));
+    int *gcol = nullptr;
+    double *gval = nullptr;
+    long long *goff = nullptr;
+    if (hnol > 0) {
+        // deterministic order of the overflow list does not matter: each row is independent
+        std::vector<int> hl(hnol), hlen(hnol);
+        HZ_CHECK(memcpy_sync(hl.data(), olist, sizeof(int) * hnol, hipMemcpyDeviceToHost));
+        std::vector<long long> hoff(hnol + 1, 0);
+        for (int i = 0; i < hnol; ++i) {
+            int L = 0;
+            HZ_CHECK(memcpy_sync(&L, rowlen + hl[i], sizeof(int), hipMemcpyDeviceToHost));
+            hoff[i + 1] = hoff[i] + L;
+        }
+        goff = arena().alloc_n<long long>((size_t)hnol + 1);
+        HZ_CHECK(memcpy_sync(goff, hoff.data(), sizeof(long long) * (hnol + 1), hipMemcpyHostToDevice));
+        gcol = arena().alloc_n<int>((size_t)hoff[hnol] + 1);
+        gval = arena().alloc_n<double>((size_t)hoff[hnol] + 1);
+        k_rap_row<false, true><<<hnol, kWave, 0, st>>>(hnol, olist, rptr, rmem, A.ia, A.ja, A.val, agg,
+                                                       rowcount, rowlen, nullptr, nullptr, nullptr,
+                                                       gcol, gval, goff);
+        HZ_LAUNCH_CHECK();
+    }
+    Ac.n = nagg;
+    Ac.ncol = nagg;
+    Ac.ia = arena().alloc_n<int>((size_t)nagg + 1);
+    Ac.nnz = (int)exclusive_scan_int(rowcount, Ac.ia, nagg);
+    Ac.ja = arena().alloc_n<int>((size_t)Ac.nnz + 1);
+    Ac.val = arena().alloc_n<double>((size_t)Ac.nnz + 1);
+    if (nagg > 0) {
+        k_rap_row<true, false><<<nagg, kWave, 0, st>>>(nagg, nullptr, rptr, rmem, A.ia, A.ja, A.val, agg,
+                                                       nullptr, nullptr, Ac.ia, Ac.ja, Ac.val,
+                                                       nullptr, nullptr, nullptr);
+        HZ_LAUNCH_CHECK();
+    }
+    if (hnol > 0) {
+        k_rap_row<true, true><<<hnol, kWave, 0, st>>>(hnol, olist, rptr, rmem, A.ia, A.ja, A.val, agg,
+                                                      nullptr, nullptr, Ac.ia, Ac.ja, Ac.val, gcol,
+                                                      gval, goff);
+        HZ_LAUNCH_CHECK();
+    }
+}
+
+void csr_transpose(const DevCSR &A, DevCSR &AT) {
+    // The coarsest matrix is small: transpose on the host with the counting
+    // sort of dcsr_trans (sparse.c:770-830) and upload.
+    const int n = A.n, m = A.ncol, nnz = A.nnz;
+    std::vector<int> ia(n + 1), ja(nnz);
+    std::vector<double> va(nnz);
+    HZ_CHECK(memcpy_sync(ia.data(), A.ia, sizeof(int) * (n + 1), hipMemcpyDeviceToHost));
+    if (nnz) {
+        HZ_CHECK(memcpy_sync(ja.data(), A.ja, sizeof(int) * nnz, hipMemcpyDeviceToHost));
+        HZ_CHECK(memcpy_sync(va.data(), A.val, sizeof(double) * nnz, hipMemcpyDeviceToHost));
+    }
+    std::vector<int> tia(m + 1, 0), tja(nnz);
+    std::vector<double> tva(nnz);
+    for (int k = 0; k < nnz; ++k) tia[ja[k] + 1]++;
+    for (int i = 0; i < m; ++i) tia[i + 1] += tia[i];
+    std::vector<int> nxt(tia.begin(), tia.end());
+    for (int i = 0; i < n; ++i)
+        for (int k = ia[i]; k < ia[i + 1]; ++k) {
+            const int pos = nxt[ja[k]]++;
+            tja[pos] = i;
+            tva[pos] = va[k];
+        }
+    AT.n = m;
+    AT.ncol = n;
+    AT.nnz = nnz;
+    AT.ia = arena().alloc_n<int>((size_t)m + 1);
+    AT.ja = arena().alloc_n<int>((size_t)nnz + 1);
+    AT.val = arena().alloc_n<double>((size_t)nnz + 1);
+    HZ_CHECK(memcpy_sync(AT.ia, tia.data(), sizeof(int) * (m + 1), hipMemcpyHostToDevice));
+    if (nnz) {
+        HZ_CHECK(memcpy_sync(AT.ja, tja.data(), sizeof(int) * nnz, hipMemcpyHostToDevice));
+        HZ_CHECK(memcpy_sync(AT.val, tva.data(), sizeof(double) * nnz, hipMemcpyHostToDevice));
+    }
+}
+
+}  // namespace hz

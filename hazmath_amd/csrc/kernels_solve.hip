@@ -1,0 +1,561 @@
+// kernels_solve.hip -- solve-phase kernels of the UA-AMG-PCG path on gfx950.
+//
+// Matrix kernels use SELL-64: one wavefront owns 64 consecutive rows, lane l
+// owns row 64*s+l and walks that row's entries in CSR order, so every per-row
+// sum is formed in the same order as the reference's CSR loops
+// (src/utilities/sparse.c:1456-1730, src/solver/smoother.c:35-92,324-381) and,
+// built with -ffp-contract=off, rounds identically.  Loads of a slice are
+// column-major (64 lanes x 8 B contiguous per step): fully coalesced, and for
+// stencil matrices the x gathers of a step are contiguous as well.
+#include <cstring>
+
+#include <rocprim/device/device_scan.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include "hz_internal.h"
+
+namespace hz {
+
+namespace {
+
+constexpr int kBlock = 256;               // 4 wavefronts
+constexpr int kSlicesPerBlock = kBlock / kWave;
+constexpr int kStreamGrid = 2048;         // fixed grid of grid-stride vector kernels
+
+__global__ void k_slice_width(int n, const int *__restrict__ ia, long long *__restrict__ wcount,
+                              int nslices) {
+    const int s = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (s >= nslices) return;
+    const int r = s * kSlice + lane;
+    int len = (r < n) ? ia[r + 1] - ia[r] : 0;
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) len = max(len, __shfl_xor(len, off, kWave));
+    if (lane == 0) wcount[s] = (long long)len * kSlice;
+}
+
+__global__ void k_sell_fill(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                            const double *__restrict__ val, const long long *__restrict__ sptr,
+                            int *__restrict__ scol, double *__restrict__ sval, int nslices) {
+    const int s = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (s >= nslices) return;
+    const int r = s * kSlice + lane;
+    const long long base = sptr[s];
+    const int w = (int)((sptr[s + 1] - base) / kSlice);
+    const int beg = (r < n) ? ia[r] : 0;
+    const int len = (r < n) ? ia[r + 1] - beg : 0;
+    for (int k = 0; k < w; ++k) {
+        const long long e = base + (long long)k * kSlice + lane;
+        if (k < len) {
+            scol[e] = ja[beg + k];
+            sval[e] = val[beg + k];
+        } else {
+            scol[e] = -1;
+            sval[e] = 0.0;
+        }
+    }
+}
+
+// y = A x (row sums from 0.0 in CSR order); optional deterministic (y, x).
+template <bool kDot>
+__global__ __launch_bounds__(kBlock) void k_spmv(int n, int nslices, const long long *__restrict__ sptr,
+                                                 const int *__restrict__ col,
+                                                 const double *__restrict__ val,
+                                                 const double *__restrict__ x, double *__restrict__ y,
+                                                 double *__restrict__ partials,
+                                                 unsigned int *__restrict__ ticket,
+                                                 double *__restrict__ dot_out) {
+    __shared__ double sh[kBlock / kWave];
+    const int blk = xcd_block_id();
+    const int s = blk * kSlicesPerBlock + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    const int r = s * kSlice + lane;
+    double acc = 0.0;
+    double prod = 0.0;
+    if (s < nslices) {
+        const long long base = sptr[s];
+        const int w = (int)((sptr[s + 1] - base) / kSlice);
+        const int *c = col + base + lane;
+        const double *v = val + base + lane;
+        int k = 0;
+        for (; k + 4 <= w; k += 4) {
+            const int c0 = c[(k + 0) * kSlice], c1 = c[(k + 1) * kSlice];
+            const int c2 = c[(k + 2) * kSlice], c3 = c[(k + 3) * kSlice];
+            const double v0 = v[(k + 0) * kSlice], v1 = v[(k + 1) * kSlice];
+            const double v2 = v[(k + 2) * kSlice], v3 = v[(k + 3) * kSlice];
+            const double x0 = c0 >= 0 ? x[c0] : 0.0, x1 = c1 >= 0 ? x[c1] : 0.0;
+            const double x2 = c2 >= 0 ? x[c2] : 0.0, x3 = c3 >= 0 ? x[c3] : 0.0;
+            if (c0 >= 0) acc = acc + v0 * x0;
+            if (c1 >= 0) acc = acc + v1 * x1;
+            if (c2 >= 0) acc = acc + v2 * x2;
+            if (c3 >= 0) acc = acc + v3 * x3;
+        }
+        for (; k < w; ++k) {
+            const int c0 = c[k * kSlice];
+            if (c0 >= 0) acc = acc + v[k * kSlice] * x[c0];
+        }
+        if (r < n) {
+            y[r] = acc;
+            if (kDot) prod = acc * x[r];
+        }
+    }
+    if (kDot) {
+        const double bs = block_sum<kBlock>(prod, sh);
+        if (threadIdx.x == 0) partials[blk] = bs;
+        if (last_block_ticket(ticket)) {
+            const double tot = final_sum<kBlock>(partials, gridDim.x, sh);
+            if (threadIdx.x == 0) {
+                *dot_out = tot;
+                *ticket = 0u;
+            }
+        }
+    }
+}
+
+// w = b - A x  (array_cp + dcsr_aAxpy(-1.0): row sum t from 0.0, then b - t)
+__global__ __launch_bounds__(kBlock) void k_residual(int n, int nslices, const long long *__restrict__ sptr,
+                                                     const int *__restrict__ col,
+                                                     const double *__restrict__ val,
+                                                     const double *__restrict__ b,
+                                                     const double *__restrict__ x,
+                                                     double *__restrict__ w) {
+    const int s = xcd_block_id() * kSlicesPerBlock + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (s >= nslices) return;
+    const int r = s * kSlice + lane;
+    const long long base = sptr[s];
+    const int wd = (int)((sptr[s + 1] - base) / kSlice);
+    const int *c = col + base + lane;
+    const double *v = val + base + lane;
+    double acc = 0.0;
+    int k = 0;
+    for (; k + 4 <= wd; k += 4) {
+        const int c0 = c[(k + 0) * kSlice], c1 = c[(k + 1) * kSlice];
+        const int c2 = c[(k + 2) * kSlice], c3 = c[(k + 3) * kSlice];
+        const double v0 = v[(k + 0) * kSlice], v1 = v[(k + 1) * kSlice];
+        const double v2 = v[(k + 2) * kSlice], v3 = v[(k + 3) * kSlice];
+        const double x0 = c0 >= 0 ? x[c0] : 0.0, x1 = c1 >= 0 ? x[c1] : 0.0;
+        const double x2 = c2 >= 0 ? x[c2] : 0.0, x3 = c3 >= 0 ? x[c3] : 0.0;
+        if (c0 >= 0) acc = acc + v0 * x0;
+        if (c1 >= 0) acc = acc + v1 * x1;
+        if (c2 >= 0) acc = acc + v2 * x2;
+        if (c3 >= 0) acc = acc + v3 * x3;
+    }
+    for (; k < wd; ++k) {
+        const int c0 = c[k * kSlice];
+        if (c0 >= 0) acc = acc + v[k * kSlice] * x[c0];
+    }
+    if (r < n) w[r] = b[r] - acc;
+}
+
+// One sweep out of place.  Jacobi (smoother.c:35-92): t = b - sum_{j!=i} a_ij x_j,
+// d = last diagonal entry; x_i <- (1-w) x_i + w t / d when |d| > SMALLREAL.
+// L1DIAG (smoother.c:324-381): t = b - sum_j a_ij x_j, d = sum_j |a_ij|; x_i += t/d.
+template <int kSmoother>
+__global__ __launch_bounds__(kBlock) void k_sweep(int n, int nslices, const long long *__restrict__ sptr,
+                                                  const int *__restrict__ col,
+                                                  const double *__restrict__ val, double wgt,
+                                                  const double *__restrict__ b,
+                                                  const double *__restrict__ x,
+                                                  double *__restrict__ xout) {
+    const int s = xcd_block_id() * kSlicesPerBlock + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (s >= nslices) return;
+    const int r = s * kSlice + lane;
+    const long long base = sptr[s];
+    const int wd = (int)((sptr[s + 1] - base) / kSlice);
+    const int *c = col + base + lane;
+    const double *v = val + base + lane;
+    double t = (r < n) ? b[r] : 0.0;
+    double d = 0.0;
+    for (int k = 0; k < wd; ++k) {
+        const int cj = c[k * kSlice];
+        const double a = v[k * kSlice];
+        if (kSmoother == HZ_SMOOTHER_L1DIAG) {
+            if (cj >= 0) {
+                t = t - a * x[cj];
+                d = d + fabs(a);
+            }
+        } else {
+            if (cj == r) d = a;
+            else if (cj >= 0) t = t - a * x[cj];
+        }
+    }
+    if (r < n) {
+        const double xi = x[r];
+        double xn = xi;
+        if (fabs(d) > kSmallReal) {
+            if (kSmoother == HZ_SMOOTHER_L1DIAG) xn = xi + t / d;
+            else xn = (1 - wgt) * xi + wgt * t / d;
+        }
+        xout[r] = xn;
+    }
+}
+
+template <int kSmoother>
+__global__ void k_smoother_diag(int n, int nslices, const long long *__restrict__ sptr,
+                                const int *__restrict__ col, const double *__restrict__ val,
+                                double *__restrict__ d) {
+    const int s = blockIdx.x * kSlicesPerBlock + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (s >= nslices) return;
+    const int r = s * kSlice + lane;
+    const long long base = sptr[s];
+    const int wd = (int)((sptr[s + 1] - base) / kSlice);
+    double dd = 0.0;
+    for (int k = 0; k < wd; ++k) {
+        const int cj = col[base + (long long)k * kSlice + lane];
+        const double a = val[base + (long long)k * kSlice + lane];
+        if (kSmoother == HZ_SMOOTHER_L1DIAG) {
+            if (cj >= 0) dd = dd + fabs(a);
+        } else if (cj == r) {
+            dd = a;
+        }
+    }
+    if (r < n) d[r] = dd;
+}
+
+// First sweep from x = 0: t = b exactly (up to the sign of zero), so
+// Jacobi gives (1-w)*0 + w*b/d and L1DIAG gives 0 + b/d.
+__global__ void k_smooth_from_zero(int n, int smoother, double wgt, const double *__restrict__ d,
+                                   const double *__restrict__ b, double *__restrict__ x) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double di = d[i];
+        double xi = 0.0;
+        if (fabs(di) > kSmallReal) {
+            if (smoother == HZ_SMOOTHER_L1DIAG) xi = 0.0 + b[i] / di;
+            else xi = (1 - wgt) * 0.0 + wgt * b[i] / di;
+        }
+        x[i] = xi;
+    }
+}
+
+// bc[c] = sum over members (ascending) of w, from 0.0 (dcsr_mxv_agg, sparse.c:1663-1679)
+__global__ void k_restrict(int nc, const int *__restrict__ rptr, const int *__restrict__ rmem,
+                           const double *__restrict__ w, double *__restrict__ bc, int smoother,
+                           double wgt, const double *__restrict__ dc, double *__restrict__ xc) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+        double s = 0.0;
+        const int e = rptr[c + 1];
+        for (int q = rptr[c]; q < e; ++q) s = s + w[rmem[q]];
+        bc[c] = s;
+        if (xc) {
+            const double di = dc[c];
+            double xi = 0.0;
+            if (fabs(di) > kSmallReal) {
+                if (smoother == HZ_SMOOTHER_L1DIAG) xi = 0.0 + s / di;
+                else xi = (1 - wgt) * 0.0 + wgt * s / di;
+            }
+            xc[c] = xi;
+        }
+    }
+}
+
+// x += alpha * P xc (dcsr_aAxpy_agg, sparse.c:1748-1784)
+__global__ void k_prolong(int n, const int *__restrict__ agg, const double *__restrict__ xc,
+                          double alpha, const double *__restrict__ alpha_dev,
+                          double *__restrict__ x) {
+    const double a = alpha_dev ? *alpha_dev : alpha;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int g = agg[i];
+        double t = 0.0;
+        if (g > HZ_UNPT) t = t + xc[g];
+        const double xi = x[i];
+        if (a == 1.0) x[i] = xi + t;
+        else if (a == -1.0) x[i] = xi - t;
+        else x[i] = xi + t * a;
+    }
+}
+
+__global__ void k_fill(double *__restrict__ x, double v, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        x[i] = v;
+}
+
+__global__ void k_copy(double *__restrict__ d, const double *__restrict__ s, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        d[i] = s[i];
+}
+
+// u += alpha p ; r -= alpha t ; then ||r||^2, ||u||_inf, ||u||^2, ||p||^2.
+// alpha = zr / temp2; block 0 also records alpha and moves zr into temp1.
+__global__ __launch_bounds__(kBlock) void k_pcg_update(int n, PcgScalars *__restrict__ s,
+                                                       const double *__restrict__ p,
+                                                       const double *__restrict__ t,
+                                                       double *__restrict__ u,
+                                                       double *__restrict__ r,
+                                                       double *__restrict__ alpha_hist, int iter,
+                                                       double *__restrict__ partials,
+                                                       unsigned int *__restrict__ ticket) {
+    __shared__ double sh[kBlock / kWave];
+    const double alpha = s->zr / s->temp2;
+    const double malpha = -alpha;
+    double rr = 0.0, ui = 0.0, uu = 0.0, pp = 0.0;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+        const double pi = p[i];
+        double ui_ = u[i];
+        double ri = r[i];
+        if (alpha == 1.0) ui_ = ui_ + pi;
+        else if (alpha == -1.0) ui_ = ui_ - pi;
+        else ui_ = ui_ + alpha * pi;
+        const double ti = t[i];
+        if (malpha == 1.0) ri = ri + ti;
+        else if (malpha == -1.0) ri = ri - ti;
+        else ri = ri + malpha * ti;
+        u[i] = ui_;
+        r[i] = ri;
+        rr += ri * ri;
+        ui = fmax(ui, fabs(ui_));
+        uu += ui_ * ui_;
+        pp += pi * pi;
+    }
+    const int G = gridDim.x;
+    double v;
+    v = block_sum<kBlock>(rr, sh);
+    if (threadIdx.x == 0) partials[0 * G + blockIdx.x] = v;
+    v = block_max<kBlock>(ui, sh);
+    if (threadIdx.x == 0) partials[1 * G + blockIdx.x] = v;
+    v = block_sum<kBlock>(uu, sh);
+    if (threadIdx.x == 0) partials[2 * G + blockIdx.x] = v;
+    v = block_sum<kBlock>(pp, sh);
+    if (threadIdx.x == 0) partials[3 * G + blockIdx.x] = v;
+    if (last_block_ticket(ticket)) {
+        const double a0 = final_sum<kBlock>(partials + 0 * G, G, sh);
+        const double a1 = final_max<kBlock>(partials + 1 * G, G, sh);
+        const double a2 = final_sum<kBlock>(partials + 2 * G, G, sh);
+        const double a3 = final_sum<kBlock>(partials + 3 * G, G, sh);
+        if (threadIdx.x == 0) {
+            s->rr = a0;
+            s->uinf = a1;
+            s->uu = a2;
+            s->pp = a3;
+            s->alpha = alpha;
+            s->temp1 = s->zr;
+            if (alpha_hist) alpha_hist[iter - 1] = alpha;
+            *ticket = 0u;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_dot(int n, const double *__restrict__ x,
+                                                const double *__restrict__ y,
+                                                double *__restrict__ out,
+                                                double *__restrict__ partials,
+                                                unsigned int *__restrict__ ticket) {
+    __shared__ double sh[kBlock / kWave];
+    double acc = 0.0;
+    for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) acc += x[i] * y[i];
+    const double v = block_sum<kBlock>(acc, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = v;
+    if (last_block_ticket(ticket)) {
+        const double tot = final_sum<kBlock>(partials, gridDim.x, sh);
+        if (threadIdx.x == 0) {
+            *out = tot;
+            *ticket = 0u;
+        }
+    }
+}
+
+// beta = zr / temp1 ; p = 1.0*z + beta*p (array_axpby, array.c:423-433)
+__global__ void k_pcg_direction(int n, PcgScalars *__restrict__ s, const double *__restrict__ z,
+                                double *__restrict__ p, double *__restrict__ beta_hist, int iter) {
+    const double beta = s->zr / s->temp1;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        p[i] = 1.0 * z[i] + beta * p[i];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        s->beta = beta;
+        if (beta_hist) beta_hist[iter - 1] = beta;
+    }
+}
+
+// Coarse scaling factor (mgcycle.c:337-341): alpha = min((x,b) / x^T A x, 2).
+__global__ __launch_bounds__(kBlock) void k_coarse_scaling(int n, int nslices,
+                                                           const long long *__restrict__ sptr,
+                                                           const int *__restrict__ col,
+                                                           const double *__restrict__ val,
+                                                           const double *__restrict__ x,
+                                                           const double *__restrict__ b,
+                                                           double *__restrict__ partials,
+                                                           unsigned int *__restrict__ ticket,
+                                                           double *__restrict__ alpha_out) {
+    __shared__ double sh[kBlock / kWave];
+    const int s = blockIdx.x * kSlicesPerBlock + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    const int r = s * kSlice + lane;
+    double xb = 0.0, xax = 0.0;
+    if (s < nslices) {
+        const long long base = sptr[s];
+        const int wd = (int)((sptr[s + 1] - base) / kSlice);
+        double acc = 0.0;
+        for (int k = 0; k < wd; ++k) {
+            const int cj = col[base + (long long)k * kSlice + lane];
+            if (cj >= 0) acc = acc + val[base + (long long)k * kSlice + lane] * x[cj];
+        }
+        if (r < n) {
+            xb = x[r] * b[r];
+            xax = x[r] * acc;
+        }
+    }
+    const int G = gridDim.x;
+    double v = block_sum<kBlock>(xb, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = v;
+    v = block_sum<kBlock>(xax, sh);
+    if (threadIdx.x == 0) partials[G + blockIdx.x] = v;
+    if (last_block_ticket(ticket)) {
+        const double a = final_sum<kBlock>(partials, G, sh);
+        const double c = final_sum<kBlock>(partials + G, G, sh);
+        if (threadIdx.x == 0) {
+            double al = a / c;
+            *alpha_out = al < 2.0 ? al : 2.0;
+            *ticket = 0u;
+        }
+    }
+}
+
+inline int stream_grid(long n) {
+    long g = (n + kBlock - 1) / kBlock;
+    if (g > kStreamGrid) g = kStreamGrid;
+    if (g < 1) g = 1;
+    return (int)g;
+}
+
+}  // namespace
+
+void sell_from_csr(const DevCSR &A, DevSELL &S) {
+    S.n = A.n;
+    S.nslices = div_up(A.n, kSlice);
+    const int ns = S.nslices;
+    long long *wcount = arena().alloc_n<long long>((size_t)ns + 1);
+    S.sptr = arena().alloc_n<long long>((size_t)ns + 1);
+    hipStream_t st = stream();
+    if (ns > 0) {
+        k_slice_width<<<div_up(ns, kSlicesPerBlock), kBlock, 0, st>>>(A.n, A.ia, wcount, ns);
+        HZ_LAUNCH_CHECK();
+    }
+    // exclusive scan of slice sizes (padded entries)
+    size_t tmp_bytes = 0;
+    HZ_CHECK(rocprim::exclusive_scan(nullptr, tmp_bytes, wcount, S.sptr, 0LL, (size_t)ns + 1,
+                                     rocprim::plus<long long>(), st));
+    HZ_CHECK(hipMemsetAsync(wcount + ns, 0, sizeof(long long), st));
+    void *tmp = arena().alloc(tmp_bytes);
+    HZ_CHECK(rocprim::exclusive_scan(tmp, tmp_bytes, wcount, S.sptr, 0LL, (size_t)ns + 1,
+                                     rocprim::plus<long long>(), st));
+    long long nent = 0;
+    HZ_CHECK(hipMemcpyAsync(&nent, S.sptr + ns, sizeof(long long), hipMemcpyDeviceToHost, st));
+    HZ_CHECK(hipStreamSynchronize(st));
+    S.nent = nent;
+    S.col = arena().alloc_n<int>((size_t)nent + 1);
+    S.val = arena().alloc_n<double>((size_t)nent + 1);
+    if (ns > 0) {
+        k_sell_fill<<<div_up(ns, kSlicesPerBlock), kBlock, 0, st>>>(A.n, A.ia, A.ja, A.val, S.sptr,
+                                                                  S.col, S.val, ns);
+        HZ_LAUNCH_CHECK();
+    }
+}
+
+void spmv(const DevSELL &S, const double *x, double *y, double *dot_out, const ReduceWs &ws) {
+    if (S.n == 0) return;
+    const int grid = div_up(S.nslices, kSlicesPerBlock);
+    if (dot_out) {
+        k_spmv<true><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col, S.val, x, y,
+                                                    ws.partials, ws.tickets, dot_out);
+    } else {
+        k_spmv<false><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col, S.val, x, y,
+                                                     nullptr, nullptr, nullptr);
+    }
+    HZ_LAUNCH_CHECK();
+}
+
+void residual(const DevSELL &S, const double *b, const double *x, double *w) {
+    if (S.n == 0) return;
+    k_residual<<<div_up(S.nslices, kSlicesPerBlock), kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr,
+                                                                           S.col, S.val, b, x, w);
+    HZ_LAUNCH_CHECK();
+}
+
+void smooth_sweep(const DevSELL &S, int smoother, double w, const double *b, const double *x,
+                  double *xout) {
+    if (S.n == 0) return;
+    const int grid = div_up(S.nslices, kSlicesPerBlock);
+    if (smoother == HZ_SMOOTHER_L1DIAG)
+        k_sweep<HZ_SMOOTHER_L1DIAG><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col,
+                                                                   S.val, w, b, x, xout);
+    else
+        k_sweep<HZ_SMOOTHER_JACOBI><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col,
+                                                                   S.val, w, b, x, xout);
+    HZ_LAUNCH_CHECK();
+}
+
+void smoother_diag(const DevSELL &S, int smoother, double *d) {
+    if (S.n == 0) return;
+    const int grid = div_up(S.nslices, kSlicesPerBlock);
+    if (smoother == HZ_SMOOTHER_L1DIAG)
+        k_smoother_diag<HZ_SMOOTHER_L1DIAG><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr,
+                                                                           S.col, S.val, d);
+    else
+        k_smoother_diag<HZ_SMOOTHER_JACOBI><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr,
+                                                                           S.col, S.val, d);
+    HZ_LAUNCH_CHECK();
+}
+
+void smooth_from_zero(int n, int smoother, double w, const double *d, const double *b, double *x) {
+    if (n == 0) return;
+    k_smooth_from_zero<<<stream_grid(n), kBlock, 0, stream()>>>(n, smoother, w, d, b, x);
+    HZ_LAUNCH_CHECK();
+}
+
+void restrict_agg(int nc, const int *rptr, const int *rmem, const double *w, double *bc,
+                  int smoother, double wgt, const double *dc, double *xc) {
+    if (nc == 0) return;
+    k_restrict<<<stream_grid(nc), kBlock, 0, stream()>>>(nc, rptr, rmem, w, bc, smoother, wgt, dc, xc);
+    HZ_LAUNCH_CHECK();
+}
+
+void prolong_agg(int n, const int *agg, const double *xc, double alpha, const double *alpha_dev,
+                 double *x) {
+    if (n == 0) return;
+    k_prolong<<<stream_grid(n), kBlock, 0, stream()>>>(n, agg, xc, alpha, alpha_dev, x);
+    HZ_LAUNCH_CHECK();
+}
+
+void fill(double *x, double v, long n) {
+    if (n == 0) return;
+    k_fill<<<stream_grid(n), kBlock, 0, stream()>>>(x, v, n);
+    HZ_LAUNCH_CHECK();
+}
+
+void copy(double *dst, const double *src, long n) {
+    if (n == 0) return;
+    k_copy<<<stream_grid(n), kBlock, 0, stream()>>>(dst, src, n);
+    HZ_LAUNCH_CHECK();
+}
+
+void pcg_update(int n, PcgScalars *s, const double *p, const double *t, double *u, double *r,
+                double *alpha_hist, int iter, const ReduceWs &ws) {
+    k_pcg_update<<<stream_grid(n), kBlock, 0, stream()>>>(n, s, p, t, u, r, alpha_hist, iter,
+                                                          ws.partials, ws.tickets);
+    HZ_LAUNCH_CHECK();
+}
+
+void dot(int n, const double *x, const double *y, double *out, const ReduceWs &ws) {
+    k_dot<<<stream_grid(n), kBlock, 0, stream()>>>(n, x, y, out, ws.partials, ws.tickets);
+    HZ_LAUNCH_CHECK();
+}
+
+void norm2sq(int n, const double *x, double *out, const ReduceWs &ws) { dot(n, x, x, out, ws); }
+
+void pcg_direction(int n, PcgScalars *s, const double *z, double *p, double *beta_hist, int iter) {
+    k_pcg_direction<<<stream_grid(n), kBlock, 0, stream()>>>(n, s, z, p, beta_hist, iter);
+    HZ_LAUNCH_CHECK();
+}
+
+void coarse_scaling_alpha(const DevSELL &S, const double *x, const double *b, PcgScalars *,
+                          double *alpha_out, const ReduceWs &ws) {
+    const int grid = div_up(S.nslices, kSlicesPerBlock);
+    k_coarse_scaling<<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col, S.val, x, b,
+                                                    ws.partials, ws.tickets, alpha_out);
+    HZ_LAUNCH_CHECK();
+}
+
+}  // namespace hz

@@ -1,0 +1,125 @@
+"""Synthetic P1 finite-element test systems in closed form (numpy, vectorised).
+
+All matrices are 0-based CSR with ascending column indices per row, unknowns
+ordered lexicographically (x fastest), Dirichlet nodes eliminated.
+
+* ``poisson2d_p1(m)``: -Δu on the unit square, P1 on the uniform right-triangle
+  mesh with m x m cells.  Interior stiffness is the 5-point stencil
+  [4, -1, -1, -1, -1]; the couplings along the triangle diagonals are exactly
+  zero and are not stored (m = 128 gives n = 16,129, nnz = 80,137).
+* ``poisson3d_kuhn(m)``: -Δu on the unit cube, P1 on m^3 cubes each split into
+  the 6 Kuhn tetrahedra along the (1,1,1) diagonal.  Stiffness is
+  h * (6 on the diagonal, -1 on the 6 axis neighbours, 0 on the 8 neighbours
+  along the face and body diagonals of the Kuhn split); the zeros ARE stored
+  (15-point rows), as an assembled FE matrix holds them (m = 128 gives
+  n = 2,048,383, nnz = 30,340,171).
+* ``uniform_rhs(n, seed)``: f_i in [-1, 1) from splitmix64(seed XOR i), the
+  same bits in every language.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def splitmix64(x: np.ndarray) -> np.ndarray:
+    """Steele, Lea & Flood (2014) SplitMix64 finaliser applied to x + golden gamma."""
+    with np.errstate(over="ignore"):
+        z = (x.astype(np.uint64) + np.uint64(0x9E3779B97F4A7C15)) & _M64
+        z = ((z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)) & _M64
+        z = ((z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)) & _M64
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform_rhs(n: int, seed: int = 1) -> np.ndarray:
+    """Counter-based U[-1, 1): f_i = 2 * (splitmix64(seed ^ i) >> 11) * 2^-53 - 1."""
+    i = np.arange(n, dtype=np.uint64)
+    z = splitmix64(np.uint64(seed) ^ i)
+    u = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return 2.0 * u - 1.0
+
+
+def _stencil_csr(shape, offsets, values):
+    """CSR of a constant stencil on the interior grid `shape` (z, y, x order),
+    offsets given as (dz, dy, dx) tuples; entries outside the grid are dropped.
+    Offsets are sorted by their linear index offset, so columns ascend."""
+    dims = len(shape)
+    nz, ny, nx = (1,) * (3 - dims) + tuple(shape)
+    n = nz * ny * nx
+    lin = [dz * ny * nx + dy * nx + dx for (dz, dy, dx) in offsets]
+    order = np.argsort(lin, kind="stable")
+    offsets = [offsets[k] for k in order]
+    values = [values[k] for k in order]
+    idx = np.arange(n, dtype=np.int64)
+    x = idx % nx
+    y = (idx // nx) % ny
+    z = idx // (nx * ny)
+    cols = np.empty((len(offsets), n), dtype=np.int64)
+    valid = np.empty((len(offsets), n), dtype=bool)
+    vals = np.empty((len(offsets), n), dtype=np.float64)
+    for k, (dz, dy, dx) in enumerate(offsets):
+        xx, yy, zz = x + dx, y + dy, z + dz
+        valid[k] = (xx >= 0) & (xx < nx) & (yy >= 0) & (yy < ny) & (zz >= 0) & (zz < nz)
+        cols[k] = zz * ny * nx + yy * nx + xx
+        vals[k] = values[k]
+    counts = valid.sum(axis=0)
+    ia = np.zeros(n + 1, dtype=np.int64)
+    np.cumsum(counts, out=ia[1:])
+    vt = valid.T.reshape(-1)
+    ja = cols.T.reshape(-1)[vt].astype(np.int32)
+    va = vals.T.reshape(-1)[vt]
+    return ia.astype(np.int32), ja, va
+
+
+def poisson2d_p1(m: int = 128):
+    """5-point P1 Laplacian on (m-1)^2 interior nodes."""
+    k = m - 1
+    offs = [(0, 0, 0), (0, 0, -1), (0, 0, 1), (0, -1, 0), (0, 1, 0)]
+    vals = [4.0, -1.0, -1.0, -1.0, -1.0]
+    return _stencil_csr((k, k), offs, vals)
+
+
+KUHN_DIAGONALS = [(0, 1, 1), (1, 0, 1), (1, 1, 0), (1, 1, 1)]
+
+
+def poisson3d_kuhn(m: int = 128):
+    """15-point P1 Kuhn-split Laplacian on (m-1)^3 interior nodes (zeros stored).
+    Offsets are (dz, dy, dx); the Kuhn diagonals (dx,dy,dz) in {(1,1,0),(1,0,1),
+    (0,1,1),(1,1,1)} and their negatives carry exact zeros."""
+    k = m - 1
+    h = 1.0 / m
+    offs = [(0, 0, 0)]
+    vals = [6.0 * h]
+    for ax in [(0, 0, 1), (0, 1, 0), (1, 0, 0)]:
+        for s in (1, -1):
+            offs.append(tuple(s * a for a in ax))
+            vals.append(-h)
+    for d in KUHN_DIAGONALS:
+        for s in (1, -1):
+            offs.append(tuple(s * a for a in d))
+            vals.append(0.0)
+    return _stencil_csr((k, k, k), offs, vals)
+
+
+def kuhn_element_stiffness_check():
+    """Assembles the P1 stiffness of one unit cube split into the 6 Kuhn
+    tetrahedra and returns the 8x8 matrix (vertex v = x + 2y + 4z).  Used by the
+    tests to confirm the stencil of poisson3d_kuhn from first principles."""
+    import itertools
+    verts = np.array([[(v >> 0) & 1, (v >> 1) & 1, (v >> 2) & 1] for v in range(8)], float)
+    K = np.zeros((8, 8))
+    for perm in itertools.permutations(range(3)):
+        # path 0 -> e_p0 -> e_p0+e_p1 -> (1,1,1)
+        p = np.zeros(3)
+        tet = [0]
+        for ax in perm:
+            p = p.copy()
+            p[ax] = 1
+            tet.append(int(p[0] + 2 * p[1] + 4 * p[2]))
+        X = verts[tet]
+        T = np.hstack([np.ones((4, 1)), X])
+        G = np.linalg.inv(T)[1:, :]          # gradients of barycentric coords (3x4)
+        vol = abs(np.linalg.det(T)) / 6.0
+        K[np.ix_(tet, tet)] += vol * G.T @ G
+    return K

@@ -192,6 +192,9 @@ typedef struct {
                               1: glibc-compatible stream generated on the GPU from the
                                  current libc state (same values; state advanced after). */
     INT  verbose_timing;   /* >0: print setup/solve wall times from hipEvents */
+    INT  return_on_fatal;  /* 0: exit() like the reference on its fatal paths
+                              (exit(16)/exit(17)/check_error); 1: return the negative
+                              code (-1016/-1017/-1013) instead. */
 } hz_options;
 
 void hz_get_options(hz_options *opt);
@@ -247,6 +250,21 @@ INT hz_smoother_sweep(dCSRmat *A, const REAL *b, REAL *x, INT smoother, REAL w);
 /* Coarse product dcsr_rap_agg(R,A,P) with R=P^T built from aggregates
  * (sparse.c:2557-2683, sparse.c:770-830). Result arrays are malloc'd. */
 INT hz_rap_agg(dCSRmat *A, const INT *aggregates, INT num_aggregates, dCSRmat *Ac);
+
+/* Average device duration of the fine-level solve kernels on the library's
+ * stream (hipEvents around `reps` back-to-back launches), for the roofline.
+ * IA/JA/val are device pointers (same matrix as hz_krylov_amg_device). */
+typedef struct {
+    INT  reps;
+    REAL spmv_ms;        /* y = A x              (dcsr_mxv) */
+    REAL residual_ms;    /* w = b - A x          (array_cp + dcsr_aAxpy) */
+    REAL jacobi_ms;      /* one Jacobi sweep     (smoother_dcsr_jacobi) */
+    REAL spmv_dot_ms;    /* y = A x with (y, x)  (PCG step) */
+    long long sell_entries;  /* padded entries streamed by the SELL-64 kernels */
+} hz_kernel_times;
+
+INT hz_time_fine_kernels(INT n, INT nnz, const INT *dIA, const INT *dJA, const REAL *dval,
+                         INT reps, hz_kernel_times *out);
 
 /* Runtime information. */
 const char *hz_version(void);

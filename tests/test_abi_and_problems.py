@@ -1,0 +1,171 @@
+"""CPU checks of the boundary and the host-side pieces (no GPU calls)."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import scipy.sparse as sp
+
+from hazmath_amd import capi, problems
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def declared_symbols():
+    names = set()
+    for h in (ROOT / "include").glob("*.h"):
+        text = re.sub(r"/\*.*?\*/", "", h.read_text(), flags=re.S)
+        for m in re.finditer(r"^\s*[A-Za-z_][\w\s\*]*?\b([A-Za-z_]\w*)\s*\(", text, flags=re.M):
+            name = m.group(1)
+            if name not in {"if", "while", "for", "return", "sizeof"}:
+                names.add(name)
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = capi.load_library()
+    names = declared_symbols()
+    assert "linear_solver_dcsr_krylov_amg" in names
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_struct_layouts_agree_between_header_and_ctypes(tmp_path):
+    """Compile the public header with gcc and compare every field offset and
+    struct size with the ctypes mirror used by the tests and the benchmark."""
+    import subprocess
+    structs = {"dCSRmat": capi.dCSRmat, "dvector": capi.dvector,
+               "linear_itsolver_param": capi.linear_itsolver_param, "AMG_param": capi.AMG_param,
+               "hz_options": capi.hz_options, "hz_solve_info": capi.hz_solve_info,
+               "hz_hierarchy": capi.hz_hierarchy, "hz_kernel_times": capi.hz_kernel_times}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "hazmath_mi355x.h"',
+             'int main(void){']
+    for s, cls in structs.items():
+        lines.append(f'printf("{s} sizeof %zu\\n", sizeof({s}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'printf("{s} {fname} %zu\\n", offsetof({s}, {fname}));')
+    lines.append("return 0;}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    for line in filter(None, out):
+        s, field, val = line.split()
+        cls = structs[s]
+        got = C.sizeof(cls) if field == "sizeof" else getattr(cls, field).offset
+        assert got == int(val), (s, field)
+    # the reference ABI: 3 ints + 3 pointers; REAL tol after SHORT/SHORT/INT
+    assert C.sizeof(capi.dCSRmat) == 40 and capi.AMG_param.tol.offset == 8
+
+
+def test_param_defaults_match_reference():
+    lib = capi.load_library()
+    amg = capi.default_amg_param(lib)
+    assert (amg.AMG_type, amg.maxit, amg.max_levels, amg.coarse_dof, amg.cycle_type,
+            amg.smoother, amg.coarse_solver, amg.aggregation_type, amg.max_aggregation) == \
+        (1, 1, 10, 200, 2, 2, 0, 4, 20)
+    assert amg.tol == 1e-6 and amg.strong_coupled == 0.04 and amg.tentative_smooth == 0.67
+    it = capi.default_itsolver_param(lib)
+    assert (it.linear_itsolver_type, it.linear_precond_type, it.linear_stop_type,
+            it.linear_maxit, it.linear_restart) == (4, 0, 1, 500, 100)
+    assert it.linear_tol == 1e-6
+
+
+def test_param_defaults_agree_with_oracle():
+    import oracle
+    lib, orc = capi.load_library(), oracle.load()
+    a, b = capi.AMG_param(), capi.AMG_param()
+    lib.param_amg_init(C.byref(a))
+    orc.orc_param_amg_init(C.byref(b))
+    assert bytes(a) == bytes(b)
+
+
+def test_coo_reader_matches_fixture(tmp_path):
+    lib = capi.load_library()
+    d = np.load(ROOT / "tests" / "golden" / "solvers_A_b.npz")
+    ia, ja, va = d["ia"], d["ja"], d["val"]
+    n = ia.size - 1
+    rows = np.repeat(np.arange(n), np.diff(ia))
+    p = tmp_path / "A.dat"
+    with open(p, "w") as f:
+        f.write(f"{n} {n} {ia[-1]}\n")
+        for r, c, v in zip(rows, ja, va):
+            f.write(f"{r} {c} {v:.15e}\n")
+    libc = C.CDLL(None)
+    libc.fopen.restype = C.c_void_p
+    libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+    libc.fclose.argtypes = [C.c_void_p]
+    lib.dcoo_read_dcsr_p.restype = C.POINTER(capi.dCSRmat)
+    lib.dcoo_read_dcsr_p.argtypes = [C.c_void_p]
+    fp = libc.fopen(str(p).encode(), b"r")
+    A = lib.dcoo_read_dcsr_p(fp)
+    libc.fclose(fp)
+    gia, gja, gva = capi.csr_to_numpy(A.contents)
+    assert np.array_equal(gia, ia) and np.array_equal(gja, ja) and np.array_equal(gva, va)
+
+
+def test_device_entry_fails_loudly_without_gpu():
+    lib = capi.load_library()
+    if lib.hz_device_count() > 0:
+        return
+    it, amg = capi.solver_params(lib)
+    ia, ja, va = problems.poisson2d_p1(8)
+    A = capi.HostCSR(ia, ja, va)
+    b = capi.HostVec(np.ones(ia.size - 1))
+    x = capi.HostVec(np.zeros(ia.size - 1))
+    assert lib.linear_solver_dcsr_krylov_amg(A.ptr(), b.ptr(), x.ptr(), C.byref(it),
+                                             C.byref(amg)) == capi.ERROR_DEVICE
+
+
+def test_poisson2d_generator():
+    ia, ja, va = problems.poisson2d_p1(128)
+    assert ia.size - 1 == 16129 and ia[-1] == 80137
+    k = 127
+    T = sp.diags([-1, 2, -1], [-1, 0, 1], shape=(k, k))
+    L = (sp.kron(sp.eye(k), T) + sp.kron(T, sp.eye(k))).tocsr()
+    assert abs(sp.csr_matrix((va, ja, ia)) - L).max() == 0
+    assert all(np.all(np.diff(ja[ia[i]:ia[i + 1]]) > 0) for i in range(0, 16129, 997))
+
+
+def test_kuhn_generator_sizes_and_stencil():
+    ia, ja, va = problems.poisson3d_kuhn(16)
+    N = 15
+    assert ia.size - 1 == N ** 3
+    assert ia[-1] == N ** 3 + 6 * N * N * (N - 1) + 6 * N * (N - 1) ** 2 + 2 * (N - 1) ** 3
+    A = sp.csr_matrix((va, ja, ia))
+    # symmetric, row sums zero in the interior, 15 stored entries in interior rows
+    assert abs(A - A.T).max() == 0
+    i = 7 + 7 * N + 7 * N * N
+    assert ia[i + 1] - ia[i] == 15
+    assert abs(A[i].sum()) < 1e-15
+    assert A[i, i] == 6.0 / 16
+
+
+def test_kuhn_stencil_from_element_assembly():
+    """Assemble the six Kuhn tetrahedra of a cube; the interior stencil must be
+    6 / -1 on the axes and zero on the Kuhn diagonals (the stored zeros)."""
+    K = problems.kuhn_element_stiffness_check()
+    st = {}
+    for c in range(8):
+        pc = np.array([(c >> 0) & 1, (c >> 1) & 1, (c >> 2) & 1])
+        for v in range(8):
+            pv = np.array([(v >> 0) & 1, (v >> 1) & 1, (v >> 2) & 1])
+            d = tuple(int(t) for t in pv - pc)
+            st[d] = st.get(d, 0.0) + K[c, v]
+    assert abs(st[(0, 0, 0)] - 6.0) < 1e-12
+    for ax in [(1, 0, 0), (0, 1, 0), (0, 0, 1)]:
+        assert abs(st[ax] + 1.0) < 1e-12
+        assert abs(st[tuple(-a for a in ax)] + 1.0) < 1e-12
+    for dzyx in problems.KUHN_DIAGONALS:
+        dxyz = tuple(reversed(dzyx))
+        assert abs(st[dxyz]) < 1e-12
+
+
+def test_uniform_rhs_is_counter_based():
+    a = problems.uniform_rhs(1000, 3)
+    b = problems.uniform_rhs(2000, 3)
+    assert np.array_equal(a, b[:1000])
+    assert a.min() >= -1 and a.max() < 1
+    assert abs(a.mean()) < 0.1
+    assert not np.array_equal(a, problems.uniform_rhs(1000, 4))

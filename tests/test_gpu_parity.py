@@ -1,0 +1,199 @@
+"""GPU parity of the MI355X library against the CPU oracle, through the C ABI.
+
+Bars: integer/structure outputs bit-exact; per-row kernels (SpMV, residual,
+Jacobi/L1 sweeps, coarse products) bit-exact as well (same summation order,
+no FMA contraction on either side); PCG iteration counts within +-1 and
+solutions within 1e-10 relative of the oracle's.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from hazmath_amd import capi, problems
+
+pytestmark = pytest.mark.gpu
+
+
+def random_csr(n, seed, maxlen=20, long_rows=(), empty_rows=(), shuffle=True, ncol=None):
+    rng = np.random.default_rng(seed)
+    ncol = ncol or n
+    rows = []
+    for i in range(n):
+        if i in empty_rows:
+            rows.append(np.zeros(0, np.int32))
+            continue
+        L = 150 if i in long_rows else int(rng.integers(1, maxlen))
+        cols = rng.choice(ncol, size=min(L, ncol), replace=False).astype(np.int32)
+        if i < ncol and i not in cols:
+            cols[0] = i
+        if not shuffle:
+            cols.sort()
+        rows.append(cols)
+    ia = np.zeros(n + 1, np.int32)
+    ia[1:] = np.cumsum([r.size for r in rows])
+    ja = np.concatenate(rows).astype(np.int32)
+    va = rng.standard_normal(ja.size)
+    va[rng.random(ja.size) < 0.05] = 0.0
+    return ia, ja, va
+
+
+def _spd_like(ia, ja, va):
+    """Make the diagonal dominant so sweeps are well defined."""
+    va = va.copy()
+    for i in range(ia.size - 1):
+        for k in range(ia[i], ia[i + 1]):
+            if ja[k] == i:
+                va[k] = 10.0 + abs(va[k])
+    return va
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_spmv_bitexact(lib, orc, seed):
+    ia, ja, va = random_csr(3000, seed, long_rows={5, 700, 2999}, empty_rows={10, 11})
+    A = capi.HostCSR(ia, ja, va)
+    x = np.random.default_rng(seed + 10).standard_normal(3000)
+    yg = np.zeros(3000)
+    yo = np.zeros(3000)
+    assert lib.hz_dcsr_mxv(A.ptr(), capi.dptr(x), capi.dptr(yg)) == 0
+    orc.orc_dcsr_mxv(A.ptr(), capi.dptr(x), capi.dptr(yo))
+    assert np.array_equal(yg, yo)
+
+
+def test_residual_bitexact(lib, orc):
+    ia, ja, va = random_csr(2500, 5, long_rows={3})
+    A = capi.HostCSR(ia, ja, va)
+    rng = np.random.default_rng(3)
+    x, b = rng.standard_normal(2500), rng.standard_normal(2500)
+    wg, wo = np.zeros(2500), np.zeros(2500)
+    assert lib.hz_dcsr_residual(A.ptr(), capi.dptr(b), capi.dptr(x), capi.dptr(wg)) == 0
+    orc.orc_dcsr_residual(A.ptr(), capi.dptr(b), capi.dptr(x), capi.dptr(wo))
+    assert np.array_equal(wg, wo)
+
+
+@pytest.mark.parametrize("smoother,w", [(1, 1.0), (1, 2.0 / 3.0), (10, 1.0)])
+def test_smoother_bitexact(lib, orc, smoother, w):
+    ia, ja, va = random_csr(2000, 9)
+    va = _spd_like(ia, ja, va)
+    A = capi.HostCSR(ia, ja, va)
+    rng = np.random.default_rng(4)
+    x0, b = rng.standard_normal(2000), rng.standard_normal(2000)
+    xg, xo = x0.copy(), x0.copy()
+    assert lib.hz_smoother_sweep(A.ptr(), capi.dptr(b), capi.dptr(xg), smoother, w) == 0
+    orc.orc_smoother_sweep(A.ptr(), capi.dptr(b), capi.dptr(xo), smoother, w)
+    assert np.array_equal(xg, xo)
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_rap_agg_bitexact(lib, orc, seed):
+    n = 4000
+    ia, ja, va = random_csr(n, seed + 20, long_rows={1, 2})
+    rng = np.random.default_rng(seed)
+    nagg = 900
+    agg = rng.integers(-1, nagg, size=n).astype(np.int32)
+    agg[:nagg] = np.arange(nagg)               # every aggregate non-empty
+    # one very large aggregate exercises the global-memory path
+    agg[rng.choice(n, 120, replace=False)] = 7
+    A = capi.HostCSR(ia, ja, va)
+    Cg, Co = capi.dCSRmat(), capi.dCSRmat()
+    assert lib.hz_rap_agg(A.ptr(), capi.dptr(agg, capi.INT), nagg, C.byref(Cg)) == 0
+    orc.orc_rap_agg(A.ptr(), capi.dptr(agg, capi.INT), nagg, C.byref(Co))
+    g, o = capi.csr_to_numpy(Cg), capi.csr_to_numpy(Co)
+    for a, b_ in zip(g, o):
+        assert np.array_equal(a, b_)
+
+
+def _hierarchies(lib, orc, libc, A, agg_type, theta, coarse_solver=32, coarse_dof=100):
+    out = []
+    for which in ("gpu", "oracle"):
+        it, amg = capi.solver_params(lib, aggregation=agg_type, theta=theta,
+                                     coarse_solver=coarse_solver, coarse_dof=coarse_dof)
+        h = capi.hz_hierarchy()
+        libc.srand(1)
+        if which == "gpu":
+            st = lib.hz_setup_hierarchy(A.ptr(), C.byref(amg), C.byref(h))
+        else:
+            st = orc.orc_setup_hierarchy(A.ptr(), C.byref(amg), C.byref(h))
+        assert st == 0
+        levels = capi.hierarchy_to_numpy(h)
+        after = libc.rand()   # libc state after the setup
+        (lib.hz_hierarchy_free if which == "gpu" else orc.orc_hierarchy_free)(C.byref(h))
+        out.append((levels, amg.strong_coupled, after))
+    return out
+
+
+@pytest.mark.parametrize("agg_type", [capi.HEC, capi.HEM, capi.VMB])
+@pytest.mark.parametrize("theta", [0.0, 0.04, 0.25])
+def test_hierarchy_bitexact_solvers_fixture(lib, orc, libc, solvers_fixture, agg_type, theta):
+    ia, ja, va, _ = solvers_fixture
+    A = capi.HostCSR(ia, ja, va)
+    (lg, thg, rg), (lo, tho, ro) = _hierarchies(lib, orc, libc, A, agg_type, theta)
+    assert len(lg) == len(lo)
+    for g, o in zip(lg, lo):
+        for key in ("ia", "ja", "val"):
+            assert np.array_equal(g[key], o[key]), key
+        assert g["theta"] == o["theta"]
+        if "agg" in o:
+            assert g["nagg"] == o["nagg"]
+            assert np.array_equal(g["agg"], o["agg"])
+    assert thg == tho          # caller's strong_coupled mutated identically
+    assert rg == ro            # libc rand() state advanced identically (HEC)
+
+
+@pytest.mark.parametrize("m", [16, 24])
+@pytest.mark.parametrize("agg_type", [capi.HEC, capi.HEM, capi.VMB])
+def test_hierarchy_bitexact_kuhn3d(lib, orc, libc, m, agg_type):
+    ia, ja, va = problems.poisson3d_kuhn(m)
+    A = capi.HostCSR(ia, ja, va)
+    (lg, _, rg), (lo, _, ro) = _hierarchies(lib, orc, libc, A, agg_type, 0.04, coarse_dof=50)
+    assert len(lg) == len(lo)
+    for g, o in zip(lg, lo):
+        for key in ("ia", "ja", "val"):
+            assert np.array_equal(g[key], o[key]), key
+        if "agg" in o:
+            assert np.array_equal(g["agg"], o["agg"])
+    assert rg == ro
+
+
+def _solve_both(lib, orc, libc, A, b, **kw):
+    res = {}
+    for which in ("gpu", "oracle"):
+        x = capi.HostVec(np.zeros(b.size))
+        bb = capi.HostVec(b)
+        libc.srand(1)
+        if which == "gpu":
+            it, amg = capi.solver_params(lib, **kw)
+            r = lib.linear_solver_dcsr_krylov_amg(A.ptr(), bb.ptr(), x.ptr(), C.byref(it), C.byref(amg))
+        else:
+            import oracle
+            it, amg = oracle.params(**kw)
+            tr = capi.orc_trace()
+            r = orc.orc_krylov_amg(A.ptr(), bb.ptr(), x.ptr(), C.byref(it), C.byref(amg), C.byref(tr))
+        res[which] = (r, x.arr.copy())
+    return res
+
+
+@pytest.mark.parametrize("agg_type", [capi.HEC, capi.VMB, capi.HEM])
+@pytest.mark.parametrize("smoother", [capi.SMOOTHER_JACOBI, capi.SMOOTHER_L1DIAG])
+@pytest.mark.parametrize("coarse_solver", [32, 0])
+def test_pcg_parity_solvers_fixture(lib, orc, libc, solvers_fixture, agg_type, smoother, coarse_solver):
+    ia, ja, va, b = solvers_fixture
+    A = capi.HostCSR(ia, ja, va)
+    res = _solve_both(lib, orc, libc, A, b, tol=1e-10, maxit=400, aggregation=agg_type,
+                      smoother=smoother, coarse_solver=coarse_solver)
+    (ig, xg), (io, xo) = res["gpu"], res["oracle"]
+    assert ig >= 0 and io >= 0
+    assert abs(ig - io) <= 1
+    assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) < 1e-10
+
+
+@pytest.mark.parametrize("m", [16, 32])
+def test_pcg_parity_kuhn3d(lib, orc, libc, m):
+    ia, ja, va = problems.poisson3d_kuhn(m)
+    b = problems.uniform_rhs(ia.size - 1, seed=3)
+    A = capi.HostCSR(ia, ja, va)
+    res = _solve_both(lib, orc, libc, A, b, tol=1e-10, maxit=400, aggregation=capi.HEC,
+                      coarse_dof=100)
+    (ig, xg), (io, xo) = res["gpu"], res["oracle"]
+    assert abs(ig - io) <= 1
+    assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) < 1e-10

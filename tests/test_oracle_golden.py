@@ -1,0 +1,280 @@
+"""Pins the CPU oracle (oracle/hz_oracle.c) before it is trusted as the checker.
+
+1. Reference golden numbers: level sizes and PCG iteration counts the compiled
+   reference produced on its own fixture examples/solvers/A.dat + b.dat
+   (tests/golden/probe_reference.json).  They pin the aggregation algorithms
+   (including HEC's libc-rand() turn order), the coarse products, the cycle
+   and the PCG stopping logic together.
+2. Mathematics: the kernels against closed forms / brute force on small inputs,
+   chosen so that a dropped term, a wrong index or a transposed operand fails.
+"""
+import ctypes as C
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+import scipy.linalg
+import scipy.sparse as sp
+
+import oracle
+from hazmath_amd import capi, problems
+
+GOLD = json.loads((Path(__file__).parent / "golden" / "probe_reference.json").read_text())
+AGG = {"VMB": 1, "HEC": 4, "HEM": 5}
+SMOOTH = {"JACOBI": 1, "L1DIAG": 10}
+
+
+def _solve(orc, libc, A, b, **kw):
+    it, amg = oracle.params(**kw)
+    x = capi.HostVec(np.zeros(b.size))
+    bb = capi.HostVec(b)
+    tr = capi.orc_trace()
+    libc.srand(1)
+    r = orc.orc_krylov_amg(A.ptr(), bb.ptr(), x.ptr(), C.byref(it), C.byref(amg), C.byref(tr))
+    return r, tr, x.arr
+
+
+def test_fixture_is_reference_matrix(solvers_fixture):
+    ia, ja, va, b = solvers_fixture
+    assert ia.size - 1 == 16384 and ia[-1] == 81408
+    assert abs(np.linalg.norm(b) - GOLD["b_norm2"]) < 1e-9
+    # A.dat is the 128 x 128 interior 5-point Laplacian (4 / -1)
+    n = 128
+    T = sp.diags([-1, 2, -1], [-1, 0, 1], shape=(n, n))
+    L = (sp.kron(sp.eye(n), T) + sp.kron(T, sp.eye(n))).tocsr()
+    A = sp.csr_matrix((va, ja, ia), shape=(n * n, n * n))
+    assert abs(A - L).max() == 0
+
+
+@pytest.mark.parametrize("name,sizes", list(GOLD["levels"].items()))
+def test_level_sizes_match_reference(solvers_fixture, name, sizes):
+    orc, libc = oracle.load(), C.CDLL(None)
+    ia, ja, va, b = solvers_fixture
+    A = capi.HostCSR(ia, ja, va)
+    r, tr, _ = _solve(orc, libc, A, b, tol=1e-8, maxit=100, aggregation=AGG[name])
+    assert list(tr.level_rows[:tr.num_levels]) == sizes
+    opc = sum(tr.level_nnz[:tr.num_levels]) / tr.level_nnz[0]
+    assert round(opc, 3) == GOLD["operator_complexity"][name]
+
+
+CASES = [(k, v, 1e-8, 100) for k, v in GOLD["iterations_tol_1e-8"].items() if "GS" not in k and "coarseCG" not in k]
+CASES += [(k, v, 1e-10, 1000) for k, v in GOLD["iterations_tol_1e-10"].items()]
+
+
+@pytest.mark.parametrize("case,expected,tol,maxit", CASES)
+def test_iterations_match_reference(solvers_fixture, case, expected, tol, maxit):
+    orc, libc = oracle.load(), C.CDLL(None)
+    ia, ja, va, b = solvers_fixture
+    A = capi.HostCSR(ia, ja, va)
+    agg, sm = case.split("+")
+    r, tr, x = _solve(orc, libc, A, b, tol=tol, maxit=maxit, aggregation=AGG[agg], smoother=SMOOTH[sm])
+    if expected == ">100":
+        assert r == capi.ERROR_SOLVER_MAXIT
+    else:
+        assert r == expected
+        Asp = sp.csr_matrix((va, ja, ia))
+        assert np.linalg.norm(b - Asp @ x) / np.linalg.norm(b) < tol
+
+
+def test_coarse_cg_iterations_match_reference(solvers_fixture):
+    orc, libc = oracle.load(), C.CDLL(None)
+    ia, ja, va, b = solvers_fixture
+    A = capi.HostCSR(ia, ja, va)
+    r, _, _ = _solve(orc, libc, A, b, tol=1e-8, maxit=100, aggregation=1, coarse_solver=0)
+    assert r == GOLD["iterations_tol_1e-8"]["VMB+JACOBI+coarseCG"]
+
+
+def test_initial_residual(solvers_fixture):
+    orc, libc = oracle.load(), C.CDLL(None)
+    ia, ja, va, b = solvers_fixture
+    A = capi.HostCSR(ia, ja, va)
+    r, tr, _ = _solve(orc, libc, A, b, tol=1e-8, maxit=100, aggregation=4)
+    assert abs(np.linalg.norm(b) - GOLD["initial_residual_norm"]) < 5e-5
+
+
+# ---------------------------------------------------------------- mathematics
+
+def _rand_csr(n, seed, density=0.05):
+    rng = np.random.default_rng(seed)
+    M = sp.random(n, n, density=density, random_state=rng, format="csr")
+    M = M + sp.eye(n) * 3
+    M.sort_indices()
+    # shuffle column order inside each row: results must follow storage order
+    ja = M.indices.copy()
+    va = M.data.copy()
+    for i in range(n):
+        s, e = M.indptr[i], M.indptr[i + 1]
+        p = rng.permutation(e - s)
+        ja[s:e] = ja[s:e][p]
+        va[s:e] = va[s:e][p]
+    return M.indptr.astype(np.int32), ja.astype(np.int32), va
+
+
+def test_mxv_and_residual_closed_form():
+    orc = oracle.load()
+    ia, ja, va = _rand_csr(300, 1)
+    A = capi.HostCSR(ia, ja, va)
+    x = np.random.default_rng(2).standard_normal(300)
+    b = np.random.default_rng(3).standard_normal(300)
+    y = np.zeros(300)
+    w = np.zeros(300)
+    orc.orc_dcsr_mxv(A.ptr(), capi.dptr(x), capi.dptr(y))
+    orc.orc_dcsr_residual(A.ptr(), capi.dptr(b), capi.dptr(x), capi.dptr(w))
+    Asp = sp.csr_matrix((va, ja, ia))
+    assert np.allclose(y, Asp @ x, rtol=1e-13, atol=1e-13)
+    assert np.allclose(w, b - Asp @ x, rtol=1e-13, atol=1e-13)
+    # storage-order sums: recompute row 7 sequentially
+    s = 0.0
+    for k in range(ia[7], ia[8]):
+        s += va[k] * x[ja[k]]
+    assert y[7] == s
+
+
+def test_jacobi_and_l1_one_line_formulas():
+    orc = oracle.load()
+    ia, ja, va = _rand_csr(200, 4)
+    A = capi.HostCSR(ia, ja, va)
+    Asp = sp.csr_matrix((va, ja, ia))
+    d = Asp.diagonal()
+    x0 = np.random.default_rng(5).standard_normal(200)
+    b = np.random.default_rng(6).standard_normal(200)
+    for w in (1.0, 2.0 / 3.0):
+        x = x0.copy()
+        orc.orc_smoother_sweep(A.ptr(), capi.dptr(b), capi.dptr(x), 1, w)
+        ref = (1 - w) * x0 + w * (b - (Asp @ x0 - d * x0)) / d
+        assert np.allclose(x, ref, rtol=1e-12, atol=1e-12)
+    x = x0.copy()
+    orc.orc_smoother_sweep(A.ptr(), capi.dptr(b), capi.dptr(x), 10, 1.0)
+    l1 = np.asarray(abs(Asp).sum(axis=1)).ravel()
+    assert np.allclose(x, x0 + (b - Asp @ x0) / l1, rtol=1e-12, atol=1e-12)
+
+
+def test_rap_agg_is_galerkin_product_with_reference_layout():
+    orc = oracle.load()
+    n, nagg = 120, 37
+    ia, ja, va = _rand_csr(n, 8, density=0.08)
+    rng = np.random.default_rng(9)
+    agg = rng.integers(-1, nagg, size=n).astype(np.int32)
+    agg[:nagg] = np.arange(nagg)
+    A = capi.HostCSR(ia, ja, va)
+    Cm = capi.dCSRmat()
+    orc.orc_rap_agg(A.ptr(), capi.dptr(agg, capi.INT), nagg, C.byref(Cm))
+    cia, cja, cva = capi.csr_to_numpy(Cm)
+    P = sp.csr_matrix((np.ones((agg >= 0).sum()), (np.nonzero(agg >= 0)[0], agg[agg >= 0])),
+                      shape=(n, nagg))
+    Asp = sp.csr_matrix((va, ja, ia), shape=(n, n))
+    G = (P.T @ Asp @ P).toarray()
+    Cd = sp.csr_matrix((cva, cja, cia), shape=(nagg, nagg)).toarray()
+    assert np.allclose(Cd, G, rtol=1e-12, atol=1e-12)
+    # layout: diagonal first, then first-discovery order of the member scan
+    for c in range(nagg):
+        row = cja[cia[c]:cia[c + 1]]
+        assert row[0] == c
+        order = []
+        for i1 in np.nonzero(agg == c)[0]:
+            for k in range(ia[i1], ia[i1 + 1]):
+                g = agg[ja[k]]
+                if g >= 0 and g != c and g not in order:
+                    order.append(g)
+        assert list(row[1:]) == order
+
+
+def test_strength_graph_definition():
+    orc = oracle.load()
+    ia, ja, va = _rand_csr(150, 11, density=0.06)
+    va = -np.abs(va)
+    Asp = sp.csr_matrix((va, ja, ia))
+    d = Asp.diagonal().copy()
+    for i in range(150):        # diagonal positive
+        for k in range(ia[i], ia[i + 1]):
+            if ja[k] == i:
+                va[k] = 3.0
+    A = capi.HostCSR(ia, ja, va)
+    N = capi.dCSRmat()
+    theta = 0.3
+    orc.orc_strength(A.ptr(), theta, C.byref(N))
+    nia, nja, nva = capi.csr_to_numpy(N)
+    for i in range(150):
+        keep = [ja[k] for k in range(ia[i], ia[i + 1])
+                if ja[k] == i or (va[k] < 0 and va[k] ** 2 >= theta ** 2 * 9.0)]
+        assert list(nja[nia[i]:nia[i + 1]]) == keep
+
+
+def test_aggregates_partition_and_pairing_invariants():
+    """Every non-isolated vertex belongs to exactly one aggregate; HEM builds
+    pairs in pass 1 so every aggregate has >= 2 members; VMB respects the cap."""
+    orc = oracle.load()
+    ia, ja, va = problems.poisson2d_p1(20)
+    A = capi.HostCSR(ia, ja, va)
+    n = ia.size - 1
+    for typ in (1, 4, 5):
+        agg = np.zeros(n, np.int32)
+        nagg = C.c_int()
+        C.CDLL(None).srand(1)
+        st = orc.orc_aggregate(A.ptr(), typ, 0.04, 4, capi.dptr(agg, capi.INT), C.byref(nagg))
+        assert st == 0
+        assert agg.min() >= 0 and agg.max() == nagg.value - 1
+        counts = np.bincount(agg, minlength=nagg.value)
+        assert counts.min() >= 1
+        if typ == 5:
+            assert counts.min() >= 2
+        if typ == 1:
+            assert counts.max() <= 4
+
+
+def test_dense_coarse_solve_is_exact():
+    """coarse_solver 32 with one level: the preconditioner is an exact solve,
+    so PCG converges in one iteration."""
+    orc, libc = oracle.load(), C.CDLL(None)
+    ia, ja, va = problems.poisson2d_p1(12)
+    b = problems.uniform_rhs(ia.size - 1, 5)
+    A = capi.HostCSR(ia, ja, va)
+    r, tr, x = _solve(orc, libc, A, b, tol=1e-10, maxit=10, coarse_dof=1000)
+    assert tr.num_levels == 1
+    assert r == 1
+    Asp = sp.csr_matrix((va, ja, ia)).toarray()
+    assert np.allclose(x, scipy.linalg.solve(Asp, b), rtol=1e-12, atol=1e-12)
+
+
+def test_pcg_matches_textbook_cg_without_coarsening():
+    """With coarse_solver 0 and one level the preconditioner is itself a CG
+    solve to 1e-8; the outer PCG solution must match a dense solve."""
+    orc, libc = oracle.load(), C.CDLL(None)
+    ia, ja, va = problems.poisson2d_p1(16)
+    b = problems.uniform_rhs(ia.size - 1, 6)
+    A = capi.HostCSR(ia, ja, va)
+    r, tr, x = _solve(orc, libc, A, b, tol=1e-10, maxit=50, coarse_dof=1000, coarse_solver=0)
+    xs = scipy.linalg.solve(sp.csr_matrix((va, ja, ia)).toarray(), b)
+    assert r > 0
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-9
+
+
+def test_condest_is_ratio_of_tridiagonal_extreme_eigenvalues():
+    orc = oracle.load()
+    rng = np.random.default_rng(12)
+    m = 25
+    alpha = rng.uniform(0.2, 1.5, m)
+    beta = rng.uniform(0.01, 0.9, m)
+    got = orc.orc_tridiag_condest(m, capi.dptr(alpha), capi.dptr(beta))
+    d0 = np.empty(m)
+    d0[0] = 1 / alpha[0]
+    d0[1:] = 1 / alpha[1:] + beta[:-1] / alpha[:-1]
+    e = np.sqrt(beta[:-1]) / alpha[:-1]
+    ev = np.linalg.eigvalsh(np.diag(d0) + np.diag(e, 1) + np.diag(e, -1))
+    assert abs(got - ev[-1] / ev[0]) < 1e-9 * ev[-1] / ev[0]
+
+
+def test_theta_adaptation_mutates_caller_param():
+    orc = oracle.load()
+    ia, ja, va = problems.poisson2d_p1(32)
+    A = capi.HostCSR(ia, ja, va)
+    it, amg = oracle.params(aggregation=5, theta=0.04, coarse_dof=10)
+    h = capi.hz_hierarchy()
+    assert orc.orc_setup_hierarchy(A.ptr(), C.byref(amg), C.byref(h)) == 0
+    thetas = [h.theta[i] for i in range(h.num_levels)]
+    # HEM halves the size each level: nagg*4 > n always -> theta halves per level
+    for l in range(h.num_levels - 1):
+        assert thetas[l + 1] == thetas[l] / 2
+    assert amg.strong_coupled == thetas[-1]
+    orc.orc_hierarchy_free(C.byref(h))
