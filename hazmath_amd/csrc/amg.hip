@@ -8,7 +8,9 @@
 // Every vector operation runs on the GPU; the host only holds the scalar
 // control flow (one 256-byte device->host copy per PCG iteration).
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 #include "amg.hpp"
@@ -60,6 +62,21 @@ Arena &arena() {
     return a;
 }
 
+bool trace_enabled() {
+    static int on = -1;
+    if (on < 0) {
+        const char *e = std::getenv("HZ_TRACE");
+        on = (e && e[0] && e[0] != '0') ? 1 : 0;
+    }
+    return on == 1;
+}
+
+double wall_ms() {
+    return std::chrono::duration<double, std::milli>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
 hipStream_t stream() {
     static hipStream_t s = nullptr;
     if (!s) HZ_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
@@ -105,8 +122,12 @@ int amg_setup(Hierarchy &H, const DevCSR &A0, AMG_param *param, bool) {
                         "amg_setup_unsmoothP_unsmoothR");
             return -1013;
         }
+        const bool tr = trace_enabled();
+        double t0 = 0, t1 = 0, t2 = 0, t3 = 0;
+        if (tr) { HZ_CHECK(hipStreamSynchronize(stream())); t0 = wall_ms(); }
         DevCSR N;
         strength_graph(L.A, param->strong_coupled, N, counts);
+        if (tr) { HZ_CHECK(hipStreamSynchronize(stream())); t1 = wall_ms(); }
         L.agg = arena().alloc_n<int>((size_t)n + 1);
         int nagg = 0, st = HZ_SUCCESS;
         if (type == HZ_HEC) {
@@ -119,9 +140,12 @@ int amg_setup(Hierarchy &H, const DevCSR &A0, AMG_param *param, bool) {
                 return -1016;
             }
             nagg = r;
+        } else if (type == HZ_HEM) {
+            st = hem_aggregate_gpu(N, L.agg, &nagg);
+            if (st == 1) st = aggregate_host(N, type, param->max_aggregation, L.agg, &nagg);
+            if (st == -1017) return -1017;
         } else {
             st = aggregate_host(N, type, param->max_aggregation, L.agg, &nagg);
-            if (st == -1017) return -1017;
         }
         L.nagg = nagg;
         // amg_setup_ua.c:1325-1328
@@ -129,12 +153,19 @@ int amg_setup(Hierarchy &H, const DevCSR &A0, AMG_param *param, bool) {
         else if (nagg * 1.25 < n) param->strong_coupled *= 2;
         if (st < 0) break;
         if (nagg < 1) break;
+        if (tr) { HZ_CHECK(hipStreamSynchronize(stream())); t2 = wall_ms(); }
         L.rptr = arena().alloc_n<int>((size_t)nagg + 1);
         L.rmem = arena().alloc_n<int>((size_t)n + 1);
         build_restriction(n, L.agg, nagg, L.rptr, L.rmem);
         Level &C = H.lv[lvl + 1];
         rap_agg(L.A, L.agg, nagg, L.rptr, L.rmem, C.A);
         sell_from_csr(C.A, C.S);
+        if (tr) {
+            HZ_CHECK(hipStreamSynchronize(stream()));
+            t3 = wall_ms();
+            std::fprintf(stderr, "[hz] level %d n=%d nnz=%d: strength %.2f ms, aggregate %.2f ms (nagg=%d), "
+                         "R+RAP+SELL %.2f ms\n", lvl, n, L.A.nnz, t1 - t0, t2 - t1, nagg, t3 - t2);
+        }
         ++lvl;
     }
     H.nl = lvl + 1;

@@ -61,7 +61,11 @@ struct PcgResult {
 PcgResult pcg_solve(Hierarchy &H, const DevSELL &A, const double *b, double *u, double tol,
                     int maxit, int stop_type, int prtlvl);
 
-// Host-staged sequential aggregation for VMB / HEM.
+// Host-staged sequential aggregation for VMB (and HEM on non-symmetric patterns).
 int aggregate_host(const DevCSR &N, int type, int max_aggregation, int *agg_dev, int *nagg);
+
+// HEM on the GPU: SUCCESS, -1017, or 1 when the strength pattern is not
+// symmetric (then the host sweep is used).
+int hem_aggregate_gpu(const DevCSR &N, int *agg, int *nagg_out);
 
 }  // namespace hz

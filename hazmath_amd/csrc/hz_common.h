@@ -66,10 +66,16 @@ inline hipError_t memcpy_sync(void *dst, const void *src, size_t bytes, hipMemcp
 
 inline int div_up(long a, long b) { return static_cast<int>((a + b - 1) / b); }
 
+// HZ_TRACE=1 in the environment prints per-level setup phases to stderr.
+bool trace_enabled();
+double wall_ms();   // monotonic clock
+
 // ---------------------------------------------------------------------------
 // Deterministic reductions: fixed tree inside a block, block partials stored by
-// block index, the last block to finish (atomic ticket) sums the partials in
-// index order.  The result depends only on the grid size, never on timing.
+// block index, and a single-block finalize kernel sums the partials in index
+// order.  The result depends only on the grid size, never on timing.  (A
+// last-block-done ticket would need a device-scope fence per block, which on
+// this chip writes back the XCD's L2 and costs far more than a launch.)
 // ---------------------------------------------------------------------------
 template <int BLOCK>
 __device__ __forceinline__ double block_sum(double v, double *sh) {
@@ -101,21 +107,6 @@ __device__ __forceinline__ double block_max(double v, double *sh) {
     }
     __syncthreads();
     return r;
-}
-
-// Called by every thread of a block after thread 0 wrote its partial(s).
-// Returns true in the block that must finalize (all partials visible).
-__device__ __forceinline__ bool last_block_ticket(unsigned int *ticket) {
-    __shared__ bool is_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned int t = atomicAdd(ticket, 1u);
-        is_last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (is_last) __threadfence();
-    return is_last;
 }
 
 // Sum of partials[0..n) by one block in a fixed order (strided per thread,

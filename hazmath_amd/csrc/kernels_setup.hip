@@ -178,6 +178,19 @@ __global__ void k_agg_count(int n, const int *__restrict__ agg, int *__restrict_
 // the in-order sum of its candidates starting from the first one.
 // ---------------------------------------------------------------------------
 constexpr int kRapCap = 512;    // candidates held in LDS per row (larger rows use global scratch)
+constexpr int kRapHash = 1024;  // LDS hash slots (>= 2 * kRapCap)
+
+__device__ __forceinline__ unsigned rap_hash(int key, unsigned mask) {
+    unsigned h = (unsigned)key * 2654435761u;
+    return (h ^ (h >> 15)) & mask;
+}
+
+// Slot of `key` (inserted before) in an open-addressing table.
+__device__ __forceinline__ unsigned rap_find(const int *keys, int key, unsigned mask) {
+    unsigned slot = rap_hash(key, mask);
+    while (keys[slot] != key) slot = (slot + 1) & mask;
+    return slot;
+}
 
 template <bool kFill, bool kGlobal>
 __global__ __launch_bounds__(kWave) void k_rap_row(
@@ -186,16 +199,19 @@ __global__ __launch_bounds__(kWave) void k_rap_row(
     const double *__restrict__ aval, const int *__restrict__ agg, int *__restrict__ rowcount,
     int *__restrict__ rowlen, const int *__restrict__ cia, int *__restrict__ cja,
     double *__restrict__ cval, int *__restrict__ gcol, double *__restrict__ gval,
-    const long long *__restrict__ goff) {
+    const long long *__restrict__ goff, int *__restrict__ gkeys, int *__restrict__ gminp,
+    const long long *__restrict__ hoff) {
     __shared__ int s_col[kGlobal ? 1 : kRapCap];
     __shared__ double s_val[(kGlobal || !kFill) ? 1 : kRapCap];
+    __shared__ int s_keys[kGlobal ? 1 : kRapHash];
+    __shared__ int s_minp[kGlobal ? 1 : kRapHash];
     const int lane = threadIdx.x;
     const int idx = blockIdx.x;
     if (idx >= nrows) return;
     const int c = rowlist ? rowlist[idx] : idx;
     int *col = kGlobal ? gcol + goff[idx] : s_col;
     double *val = kGlobal ? gval + goff[idx] : s_val;
-    // 1) enumerate candidates
+    // 1) enumerate candidates (members ascending x stored row order)
     int L = 0;
     const int q1 = rptr[c + 1];
     for (int q = rptr[c]; q < q1; ++q) {
@@ -214,22 +230,49 @@ __global__ __launch_bounds__(kWave) void k_rap_row(
         if (!kFill && lane == 0) { rowcount[c] = -1; rowlen[c] = L; }
         return;
     }
+    // 2) hash table of distinct coarse columns -> first position
+    unsigned mask;
+    int *keys, *minp;
+    if (kGlobal) {
+        const long long h0 = hoff[idx];
+        mask = (unsigned)(hoff[idx + 1] - h0) - 1u;
+        keys = gkeys + h0;
+        minp = gminp + h0;
+    } else {
+        mask = kRapHash - 1u;
+        keys = s_keys;
+        minp = s_minp;
+    }
+    for (unsigned t = lane; t <= mask; t += kWave) {
+        keys[t] = -1;
+        minp[t] = INT_MAX;
+    }
     __syncthreads();
-    // 2) first appearances (excluding UNPT and the diagonal column c)
-    int nfirst_before = 0;   // running count of firsts in earlier chunks
+    for (int p = lane; p < L; p += kWave) {
+        const int key = col[p];
+        if (key <= HZ_UNPT || key == c) continue;
+        unsigned slot = rap_hash(key, mask);
+        while (true) {
+            const int prev = atomicCAS(&keys[slot], -1, key);
+            if (prev == -1 || prev == key) break;
+            slot = (slot + 1) & mask;
+        }
+        atomicMin(&minp[slot], p);
+    }
+    __syncthreads();
+    // 3) first appearances, their ranks and (fill) in-order sums
+    int nfirst_before = 0;
     for (int base = 0; base < L; base += kWave) {
         const int p = base + lane;
         bool first = false;
         int cp = -1;
         if (p < L) {
             cp = col[p];
-            first = (cp > HZ_UNPT) && (cp != c);
-            for (int q = 0; q < p && first; ++q)
-                if (col[q] == cp) first = false;
+            if (cp > HZ_UNPT && cp != c) first = (minp[rap_find(keys, cp, mask)] == p);
         }
-        const unsigned long long mask = __ballot(first);
+        const unsigned long long m = __ballot(first);
         if (kFill && first) {
-            const int rank = nfirst_before + __popcll(mask & ((1ull << lane) - 1ull));
+            const int rank = nfirst_before + __popcll(m & ((1ull << lane) - 1ull));
             double v = val[p];
             for (int q = p + 1; q < L; ++q)
                 if (col[q] == cp) v = v + val[q];
@@ -237,7 +280,7 @@ __global__ __launch_bounds__(kWave) void k_rap_row(
             cja[pos] = cp;
             cval[pos] = v;
         }
-        nfirst_before += __popcll(mask);
+        nfirst_before += __popcll(m);
     }
     if (lane == 0) {
         if (kFill) {
@@ -367,7 +410,8 @@ void rap_agg(const DevCSR &A, const int *agg, int nagg, const int *rptr, const i
     if (nagg > 0) {
         k_rap_row<false, false><<<nagg, kWave, 0, st>>>(nagg, nullptr, rptr, rmem, A.ia, A.ja, A.val,
                                                         agg, rowcount, rowlen, nullptr, nullptr,
-                                                        nullptr, nullptr, nullptr, nullptr);
+                                                        nullptr, nullptr, nullptr, nullptr, nullptr,
+                                                        nullptr, nullptr);
         HZ_LAUNCH_CHECK();
     }
     // rows whose candidate list exceeds the LDS capacity
@@ -384,23 +428,32 @@ void rap_agg(const DevCSR &A, const int *agg, int nagg, const int *rptr, const i
     int *gcol = nullptr;
     double *gval = nullptr;
     long long *goff = nullptr;
+    int *gkeys = nullptr, *gminp = nullptr;
+    long long *hoffd = nullptr;
     if (hnol > 0) {
         // deterministic order of the overflow list does not matter: each row is independent
         std::vector<int> hl(hnol), hlen(hnol);
         HZ_CHECK(memcpy_sync(hl.data(), olist, sizeof(int) * hnol, hipMemcpyDeviceToHost));
-        std::vector<long long> hoff(hnol + 1, 0);
+        std::vector<long long> hoff(hnol + 1, 0), hh(hnol + 1, 0);
         for (int i = 0; i < hnol; ++i) {
             int L = 0;
             HZ_CHECK(memcpy_sync(&L, rowlen + hl[i], sizeof(int), hipMemcpyDeviceToHost));
             hoff[i + 1] = hoff[i] + L;
+            long long H = 1;
+            while (H < 2LL * L) H <<= 1;
+            hh[i + 1] = hh[i] + H;
         }
+        hoffd = arena().alloc_n<long long>((size_t)hnol + 1);
+        HZ_CHECK(memcpy_sync(hoffd, hh.data(), sizeof(long long) * (hnol + 1), hipMemcpyHostToDevice));
+        gkeys = arena().alloc_n<int>((size_t)hh[hnol] + 1);
+        gminp = arena().alloc_n<int>((size_t)hh[hnol] + 1);
         goff = arena().alloc_n<long long>((size_t)hnol + 1);
         HZ_CHECK(memcpy_sync(goff, hoff.data(), sizeof(long long) * (hnol + 1), hipMemcpyHostToDevice));
         gcol = arena().alloc_n<int>((size_t)hoff[hnol] + 1);
         gval = arena().alloc_n<double>((size_t)hoff[hnol] + 1);
         k_rap_row<false, true><<<hnol, kWave, 0, st>>>(hnol, olist, rptr, rmem, A.ia, A.ja, A.val, agg,
                                                        rowcount, rowlen, nullptr, nullptr, nullptr,
-                                                       gcol, gval, goff);
+                                                       gcol, gval, goff, gkeys, gminp, hoffd);
         HZ_LAUNCH_CHECK();
     }
     Ac.n = nagg;
@@ -412,13 +465,14 @@ void rap_agg(const DevCSR &A, const int *agg, int nagg, const int *rptr, const i
     if (nagg > 0) {
         k_rap_row<true, false><<<nagg, kWave, 0, st>>>(nagg, nullptr, rptr, rmem, A.ia, A.ja, A.val, agg,
                                                        nullptr, nullptr, Ac.ia, Ac.ja, Ac.val,
-                                                       nullptr, nullptr, nullptr);
+                                                       nullptr, nullptr, nullptr, nullptr, nullptr,
+                                                       nullptr);
         HZ_LAUNCH_CHECK();
     }
     if (hnol > 0) {
         k_rap_row<true, true><<<hnol, kWave, 0, st>>>(hnol, olist, rptr, rmem, A.ia, A.ja, A.val, agg,
                                                       nullptr, nullptr, Ac.ia, Ac.ja, Ac.val, gcol,
-                                                      gval, goff);
+                                                      gval, goff, gkeys, gminp, hoffd);
         HZ_LAUNCH_CHECK();
     }
 }

@@ -63,9 +63,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n, int nslices, const long 
                                                  const int *__restrict__ col,
                                                  const double *__restrict__ val,
                                                  const double *__restrict__ x, double *__restrict__ y,
-                                                 double *__restrict__ partials,
-                                                 unsigned int *__restrict__ ticket,
-                                                 double *__restrict__ dot_out) {
+                                                 double *__restrict__ partials) {
     __shared__ double sh[kBlock / kWave];
     const int blk = xcd_block_id();
     const int s = blk * kSlicesPerBlock + threadIdx.x / kWave;
@@ -103,13 +101,6 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n, int nslices, const long 
     if (kDot) {
         const double bs = block_sum<kBlock>(prod, sh);
         if (threadIdx.x == 0) partials[blk] = bs;
-        if (last_block_ticket(ticket)) {
-            const double tot = final_sum<kBlock>(partials, gridDim.x, sh);
-            if (threadIdx.x == 0) {
-                *dot_out = tot;
-                *ticket = 0u;
-            }
-        }
     }
 }
 
@@ -278,16 +269,14 @@ __global__ void k_copy(double *__restrict__ d, const double *__restrict__ s, lon
         d[i] = s[i];
 }
 
-// u += alpha p ; r -= alpha t ; then ||r||^2, ||u||_inf, ||u||^2, ||p||^2.
-// alpha = zr / temp2; block 0 also records alpha and moves zr into temp1.
-__global__ __launch_bounds__(kBlock) void k_pcg_update(int n, PcgScalars *__restrict__ s,
+// u += alpha p ; r -= alpha t ; then per-block partials of ||r||^2, ||u||_inf,
+// ||u||^2, ||p||^2 (finalized by k_final_update).  alpha = zr / temp2.
+__global__ __launch_bounds__(kBlock) void k_pcg_update(int n, const PcgScalars *__restrict__ s,
                                                        const double *__restrict__ p,
                                                        const double *__restrict__ t,
                                                        double *__restrict__ u,
                                                        double *__restrict__ r,
-                                                       double *__restrict__ alpha_hist, int iter,
-                                                       double *__restrict__ partials,
-                                                       unsigned int *__restrict__ ticket) {
+                                                       double *__restrict__ partials) {
     __shared__ double sh[kBlock / kWave];
     const double alpha = s->zr / s->temp2;
     const double malpha = -alpha;
@@ -320,41 +309,47 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, PcgScalars *__rest
     if (threadIdx.x == 0) partials[2 * G + blockIdx.x] = v;
     v = block_sum<kBlock>(pp, sh);
     if (threadIdx.x == 0) partials[3 * G + blockIdx.x] = v;
-    if (last_block_ticket(ticket)) {
-        const double a0 = final_sum<kBlock>(partials + 0 * G, G, sh);
-        const double a1 = final_max<kBlock>(partials + 1 * G, G, sh);
-        const double a2 = final_sum<kBlock>(partials + 2 * G, G, sh);
-        const double a3 = final_sum<kBlock>(partials + 3 * G, G, sh);
-        if (threadIdx.x == 0) {
-            s->rr = a0;
-            s->uinf = a1;
-            s->uu = a2;
-            s->pp = a3;
-            s->alpha = alpha;
-            s->temp1 = s->zr;
-            if (alpha_hist) alpha_hist[iter - 1] = alpha;
-            *ticket = 0u;
-        }
+}
+
+constexpr int kFinBlock = 1024;
+
+// Single-block finalizers: the partials are summed in block-index order, so the
+// totals depend only on the grid size.
+__global__ __launch_bounds__(kFinBlock) void k_final_sum(const double *__restrict__ partials, int G,
+                                                         double *__restrict__ out) {
+    __shared__ double sh[kFinBlock / kWave];
+    const double t = final_sum<kFinBlock>(partials, G, sh);
+    if (threadIdx.x == 0) *out = t;
+}
+
+__global__ __launch_bounds__(kFinBlock) void k_final_update(PcgScalars *__restrict__ s,
+                                                            const double *__restrict__ partials, int G,
+                                                            double *__restrict__ alpha_hist, int iter) {
+    __shared__ double sh[kFinBlock / kWave];
+    const double a0 = final_sum<kFinBlock>(partials + 0 * G, G, sh);
+    const double a1 = final_max<kFinBlock>(partials + 1 * G, G, sh);
+    const double a2 = final_sum<kFinBlock>(partials + 2 * G, G, sh);
+    const double a3 = final_sum<kFinBlock>(partials + 3 * G, G, sh);
+    if (threadIdx.x == 0) {
+        const double alpha = s->zr / s->temp2;
+        s->rr = a0;
+        s->uinf = a1;
+        s->uu = a2;
+        s->pp = a3;
+        s->alpha = alpha;
+        s->temp1 = s->zr;
+        if (alpha_hist) alpha_hist[iter - 1] = alpha;
     }
 }
 
 __global__ __launch_bounds__(kBlock) void k_dot(int n, const double *__restrict__ x,
                                                 const double *__restrict__ y,
-                                                double *__restrict__ out,
-                                                double *__restrict__ partials,
-                                                unsigned int *__restrict__ ticket) {
+                                                double *__restrict__ partials) {
     __shared__ double sh[kBlock / kWave];
     double acc = 0.0;
     for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) acc += x[i] * y[i];
     const double v = block_sum<kBlock>(acc, sh);
     if (threadIdx.x == 0) partials[blockIdx.x] = v;
-    if (last_block_ticket(ticket)) {
-        const double tot = final_sum<kBlock>(partials, gridDim.x, sh);
-        if (threadIdx.x == 0) {
-            *out = tot;
-            *ticket = 0u;
-        }
-    }
 }
 
 // beta = zr / temp1 ; p = 1.0*z + beta*p (array_axpby, array.c:423-433)
@@ -376,9 +371,7 @@ __global__ __launch_bounds__(kBlock) void k_coarse_scaling(int n, int nslices,
                                                            const double *__restrict__ val,
                                                            const double *__restrict__ x,
                                                            const double *__restrict__ b,
-                                                           double *__restrict__ partials,
-                                                           unsigned int *__restrict__ ticket,
-                                                           double *__restrict__ alpha_out) {
+                                                           double *__restrict__ partials) {
     __shared__ double sh[kBlock / kWave];
     const int s = blockIdx.x * kSlicesPerBlock + threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
@@ -402,14 +395,16 @@ __global__ __launch_bounds__(kBlock) void k_coarse_scaling(int n, int nslices,
     if (threadIdx.x == 0) partials[blockIdx.x] = v;
     v = block_sum<kBlock>(xax, sh);
     if (threadIdx.x == 0) partials[G + blockIdx.x] = v;
-    if (last_block_ticket(ticket)) {
-        const double a = final_sum<kBlock>(partials, G, sh);
-        const double c = final_sum<kBlock>(partials + G, G, sh);
-        if (threadIdx.x == 0) {
-            double al = a / c;
-            *alpha_out = al < 2.0 ? al : 2.0;
-            *ticket = 0u;
-        }
+}
+
+__global__ __launch_bounds__(kFinBlock) void k_final_cs(const double *__restrict__ partials, int G,
+                                                        double *__restrict__ alpha_out) {
+    __shared__ double sh[kFinBlock / kWave];
+    const double a = final_sum<kFinBlock>(partials, G, sh);
+    const double c = final_sum<kFinBlock>(partials + G, G, sh);
+    if (threadIdx.x == 0) {
+        const double al = a / c;
+        *alpha_out = al < 2.0 ? al : 2.0;
     }
 }
 
@@ -459,10 +454,12 @@ void spmv(const DevSELL &S, const double *x, double *y, double *dot_out, const R
     const int grid = div_up(S.nslices, kSlicesPerBlock);
     if (dot_out) {
         k_spmv<true><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col, S.val, x, y,
-                                                    ws.partials, ws.tickets, dot_out);
+                                                    ws.partials);
+        HZ_LAUNCH_CHECK();
+        k_final_sum<<<1, kFinBlock, 0, stream()>>>(ws.partials, grid, dot_out);
     } else {
         k_spmv<false><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col, S.val, x, y,
-                                                     nullptr, nullptr, nullptr);
+                                                     nullptr);
     }
     HZ_LAUNCH_CHECK();
 }
@@ -533,13 +530,18 @@ void copy(double *dst, const double *src, long n) {
 
 void pcg_update(int n, PcgScalars *s, const double *p, const double *t, double *u, double *r,
                 double *alpha_hist, int iter, const ReduceWs &ws) {
-    k_pcg_update<<<stream_grid(n), kBlock, 0, stream()>>>(n, s, p, t, u, r, alpha_hist, iter,
-                                                          ws.partials, ws.tickets);
+    const int G = stream_grid(n);
+    k_pcg_update<<<G, kBlock, 0, stream()>>>(n, s, p, t, u, r, ws.partials);
+    HZ_LAUNCH_CHECK();
+    k_final_update<<<1, kFinBlock, 0, stream()>>>(s, ws.partials, G, alpha_hist, iter);
     HZ_LAUNCH_CHECK();
 }
 
 void dot(int n, const double *x, const double *y, double *out, const ReduceWs &ws) {
-    k_dot<<<stream_grid(n), kBlock, 0, stream()>>>(n, x, y, out, ws.partials, ws.tickets);
+    const int G = stream_grid(n);
+    k_dot<<<G, kBlock, 0, stream()>>>(n, x, y, ws.partials);
+    HZ_LAUNCH_CHECK();
+    k_final_sum<<<1, kFinBlock, 0, stream()>>>(ws.partials, G, out);
     HZ_LAUNCH_CHECK();
 }
 
@@ -554,7 +556,9 @@ void coarse_scaling_alpha(const DevSELL &S, const double *x, const double *b, Pc
                           double *alpha_out, const ReduceWs &ws) {
     const int grid = div_up(S.nslices, kSlicesPerBlock);
     k_coarse_scaling<<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col, S.val, x, b,
-                                                    ws.partials, ws.tickets, alpha_out);
+                                                    ws.partials);
+    HZ_LAUNCH_CHECK();
+    k_final_cs<<<1, kFinBlock, 0, stream()>>>(ws.partials, grid, alpha_out);
     HZ_LAUNCH_CHECK();
 }
 
