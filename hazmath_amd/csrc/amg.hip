@@ -273,17 +273,54 @@ void presmooth(Hierarchy &H, int l, CycleState &cs) {
     }
 }
 
-void postsmooth(Hierarchy &H, int l) {
+// Levels up to this many rows are latency-bound: they use the fused
+// one-thread-per-coarse-row restriction and the prolongation folded into the
+// sweep (HZ_FUSE_DOWN_MAX overrides the default).
+int fused_down_max_rows() {
+    static int v = -1;
+    if (v < 0) {
+        const char *e = std::getenv("HZ_FUSE_DOWN_MAX");
+        v = e ? std::atoi(e) : 300000;
+    }
+    return v;
+}
+
+// Prolongation x_l += alpha P x_{l+1} followed by the post-smoothing sweeps;
+// the prolongation is folded into the first sweep.  dot_out: when non-null
+// and the fused sweep is the last one, it also yields sum_i x_i b_i.
+void prolong_postsmooth(Hierarchy &H, int l, double alpha, const double *alpha_dev, double *dot_out,
+                        bool *dot_done) {
     Level &L = H.lv[l];
+    Level &C = H.lv[l + 1];
     const CycleParams &cp = H.cp;
-    for (int s = 0; s < cp.postsmooth; ++s) {
+    if (cp.postsmooth <= 0) {
+        prolong_agg(L.A.n, L.agg, C.x, alpha, alpha_dev, L.x);
+        return;
+    }
+    const bool want_dot = dot_out && cp.postsmooth == 1;
+    if (L.A.n <= fused_down_max_rows()) {
+        // latency-bound level: one kernel
+        up_fused(L.S, cp.smoother, cp.jacobi_w, L.agg, C.x, alpha, alpha_dev, L.b, L.x, L.xt,
+                 want_dot ? dot_out : nullptr, H.ws);
+    } else {
+        // bandwidth-bound level: gathering P xc per matrix entry costs more than one pass
+        prolong_agg(L.A.n, L.agg, C.x, alpha, alpha_dev, L.x);
+        if (want_dot)
+            smooth_sweep_dot(L.S, cp.smoother, cp.jacobi_w, L.b, L.x, L.xt, dot_out, H.ws);
+        else
+            smooth_sweep(L.S, cp.smoother, cp.jacobi_w, L.b, L.x, L.xt);
+    }
+    if (want_dot && dot_done) *dot_done = true;
+    std::swap(L.x, L.xt);
+    for (int s = 1; s < cp.postsmooth; ++s) {
         smooth_sweep(L.S, cp.smoother, cp.jacobi_w, L.b, L.x, L.xt);
         std::swap(L.x, L.xt);
     }
 }
 
+
 // mgcycle, mgcycle.c:245-409 (UA transfers, no Schwarz levels)
-void mgcycle(Hierarchy &H, CycleState &cs) {
+void mgcycle(Hierarchy &H, CycleState &cs, double *zr_out, bool *zr_done) {
     const CycleParams &cp = H.cp;
     const int nl = H.nl;
     int l = 0;
@@ -296,11 +333,16 @@ forward:
         Level &C = H.lv[l + 1];
         cs.num_lvl[l]++;
         presmooth(H, l, cs);
-        residual(L.S, L.b, L.x, L.w);
         // the next level restarts from zero; fuse its first sweep into the restriction
         const bool fuse = (l + 1 < nl - 1) && cp.presmooth > 0;
-        restrict_agg(L.nagg, L.rptr, L.rmem, L.w, C.b, cp.smoother, cp.jacobi_w,
-                     fuse ? C.dsm : nullptr, fuse ? C.x : nullptr);
+        if (L.A.n <= fused_down_max_rows()) {
+            down_fused(L.nagg, L.rptr, L.rmem, L.A, L.b, L.x, C.b, cp.smoother, cp.jacobi_w,
+                       fuse ? C.dsm : nullptr, fuse ? C.x : nullptr);
+        } else {
+            residual(L.S, L.b, L.x, L.w);
+            restrict_agg(L.nagg, L.rptr, L.rmem, L.w, C.b, cp.smoother, cp.jacobi_w,
+                         fuse ? C.dsm : nullptr, fuse ? C.x : nullptr);
+        }
         ++l;
         cs.zero[l] = true;
         cs.first_done[l] = fuse;
@@ -323,8 +365,8 @@ forward:
             coarse_scaling_alpha(C.S, C.x, C.b, H.sc, H.alpha_cs, H.ws);
             alpha_on_device = true;
         }
-        prolong_agg(L.A.n, L.agg, C.x, alpha, alpha_on_device ? H.alpha_cs : nullptr, L.x);
-        postsmooth(H, l);
+        prolong_postsmooth(H, l, alpha, alpha_on_device ? H.alpha_cs : nullptr,
+                           l == 0 ? zr_out : nullptr, l == 0 ? zr_done : nullptr);
         if (cs.num_lvl[l] < cp.cycle_type) break;
         cs.num_lvl[l] = 0;
     }
@@ -334,7 +376,9 @@ forward:
 }  // namespace
 
 // precond_amg (precond.c:164-183): b0 = r, x0 = 0, maxit cycles, z = x0.
-void amg_precond(Hierarchy &H, const double *r, double *z) {
+// When zr_out is given and the last level-0 sweep can produce it, (z, r) is
+// written there and true is returned.
+bool amg_precond(Hierarchy &H, const double *r, double *z, double *zr_out) {
     Level &L0 = H.lv[0];
     const CycleParams &cp = H.cp;
     const int n = L0.A.n;
@@ -348,7 +392,7 @@ void amg_precond(Hierarchy &H, const double *r, double *z) {
             for (int it = 0; it < cp.maxit; ++it) coarse_cg(L0.S, r, z, cp.coarse_tol, 20 * n, L0.cgwork);
         }
         if (cp.maxit <= 0) fill(z, 0.0, n);
-        return;
+        return false;
     }
     // Arrange the ping-pong so that the last sweep of level 0 writes into z.
     const int swaps = (cp.presmooth > 0 ? cp.presmooth - 1 : 0) + cp.postsmooth +
@@ -367,11 +411,14 @@ void amg_precond(Hierarchy &H, const double *r, double *z) {
         cs.first_done[i] = false;
     }
     cs.zero[0] = true;
-    for (int it = 0; it < cp.maxit; ++it) mgcycle(H, cs);
+    bool zr_done = false;
+    for (int it = 0; it < cp.maxit; ++it)
+        mgcycle(H, cs, it == cp.maxit - 1 ? zr_out : nullptr, &zr_done);
     if (cp.maxit <= 0) fill(L0.x, 0.0, n);
     if (L0.x != z) copy(z, L0.x, n);
     L0.x = nullptr;
     L0.xt = nullptr;
+    return zr_done;
 }
 
 // ---------------------------------------------------------------------------
@@ -441,6 +488,23 @@ PcgScalars *pinned_scalars() {
     return h;
 }
 
+hipEvent_t fetch_event() {
+    static hipEvent_t e = nullptr;
+    if (!e) HZ_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return e;
+}
+
+// Queue a copy of the device scalars into pinned memory (non-blocking).
+void fetch_async(const Hierarchy &H) {
+    HZ_CHECK(hipMemcpyAsync(pinned_scalars(), H.sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, stream()));
+    HZ_CHECK(hipEventRecord(fetch_event(), stream()));
+}
+
+void fetch_wait(PcgScalars &out) {
+    HZ_CHECK(hipEventSynchronize(fetch_event()));
+    out = *pinned_scalars();
+}
+
 void fetch(const Hierarchy &H, PcgScalars &out) {
     PcgScalars *h = pinned_scalars();
     HZ_CHECK(hipMemcpyAsync(h, H.sc, sizeof(PcgScalars), hipMemcpyDeviceToHost, stream()));
@@ -469,12 +533,14 @@ PcgResult pcg_solve(Hierarchy &H, const DevSELL &A, const double *b, double *u, 
     PcgScalars hs{};
     double *sc_rr = &H.sc->rr, *sc_zr = &H.sc->zr;
 
+    auto precond_zr = [&](const double *rr_, double *zz_) {
+        if (!amg_precond(H, rr_, zz_, sc_zr)) dot(m, zz_, rr_, sc_zr, H.ws);
+    };
     auto measure = [&](double &absr) -> double {
         // r = b - A u; then the stop-type residual
         residual(A, b, u, r);
         if (stop == HZ_STOP_REL_PRECRES) {
-            amg_precond(H, r, z);
-            dot(m, z, r, sc_zr, H.ws);
+            precond_zr(r, z);
             fetch(H, hs);
             absr = std::sqrt(std::fabs(hs.zr));
             return absr / normr0;
@@ -485,8 +551,9 @@ PcgResult pcg_solve(Hierarchy &H, const DevSELL &A, const double *b, double *u, 
         return (stop == HZ_STOP_MOD_REL_RES) ? absr / normu : absr / normr0;
     };
 
+    const bool speculate = (stop != HZ_STOP_REL_PRECRES);
     residual(A, b, u, r);
-    amg_precond(H, r, z);
+    precond_zr(r, z);
     switch (stop) {
         case HZ_STOP_REL_RES:
             norm2sq(m, r, sc_rr, H.ws);
@@ -517,16 +584,24 @@ PcgResult pcg_solve(Hierarchy &H, const DevSELL &A, const double *b, double *u, 
     }
     if (relres < tol || absres0 < 1e-3 * tol) goto FINISHED;
     itinfo(prt, stop, iter, relres, absres0, 0.0);
-    copy(p, z, m);
-    dot(m, z, r, sc_zr, H.ws);   // temp1 = (z, r) lives in sc->zr until the first update
+    copy(p, z, m);   // (z, r) already sits in sc->zr: it becomes temp1 at the first update
 
+    // The preconditioner and the direction update of an iteration are queued
+    // before the host has read that iteration's scalars (they do not touch u or
+    // r); the rare restart / convergence paths below redo or drop that work.
     while (iter++ < MaxIt) {
         spmv(A, p, t, &H.sc->temp2, H.ws);
         pcg_update(m, H.sc, p, t, u, r, alpha_hist, iter, H.ws);
-        fetch(H, hs);
+        fetch_async(H);
+        if (speculate) {
+            precond_zr(r, z);
+            pcg_direction(m, H.sc, z, p, beta_hist, iter);
+        }
+        fetch_wait(hs);
         const double alpha = hs.alpha;
+        bool redo = false;
         if (stop == HZ_STOP_REL_PRECRES) {
-            amg_precond(H, r, z);
+            amg_precond(H, r, z, nullptr);
             dot(m, z, r, sc_zr, H.ws);
             fetch(H, hs);
             absres = std::sqrt(std::fabs(hs.zr));
@@ -555,6 +630,7 @@ PcgResult pcg_solve(Hierarchy &H, const DevSELL &A, const double *b, double *u, 
                             "dcsr_pcg_w_cond_est", 0LL);
             }
             relres = measure(absres);
+            redo = true;
             if (prt >= 4) std::printf("### HAZMATH WARNING: The actual relative residual = %e!\n", relres);
             if (relres < tol) break;
             if (stag >= 20) {
@@ -573,6 +649,7 @@ PcgResult pcg_solve(Hierarchy &H, const DevSELL &A, const double *b, double *u, 
         if (relres < tol) {
             const double computed = relres;
             relres = measure(absres);
+            redo = true;
             if (relres < tol) break;
             if (prt >= 4) {
                 std::printf("### HAZMATH WARNING: The computed relative residual = %e!\n", computed);
@@ -591,9 +668,14 @@ PcgResult pcg_solve(Hierarchy &H, const DevSELL &A, const double *b, double *u, 
             ++restart_step;
         }
         absres0 = absres;
-        if (stop != HZ_STOP_REL_PRECRES) amg_precond(H, r, z);
-        dot(m, z, r, sc_zr, H.ws);
-        pcg_direction(m, H.sc, z, p, beta_hist, iter);
+        if (!speculate) {
+            dot(m, z, r, sc_zr, H.ws);
+            pcg_direction(m, H.sc, z, p, beta_hist, iter);
+        } else if (redo) {
+            // r was recomputed (and p reset): redo z = B(r), (z, r) and p
+            precond_zr(r, z);
+            pcg_direction(m, H.sc, z, p, beta_hist, iter);
+        }
     }
     {
         if (stag == 1 && restart_step == 1) iter_condest = iter;

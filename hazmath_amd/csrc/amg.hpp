@@ -47,8 +47,9 @@ struct Hierarchy {
 // (-1016 / -1017 / -1013) for the reference's fatal paths.
 int amg_setup(Hierarchy &H, const DevCSR &A0, AMG_param *param, bool keep_for_export);
 void amg_prepare_cycle(Hierarchy &H, const AMG_param *param);
-// z = B(r): cp.maxit cycles from a zero iterate.
-void amg_precond(Hierarchy &H, const double *r, double *z);
+// z = B(r): cp.maxit cycles from a zero iterate.  Writes (z, r) to zr_out and
+// returns true when the last fine sweep produced it.
+bool amg_precond(Hierarchy &H, const double *r, double *z, double *zr_out);
 
 struct PcgResult {
     int iter = 0;

@@ -57,6 +57,9 @@ void residual(const DevSELL &S, const double *b, const double *x, double *w);
 // One smoothing sweep out of place: xout = sweep(x).  smoother: 1 Jacobi (weight w), 10 L1DIAG.
 void smooth_sweep(const DevSELL &S, int smoother, double w, const double *b, const double *x,
                   double *xout);
+// Same sweep, also writing sum_i xout_i * b_i (deterministic) to dot_out.
+void smooth_sweep_dot(const DevSELL &S, int smoother, double w, const double *b, const double *x,
+                      double *xout, double *dot_out, const ReduceWs &ws);
 // First sweep from a zero iterate: x = w*b/d (Jacobi) or b/d_l1 (L1DIAG); d precomputed.
 void smooth_from_zero(int n, int smoother, double w, const double *d, const double *b, double *x);
 // Smoother diagonal: Jacobi uses the last stored diagonal entry of each row
@@ -65,6 +68,15 @@ void smoother_diag(const DevSELL &S, int smoother, double *d);
 // bc = R w (member sums, ascending members);  optional fused zero-iterate sweep of the coarse level.
 void restrict_agg(int nc, const int *rptr, const int *rmem, const double *w, double *bc,
                   int smoother, double wgt, const double *dc, double *xc);
+// Fused residual + restriction (+ coarse zero-iterate sweep when xc != null),
+// one thread per coarse row reading the member rows of the CSR.
+void down_fused(int nc, const int *rptr, const int *rmem, const DevCSR &A, const double *b,
+                const double *x, double *bc, int smoother, double wgt, const double *dc, double *xc);
+// Fused prolongation + one post-smoothing sweep: xout = sweep(x + alpha P xc).
+// dot_out (optional): sum_i xout_i * b_i.
+void up_fused(const DevSELL &S, int smoother, double w, const int *agg, const double *xc,
+              double alpha, const double *alpha_dev, const double *b, const double *x, double *xout,
+              double *dot_out, const ReduceWs &ws);
 // x += alpha * P xc  (aggregate gather-add)
 void prolong_agg(int n, const int *agg, const double *xc, double alpha, const double *alpha_dev,
                  double *x);

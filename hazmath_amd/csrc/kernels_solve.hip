@@ -140,47 +140,120 @@ __global__ __launch_bounds__(kBlock) void k_residual(int n, int nslices, const l
     if (r < n) w[r] = b[r] - acc;
 }
 
-// One sweep out of place.  Jacobi (smoother.c:35-92): t = b - sum_{j!=i} a_ij x_j,
-// d = last diagonal entry; x_i <- (1-w) x_i + w t / d when |d| > SMALLREAL.
+// x_j + alpha * (P xc)_j formed exactly like dcsr_aAxpy_agg (sparse.c:1748-1784):
+// t = 0.0 (+ xc[g] when row j of P is not empty), then x + t (alpha = 1),
+// x - t (alpha = -1) or x + t * alpha.
+__device__ __forceinline__ double prolong_value(double xj, int g, double xcg, double a) {
+    double t = 0.0;
+    if (g > HZ_UNPT) t = t + xcg;
+    if (a == 1.0) return xj + t;
+    if (a == -1.0) return xj - t;
+    return xj + t * a;
+}
+
+// One smoothing sweep out of place, optionally with the prolongation of the
+// level below folded in (every x value read is x + alpha P xc) and optionally
+// with per-block partials of sum_i xout_i b_i.
+// Jacobi (smoother.c:35-92): t = b - sum_{j!=i} a_ij x_j, d = last diagonal
+// entry; x_i <- (1-w) x_i + w t / d when |d| > SMALLREAL.
 // L1DIAG (smoother.c:324-381): t = b - sum_j a_ij x_j, d = sum_j |a_ij|; x_i += t/d.
-template <int kSmoother>
+// Entries are consumed four at a time with all their loads issued first, so a
+// row costs a few dependent memory round trips instead of one per entry.
+template <int kSmoother, bool kProlong, bool kDot>
 __global__ __launch_bounds__(kBlock) void k_sweep(int n, int nslices, const long long *__restrict__ sptr,
                                                   const int *__restrict__ col,
                                                   const double *__restrict__ val, double wgt,
+                                                  const int *__restrict__ agg,
+                                                  const double *__restrict__ xc, double alpha,
+                                                  const double *__restrict__ alpha_dev,
                                                   const double *__restrict__ b,
                                                   const double *__restrict__ x,
-                                                  double *__restrict__ xout) {
-    const int s = xcd_block_id() * kSlicesPerBlock + threadIdx.x / kWave;
+                                                  double *__restrict__ xout,
+                                                  double *__restrict__ partials) {
+    __shared__ double sh[kBlock / kWave];
+    const int blk = xcd_block_id();
+    const int s = blk * kSlicesPerBlock + threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
-    if (s >= nslices) return;
     const int r = s * kSlice + lane;
-    const long long base = sptr[s];
-    const int wd = (int)((sptr[s + 1] - base) / kSlice);
-    const int *c = col + base + lane;
-    const double *v = val + base + lane;
-    double t = (r < n) ? b[r] : 0.0;
-    double d = 0.0;
-    for (int k = 0; k < wd; ++k) {
-        const int cj = c[k * kSlice];
-        const double a = v[k * kSlice];
-        if (kSmoother == HZ_SMOOTHER_L1DIAG) {
-            if (cj >= 0) {
-                t = t - a * x[cj];
-                d = d + fabs(a);
+    const double al = kProlong ? (alpha_dev ? *alpha_dev : alpha) : 1.0;
+    double prod = 0.0;
+    if (s < nslices) {
+        const long long base = sptr[s];
+        const int wd = (int)((sptr[s + 1] - base) / kSlice);
+        const int *c = col + base + lane;
+        const double *v = val + base + lane;
+        double t = (r < n) ? b[r] : 0.0;
+        double d = 0.0;
+        int k = 0;
+        for (; k + 4 <= wd; k += 4) {
+            int cc[4];
+            double vv[4], xx[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                cc[u] = c[(k + u) * kSlice];
+                vv[u] = v[(k + u) * kSlice];
             }
-        } else {
-            if (cj == r) d = a;
-            else if (cj >= 0) t = t - a * x[cj];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) xx[u] = cc[u] >= 0 ? x[cc[u]] : 0.0;
+            if (kProlong) {
+                int gg[4];
+                double xg[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) gg[u] = cc[u] >= 0 ? agg[cc[u]] : HZ_UNPT;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) xg[u] = gg[u] > HZ_UNPT ? xc[gg[u]] : 0.0;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) xx[u] = prolong_value(xx[u], gg[u], xg[u], al);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (kSmoother == HZ_SMOOTHER_L1DIAG) {
+                    if (cc[u] >= 0) {
+                        t = t - vv[u] * xx[u];
+                        d = d + fabs(vv[u]);
+                    }
+                } else {
+                    if (cc[u] == r) d = vv[u];
+                    else if (cc[u] >= 0) t = t - vv[u] * xx[u];
+                }
+            }
+        }
+        for (; k < wd; ++k) {
+            const int cj = c[k * kSlice];
+            const double a = v[k * kSlice];
+            double xj = cj >= 0 ? x[cj] : 0.0;
+            if (kProlong && cj >= 0) {
+                const int g = agg[cj];
+                xj = prolong_value(xj, g, g > HZ_UNPT ? xc[g] : 0.0, al);
+            }
+            if (kSmoother == HZ_SMOOTHER_L1DIAG) {
+                if (cj >= 0) {
+                    t = t - a * xj;
+                    d = d + fabs(a);
+                }
+            } else {
+                if (cj == r) d = a;
+                else if (cj >= 0) t = t - a * xj;
+            }
+        }
+        if (r < n) {
+            double xi = x[r];
+            if (kProlong) {
+                const int g = agg[r];
+                xi = prolong_value(xi, g, g > HZ_UNPT ? xc[g] : 0.0, al);
+            }
+            double xn = xi;
+            if (fabs(d) > kSmallReal) {
+                if (kSmoother == HZ_SMOOTHER_L1DIAG) xn = xi + t / d;
+                else xn = (1 - wgt) * xi + wgt * t / d;
+            }
+            xout[r] = xn;
+            if (kDot) prod = xn * b[r];
         }
     }
-    if (r < n) {
-        const double xi = x[r];
-        double xn = xi;
-        if (fabs(d) > kSmallReal) {
-            if (kSmoother == HZ_SMOOTHER_L1DIAG) xn = xi + t / d;
-            else xn = (1 - wgt) * xi + wgt * t / d;
-        }
-        xout[r] = xn;
+    if (kDot) {
+        const double bs = block_sum<kBlock>(prod, sh);
+        if (threadIdx.x == 0) partials[blk] = bs;
     }
 }
 
@@ -230,6 +303,52 @@ __global__ void k_restrict(int nc, const int *__restrict__ rptr, const int *__re
         double s = 0.0;
         const int e = rptr[c + 1];
         for (int q = rptr[c]; q < e; ++q) s = s + w[rmem[q]];
+        bc[c] = s;
+        if (xc) {
+            const double di = dc[c];
+            double xi = 0.0;
+            if (fabs(di) > kSmallReal) {
+                if (smoother == HZ_SMOOTHER_L1DIAG) xi = 0.0 + s / di;
+                else xi = (1 - wgt) * 0.0 + wgt * s / di;
+            }
+            xc[c] = xi;
+        }
+    }
+}
+
+// Residual, restriction and the coarse level's first (zero-iterate) sweep in
+// one pass, one thread per coarse row: w_m = b_m - A_m x for each member m
+// (row sum from 0.0 in stored order), bc = 0.0 + w_m0 + w_m1 + ... in member
+// order, then xc from bc.  Same operations and order as residual + restrict.
+__global__ void k_down_fused(int nc, const int *__restrict__ rptr, const int *__restrict__ rmem,
+                             const int *__restrict__ ia, const int *__restrict__ ja,
+                             const double *__restrict__ a, const double *__restrict__ b,
+                             const double *__restrict__ x, double *__restrict__ bc, int smoother,
+                             double wgt, const double *__restrict__ dc, double *__restrict__ xc) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+        double s = 0.0;
+        const int e = rptr[c + 1];
+        for (int q = rptr[c]; q < e; ++q) {
+            const int m = rmem[q];
+            const int k1 = ia[m + 1];
+            double t = 0.0;
+            int k = ia[m];
+            for (; k + 4 <= k1; k += 4) {
+                int jj[4];
+                double aa[4], xx[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    jj[u] = ja[k + u];
+                    aa[u] = a[k + u];
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) xx[u] = x[jj[u]];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) t = t + aa[u] * xx[u];
+            }
+            for (; k < k1; ++k) t = t + a[k] * x[ja[k]];
+            s = s + (b[m] - t);
+        }
         bc[c] = s;
         if (xc) {
             const double di = dc[c];
@@ -471,16 +590,33 @@ void residual(const DevSELL &S, const double *b, const double *x, double *w) {
     HZ_LAUNCH_CHECK();
 }
 
+namespace {
+template <bool kProlong, bool kDot>
+void launch_sweep(const DevSELL &S, int smoother, double w, const int *agg, const double *xc,
+                  double alpha, const double *alpha_dev, const double *b, const double *x, double *xout,
+                  double *partials) {
+    const int grid = div_up(S.nslices, kSlicesPerBlock);
+    if (smoother == HZ_SMOOTHER_L1DIAG)
+        k_sweep<HZ_SMOOTHER_L1DIAG, kProlong, kDot><<<grid, kBlock, 0, stream()>>>(
+            S.n, S.nslices, S.sptr, S.col, S.val, w, agg, xc, alpha, alpha_dev, b, x, xout, partials);
+    else
+        k_sweep<HZ_SMOOTHER_JACOBI, kProlong, kDot><<<grid, kBlock, 0, stream()>>>(
+            S.n, S.nslices, S.sptr, S.col, S.val, w, agg, xc, alpha, alpha_dev, b, x, xout, partials);
+    HZ_LAUNCH_CHECK();
+}
+}  // namespace
+
 void smooth_sweep(const DevSELL &S, int smoother, double w, const double *b, const double *x,
                   double *xout) {
     if (S.n == 0) return;
-    const int grid = div_up(S.nslices, kSlicesPerBlock);
-    if (smoother == HZ_SMOOTHER_L1DIAG)
-        k_sweep<HZ_SMOOTHER_L1DIAG><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col,
-                                                                   S.val, w, b, x, xout);
-    else
-        k_sweep<HZ_SMOOTHER_JACOBI><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col,
-                                                                   S.val, w, b, x, xout);
+    launch_sweep<false, false>(S, smoother, w, nullptr, nullptr, 1.0, nullptr, b, x, xout, nullptr);
+}
+
+void smooth_sweep_dot(const DevSELL &S, int smoother, double w, const double *b, const double *x,
+                      double *xout, double *dot_out, const ReduceWs &ws) {
+    if (S.n == 0) return;
+    launch_sweep<false, true>(S, smoother, w, nullptr, nullptr, 1.0, nullptr, b, x, xout, ws.partials);
+    k_final_sum<<<1, kFinBlock, 0, stream()>>>(ws.partials, div_up(S.nslices, kSlicesPerBlock), dot_out);
     HZ_LAUNCH_CHECK();
 }
 
@@ -507,6 +643,27 @@ void restrict_agg(int nc, const int *rptr, const int *rmem, const double *w, dou
     if (nc == 0) return;
     k_restrict<<<stream_grid(nc), kBlock, 0, stream()>>>(nc, rptr, rmem, w, bc, smoother, wgt, dc, xc);
     HZ_LAUNCH_CHECK();
+}
+
+void down_fused(int nc, const int *rptr, const int *rmem, const DevCSR &A, const double *b,
+                const double *x, double *bc, int smoother, double wgt, const double *dc, double *xc) {
+    if (nc == 0) return;
+    k_down_fused<<<stream_grid(nc), kBlock, 0, stream()>>>(nc, rptr, rmem, A.ia, A.ja, A.val, b, x, bc,
+                                                           smoother, wgt, dc, xc);
+    HZ_LAUNCH_CHECK();
+}
+
+void up_fused(const DevSELL &S, int smoother, double w, const int *agg, const double *xc,
+              double alpha, const double *alpha_dev, const double *b, const double *x, double *xout,
+              double *dot_out, const ReduceWs &ws) {
+    if (S.n == 0) return;
+    if (dot_out) {
+        launch_sweep<true, true>(S, smoother, w, agg, xc, alpha, alpha_dev, b, x, xout, ws.partials);
+        k_final_sum<<<1, kFinBlock, 0, stream()>>>(ws.partials, div_up(S.nslices, kSlicesPerBlock), dot_out);
+        HZ_LAUNCH_CHECK();
+    } else {
+        launch_sweep<true, false>(S, smoother, w, agg, xc, alpha, alpha_dev, b, x, xout, nullptr);
+    }
 }
 
 void prolong_agg(int n, const int *agg, const double *xc, double alpha, const double *alpha_dev,

@@ -197,3 +197,91 @@ def test_pcg_parity_kuhn3d(lib, orc, libc, m):
     (ig, xg), (io, xo) = res["gpu"], res["oracle"]
     assert abs(ig - io) <= 1
     assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) < 1e-10
+
+
+@pytest.mark.parametrize("variant", [
+    dict(cycle=capi.W_CYCLE),
+    dict(coarse_scaling=1),
+    dict(stop_type=capi.STOP_REL_PRECRES),
+    dict(stop_type=capi.STOP_MOD_REL_RES),
+    dict(amg_maxit=2),
+    dict(aggregation=capi.VMB, cycle=capi.W_CYCLE, coarse_solver=0),
+])
+def test_pcg_parity_cycle_variants(lib, orc, libc, solvers_fixture, variant):
+    ia, ja, va, b = solvers_fixture
+    A = capi.HostCSR(ia, ja, va)
+    kw = dict(tol=1e-10, maxit=400, aggregation=capi.HEC, smoother=capi.SMOOTHER_L1DIAG)
+    kw.update(variant)
+    res = _solve_both(lib, orc, libc, A, b, **kw)
+    (ig, xg), (io, xo) = res["gpu"], res["oracle"]
+    assert ig >= 0 and abs(ig - io) <= 1
+    assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) < 1e-10
+
+
+@pytest.mark.parametrize("sweeps", [(2, 2), (0, 1), (1, 0)])
+def test_pcg_parity_sweep_counts(lib, orc, libc, sweeps):
+    import oracle
+    ia, ja, va = problems.poisson2d_p1(48)
+    b = problems.uniform_rhs(ia.size - 1, seed=11)
+    A = capi.HostCSR(ia, ja, va)
+    res = {}
+    for which in ("gpu", "oracle"):
+        x = capi.HostVec(np.zeros(b.size))
+        bb = capi.HostVec(b)
+        libc.srand(1)
+        if which == "gpu":
+            it, amg = capi.solver_params(lib, tol=1e-10, maxit=400, aggregation=capi.HEM,
+                                         smoother=capi.SMOOTHER_L1DIAG)
+        else:
+            it, amg = oracle.params(tol=1e-10, maxit=400, aggregation=capi.HEM, smoother=10)
+        amg.presmooth_iter, amg.postsmooth_iter = sweeps
+        if which == "gpu":
+            r = lib.linear_solver_dcsr_krylov_amg(A.ptr(), bb.ptr(), x.ptr(), C.byref(it), C.byref(amg))
+        else:
+            tr = capi.orc_trace()
+            r = orc.orc_krylov_amg(A.ptr(), bb.ptr(), x.ptr(), C.byref(it), C.byref(amg), C.byref(tr))
+        res[which] = (r, x.arr.copy())
+    (ig, xg), (io, xo) = res["gpu"], res["oracle"]
+    assert abs(ig - io) <= 1
+    if io > 0:
+        assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) < 1e-10
+
+
+def test_jacobi_weight_extension(lib, orc, libc):
+    """Damped Jacobi (w = 2/3) through hz_options against the oracle with the
+    same weight."""
+    import oracle
+    ia, ja, va = problems.poisson2d_p1(64)
+    b = problems.uniform_rhs(ia.size - 1, seed=2)
+    A = capi.HostCSR(ia, ja, va)
+    opts = capi.hz_options()
+    lib.hz_get_options(C.byref(opts))
+    opts.jacobi_weight = 2.0 / 3.0
+    lib.hz_set_options(C.byref(opts))
+    orc.orc_set_jacobi_weight(2.0 / 3.0)
+    try:
+        res = _solve_both(lib, orc, libc, A, b, tol=1e-10, maxit=400, aggregation=capi.HEM)
+    finally:
+        opts.jacobi_weight = 1.0
+        lib.hz_set_options(C.byref(opts))
+        orc.orc_set_jacobi_weight(1.0)
+    (ig, xg), (io, xo) = res["gpu"], res["oracle"]
+    assert ig > 0 and abs(ig - io) <= 1
+    assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) < 1e-10
+
+
+def test_solution_quality_kuhn3d_64(lib):
+    """Size-independent check at a size the oracle does not run in the test
+    budget: the true FP64 residual of the returned solution meets the tolerance."""
+    import scipy.sparse as sp
+    ia, ja, va = problems.poisson3d_kuhn(64)
+    b = problems.uniform_rhs(ia.size - 1, seed=5)
+    A = capi.HostCSR(ia, ja, va)
+    x = capi.HostVec(np.zeros(b.size))
+    bb = capi.HostVec(b)
+    it, amg = capi.solver_params(lib, tol=1e-10, maxit=400, aggregation=capi.HEM,
+                                 smoother=capi.SMOOTHER_L1DIAG, coarse_dof=200)
+    r = lib.linear_solver_dcsr_krylov_amg(A.ptr(), bb.ptr(), x.ptr(), C.byref(it), C.byref(amg))
+    assert r > 0
+    Asp = sp.csr_matrix((va, ja, ia))
+    assert np.linalg.norm(b - Asp @ x.arr) / np.linalg.norm(b) < 1e-10
