@@ -239,6 +239,9 @@ def _declare(lib):
     lib.hz_smoother_sweep.restype = INT
     lib.hz_rap_agg.argtypes = [P(dCSRmat), P(INT), INT, P(dCSRmat)]
     lib.hz_rap_agg.restype = INT
+    lib.hz_dense_inverse.argtypes = [INT, P(REAL)]
+    lib.hz_dense_inverse.restype = INT
+    lib.hz_dense_mfma_active.restype = INT
     lib.hz_time_fine_kernels.argtypes = [INT, INT, C.c_void_p, C.c_void_p, C.c_void_p, INT,
                                          P(hz_kernel_times)]
     lib.hz_time_fine_kernels.restype = INT

@@ -84,23 +84,6 @@ hipStream_t stream() {
 }
 
 // ---------------------------------------------------------------------------
-// HEC turn order: iarray_shuffle (src/utilities/array.c:302-318) with libc rand()
-// ---------------------------------------------------------------------------
-void hec_turn_order(int n, int *perm_dev, int) {
-    std::vector<int> perm(n);
-    for (int i = 0; i < n; ++i) perm[i] = i;
-    if (n > 1) {
-        for (int i = 0; i < n - 1; ++i) {
-            const int j = i + rand() / (RAND_MAX / (n - i) + 1);
-            std::swap(perm[i], perm[j]);
-        }
-    }
-    HZ_CHECK(hipMemcpyAsync(perm_dev, perm.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice,
-                            stream()));
-    HZ_CHECK(hipStreamSynchronize(stream()));
-}
-
-// ---------------------------------------------------------------------------
 // Setup
 // ---------------------------------------------------------------------------
 int amg_setup(Hierarchy &H, const DevCSR &A0, AMG_param *param, bool) {
@@ -132,7 +115,7 @@ int amg_setup(Hierarchy &H, const DevCSR &A0, AMG_param *param, bool) {
         int nagg = 0, st = HZ_SUCCESS;
         if (type == HZ_HEC) {
             int *perm = arena().alloc_n<int>((size_t)n + 1);
-            hec_turn_order(n, perm, 0);
+            hec_turn_order(n, perm, H.hec_rng);
             const int r = hec_aggregate(N, perm, L.agg);
             if (r == -1016) {
                 std::fprintf(stderr, "\n\n%%%% ERROR Stop: negative index encountered in %s()\n\n",

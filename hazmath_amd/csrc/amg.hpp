@@ -40,6 +40,7 @@ struct Hierarchy {
     PcgScalars *sc = nullptr;
     double *alpha_cs = nullptr;   // coarse scaling factor (device)
     double *x0buf = nullptr;      // level-0 ping-pong partner of the preconditioner output
+    int hec_rng = 0;              // 0: GPU stream from the libc state (checked), 1: host rand()
 };
 
 // Runs the reference's setup loop on the device.  A0 arrays are borrowed.

@@ -101,6 +101,7 @@ INT core_solve(const DevCSR &A, const double *db, double *dx, linear_itsolver_pa
     if (st < 0) return st;
     Hierarchy H;
     H.cp.jacobi_w = g_opt.jacobi_weight;
+    H.hec_rng = g_opt.hec_rng_source;
     st = amg_setup(H, A, amg, false);
     if (st <= -1000) return fatal(st);
     if (st < 0) return st;
@@ -374,6 +375,7 @@ INT hz_setup_hierarchy(dCSRmat *A, AMG_param *amgparam, hz_hierarchy *out) {
         arena().reset();
         const DevCSR dA = upload_csr(A);
         Hierarchy H;
+        H.hec_rng = g_opt.hec_rng_source;
         int st = amg_setup(H, dA, amgparam, true);
         if (st <= -1000) return fatal(st);
         if (st < 0) return st;
@@ -527,6 +529,31 @@ INT hz_rap_agg(dCSRmat *A, const INT *aggregates, INT num_aggregates, dCSRmat *A
         return HZ_SUCCESS;
     } catch (const DeviceError &) {
         return HZ_ERROR_DEVICE;
+    }
+}
+
+INT hz_dense_inverse(INT n, REAL *M) {
+    if (n < 0) return HZ_ERROR_INPUT_PAR;
+    if (n == 0) return HZ_SUCCESS;
+    if (!device_ready()) return HZ_ERROR_DEVICE;
+    try {
+        arena().reset();
+        double *dM = upload(M, (size_t)n * n);
+        const int st = dense_inverse_inplace(n, dM);
+        if (st != 0) return HZ_ERROR_SOLVER_MISC;
+        HZ_CHECK(memcpy_sync(M, dM, sizeof(double) * (size_t)n * n, hipMemcpyDeviceToHost));
+        return HZ_SUCCESS;
+    } catch (const DeviceError &) {
+        return HZ_ERROR_DEVICE;
+    }
+}
+
+INT hz_dense_mfma_active(void) {
+    if (!device_ready()) return 0;
+    try {
+        return dense_mfma_active() ? 1 : 0;
+    } catch (const DeviceError &) {
+        return 0;
     }
 }
 

@@ -116,6 +116,10 @@ void csr_transpose(const DevCSR &A, DevCSR &AT);
 int coarse_dense_setup(const DevCSR &A, double *Ainv);
 // x = Ainv b
 void coarse_dense_solve(int n, const double *Ainv, const double *b, double *x);
+// In-place inverse of a dense row-major n x n device matrix (block Gauss-Jordan,
+// rank-64 updates on the FP64 matrix cores).  Returns nonzero on a zero pivot.
+int dense_inverse_inplace(int n, double *M);
+bool dense_mfma_active();
 // Single-workgroup CG for coarse_solver = 0 (coarse_itsolver, mgcycle.c:34-53).
 void coarse_cg(const DevSELL &S, const double *b, double *x, double tol, int maxit, double *work);
 

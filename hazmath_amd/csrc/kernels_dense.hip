@@ -10,6 +10,10 @@
 // coarse_solver = 0: unpreconditioned CG to the cycle tolerance
 // (coarse_itsolver, src/solver/mgcycle.c:34-53 -> dcsr_pcg, krylov.c:122-372),
 // run inside one workgroup so the whole coarse solve is a single launch.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
 #include "hz_internal.h"
 
 namespace hz {
@@ -74,6 +78,119 @@ __global__ __launch_bounds__(kDenseBlock) void k_ldlt_inverse(int n, double *__r
         double s = 0.0;
         for (int k = k0; k < n; ++k) s += X[(size_t)k * n + i] * X[(size_t)k * n + jj] / M[(size_t)k * n + k];
         Ainv[t] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Blocked Gauss-Jordan inversion for larger coarsest levels.  Per 64-column
+// block K (P = A_KK):  R = P^{-1} A_K,: ;  A_i,: -= A_i,K R (i not in K);
+// A_i,K = -A_i,K P^{-1};  A_K,: = R with A_KK = P^{-1}.  The rank-64 updates
+// are C = beta C + alpha A B products on the FP64 matrix cores
+// (v_mfma_f64_16x16x4f64): a 256-thread workgroup owns a 64x64 tile of C, each
+// wavefront a 32x32 quarter as 2x2 MFMA 16x16 blocks, operands staged in LDS.
+// ---------------------------------------------------------------------------
+typedef double d4 __attribute__((ext_vector_type(4)));
+constexpr int kTile = 64;
+
+template <bool kMfma>
+__global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const double *__restrict__ A, int lda,
+                                              const double *__restrict__ B, int ldb, double *__restrict__ C,
+                                              int ldc, double alpha, double beta) {
+    __shared__ double As[kTile][kTile + 1];
+    __shared__ double Bs[kTile][kTile + 1];
+    const int tm = blockIdx.y * kTile, tn = blockIdx.x * kTile;
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < kTile * kTile; idx += 256) {
+        const int r = idx / kTile, k = idx % kTile;
+        As[r][k] = (tm + r < M && k < K) ? A[(size_t)(tm + r) * lda + k] : 0.0;
+        const int kk = idx / kTile, c = idx % kTile;
+        Bs[kk][c] = (kk < K && tn + c < N) ? B[(size_t)kk * ldb + tn + c] : 0.0;
+    }
+    __syncthreads();
+    const int w = tid / kWave, lane = tid % kWave;
+    const int wm = (w / 2) * 32, wn = (w % 2) * 32;
+    if (kMfma) {
+        d4 acc[2][2];
+        for (int mi = 0; mi < 2; ++mi)
+            for (int ni = 0; ni < 2; ++ni) acc[mi][ni] = d4{0.0, 0.0, 0.0, 0.0};
+        // A operand: lane holds A[row = lane % 16][k = lane / 16]; B operand:
+        // B[k = lane / 16][col = lane % 16]; accumulator element r of lane holds
+        // D[row = lane / 16 + 4 r][col = lane % 16].
+        for (int ks = 0; ks < kTile; ks += 4) {
+            const int kq = ks + lane / 16;
+            const double a0 = As[wm + 0 + lane % 16][kq], a1 = As[wm + 16 + lane % 16][kq];
+            const double b0 = Bs[kq][wn + 0 + lane % 16], b1 = Bs[kq][wn + 16 + lane % 16];
+            acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+        }
+        for (int mi = 0; mi < 2; ++mi)
+            for (int ni = 0; ni < 2; ++ni)
+                for (int r = 0; r < 4; ++r) {
+                    const int row = tm + wm + mi * 16 + lane / 16 + 4 * r;
+                    const int col = tn + wn + ni * 16 + lane % 16;
+                    if (row < M && col < N) {
+                        double *c = &C[(size_t)row * ldc + col];
+                        *c = (beta == 0.0 ? 0.0 : beta * *c) + alpha * acc[mi][ni][r];
+                    }
+                }
+    } else {
+        // plain FMA path: each thread 16 outputs of the tile
+        for (int e = tid; e < kTile * kTile; e += 256) {
+            const int r = e / kTile, cidx = e % kTile;
+            double s = 0.0;
+            for (int k = 0; k < kTile; ++k) s += As[r][k] * Bs[k][cidx];
+            const int row = tm + r, col = tn + cidx;
+            if (row < M && col < N) {
+                double *c = &C[(size_t)row * ldc + col];
+                *c = (beta == 0.0 ? 0.0 : beta * *c) + alpha * s;
+            }
+        }
+    }
+}
+
+// In-place Gauss-Jordan inverse of the b x b block at A[k0][k0] (ld n) into P.
+__global__ __launch_bounds__(256) void k_block_inverse(int n, int k0, int b, const double *__restrict__ A,
+                                                       double *__restrict__ P, int *__restrict__ status) {
+    __shared__ double s[kTile][kTile + 1];
+    __shared__ double colk[kTile];
+    const int tid = threadIdx.x;
+    for (int idx = tid; idx < b * b; idx += 256) s[idx / b][idx % b] = A[(size_t)(k0 + idx / b) * n + k0 + idx % b];
+    __syncthreads();
+    for (int k = 0; k < b; ++k) {
+        const double piv = s[k][k];
+        if (piv == 0.0) {
+            if (tid == 0) status[0] = 1;
+            return;
+        }
+        for (int i = tid; i < b; i += 256) colk[i] = s[i][k];
+        __syncthreads();
+        for (int j = tid; j < b; j += 256)
+            if (j != k) s[k][j] = s[k][j] / piv;
+        __syncthreads();
+        if (tid == 0) s[k][k] = 1.0 / piv;
+        __syncthreads();
+        for (int idx = tid; idx < b * b; idx += 256) {
+            const int i = idx / b, j = idx % b;
+            if (i == k) continue;
+            if (j != k) s[i][j] -= colk[i] * s[k][j];
+        }
+        __syncthreads();
+        for (int i = tid; i < b; i += 256)
+            if (i != k) s[i][k] = -colk[i] * s[k][k];
+        __syncthreads();
+    }
+    for (int idx = tid; idx < b * b; idx += 256) P[idx] = s[idx / b][idx % b];
+}
+
+// Copy a rows x cols block between row-major buffers.
+__global__ void k_copy_block(int rows, int cols, const double *__restrict__ src, int lds,
+                             double *__restrict__ dst, int ldd, double scale) {
+    const long total = (long)rows * cols;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+        const int r = (int)(e / cols), c = (int)(e % cols);
+        dst[(size_t)r * ldd + c] = scale * src[(size_t)r * lds + c];
     }
 }
 
@@ -228,9 +345,96 @@ __global__ __launch_bounds__(kCgBlock) void k_coarse_cg(int n, int nslices, cons
 
 }  // namespace
 
+namespace {
+
+void launch_gemm(bool mfma, int M, int N, int K, const double *A, int lda, const double *B, int ldb, double *C,
+                 int ldc, double alpha, double beta) {
+    if (M <= 0 || N <= 0) return;
+    dim3 grid(div_up(N, kTile), div_up(M, kTile));
+    if (mfma)
+        k_gemm<true><<<grid, 256, 0, stream()>>>(M, N, K, A, lda, B, ldb, C, ldc, alpha, beta);
+    else
+        k_gemm<false><<<grid, 256, 0, stream()>>>(M, N, K, A, lda, B, ldb, C, ldc, alpha, beta);
+    HZ_LAUNCH_CHECK();
+}
+
+// The MFMA operand/accumulator lane layout is checked once against the FMA
+// kernel on a 64x64x64 product; a mismatch keeps the FMA kernel.
+bool mfma_layout_ok() {
+    static int ok = -1;
+    if (ok >= 0) return ok == 1;
+    const int n = kTile;
+    std::vector<double> ha(n * n), hb(n * n);
+    for (int i = 0; i < n * n; ++i) {
+        ha[i] = std::sin(0.37 * i + 0.1) + (i % 7) * 0.125;
+        hb[i] = std::cos(0.23 * i - 0.4) - (i % 5) * 0.25;
+    }
+    double *d = arena().alloc_n<double>(4 * (size_t)n * n);
+    HZ_CHECK(memcpy_sync(d, ha.data(), sizeof(double) * n * n, hipMemcpyHostToDevice));
+    HZ_CHECK(memcpy_sync(d + n * n, hb.data(), sizeof(double) * n * n, hipMemcpyHostToDevice));
+    launch_gemm(true, n, n, n, d, n, d + n * n, n, d + 2 * n * n, n, 1.0, 0.0);
+    launch_gemm(false, n, n, n, d, n, d + n * n, n, d + 3 * n * n, n, 1.0, 0.0);
+    std::vector<double> r1(n * n), r2(n * n);
+    HZ_CHECK(memcpy_sync(r1.data(), d + 2 * n * n, sizeof(double) * n * n, hipMemcpyDeviceToHost));
+    HZ_CHECK(memcpy_sync(r2.data(), d + 3 * n * n, sizeof(double) * n * n, hipMemcpyDeviceToHost));
+    double err = 0.0, mag = 0.0;
+    for (int i = 0; i < n * n; ++i) {
+        err = std::max(err, std::fabs(r1[i] - r2[i]));
+        mag = std::max(mag, std::fabs(r2[i]));
+    }
+    ok = (err <= 1e-12 * mag) ? 1 : 0;
+    if (!ok) std::fprintf(stderr, "hazmath_mi355x: MFMA layout check failed (%g); using FMA GEMM\n", err);
+    return ok == 1;
+}
+
+// In-place inverse of the dense n x n matrix M by 64-column block Gauss-Jordan.
+int blocked_inverse(int n, double *M) {
+    const bool mfma = mfma_layout_ok();
+    double *Pinv = arena().alloc_n<double>((size_t)kTile * kTile);
+    double *R = arena().alloc_n<double>((size_t)kTile * n);
+    double *Cb = arena().alloc_n<double>((size_t)n * kTile);
+    int *status = arena().alloc_n<int>(1);
+    HZ_CHECK(hipMemsetAsync(status, 0, sizeof(int), stream()));
+    const int g = std::min(4096, div_up((long)n * kTile, 256));
+    for (int k0 = 0; k0 < n; k0 += kTile) {
+        const int b = std::min(kTile, n - k0);
+        const int below = n - k0 - b;
+        k_block_inverse<<<1, 256, 0, stream()>>>(n, k0, b, M, Pinv, status);
+        HZ_LAUNCH_CHECK();
+        launch_gemm(mfma, b, n, b, Pinv, b, M + (size_t)k0 * n, n, R, n, 1.0, 0.0);           // R = P^-1 A_K,:
+        k_copy_block<<<g, 256, 0, stream()>>>(n, b, M + k0, n, Cb, b, 1.0);                   // Cb = A_:,K
+        HZ_LAUNCH_CHECK();
+        launch_gemm(mfma, k0, n, b, Cb, b, R, n, M, n, -1.0, 1.0);                            // rows above K
+        launch_gemm(mfma, below, n, b, Cb + (size_t)(k0 + b) * b, b, R, n, M + (size_t)(k0 + b) * n, n, -1.0, 1.0);
+        launch_gemm(mfma, k0, b, b, Cb, b, Pinv, b, M + k0, n, -1.0, 0.0);                    // A_i,K = -A_i,K P^-1
+        launch_gemm(mfma, below, b, b, Cb + (size_t)(k0 + b) * b, b, Pinv, b, M + (size_t)(k0 + b) * n + k0, n,
+                    -1.0, 0.0);
+        k_copy_block<<<g, 256, 0, stream()>>>(b, n, R, n, M + (size_t)k0 * n, n, 1.0);        // A_K,: = R
+        k_copy_block<<<1, 256, 0, stream()>>>(b, b, Pinv, b, M + (size_t)k0 * n + k0, n, 1.0); // A_KK = P^-1
+        HZ_LAUNCH_CHECK();
+    }
+    int hs = 0;
+    HZ_CHECK(memcpy_sync(&hs, status, sizeof(int), hipMemcpyDeviceToHost));
+    return hs;
+}
+
+constexpr int kSmallDense = 256;   // single-workgroup L D L^T below this size
+
+}  // namespace
+
+int dense_inverse_inplace(int n, double *M) { return blocked_inverse(n, M); }
+
+bool dense_mfma_active() { return mfma_layout_ok(); }
+
 int coarse_dense_setup(const DevCSR &A, double *Ainv) {
     const int n = A.n;
     hipStream_t st = stream();
+    if (n > kSmallDense) {
+        HZ_CHECK(hipMemsetAsync(Ainv, 0, sizeof(double) * (size_t)n * n, st));
+        k_dense_from_csr<<<div_up(n, 256), 256, 0, st>>>(n, A.ia, A.ja, A.val, Ainv);
+        HZ_LAUNCH_CHECK();
+        return blocked_inverse(n, Ainv);
+    }
     double *M = arena().alloc_n<double>((size_t)n * n + 1);
     double *X = arena().alloc_n<double>((size_t)n * n + 1);
     int *status = arena().alloc_n<int>(1);

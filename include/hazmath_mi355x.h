@@ -188,9 +188,12 @@ dvector *dvector_read_p(FILE *fp);
 typedef struct {
     REAL jacobi_weight;    /* damping w of SMOOTHER_JACOBI; the reference fixes w=1.0
                               (src/solver/smoother.c:47).  1.0 = reference behaviour. */
-    INT  hec_rng_source;   /* 0: libc rand() drawn on the host (reference order);
-                              1: glibc-compatible stream generated on the GPU from the
-                                 current libc state (same values; state advanced after). */
+    INT  hec_rng_source;   /* HEC turn order (iarray_shuffle with libc rand()):
+                              0: the caller's glibc random() state is read, its stream is
+                                 generated and the shuffle run on the GPU, and the libc state
+                                 is advanced exactly as rand() would have been (checked; falls
+                                 back to 1 with any other libc);
+                              1: rand() called on the host, the reference's own loop. */
     INT  verbose_timing;   /* >0: print setup/solve wall times from hipEvents */
     INT  return_on_fatal;  /* 0: exit() like the reference on its fatal paths
                               (exit(16)/exit(17)/check_error); 1: return the negative
@@ -250,6 +253,14 @@ INT hz_smoother_sweep(dCSRmat *A, const REAL *b, REAL *x, INT smoother, REAL w);
 /* Coarse product dcsr_rap_agg(R,A,P) with R=P^T built from aggregates
  * (sparse.c:2557-2683, sparse.c:770-830). Result arrays are malloc'd. */
 INT hz_rap_agg(dCSRmat *A, const INT *aggregates, INT num_aggregates, dCSRmat *Ac);
+/* In-place inverse of a dense row-major n x n host matrix with the kernels that
+ * build the coarsest-level inverse of coarse_solver 32 (stands in for the
+ * U^T D U factorization, hazmath_direct.c:633-854).  Returns
+ * HZ_ERROR_SOLVER_MISC on a zero pivot. */
+INT hz_dense_inverse(INT n, REAL *M);
+/* 1 when the dense inverse runs its rank-64 updates on the FP64 matrix cores
+ * (the one-time lane-layout self-check passed), 0 when it uses the FMA kernel. */
+INT hz_dense_mfma_active(void);
 
 /* Average device duration of the fine-level solve kernels on the library's
  * stream (hipEvents around `reps` back-to-back launches), for the roofline.

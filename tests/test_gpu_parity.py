@@ -285,3 +285,70 @@ def test_solution_quality_kuhn3d_64(lib):
     assert r > 0
     Asp = sp.csr_matrix((va, ja, ia))
     assert np.linalg.norm(b - Asp @ x.arr) / np.linalg.norm(b) < 1e-10
+
+
+def test_hec_turn_order_host_rand_matches_gpu_stream(lib, orc, libc):
+    """hec_rng_source = 1 (rand() called on the host, the reference loop) and the
+    default GPU generation of the glibc stream give the same hierarchy and leave
+    libc's state in the same place."""
+    ia, ja, va = problems.poisson3d_kuhn(20)
+    A = capi.HostCSR(ia, ja, va)
+    opts = capi.hz_options()
+    lib.hz_get_options(C.byref(opts))
+    saved = opts.hec_rng_source
+    per_source = {}
+    try:
+        for src in (0, 1):
+            opts.hec_rng_source = src
+            lib.hz_set_options(C.byref(opts))
+            (lg, _, rg), (lo, _, ro) = _hierarchies(lib, orc, libc, A, capi.HEC, 0.0, coarse_dof=40)
+            assert rg == ro
+            assert len(lg) == len(lo)
+            for g, o in zip(lg, lo):
+                if "agg" in o:
+                    assert np.array_equal(g["agg"], o["agg"])
+            per_source[src] = (lg, rg)
+    finally:
+        opts.hec_rng_source = saved
+        lib.hz_set_options(C.byref(opts))
+    assert per_source[0][1] == per_source[1][1]
+
+
+def test_dense_inverse_uses_matrix_cores(lib):
+    """The FP64 MFMA lane layout assumed by the GEMM passes its self-check."""
+    assert lib.hz_dense_mfma_active() == 1
+
+
+@pytest.mark.parametrize("n", [1, 17, 64, 65, 200, 257, 640, 1000])
+def test_dense_inverse_blocked(lib, n):
+    """Block Gauss-Jordan inverse (FP64 matrix cores) of an SPD matrix against
+    LAPACK: A X = I and X equal to numpy's inverse to 1e-12 relative."""
+    rng = np.random.default_rng(n)
+    G = rng.standard_normal((n, n))
+    A = G @ G.T / n + np.eye(n)
+    X = np.ascontiguousarray(A.copy())
+    assert lib.hz_dense_inverse(n, capi.dptr(X)) == 0
+    Xr = np.linalg.inv(A)
+    assert np.linalg.norm(X - Xr) / np.linalg.norm(Xr) < 1e-12
+    assert np.linalg.norm(A @ X - np.eye(n)) < 1e-11 * n
+
+
+def test_dense_inverse_zero_pivot_reports_error(lib):
+    A = np.ones((70, 70))   # first pivot block singular after one step
+    assert lib.hz_dense_inverse(70, capi.dptr(A)) == -47
+
+
+def test_pcg_parity_large_coarsest(lib, orc, libc):
+    """A coarsest level above 256 rows takes the blocked inverse; iteration count
+    and solution match the oracle's dense LDL^T coarse solve."""
+    ia, ja, va = problems.poisson2d_p1(64)
+    b = problems.uniform_rhs(ia.size - 1, seed=4)
+    A = capi.HostCSR(ia, ja, va)
+    res = _solve_both(lib, orc, libc, A, b, tol=1e-10, maxit=400, aggregation=capi.HEM,
+                      coarse_dof=1000)
+    info = capi.hz_solve_info()
+    lib.hz_get_last_info(C.byref(info))
+    assert info.level_rows[info.num_levels - 1] > 256
+    (ig, xg), (io, xo) = res["gpu"], res["oracle"]
+    assert ig > 0 and abs(ig - io) <= 1
+    assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) < 1e-10
