@@ -16,6 +16,7 @@
 // aggregate, first in row order on ties, sizes growing as they join) runs in
 // rounds in which a leftover decides only when it is the lowest undecided
 // leftover touching each of its candidate aggregates.
+#include <algorithm>
 #include <climits>
 #include <cstring>
 
@@ -29,6 +30,18 @@ constexpr int kBlock = 256;
 constexpr int kUnk = -3;      // undetermined pick / state
 constexpr int kNone = -1;     // no candidate
 constexpr int kFreeUnk = 0, kFreeT = 1, kFreeF = 2;
+
+// State words (fstate, pick, linkpar) are read and written with relaxed
+// agent-scope atomics: they are coherent across the XCDs' L2 caches, which
+// lets workgroups see each other's decisions while they run.  Each word only
+// moves from unknown to its final value (links only move downward), so no
+// ordering between words is needed.
+__device__ __forceinline__ int ld(const int *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st(int *p, int v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 inline int grid_for(long n) {
     long g = (n + kBlock - 1) / kBlock;
@@ -83,14 +96,14 @@ __global__ void k_prefs(int n, const int *__restrict__ ia, const int *__restrict
 
 // taken(j, t): 1 true, 0 false, -1 unknown
 __device__ __forceinline__ int taken(int j, int t, const int *__restrict__ ia, const int *__restrict__ ja,
-                                     const volatile int *fstate, const volatile int *pick) {
+                                     const int *fstate, const int *pick) {
     int res = 0;
     for (int q = ia[j]; q < ia[j + 1]; ++q) {
         const int i = ja[q];
         if (i == j || i >= t) continue;
-        const int fi = fstate[i];
+        const int fi = ld(fstate + i);
         if (fi == kFreeF) continue;
-        const int pi = pick[i];
+        const int pi = ld(pick + i);
         if (pi == kUnk) { res = -1; continue; }
         if (pi != j) continue;          // picks someone else (or none)
         if (fi == kFreeT) return 1;
@@ -103,16 +116,15 @@ __device__ __forceinline__ int taken(int j, int t, const int *__restrict__ ia, c
 // exactly one lower neighbour i can still take k (pick(i) == k decided,
 // free(i) unknown), *single = i: then free(k) = not free(i) exactly.
 __device__ __forceinline__ int free_eval(int k, const int *__restrict__ ia, const int *__restrict__ ja,
-                                         const volatile int *fstate, const volatile int *pick,
-                                         int *single) {
+                                         const int *fstate, const int *pick, int *single) {
     int res = 0, cand = -1, ncand = 0;
     bool ambiguous = false;
     for (int q = ia[k]; q < ia[k + 1]; ++q) {
         const int i = ja[q];
         if (i >= k) continue;
-        const int fi = fstate[i];
+        const int fi = ld(fstate + i);
         if (fi == kFreeF) continue;
-        const int pi = pick[i];
+        const int pi = ld(pick + i);
         if (pi == kUnk) { res = -1; ambiguous = true; continue; }
         if (pi != k) continue;
         if (fi == kFreeT) return 1;
@@ -124,58 +136,162 @@ __device__ __forceinline__ int free_eval(int k, const int *__restrict__ ia, cons
     return res;
 }
 
-__global__ void k_hem_round(int n, const int *__restrict__ ia, const int *__restrict__ ja,
-                            const int *__restrict__ pref, const int *__restrict__ npref,
-                            int *fstate, int *pick, int *linkpar, int *__restrict__ undecided) {
-    int local = 0;
-    const int blk = xcd_block_id();
-    for (int k = blk * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        int f = ((volatile int *)fstate)[k];
-        if (f == kFreeF) continue;
-        int p = ((volatile int *)pick)[k];
-        if (f == kFreeT && p != kUnk) continue;
-        if (f == kFreeUnk) {
-            int single = -1;
-            const int t = free_eval(k, ia, ja, fstate, pick, &single);
-            if (t == 1) { f = kFreeF; fstate[k] = f; }
-            else if (t == 0) { f = kFreeT; fstate[k] = f; }
-            else if (single >= 0 && ((volatile int *)linkpar)[k] < 0) linkpar[k] = 2 * single + 1;
+// One update of vertex k (free test, link creation and one pointer jump along
+// free(k) = free(a) XOR parity, pick).  Links only point to lower vertices and
+// every update is monotone (unknown -> decided, links strictly downward), so
+// updates may be repeated and interleaved in any order.  Returns bit 0: state
+// changed, bit 1: k still undecided.
+__device__ __forceinline__ int hem_vertex(int k, const int *__restrict__ ia, const int *__restrict__ ja,
+                                          const int *__restrict__ pref, const int *__restrict__ npref,
+                                          int *fstate, int *pick, int *linkpar) {
+    int f = ld(fstate + k);
+    if (f == kFreeF) return 0;
+    int p = ld(pick + k);
+    if (f == kFreeT && p != kUnk) return 0;
+    int ch = 0;
+    if (f == kFreeUnk) {
+        int single = -1;
+        const int t = free_eval(k, ia, ja, fstate, pick, &single);
+        if (t == 1) {
+            st(fstate + k, kFreeF);
+            return 1;
         }
-        if (f != kFreeF && p == kUnk) {
-            const int b = ia[k], m = npref[k];
-            int np = kNone;
-            for (int r = 0; r < m; ++r) {
-                const int j = ja[pref[b + r]];
-                const int t = taken(j, k, ia, ja, fstate, pick);
-                if (t == 1) continue;
-                np = (t == 0) ? j : kUnk;
-                break;
+        if (t == 0) {
+            f = kFreeT;
+            st(fstate + k, f);
+            ch = 1;
+        } else {
+            int lp = ld(linkpar + k);
+            if (lp < 0 && single >= 0) {
+                lp = 2 * single + 1;
+                st(linkpar + k, lp);
+                ch = 1;
             }
-            if (np != kUnk) { p = np; pick[k] = p; }
+            if (lp >= 0) {
+                const int a = lp >> 1, par = lp & 1;
+                const int fa = ld(fstate + a);
+                if (fa != kFreeUnk) {
+                    f = ((fa == kFreeT) != (par == 1)) ? kFreeT : kFreeF;
+                    st(fstate + k, f);
+                    ch = 1;
+                    if (f == kFreeF) return 1;
+                } else {
+                    const int la = ld(linkpar + a);
+                    if (la >= 0) {
+                        st(linkpar + k, ((la >> 1) << 1) | (par ^ (la & 1)));
+                        ch = 1;
+                    }
+                }
+            }
         }
-        if (f == kFreeUnk || (f == kFreeT && p == kUnk)) ++local;
+    }
+    if (p == kUnk) {
+        const int b = ia[k], m = npref[k];
+        int np = kNone;
+        for (int r = 0; r < m; ++r) {
+            const int j = ja[pref[b + r]];
+            const int t = taken(j, k, ia, ja, fstate, pick);
+            if (t == 1) continue;
+            np = (t == 0) ? j : kUnk;
+            break;
+        }
+        if (np != kUnk) {
+            p = np;
+            st(pick + k, p);
+            ch = 1;
+        }
+    }
+    const int und = (f == kFreeUnk || p == kUnk) ? 2 : 0;
+    return ch | und;
+}
+
+// One pass-1 round.  Each workgroup owns contiguous chunks of kHemChunk
+// vertices and repeats its updates until a whole sweep over the chunk changes
+// nothing (at most kHemLocalIters sweeps), so dependency chains that stay
+// inside a chunk resolve within one launch.  Every workgroup finishes after a
+// bounded number of sweeps; results of other workgroups are read as they
+// appear but never waited for.
+constexpr int kHemPer = 4;
+constexpr int kHemChunk = kBlock * kHemPer;
+constexpr int kHemLocalIters = 64;
+
+__global__ __launch_bounds__(kBlock) void k_hem_round(int n, const int *__restrict__ ia,
+                                                      const int *__restrict__ ja,
+                                                      const int *__restrict__ pref,
+                                                      const int *__restrict__ npref, int *fstate, int *pick,
+                                                      int *linkpar, int *__restrict__ undecided) {
+    __shared__ int changed[2];
+    const int nchunks = (n + kHemChunk - 1) / kHemChunk;
+    int local = 0;
+    for (int c = xcd_block_id(); c < nchunks; c += gridDim.x) {
+        const int base = c * kHemChunk;
+        int und = 0;
+        for (int it = 0; it < kHemLocalIters; ++it) {
+            if (threadIdx.x == 0) changed[it & 1] = 0;
+            __syncthreads();
+            int mych = 0;
+            und = 0;
+#pragma unroll
+            for (int s = 0; s < kHemPer; ++s) {
+                const int k = base + s * kBlock + threadIdx.x;
+                if (k < n) {
+                    const int r = hem_vertex(k, ia, ja, pref, npref, fstate, pick, linkpar);
+                    mych |= r & 1;
+                    und += r >> 1;
+                }
+            }
+            if (mych) changed[it & 1] = 1;
+            __threadfence_block();
+            __syncthreads();
+            if (!changed[it & 1]) break;
+        }
+        __syncthreads();   // every thread has read its last flag before the next chunk resets one
+        local += und;
     }
     if (local) atomicAdd(undecided, local);
 }
 
-// Pointer jumping along free(k) = free(a) XOR parity links (packed as
-// 2*a + parity so a link is read in one load).
-__global__ void k_hem_jump(int n, int *fstate, int *linkpar) {
-    const int blk = xcd_block_id();
-    for (int k = blk * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        const int lp = ((volatile int *)linkpar)[k];
-        if (lp < 0) continue;
-        if (((volatile int *)fstate)[k] != kFreeUnk) { linkpar[k] = -1; continue; }
-        const int a = lp >> 1, par = lp & 1;
-        const int fa = ((volatile int *)fstate)[a];
-        if (fa != kFreeUnk) {
-            const bool freek = (fa == kFreeT) != (par == 1);
-            fstate[k] = freek ? kFreeT : kFreeF;
-            linkpar[k] = -1;
-            continue;
+// Pass 1 in one launch.  Workgroups take chunks of kHemChunk vertices by
+// ticket in index order and sweep a chunk until every vertex in it is
+// decided.  A vertex depends only on lower vertices, so the lowest unfinished
+// chunk is always held by a running workgroup and decides at least its lowest
+// undecided vertex in every sweep.  A chunk that needs more than
+// kPipeMaxSweeps sweeps raises *status and every workgroup stops (the host
+// then finishes with k_hem_round); the kernel therefore always terminates.
+constexpr int kPipeMaxSweeps = 8192;
+
+__global__ __launch_bounds__(kBlock) void k_hem_pipeline(int n, const int *__restrict__ ia,
+                                                         const int *__restrict__ ja,
+                                                         const int *__restrict__ pref,
+                                                         const int *__restrict__ npref, int *fstate, int *pick,
+                                                         int *linkpar, int *__restrict__ ticket,
+                                                         int *__restrict__ status) {
+    __shared__ int chunk;
+    const int nchunks = (n + kHemChunk - 1) / kHemChunk;
+    for (;;) {
+        if (threadIdx.x == 0) chunk = atomicAdd(ticket, 1);
+        __syncthreads();
+        const int c = chunk;
+        __syncthreads();
+        if (c >= nchunks) return;
+        const int base = c * kHemChunk;
+        for (int it = 0;; ++it) {
+            int und = 0;
+#pragma unroll
+            for (int s = 0; s < kHemPer; ++s) {
+                const int k = base + s * kBlock + threadIdx.x;
+                if (k < n) und |= hem_vertex(k, ia, ja, pref, npref, fstate, pick, linkpar) >> 1;
+            }
+            if (!__syncthreads_or(und)) break;
+            if ((it & 15) == 15) {
+                int stop = 0;
+                if (threadIdx.x == 0) {
+                    if (it >= kPipeMaxSweeps) st(status, 1);
+                    stop = ld(status);
+                }
+                if (__syncthreads_or(stop)) return;
+            }
         }
-        const int la = ((volatile int *)linkpar)[a];
-        if (la >= 0) linkpar[k] = ((la >> 1) << 1) | (par ^ (la & 1));
     }
 }
 
@@ -310,24 +426,38 @@ int hem_aggregate_gpu(const DevCSR &N, int *agg, int *nagg_out) {
     k_prefs<<<g, kBlock, 0, st>>>(n, N.ia, N.ja, N.val, pref, npref, fstate);
     k_fill_i<<<g, kBlock, 0, st>>>(pick, kUnk, n);
     HZ_LAUNCH_CHECK();
-    int *undecided = flagbuf + 1;
     int *linkpar = arena().alloc_n<int>((size_t)n + 1);
     k_fill_i<<<g, kBlock, 0, st>>>(linkpar, -1, n);
-    const int kRoundsPerCheck = 3, kJumps = 4;
+    // one undecided counter per round, cleared once
+    const int kCounterSlots = 4096;
+    int *undecided = arena().alloc_n<int>(kCounterSlots);
+    HZ_CHECK(hipMemsetAsync(undecided, 0, sizeof(int) * kCounterSlots, st));
+    const int nchunks = (n + kHemChunk - 1) / kHemChunk;
+    const int gr = std::max(1, std::min(8192, nchunks));
+    int *pipe = arena().alloc_n<int>(2);   // ticket, status
+    HZ_CHECK(hipMemsetAsync(pipe, 0, 2 * sizeof(int), st));
+    k_hem_pipeline<<<std::min(nchunks, 2048), kBlock, 0, st>>>(n, N.ia, N.ja, pref, npref, fstate, pick,
+                                                              linkpar, pipe, pipe + 1);
+    HZ_LAUNCH_CHECK();
+    int pipe_status = 0;
+    HZ_CHECK(memcpy_sync(&pipe_status, pipe + 1, sizeof(int), hipMemcpyDeviceToHost));
+    const int kRoundsPerCheck = 2;
     int rounds1 = 0, rounds2 = 0;
-    for (int round = 0;; round += kRoundsPerCheck) {
-        for (int r = 0; r < kRoundsPerCheck; ++r) {
-            HZ_CHECK(hipMemsetAsync(undecided, 0, sizeof(int), st));
-            k_hem_round<<<g, kBlock, 0, st>>>(n, N.ia, N.ja, pref, npref, fstate, pick, linkpar,
-                                              undecided);
-            for (int j = 0; j < kJumps; ++j) k_hem_jump<<<g, kBlock, 0, st>>>(n, fstate, linkpar);
+    for (int round = 0; pipe_status != 0; round += kRoundsPerCheck) {
+        if (round + kRoundsPerCheck > kCounterSlots) {
+            HZ_CHECK(hipMemsetAsync(undecided, 0, sizeof(int) * kCounterSlots, st));
+            round = 0;   // only the slot index restarts; progress is kept in the state arrays
         }
+        for (int r = 0; r < kRoundsPerCheck; ++r)
+            k_hem_round<<<gr, kBlock, 0, st>>>(n, N.ia, N.ja, pref, npref, fstate, pick, linkpar,
+                                               undecided + round + r);
         HZ_LAUNCH_CHECK();
-        rounds1 = round + kRoundsPerCheck;
+        rounds1 += kRoundsPerCheck;
         int und = 0;
-        HZ_CHECK(memcpy_sync(&und, undecided, sizeof(int), hipMemcpyDeviceToHost));
+        HZ_CHECK(memcpy_sync(&und, undecided + round + kRoundsPerCheck - 1, sizeof(int),
+                             hipMemcpyDeviceToHost));
         if (und == 0) break;
-        if (round > n + 8) return 1;   // cannot happen: the lowest undecided vertex decides every round
+        if (rounds1 > n + 8) return 1;   // cannot happen: the lowest undecided vertex decides every round
     }
     // pass-1 ids: initiators ranked in index order
     int *flag = arena().alloc_n<int>((size_t)n + 1);
@@ -373,8 +503,8 @@ int hem_aggregate_gpu(const DevCSR &N, int *agg, int *nagg_out) {
     }
     *nagg_out = nc;
     if (trace_enabled())
-        std::fprintf(stderr, "[hz]   HEM n=%d: pass-1 rounds %d, leftovers %d, pass-2 rounds %d\n", n,
-                     rounds1, nl, rounds2);
+        std::fprintf(stderr, "[hz]   HEM n=%d: pass-1 %s, fallback rounds %d, leftovers %d, pass-2 rounds %d\n",
+                     n, pipe_status ? "pipeline gave up" : "pipeline", rounds1, nl, rounds2);
     return HZ_SUCCESS;
 }
 
