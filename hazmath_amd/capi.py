@@ -96,6 +96,13 @@ class hz_hierarchy(C.Structure):
                 ("num_aggregates", INT * MAX_AMG_LVL), ("theta", REAL * MAX_AMG_LVL)]
 
 
+PRECOND_FN = C.CFUNCTYPE(None, C.POINTER(REAL), C.POINTER(REAL), C.c_void_p)
+
+
+class precond(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("fct", PRECOND_FN), ("setup_time", REAL)]
+
+
 class hz_kernel_times(C.Structure):
     _fields_ = [("reps", INT), ("spmv_ms", REAL), ("residual_ms", REAL), ("jacobi_ms", REAL),
                 ("spmv_dot_ms", REAL), ("sell_entries", C.c_longlong)]
@@ -242,6 +249,14 @@ def _declare(lib):
     lib.hz_dense_inverse.argtypes = [INT, P(REAL)]
     lib.hz_dense_inverse.restype = INT
     lib.hz_dense_mfma_active.restype = INT
+    lib.linear_solver_amg.argtypes = [P(dCSRmat), P(dvector), P(dvector), P(AMG_param)]
+    lib.linear_solver_amg.restype = INT
+    lib.create_precond_amg.argtypes = [P(dCSRmat), P(AMG_param)]
+    lib.create_precond_amg.restype = P(precond)
+    lib.apply_precond.argtypes = [P(REAL), P(REAL), P(precond)]
+    lib.hz_precond_amg_apply.argtypes = [P(REAL), P(REAL), C.c_void_p]
+    lib.hz_precond_amg_apply_device.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.hz_precond_free.argtypes = [P(precond)]
     lib.hz_time_fine_kernels.argtypes = [INT, INT, C.c_void_p, C.c_void_p, C.c_void_p, INT,
                                          P(hz_kernel_times)]
     lib.hz_time_fine_kernels.restype = INT

@@ -32,7 +32,7 @@ void *Arena::alloc(size_t bytes) {
         }
         ++cur_;
     }
-    const size_t sz = std::max(bytes, size_t(256) << 20);
+    const size_t sz = std::max(bytes, min_chunk_);
     char *p = nullptr;
     HZ_CHECK(hipMalloc(&p, sz));
     chunks_.push_back({p, sz, bytes});
@@ -57,9 +57,25 @@ size_t Arena::bytes_in_use() const {
     return s;
 }
 
-Arena &arena() {
-    static Arena a;
-    return a;
+namespace {
+Arena g_default_arena;
+Arena *g_current_arena = &g_default_arena;
+}  // namespace
+
+Arena &arena() { return *g_current_arena; }
+
+ArenaScope::ArenaScope(Arena *a) : prev_(g_current_arena) { g_current_arena = a; }
+ArenaScope::~ArenaScope() { g_current_arena = prev_; }
+
+Arena::Mark Arena::mark() const {
+    if (chunks_.empty()) return {0, 0};
+    return {cur_, chunks_[cur_].used};
+}
+
+void Arena::rewind(const Mark &m) {
+    for (size_t c = m.chunk + 1; c < chunks_.size(); ++c) chunks_[c].used = 0;
+    if (m.chunk < chunks_.size()) chunks_[m.chunk].used = m.used;
+    cur_ = m.chunk;
 }
 
 bool trace_enabled() {
@@ -335,7 +351,9 @@ forward:
         if (cp.coarse_solver == HZ_SOLVER_UMFPACK) {
             coarse_dense_solve(C.A.n, C.Ainv, C.b, C.x);
         } else {
-            fill(C.x, 0.0, C.A.n);
+            // CG from the level's iterate: zero below the finest level, the
+            // caller's guess when the finest level is also the coarsest
+            if (cs.zero[nl - 1]) fill(C.x, 0.0, C.A.n);
             coarse_cg(C.S, C.b, C.x, cp.coarse_tol, 20 * C.A.n, C.cgwork);
         }
         cs.zero[nl - 1] = false;
@@ -687,6 +705,69 @@ FINISHED:
     HZ_CHECK(hipStreamSynchronize(stream()));
     res.relres = relres;
     res.iter = (iter > MaxIt) ? HZ_ERROR_SOLVER_MAXIT : iter;
+    return res;
+}
+
+// ---------------------------------------------------------------------------
+// AMG as a stand-alone iterative solver, amg_solve (amg_solve.c:109-161):
+// cycles from the caller's x until ||b - A x|| / ||b|| < tol.
+// ---------------------------------------------------------------------------
+AmgSolveResult amg_solve(Hierarchy &H, const double *b, double *x, double tol, int MaxIt, int prt) {
+    AmgSolveResult res;
+    Level &L0 = H.lv[0];
+    const int n = L0.A.n;
+    double *r = arena().alloc_n<double>((size_t)n + 1);
+    PcgScalars hs{};
+    norm2sq(n, b, &H.sc->rr, H.ws);
+    fetch(H, hs);
+    const double sumb = std::sqrt(hs.rr);
+    double relres1 = kBigReal, absres0 = sumb;
+    itinfo(prt, HZ_STOP_REL_RES, 0, 1.0, sumb, 0.0);
+    // the cycle ping-pongs level 0 between x and a scratch vector
+    L0.b = const_cast<double *>(b);
+    L0.x = x;
+    L0.xt = H.x0buf;
+    CycleState cs;
+    for (int i = 0; i < HZ_MAX_AMG_LVL; ++i) {
+        cs.zero[i] = false;
+        cs.first_done[i] = false;
+    }
+    int iter = 0;
+    // the reference evaluates both operands of '&' before testing
+    while ((++iter <= MaxIt) & (sumb > kSmallReal)) {
+        bool unused = false;
+        if (H.nl == 1) {
+            if (H.cp.coarse_solver == HZ_SOLVER_UMFPACK)
+                coarse_dense_solve(n, L0.Ainv, b, L0.x);
+            else
+                coarse_cg(L0.S, b, L0.x, H.cp.coarse_tol, 20 * n, L0.cgwork);
+        } else {
+            mgcycle(H, cs, nullptr, &unused);
+        }
+        residual(L0.S, b, L0.x, r);
+        norm2sq(n, r, &H.sc->rr, H.ws);
+        fetch(H, hs);
+        const double absres = std::sqrt(hs.rr);
+        relres1 = absres / sumb;
+        const double factor = absres / absres0;
+        absres0 = absres;
+        itinfo(prt, HZ_STOP_REL_RES, iter, relres1, absres, factor);
+        if (relres1 < tol) break;
+    }
+    if (L0.x != x) copy(x, L0.x, n);
+    L0.x = nullptr;
+    L0.xt = nullptr;
+    L0.b = nullptr;
+    if (prt > 0) {
+        if (iter > MaxIt)
+            std::printf("### HAZMATH WARNING: Max iter %lld reached with rel. resid. %e.\n", (long long)MaxIt,
+                        relres1);
+        else if (iter >= 0)
+            std::printf("Num_iter(amg_solve.c) = %lld with relative residual %e.\n", (long long)iter, relres1);
+    }
+    HZ_CHECK(hipStreamSynchronize(stream()));
+    res.iter = iter;
+    res.relres = relres1;
     return res;
 }
 

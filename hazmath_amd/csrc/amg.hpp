@@ -63,6 +63,16 @@ struct PcgResult {
 PcgResult pcg_solve(Hierarchy &H, const DevSELL &A, const double *b, double *u, double tol,
                     int maxit, int stop_type, int prtlvl);
 
+struct AmgSolveResult {
+    int iter = 0;
+    double relres = 0.0;
+};
+
+// AMG cycles as the solver, amg_solve (amg_solve.c:109-161); x holds the
+// initial guess on entry.  Returns the reference's iteration count (MaxIt+1
+// when it did not converge).
+AmgSolveResult amg_solve(Hierarchy &H, const double *b, double *x, double tol, int MaxIt, int prt);
+
 // Host-staged sequential aggregation for VMB (and HEM on non-symmetric patterns).
 int aggregate_host(const DevCSR &N, int type, int max_aggregation, int *agg_dev, int *nagg);
 

@@ -55,6 +55,13 @@ void download_csr(const DevCSR &d, dCSRmat *A) {
     }
 }
 
+// check_error (message.c:191ff) exits with the status; return_on_fatal returns it.
+int fatal_status(int code) {
+    if (g_opt.return_on_fatal) return code;
+    std::fflush(stdout);
+    std::exit(code);
+}
+
 int fatal(int code) {
     // reference: exit(16) / exit(17) / check_error -> exit(status)
     if (g_opt.return_on_fatal) return code;
@@ -154,6 +161,17 @@ INT core_solve(const DevCSR &A, const double *db, double *dx, linear_itsolver_pa
     std::fflush(stdout);
     return iter;
 }
+
+// Preconditioner object behind precond.data (create_precond_amg): the
+// hierarchy lives in its own arena so later library calls do not reuse its
+// memory, plus device staging vectors for host-array applications.
+struct PrecondAmg {
+    Arena arena{size_t(16) << 20};
+    Hierarchy H;
+    int n = 0;
+    double *r = nullptr, *z = nullptr;
+    Arena::Mark mark{0, 0};
+};
 
 }  // namespace
 
@@ -555,6 +573,139 @@ INT hz_dense_mfma_active(void) {
     } catch (const DeviceError &) {
         return 0;
     }
+}
+
+// itsolver.c:524-640 (UA setup + amg_solve).  x holds the initial guess.
+INT linear_solver_amg(dCSRmat *A, dvector *b, dvector *x, AMG_param *param) {
+    std::memset(&g_info, 0, sizeof(g_info));
+    const int prt = param->print_level;
+    if (A->row != A->col) {
+        std::printf("### ERROR: A is not a square matrix!\n");
+        return fatal_status(HZ_ERROR_MAT_SIZE);
+    }
+    if (A->nnz <= 0) {
+        std::printf("### ERROR: A has no nonzero entries!\n");
+        return fatal_status(HZ_ERROR_MAT_SIZE);
+    }
+    if (!device_ready()) return HZ_ERROR_DEVICE;
+    try {
+        arena().reset();
+        const auto t0 = std::chrono::steady_clock::now();
+        int st = check_supported(nullptr, param);
+        if (st < 0) return st;
+        const DevCSR dA = upload_csr(A);
+        const double *db = upload(b->val, (size_t)A->row);
+        double *dx = upload(x->val, (size_t)A->row);
+        if (prt > 0) std::printf("\n Calling UA AMG ...\n");
+        Hierarchy H;
+        H.cp.jacobi_w = g_opt.jacobi_weight;
+        H.hec_rng = g_opt.hec_rng_source;
+        st = amg_setup(H, dA, param, false);
+        if (st <= -1000) return fatal(st);
+        if (st < 0) return st;   // the reference falls back to GMRES here; not on this path
+        amg_prepare_cycle(H, param);
+        H.cp.jacobi_w = g_opt.jacobi_weight;
+        // amg_solve hands the caller's param to mgcycle: coarse tolerance tol * 1e-2
+        H.cp.coarse_tol = param->tol * 1e-2;
+        const AmgSolveResult r = amg_solve(H, db, dx, param->tol, param->maxit, prt);
+        HZ_CHECK(memcpy_sync(x->val, dx, sizeof(double) * (size_t)A->row, hipMemcpyDeviceToHost));
+        const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        g_info.iterations = r.iter;
+        g_info.num_levels = H.nl;
+        for (int l = 0; l < H.nl && l < HZ_MAX_AMG_LVL; ++l) {
+            g_info.level_rows[l] = H.lv[l].A.n;
+            g_info.level_nnz[l] = H.lv[l].A.nnz;
+        }
+        g_info.total_ms = ms;
+        g_info.final_relres = r.relres;
+        if (prt > 0) {
+            std::printf("AMG totally costs %f seconds\n", ms * 1e-3);
+            std::printf("**********************************************************\n");
+        }
+        std::fflush(stdout);
+        return r.iter;
+    } catch (const DeviceError &) {
+        return HZ_ERROR_DEVICE;
+    }
+}
+
+// precond_amg (precond.c:164-183) on host arrays: z = B r.
+void hz_precond_amg_apply(REAL *r, REAL *z, void *data) {
+    auto *P = static_cast<PrecondAmg *>(data);
+    try {
+        ArenaScope scope(&P->arena);
+        HZ_CHECK(hipMemcpyAsync(P->r, r, sizeof(double) * (size_t)P->n, hipMemcpyHostToDevice, stream()));
+        amg_precond(P->H, P->r, P->z, nullptr);
+        HZ_CHECK(memcpy_sync(z, P->z, sizeof(double) * (size_t)P->n, hipMemcpyDeviceToHost));
+        P->arena.rewind(P->mark);
+    } catch (const DeviceError &) {
+        std::fprintf(stderr, "hazmath_mi355x: preconditioner application failed\n");
+        std::memset(z, 0, sizeof(double) * (size_t)P->n);
+    }
+}
+
+void hz_precond_amg_apply_device(const REAL *dr, REAL *dz, void *data) {
+    auto *P = static_cast<PrecondAmg *>(data);
+    try {
+        ArenaScope scope(&P->arena);
+        amg_precond(P->H, dr, dz, nullptr);
+        HZ_CHECK(hipStreamSynchronize(stream()));
+        P->arena.rewind(P->mark);
+    } catch (const DeviceError &) {
+        std::fprintf(stderr, "hazmath_mi355x: preconditioner application failed\n");
+    }
+}
+
+// helper.c:206-283: UA setup once; pc->fct applies one preconditioner step.
+precond *create_precond_amg(dCSRmat *A, AMG_param *amgparam) {
+    if (!device_ready()) return nullptr;
+    if (check_supported(nullptr, amgparam) < 0) return nullptr;
+    auto *P = new PrecondAmg();
+    try {
+        const auto t0 = std::chrono::steady_clock::now();
+        ArenaScope scope(&P->arena);
+        const DevCSR dA = upload_csr(A);
+        if (amgparam->print_level > 0) std::printf("\n Calling UA AMG ...\n");
+        P->H.cp.jacobi_w = g_opt.jacobi_weight;
+        P->H.hec_rng = g_opt.hec_rng_source;
+        const int st = amg_setup(P->H, dA, amgparam, false);
+        if (st < 0) {
+            std::printf("\n In function create_precond: AMG data failed to set up \n");
+            P->arena.release();
+            delete P;
+            return nullptr;
+        }
+        amg_prepare_cycle(P->H, amgparam);
+        P->H.cp.jacobi_w = g_opt.jacobi_weight;
+        P->n = A->row;
+        P->r = arena().alloc_n<double>((size_t)P->n + 1);
+        P->z = arena().alloc_n<double>((size_t)P->n + 1);
+        HZ_CHECK(hipStreamSynchronize(stream()));
+        P->mark = P->arena.mark();
+        auto *pc = static_cast<precond *>(std::calloc(1, sizeof(precond)));
+        pc->data = P;
+        pc->fct = hz_precond_amg_apply;
+        pc->setup_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return pc;
+    } catch (const DeviceError &) {
+        P->arena.release();
+        delete P;
+        return nullptr;
+    }
+}
+
+// helper.c:1254-1260
+void apply_precond(REAL *r, REAL *z, precond *pc) { pc->fct(r, z, pc->data); }
+
+void hz_precond_free(precond *pc) {
+    if (!pc) return;
+    auto *P = static_cast<PrecondAmg *>(pc->data);
+    if (P) {
+        (void)hipStreamSynchronize(stream());
+        P->arena.release();
+        delete P;
+    }
+    std::free(pc);
 }
 
 }  // extern "C"

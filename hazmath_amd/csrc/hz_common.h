@@ -40,19 +40,36 @@ struct DeviceError {
 // the pool is warm.  Not thread-safe (the reference path is not re-entrant).
 class Arena {
   public:
+    explicit Arena(size_t min_chunk = size_t(256) << 20) : min_chunk_(min_chunk) {}
     void *alloc(size_t bytes);
     template <class T>
     T *alloc_n(size_t n) { return static_cast<T *>(alloc(n * sizeof(T))); }
     void reset();            // forget all allocations, keep chunks
     void release();          // hipFree every chunk
     size_t bytes_in_use() const;
+    struct Mark { size_t chunk, used; };
+    Mark mark() const;               // current fill position
+    void rewind(const Mark &m);      // forget allocations made after m
   private:
     struct Chunk { char *base; size_t size; size_t used; };
     std::vector<Chunk> chunks_;
     size_t cur_ = 0;
+    size_t min_chunk_;
 };
 
+// The arena that allocations go to: the library-wide one, or an object's own
+// (a preconditioner keeps its hierarchy across calls) while an ArenaScope for
+// it is alive.
 Arena &arena();
+class ArenaScope {
+  public:
+    explicit ArenaScope(Arena *a);
+    ~ArenaScope();
+    ArenaScope(const ArenaScope &) = delete;
+    ArenaScope &operator=(const ArenaScope &) = delete;
+  private:
+    Arena *prev_;
+};
 hipStream_t stream();
 
 // Every device<->host copy goes through the library's stream: a blocking

@@ -111,7 +111,7 @@ void rap_agg(const DevCSR &A, const int *agg, int nagg, const int *rptr, const i
 void csr_transpose(const DevCSR &A, DevCSR &AT);
 
 // ---- kernels_dense.hip ----------------------------------------------------
-// Dense LDL^T factorization of the coarsest matrix and its explicit inverse.
+// Explicit inverse of the dense coarsest matrix (block Gauss-Jordan).
 // Returns 0 on success.
 int coarse_dense_setup(const DevCSR &A, double *Ainv);
 // x = Ainv b

@@ -2,10 +2,12 @@
 //
 // coarse_solver = 32 (SOLVER_UMFPACK): the reference factors the coarsest
 // matrix once with its sparse U^T D U (src/solver/hazmath_direct.c:633-854)
-// and solves with it every cycle.  Here the coarsest matrix is factored once
-// as a dense L D L^T and its explicit inverse is formed; each cycle then costs
-// one dense matrix-vector product spread over many workgroups instead of two
-// latency-bound triangular sweeps.  Results equal the reference's to rounding.
+// and solves with it every cycle.  Here the explicit inverse of the dense
+// coarsest matrix is formed once by block Gauss-Jordan elimination (no
+// pivoting, like the reference's factorization; rank-64 updates on the FP64
+// matrix cores), and each cycle costs one dense matrix-vector product spread
+// over many workgroups instead of two latency-bound triangular sweeps.
+// Results equal the reference's to rounding.
 //
 // coarse_solver = 0: unpreconditioned CG to the cycle tolerance
 // (coarse_itsolver, src/solver/mgcycle.c:34-53 -> dcsr_pcg, krylov.c:122-372),
@@ -20,65 +22,10 @@ namespace hz {
 
 namespace {
 
-constexpr int kDenseBlock = 1024;
-
 __global__ void k_dense_from_csr(int n, const int *__restrict__ ia, const int *__restrict__ ja,
                                  const double *__restrict__ val, double *__restrict__ M) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         for (int k = ia[i]; k < ia[i + 1]; ++k) M[(size_t)i * n + ja[k]] += val[k];
-}
-
-// Right-looking L D L^T in place (lower triangle holds L, diagonal holds D),
-// then X = L^{-1} and Ainv = X^T D^{-1} X.  One workgroup; n is the coarsest
-// size (small).  status[0] = 1 on a zero pivot.
-__global__ __launch_bounds__(kDenseBlock) void k_ldlt_inverse(int n, double *__restrict__ M,
-                                                              double *__restrict__ X,
-                                                              double *__restrict__ Ainv,
-                                                              int *__restrict__ status) {
-    const int tid = threadIdx.x;
-    for (int j = 0; j < n; ++j) {
-        const double dj = M[(size_t)j * n + j];
-        if (dj == 0.0) {
-            if (tid == 0) status[0] = 1;
-            return;
-        }
-        // column j of L
-        for (int i = j + 1 + tid; i < n; i += kDenseBlock) M[(size_t)i * n + j] = M[(size_t)i * n + j] / dj;
-        __syncthreads();
-        // trailing lower triangle: M[i][k] -= L[i][j] * dj * L[k][j], j < k <= i
-        const int m = n - j - 1;
-        const long tri = (long)m * (m + 1) / 2;
-        for (long t = tid; t < tri; t += kDenseBlock) {
-            // map t -> (i, k) with 0 <= k <= i < m
-            int i = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-            while ((long)(i + 1) * (i + 2) / 2 <= t) ++i;
-            while ((long)i * (i + 1) / 2 > t) --i;
-            const int k = (int)(t - (long)i * (i + 1) / 2);
-            const int gi = j + 1 + i, gk = j + 1 + k;
-            M[(size_t)gi * n + gk] -= M[(size_t)gi * n + j] * dj * M[(size_t)gk * n + j];
-        }
-        __syncthreads();
-    }
-    // X = L^{-1}: column c solves L x = e_c (unit lower triangular)
-    for (int c = tid; c < n; c += kDenseBlock) {
-        for (int i = 0; i < c; ++i) X[(size_t)i * n + c] = 0.0;
-        X[(size_t)c * n + c] = 1.0;
-        for (int i = c + 1; i < n; ++i) {
-            double s = 0.0;
-            for (int k = c; k < i; ++k) s += M[(size_t)i * n + k] * X[(size_t)k * n + c];
-            X[(size_t)i * n + c] = -s;
-        }
-    }
-    __syncthreads();
-    // Ainv[i][j] = sum_{k >= max(i,j)} X[k][i] X[k][j] / D[k]
-    const long nn = (long)n * n;
-    for (long t = tid; t < nn; t += kDenseBlock) {
-        const int i = (int)(t / n), jj = (int)(t % n);
-        const int k0 = i > jj ? i : jj;
-        double s = 0.0;
-        for (int k = k0; k < n; ++k) s += X[(size_t)k * n + i] * X[(size_t)k * n + jj] / M[(size_t)k * n + k];
-        Ainv[t] = s;
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -418,8 +365,6 @@ int blocked_inverse(int n, double *M) {
     return hs;
 }
 
-constexpr int kSmallDense = 256;   // single-workgroup L D L^T below this size
-
 }  // namespace
 
 int dense_inverse_inplace(int n, double *M) { return blocked_inverse(n, M); }
@@ -428,26 +373,12 @@ bool dense_mfma_active() { return mfma_layout_ok(); }
 
 int coarse_dense_setup(const DevCSR &A, double *Ainv) {
     const int n = A.n;
+    if (n == 0) return 0;
     hipStream_t st = stream();
-    if (n > kSmallDense) {
-        HZ_CHECK(hipMemsetAsync(Ainv, 0, sizeof(double) * (size_t)n * n, st));
-        k_dense_from_csr<<<div_up(n, 256), 256, 0, st>>>(n, A.ia, A.ja, A.val, Ainv);
-        HZ_LAUNCH_CHECK();
-        return blocked_inverse(n, Ainv);
-    }
-    double *M = arena().alloc_n<double>((size_t)n * n + 1);
-    double *X = arena().alloc_n<double>((size_t)n * n + 1);
-    int *status = arena().alloc_n<int>(1);
-    HZ_CHECK(hipMemsetAsync(M, 0, sizeof(double) * (size_t)n * n, st));
-    HZ_CHECK(hipMemsetAsync(status, 0, sizeof(int), st));
-    k_dense_from_csr<<<div_up(n, 256), 256, 0, st>>>(n, A.ia, A.ja, A.val, M);
+    HZ_CHECK(hipMemsetAsync(Ainv, 0, sizeof(double) * (size_t)n * n, st));
+    k_dense_from_csr<<<div_up(n, 256), 256, 0, st>>>(n, A.ia, A.ja, A.val, Ainv);
     HZ_LAUNCH_CHECK();
-    k_ldlt_inverse<<<1, kDenseBlock, 0, st>>>(n, M, X, Ainv, status);
-    HZ_LAUNCH_CHECK();
-    int hs = 0;
-    HZ_CHECK(hipMemcpyAsync(&hs, status, sizeof(int), hipMemcpyDeviceToHost, st));
-    HZ_CHECK(hipStreamSynchronize(st));
-    return hs;
+    return blocked_inverse(n, Ainv);
 }
 
 void coarse_dense_solve(int n, const double *Ainv, const double *b, double *x) {

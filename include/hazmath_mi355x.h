@@ -37,6 +37,7 @@ extern "C" {
 #define HZ_ERROR_OPEN_FILE      -10
 #define HZ_ERROR_WRONG_FILE     -11
 #define HZ_ERROR_INPUT_PAR      -13
+#define HZ_ERROR_MAT_SIZE       -15
 #define HZ_ERROR_ALLOC_MEM      -20
 #define HZ_ERROR_DATA_STRUCTURE -21
 #define HZ_ERROR_AMG_SMOOTH_TYPE -31
@@ -179,6 +180,31 @@ void dvec_set(INT n, dvector *x, const REAL val);
 dCSRmat *dcoo_read_dcsr_p(FILE *fp);
 dvector *dvector_read_p(FILE *fp);
 
+/* Replaces linear_solver_amg, src/solver/itsolver.c:524-640: UA-AMG setup, then
+ * AMG cycles as the solver (amg_solve, src/solver/amg_solve.c:109-161) from the
+ * initial guess in x until ||b-Ax||/||b|| < param->tol.  Returns the iteration
+ * count (param->maxit + 1 when it did not converge, like the reference), or a
+ * negative error code.  Where the reference falls back to GMRES after a failed
+ * setup, this returns the setup status. */
+INT linear_solver_amg(dCSRmat *A, dvector *b, dvector *x, AMG_param *param);
+
+/* precond: include/solver.h:25-36. */
+typedef struct {
+    void *data;
+    void (*fct)(REAL *, REAL *, void *);
+    REAL setup_time;
+} precond;
+
+/* Replaces create_precond_amg, src/haznics/helper.c:206-283: UA-AMG setup once
+ * on the GPU (the hierarchy stays in device memory), returns a preconditioner
+ * whose fct applies precond_amg (src/solver/precond.c:164-183: amgparam->maxit
+ * cycles from zero) to HOST arrays.  NULL when setup fails.  Like the
+ * reference, amgparam->strong_coupled is updated per level. */
+precond *create_precond_amg(dCSRmat *A, AMG_param *amgparam);
+
+/* Replaces apply_precond, src/haznics/helper.c:1254-1260: pc->fct(r, z, pc->data). */
+void apply_precond(REAL *r, REAL *z, precond *pc);
+
 /* ---------------------------------------------------------------------------
  * Build-owned extensions (no HAZmath counterpart).
  * ------------------------------------------------------------------------- */
@@ -276,6 +302,15 @@ typedef struct {
 
 INT hz_time_fine_kernels(INT n, INT nnz, const INT *dIA, const INT *dJA, const REAL *dval,
                          INT reps, hz_kernel_times *out);
+
+/* The fct installed by create_precond_amg (host arrays r, z of length A->row). */
+void hz_precond_amg_apply(REAL *r, REAL *z, void *data);
+/* Same action on device arrays (pc->data of a create_precond_amg result);
+ * returns after the result is in dz. */
+void hz_precond_amg_apply_device(const REAL *dr, REAL *dz, void *data);
+/* Releases a preconditioner from create_precond_amg (device memory included);
+ * the reference leaves this to its Python bindings. */
+void hz_precond_free(precond *pc);
 
 /* Runtime information. */
 const char *hz_version(void);

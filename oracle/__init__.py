@@ -44,6 +44,12 @@ def load():
     lib.orc_tridiag_condest.argtypes = [INT, P(REAL), P(REAL)]
     lib.orc_tridiag_condest.restype = REAL
     lib.orc_set_jacobi_weight.argtypes = [REAL]
+    lib.orc_linear_solver_amg.argtypes = [P(dCSRmat), P(dvector), P(dvector), P(AMG_param)]
+    lib.orc_linear_solver_amg.restype = INT
+    lib.orc_precond_create.argtypes = [P(dCSRmat), P(AMG_param)]
+    lib.orc_precond_create.restype = C.c_void_p
+    lib.orc_precond_apply.argtypes = [C.c_void_p, P(REAL), P(REAL)]
+    lib.orc_precond_free.argtypes = [C.c_void_p]
     _LIB = lib
     return lib
 

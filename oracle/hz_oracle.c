@@ -947,6 +947,26 @@ static void orc_precond_amg(const REAL *r, REAL *z, void *data) {
 static REAL g_orc_jacobi_w = 1.0;
 void orc_set_jacobi_weight(REAL w) { g_orc_jacobi_w = w; }
 
+/* param_amg_to_prec + param_prec_to_amg (parameters.c:604-675): the cycle runs
+ * with param_amg_init defaults overwritten by the caller's cycle fields. */
+static void orc_prec_data_init(orc_prec_data *pd, orc_amg *m, const AMG_param *amgparam) {
+    pd->m = m;
+    orc_param_amg_init(&pd->cyc);
+    pd->cyc.AMG_type = amgparam->AMG_type;
+    pd->cyc.print_level = amgparam->print_level;
+    pd->cyc.cycle_type = amgparam->cycle_type;
+    pd->cyc.smoother = amgparam->smoother;
+    pd->cyc.presmooth_iter = amgparam->presmooth_iter;
+    pd->cyc.postsmooth_iter = amgparam->postsmooth_iter;
+    pd->cyc.relaxation = amgparam->relaxation;
+    pd->cyc.polynomial_degree = amgparam->polynomial_degree;
+    pd->cyc.coarse_solver = amgparam->coarse_solver;
+    pd->cyc.coarse_scaling = amgparam->coarse_scaling;
+    pd->cyc.amli_degree = amgparam->amli_degree;
+    pd->maxit = amgparam->maxit;
+    pd->jacobi_w = g_orc_jacobi_w;
+}
+
 /* linear_solver_dcsr_krylov_amg, itsolver.c:846-931, with the CG branch of
  * solver_dcsr_linear_itsolver, itsolver.c:135-211. */
 INT orc_krylov_amg(dCSRmat *A, dvector *b, dvector *x, linear_itsolver_param *itparam,
@@ -974,21 +994,7 @@ INT orc_krylov_amg(dCSRmat *A, dvector *b, dvector *x, linear_itsolver_param *it
     }
     if (status < 0) { orc_amg_free(m); free(m); return status; }
     orc_prec_data pd;
-    pd.m = m;
-    orc_param_amg_init(&pd.cyc);
-    pd.cyc.AMG_type = amgparam->AMG_type;
-    pd.cyc.print_level = amgparam->print_level;
-    pd.cyc.cycle_type = amgparam->cycle_type;
-    pd.cyc.smoother = amgparam->smoother;
-    pd.cyc.presmooth_iter = amgparam->presmooth_iter;
-    pd.cyc.postsmooth_iter = amgparam->postsmooth_iter;
-    pd.cyc.relaxation = amgparam->relaxation;
-    pd.cyc.polynomial_degree = amgparam->polynomial_degree;
-    pd.cyc.coarse_solver = amgparam->coarse_solver;
-    pd.cyc.coarse_scaling = amgparam->coarse_scaling;
-    pd.cyc.amli_degree = amgparam->amli_degree;
-    pd.maxit = amgparam->maxit;
-    pd.jacobi_w = g_orc_jacobi_w;
+    orc_prec_data_init(&pd, m, amgparam);
     INT iter;
     REAL condest = 0.0;
     if (itparam->linear_tol < ORC_SMALLREAL)
@@ -1017,6 +1023,76 @@ INT orc_krylov_amg(dCSRmat *A, dvector *b, dvector *x, linear_itsolver_param *it
     orc_amg_free(m);
     free(m);
     return iter;
+}
+
+/* linear_solver_amg, itsolver.c:524-640, with amg_solve, amg_solve.c:109-161:
+ * AMG cycles from the caller's x until ||b - A x|| / ||b|| < tol. */
+INT orc_linear_solver_amg(dCSRmat *A, dvector *b, dvector *x, AMG_param *param) {
+    const SHORT prt = param->print_level;
+    if (A->row != A->col || A->nnz <= 0) return -15;   /* ERROR_MAT_SIZE (check_error exits) */
+    if (param->smoother != HZ_SMOOTHER_JACOBI && param->smoother != HZ_SMOOTHER_L1DIAG)
+        return HZ_ERROR_AMG_SMOOTH_TYPE;
+    orc_amg *m = (orc_amg *)xcalloc(1, sizeof(orc_amg));
+    if (prt > 0) printf("\n Calling UA AMG ...\n");
+    INT status = orc_setup(m, A, param);
+    if (status < 0) { orc_amg_free(m); free(m); return status; }
+    /* amg_solve runs mgcycle with the caller's param directly */
+    orc_prec_data pd;
+    pd.m = m;
+    pd.cyc = *param;
+    pd.maxit = 1;
+    pd.jacobi_w = g_orc_jacobi_w;
+    orc_level *L0 = &m->lv[0];
+    const INT n = L0->A.row;
+    memcpy(L0->b, b->val, sizeof(REAL) * (size_t)n);
+    memcpy(L0->x, x->val, sizeof(REAL) * (size_t)n);
+    const INT MaxIt = param->maxit;
+    const REAL tol = param->tol;
+    const REAL sumb = orc_norm2(n, L0->b);
+    REAL relres1 = ORC_BIGREAL, absres0 = sumb, absres, factor;
+    INT iter = 0;
+    orc_itinfo(prt, HZ_STOP_REL_RES, iter, 1.0, sumb, 0.0);
+    while ((++iter <= MaxIt) & (sumb > ORC_SMALLREAL)) {
+        orc_mgcycle(&pd);
+        orc_dcsr_residual(&L0->A, L0->b, L0->x, L0->w);
+        absres = orc_norm2(n, L0->w);
+        relres1 = absres / sumb;
+        factor = absres / absres0;
+        absres0 = absres;
+        orc_itinfo(prt, HZ_STOP_REL_RES, iter, relres1, absres, factor);
+        if (relres1 < tol) break;
+    }
+    if (prt > 0) {
+        if (iter > MaxIt)
+            printf("### HAZMATH WARNING: Max iter %lld reached with rel. resid. %e.\n", (long long)MaxIt, relres1);
+        else if (iter >= 0)
+            printf("Num_iter(amg_solve.c) = %lld with relative residual %e.\n", (long long)iter, relres1);
+    }
+    memcpy(x->val, L0->x, sizeof(REAL) * (size_t)n);
+    fflush(stdout);
+    orc_amg_free(m);
+    free(m);
+    return iter;
+}
+
+/* create_precond_amg (helper.c:206-283) / precond_amg (precond.c:164-183). */
+void *orc_precond_create(dCSRmat *A, AMG_param *amgparam) {
+    if (amgparam->smoother != HZ_SMOOTHER_JACOBI && amgparam->smoother != HZ_SMOOTHER_L1DIAG) return NULL;
+    orc_amg *m = (orc_amg *)xcalloc(1, sizeof(orc_amg));
+    if (orc_setup(m, A, amgparam) < 0) { orc_amg_free(m); free(m); return NULL; }
+    orc_prec_data *pd = (orc_prec_data *)xcalloc(1, sizeof(orc_prec_data));
+    orc_prec_data_init(pd, m, amgparam);
+    return pd;
+}
+
+void orc_precond_apply(void *h, const REAL *r, REAL *z) { orc_precond_amg(r, z, h); }
+
+void orc_precond_free(void *h) {
+    orc_prec_data *pd = (orc_prec_data *)h;
+    if (!pd) return;
+    orc_amg_free(pd->m);
+    free(pd->m);
+    free(pd);
 }
 
 INT orc_setup_hierarchy(dCSRmat *A, AMG_param *amgparam, hz_hierarchy *out) {

@@ -58,6 +58,12 @@ INT  orc_aggregate(const dCSRmat *A, INT type, REAL theta, INT max_aggregation,
 REAL orc_tridiag_condest(INT n, const REAL *alpha, const REAL *beta);
 /* Jacobi damping used by the V-cycle (reference: fixed 1.0, smoother.c:47). */
 void orc_set_jacobi_weight(REAL w);
+/* linear_solver_amg + amg_solve (itsolver.c:524-640, amg_solve.c:109-161) */
+INT  orc_linear_solver_amg(dCSRmat *A, dvector *b, dvector *x, AMG_param *param);
+/* create_precond_amg / precond_amg (helper.c:206-283, precond.c:164-183) */
+void *orc_precond_create(dCSRmat *A, AMG_param *amgparam);
+void orc_precond_apply(void *h, const REAL *r, REAL *z);
+void orc_precond_free(void *h);
 
 #ifdef __cplusplus
 }

@@ -37,7 +37,8 @@ def test_struct_layouts_agree_between_header_and_ctypes(tmp_path):
     structs = {"dCSRmat": capi.dCSRmat, "dvector": capi.dvector,
                "linear_itsolver_param": capi.linear_itsolver_param, "AMG_param": capi.AMG_param,
                "hz_options": capi.hz_options, "hz_solve_info": capi.hz_solve_info,
-               "hz_hierarchy": capi.hz_hierarchy, "hz_kernel_times": capi.hz_kernel_times}
+               "hz_hierarchy": capi.hz_hierarchy, "hz_kernel_times": capi.hz_kernel_times,
+               "precond": capi.precond}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "hazmath_mi355x.h"',
              'int main(void){']
     for s, cls in structs.items():

@@ -352,3 +352,83 @@ def test_pcg_parity_large_coarsest(lib, orc, libc):
     (ig, xg), (io, xo) = res["gpu"], res["oracle"]
     assert ig > 0 and abs(ig - io) <= 1
     assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) < 1e-10
+
+
+@pytest.mark.parametrize("agg_type,smoother,coarse_solver", [
+    (capi.HEM, capi.SMOOTHER_L1DIAG, 32), (capi.HEC, capi.SMOOTHER_JACOBI, 32),
+    (capi.VMB, capi.SMOOTHER_JACOBI, 0)])
+def test_linear_solver_amg_parity(lib, orc, libc, agg_type, smoother, coarse_solver):
+    """AMG as the solver (linear_solver_amg / amg_solve) from a nonzero guess:
+    iteration count within +-1 and solution within 1e-10 of the oracle's."""
+    import oracle
+    ia, ja, va = problems.poisson2d_p1(64)
+    n = ia.size - 1
+    b = problems.uniform_rhs(n, seed=6)
+    x0 = np.random.default_rng(1).standard_normal(n) * 0.01
+    A = capi.HostCSR(ia, ja, va)
+    out = {}
+    for which in ("gpu", "oracle"):
+        kw = dict(aggregation=agg_type, smoother=smoother, coarse_dof=60, coarse_solver=coarse_solver)
+        _, amg = (capi.solver_params(lib, **kw) if which == "gpu" else oracle.params(**kw))
+        amg.tol, amg.maxit = 1e-10, 300
+        x = capi.HostVec(x0.copy())
+        bb = capi.HostVec(b)
+        libc.srand(1)
+        f = lib.linear_solver_amg if which == "gpu" else orc.orc_linear_solver_amg
+        out[which] = (f(A.ptr(), bb.ptr(), x.ptr(), C.byref(amg)), x.arr.copy())
+    (ig, xg), (io, xo) = out["gpu"], out["oracle"]
+    assert 0 < io <= 300 and abs(ig - io) <= 1
+    assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) < 1e-10
+
+
+def test_linear_solver_amg_edge_semantics(lib, libc):
+    ia, ja, va = problems.poisson2d_p1(16)
+    n = ia.size - 1
+    A = capi.HostCSR(ia, ja, va)
+    _, amg = capi.solver_params(lib, aggregation=capi.HEM, smoother=capi.SMOOTHER_L1DIAG, coarse_dof=20)
+    amg.tol, amg.maxit = 1e-14, 3
+    x = capi.HostVec(np.zeros(n))
+    assert lib.linear_solver_amg(A.ptr(), capi.HostVec(np.ones(n)).ptr(), x.ptr(), C.byref(amg)) == 4
+    _, amg = capi.solver_params(lib, aggregation=capi.HEM, smoother=capi.SMOOTHER_L1DIAG, coarse_dof=20)
+    amg.tol, amg.maxit = 1e-8, 10
+    x = capi.HostVec(np.full(n, 0.5))
+    assert lib.linear_solver_amg(A.ptr(), capi.HostVec(np.zeros(n)).ptr(), x.ptr(), C.byref(amg)) == 1
+    assert np.all(x.arr == 0.5)
+
+
+def test_precond_object_parity_and_isolation(lib, orc, libc):
+    """create_precond_amg / apply_precond against the oracle's precond_amg; the
+    hierarchy survives other library calls (its own device arena) and the
+    device-array variant gives the same z."""
+    import torch
+    import oracle
+    ia, ja, va = problems.poisson2d_p1(80)
+    n = ia.size - 1
+    A = capi.HostCSR(ia, ja, va)
+    _, amg = capi.solver_params(lib, aggregation=capi.HEC, smoother=capi.SMOOTHER_L1DIAG, coarse_dof=80)
+    libc.srand(1)
+    pc = lib.create_precond_amg(A.ptr(), C.byref(amg))
+    assert pc
+    _, amgo = oracle.params(aggregation=capi.HEC, smoother=capi.SMOOTHER_L1DIAG, coarse_dof=80)
+    libc.srand(1)
+    h = orc.orc_precond_create(A.ptr(), C.byref(amgo))
+    assert amg.strong_coupled == amgo.strong_coupled
+    rng = np.random.default_rng(5)
+    try:
+        for k in range(3):
+            r = rng.standard_normal(n)
+            zg, zo = np.zeros(n), np.zeros(n)
+            lib.apply_precond(capi.dptr(r), capi.dptr(zg), pc)
+            orc.orc_precond_apply(h, capi.dptr(r), capi.dptr(zo))
+            assert np.linalg.norm(zg - zo) / np.linalg.norm(zo) < 1e-12
+            # an unrelated call reuses the library's working memory
+            y = np.zeros(n)
+            assert lib.hz_dcsr_mxv(A.ptr(), capi.dptr(r), capi.dptr(y)) == 0
+        dr = torch.from_numpy(r).cuda()
+        dz = torch.zeros(n, dtype=torch.float64, device="cuda")
+        torch.cuda.synchronize()
+        lib.hz_precond_amg_apply_device(dr.data_ptr(), dz.data_ptr(), pc.contents.data)
+        assert np.array_equal(dz.cpu().numpy(), zg)
+    finally:
+        lib.hz_precond_free(pc)
+        orc.orc_precond_free(h)

@@ -278,3 +278,87 @@ def test_theta_adaptation_mutates_caller_param():
         assert thetas[l + 1] == thetas[l] / 2
     assert amg.strong_coupled == thetas[-1]
     orc.orc_hierarchy_free(C.byref(h))
+
+
+def _amg_param(**kw):
+    _, amg = oracle.params(**kw)
+    return amg
+
+
+def test_oracle_amg_solve_is_stationary_iteration_of_its_preconditioner(orc, libc):
+    """amg_solve (amg_solve.c:109-161) with a linear cycle (Jacobi/L1 sweeps,
+    dense coarsest solve) equals x_{k+1} = x_k + B (b - A x_k) with B the
+    preconditioner (one cycle from zero, precond.c:164-183): same iteration
+    count, same iterate to rounding.  The two are computed by different code
+    paths (cycle from x_k vs. correction from zero)."""
+    ia, ja, va = problems.poisson2d_p1(40)
+    n = ia.size - 1
+    b = problems.uniform_rhs(n, seed=8)
+    A = capi.HostCSR(ia, ja, va)
+    Asp = sp.csr_matrix((va, ja, ia), shape=(n, n))
+    x0 = np.random.default_rng(3).standard_normal(n) * 0.1
+    for agg, sm in ((capi.HEM, capi.SMOOTHER_L1DIAG), (capi.VMB, capi.SMOOTHER_JACOBI)):
+        amg = _amg_param(aggregation=agg, smoother=sm, coarse_dof=50)
+        amg.tol, amg.maxit = 1e-9, 200
+        x = capi.HostVec(x0.copy())
+        libc.srand(1)
+        it = orc.orc_linear_solver_amg(A.ptr(), capi.HostVec(b).ptr(), x.ptr(), C.byref(amg))
+        assert 0 < it <= 200
+        assert np.linalg.norm(b - Asp @ x.arr) / np.linalg.norm(b) < 1e-9
+        # the same iteration through the preconditioner object
+        amg2 = _amg_param(aggregation=agg, smoother=sm, coarse_dof=50)
+        libc.srand(1)
+        h = orc.orc_precond_create(A.ptr(), C.byref(amg2))
+        assert h
+        xs = x0.copy()
+        z = np.zeros(n)
+        k = 0
+        nb = np.linalg.norm(b)
+        while k < 200:
+            k += 1
+            r = b - Asp @ xs
+            orc.orc_precond_apply(h, capi.dptr(np.ascontiguousarray(r)), capi.dptr(z))
+            xs = xs + z
+            if np.linalg.norm(b - Asp @ xs) / nb < 1e-9:
+                break
+        orc.orc_precond_free(h)
+        assert abs(k - it) <= 1
+        assert np.linalg.norm(xs - x.arr) / np.linalg.norm(xs) < 1e-8
+
+
+def test_oracle_amg_solve_reference_edge_semantics(orc, libc):
+    """amg_solve returns MaxIt + 1 when it does not converge and 1 with x
+    untouched when ||b|| is below SMALLREAL (the loop test evaluates ++iter)."""
+    ia, ja, va = problems.poisson2d_p1(16)
+    n = ia.size - 1
+    A = capi.HostCSR(ia, ja, va)
+    amg = _amg_param(aggregation=capi.HEM, smoother=capi.SMOOTHER_L1DIAG, coarse_dof=20)
+    amg.tol, amg.maxit = 1e-14, 3
+    x = capi.HostVec(np.zeros(n))
+    libc.srand(1)
+    assert orc.orc_linear_solver_amg(A.ptr(), capi.HostVec(np.ones(n)).ptr(), x.ptr(), C.byref(amg)) == 4
+    amg = _amg_param(aggregation=capi.HEM, smoother=capi.SMOOTHER_L1DIAG, coarse_dof=20)
+    amg.tol, amg.maxit = 1e-8, 10
+    x = capi.HostVec(np.full(n, 0.5))
+    libc.srand(1)
+    assert orc.orc_linear_solver_amg(A.ptr(), capi.HostVec(np.zeros(n)).ptr(), x.ptr(), C.byref(amg)) == 1
+    assert np.all(x.arr == 0.5)
+
+
+def test_oracle_preconditioner_is_symmetric_positive(orc, libc):
+    """B from a V(1,1) cycle with Jacobi/L1 sweeps and an exact coarse solve is
+    symmetric and positive definite: (B u, v) = (u, B v) and (B u, u) > 0."""
+    ia, ja, va = problems.poisson2d_p1(24)
+    n = ia.size - 1
+    A = capi.HostCSR(ia, ja, va)
+    amg = _amg_param(aggregation=capi.HEC, smoother=capi.SMOOTHER_L1DIAG, coarse_dof=30)
+    libc.srand(1)
+    h = orc.orc_precond_create(A.ptr(), C.byref(amg))
+    rng = np.random.default_rng(0)
+    u, v = rng.standard_normal(n), rng.standard_normal(n)
+    Bu, Bv = np.zeros(n), np.zeros(n)
+    orc.orc_precond_apply(h, capi.dptr(u), capi.dptr(Bu))
+    orc.orc_precond_apply(h, capi.dptr(v), capi.dptr(Bv))
+    orc.orc_precond_free(h)
+    assert abs(Bu @ v - u @ Bv) < 1e-12 * np.linalg.norm(Bu) * np.linalg.norm(v)
+    assert Bu @ u > 0 and Bv @ v > 0
