@@ -44,6 +44,16 @@ def measured_traffic(n, nnz):
 
 AGG = {"hem": capi.HEM, "hec": capi.HEC, "vmb": capi.VMB}
 SMOOTHER = {"jacobi": capi.SMOOTHER_JACOBI, "l1": capi.SMOOTHER_L1DIAG}
+PRECISION = {"fp64": 0, "fp32": 1, "bf16": 2}
+DTYPE = {"fp64": "f64", "fp32": "f32 cycle, f64 PCG", "bf16": "bf16 values/f32 cycle, f64 PCG"}
+
+
+def make_problem(a, m):
+    if a.problem == "coef":
+        ia, ja, va, _ = problems.poisson3d_kuhn_coef(m)
+    else:
+        ia, ja, va = problems.poisson3d_kuhn(m)
+    return ia, ja, va
 
 
 def parse():
@@ -59,6 +69,11 @@ def parse():
     p.add_argument("--theta", type=float, default=0.04)
     p.add_argument("--tol", type=float, default=1e-10)
     p.add_argument("--kernel-reps", type=int, default=50)
+    p.add_argument("--problem", choices=("poisson", "coef"), default="poisson",
+                   help="poisson: -Laplace; coef: kappa piecewise constant on 8^3 blocks, "
+                        "log-uniform in [1e-3, 1e3]")
+    p.add_argument("--precision", choices=PRECISION, default="fp64",
+                   help="arithmetic of the AMG cycle (PCG and coarsest solve stay FP64)")
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--cpu-sample-m", type=int, default=128,
                    help="cells per direction of the CPU-baseline sample")
@@ -115,10 +130,11 @@ def main():
     lib.hz_get_options(C.byref(opts))
     opts.jacobi_weight = a.jacobi_weight
     opts.return_on_fatal = 1
+    opts.vcycle_precision = PRECISION[a.precision]
     lib.hz_set_options(C.byref(opts))
 
     t_gen = time.perf_counter()
-    ia, ja, va = problems.poisson3d_kuhn(a.m)
+    ia, ja, va = make_problem(a, a.m)
     n, nnz = ia.size - 1, int(ia[-1])
     b = problems.uniform_rhs(n, seed=1)
     t_gen = time.perf_counter() - t_gen
@@ -206,7 +222,7 @@ def main():
         if a.cpu_sample_m == a.m:
             ci, cj, cv, cb = ia, ja, va, b
         else:
-            ci, cj, cv = problems.poisson3d_kuhn(a.cpu_sample_m)
+            ci, cj, cv = make_problem(a, a.cpu_sample_m)
             cb = problems.uniform_rhs(ci.size - 1, seed=1)
         print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
         t_cpu, r_cpu, tr, x_cpu = cpu_baseline(a, ci, cj, cv, cb)
@@ -219,6 +235,7 @@ def main():
                "iterations": int(r_cpu), "setup_s": tr.setup_s, "solve_s": tr.solve_s,
                "host_cpu": os.uname().machine, "host_nproc": os.cpu_count()}
         if a.cpu_sample_m == a.m:
+            # error of the GPU solution against the FP64 oracle's
             cpu["x_rel_diff_vs_gpu"] = float(np.linalg.norm(x_cpu - x) / np.linalg.norm(x_cpu))
 
     line = {
@@ -232,14 +249,16 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": DTYPE[a.precision],
         "data": "synthetic",
         "config": {
-            "workload": (f"3-D Kuhn P1 Poisson, {a.m}^3 cells, n={n}, nnz={nnz} (15-point rows, "
+            "workload": (f"3-D Kuhn P1 {'Poisson' if a.problem == 'poisson' else 'diffusion, kappa on 8^3 blocks log-U[1e-3,1e3]'}, "
+                         f"{a.m}^3 cells, n={n}, nnz={nnz} (15-point rows, "
                          f"zeros stored), seeded U[-1,1) RHS, x0=0, UA-AMG ({a.agg.upper()} "
                          f"aggregation, theta={a.theta}, V(1,1) {a.smoother} w={a.jacobi_weight}, "
                          f"coarse_dof={a.coarse_dof}, dense coarsest solve) PCG to {a.tol:g}, "
-                         "plain FP64 drop-in"),
+                         + ("plain FP64 drop-in" if a.precision == "fp64" else
+                            f"{a.precision.upper()} cycle under FP64 PCG")),
             "levels": infos[-1][4],
             "level_rows": infos[-1][6],
             "level_nnz": infos[-1][7],

@@ -79,7 +79,7 @@ class AMG_param(C.Structure):
 
 class hz_options(C.Structure):
     _fields_ = [("jacobi_weight", REAL), ("hec_rng_source", INT), ("verbose_timing", INT),
-                ("return_on_fatal", INT)]
+                ("return_on_fatal", INT), ("vcycle_precision", INT)]
 
 
 class hz_solve_info(C.Structure):

@@ -238,6 +238,17 @@ void amg_prepare_cycle(Hierarchy &H, const AMG_param *param) {
             L.dsm = arena().alloc_n<double>(n);
             smoother_diag(L.S, cp.smoother, L.dsm);
         }
+        if (cp.precision > 0) {
+            L.bf = arena().alloc_n<float>(n);
+            L.xf = arena().alloc_n<float>(n);
+            if (l < H.nl - 1) {
+                L.xtf = arena().alloc_n<float>(n);
+                L.wf = arena().alloc_n<float>(n);
+                L.dsmf = arena().alloc_n<float>(n);
+                d2f(L.dsm, L.dsmf, L.A.n);
+                low_values(L.S, L.A, (cp.precision == 2 && l > 0) ? kLowBF16 : kLowF32, L.low);
+            }
+        }
     }
     (void)n0;
 }
@@ -379,7 +390,140 @@ forward:
 // precond_amg (precond.c:164-183): b0 = r, x0 = 0, maxit cycles, z = x0.
 // When zr_out is given and the last level-0 sweep can produce it, (z, r) is
 // written there and true is returned.
+// ---------------------------------------------------------------------------
+// The same cycle on FP32 vectors and low-precision values (kernels_mixed.hip).
+// ---------------------------------------------------------------------------
+void presmooth_m(Hierarchy &H, int l, CycleState &cs) {
+    Level &L = H.lv[l];
+    const CycleParams &cp = H.cp;
+    const float w = (float)cp.jacobi_w;
+    int sweeps = cp.presmooth;
+    if (cs.zero[l]) {
+        if (sweeps == 0) {
+            fill_f(L.xf, L.A.n);
+        } else if (cs.first_done[l]) {
+            --sweeps;
+        } else {
+            // every level below 0 enters with its first sweep fused into the
+            // restriction, and level 0 in enter_m, so this only zeroes
+            fill_f(L.xf, L.A.n);
+        }
+        cs.zero[l] = false;
+        cs.first_done[l] = false;
+    }
+    for (int s = 0; s < sweeps; ++s) {
+        sweep_m(L.S, L.low, cp.smoother, w, nullptr, nullptr, L.bf, L.xf, L.xtf, nullptr, nullptr, nullptr, H.ws);
+        std::swap(L.xf, L.xtf);
+    }
+}
+
+// Prolongation + post-smoothing of level l; on the final pass over level 0 the
+// last sweep writes z in FP64 (and (z, r) into dot_out).  Returns true when z
+// was written that way.
+bool prolong_postsmooth_m(Hierarchy &H, int l, bool final, double *z, const double *r, double *dot_out) {
+    Level &L = H.lv[l];
+    Level &C = H.lv[l + 1];
+    const CycleParams &cp = H.cp;
+    const float w = (float)cp.jacobi_w;
+    if (cp.postsmooth <= 0) {
+        prolong_m(L.A.n, L.agg, C.xf, L.xf);
+        return false;
+    }
+    for (int s = 0; s < cp.postsmooth; ++s) {
+        const bool last = final && s == cp.postsmooth - 1;
+        const int *agg = nullptr;
+        const float *xc = nullptr;
+        if (s == 0) {
+            if (L.A.n <= fused_down_max_rows()) {
+                agg = L.agg;
+                xc = C.xf;
+            } else {
+                prolong_m(L.A.n, L.agg, C.xf, L.xf);
+            }
+        }
+        sweep_m(L.S, L.low, cp.smoother, w, agg, xc, L.bf, L.xf, L.xtf, last ? z : nullptr, r,
+                last ? dot_out : nullptr, H.ws);
+        if (last) return true;
+        std::swap(L.xf, L.xtf);
+    }
+    return false;
+}
+
+bool mgcycle_m(Hierarchy &H, CycleState &cs, bool final, double *z, const double *r, double *zr_out) {
+    const CycleParams &cp = H.cp;
+    const int nl = H.nl;
+    const float w = (float)cp.jacobi_w;
+    bool z_written = false;
+    int l = 0;
+    for (int i = 0; i < HZ_MAX_AMG_LVL; ++i) cs.num_lvl[i] = 0;
+forward:
+    while (l < nl - 1) {
+        Level &L = H.lv[l];
+        Level &C = H.lv[l + 1];
+        cs.num_lvl[l]++;
+        presmooth_m(H, l, cs);
+        const bool fuse = (l + 1 < nl - 1) && cp.presmooth > 0;
+        if (L.A.n <= fused_down_max_rows()) {
+            down_fused_m(L.nagg, L.rptr, L.rmem, L.A, L.low, L.bf, L.xf, C.bf, cp.smoother, w,
+                         fuse ? C.dsmf : nullptr, fuse ? C.xf : nullptr);
+        } else {
+            residual_m(L.S, L.low, L.bf, L.xf, L.wf);
+            restrict_m(L.nagg, L.rptr, L.rmem, L.wf, C.bf, cp.smoother, w, fuse ? C.dsmf : nullptr,
+                       fuse ? C.xf : nullptr);
+        }
+        ++l;
+        cs.zero[l] = true;
+        cs.first_done[l] = fuse;
+    }
+    {
+        Level &C = H.lv[nl - 1];
+        if (cp.coarse_solver == HZ_SOLVER_UMFPACK) {
+            coarse_dense_solve_m(C.A.n, C.Ainv, C.bf, C.xf);
+        } else {
+            f2d(C.bf, C.b, C.A.n);
+            fill(C.x, 0.0, C.A.n);
+            coarse_cg(C.S, C.b, C.x, cp.coarse_tol, 20 * C.A.n, C.cgwork);
+            d2f(C.x, C.xf, C.A.n);
+        }
+        cs.zero[nl - 1] = false;
+    }
+    while (l > 0) {
+        --l;
+        if (prolong_postsmooth_m(H, l, final && l == 0, z, r, zr_out)) z_written = true;
+        if (cs.num_lvl[l] < cp.cycle_type) break;
+        cs.num_lvl[l] = 0;
+    }
+    if (l > 0) goto forward;
+    return z_written;
+}
+
+bool amg_precond_m(Hierarchy &H, const double *r, double *z, double *zr_out) {
+    Level &L0 = H.lv[0];
+    const CycleParams &cp = H.cp;
+    const int n = L0.A.n;
+    if (cp.maxit <= 0) {
+        fill(z, 0.0, n);
+        return false;
+    }
+    enter_m(n, r, cp.smoother, (float)cp.jacobi_w, L0.dsmf, L0.bf, cp.presmooth > 0 ? L0.xf : nullptr);
+    CycleState cs;
+    for (int i = 0; i < HZ_MAX_AMG_LVL; ++i) {
+        cs.zero[i] = false;
+        cs.first_done[i] = false;
+    }
+    cs.zero[0] = true;
+    cs.first_done[0] = cp.presmooth > 0;
+    bool z_written = false;
+    for (int it = 0; it < cp.maxit; ++it) z_written = mgcycle_m(H, cs, it == cp.maxit - 1, z, r, zr_out);
+    if (!z_written) {
+        f2d(L0.xf, z, n);
+        return false;
+    }
+    return zr_out != nullptr;
+}
+
 bool amg_precond(Hierarchy &H, const double *r, double *z, double *zr_out) {
+    if (H.cp.precision > 0 && H.nl > 1) return amg_precond_m(H, r, z, zr_out);
     Level &L0 = H.lv[0];
     const CycleParams &cp = H.cp;
     const int n = L0.A.n;

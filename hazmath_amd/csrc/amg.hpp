@@ -19,6 +19,10 @@ struct Level {
     double theta = 0.0;       // strong_coupled used to aggregate this level
     double *Ainv = nullptr;   // coarsest, coarse_solver 32
     double *cgwork = nullptr; // coarsest, coarse_solver 0
+    // FP32 cycle (CycleParams::precision > 0)
+    LowMatrix low;            // values in FP32 or BF16 (non-coarsest levels)
+    float *dsmf = nullptr;    // smoother diagonal rounded to FP32
+    float *bf = nullptr, *xf = nullptr, *xtf = nullptr, *wf = nullptr;
 };
 
 struct CycleParams {
@@ -30,6 +34,7 @@ struct CycleParams {
     double coarse_tol = 1e-8;   // param_amg_init tol 1e-6 times 1e-2 (mgcycle.c:255)
     double jacobi_w = 1.0;
     int maxit = 1;              // cycles per preconditioner application
+    int precision = 0;          // 0 FP64 cycle; 1 FP32 cycle; 2 FP32 cycle, BF16 values below level 0
 };
 
 struct Hierarchy {

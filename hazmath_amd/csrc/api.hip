@@ -11,7 +11,7 @@ using namespace hz;
 
 namespace {
 
-hz_options g_opt = {1.0, 0, 0, 0};
+hz_options g_opt = {1.0, 0, 0, 0, 0};
 hz_solve_info g_info;
 
 bool device_ready() {
@@ -86,6 +86,14 @@ int check_supported(const linear_itsolver_param *it, const AMG_param *amg) {
         std::printf("### ERROR: cycle type %lld is not available in this build\n", (long long)amg->cycle_type);
         return HZ_ERROR_INPUT_PAR;
     }
+    if (g_opt.vcycle_precision < 0 || g_opt.vcycle_precision > 2) {
+        std::printf("### ERROR: vcycle_precision %lld is not 0, 1 or 2\n", (long long)g_opt.vcycle_precision);
+        return HZ_ERROR_INPUT_PAR;
+    }
+    if (g_opt.vcycle_precision > 0 && amg->coarse_scaling == 1) {
+        std::printf("### ERROR: coarse scaling needs the FP64 cycle (vcycle_precision = 0)\n");
+        return HZ_ERROR_INPUT_PAR;
+    }
     if (amg->Schwarz_levels > 0) {
         std::printf("### ERROR: Schwarz smoothing is not available in this build\n");
         return HZ_ERROR_INPUT_PAR;
@@ -112,6 +120,7 @@ INT core_solve(const DevCSR &A, const double *db, double *dx, linear_itsolver_pa
     st = amg_setup(H, A, amg, false);
     if (st <= -1000) return fatal(st);
     if (st < 0) return st;
+    H.cp.precision = g_opt.vcycle_precision;
     amg_prepare_cycle(H, amg);
     H.cp.jacobi_w = g_opt.jacobi_weight;
     HZ_CHECK(hipEventRecord(e1, stream()));
@@ -675,6 +684,7 @@ precond *create_precond_amg(dCSRmat *A, AMG_param *amgparam) {
             delete P;
             return nullptr;
         }
+        P->H.cp.precision = g_opt.vcycle_precision;
         amg_prepare_cycle(P->H, amgparam);
         P->H.cp.jacobi_w = g_opt.jacobi_weight;
         P->n = A->row;

@@ -123,6 +123,34 @@ bool dense_mfma_active();
 // Single-workgroup CG for coarse_solver = 0 (coarse_itsolver, mgcycle.c:34-53).
 void coarse_cg(const DevSELL &S, const double *b, double *x, double tol, int maxit, double *work);
 
+// ---- kernels_mixed.hip: the V-cycle in FP32 / BF16 ------------------------
+constexpr int kLowF32 = 1, kLowBF16 = 2;
+// Low-precision copies of a level's values: SELL-64 order (sharing DevSELL's
+// column indices) and CSR order (sharing DevCSR's), FP32 or BF16.
+struct LowMatrix {
+    int kind = 0;
+    void *sell = nullptr;
+    void *csr = nullptr;
+};
+void low_values(const DevSELL &S, const DevCSR &A, int kind, LowMatrix &out);
+void d2f(const double *a, float *b, long n);
+void f2d(const float *a, double *b, long n);
+void fill_f(float *x, long n);
+// b = fl(r); x = first sweep from zero when x != null
+void enter_m(int n, const double *r, int smoother, float w, const float *d, float *b, float *x);
+// xout = sweep(x [+ P xc when agg != null]); with out64: writes the result in
+// FP64 to out64 instead and sum_i out64_i r64_i to dot_out.
+void sweep_m(const DevSELL &S, const LowMatrix &M, int smoother, float w, const int *agg, const float *xc,
+             const float *b, const float *x, float *xout, double *out64, const double *r64, double *dot_out,
+             const ReduceWs &ws);
+void residual_m(const DevSELL &S, const LowMatrix &M, const float *b, const float *x, float *w);
+void restrict_m(int nc, const int *rptr, const int *rmem, const float *w, float *bc, int smoother, float wgt,
+                const float *dc, float *xc);
+void down_fused_m(int nc, const int *rptr, const int *rmem, const DevCSR &A, const LowMatrix &M, const float *b,
+                  const float *x, float *bc, int smoother, float wgt, const float *dc, float *xc);
+void prolong_m(int n, const int *agg, const float *xc, float *x);
+void coarse_dense_solve_m(int n, const double *Ainv, const float *b, float *x);
+
 // ---- HEC turn order -------------------------------------------------------
 // Turn order of HEC from libc rand() (iarray_shuffle, array.c:302-318).
 void hec_turn_order(int n, int *perm_dev, int rng_source);

@@ -1,0 +1,417 @@
+// kernels_mixed.hip -- the V-cycle in FP32 (matrix values FP32, or BF16 on
+// coarse levels) under an FP64 PCG.
+//
+// The hierarchy is the FP64 one (same aggregates, same Galerkin products);
+// only the values the cycle streams are stored again in low precision, sharing
+// the SELL-64 column indices.  Vectors of the cycle are FP32 and every sum is
+// accumulated in FP32.  The preconditioner's input r and output z stay FP64:
+// the first kernel of a cycle rounds r to FP32 and the last post-smoothing
+// sweep of level 0 writes z in FP64 together with the partials of (z, r), so
+// the PCG sees the same interface as with the FP64 cycle.  The coarsest solve
+// uses the FP64 inverse (FP32 in and out, FP64 accumulation).
+#include <algorithm>
+
+#include "hz_internal.h"
+
+namespace hz {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kSlicesPerBlock = kBlock / kWave;
+constexpr int kFinBlock = 1024;
+constexpr int kStreamGrid = 2048;
+constexpr float kSmallRealF = 1e-20f;
+
+inline int grid_for(long n) {
+    long g = (n + kBlock - 1) / kBlock;
+    return (int)std::max(1L, std::min<long>(g, kStreamGrid));
+}
+
+// bf16 stored as the upper half of the FP32 bit pattern
+__device__ __forceinline__ float ldv(const float *p) { return *p; }
+__device__ __forceinline__ float ldv(const unsigned short *p) {
+    return __uint_as_float(((unsigned int)*p) << 16);
+}
+
+__device__ __forceinline__ unsigned short f2bf(float f) {
+    const unsigned int u = __float_as_uint(f);
+    const unsigned int r = u + 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even
+    return (unsigned short)(r >> 16);
+}
+
+__global__ void k_vals_f32(long long n, const double *__restrict__ v, float *__restrict__ o) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        o[i] = (float)v[i];
+}
+
+__global__ void k_vals_bf16(long long n, const double *__restrict__ v, unsigned short *__restrict__ o) {
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        o[i] = f2bf((float)v[i]);
+}
+
+__global__ void k_d2f(long n, const double *__restrict__ a, float *__restrict__ b) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        b[i] = (float)a[i];
+}
+
+__global__ void k_f2d(long n, const float *__restrict__ a, double *__restrict__ b) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        b[i] = (double)a[i];
+}
+
+__global__ void k_fill_f(float *__restrict__ x, long n) {
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        x[i] = 0.0f;
+}
+
+__device__ __forceinline__ float zero_sweep(int smoother, float w, float b, float d) {
+    if (fabsf(d) > kSmallRealF) return smoother == HZ_SMOOTHER_L1DIAG ? b / d : w * b / d;
+    return 0.0f;
+}
+
+// b = fl(r); optionally x = first sweep from zero
+__global__ void k_enter(int n, const double *__restrict__ r, int smoother, float w, const float *__restrict__ d,
+                        float *__restrict__ b, float *__restrict__ x) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float bi = (float)r[i];
+        b[i] = bi;
+        if (x) x[i] = zero_sweep(smoother, w, bi, d[i]);
+    }
+}
+
+// One sweep (Jacobi with weight w / L1DIAG) in FP32, optionally with the
+// prolongation x + P xc folded into every x read, optionally writing the
+// result in FP64 (out64) with block partials of sum_i out64_i r64_i.
+template <int kSmoother, bool kProlong, bool kOut64, class MT>
+__global__ __launch_bounds__(kBlock) void k_sweep_m(int n, int nslices, const long long *__restrict__ sptr,
+                                                    const int *__restrict__ col, const MT *__restrict__ val,
+                                                    float wgt, const int *__restrict__ agg,
+                                                    const float *__restrict__ xc, const float *__restrict__ b,
+                                                    const float *__restrict__ x, float *__restrict__ xout,
+                                                    double *__restrict__ out64, const double *__restrict__ r64,
+                                                    double *__restrict__ partials) {
+    __shared__ double sh[kBlock / kWave];
+    const int blk = xcd_block_id();
+    const int s = blk * kSlicesPerBlock + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    const int r = s * kSlice + lane;
+    double prod = 0.0;
+    if (s < nslices) {
+        const long long base = sptr[s];
+        const int wd = (int)((sptr[s + 1] - base) / kSlice);
+        const int *c = col + base + lane;
+        const MT *v = val + base + lane;
+        float t = (r < n) ? b[r] : 0.0f;
+        float d = 0.0f;
+        int k = 0;
+        for (; k + 4 <= wd; k += 4) {
+            int cc[4];
+            float vv[4], xx[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                cc[u] = c[(k + u) * kSlice];
+                vv[u] = ldv(v + (k + u) * kSlice);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) xx[u] = cc[u] >= 0 ? x[cc[u]] : 0.0f;
+            if (kProlong) {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int g = cc[u] >= 0 ? agg[cc[u]] : HZ_UNPT;
+                    if (g > HZ_UNPT) xx[u] = xx[u] + xc[g];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (kSmoother == HZ_SMOOTHER_L1DIAG) {
+                    if (cc[u] >= 0) {
+                        t = t - vv[u] * xx[u];
+                        d = d + fabsf(vv[u]);
+                    }
+                } else {
+                    if (cc[u] == r) d = vv[u];
+                    else if (cc[u] >= 0) t = t - vv[u] * xx[u];
+                }
+            }
+        }
+        for (; k < wd; ++k) {
+            const int cj = c[k * kSlice];
+            const float a = ldv(v + k * kSlice);
+            float xj = cj >= 0 ? x[cj] : 0.0f;
+            if (kProlong && cj >= 0) {
+                const int g = agg[cj];
+                if (g > HZ_UNPT) xj = xj + xc[g];
+            }
+            if (kSmoother == HZ_SMOOTHER_L1DIAG) {
+                if (cj >= 0) {
+                    t = t - a * xj;
+                    d = d + fabsf(a);
+                }
+            } else {
+                if (cj == r) d = a;
+                else if (cj >= 0) t = t - a * xj;
+            }
+        }
+        if (r < n) {
+            float xi = x[r];
+            if (kProlong) {
+                const int g = agg[r];
+                if (g > HZ_UNPT) xi = xi + xc[g];
+            }
+            float xn = xi;
+            if (fabsf(d) > kSmallRealF) {
+                if (kSmoother == HZ_SMOOTHER_L1DIAG) xn = xi + t / d;
+                else xn = (1.0f - wgt) * xi + wgt * t / d;
+            }
+            if (kOut64) {
+                const double z = (double)xn;
+                out64[r] = z;
+                prod = z * r64[r];
+            } else {
+                xout[r] = xn;
+            }
+        }
+    }
+    if (kOut64) {
+        const double bs = block_sum<kBlock>(prod, sh);
+        if (threadIdx.x == 0) partials[blk] = bs;
+    }
+}
+
+template <class MT>
+__global__ __launch_bounds__(kBlock) void k_residual_m(int n, int nslices, const long long *__restrict__ sptr,
+                                                       const int *__restrict__ col, const MT *__restrict__ val,
+                                                       const float *__restrict__ b, const float *__restrict__ x,
+                                                       float *__restrict__ w) {
+    const int s = xcd_block_id() * kSlicesPerBlock + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (s >= nslices) return;
+    const int r = s * kSlice + lane;
+    const long long base = sptr[s];
+    const int wd = (int)((sptr[s + 1] - base) / kSlice);
+    const int *c = col + base + lane;
+    const MT *v = val + base + lane;
+    float acc = 0.0f;
+    int k = 0;
+    for (; k + 4 <= wd; k += 4) {
+        int cc[4];
+        float vv[4], xx[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            cc[u] = c[(k + u) * kSlice];
+            vv[u] = ldv(v + (k + u) * kSlice);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) xx[u] = cc[u] >= 0 ? x[cc[u]] : 0.0f;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (cc[u] >= 0) acc = acc + vv[u] * xx[u];
+    }
+    for (; k < wd; ++k) {
+        const int cj = c[k * kSlice];
+        if (cj >= 0) acc = acc + ldv(v + k * kSlice) * x[cj];
+    }
+    if (r < n) w[r] = b[r] - acc;
+}
+
+__global__ void k_restrict_m(int nc, const int *__restrict__ rptr, const int *__restrict__ rmem,
+                             const float *__restrict__ w, float *__restrict__ bc, int smoother, float wgt,
+                             const float *__restrict__ dc, float *__restrict__ xc) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+        float s = 0.0f;
+        for (int q = rptr[c]; q < rptr[c + 1]; ++q) s = s + w[rmem[q]];
+        bc[c] = s;
+        if (xc) xc[c] = zero_sweep(smoother, wgt, s, dc[c]);
+    }
+}
+
+// residual + restriction (+ coarse zero-iterate sweep), one thread per coarse
+// row, for latency-bound levels (values read from the CSR copy).
+template <class MT>
+__global__ void k_down_fused_m(int nc, const int *__restrict__ rptr, const int *__restrict__ rmem,
+                               const int *__restrict__ ia, const int *__restrict__ ja, const MT *__restrict__ a,
+                               const float *__restrict__ b, const float *__restrict__ x, float *__restrict__ bc,
+                               int smoother, float wgt, const float *__restrict__ dc, float *__restrict__ xc) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+        float s = 0.0f;
+        for (int q = rptr[c]; q < rptr[c + 1]; ++q) {
+            const int m = rmem[q];
+            float t = 0.0f;
+            for (int k = ia[m]; k < ia[m + 1]; ++k) t = t + ldv(a + k) * x[ja[k]];
+            s = s + (b[m] - t);
+        }
+        bc[c] = s;
+        if (xc) xc[c] = zero_sweep(smoother, wgt, s, dc[c]);
+    }
+}
+
+__global__ void k_prolong_m(int n, const int *__restrict__ agg, const float *__restrict__ xc,
+                            float *__restrict__ x) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int g = agg[i];
+        if (g > HZ_UNPT) x[i] = x[i] + xc[g];
+    }
+}
+
+// x = Ainv b with FP64 Ainv and accumulation, FP32 b and x; a wavefront per row
+__global__ __launch_bounds__(256) void k_gemv_m(int n, const double *__restrict__ Ainv,
+                                                const float *__restrict__ b, float *__restrict__ x) {
+    const int row = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (row >= n) return;
+    double s = 0.0;
+    const double *a = Ainv + (size_t)row * n;
+    for (int k = lane; k < n; k += kWave) s += a[k] * (double)b[k];
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, kWave);
+    if (lane == 0) x[row] = (float)s;
+}
+
+__global__ __launch_bounds__(kFinBlock) void k_final_sum_m(const double *__restrict__ partials, int G,
+                                                           double *__restrict__ out) {
+    __shared__ double sh[kFinBlock / kWave];
+    const double t = final_sum<kFinBlock>(partials, G, sh);
+    if (threadIdx.x == 0) *out = t;
+}
+
+template <int kSm, bool kPro, bool kOut, class MT>
+void launch_sweep_m(const DevSELL &S, const MT *val, float w, const int *agg, const float *xc, const float *b,
+                    const float *x, float *xout, double *out64, const double *r64, double *partials) {
+    const int grid = div_up(S.nslices, kSlicesPerBlock);
+    k_sweep_m<kSm, kPro, kOut, MT><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col, val, w, agg,
+                                                                   xc, b, x, xout, out64, r64, partials);
+    HZ_LAUNCH_CHECK();
+}
+
+template <int kSm, class MT>
+void dispatch_sweep_m(const DevSELL &S, const MT *val, float w, const int *agg, const float *xc,
+                      const float *b, const float *x, float *xout, double *out64, const double *r64,
+                      double *partials) {
+    if (agg) {
+        if (out64) launch_sweep_m<kSm, true, true>(S, val, w, agg, xc, b, x, xout, out64, r64, partials);
+        else launch_sweep_m<kSm, true, false>(S, val, w, agg, xc, b, x, xout, out64, r64, partials);
+    } else {
+        if (out64) launch_sweep_m<kSm, false, true>(S, val, w, agg, xc, b, x, xout, out64, r64, partials);
+        else launch_sweep_m<kSm, false, false>(S, val, w, agg, xc, b, x, xout, out64, r64, partials);
+    }
+}
+
+}  // namespace
+
+void low_values(const DevSELL &S, const DevCSR &A, int kind, LowMatrix &out) {
+    out.kind = kind;
+    const long long ne = S.nent, nz = A.nnz;
+    const int gs = (int)std::max(1LL, std::min(8192LL, (ne + kBlock - 1) / kBlock));
+    const int gc = (int)std::max(1LL, std::min(8192LL, (nz + kBlock - 1) / kBlock));
+    if (kind == kLowF32) {
+        float *s = arena().alloc_n<float>((size_t)ne + 1);
+        float *c = arena().alloc_n<float>((size_t)nz + 1);
+        if (ne) k_vals_f32<<<gs, kBlock, 0, stream()>>>(ne, S.val, s);
+        if (nz) k_vals_f32<<<gc, kBlock, 0, stream()>>>(nz, A.val, c);
+        out.sell = s;
+        out.csr = c;
+    } else {
+        unsigned short *s = arena().alloc_n<unsigned short>((size_t)ne + 1);
+        unsigned short *c = arena().alloc_n<unsigned short>((size_t)nz + 1);
+        if (ne) k_vals_bf16<<<gs, kBlock, 0, stream()>>>(ne, S.val, s);
+        if (nz) k_vals_bf16<<<gc, kBlock, 0, stream()>>>(nz, A.val, c);
+        out.sell = s;
+        out.csr = c;
+    }
+    HZ_LAUNCH_CHECK();
+}
+
+void d2f(const double *a, float *b, long n) {
+    if (n <= 0) return;
+    k_d2f<<<grid_for(n), kBlock, 0, stream()>>>(n, a, b);
+    HZ_LAUNCH_CHECK();
+}
+
+void f2d(const float *a, double *b, long n) {
+    if (n <= 0) return;
+    k_f2d<<<grid_for(n), kBlock, 0, stream()>>>(n, a, b);
+    HZ_LAUNCH_CHECK();
+}
+
+void fill_f(float *x, long n) {
+    if (n <= 0) return;
+    k_fill_f<<<grid_for(n), kBlock, 0, stream()>>>(x, n);
+    HZ_LAUNCH_CHECK();
+}
+
+void enter_m(int n, const double *r, int smoother, float w, const float *d, float *b, float *x) {
+    if (n <= 0) return;
+    k_enter<<<grid_for(n), kBlock, 0, stream()>>>(n, r, smoother, w, d, b, x);
+    HZ_LAUNCH_CHECK();
+}
+
+void sweep_m(const DevSELL &S, const LowMatrix &M, int smoother, float w, const int *agg, const float *xc,
+             const float *b, const float *x, float *xout, double *out64, const double *r64, double *dot_out,
+             const ReduceWs &ws) {
+    if (S.n == 0) return;
+    double *partials = out64 ? ws.partials : nullptr;
+    if (M.kind == kLowF32) {
+        const float *v = static_cast<const float *>(M.sell);
+        if (smoother == HZ_SMOOTHER_L1DIAG)
+            dispatch_sweep_m<HZ_SMOOTHER_L1DIAG>(S, v, w, agg, xc, b, x, xout, out64, r64, partials);
+        else
+            dispatch_sweep_m<HZ_SMOOTHER_JACOBI>(S, v, w, agg, xc, b, x, xout, out64, r64, partials);
+    } else {
+        const unsigned short *v = static_cast<const unsigned short *>(M.sell);
+        if (smoother == HZ_SMOOTHER_L1DIAG)
+            dispatch_sweep_m<HZ_SMOOTHER_L1DIAG>(S, v, w, agg, xc, b, x, xout, out64, r64, partials);
+        else
+            dispatch_sweep_m<HZ_SMOOTHER_JACOBI>(S, v, w, agg, xc, b, x, xout, out64, r64, partials);
+    }
+    if (out64 && dot_out) {
+        k_final_sum_m<<<1, kFinBlock, 0, stream()>>>(ws.partials, div_up(S.nslices, kSlicesPerBlock), dot_out);
+        HZ_LAUNCH_CHECK();
+    }
+}
+
+void residual_m(const DevSELL &S, const LowMatrix &M, const float *b, const float *x, float *w) {
+    if (S.n == 0) return;
+    const int grid = div_up(S.nslices, kSlicesPerBlock);
+    if (M.kind == kLowF32)
+        k_residual_m<float><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col,
+                                                           static_cast<const float *>(M.sell), b, x, w);
+    else
+        k_residual_m<unsigned short><<<grid, kBlock, 0, stream()>>>(
+            S.n, S.nslices, S.sptr, S.col, static_cast<const unsigned short *>(M.sell), b, x, w);
+    HZ_LAUNCH_CHECK();
+}
+
+void restrict_m(int nc, const int *rptr, const int *rmem, const float *w, float *bc, int smoother, float wgt,
+                const float *dc, float *xc) {
+    if (nc <= 0) return;
+    k_restrict_m<<<grid_for(nc), kBlock, 0, stream()>>>(nc, rptr, rmem, w, bc, smoother, wgt, dc, xc);
+    HZ_LAUNCH_CHECK();
+}
+
+void down_fused_m(int nc, const int *rptr, const int *rmem, const DevCSR &A, const LowMatrix &M, const float *b,
+                  const float *x, float *bc, int smoother, float wgt, const float *dc, float *xc) {
+    if (nc <= 0) return;
+    if (M.kind == kLowF32)
+        k_down_fused_m<float><<<grid_for(nc), kBlock, 0, stream()>>>(
+            nc, rptr, rmem, A.ia, A.ja, static_cast<const float *>(M.csr), b, x, bc, smoother, wgt, dc, xc);
+    else
+        k_down_fused_m<unsigned short><<<grid_for(nc), kBlock, 0, stream()>>>(
+            nc, rptr, rmem, A.ia, A.ja, static_cast<const unsigned short *>(M.csr), b, x, bc, smoother, wgt, dc,
+            xc);
+    HZ_LAUNCH_CHECK();
+}
+
+void prolong_m(int n, const int *agg, const float *xc, float *x) {
+    if (n <= 0) return;
+    k_prolong_m<<<grid_for(n), kBlock, 0, stream()>>>(n, agg, xc, x);
+    HZ_LAUNCH_CHECK();
+}
+
+void coarse_dense_solve_m(int n, const double *Ainv, const float *b, float *x) {
+    if (n <= 0) return;
+    k_gemv_m<<<div_up(n, 4), 256, 0, stream()>>>(n, Ainv, b, x);
+    HZ_LAUNCH_CHECK();
+}
+
+}  // namespace hz

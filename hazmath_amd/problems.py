@@ -13,6 +13,9 @@ ordered lexicographically (x fastest), Dirichlet nodes eliminated.
   along the face and body diagonals of the Kuhn split); the zeros ARE stored
   (15-point rows), as an assembled FE matrix holds them (m = 128 gives
   n = 2,048,383, nnz = 30,340,171).
+* ``poisson3d_kuhn_coef(m, nblocks, seed)``: -div(kappa grad u) on the same
+  mesh with kappa piecewise constant on nblocks^3 blocks, log-uniform in
+  [1e-3, 1e3] (same pattern as poisson3d_kuhn).
 * ``uniform_rhs(n, seed)``: f_i in [-1, 1) from splitmix64(seed XOR i), the
   same bits in every language.
 """
@@ -41,8 +44,9 @@ def uniform_rhs(n: int, seed: int = 1) -> np.ndarray:
 
 
 def _stencil_csr(shape, offsets, values):
-    """CSR of a constant stencil on the interior grid `shape` (z, y, x order),
-    offsets given as (dz, dy, dx) tuples; entries outside the grid are dropped.
+    """CSR of a stencil on the interior grid `shape` (z, y, x order), offsets
+    given as (dz, dy, dx) tuples with a constant or per-row value array each;
+    entries outside the grid are dropped.
     Offsets are sorted by their linear index offset, so columns ascend."""
     dims = len(shape)
     nz, ny, nx = (1,) * (3 - dims) + tuple(shape)
@@ -57,7 +61,7 @@ def _stencil_csr(shape, offsets, values):
     z = idx // (nx * ny)
     cols = np.empty((len(offsets), n), dtype=np.int64)
     valid = np.empty((len(offsets), n), dtype=bool)
-    vals = np.empty((len(offsets), n), dtype=np.float64)
+    vals = np.empty((len(offsets), n), dtype=np.float64)   # a value per offset, or per row
     for k, (dz, dy, dx) in enumerate(offsets):
         xx, yy, zz = x + dx, y + dy, z + dz
         valid[k] = (xx >= 0) & (xx < nx) & (yy >= 0) & (yy < ny) & (zz >= 0) & (zz < nz)
@@ -100,6 +104,69 @@ def poisson3d_kuhn(m: int = 128):
             offs.append(tuple(s * a for a in d))
             vals.append(0.0)
     return _stencil_csr((k, k, k), offs, vals)
+
+
+# 6 x the P1 stiffness of one unit cube split into the 6 Kuhn tetrahedra
+# (vertex v = x + 2y + 4z); kuhn_element_stiffness_check() derives it from the
+# tetrahedra and the tests compare the two.
+KUHN_CUBE_K6 = np.array([
+    [6, -2, -2, 0, -2, 0, 0, 0],
+    [-2, 4, 0, -1, 0, -1, 0, 0],
+    [-2, 0, 4, -1, 0, 0, -1, 0],
+    [0, -1, -1, 4, 0, 0, 0, -2],
+    [-2, 0, 0, 0, 4, -1, -1, 0],
+    [0, -1, 0, 0, -1, 4, 0, -2],
+    [0, 0, -1, 0, -1, 0, 4, -2],
+    [0, 0, 0, -2, 0, -2, -2, 6]], dtype=np.float64)
+
+
+def block_coefficients(nblocks: int = 8, seed: int = 1, log10_lo: float = -3.0,
+                       log10_hi: float = 3.0) -> np.ndarray:
+    """kappa on nblocks^3 blocks, log-uniform in [10^lo, 10^hi]:
+    kappa_b = 10^(lo + (hi - lo) u_b), u_b from splitmix64(seed ^ b), b = bx +
+    nb (by + nb bz).  Returned as an array indexed [bz, by, bx]."""
+    nb = nblocks
+    b = np.arange(nb ** 3, dtype=np.uint64)
+    u = (splitmix64(np.uint64(seed) ^ b) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    return (10.0 ** (log10_lo + (log10_hi - log10_lo) * u)).reshape(nb, nb, nb)
+
+
+def poisson3d_kuhn_coef(m: int = 128, nblocks: int = 8, seed: int = 1, log10_lo: float = -3.0,
+                        log10_hi: float = 3.0):
+    """-div(kappa grad u) on the unit cube, P1 on the m^3 Kuhn-split cubes, kappa
+    piecewise constant on nblocks^3 blocks of cubes (block_coefficients).  Same
+    15-point pattern as poisson3d_kuhn (the Kuhn-diagonal couplings stay exact
+    zeros: they vanish cube by cube); entry (p, p+d) = h/6 sum over the cubes
+    holding both nodes of kappa_cube * K6[corner(p), corner(p+d)].  With kappa = 1
+    it equals poisson3d_kuhn(m).  Returns (ia, ja, val, kappa_blocks)."""
+    k = m - 1
+    h = 1.0 / m
+    kb = block_coefficients(nblocks, seed, log10_lo, log10_hi)
+    cell_block = (np.arange(m) * nblocks) // m
+    kc = kb[np.ix_(cell_block, cell_block, cell_block)]      # [Z, Y, X] per cube
+    offs = [(0, 0, 0)]
+    for ax in [(0, 0, 1), (0, 1, 0), (1, 0, 0)]:
+        for sgn in (1, -1):
+            offs.append(tuple(sgn * a for a in ax))
+    for d in KUHN_DIAGONALS:
+        for sgn in (1, -1):
+            offs.append(tuple(sgn * a for a in d))
+    vals = []
+    for (dz, dy, dx) in offs:
+        acc = np.zeros((k, k, k))
+        for cz in (0, 1):
+            for cy in (0, 1):
+                for cx in (0, 1):
+                    ex, ey, ez = cx + dx, cy + dy, cz + dz
+                    if not (0 <= ex <= 1 and 0 <= ey <= 1 and 0 <= ez <= 1):
+                        continue
+                    w = KUHN_CUBE_K6[cx + 2 * cy + 4 * cz, ex + 2 * ey + 4 * ez]
+                    if w == 0.0:
+                        continue
+                    acc += w * kc[1 - cz:m - cz, 1 - cy:m - cy, 1 - cx:m - cx]
+        vals.append((h * acc.reshape(-1)) / 6.0)
+    ia, ja, va = _stencil_csr((k, k, k), offs, vals)
+    return ia, ja, va, kb
 
 
 def kuhn_element_stiffness_check():

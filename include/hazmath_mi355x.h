@@ -224,6 +224,13 @@ typedef struct {
     INT  return_on_fatal;  /* 0: exit() like the reference on its fatal paths
                               (exit(16)/exit(17)/check_error); 1: return the negative
                               code (-1016/-1017/-1013) instead. */
+    INT  vcycle_precision; /* arithmetic of the AMG preconditioner's cycle; the PCG
+                              (residuals, dot products, SpMV) and the coarsest solve
+                              stay FP64 in every mode:
+                              0: FP64 (the reference's);
+                              1: FP32 vectors and matrix values, FP32 sums;
+                              2: as 1 with BF16 matrix values on levels below the finest.
+                              Modes 1 and 2 need coarse_scaling = OFF. */
 } hz_options;
 
 void hz_get_options(hz_options *opt);

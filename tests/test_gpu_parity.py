@@ -432,3 +432,82 @@ def test_precond_object_parity_and_isolation(lib, orc, libc):
     finally:
         lib.hz_precond_free(pc)
         orc.orc_precond_free(h)
+
+
+def _set_precision(lib, p):
+    opts = capi.hz_options()
+    lib.hz_get_options(C.byref(opts))
+    old = opts.vcycle_precision
+    opts.vcycle_precision = p
+    lib.hz_set_options(C.byref(opts))
+    return old
+
+
+@pytest.mark.parametrize("precision,ztol", [(1, 2e-5), (2, 5e-2)])
+def test_mixed_precision_cycle_vs_fp64_cycle(lib, libc, precision, ztol):
+    """One preconditioner application with the FP32 cycle (FP32 or BF16
+    values) against the FP64 cycle on the same hierarchy: relative difference
+    within the storage precision's reach (FP32: 2e-5, BF16 values: 5e-2)."""
+    ia, ja, va, _ = problems.poisson3d_kuhn_coef(24)
+    n = ia.size - 1
+    A = capi.HostCSR(ia, ja, va)
+    r = np.random.default_rng(2).standard_normal(n)
+    z = {}
+    for p in (0, precision):
+        old = _set_precision(lib, p)
+        try:
+            _, amg = capi.solver_params(lib, aggregation=capi.HEC, smoother=capi.SMOOTHER_L1DIAG, coarse_dof=100)
+            libc.srand(1)
+            pc = lib.create_precond_amg(A.ptr(), C.byref(amg))
+            assert pc
+            zz = np.zeros(n)
+            lib.apply_precond(capi.dptr(r), capi.dptr(zz), pc)
+            lib.hz_precond_free(pc)
+            z[p] = zz
+        finally:
+            _set_precision(lib, old)
+    rel = np.linalg.norm(z[precision] - z[0]) / np.linalg.norm(z[0])
+    assert rel < ztol, rel
+
+
+@pytest.mark.parametrize("precision", [1, 2])
+@pytest.mark.parametrize("problem", ["poisson", "coef"])
+def test_mixed_precision_pcg_reaches_fp64_tolerance(lib, orc, libc, precision, problem):
+    """PCG with the FP32 / BF16-valued cycle reaches 1e-10 in the FP64 true
+    residual, within a few iterations of the FP64 cycle, and its solution is
+    within 1e-8 of the FP64 oracle's."""
+    import scipy.sparse as sp
+    if problem == "poisson":
+        ia, ja, va = problems.poisson3d_kuhn(32)
+    else:
+        ia, ja, va, _ = problems.poisson3d_kuhn_coef(32)
+    n = ia.size - 1
+    b = problems.uniform_rhs(n, seed=4)
+    A = capi.HostCSR(ia, ja, va)
+    kw = dict(tol=1e-10, maxit=500, aggregation=capi.HEM, smoother=capi.SMOOTHER_L1DIAG, coarse_dof=200,
+              theta=0.04)
+    res = {}
+    for p in (0, precision):
+        old = _set_precision(lib, p)
+        try:
+            res[p] = _solve_both(lib, orc, libc, A, b, **kw)["gpu"] if p == 0 else None
+            if p:
+                x = capi.HostVec(np.zeros(n))
+                it, amg = capi.solver_params(lib, **kw)
+                libc.srand(1)
+                res[p] = (lib.linear_solver_dcsr_krylov_amg(A.ptr(), capi.HostVec(b).ptr(), x.ptr(),
+                                                            C.byref(it), C.byref(amg)), x.arr.copy())
+        finally:
+            _set_precision(lib, old)
+    import oracle
+    it, amg = oracle.params(**kw)
+    xo = capi.HostVec(np.zeros(n))
+    libc.srand(1)
+    io = orc.orc_krylov_amg(A.ptr(), capi.HostVec(b).ptr(), xo.ptr(), C.byref(it), C.byref(amg),
+                            C.byref(capi.orc_trace()))
+    (i64, _), (im, xm) = res[0], res[precision]
+    assert io > 0 and abs(i64 - io) <= 1
+    assert 0 < im <= i64 + 4, (im, i64)
+    Asp = sp.csr_matrix((va, ja, ia), shape=(n, n))
+    assert np.linalg.norm(b - Asp @ xm) / np.linalg.norm(b) < 1e-10
+    assert np.linalg.norm(xm - xo.arr) / np.linalg.norm(xo.arr) < 1e-8
