@@ -96,6 +96,31 @@ class hz_hierarchy(C.Structure):
                 ("num_aggregates", INT * MAX_AMG_LVL), ("theta", REAL * MAX_AMG_LVL)]
 
 
+class input_param(C.Structure):
+    _fields_ = [("print_level", SHORT), ("inifile", C.c_char * 256), ("gridfile", C.c_char * 256),
+                ("output_dir", C.c_char * 256), ("read_mesh_from_file", INT), ("spatial_dim", INT),
+                ("refinement_type", INT), ("refinement_levels", INT), ("boundary_codes", INT),
+                ("nquad", INT), ("FE_type", INT), ("Mass_lump", INT), ("time_start", REAL),
+                ("time_step_type", INT), ("time_steps", INT), ("time_step_size", REAL),
+                ("rhs_time_dep", INT), ("nonlinear_itsolver_type", INT),
+                ("nonlinear_itsolver_maxit", INT), ("nonlinear_itsolver_tol", REAL),
+                ("nonlinear_itsolver_toltype", INT), ("fas_presmoothers", INT),
+                ("fas_postsmoothers", INT), ("fas_smooth_tol", INT), ("linear_itsolver_type", INT),
+                ("linear_itsolver_maxit", INT), ("linear_itsolver_tol", REAL),
+                ("linear_stop_type", SHORT), ("linear_restart", INT), ("linear_precond_type", INT),
+                ("Schwarz_mmsize", INT), ("Schwarz_maxlvl", INT), ("Schwarz_type", INT),
+                ("Schwarz_blksolver", INT), ("AMG_type", SHORT), ("AMG_levels", SHORT),
+                ("AMG_cycle_type", SHORT), ("AMG_smoother", SHORT), ("AMG_relaxation", REAL),
+                ("AMG_presmooth_iter", SHORT), ("AMG_postsmooth_iter", SHORT),
+                ("AMG_polynomial_degree", SHORT), ("AMG_coarse_dof", INT), ("AMG_tol", REAL),
+                ("AMG_maxit", INT), ("AMG_coarse_solver", SHORT), ("AMG_coarse_scaling", SHORT),
+                ("AMG_amli_degree", SHORT), ("AMG_nl_amli_krylov_type", SHORT),
+                ("AMG_Schwarz_levels", INT), ("AMG_fpwr", REAL), ("AMG_aggregation_type", SHORT),
+                ("AMG_strong_coupled", REAL), ("AMG_max_aggregation", INT),
+                ("AMG_smooth_filter", SHORT), ("AMG_tentative_smooth", REAL),
+                ("HX_smooth_iter", SHORT), ("BSR_alpha", REAL), ("BSR_omega", REAL)]
+
+
 PRECOND_FN = C.CFUNCTYPE(None, C.POINTER(REAL), C.POINTER(REAL), C.c_void_p)
 
 
@@ -249,6 +274,12 @@ def _declare(lib):
     lib.hz_dense_inverse.argtypes = [INT, P(REAL)]
     lib.hz_dense_inverse.restype = INT
     lib.hz_dense_mfma_active.restype = INT
+    lib.param_input_init.argtypes = [P(input_param)]
+    lib.param_input.argtypes = [C.c_char_p, P(input_param)]
+    lib.param_linear_solver_set.argtypes = [P(linear_itsolver_param), P(input_param)]
+    lib.param_amg_set.argtypes = [P(AMG_param), P(input_param)]
+    lib.python_wrapper_krylov_amg.argtypes = [P(INT), P(INT), P(INT), P(INT), P(REAL), P(REAL), P(REAL),
+                                              P(REAL), P(INT), P(INT)]
     lib.linear_solver_amg.argtypes = [P(dCSRmat), P(dvector), P(dvector), P(AMG_param)]
     lib.linear_solver_amg.restype = INT
     lib.create_precond_amg.argtypes = [P(dCSRmat), P(AMG_param)]

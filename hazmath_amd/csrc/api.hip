@@ -10,9 +10,20 @@
 using namespace hz;
 
 namespace {
-
 hz_options g_opt = {1.0, 0, 0, 0, 0};
 hz_solve_info g_info;
+}  // namespace
+
+namespace hz {
+// check_error (message.c:191ff) exits with the status; return_on_fatal returns it.
+int fatal_status(int code) {
+    if (g_opt.return_on_fatal) return code;
+    std::fflush(stdout);
+    std::exit(code);
+}
+}  // namespace hz
+
+namespace {
 
 bool device_ready() {
     int n = 0;
@@ -55,12 +66,6 @@ void download_csr(const DevCSR &d, dCSRmat *A) {
     }
 }
 
-// check_error (message.c:191ff) exits with the status; return_on_fatal returns it.
-int fatal_status(int code) {
-    if (g_opt.return_on_fatal) return code;
-    std::fflush(stdout);
-    std::exit(code);
-}
 
 int fatal(int code) {
     // reference: exit(16) / exit(17) / check_error -> exit(status)

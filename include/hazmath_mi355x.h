@@ -55,6 +55,7 @@ extern "C" {
 #define HZ_SOLVER_DEFAULT   0
 #define HZ_SOLVER_CG        1
 #define HZ_SOLVER_UMFPACK  32
+#define HZ_SOLVER_AMG      21
 #define HZ_STOP_REL_RES     1
 #define HZ_STOP_REL_PRECRES 2
 #define HZ_STOP_MOD_REL_RES 3
@@ -204,6 +205,84 @@ precond *create_precond_amg(dCSRmat *A, AMG_param *amgparam);
 
 /* Replaces apply_precond, src/haznics/helper.c:1254-1260: pc->fct(r, z, pc->data). */
 void apply_precond(REAL *r, REAL *z, precond *pc);
+
+/* input_param: include/param.h:23-132 (field for field). */
+typedef struct {
+    SHORT print_level;
+    char  inifile[256];
+    char  gridfile[256];
+    char  output_dir[256];
+    INT   read_mesh_from_file;
+    INT   spatial_dim;
+    INT   refinement_type;
+    INT   refinement_levels;
+    INT   boundary_codes;
+    INT   nquad;
+    INT   FE_type;
+    INT   Mass_lump;
+    REAL  time_start;
+    INT   time_step_type;
+    INT   time_steps;
+    REAL  time_step_size;
+    INT   rhs_time_dep;
+    INT   nonlinear_itsolver_type;
+    INT   nonlinear_itsolver_maxit;
+    REAL  nonlinear_itsolver_tol;
+    INT   nonlinear_itsolver_toltype;
+    INT   fas_presmoothers;
+    INT   fas_postsmoothers;
+    INT   fas_smooth_tol;
+    INT   linear_itsolver_type;
+    INT   linear_itsolver_maxit;
+    REAL  linear_itsolver_tol;
+    SHORT linear_stop_type;
+    INT   linear_restart;
+    INT   linear_precond_type;
+    INT   Schwarz_mmsize;
+    INT   Schwarz_maxlvl;
+    INT   Schwarz_type;
+    INT   Schwarz_blksolver;
+    SHORT AMG_type;
+    SHORT AMG_levels;
+    SHORT AMG_cycle_type;
+    SHORT AMG_smoother;
+    REAL  AMG_relaxation;
+    SHORT AMG_presmooth_iter;
+    SHORT AMG_postsmooth_iter;
+    SHORT AMG_polynomial_degree;
+    INT   AMG_coarse_dof;
+    REAL  AMG_tol;
+    INT   AMG_maxit;
+    SHORT AMG_coarse_solver;
+    SHORT AMG_coarse_scaling;
+    SHORT AMG_amli_degree;
+    SHORT AMG_nl_amli_krylov_type;
+    INT   AMG_Schwarz_levels;
+    REAL  AMG_fpwr;
+    SHORT AMG_aggregation_type;
+    REAL  AMG_strong_coupled;
+    INT   AMG_max_aggregation;
+    SHORT AMG_smooth_filter;
+    REAL  AMG_tentative_smooth;
+    SHORT HX_smooth_iter;
+    REAL  BSR_alpha;
+    REAL  BSR_omega;
+} input_param;
+
+/* Replaces param_input_init, src/utilities/parameters.c:39-141. */
+void param_input_init(input_param *inparam);
+/* Replaces param_input, src/utilities/input.c:42-830: defaults, then the
+ * "keyword = value" entries of filenm (NULL: defaults only).  A file that
+ * cannot be opened is fatal like check_error(ERROR_OPEN_FILE). */
+void param_input(const char *filenm, input_param *inparam);
+/* Replace param_linear_solver_set / param_amg_set, src/utilities/parameters.c:263-348. */
+void param_linear_solver_set(linear_itsolver_param *itsparam, input_param *inparam);
+void param_amg_set(AMG_param *amgparam, input_param *inparam);
+/* Replaces python_wrapper_krylov_amg, src/utilities/wrapper.c:36-79: parameters
+ * from ./input.dat with tol, maxit and print level from the arguments; solves
+ * in place on the caller's raw CSR arrays (u holds the guess and the result). */
+void python_wrapper_krylov_amg(INT *n, INT *nnz, INT *ia, INT *ja, REAL *a, REAL *b, REAL *u,
+                               REAL *tol, INT *maxit, INT *print_lvl);
 
 /* ---------------------------------------------------------------------------
  * Build-owned extensions (no HAZmath counterpart).

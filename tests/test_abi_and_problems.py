@@ -170,3 +170,110 @@ def test_uniform_rhs_is_counter_based():
     assert a.min() >= -1 and a.max() < 1
     assert abs(a.mean()) < 0.1
     assert not np.array_equal(a, problems.uniform_rhs(1000, 4))
+
+
+def test_public_structs_match_reference_layout():
+    """Every field offset and size of the public structs in
+    include/hazmath_mi355x.h equals the reference's (layout compiled from the
+    reference headers into tests/golden/abi_layout_reference.json)."""
+    import json
+    import sys
+    sys_path_golden = Path(__file__).parent / "golden"
+    sys.path.insert(0, str(sys_path_golden))
+    import make_abi_layout as mal
+    ref = json.loads((sys_path_golden / "abi_layout_reference.json").read_text())
+    mine = mal.compile_and_run(mal.layout_source(["hazmath_mi355x.h"]), ROOT / "include")
+    assert mine == ref
+
+
+INPUT_DAT = """%----------------------------------------------%
+% parameters for the parser test                 %
+%----------------------------------------------%
+print_level   = 3;   % trailing text is skipped
+gridfile      = ../grids/3D/cube.haz
+[section] anything here is a comment
+linear_itsolver_type    = 1
+linear_itsolver_maxit   = 77
+linear_itsolver_tol     = 2.5e-9
+linear_stop_type        = 2
+linear_precond_type     = 2
+AMG_type                = UA
+AMG_cycle_type          = w
+AMG_levels              = 12
+AMG_tol                 = 1e-7
+AMG_maxit               = 3
+AMG_smoother            = L1DIAG
+AMG_coarse_scaling      = oN
+AMG_smooth_filter       = OFF
+AMG_presmooth_iter      = 2
+AMG_postsmooth_iter     = 3
+AMG_coarse_dof          = 150
+AMG_coarse_solver       = 32
+AMG_aggregation_type    = 5
+AMG_strong_coupled      = 0.125
+AMG_max_aggregation     = 9
+not_a_keyword           = 5
+fas_smooth_tol          = 7.9
+BSR_omega               = 0.5
+"""
+
+
+def _parse(lib, text, tmp_path):
+    f = tmp_path / "input.dat"
+    f.write_text(text)
+    p = capi.input_param()
+    lib.param_input(str(f).encode(), C.byref(p))
+    return p
+
+
+def test_param_input_reads_reference_format(tmp_path, capfd):
+    """param_input (input.c:42-830): keyword = value lines, '%'/'['/'|'
+    comments, names mapped to the macro.h codes, rest of line ignored, an
+    unknown keyword only warns."""
+    lib = capi.load_library()
+    p = _parse(lib, INPUT_DAT, tmp_path)
+    out = capfd.readouterr().out
+    assert "Unknown input keyword not_a_keyword" in out
+    assert p.print_level == 3 and p.gridfile == b"../grids/3D/cube.haz"
+    assert p.linear_itsolver_type == 1 and p.linear_itsolver_maxit == 77
+    assert p.linear_itsolver_tol == 2.5e-9 and p.linear_stop_type == 2 and p.linear_precond_type == 2
+    assert p.AMG_type == 1 and p.AMG_cycle_type == 2 and p.AMG_levels == 12
+    assert p.AMG_tol == 1e-7 and p.AMG_maxit == 3 and p.AMG_smoother == 10
+    assert p.AMG_coarse_scaling == 1 and p.AMG_smooth_filter == 0
+    assert (p.AMG_presmooth_iter, p.AMG_postsmooth_iter, p.AMG_coarse_dof) == (2, 3, 150)
+    assert (p.AMG_coarse_solver, p.AMG_aggregation_type, p.AMG_max_aggregation) == (32, 5, 9)
+    assert p.AMG_strong_coupled == 0.125 and p.fas_smooth_tol == 7 and p.BSR_omega == 0.5
+    # untouched fields keep param_input_init's defaults (parameters.c:39-141)
+    assert p.AMG_relaxation == 1.2 and p.linear_restart == 25 and p.Schwarz_mmsize == 200
+    assert p.inifile == str(tmp_path / "input.dat").encode()
+
+
+def test_param_input_stops_at_malformed_entry(tmp_path):
+    lib = capi.load_library()
+    p = _parse(lib, "AMG_levels = 7\nAMG_smoother = bogus\nAMG_coarse_dof = 99\n", tmp_path)
+    assert p.AMG_levels == 7
+    assert p.AMG_smoother == 2          # default GS: the bad value ended the reading
+    assert p.AMG_coarse_dof == 200      # never reached
+    p = _parse(lib, "AMG_levels 7\n", tmp_path)     # missing '='
+    assert p.AMG_levels == 10
+
+
+def test_param_setters_copy_like_reference(tmp_path):
+    """param_linear_solver_set / param_amg_set (parameters.c:263-348),
+    including the SOLVER_AMG swap of (tol, maxit) between the two."""
+    lib = capi.load_library()
+    p = _parse(lib, INPUT_DAT, tmp_path)
+    it, amg = capi.linear_itsolver_param(), capi.AMG_param()
+    lib.param_linear_solver_init(C.byref(it))
+    lib.param_amg_init(C.byref(amg))
+    lib.param_linear_solver_set(C.byref(it), C.byref(p))
+    lib.param_amg_set(C.byref(amg), C.byref(p))
+    assert (it.linear_itsolver_type, it.linear_maxit, it.linear_tol) == (1, 77, 2.5e-9)
+    assert (amg.maxit, amg.tol, amg.cycle_type, amg.smoother) == (3, 1e-7, 2, 10)
+    assert (amg.aggregation_type, amg.strong_coupled, amg.coarse_scaling) == (5, 0.125, 1)
+    p2 = _parse(lib, "linear_itsolver_type = 21\nlinear_itsolver_maxit = 40\nlinear_itsolver_tol = 1e-5\n"
+                "AMG_tol = 1e-3\nAMG_maxit = 4\n", tmp_path)
+    lib.param_linear_solver_set(C.byref(it), C.byref(p2))
+    lib.param_amg_set(C.byref(amg), C.byref(p2))
+    assert (it.linear_maxit, it.linear_tol) == (4, 1e-3)
+    assert (amg.maxit, amg.tol) == (40, 1e-5)

@@ -511,3 +511,31 @@ def test_mixed_precision_pcg_reaches_fp64_tolerance(lib, orc, libc, precision, p
     Asp = sp.csr_matrix((va, ja, ia), shape=(n, n))
     assert np.linalg.norm(b - Asp @ xm) / np.linalg.norm(b) < 1e-10
     assert np.linalg.norm(xm - xo.arr) / np.linalg.norm(xo.arr) < 1e-8
+
+
+def test_python_wrapper_reads_input_dat(lib, libc, tmp_path, monkeypatch):
+    """python_wrapper_krylov_amg (wrapper.c:36-79): parameters from
+    ./input.dat, tol/maxit/print level from the arguments, solve in place on
+    raw arrays; equals the drop-in called with the same parameters."""
+    (tmp_path / "input.dat").write_text(
+        "linear_itsolver_type = 1\nlinear_stop_type = 1\nAMG_type = UA\nAMG_cycle_type = V\n"
+        "AMG_levels = 20\nAMG_smoother = L1DIAG\nAMG_coarse_dof = 100\nAMG_coarse_solver = 32\n"
+        "AMG_aggregation_type = 5\nAMG_strong_coupled = 0.0\nAMG_maxit = 1\nAMG_tol = 1e-6\n")
+    monkeypatch.chdir(tmp_path)
+    ia, ja, va = problems.poisson2d_p1(48)
+    n = ia.size - 1
+    b = problems.uniform_rhs(n, seed=9)
+    u = np.zeros(n)
+    args = [C.byref(INTv) for INTv in (capi.INT(n), capi.INT(int(ia[-1])))]
+    libc.srand(1)
+    lib.python_wrapper_krylov_amg(args[0], args[1], capi.dptr(ia, capi.INT), capi.dptr(ja, capi.INT),
+                                  capi.dptr(va), capi.dptr(b), capi.dptr(u), C.byref(capi.REAL(1e-10)),
+                                  C.byref(capi.INT(300)), C.byref(capi.INT(0)))
+    x = capi.HostVec(np.zeros(n))
+    it, amg = capi.solver_params(lib, tol=1e-10, maxit=300, aggregation=capi.HEM, smoother=capi.SMOOTHER_L1DIAG,
+                                 coarse_dof=100, theta=0.0)
+    libc.srand(1)
+    r = lib.linear_solver_dcsr_krylov_amg(capi.HostCSR(ia, ja, va).ptr(), capi.HostVec(b).ptr(), x.ptr(),
+                                          C.byref(it), C.byref(amg))
+    assert r > 0
+    assert np.array_equal(u, x.arr)
