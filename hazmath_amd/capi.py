@@ -40,6 +40,7 @@ SOLVER_UMFPACK = 32
 STOP_REL_RES, STOP_REL_PRECRES, STOP_MOD_REL_RES = 1, 2, 3
 PREC_AMG = 2
 UA_AMG = 1
+SA_AMG = 2
 VMB, HEC, HEM = 1, 4, 5
 V_CYCLE, W_CYCLE = 1, 2
 SMOOTHER_JACOBI, SMOOTHER_GS, SMOOTHER_L1DIAG = 1, 2, 10
@@ -93,7 +94,8 @@ class hz_solve_info(C.Structure):
 class hz_hierarchy(C.Structure):
     _fields_ = [("num_levels", INT), ("A", dCSRmat * MAX_AMG_LVL),
                 ("aggregates", C.POINTER(INT) * MAX_AMG_LVL),
-                ("num_aggregates", INT * MAX_AMG_LVL), ("theta", REAL * MAX_AMG_LVL)]
+                ("num_aggregates", INT * MAX_AMG_LVL), ("theta", REAL * MAX_AMG_LVL),
+                ("P", dCSRmat * MAX_AMG_LVL)]
 
 
 class input_param(C.Structure):
@@ -195,6 +197,8 @@ def hierarchy_to_numpy(h: hz_hierarchy):
         if level < h.num_levels - 1 and bool(h.aggregates[level]):
             d["agg"] = np.ctypeslib.as_array(h.aggregates[level], shape=(ia.size - 1,)).copy()
             d["nagg"] = h.num_aggregates[level]
+            if bool(h.P[level].IA):
+                d["P"] = csr_to_numpy(h.P[level])
         out.append(d)
     return out
 

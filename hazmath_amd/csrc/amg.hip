@@ -153,11 +153,19 @@ int amg_setup(Hierarchy &H, const DevCSR &A0, AMG_param *param, bool) {
         if (st < 0) break;
         if (nagg < 1) break;
         if (tr) { HZ_CHECK(hipStreamSynchronize(stream())); t2 = wall_ms(); }
-        L.rptr = arena().alloc_n<int>((size_t)nagg + 1);
-        L.rmem = arena().alloc_n<int>((size_t)n + 1);
-        build_restriction(n, L.agg, nagg, L.rptr, L.rmem);
         Level &C = H.lv[lvl + 1];
-        rap_agg(L.A, L.agg, nagg, L.rptr, L.rmem, C.A);
+        if (param->AMG_type == HZ_SA_AMG) {
+            // amg_setup_smoothP_smoothR: smoothed P, R = P^T, C = R A P
+            sa_prolongation(L.A, N, L.agg, nagg, param->tentative_smooth, param->smooth_filter, L.P);
+            csr_transpose_gpu(L.P, L.R);
+            rap_general(L.R, L.A, L.P, C.A);
+            L.sa = true;
+        } else {
+            L.rptr = arena().alloc_n<int>((size_t)nagg + 1);
+            L.rmem = arena().alloc_n<int>((size_t)n + 1);
+            build_restriction(n, L.agg, nagg, L.rptr, L.rmem);
+            rap_agg(L.A, L.agg, nagg, L.rptr, L.rmem, C.A);
+        }
         sell_from_csr(C.A, C.S);
         if (tr) {
             HZ_CHECK(hipStreamSynchronize(stream()));
@@ -304,11 +312,18 @@ void prolong_postsmooth(Hierarchy &H, int l, double alpha, const double *alpha_d
     Level &C = H.lv[l + 1];
     const CycleParams &cp = H.cp;
     if (cp.postsmooth <= 0) {
-        prolong_agg(L.A.n, L.agg, C.x, alpha, alpha_dev, L.x);
+        if (L.sa) csr_prolong(L.P, C.x, alpha, alpha_dev, L.x);
+        else prolong_agg(L.A.n, L.agg, C.x, alpha, alpha_dev, L.x);
         return;
     }
     const bool want_dot = dot_out && cp.postsmooth == 1;
-    if (L.A.n <= fused_down_max_rows()) {
+    if (L.sa) {
+        csr_prolong(L.P, C.x, alpha, alpha_dev, L.x);   // dcsr_aAxpy(alpha, P), mgcycle.c:346
+        if (want_dot)
+            smooth_sweep_dot(L.S, cp.smoother, cp.jacobi_w, L.b, L.x, L.xt, dot_out, H.ws);
+        else
+            smooth_sweep(L.S, cp.smoother, cp.jacobi_w, L.b, L.x, L.xt);
+    } else if (L.A.n <= fused_down_max_rows()) {
         // latency-bound level: one kernel
         up_fused(L.S, cp.smoother, cp.jacobi_w, L.agg, C.x, alpha, alpha_dev, L.b, L.x, L.xt,
                  want_dot ? dot_out : nullptr, H.ws);
@@ -344,8 +359,11 @@ forward:
         cs.num_lvl[l]++;
         presmooth(H, l, cs);
         // the next level restarts from zero; fuse its first sweep into the restriction
-        const bool fuse = (l + 1 < nl - 1) && cp.presmooth > 0;
-        if (L.A.n <= fused_down_max_rows()) {
+        const bool fuse = (l + 1 < nl - 1) && cp.presmooth > 0 && !L.sa;
+        if (L.sa) {
+            residual(L.S, L.b, L.x, L.w);
+            csr_mxv(L.R, L.w, C.b);    // dcsr_mxv(R), mgcycle.c:302
+        } else if (L.A.n <= fused_down_max_rows()) {
             down_fused(L.nagg, L.rptr, L.rmem, L.A, L.b, L.x, C.b, cp.smoother, cp.jacobi_w,
                        fuse ? C.dsm : nullptr, fuse ? C.x : nullptr);
         } else {

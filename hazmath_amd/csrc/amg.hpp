@@ -17,6 +17,8 @@ struct Level {
     int *rmem = nullptr;
     double *b = nullptr, *x = nullptr, *w = nullptr, *xt = nullptr;
     double theta = 0.0;       // strong_coupled used to aggregate this level
+    bool sa = false;          // smoothed aggregation: transfers use P and R below
+    DevCSR P, R;
     double *Ainv = nullptr;   // coarsest, coarse_solver 32
     double *cgwork = nullptr; // coarsest, coarse_solver 0
     // FP32 cycle (CycleParams::precision > 0)

@@ -83,8 +83,12 @@ int check_supported(const linear_itsolver_param *it, const AMG_param *amg) {
         std::printf("### ERROR: Wrong smoother type %lld!\n", (long long)amg->smoother);
         return HZ_ERROR_AMG_SMOOTH_TYPE;
     }
-    if (amg->AMG_type == HZ_SA_AMG) {
-        std::printf("### ERROR: smoothed aggregation is not available in this build\n");
+    if (amg->AMG_type != HZ_UA_AMG && amg->AMG_type != HZ_SA_AMG) {
+        std::printf("### ERROR: AMG type %lld is not available in this build\n", (long long)amg->AMG_type);
+        return HZ_ERROR_INPUT_PAR;
+    }
+    if (amg->AMG_type == HZ_SA_AMG && g_opt.vcycle_precision > 0) {
+        std::printf("### ERROR: smoothed aggregation needs the FP64 cycle (vcycle_precision = 0)\n");
         return HZ_ERROR_INPUT_PAR;
     }
     if (amg->cycle_type != HZ_V_CYCLE && amg->cycle_type != HZ_W_CYCLE) {
@@ -420,6 +424,7 @@ INT hz_setup_hierarchy(dCSRmat *A, AMG_param *amgparam, hz_hierarchy *out) {
                 out->aggregates[l] = (INT *)std::calloc((size_t)H.lv[l].A.n + 1, sizeof(INT));
                 HZ_CHECK(memcpy_sync(out->aggregates[l], H.lv[l].agg, sizeof(int) * H.lv[l].A.n,
                                    hipMemcpyDeviceToHost));
+                if (H.lv[l].sa) download_csr(H.lv[l].P, &out->P[l]);
             }
         }
         return st;
@@ -431,6 +436,7 @@ INT hz_setup_hierarchy(dCSRmat *A, AMG_param *amgparam, hz_hierarchy *out) {
 void hz_hierarchy_free(hz_hierarchy *h) {
     for (int l = 0; l < HZ_MAX_AMG_LVL; ++l) {
         dcsr_free(&h->A[l]);
+        dcsr_free(&h->P[l]);
         std::free(h->aggregates[l]);
         h->aggregates[l] = nullptr;
     }

@@ -123,6 +123,19 @@ bool dense_mfma_active();
 // Single-workgroup CG for coarse_solver = 0 (coarse_itsolver, mgcycle.c:34-53).
 void coarse_cg(const DevSELL &S, const double *b, double *x, double tol, int maxit, double *work);
 
+// ---- kernels_sa.hip: smoothed aggregation ---------------------------------
+// P = (I - omega D^-1 M) P_tent with M = A (filter 0) or the filtered strength
+// graph N (filter 1; N's diagonal is modified in place).
+void sa_prolongation(const DevCSR &A, DevCSR &N, const int *agg, int nagg, double omega, int filter,
+                     DevCSR &P);
+// AT = A^T with every row listing its source rows in increasing order (dcsr_trans).
+void csr_transpose_gpu(const DevCSR &A, DevCSR &AT);
+// C = R A P with the layout and summation order of dcsr_rap.
+void rap_general(const DevCSR &R, const DevCSR &A, const DevCSR &P, DevCSR &C);
+// y = A x (row sums from 0.0, CSR order) and x += alpha P xc (dcsr_aAxpy).
+void csr_mxv(const DevCSR &A, const double *x, double *y);
+void csr_prolong(const DevCSR &P, const double *xc, double alpha, const double *alpha_dev, double *x);
+
 // ---- kernels_mixed.hip: the V-cycle in FP32 / BF16 ------------------------
 constexpr int kLowF32 = 1, kLowBF16 = 2;
 // Low-precision copies of a level's values: SELL-64 order (sharing DevSELL's

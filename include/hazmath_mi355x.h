@@ -349,6 +349,7 @@ typedef struct {
     INT *aggregates[HZ_MAX_AMG_LVL];
     INT num_aggregates[HZ_MAX_AMG_LVL];
     REAL theta[HZ_MAX_AMG_LVL];   /* strong_coupled in force when level l was aggregated */
+    dCSRmat P[HZ_MAX_AMG_LVL];    /* smoothed aggregation: prolongation of level l (empty for UA) */
 } hz_hierarchy;
 
 INT hz_setup_hierarchy(dCSRmat *A, AMG_param *amgparam, hz_hierarchy *out);

@@ -509,6 +509,7 @@ INT orc_rap_agg(const dCSRmat *A, const INT *agg, INT nagg, dCSRmat *Ac) {
 /* ------------------------------------------------------------------ */
 typedef struct {
     dCSRmat A;
+    dCSRmat P, R; /* smoothed aggregation: P = S P_tent and R = P^T (empty for UA) */
     INT *agg;     /* P: row i -> column agg[i], empty when agg[i] == UNPT */
     INT nagg;
     INT *rp, *rm; /* R = P^T rows (members ascending) */
@@ -527,6 +528,8 @@ static void orc_amg_free(orc_amg *m) {
     for (INT l = 0; l < HZ_MAX_AMG_LVL; ++l) {
         orc_level *L = &m->lv[l];
         csr_free(&L->A);
+        csr_free(&L->P);
+        csr_free(&L->R);
         free(L->agg); free(L->rp); free(L->rm);
         free(L->b); free(L->x); free(L->w); free(L->L); free(L->D);
         memset(L, 0, sizeof(*L));
@@ -571,7 +574,155 @@ static void orc_coarse_solve(const orc_level *L, const REAL *b, REAL *x) {
     }
 }
 
-/* amg_setup_unsmoothP_unsmoothR, amg_setup_ua.c:1228-1443. */
+/* ------------------------------------------------------------------ */
+/* Smoothed aggregation (amg_setup_smoothP_smoothR, amg_setup_ua.c:1461-1687) */
+/* ------------------------------------------------------------------ */
+
+/* dcsr_mxm (sparse.c:1832-1918): row i of C lists the columns of B reached
+ * from row i of A in order of first discovery (A row order, then B row
+ * order); C[i][j] = 0 + sum of a_ik b_kj in the same order. */
+static void orc_mxm(const dCSRmat *A, const dCSRmat *B, dCSRmat *C) {
+    const INT n = A->row;
+    INT *mark = (INT *)xcalloc((size_t)B->col + 1, sizeof(INT));
+    for (INT j = 0; j < B->col; ++j) mark[j] = -1;
+    C->row = n; C->col = B->col;
+    C->IA = (INT *)xcalloc((size_t)n + 1, sizeof(INT));
+    INT cap = 16, pos = 0;
+    C->JA = (INT *)xcalloc((size_t)cap, sizeof(INT));
+    C->val = (REAL *)xcalloc((size_t)cap, sizeof(REAL));
+    for (INT i = 0; i < n; ++i) {
+        INT row0 = pos;
+        C->IA[i] = pos;
+        for (INT k = A->IA[i]; k < A->IA[i + 1]; ++k) {
+            INT r = A->JA[k];
+            for (INT l = B->IA[r]; l < B->IA[r + 1]; ++l) {
+                INT j = B->JA[l];
+                if (mark[j] < row0) {
+                    if (pos == cap) {
+                        cap *= 2;
+                        C->JA = (INT *)realloc(C->JA, sizeof(INT) * (size_t)cap);
+                        C->val = (REAL *)realloc(C->val, sizeof(REAL) * (size_t)cap);
+                    }
+                    mark[j] = pos;
+                    C->JA[pos] = j;
+                    C->val[pos] = 0.0;
+                    ++pos;
+                }
+                C->val[mark[j]] += A->val[k] * B->val[l];
+            }
+        }
+    }
+    C->IA[n] = pos;
+    C->nnz = pos;
+    free(mark);
+}
+
+/* dcsr_rap (sparse.c:2401-2538): row ic of RAP starts with ic (value 0.0 plus
+ * its products), then the coarse columns in order of first discovery over
+ * (R row ic, A row, P row); every entry accumulates its products
+ * (r_ij a_jk) p_kl in that traversal order, the first one assigned. */
+static void orc_rap(const dCSRmat *R, const dCSRmat *A, const dCSRmat *P, dCSRmat *C) {
+    const INT nc = R->row;
+    INT *mark = (INT *)xcalloc((size_t)P->col + 1, sizeof(INT));
+    for (INT j = 0; j < P->col; ++j) mark[j] = -1;
+    C->row = nc; C->col = P->col;
+    C->IA = (INT *)xcalloc((size_t)nc + 1, sizeof(INT));
+    INT cap = 16, pos = 0;
+    C->JA = (INT *)xcalloc((size_t)cap, sizeof(INT));
+    C->val = (REAL *)xcalloc((size_t)cap, sizeof(REAL));
+    for (INT ic = 0; ic < nc; ++ic) {
+        INT row0 = pos;
+        C->IA[ic] = pos;
+        if (pos == cap) {
+            cap *= 2;
+            C->JA = (INT *)realloc(C->JA, sizeof(INT) * (size_t)cap);
+            C->val = (REAL *)realloc(C->val, sizeof(REAL) * (size_t)cap);
+        }
+        mark[ic] = pos;
+        C->JA[pos] = ic;
+        C->val[pos] = 0.0;
+        ++pos;
+        for (INT q1 = R->IA[ic]; q1 < R->IA[ic + 1]; ++q1) {
+            const REAL r = R->val[q1];
+            const INT i1 = R->JA[q1];
+            for (INT q2 = A->IA[i1]; q2 < A->IA[i1 + 1]; ++q2) {
+                const REAL ra = r * A->val[q2];
+                const INT i2 = A->JA[q2];
+                for (INT q3 = P->IA[i2]; q3 < P->IA[i2 + 1]; ++q3) {
+                    const REAL rap = ra * P->val[q3];
+                    const INT i3 = P->JA[q3];
+                    if (mark[i3] < row0) {
+                        if (pos == cap) {
+                            cap *= 2;
+                            C->JA = (INT *)realloc(C->JA, sizeof(INT) * (size_t)cap);
+                            C->val = (REAL *)realloc(C->val, sizeof(REAL) * (size_t)cap);
+                        }
+                        mark[i3] = pos;
+                        C->JA[pos] = i3;
+                        C->val[pos] = rap;
+                        ++pos;
+                    } else {
+                        C->val[mark[i3]] += rap;
+                    }
+                }
+            }
+        }
+    }
+    C->IA[nc] = pos;
+    C->nnz = pos;
+    free(mark);
+}
+
+/* smooth_aggregation_p (amg_setup_ua.c:472-572): S = I - omega D^-1 M with M = A
+ * (filter off) or the filtered strength graph N (filter on: the diagonal of N
+ * absorbs the dropped off-diagonal row sum of A; N is modified in place), D
+ * the first diagonal entry of M (1.0 where |d| < 1e-6); then P = S P_tent. */
+static void orc_smooth_p(const dCSRmat *A, dCSRmat *N, const INT *agg, INT nagg, REAL omega, SHORT filter,
+                         dCSRmat *P) {
+    const INT n = A->row;
+    const dCSRmat *M = A;
+    if (filter == 1) {
+        for (INT i = 0; i < n; ++i) {
+            REAL sa = 0.0, sn = 0.0;
+            for (INT k = A->IA[i]; k < A->IA[i + 1]; ++k) if (A->JA[k] != i) sa += A->val[k];
+            for (INT k = N->IA[i]; k < N->IA[i + 1]; ++k) if (N->JA[k] != i) sn += N->val[k];
+            for (INT k = N->IA[i]; k < N->IA[i + 1]; ++k) if (N->JA[k] == i) N->val[k] += sa - sn;
+        }
+        M = N;
+    }
+    REAL *d = (REAL *)xcalloc((size_t)n, sizeof(REAL));
+    for (INT i = 0; i < n; ++i)
+        for (INT k = M->IA[i]; k < M->IA[i + 1]; ++k)
+            if (M->JA[k] == i) { d[i] = M->val[k]; break; }
+    for (INT i = 0; i < n; ++i) if (fabs(d[i]) < 1e-6) d[i] = 1.0;
+    dCSRmat S;
+    csr_copy(M, &S);
+    for (INT i = 0; i < n; ++i)
+        for (INT k = S.IA[i]; k < S.IA[i + 1]; ++k)
+            S.val[k] = (S.JA[k] == i) ? 1 - omega * M->val[k] / d[i] : -omega * M->val[k] / d[i];
+    free(d);
+    /* tentative P (form_tentative_p, amg_setup_ua.c:251-295): (agg[i], 1.0) */
+    dCSRmat T;
+    T.row = n; T.col = nagg;
+    T.IA = (INT *)xcalloc((size_t)n + 1, sizeof(INT));
+    INT cnt = 0;
+    for (INT i = 0; i < n; ++i) if (agg[i] > HZ_UNPT) ++cnt;
+    T.JA = (INT *)xcalloc((size_t)cnt + 1, sizeof(INT));
+    T.val = (REAL *)xcalloc((size_t)cnt + 1, sizeof(REAL));
+    cnt = 0;
+    for (INT i = 0; i < n; ++i) {
+        T.IA[i] = cnt;
+        if (agg[i] > HZ_UNPT) { T.JA[cnt] = agg[i]; T.val[cnt] = 1.0; ++cnt; }
+    }
+    T.IA[n] = cnt;
+    T.nnz = cnt;
+    orc_mxm(&S, &T, P);
+    csr_free(&S);
+    csr_free(&T);
+}
+
+/* amg_setup_unsmoothP_unsmoothR, amg_setup_ua.c:1228-1443, and for
+ * AMG_type = SA amg_setup_smoothP_smoothR, amg_setup_ua.c:1461-1687. */
 static INT orc_setup(orc_amg *m, const dCSRmat *A, AMG_param *param) {
     memset(m, 0, sizeof(*m));
     const SHORT min_cdof = (SHORT)(param->coarse_dof > 1 ? param->coarse_dof : 1);
@@ -601,6 +752,16 @@ static INT orc_setup(orc_amg *m, const dCSRmat *A, AMG_param *param) {
         else if (L->nagg * 1.25 < n) param->strong_coupled *= 2;
         if (st < 0) { status = HZ_SUCCESS; break; }
         if (L->nagg < ORC_MIN_CDOF) break;
+        if (param->AMG_type == HZ_SA_AMG) {
+            dCSRmat N;
+            orc_strength(&L->A, L->theta, &N);
+            orc_smooth_p(&L->A, &N, L->agg, L->nagg, param->tentative_smooth, param->smooth_filter, &L->P);
+            csr_free(&N);
+            csr_transpose(&L->P, &L->R);
+            orc_rap(&L->R, &L->A, &L->P, &m->lv[lvl + 1].A);
+            ++lvl;
+            continue;
+        }
         /* R members (dcsr_trans of the tentative P) */
         L->rp = (INT *)xcalloc((size_t)L->nagg + 1, sizeof(INT));
         L->rm = (INT *)xcalloc((size_t)n, sizeof(INT));
@@ -895,10 +1056,14 @@ forward:
         orc_smooth(pd, L, p->presmooth_iter);
         orc_dcsr_residual(&L->A, L->b, L->x, L->w);
         orc_level *C = &m->lv[l + 1];
-        for (INT ic = 0; ic < L->nagg; ++ic) {   /* dcsr_mxv_agg, sparse.c:1663-1679 */
-            REAL s = 0.0;
-            for (INT q = L->rp[ic]; q < L->rp[ic + 1]; ++q) s += L->w[L->rm[q]];
-            C->b[ic] = s;
+        if (L->P.IA) {
+            orc_dcsr_mxv(&L->R, L->w, C->b);        /* dcsr_mxv (mgcycle.c:302) */
+        } else {
+            for (INT ic = 0; ic < L->nagg; ++ic) {   /* dcsr_mxv_agg, sparse.c:1663-1679 */
+                REAL s = 0.0;
+                for (INT q = L->rp[ic]; q < L->rp[ic + 1]; ++q) s += L->w[L->rm[q]];
+                C->b[ic] = s;
+            }
         }
         ++l;
         memset(m->lv[l].x, 0, sizeof(REAL) * (size_t)m->lv[l].A.row);
@@ -920,9 +1085,13 @@ forward:
             alpha = orc_dot(C->A.row, C->x, C->b) / orc_dcsr_vmv(&C->A, C->x, C->x);
             if (alpha > 2.0) alpha = 2.0;
         }
-        for (INT i = 0; i < L->A.row; ++i) {    /* dcsr_aAxpy_agg, sparse.c:1748-1784 */
+        for (INT i = 0; i < L->A.row; ++i) {    /* dcsr_aAxpy_agg, sparse.c:1748-1784 / dcsr_aAxpy 1693-1730 */
             REAL t = 0.0;
-            if (L->agg[i] > HZ_UNPT) t += C->x[L->agg[i]];
+            if (L->P.IA) {
+                for (INT k = L->P.IA[i]; k < L->P.IA[i + 1]; ++k) t += L->P.val[k] * C->x[L->P.JA[k]];
+            } else if (L->agg[i] > HZ_UNPT) {
+                t += C->x[L->agg[i]];
+            }
             if (alpha == 1.0) L->x[i] += t;
             else if (alpha == -1.0) L->x[i] -= t;
             else L->x[i] += t * alpha;
@@ -1108,6 +1277,8 @@ INT orc_setup_hierarchy(dCSRmat *A, AMG_param *amgparam, hz_hierarchy *out) {
             out->aggregates[l] = m->lv[l].agg;
             out->num_aggregates[l] = m->lv[l].nagg;
             m->lv[l].agg = NULL;
+            out->P[l] = m->lv[l].P;
+            memset(&m->lv[l].P, 0, sizeof(dCSRmat));
         }
     }
     orc_amg_free(m);
@@ -1118,6 +1289,7 @@ INT orc_setup_hierarchy(dCSRmat *A, AMG_param *amgparam, hz_hierarchy *out) {
 void orc_hierarchy_free(hz_hierarchy *h) {
     for (INT l = 0; l < HZ_MAX_AMG_LVL; ++l) {
         csr_free(&h->A[l]);
+        csr_free(&h->P[l]);
         free(h->aggregates[l]);
         h->aggregates[l] = NULL;
     }

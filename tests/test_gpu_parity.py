@@ -539,3 +539,104 @@ def test_python_wrapper_reads_input_dat(lib, libc, tmp_path, monkeypatch):
                                           C.byref(it), C.byref(amg))
     assert r > 0
     assert np.array_equal(u, x.arr)
+
+
+def _sa_param(params_fn, agg_type, filt, omega=0.67, **kw):
+    it, amg = params_fn(aggregation=agg_type, **kw)
+    amg.AMG_type = capi.SA_AMG
+    amg.smooth_filter = filt
+    amg.tentative_smooth = omega
+    return it, amg
+
+
+@pytest.mark.parametrize("agg_type", [capi.HEC, capi.HEM, capi.VMB])
+@pytest.mark.parametrize("filt", [1, 0])
+def test_sa_hierarchy_bitexact(lib, orc, libc, solvers_fixture, agg_type, filt):
+    """Smoothed aggregation setup (amg_setup_smoothP_smoothR): aggregates,
+    smoothed P (structure, column order and values) and every coarse matrix
+    bit-identical to the oracle."""
+    import oracle
+    ia, ja, va, _ = solvers_fixture
+    A = capi.HostCSR(ia, ja, va)
+    out = []
+    for which in ("gpu", "oracle"):
+        fn = (lambda **k: capi.solver_params(lib, **k)) if which == "gpu" else oracle.params
+        _, amg = _sa_param(fn, agg_type, filt, theta=0.04, coarse_dof=100)
+        h = capi.hz_hierarchy()
+        libc.srand(1)
+        st = (lib.hz_setup_hierarchy if which == "gpu" else orc.orc_setup_hierarchy)(A.ptr(), C.byref(amg),
+                                                                                      C.byref(h))
+        assert st == 0
+        out.append(capi.hierarchy_to_numpy(h))
+        (lib.hz_hierarchy_free if which == "gpu" else orc.orc_hierarchy_free)(C.byref(h))
+    g, o = out
+    assert len(g) == len(o) >= 3
+    for lg, lo in zip(g, o):
+        for key in ("ia", "ja", "val"):
+            assert np.array_equal(lg[key], lo[key]), key
+        if "agg" in lo:
+            assert np.array_equal(lg["agg"], lo["agg"])
+            for a, b_ in zip(lg["P"], lo["P"]):
+                assert np.array_equal(a, b_)
+
+
+@pytest.mark.parametrize("agg_type,smoother", [(capi.HEC, capi.SMOOTHER_L1DIAG), (capi.HEM, capi.SMOOTHER_L1DIAG),
+                                               (capi.VMB, capi.SMOOTHER_JACOBI)])
+def test_sa_pcg_parity(lib, orc, libc, solvers_fixture, agg_type, smoother):
+    import oracle
+    ia, ja, va, b = solvers_fixture
+    A = capi.HostCSR(ia, ja, va)
+    res = {}
+    for which in ("gpu", "oracle"):
+        fn = (lambda **k: capi.solver_params(lib, **k)) if which == "gpu" else oracle.params
+        it, amg = _sa_param(fn, agg_type, 1, tol=1e-10, maxit=300, smoother=smoother, coarse_dof=100)
+        x = capi.HostVec(np.zeros(b.size))
+        libc.srand(1)
+        if which == "gpu":
+            r = lib.linear_solver_dcsr_krylov_amg(A.ptr(), capi.HostVec(b).ptr(), x.ptr(), C.byref(it),
+                                                  C.byref(amg))
+        else:
+            r = orc.orc_krylov_amg(A.ptr(), capi.HostVec(b).ptr(), x.ptr(), C.byref(it), C.byref(amg),
+                                   C.byref(capi.orc_trace()))
+        res[which] = (r, x.arr.copy())
+    (ig, xg), (io, xo) = res["gpu"], res["oracle"]
+    assert io > 0 and abs(ig - io) <= 1
+    assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) < 1e-10
+
+
+def test_sa_kuhn3d_hierarchy_and_solve(lib, orc, libc):
+    """SA on the 3-D Kuhn problem (m = 16): hierarchy bit-exact (large rows
+    exercise the Galerkin tables) and PCG parity."""
+    import oracle
+    ia, ja, va = problems.poisson3d_kuhn(16)
+    b = problems.uniform_rhs(ia.size - 1, seed=2)
+    A = capi.HostCSR(ia, ja, va)
+    out, res = [], {}
+    for which in ("gpu", "oracle"):
+        fn = (lambda **k: capi.solver_params(lib, **k)) if which == "gpu" else oracle.params
+        it, amg = _sa_param(fn, capi.HEC, 1, tol=1e-10, maxit=300, smoother=capi.SMOOTHER_L1DIAG,
+                            coarse_dof=60, theta=0.04)
+        h = capi.hz_hierarchy()
+        libc.srand(1)
+        st = (lib.hz_setup_hierarchy if which == "gpu" else orc.orc_setup_hierarchy)(A.ptr(), C.byref(amg),
+                                                                                      C.byref(h))
+        assert st == 0
+        out.append(capi.hierarchy_to_numpy(h))
+        (lib.hz_hierarchy_free if which == "gpu" else orc.orc_hierarchy_free)(C.byref(h))
+        it, amg = _sa_param(fn, capi.HEC, 1, tol=1e-10, maxit=300, smoother=capi.SMOOTHER_L1DIAG,
+                            coarse_dof=60, theta=0.04)
+        x = capi.HostVec(np.zeros(b.size))
+        libc.srand(1)
+        if which == "gpu":
+            r = lib.linear_solver_dcsr_krylov_amg(A.ptr(), capi.HostVec(b).ptr(), x.ptr(), C.byref(it),
+                                                  C.byref(amg))
+        else:
+            r = orc.orc_krylov_amg(A.ptr(), capi.HostVec(b).ptr(), x.ptr(), C.byref(it), C.byref(amg),
+                                   C.byref(capi.orc_trace()))
+        res[which] = (r, x.arr.copy())
+    for lg, lo in zip(*out):
+        for key in ("ia", "ja", "val"):
+            assert np.array_equal(lg[key], lo[key]), key
+    (ig, xg), (io, xo) = res["gpu"], res["oracle"]
+    assert io > 0 and abs(ig - io) <= 1
+    assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) < 1e-10

@@ -362,3 +362,88 @@ def test_oracle_preconditioner_is_symmetric_positive(orc, libc):
     orc.orc_precond_free(h)
     assert abs(Bu @ v - u @ Bv) < 1e-12 * np.linalg.norm(Bu) * np.linalg.norm(v)
     assert Bu @ u > 0 and Bv @ v > 0
+
+
+def _sa_hierarchy(orc, libc, ia, ja, va, agg=capi.HEC, theta=0.04, omega=0.67, filt=1, coarse_dof=40):
+    A = capi.HostCSR(ia, ja, va)
+    _, amg = oracle.params(aggregation=agg, theta=theta, coarse_dof=coarse_dof)
+    amg.AMG_type = capi.SA_AMG
+    amg.tentative_smooth = omega
+    amg.smooth_filter = filt
+    h = capi.hz_hierarchy()
+    libc.srand(1)
+    assert orc.orc_setup_hierarchy(A.ptr(), C.byref(amg), C.byref(h)) == 0
+    lv = capi.hierarchy_to_numpy(h)
+    orc.orc_hierarchy_free(C.byref(h))
+    return lv
+
+
+def _strength_np(orc, ia, ja, va, theta):
+    A = capi.HostCSR(ia, ja, va)
+    N = capi.dCSRmat()
+    orc.orc_strength(A.ptr(), C.c_double(theta), C.byref(N))
+    return capi.csr_to_numpy(N)
+
+
+@pytest.mark.parametrize("filt", [1, 0])
+def test_oracle_sa_prolongation_is_smoothed_tentative(orc, libc, filt):
+    """P = (I - omega D_F^-1 A_F) P_tent (smooth_aggregation_p,
+    amg_setup_ua.c:472-572) with A_F the filtered strength graph, whose row
+    sums equal A's (filter on) or A itself (filter off); columns of a row in
+    order of first discovery; Galerkin A_c = P^T A P."""
+    ia, ja, va = problems.poisson2d_p1(20)
+    lv = _sa_hierarchy(orc, libc, ia, ja, va, filt=filt)
+    assert len(lv) >= 3
+    for l in range(len(lv) - 1):
+        L, Cn = lv[l], lv[l + 1]
+        n = L["ia"].size - 1
+        Asp = sp.csr_matrix((L["val"], L["ja"], L["ia"]), shape=(n, n))
+        if filt:
+            nia, nja, nva = _strength_np(orc, L["ia"], L["ja"], L["val"], L["theta"])
+            Nsp = sp.csr_matrix((nva, nja, nia), shape=(n, n)).toarray()
+            off = Nsp - np.diag(np.diag(Nsp))
+            AF = off + np.diag(np.diag(Nsp) + (Asp.toarray().sum(1) - np.diag(Asp.toarray())) - off.sum(1))
+            assert np.allclose(AF.sum(1), Asp.toarray().sum(1), atol=1e-12)
+        else:
+            AF = Asp.toarray()
+        d = np.diag(AF).copy()
+        d[np.abs(d) < 1e-6] = 1.0
+        T = np.zeros((n, L["nagg"]))
+        ok = L["agg"] > -1
+        T[np.arange(n)[ok], L["agg"][ok]] = 1.0
+        Pd = (np.eye(n) - 0.67 * AF / d[:, None]) @ T
+        pia, pja, pva = L["P"]
+        Psp = sp.csr_matrix((pva, pja, pia), shape=(n, L["nagg"]))
+        assert np.allclose(Psp.toarray(), Pd, atol=1e-13)
+        # columns: first discovery of agg(k) over the smoother's row
+        pattern_src = (sp.csr_matrix(AF) if filt else Asp)
+        for i in range(0, n, 7):
+            cols = []
+            for k in pattern_src.indices[pattern_src.indptr[i]:pattern_src.indptr[i + 1]] if not filt else []:
+                if L["agg"][k] > -1 and L["agg"][k] not in cols:
+                    cols.append(L["agg"][k])
+            if not filt:
+                assert list(pja[pia[i]:pia[i + 1]]) == cols
+        Ac = sp.csr_matrix((Cn["val"], Cn["ja"], Cn["ia"]), shape=(L["nagg"], L["nagg"]))
+        assert np.allclose(Ac.toarray(), Psp.T.toarray() @ Asp.toarray() @ Psp.toarray(), atol=1e-12)
+        if l + 1 < len(lv) - 1:   # the coarsest level is stored transposed (sorted) for coarse_solver 32
+            assert all(Cn["ja"][Cn["ia"][r]] == r for r in range(L["nagg"]))   # diagonal first (dcsr_rap)
+
+
+def test_oracle_sa_pcg_beats_ua(orc, libc, solvers_fixture):
+    """Smoothed aggregation (omega = 0.67, filtered) needs fewer PCG iterations
+    than plain aggregation on the reference fixture, with the same aggregates."""
+    ia, ja, va, b = solvers_fixture
+    A = capi.HostCSR(ia, ja, va)
+    its = {}
+    for typ in (capi.UA_AMG, capi.SA_AMG):
+        it, amg = oracle.params(tol=1e-10, maxit=300, aggregation=capi.HEC, smoother=capi.SMOOTHER_JACOBI,
+                                coarse_dof=100)
+        amg.AMG_type = typ
+        x = capi.HostVec(np.zeros(b.size))
+        libc.srand(1)
+        its[typ] = orc.orc_krylov_amg(A.ptr(), capi.HostVec(b).ptr(), x.ptr(), C.byref(it), C.byref(amg),
+                                      C.byref(capi.orc_trace()))
+        Asp = sp.csr_matrix((va, ja, ia))
+        assert np.linalg.norm(b - Asp @ x.arr) / np.linalg.norm(b) < 1e-10
+    assert 0 < its[capi.SA_AMG] < its[capi.UA_AMG], its
