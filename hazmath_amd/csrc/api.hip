@@ -25,9 +25,68 @@ int fatal_status(int code) {
 
 namespace {
 
+// The caller's libc random() generator (HEC draws its turn order from it, as
+// the reference does).  Initialising the HIP runtime consumes random() values,
+// so the state is saved around that initialisation and put back.
+struct RandomSnapshot {
+    char *buf = nullptr;      // header word of the active state
+    int words = 0;            // header + degree
+    int32_t data[64];
+};
+
+alignas(8) char g_switch_state[128];
+
+bool save_random(RandomSnapshot &s) {
+    static const int kDegrees[5] = {0, 7, 15, 31, 63};
+    char *cur = initstate(1, g_switch_state, sizeof(g_switch_state));   // records cur's position
+    if (!cur) return false;
+    int32_t hdr;
+    std::memcpy(&hdr, cur, sizeof(hdr));
+    setstate(cur);
+    const int type = hdr % 5;
+    if (type < 0 || type > 4) return false;
+    s.buf = cur;
+    s.words = 1 + kDegrees[type];
+    std::memcpy(s.data, cur, sizeof(int32_t) * s.words);
+    return true;
+}
+
+void restore_random(const RandomSnapshot &s) {
+    setstate(g_switch_state);
+    std::memcpy(s.buf, s.data, sizeof(int32_t) * s.words);
+    setstate(s.buf);
+}
+
+bool runtime_up() {
+    static int ready = -1;
+    if (ready < 0) {
+        RandomSnapshot snap;
+        const bool saved = save_random(snap);
+        int n = 0;
+        ready = (hipGetDeviceCount(&n) == hipSuccess && n >= 1) ? 1 : 0;
+        if (ready) {
+            // bring the runtime fully up now: context, the library's stream,
+            // device and pinned allocations, a launch
+            try {
+                (void)hipFree(nullptr);
+                void *d = nullptr, *h = nullptr;
+                HZ_CHECK(hipMalloc(&d, 256));
+                HZ_CHECK(hipHostMalloc(&h, 256, hipHostMallocDefault));
+                HZ_CHECK(hipMemsetAsync(d, 0, 256, stream()));
+                HZ_CHECK(hipStreamSynchronize(stream()));
+                (void)hipFree(d);
+                (void)hipHostFree(h);
+            } catch (const DeviceError &) {
+                ready = 0;
+            }
+        }
+        if (saved) restore_random(snap);
+    }
+    return ready == 1;
+}
+
 bool device_ready() {
-    int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess || n < 1) {
+    if (!runtime_up()) {
         std::fprintf(stderr, "hazmath_mi355x: no HIP device available\n");
         return false;
     }
@@ -362,7 +421,7 @@ const char *hz_version(void) { return "hazmath_mi355x 0.1 (gfx950)"; }
 
 INT hz_device_count(void) {
     int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    if (!runtime_up() || hipGetDeviceCount(&n) != hipSuccess) return 0;
     return n;
 }
 

@@ -354,10 +354,10 @@ def test_pcg_parity_large_coarsest(lib, orc, libc):
     assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) < 1e-10
 
 
-@pytest.mark.parametrize("agg_type,smoother,coarse_solver", [
-    (capi.HEM, capi.SMOOTHER_L1DIAG, 32), (capi.HEC, capi.SMOOTHER_JACOBI, 32),
-    (capi.VMB, capi.SMOOTHER_JACOBI, 0)])
-def test_linear_solver_amg_parity(lib, orc, libc, agg_type, smoother, coarse_solver):
+@pytest.mark.parametrize("agg_type,smoother,coarse_solver,tol", [
+    (capi.HEM, capi.SMOOTHER_L1DIAG, 32, 1e-10), (capi.HEC, capi.SMOOTHER_L1DIAG, 0, 1e-6),
+    (capi.VMB, capi.SMOOTHER_JACOBI, 0, 1e-6)])
+def test_linear_solver_amg_parity(lib, orc, libc, agg_type, smoother, coarse_solver, tol):
     """AMG as the solver (linear_solver_amg / amg_solve) from a nonzero guess:
     iteration count within +-1 and solution within 1e-10 of the oracle's."""
     import oracle
@@ -370,14 +370,14 @@ def test_linear_solver_amg_parity(lib, orc, libc, agg_type, smoother, coarse_sol
     for which in ("gpu", "oracle"):
         kw = dict(aggregation=agg_type, smoother=smoother, coarse_dof=60, coarse_solver=coarse_solver)
         _, amg = (capi.solver_params(lib, **kw) if which == "gpu" else oracle.params(**kw))
-        amg.tol, amg.maxit = 1e-10, 300
+        amg.tol, amg.maxit = tol, 400
         x = capi.HostVec(x0.copy())
         bb = capi.HostVec(b)
         libc.srand(1)
         f = lib.linear_solver_amg if which == "gpu" else orc.orc_linear_solver_amg
         out[which] = (f(A.ptr(), bb.ptr(), x.ptr(), C.byref(amg)), x.arr.copy())
     (ig, xg), (io, xo) = out["gpu"], out["oracle"]
-    assert 0 < io <= 300 and abs(ig - io) <= 1
+    assert 0 < io <= 400 and abs(ig - io) <= 1
     assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) < 1e-10
 
 
@@ -400,7 +400,6 @@ def test_precond_object_parity_and_isolation(lib, orc, libc):
     """create_precond_amg / apply_precond against the oracle's precond_amg; the
     hierarchy survives other library calls (its own device arena) and the
     device-array variant gives the same z."""
-    import torch
     import oracle
     ia, ja, va = problems.poisson2d_p1(80)
     n = ia.size - 1
@@ -424,14 +423,32 @@ def test_precond_object_parity_and_isolation(lib, orc, libc):
             # an unrelated call reuses the library's working memory
             y = np.zeros(n)
             assert lib.hz_dcsr_mxv(A.ptr(), capi.dptr(r), capi.dptr(y)) == 0
-        dr = torch.from_numpy(r).cuda()
-        dz = torch.zeros(n, dtype=torch.float64, device="cuda")
-        torch.cuda.synchronize()
-        lib.hz_precond_amg_apply_device(dr.data_ptr(), dz.data_ptr(), pc.contents.data)
-        assert np.array_equal(dz.cpu().numpy(), zg)
+        hip = _hip_runtime()
+        dr, dz = C.c_void_p(), C.c_void_p()
+        assert hip.hipMalloc(C.byref(dr), C.c_size_t(8 * n)) == 0
+        assert hip.hipMalloc(C.byref(dz), C.c_size_t(8 * n)) == 0
+        try:
+            assert hip.hipMemcpy(dr, capi.dptr(r), C.c_size_t(8 * n), 1) == 0   # host to device
+            lib.hz_precond_amg_apply_device(dr, dz, pc.contents.data)
+            zd = np.zeros(n)
+            assert hip.hipMemcpy(capi.dptr(zd), dz, C.c_size_t(8 * n), 2) == 0  # device to host
+            assert np.array_equal(zd, zg)
+        finally:
+            hip.hipFree(dr)
+            hip.hipFree(dz)
     finally:
         lib.hz_precond_free(pc)
         orc.orc_precond_free(h)
+
+
+def _hip_runtime():
+    """The HIP runtime the library itself is linked against (device buffers for
+    the device-pointer entry points)."""
+    hip = C.CDLL("libamdhip64.so.7")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipFree.argtypes = [C.c_void_p]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    return hip
 
 
 def _set_precision(lib, p):
@@ -554,9 +571,13 @@ def _sa_param(params_fn, agg_type, filt, omega=0.67, **kw):
 def test_sa_hierarchy_bitexact(lib, orc, libc, solvers_fixture, agg_type, filt):
     """Smoothed aggregation setup (amg_setup_smoothP_smoothR): aggregates,
     smoothed P (structure, column order and values) and every coarse matrix
-    bit-identical to the oracle."""
+    bit-identical to the oracle.  (Pairwise HEM aggregates make smoothed
+    coarse levels nearly dense, so HEM runs on a smaller 2-D problem.)"""
     import oracle
-    ia, ja, va, _ = solvers_fixture
+    if agg_type == capi.HEM:
+        ia, ja, va = problems.poisson2d_p1(40)
+    else:
+        ia, ja, va, _ = solvers_fixture
     A = capi.HostCSR(ia, ja, va)
     out = []
     for which in ("gpu", "oracle"):
@@ -584,7 +605,11 @@ def test_sa_hierarchy_bitexact(lib, orc, libc, solvers_fixture, agg_type, filt):
                                                (capi.VMB, capi.SMOOTHER_JACOBI)])
 def test_sa_pcg_parity(lib, orc, libc, solvers_fixture, agg_type, smoother):
     import oracle
-    ia, ja, va, b = solvers_fixture
+    if agg_type == capi.HEM:   # smoothed pairwise aggregates: nearly dense coarse levels
+        ia, ja, va = problems.poisson2d_p1(40)
+        b = problems.uniform_rhs(ia.size - 1, seed=3)
+    else:
+        ia, ja, va, b = solvers_fixture
     A = capi.HostCSR(ia, ja, va)
     res = {}
     for which in ("gpu", "oracle"):
@@ -640,3 +665,16 @@ def test_sa_kuhn3d_hierarchy_and_solve(lib, orc, libc):
     (ig, xg), (io, xo) = res["gpu"], res["oracle"]
     assert io > 0 and abs(ig - io) <= 1
     assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) < 1e-10
+
+
+def test_first_call_in_process_keeps_caller_rand_stream():
+    """The first library call of a process brings up the HIP runtime, which
+    consumes libc random() values; the caller's stream must be untouched so
+    that HEC (iarray_shuffle with rand(), array.c:302-318) sees srand(1)."""
+    import subprocess
+    import sys
+    from pathlib import Path
+    script = Path(__file__).parent / "first_call_hec.py"
+    r = subprocess.run([sys.executable, str(script)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert "aggregates_equal=1 rand_equal=1" in r.stdout, r.stdout
