@@ -284,6 +284,14 @@ def _declare(lib):
     lib.param_amg_set.argtypes = [P(AMG_param), P(input_param)]
     lib.python_wrapper_krylov_amg.argtypes = [P(INT), P(INT), P(INT), P(INT), P(REAL), P(REAL), P(REAL),
                                               P(REAL), P(INT), P(INT)]
+    F = C.c_float
+    lib.hz_mlp_init.argtypes = [P(F), C.c_ulonglong]
+    lib.hz_mlp_forward.argtypes = [P(F), P(F), INT, P(F)]
+    lib.hz_mlp_forward.restype = INT
+    lib.hz_mlp_loss_grad.argtypes = [P(F), P(F), P(F), INT, P(F), P(F)]
+    lib.hz_mlp_loss_grad.restype = INT
+    lib.hz_mlp_train.argtypes = [P(F), P(F), P(F), INT, INT, INT, F, C.c_ulonglong, P(F)]
+    lib.hz_mlp_train.restype = INT
     lib.linear_solver_amg.argtypes = [P(dCSRmat), P(dvector), P(dvector), P(AMG_param)]
     lib.linear_solver_amg.restype = INT
     lib.create_precond_amg.argtypes = [P(dCSRmat), P(AMG_param)]
@@ -313,6 +321,15 @@ def load_library(path: os.PathLike | None = None):
     if path is None:
         _LIB = lib
     return lib
+
+
+MLP_INPUTS, MLP_OUTPUTS, MLP_PARAMS = 12, 2, 6288
+
+
+def fptr(arr: np.ndarray):
+    """float32 pointer of a contiguous float32 array."""
+    assert arr.dtype == np.float32 and arr.flags.c_contiguous
+    return arr.ctypes.data_as(C.POINTER(C.c_float))
 
 
 def dptr(arr: np.ndarray, ctype=REAL):

@@ -788,4 +788,126 @@ void hz_precond_free(precond *pc) {
     std::free(pc);
 }
 
+void hz_mlp_init(float *theta, unsigned long long seed) {
+    std::memset(theta, 0, sizeof(float) * HZ_MLP_PARAMS);
+    auto next = [&seed]() {
+        unsigned long long z = (seed += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+    };
+    struct Layer { int off, fin, fout, rows, cols; };
+    const Layer layers[3] = {{0, HZ_MLP_INPUTS, 64, 16, 64}, {16 * 64 + 64, 64, 64, 64, 64},
+                             {16 * 64 + 64 + 64 * 64 + 64, 64, HZ_MLP_OUTPUTS, 64, 16}};
+    for (const Layer &L : layers) {
+        const double a = std::sqrt(6.0 / (L.fin + L.fout));
+        for (int i = 0; i < L.fin; ++i)
+            for (int j = 0; j < L.fout; ++j) theta[L.off + i * L.cols + j] = (float)((2.0 * next() - 1.0) * a);
+    }
+}
+
+INT hz_mlp_forward(const float *theta, const float *x, INT B, float *y) {
+    if (B < 0) return HZ_ERROR_INPUT_PAR;
+    if (B == 0) return HZ_SUCCESS;
+    if (!device_ready()) return HZ_ERROR_DEVICE;
+    try {
+        arena().reset();
+        const float *dth = upload(theta, HZ_MLP_PARAMS);
+        const float *dx = upload(x, (size_t)B * HZ_MLP_INPUTS);
+        MlpWork w = mlp_work(B);
+        float *dy = arena().alloc_n<float>((size_t)B * HZ_MLP_OUTPUTS);
+        mlp_inputs(dx, nullptr, B, w);
+        mlp_forward(dth, w, B);
+        mlp_outputs(B, w, dy);
+        HZ_CHECK(memcpy_sync(y, dy, sizeof(float) * B * HZ_MLP_OUTPUTS, hipMemcpyDeviceToHost));
+        return HZ_SUCCESS;
+    } catch (const DeviceError &) {
+        return HZ_ERROR_DEVICE;
+    }
+}
+
+INT hz_mlp_loss_grad(const float *theta, const float *x, const float *t, INT B, float *grad, float *loss) {
+    if (B <= 0) return HZ_ERROR_INPUT_PAR;
+    if (!device_ready()) return HZ_ERROR_DEVICE;
+    try {
+        arena().reset();
+        const float *dth = upload(theta, HZ_MLP_PARAMS);
+        const float *dx = upload(x, (size_t)B * HZ_MLP_INPUTS);
+        const float *dt = upload(t, (size_t)B * HZ_MLP_OUTPUTS);
+        MlpWork w = mlp_work(B);
+        float *dg = arena().alloc_n<float>(HZ_MLP_PARAMS);
+        float *dl = arena().alloc_n<float>(1);
+        HZ_CHECK(hipMemsetAsync(dg, 0, sizeof(float) * HZ_MLP_PARAMS, stream()));
+        mlp_inputs(dx, nullptr, B, w);
+        mlp_targets(dt, nullptr, B, w);
+        mlp_forward(dth, w, B);
+        mlp_backward(dth, w, B, dg, dl);
+        HZ_CHECK(memcpy_sync(grad, dg, sizeof(float) * HZ_MLP_PARAMS, hipMemcpyDeviceToHost));
+        HZ_CHECK(memcpy_sync(loss, dl, sizeof(float), hipMemcpyDeviceToHost));
+        return HZ_SUCCESS;
+    } catch (const DeviceError &) {
+        return HZ_ERROR_DEVICE;
+    }
+}
+
+INT hz_mlp_train(float *theta, const float *x, const float *t, INT B, INT epochs, INT batch, float lr,
+                 unsigned long long seed, float *loss_hist) {
+    if (B <= 0 || epochs < 0 || batch <= 0) return HZ_ERROR_INPUT_PAR;
+    if (!device_ready()) return HZ_ERROR_DEVICE;
+    try {
+        arena().reset();
+        float *dth = upload(theta, HZ_MLP_PARAMS);
+        const float *dx = upload(x, (size_t)B * HZ_MLP_INPUTS);
+        const float *dt = upload(t, (size_t)B * HZ_MLP_OUTPUTS);
+        const int bs = std::min<int>(batch, B);
+        MlpWork w = mlp_work(std::max<int>(bs, B));
+        float *dg = arena().alloc_n<float>(HZ_MLP_PARAMS);
+        float *dm = arena().alloc_n<float>(HZ_MLP_PARAMS);
+        float *dv = arena().alloc_n<float>(HZ_MLP_PARAMS);
+        float *dl = arena().alloc_n<float>(1);
+        int *didx = arena().alloc_n<int>((size_t)B);
+        HZ_CHECK(hipMemsetAsync(dm, 0, sizeof(float) * HZ_MLP_PARAMS, stream()));
+        HZ_CHECK(hipMemsetAsync(dv, 0, sizeof(float) * HZ_MLP_PARAMS, stream()));
+        auto full_loss = [&]() {
+            mlp_inputs(dx, nullptr, B, w);
+            mlp_targets(dt, nullptr, B, w);
+            mlp_forward(dth, w, B);
+            mlp_backward(dth, w, B, dg, dl);
+            float l = 0.0f;
+            HZ_CHECK(memcpy_sync(&l, dl, sizeof(float), hipMemcpyDeviceToHost));
+            return l;
+        };
+        std::vector<int> perm(B);
+        unsigned long long s = seed;
+        auto next = [&s]() {
+            unsigned long long z = (s += 0x9E3779B97F4A7C15ull);
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            return z ^ (z >> 31);
+        };
+        int step = 0;
+        for (int e = 0; e < epochs; ++e) {
+            loss_hist[e] = full_loss();
+            for (int i = 0; i < B; ++i) perm[i] = i;
+            for (int i = B - 1; i > 0; --i) std::swap(perm[i], perm[next() % (unsigned long long)(i + 1)]);
+            HZ_CHECK(hipMemcpyAsync(didx, perm.data(), sizeof(int) * B, hipMemcpyHostToDevice, stream()));
+            for (int o = 0; o < B; o += bs) {
+                const int nb = std::min(bs, B - o);
+                mlp_inputs(dx, didx + o, nb, w);
+                mlp_targets(dt, didx + o, nb, w);
+                mlp_forward(dth, w, nb);
+                mlp_backward(dth, w, nb, dg, dl);
+                mlp_adam(dth, dg, dm, dv, lr, ++step);
+            }
+            HZ_CHECK(hipStreamSynchronize(stream()));   // perm is reused next epoch
+        }
+        loss_hist[epochs] = full_loss();
+        HZ_CHECK(memcpy_sync(theta, dth, sizeof(float) * HZ_MLP_PARAMS, hipMemcpyDeviceToHost));
+        return HZ_SUCCESS;
+    } catch (const DeviceError &) {
+        return HZ_ERROR_DEVICE;
+    }
+}
+
 }  // extern "C"

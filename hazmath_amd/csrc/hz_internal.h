@@ -136,6 +136,24 @@ void rap_general(const DevCSR &R, const DevCSR &A, const DevCSR &P, DevCSR &C);
 void csr_mxv(const DevCSR &A, const double *x, double *y);
 void csr_prolong(const DevCSR &P, const double *xc, double alpha, const double *alpha_dev, double *x);
 
+// ---- kernels_mlp.hip: the per-aggregate parameter network -----------------
+constexpr int kMlpWgradBlocks = 256;
+struct MlpWork {
+    int cap = 0;   // rows (multiple of 64)
+    float *X = nullptr, *Z1 = nullptr, *H1 = nullptr, *Z2 = nullptr, *H2 = nullptr, *Y = nullptr;
+    float *T = nullptr, *dY = nullptr, *dH = nullptr, *dZ = nullptr, *part = nullptr;
+    double *lpart = nullptr;
+};
+MlpWork mlp_work(int capacity);
+// X = rows idx[0..B) (or 0..B) of x [.][12], padded
+void mlp_inputs(const float *x, const int *idx, int B, const MlpWork &w);
+void mlp_targets(const float *t, const int *idx, int B, const MlpWork &w);
+void mlp_forward(const float *theta, const MlpWork &w, int B);
+void mlp_outputs(int B, const MlpWork &w, float *y);   // y [B][2]
+// gradient of the loss w.r.t. theta (after mlp_forward with the same inputs) and the loss
+void mlp_backward(const float *theta, const MlpWork &w, int B, float *grad, float *loss);
+void mlp_adam(float *theta, const float *grad, float *m, float *v, float lr, int step);
+
 // ---- kernels_mixed.hip: the V-cycle in FP32 / BF16 ------------------------
 constexpr int kLowF32 = 1, kLowBF16 = 2;
 // Low-precision copies of a level's values: SELL-64 order (sharing DevSELL's

@@ -399,6 +399,29 @@ void hz_precond_amg_apply_device(const REAL *dr, REAL *dz, void *data);
  * the reference leaves this to its Python bindings. */
 void hz_precond_free(precond *pc);
 
+/* ---------------------------------------------------------------------------
+ * Per-aggregate parameter network (no HAZmath counterpart): MLP 12 -> 64 -> 64
+ * -> 2 with GELU in FP32, forward/backward on the FP32 matrix cores.  theta
+ * holds HZ_MLP_PARAMS floats: W1[16][64] (input rows 12..15 unused), b1[64],
+ * W2[64][64], b2[64], W3[64][16] (output columns 2..15 unused), b3[16].
+ * Loss: mean squared error over the two outputs.  Host arrays throughout.
+ * ------------------------------------------------------------------------- */
+#define HZ_MLP_INPUTS   12
+#define HZ_MLP_OUTPUTS  2
+#define HZ_MLP_PARAMS   6288
+/* Glorot-uniform weights from splitmix64(seed), zero biases and padding. */
+void hz_mlp_init(float *theta, unsigned long long seed);
+/* y[B][2] = MLP(x[B][12]). */
+INT hz_mlp_forward(const float *theta, const float *x, INT B, float *y);
+/* grad[HZ_MLP_PARAMS] = dLoss/dtheta and *loss for targets t[B][2]. */
+INT hz_mlp_loss_grad(const float *theta, const float *x, const float *t, INT B, float *grad, float *loss);
+/* Adam (lr, beta 0.9/0.999) over `epochs` shuffled passes with mini-batches of
+ * `batch` rows (shuffles from splitmix64(seed)); theta is updated in place and
+ * loss_hist[e] (epochs+1 entries) holds the full-data loss before epoch e and
+ * after the last one. */
+INT hz_mlp_train(float *theta, const float *x, const float *t, INT B, INT epochs, INT batch, float lr,
+                 unsigned long long seed, float *loss_hist);
+
 /* Runtime information. */
 const char *hz_version(void);
 INT hz_device_count(void);

@@ -277,3 +277,20 @@ def test_param_setters_copy_like_reference(tmp_path):
     lib.param_amg_set(C.byref(amg), C.byref(p2))
     assert (it.linear_maxit, it.linear_tol) == (4, 1e-3)
     assert (amg.maxit, amg.tol) == (40, 1e-5)
+
+
+def test_mlp_init_layout():
+    """hz_mlp_init: deterministic, padding and biases zero, Glorot range."""
+    from oracle import mlp
+    lib = capi.load_library()
+    a = np.zeros(capi.MLP_PARAMS, np.float32)
+    b = np.zeros(capi.MLP_PARAMS, np.float32)
+    lib.hz_mlp_init(capi.fptr(a), 7)
+    lib.hz_mlp_init(capi.fptr(b), 7)
+    assert np.array_equal(a, b)
+    assert capi.MLP_PARAMS == mlp.N_PARAMS
+    assert np.all(a[mlp.OFF_W1:mlp.OFF_B1].reshape(16, 64)[12:] == 0)
+    assert np.all(a[mlp.OFF_W3:mlp.OFF_B3].reshape(64, 16)[:, 2:] == 0)
+    assert np.all(a[mlp.OFF_B1:mlp.OFF_W2] == 0) and np.all(a[mlp.OFF_B3:] == 0)
+    w1 = a[mlp.OFF_W1:mlp.OFF_B1].reshape(16, 64)[:12]
+    assert np.abs(w1).max() <= np.sqrt(6.0 / 76) and np.abs(w1).std() > 0.05

@@ -447,3 +447,57 @@ def test_oracle_sa_pcg_beats_ua(orc, libc, solvers_fixture):
         Asp = sp.csr_matrix((va, ja, ia))
         assert np.linalg.norm(b - Asp @ x.arr) / np.linalg.norm(b) < 1e-10
     assert 0 < its[capi.SA_AMG] < its[capi.UA_AMG], its
+
+
+def _mlp_case(seed=0, B=37):
+    from oracle import mlp
+    rng = np.random.default_rng(seed)
+    theta = np.zeros(mlp.N_PARAMS)
+    for a, b_ in ((mlp.OFF_W1, mlp.OFF_B1), (mlp.OFF_W2, mlp.OFF_B2), (mlp.OFF_W3, mlp.OFF_B3)):
+        theta[a:b_] = rng.standard_normal(b_ - a) * 0.3
+    theta[mlp.OFF_B1:mlp.OFF_W2] = rng.standard_normal(64) * 0.1
+    # padding stays zero
+    theta[mlp.OFF_W1:mlp.OFF_B1].reshape(16, 64)[12:] = 0.0
+    theta[mlp.OFF_W3:mlp.OFF_B3].reshape(64, 16)[:, 2:] = 0.0
+    x = rng.standard_normal((B, 12))
+    t = rng.standard_normal((B, 2))
+    return theta, x, t
+
+
+def test_mlp_oracle_gradient_matches_finite_differences():
+    from oracle import mlp
+    theta, x, t = _mlp_case()
+    _, g = mlp.loss_grad(theta, x, t)
+    rng = np.random.default_rng(1)
+    idx = np.concatenate([rng.choice(np.arange(mlp.OFF_W1, mlp.OFF_B1).reshape(16, 64)[:12].ravel(), 8),
+                          rng.choice(np.arange(mlp.OFF_B1, mlp.OFF_W2), 4),
+                          rng.choice(np.arange(mlp.OFF_W2, mlp.OFF_B2), 8),
+                          rng.choice(np.arange(mlp.OFF_B2, mlp.OFF_W3), 4),
+                          rng.choice(np.arange(mlp.OFF_W3, mlp.OFF_B3).reshape(64, 16)[:, :2].ravel(), 8),
+                          np.arange(mlp.OFF_B3, mlp.OFF_B3 + 2)])
+    for i in idx:
+        h = 1e-6
+        tp, tm = theta.copy(), theta.copy()
+        tp[i] += h
+        tm[i] -= h
+        fd = (mlp.loss(tp, x, t) - mlp.loss(tm, x, t)) / (2 * h)
+        assert abs(fd - g[i]) < 1e-7 + 1e-6 * abs(fd), (i, fd, g[i])
+
+
+def test_mlp_oracle_matches_torch_autograd():
+    import torch
+    from oracle import mlp
+    theta, x, t = _mlp_case(seed=3, B=50)
+    loss, g = mlp.loss_grad(theta, x, t)
+    th = torch.tensor(theta, dtype=torch.float64, requires_grad=True)
+    X, T = torch.tensor(x), torch.tensor(t)
+    W1 = th[mlp.OFF_W1:mlp.OFF_B1].reshape(16, 64)[:12]
+    W2 = th[mlp.OFF_W2:mlp.OFF_B2].reshape(64, 64)
+    W3 = th[mlp.OFF_W3:mlp.OFF_B3].reshape(64, 16)[:, :2]
+    gelu = torch.nn.functional.gelu
+    y = gelu(gelu(X @ W1 + th[mlp.OFF_B1:mlp.OFF_W2]) @ W2 + th[mlp.OFF_B2:mlp.OFF_W3]) @ W3 \
+        + th[mlp.OFF_B3:mlp.OFF_B3 + 2]
+    L = ((y - T) ** 2).mean()
+    L.backward()
+    assert abs(L.item() - loss) < 1e-12
+    assert np.allclose(th.grad.numpy(), g, atol=1e-12, rtol=1e-10)
