@@ -44,6 +44,7 @@ SA_AMG = 2
 VMB, HEC, HEM = 1, 4, 5
 V_CYCLE, W_CYCLE = 1, 2
 SMOOTHER_JACOBI, SMOOTHER_GS, SMOOTHER_L1DIAG = 1, 2, 10
+SMOOTHER_CHEBY = 91
 
 
 class dCSRmat(C.Structure):
@@ -88,7 +89,7 @@ class hz_solve_info(C.Structure):
                 ("level_rows", INT * MAX_AMG_LVL), ("level_nnz", INT * MAX_AMG_LVL),
                 ("setup_ms", REAL), ("solve_ms", REAL), ("total_ms", REAL),
                 ("final_relres", REAL), ("condest", REAL), ("relres_count", INT),
-                ("relres_hist", REAL * 4096)]
+                ("relres_hist", REAL * 4096), ("fallback_used", INT)]
 
 
 class hz_hierarchy(C.Structure):
@@ -292,6 +293,17 @@ def _declare(lib):
     lib.hz_mlp_loss_grad.restype = INT
     lib.hz_mlp_train.argtypes = [P(F), P(F), P(F), INT, INT, INT, F, C.c_ulonglong, P(F)]
     lib.hz_mlp_train.restype = INT
+    lib.hz_krylov_amg_learned.argtypes = [P(dCSRmat), P(dvector), P(dvector), P(linear_itsolver_param),
+                                          P(AMG_param), P(F), INT]
+    lib.hz_krylov_amg_learned.restype = INT
+    lib.hz_two_grid_rate.argtypes = [P(dCSRmat), P(AMG_param), REAL, REAL, INT, P(REAL)]
+    lib.hz_two_grid_rate.restype = INT
+    lib.hz_aggregate_features.argtypes = [P(dCSRmat), P(INT), INT, INT, P(F)]
+    lib.hz_aggregate_features.restype = INT
+    lib.hz_cheby_smooth.argtypes = [P(dCSRmat), P(REAL), P(REAL), INT, REAL]
+    lib.hz_cheby_smooth.restype = INT
+    lib.hz_sa_prolongation.argtypes = [P(dCSRmat), P(INT), INT, REAL, REAL, P(F), INT, P(dCSRmat)]
+    lib.hz_sa_prolongation.restype = INT
     lib.linear_solver_amg.argtypes = [P(dCSRmat), P(dvector), P(dvector), P(AMG_param)]
     lib.linear_solver_amg.restype = INT
     lib.create_precond_amg.argtypes = [P(dCSRmat), P(AMG_param)]

@@ -156,7 +156,27 @@ int amg_setup(Hierarchy &H, const DevCSR &A0, AMG_param *param, bool) {
         Level &C = H.lv[lvl + 1];
         if (param->AMG_type == HZ_SA_AMG) {
             // amg_setup_smoothP_smoothR: smoothed P, R = P^T, C = R A P
-            sa_prolongation(L.A, N, L.agg, nagg, param->tentative_smooth, param->smooth_filter, L.P);
+            const float *omega_agg = nullptr;
+            if (H.mlp_theta) {
+                // learned parameters: features of the aggregates -> network ->
+                // per-aggregate damping and this level's Chebyshev bound scale
+                int *rp = arena().alloc_n<int>((size_t)nagg + 1);
+                int *rm = arena().alloc_n<int>((size_t)n + 1);
+                build_restriction(n, L.agg, nagg, rp, rm);
+                float *feat = arena().alloc_n<float>((size_t)nagg * HZ_MLP_INPUTS + 1);
+                aggregate_features(L.A, L.agg, nagg, rp, rm, lvl, feat);
+                MlpWork w = mlp_work(nagg);
+                float *out = arena().alloc_n<float>((size_t)nagg * HZ_MLP_OUTPUTS + 1);
+                mlp_inputs(feat, nullptr, nagg, w);
+                mlp_forward(H.mlp_theta, w, nagg);
+                mlp_outputs(nagg, w, out);
+                float *om = arena().alloc_n<float>((size_t)nagg + 1);
+                const double scale = learned_parameters(nagg, out, om);
+                if ((int)H.cheb_scale.size() < max_levels) H.cheb_scale.assign(max_levels, 1.0);
+                H.cheb_scale[lvl] = scale;
+                omega_agg = om;
+            }
+            sa_prolongation(L.A, N, L.agg, nagg, param->tentative_smooth, param->smooth_filter, L.P, omega_agg);
             csr_transpose_gpu(L.P, L.R);
             rap_general(L.R, L.A, L.P, C.A);
             L.sa = true;
@@ -245,6 +265,11 @@ void amg_prepare_cycle(Hierarchy &H, const AMG_param *param) {
         if (l < H.nl - 1) {
             L.dsm = arena().alloc_n<double>(n);
             smoother_diag(L.S, cp.smoother, L.dsm);
+            if (cp.smoother == HZ_SMOOTHER_CHEBY) {
+                const double sc = l < (int)H.cheb_scale.size() ? H.cheb_scale[l] : 1.0;
+                L.cheb = cheby_prepare(L.A, sc);
+                L.cwork = arena().alloc_n<double>(n);
+            }
         }
         if (cp.precision > 0) {
             L.bf = arena().alloc_n<float>(n);
@@ -273,6 +298,13 @@ void presmooth(Hierarchy &H, int l, CycleState &cs) {
     Level &L = H.lv[l];
     const CycleParams &cp = H.cp;
     int sweeps = cp.presmooth;
+    if (cp.smoother == HZ_SMOOTHER_CHEBY) {
+        if (cs.zero[l]) fill(L.x, 0.0, L.A.n);
+        cs.zero[l] = false;
+        cs.first_done[l] = false;
+        cheby_apply(L.S, L.cheb, sweeps, L.b, L.x, L.xt, L.cwork);
+        return;
+    }
     if (cs.zero[l]) {
         if (sweeps == 0) {
             fill(L.x, 0.0, L.A.n);
@@ -311,9 +343,10 @@ void prolong_postsmooth(Hierarchy &H, int l, double alpha, const double *alpha_d
     Level &L = H.lv[l];
     Level &C = H.lv[l + 1];
     const CycleParams &cp = H.cp;
-    if (cp.postsmooth <= 0) {
+    if (cp.postsmooth <= 0 || cp.smoother == HZ_SMOOTHER_CHEBY) {
         if (L.sa) csr_prolong(L.P, C.x, alpha, alpha_dev, L.x);
         else prolong_agg(L.A.n, L.agg, C.x, alpha, alpha_dev, L.x);
+        if (cp.smoother == HZ_SMOOTHER_CHEBY) cheby_apply(L.S, L.cheb, cp.postsmooth, L.b, L.x, L.xt, L.cwork);
         return;
     }
     const bool want_dot = dot_out && cp.postsmooth == 1;
@@ -359,7 +392,7 @@ forward:
         cs.num_lvl[l]++;
         presmooth(H, l, cs);
         // the next level restarts from zero; fuse its first sweep into the restriction
-        const bool fuse = (l + 1 < nl - 1) && cp.presmooth > 0 && !L.sa;
+        const bool fuse = (l + 1 < nl - 1) && cp.presmooth > 0 && !L.sa && cp.smoother != HZ_SMOOTHER_CHEBY;
         if (L.sa) {
             residual(L.S, L.b, L.x, L.w);
             csr_mxv(L.R, L.w, C.b);    // dcsr_mxv(R), mgcycle.c:302
@@ -931,6 +964,59 @@ AmgSolveResult amg_solve(Hierarchy &H, const double *b, double *x, double tol, i
     res.iter = iter;
     res.relres = relres1;
     return res;
+}
+
+// ---------------------------------------------------------------------------
+// Error contraction of the cycle as a stationary iteration: b = 0, x_0 random
+// (splitmix64(seed)), rate = (||x_K||_A / ||x_{K/2}||_A)^(1/(K - K/2)), an
+// estimate of the asymptotic factor.  Used to label parameter choices by their
+// two-grid convergence factor.
+// ---------------------------------------------------------------------------
+double error_contraction(Hierarchy &H, int iters, unsigned long long seed) {
+    Level &L0 = H.lv[0];
+    const int n = L0.A.n;
+    std::vector<double> hx(n);
+    unsigned long long s = seed;
+    for (int i = 0; i < n; ++i) {
+        unsigned long long z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        hx[i] = 2.0 * ((double)(z >> 11) * (1.0 / 9007199254740992.0)) - 1.0;
+    }
+    double *x = arena().alloc_n<double>((size_t)n + 1);
+    double *b = arena().alloc_n<double>((size_t)n + 1);
+    double *w = arena().alloc_n<double>((size_t)n + 1);
+    HZ_CHECK(memcpy_sync(x, hx.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    fill(b, 0.0, n);
+    PcgScalars hs{};
+    auto anorm = [&](const double *v) {
+        spmv(L0.S, v, w, &H.sc->rr, H.ws);
+        fetch(H, hs);
+        return std::sqrt(std::fabs(hs.rr));
+    };
+    const double e0 = anorm(x);
+    L0.b = b;
+    L0.x = x;
+    L0.xt = H.x0buf;
+    CycleState cs;
+    for (int i = 0; i < HZ_MAX_AMG_LVL; ++i) {
+        cs.zero[i] = false;
+        cs.first_done[i] = false;
+    }
+    // the first half of the iterations removes the fast transient; the rate is
+    // measured over the second half
+    const int half = iters / 2;
+    double eh = e0;
+    for (int it = 0; it < iters; ++it) {
+        bool unused = false;
+        mgcycle(H, cs, nullptr, &unused);
+        if (it + 1 == half) eh = anorm(L0.x);
+    }
+    const double ek = anorm(L0.x);
+    L0.x = L0.xt = L0.b = nullptr;
+    if (eh == 0.0 || iters - half <= 0) return 0.0;
+    return std::pow(ek / eh, 1.0 / (iters - half));
 }
 
 }  // namespace hz

@@ -19,6 +19,8 @@ struct Level {
     double theta = 0.0;       // strong_coupled used to aggregate this level
     bool sa = false;          // smoothed aggregation: transfers use P and R below
     DevCSR P, R;
+    ChebLevel cheb;           // Chebyshev smoother data
+    double *cwork = nullptr;  // Chebyshev direction vector
     double *Ainv = nullptr;   // coarsest, coarse_solver 32
     double *cgwork = nullptr; // coarsest, coarse_solver 0
     // FP32 cycle (CycleParams::precision > 0)
@@ -48,6 +50,10 @@ struct Hierarchy {
     double *alpha_cs = nullptr;   // coarse scaling factor (device)
     double *x0buf = nullptr;      // level-0 ping-pong partner of the preconditioner output
     int hec_rng = 0;              // 0: GPU stream from the libc state (checked), 1: host rand()
+    // learned parameters: when set (SA only), each level's per-aggregate damping
+    // and Chebyshev bound scale come from the network with these weights
+    const float *mlp_theta = nullptr;
+    std::vector<double> cheb_scale;   // per level (default 1.0)
 };
 
 // Runs the reference's setup loop on the device.  A0 arrays are borrowed.
@@ -79,6 +85,9 @@ struct AmgSolveResult {
 // initial guess on entry.  Returns the reference's iteration count (MaxIt+1
 // when it did not converge).
 AmgSolveResult amg_solve(Hierarchy &H, const double *b, double *x, double tol, int MaxIt, int prt);
+
+// (||x_K||_A / ||x_{K/2}||_A)^(1/(K-K/2)) of the cycle iterated on A x = 0 from a random x_0.
+double error_contraction(Hierarchy &H, int iters, unsigned long long seed);
 
 // Host-staged sequential aggregation for VMB (and HEM on non-symmetric patterns).
 int aggregate_host(const DevCSR &N, int type, int max_aggregation, int *agg_dev, int *nagg);

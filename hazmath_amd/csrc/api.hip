@@ -138,12 +138,17 @@ int fatal(int code) {
 // Features of AMG_param this build implements on the GPU.
 int check_supported(const linear_itsolver_param *it, const AMG_param *amg) {
     (void)it;
-    if (amg->smoother != HZ_SMOOTHER_JACOBI && amg->smoother != HZ_SMOOTHER_L1DIAG) {
+    if (amg->smoother != HZ_SMOOTHER_JACOBI && amg->smoother != HZ_SMOOTHER_L1DIAG &&
+        amg->smoother != HZ_SMOOTHER_CHEBY) {
         std::printf("### ERROR: Wrong smoother type %lld!\n", (long long)amg->smoother);
         return HZ_ERROR_AMG_SMOOTH_TYPE;
     }
     if (amg->AMG_type != HZ_UA_AMG && amg->AMG_type != HZ_SA_AMG) {
         std::printf("### ERROR: AMG type %lld is not available in this build\n", (long long)amg->AMG_type);
+        return HZ_ERROR_INPUT_PAR;
+    }
+    if (amg->smoother == HZ_SMOOTHER_CHEBY && g_opt.vcycle_precision > 0) {
+        std::printf("### ERROR: the Chebyshev smoother needs the FP64 cycle (vcycle_precision = 0)\n");
         return HZ_ERROR_INPUT_PAR;
     }
     if (amg->AMG_type == HZ_SA_AMG && g_opt.vcycle_precision > 0) {
@@ -171,7 +176,7 @@ int check_supported(const linear_itsolver_param *it, const AMG_param *amg) {
 
 // linear_solver_dcsr_krylov_amg (itsolver.c:846-931) on device-resident data.
 INT core_solve(const DevCSR &A, const double *db, double *dx, linear_itsolver_param *it,
-               AMG_param *amg) {
+               AMG_param *amg, const float *mlp_theta = nullptr) {
     std::memset(&g_info, 0, sizeof(g_info));
     const short prt = it->linear_print_level;
     hipEvent_t e0, e1, e2;
@@ -185,6 +190,7 @@ INT core_solve(const DevCSR &A, const double *db, double *dx, linear_itsolver_pa
     Hierarchy H;
     H.cp.jacobi_w = g_opt.jacobi_weight;
     H.hec_rng = g_opt.hec_rng_source;
+    H.mlp_theta = mlp_theta;
     st = amg_setup(H, A, amg, false);
     if (st <= -1000) return fatal(st);
     if (st < 0) return st;
@@ -904,6 +910,143 @@ INT hz_mlp_train(float *theta, const float *x, const float *t, INT B, INT epochs
         }
         loss_hist[epochs] = full_loss();
         HZ_CHECK(memcpy_sync(theta, dth, sizeof(float) * HZ_MLP_PARAMS, hipMemcpyDeviceToHost));
+        return HZ_SUCCESS;
+    } catch (const DeviceError &) {
+        return HZ_ERROR_DEVICE;
+    }
+}
+
+INT hz_krylov_amg_learned(dCSRmat *A, dvector *b, dvector *x, linear_itsolver_param *itparam,
+                          AMG_param *amgparam, const float *theta, INT budget) {
+    if (!device_ready()) return HZ_ERROR_DEVICE;
+    try {
+        arena().reset();
+        const size_t n = (size_t)A->row;
+        const DevCSR dA = upload_csr(A);
+        const double *db = upload(b->val, n);
+        double *dx = upload(x->val, n);
+        double *dx0 = upload(x->val, n);
+        const float *dth = upload(theta, HZ_MLP_PARAMS);
+        // learned method: SA with network damping + Chebyshev smoother
+        AMG_param lp = *amgparam;
+        lp.AMG_type = HZ_SA_AMG;
+        lp.smoother = HZ_SMOOTHER_CHEBY;
+        linear_itsolver_param li = *itparam;
+        li.linear_maxit = std::min<int>(itparam->linear_maxit, budget);
+        INT st = core_solve(dA, db, dx, &li, &lp, dth);
+        bool ok = st >= 0 && st <= budget;
+        if (ok) {
+            // the FP64 true relative residual is the judge
+            DevSELL S;
+            sell_from_csr(dA, S);
+            double *w = arena().alloc_n<double>(n + 1);
+            residual(S, db, dx, w);
+            std::vector<double> hw(n), hb(n);
+            HZ_CHECK(memcpy_sync(hw.data(), w, sizeof(double) * n, hipMemcpyDeviceToHost));
+            double rr = 0.0, bb = 0.0;
+            for (size_t i = 0; i < n; ++i) {
+                rr += hw[i] * hw[i];
+                bb += b->val[i] * b->val[i];
+            }
+            ok = std::sqrt(rr) <= itparam->linear_tol * std::max(std::sqrt(bb), 1e-300) * 1.0000001;
+        }
+        const hz_solve_info learned_info = g_info;
+        if (!ok) {
+            HZ_CHECK(hipMemcpyAsync(dx, dx0, sizeof(double) * n, hipMemcpyDeviceToDevice, stream()));
+            AMG_param pp = *amgparam;
+            st = core_solve(dA, db, dx, itparam, &pp);
+            g_info.fallback_used = 1;
+        } else {
+            g_info = learned_info;
+            g_info.fallback_used = 0;
+        }
+        HZ_CHECK(memcpy_sync(x->val, dx, sizeof(double) * n, hipMemcpyDeviceToHost));
+        return st;
+    } catch (const DeviceError &) {
+        return HZ_ERROR_DEVICE;
+    }
+}
+
+INT hz_two_grid_rate(dCSRmat *A, AMG_param *amgparam, REAL omega, REAL lscale, INT iters, REAL *rate) {
+    *rate = 0.0;
+    if (!device_ready()) return HZ_ERROR_DEVICE;
+    try {
+        arena().reset();
+        const DevCSR dA = upload_csr(A);
+        AMG_param p = *amgparam;
+        p.AMG_type = HZ_SA_AMG;
+        p.tentative_smooth = omega;
+        p.max_levels = 2;
+        p.coarse_dof = 1;
+        p.coarse_solver = HZ_SOLVER_UMFPACK;
+        p.smoother = HZ_SMOOTHER_CHEBY;
+        p.cycle_type = HZ_V_CYCLE;
+        Hierarchy H;
+        H.hec_rng = g_opt.hec_rng_source;
+        H.cheb_scale.assign(2, lscale);
+        const int st = amg_setup(H, dA, &p, false);
+        if (st <= -1000) return fatal(st);
+        if (st < 0) return st;
+        amg_prepare_cycle(H, &p);
+        *rate = error_contraction(H, iters, 0x5eedULL);
+        return HZ_SUCCESS;
+    } catch (const DeviceError &) {
+        return HZ_ERROR_DEVICE;
+    }
+}
+
+INT hz_aggregate_features(dCSRmat *A, const INT *aggregates, INT num_aggregates, INT level, float *feat) {
+    if (!device_ready()) return HZ_ERROR_DEVICE;
+    try {
+        arena().reset();
+        const DevCSR dA = upload_csr(A);
+        int *dagg = upload(aggregates, (size_t)A->row);
+        int *rp = arena().alloc_n<int>((size_t)num_aggregates + 1);
+        int *rm = arena().alloc_n<int>((size_t)A->row + 1);
+        build_restriction(A->row, dagg, num_aggregates, rp, rm);
+        float *df = arena().alloc_n<float>((size_t)num_aggregates * HZ_MLP_INPUTS + 1);
+        aggregate_features(dA, dagg, num_aggregates, rp, rm, level, df);
+        HZ_CHECK(memcpy_sync(feat, df, sizeof(float) * num_aggregates * HZ_MLP_INPUTS, hipMemcpyDeviceToHost));
+        return HZ_SUCCESS;
+    } catch (const DeviceError &) {
+        return HZ_ERROR_DEVICE;
+    }
+}
+
+INT hz_cheby_smooth(dCSRmat *A, const REAL *b, REAL *x, INT steps, REAL scale) {
+    if (!device_ready()) return HZ_ERROR_DEVICE;
+    try {
+        arena().reset();
+        const DevCSR dA = upload_csr(A);
+        DevSELL S;
+        sell_from_csr(dA, S);
+        const double *db = upload(b, (size_t)A->row);
+        double *dx = upload(x, (size_t)A->row);
+        double *dt = arena().alloc_n<double>((size_t)A->row + 1);
+        double *dd = arena().alloc_n<double>((size_t)A->row + 1);
+        const ChebLevel c = cheby_prepare(dA, scale);
+        cheby_apply(S, c, steps, db, dx, dt, dd);
+        HZ_CHECK(memcpy_sync(x, dx, sizeof(double) * A->row, hipMemcpyDeviceToHost));
+        return HZ_SUCCESS;
+    } catch (const DeviceError &) {
+        return HZ_ERROR_DEVICE;
+    }
+}
+
+INT hz_sa_prolongation(dCSRmat *A, const INT *aggregates, INT num_aggregates, REAL theta, REAL omega,
+                       const float *omega_agg, INT filter, dCSRmat *P) {
+    if (!device_ready()) return HZ_ERROR_DEVICE;
+    try {
+        arena().reset();
+        const DevCSR dA = upload_csr(A);
+        int *dagg = upload(aggregates, (size_t)A->row);
+        int *counts = arena().alloc_n<int>((size_t)A->row + 1);
+        DevCSR N;
+        strength_graph(dA, theta, N, counts);
+        const float *dom = omega_agg ? upload(omega_agg, (size_t)num_aggregates) : nullptr;
+        DevCSR dP;
+        sa_prolongation(dA, N, dagg, num_aggregates, omega, filter, dP, dom);
+        download_csr(dP, P);
         return HZ_SUCCESS;
     } catch (const DeviceError &) {
         return HZ_ERROR_DEVICE;

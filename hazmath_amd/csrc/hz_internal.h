@@ -125,9 +125,10 @@ void coarse_cg(const DevSELL &S, const double *b, double *x, double tol, int max
 
 // ---- kernels_sa.hip: smoothed aggregation ---------------------------------
 // P = (I - omega D^-1 M) P_tent with M = A (filter 0) or the filtered strength
-// graph N (filter 1; N's diagonal is modified in place).
+// graph N (filter 1; N's diagonal is modified in place).  omega_agg (optional):
+// row i uses omega_agg[agg[i]] instead of omega.
 void sa_prolongation(const DevCSR &A, DevCSR &N, const int *agg, int nagg, double omega, int filter,
-                     DevCSR &P);
+                     DevCSR &P, const float *omega_agg = nullptr);
 // AT = A^T with every row listing its source rows in increasing order (dcsr_trans).
 void csr_transpose_gpu(const DevCSR &A, DevCSR &AT);
 // C = R A P with the layout and summation order of dcsr_rap.
@@ -135,6 +136,23 @@ void rap_general(const DevCSR &R, const DevCSR &A, const DevCSR &P, DevCSR &C);
 // y = A x (row sums from 0.0, CSR order) and x += alpha P xc (dcsr_aAxpy).
 void csr_mxv(const DevCSR &A, const double *x, double *y);
 void csr_prolong(const DevCSR &P, const double *xc, double alpha, const double *alpha_dev, double *x);
+
+// ---- kernels_learned.hip: Chebyshev smoother, aggregate features ----------
+struct ChebLevel {
+    double *diag = nullptr;   // first diagonal entry per row
+    double lmax = 0.0, lmin = 0.0;
+};
+// diag and lmax = scale * Gershgorin bound of D^-1 A, lmin = lmax / 30
+ChebLevel cheby_prepare(const DevCSR &A, double scale);
+// `steps` Chebyshev steps from x (x and xt swap roles each step; x ends on the result)
+void cheby_apply(const DevSELL &S, const ChebLevel &c, int steps, const double *b, double *&x, double *&xt,
+                 double *dwork);
+// network outputs [nagg][2] -> omega[c] = clamp(out[c][0], 0, 1.6); returns
+// the level's Chebyshev bound scale clamp(mean_c out[c][1], 0.2, 1.5)
+double learned_parameters(int nagg, const float *out, float *omega);
+// feat [nagg][12] (FP32) of level `level`
+void aggregate_features(const DevCSR &A, const int *agg, int nagg, const int *rptr, const int *rmem, int level,
+                        float *feat);
 
 // ---- kernels_mlp.hip: the per-aggregate parameter network -----------------
 constexpr int kMlpWgradBlocks = 256;

@@ -1,0 +1,260 @@
+// kernels_learned.hip -- pieces of the learned-parameter AMG that are not in
+// the reference: the Chebyshev smoother (HZ_SMOOTHER_CHEBY) and the
+// per-aggregate features the parameter network reads.
+//
+// Chebyshev smoother: k steps of the Chebyshev iteration for A x = b with the
+// Jacobi preconditioner D = diag(A) on the interval [lmin, lmax] of D^-1 A
+// (Saad, Iterative Methods for Sparse Linear Systems, 2nd ed., Alg. 12.1):
+//   theta = (lmax+lmin)/2, delta = (lmax-lmin)/2, sigma = theta/delta,
+//   rho_0 = 1/sigma, d_0 = D^-1 r_0 / theta,
+//   rho_k = 1/(2 sigma - rho_{k-1}), d_k = rho_k rho_{k-1} d_{k-1} + (2 rho_k/delta) D^-1 r_k,
+//   x_{k+1} = x_k + d_k,  r = b - A x recomputed every step.
+// lmax = scale * (Gershgorin bound max_i sum_j |a_ij| / |a_ii|), lmin = lmax / 30.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "hz_internal.h"
+
+namespace hz {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kSlicesPerBlock = kBlock / kWave;
+
+inline int grid_for(long n) {
+    long g = (n + kBlock - 1) / kBlock;
+    return (int)std::max(1L, std::min<long>(g, 4096));
+}
+
+// d_i = first diagonal entry; partials of max_i sum_j |a_ij| / |d_i|
+__global__ __launch_bounds__(kBlock) void k_cheb_diag(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                                                      const double *__restrict__ a, double *__restrict__ d,
+                                                      double *__restrict__ part) {
+    __shared__ double sh[kBlock / kWave];
+    double m = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double di = 0.0, s = 0.0;
+        bool found = false;
+        for (int k = ia[i]; k < ia[i + 1]; ++k) {
+            s += fabs(a[k]);
+            if (!found && ja[k] == i) {
+                di = a[k];
+                found = true;
+            }
+        }
+        d[i] = di;
+        if (di != 0.0) m = fmax(m, s / fabs(di));
+    }
+    const double bm = block_max<kBlock>(m, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = bm;
+}
+
+__global__ void k_max_final(int G, const double *__restrict__ part, double *__restrict__ out) {
+    if (threadIdx.x == 0) {
+        double m = 0.0;
+        for (int g = 0; g < G; ++g) m = fmax(m, part[g]);
+        *out = m;
+    }
+}
+
+// one Chebyshev step: xout = x + dnew, dnew = c1 d + c2 (b - A x) / diag
+__global__ __launch_bounds__(kBlock) void k_cheb_step(int n, int nslices, const long long *__restrict__ sptr,
+                                                      const int *__restrict__ col, const double *__restrict__ val,
+                                                      const double *__restrict__ diag, double c1, double c2,
+                                                      const double *__restrict__ b, const double *__restrict__ x,
+                                                      double *__restrict__ xout, double *__restrict__ d) {
+    const int s = xcd_block_id() * kSlicesPerBlock + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (s >= nslices) return;
+    const int r = s * kSlice + lane;
+    const long long base = sptr[s];
+    const int wd = (int)((sptr[s + 1] - base) / kSlice);
+    const int *c = col + base + lane;
+    const double *v = val + base + lane;
+    double t = (r < n) ? b[r] : 0.0;
+    int k = 0;
+    for (; k + 4 <= wd; k += 4) {
+        int cc[4];
+        double vv[4], xx[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            cc[u] = c[(k + u) * kSlice];
+            vv[u] = v[(k + u) * kSlice];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) xx[u] = cc[u] >= 0 ? x[cc[u]] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (cc[u] >= 0) t = t - vv[u] * xx[u];
+    }
+    for (; k < wd; ++k) {
+        const int cj = c[k * kSlice];
+        if (cj >= 0) t = t - v[k * kSlice] * x[cj];
+    }
+    if (r < n) {
+        const double di = diag[r];
+        const double dn = (c1 == 0.0 ? 0.0 : c1 * d[r]) + (di != 0.0 ? c2 * t / di : 0.0);
+        d[r] = dn;
+        xout[r] = x[r] + dn;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Per-aggregate features (12, FP32), averaged over the aggregate's members
+// (R rows).  Per row i with a_ii the first diagonal entry, s = sum_{j!=i}|a_ij|,
+// q = sum_{j!=i} a_ij, mx / mn the largest / smallest nonzero |a_ij| (j != i),
+// pos = sum of positive off-diagonal entries, in = sum_{j!=i, agg j = agg i}|a_ij|:
+//  0 log2(size)      1 mean s/a_ii     2 mean q/a_ii    3 mean mx/s
+//  4 mean mn/mx      5 mean in/s       6 mean pos/s     7 mean row length / 16
+//  8 level / 10      9 std of log10|a_ii|  10 max mx/s  11 min s/a_ii
+// (ratios with a zero denominator count as 0).
+// ---------------------------------------------------------------------------
+__global__ void k_row_stats(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                            const double *__restrict__ a, const int *__restrict__ agg, double *__restrict__ rs) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double di = 0.0, s = 0.0, q = 0.0, mx = 0.0, mn = 0.0, pos = 0.0, in = 0.0;
+        bool found = false;
+        const int gi = agg[i];
+        for (int k = ia[i]; k < ia[i + 1]; ++k) {
+            const int j = ja[k];
+            const double v = a[k];
+            if (j == i) {
+                if (!found) {
+                    di = v;
+                    found = true;
+                }
+                continue;
+            }
+            const double av = fabs(v);
+            s += av;
+            q += v;
+            if (v > 0.0) pos += v;
+            if (av > 0.0) {
+                mx = fmax(mx, av);
+                mn = (mn == 0.0) ? av : fmin(mn, av);
+            }
+            if (gi > HZ_UNPT && agg[j] == gi) in += av;
+        }
+        double *o = rs + (size_t)i * 9;
+        const double ad = fabs(di);
+        o[0] = ad > 0.0 ? s / ad : 0.0;
+        o[1] = ad > 0.0 ? q / ad : 0.0;
+        o[2] = s > 0.0 ? mx / s : 0.0;
+        o[3] = mx > 0.0 ? mn / mx : 0.0;
+        o[4] = s > 0.0 ? in / s : 0.0;
+        o[5] = s > 0.0 ? pos / s : 0.0;
+        o[6] = (double)(ia[i + 1] - ia[i]) / 16.0;
+        o[7] = ad > 0.0 ? log10(ad) : 0.0;
+        o[8] = 0.0;
+    }
+}
+
+__global__ void k_agg_features(int nagg, const int *__restrict__ rptr, const int *__restrict__ rmem,
+                               const double *__restrict__ rs, int level, float *__restrict__ feat) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nagg; c += gridDim.x * blockDim.x) {
+        const int b = rptr[c], e = rptr[c + 1], m = e - b;
+        double sum[7] = {0, 0, 0, 0, 0, 0, 0}, lg = 0.0, lg2 = 0.0, mxs = 0.0, mins = 0.0;
+        for (int q = b; q < e; ++q) {
+            const double *o = rs + (size_t)rmem[q] * 9;
+            for (int t = 0; t < 7; ++t) sum[t] += o[t];
+            lg += o[7];
+            lg2 += o[7] * o[7];
+            mxs = fmax(mxs, o[2]);
+            mins = (q == b) ? o[0] : fmin(mins, o[0]);
+        }
+        float *f = feat + (size_t)c * HZ_MLP_INPUTS;
+        const double inv = m > 0 ? 1.0 / m : 0.0;
+        f[0] = (float)(m > 0 ? log2((double)m) : 0.0);
+        for (int t = 0; t < 7; ++t) f[1 + t] = (float)(sum[t] * inv);
+        f[8] = (float)(level / 10.0);
+        const double mu = lg * inv;
+        f[9] = (float)(m > 0 ? sqrt(fmax(lg2 * inv - mu * mu, 0.0)) : 0.0);
+        f[10] = (float)mxs;
+        f[11] = (float)mins;
+    }
+}
+
+}  // namespace
+
+ChebLevel cheby_prepare(const DevCSR &A, double scale) {
+    ChebLevel c;
+    c.diag = arena().alloc_n<double>((size_t)A.n + 1);
+    const int g = grid_for(A.n);
+    double *part = arena().alloc_n<double>((size_t)g + 1);
+    double *rho = arena().alloc_n<double>(1);
+    k_cheb_diag<<<g, kBlock, 0, stream()>>>(A.n, A.ia, A.ja, A.val, c.diag, part);
+    k_max_final<<<1, 64, 0, stream()>>>(g, part, rho);
+    HZ_LAUNCH_CHECK();
+    double hr = 0.0;
+    HZ_CHECK(memcpy_sync(&hr, rho, sizeof(double), hipMemcpyDeviceToHost));
+    c.lmax = scale * hr;
+    c.lmin = c.lmax / 30.0;
+    return c;
+}
+
+void cheby_apply(const DevSELL &S, const ChebLevel &c, int steps, const double *b, double *&x, double *&xt,
+                 double *dwork) {
+    if (S.n == 0 || steps <= 0) return;
+    const double theta = 0.5 * (c.lmax + c.lmin), delta = 0.5 * (c.lmax - c.lmin);
+    const double sigma = theta / delta;
+    double rho = 1.0 / sigma;
+    const int grid = div_up(S.nslices, kSlicesPerBlock);
+    for (int k = 0; k < steps; ++k) {
+        double c1 = 0.0, c2 = 1.0 / theta;
+        if (k > 0) {
+            const double rn = 1.0 / (2.0 * sigma - rho);
+            c1 = rn * rho;
+            c2 = 2.0 * rn / delta;
+            rho = rn;
+        }
+        k_cheb_step<<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col, S.val, c.diag, c1, c2, b, x, xt,
+                                                   dwork);
+        HZ_LAUNCH_CHECK();
+        std::swap(x, xt);
+    }
+}
+
+void aggregate_features(const DevCSR &A, const int *agg, int nagg, const int *rptr, const int *rmem, int level,
+                        float *feat) {
+    double *rs = arena().alloc_n<double>((size_t)A.n * 9 + 1);
+    k_row_stats<<<grid_for(A.n), kBlock, 0, stream()>>>(A.n, A.ia, A.ja, A.val, agg, rs);
+    k_agg_features<<<grid_for(nagg), kBlock, 0, stream()>>>(nagg, rptr, rmem, rs, level, feat);
+    HZ_LAUNCH_CHECK();
+}
+
+}  // namespace hz
+
+namespace hz {
+
+namespace {
+
+__global__ __launch_bounds__(kBlock) void k_learned_params(int nagg, const float *__restrict__ out,
+                                                           float *__restrict__ omega, double *__restrict__ part) {
+    __shared__ double sh[kBlock / kWave];
+    double s = 0.0;
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nagg; c += gridDim.x * blockDim.x) {
+        omega[c] = fminf(fmaxf(out[2 * c], 0.0f), 1.6f);
+        s += (double)out[2 * c + 1];
+    }
+    const double bs = block_sum<kBlock>(s, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
+}  // namespace
+
+double learned_parameters(int nagg, const float *out, float *omega) {
+    if (nagg <= 0) return 1.0;
+    const int g = grid_for(nagg);
+    double *part = arena().alloc_n<double>((size_t)g + 1);
+    k_learned_params<<<g, kBlock, 0, stream()>>>(nagg, out, omega, part);
+    HZ_LAUNCH_CHECK();
+    std::vector<double> h(g);
+    HZ_CHECK(memcpy_sync(h.data(), part, sizeof(double) * g, hipMemcpyDeviceToHost));
+    double s = 0.0;
+    for (double v : h) s += v;
+    return std::min(1.5, std::max(0.2, s / nagg));
+}
+
+}  // namespace hz

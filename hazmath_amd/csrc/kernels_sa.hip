@@ -83,11 +83,13 @@ __global__ void k_sa_count(int n, const int *__restrict__ ia, const int *__restr
 // S_ik = 1 - omega M_ik / d_i (k == i) or -omega M_ik / d_i.
 __global__ void k_sa_fill(int n, const int *__restrict__ ia, const int *__restrict__ ja,
                           const double *__restrict__ mval, const int *__restrict__ agg,
-                          const double *__restrict__ d, double omega, const int *__restrict__ pia,
-                          int *__restrict__ pja, double *__restrict__ pval) {
+                          const double *__restrict__ d, double omega_all, const float *__restrict__ omega_agg,
+                          const int *__restrict__ pia, int *__restrict__ pja, double *__restrict__ pval) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int b = ia[i], e = ia[i + 1];
         const double di = d[i];
+        // per-aggregate damping: row i uses its own aggregate's value
+        const double omega = (omega_agg && agg[i] > HZ_UNPT) ? (double)omega_agg[agg[i]] : omega_all;
         int pos = pia[i];
         for (int q = b; q < e; ++q) {
             const int c = agg[ja[q]];
@@ -240,7 +242,7 @@ __global__ void k_csr_aAxpy(int n, const int *__restrict__ ia, const int *__rest
 }  // namespace
 
 void sa_prolongation(const DevCSR &A, DevCSR &N, const int *agg, int nagg, double omega, int filter,
-                     DevCSR &P) {
+                     DevCSR &P, const float *omega_agg) {
     hipStream_t st = stream();
     const int n = A.n;
     const DevCSR &M = (filter == 1) ? N : A;
@@ -259,7 +261,8 @@ void sa_prolongation(const DevCSR &A, DevCSR &N, const int *agg, int nagg, doubl
     P.nnz = (int)exclusive_scan_int(cnt, P.ia, n);
     P.ja = arena().alloc_n<int>((size_t)P.nnz + 1);
     P.val = arena().alloc_n<double>((size_t)P.nnz + 1);
-    k_sa_fill<<<grid_for(n), kBlock, 0, st>>>(n, M.ia, M.ja, M.val, agg, d, omega, P.ia, P.ja, P.val);
+    k_sa_fill<<<grid_for(n), kBlock, 0, st>>>(n, M.ia, M.ja, M.val, agg, d, omega, omega_agg, P.ia, P.ja,
+                                              P.val);
     HZ_LAUNCH_CHECK();
 }
 

@@ -16,6 +16,9 @@ ordered lexicographically (x fastest), Dirichlet nodes eliminated.
 * ``poisson3d_kuhn_coef(m, nblocks, seed)``: -div(kappa grad u) on the same
   mesh with kappa piecewise constant on nblocks^3 blocks, log-uniform in
   [1e-3, 1e3] (same pattern as poisson3d_kuhn).
+* ``aniso2d_p1(m, theta, eps)`` / ``aniso3d_kuhn(m, theta, eps)``: rotated
+  anisotropic diffusion (tensor diag(1, eps) rotated by theta in the x-y plane)
+  on the same meshes.
 * ``uniform_rhs(n, seed)``: f_i in [-1, 1) from splitmix64(seed XOR i), the
   same bits in every language.
 """
@@ -167,6 +170,80 @@ def poisson3d_kuhn_coef(m: int = 128, nblocks: int = 8, seed: int = 1, log10_lo:
         vals.append((h * acc.reshape(-1)) / 6.0)
     ia, ja, va = _stencil_csr((k, k, k), offs, vals)
     return ia, ja, va, kb
+
+
+def _p1_cell_matrix(dim: int, K: np.ndarray) -> np.ndarray:
+    """P1 stiffness (unit cell) with constant tensor K: the square split along
+    (1,1) into two triangles (2-D, vertex x + 2y) or the cube split into the 6
+    Kuhn tetrahedra (3-D, vertex x + 2y + 4z)."""
+    import itertools
+    nv = 2 ** dim
+    verts = np.array([[(v >> a) & 1 for a in range(dim)] for v in range(nv)], float)
+    M = np.zeros((nv, nv))
+    for perm in itertools.permutations(range(dim)):
+        p = np.zeros(dim)
+        simplex = [0]
+        for ax in perm:
+            p = p.copy()
+            p[ax] = 1
+            simplex.append(int(sum(int(p[a]) << a for a in range(dim))))
+        X = verts[simplex]
+        T = np.hstack([np.ones((dim + 1, 1)), X])
+        G = np.linalg.inv(T)[1:, :]
+        vol = abs(np.linalg.det(T)) / (2.0 if dim == 2 else 6.0)
+        M[np.ix_(simplex, simplex)] += vol * G.T @ K @ G
+    return M
+
+
+def anisotropic_tensor(dim: int, theta: float, eps: float) -> np.ndarray:
+    """R(theta) diag(1, eps[, eps]) R(theta)^T, the rotation in the x-y plane."""
+    c, s = np.cos(theta), np.sin(theta)
+    R = np.eye(dim)
+    R[:2, :2] = [[c, -s], [s, c]]
+    return R @ np.diag([1.0] + [eps] * (dim - 1)) @ R.T
+
+
+def _constant_cell_stencil(dim: int, m: int, Mc: np.ndarray, scale: float):
+    """CSR of the assembled operator on the (m-1)^dim interior nodes: entry
+    (p, p+d) = scale * sum over cells holding both nodes of Mc[corner(p), corner(p+d)]."""
+    k = m - 1
+    if dim == 2:
+        offs3 = [(0, 0, 0), (0, 0, 1), (0, 0, -1), (0, 1, 0), (0, -1, 0), (0, 1, 1), (0, -1, -1)]
+    else:
+        offs3 = [(0, 0, 0)]
+        for ax in [(0, 0, 1), (0, 1, 0), (1, 0, 0)]:
+            for sg in (1, -1):
+                offs3.append(tuple(sg * a for a in ax))
+        for d in KUHN_DIAGONALS:
+            for sg in (1, -1):
+                offs3.append(tuple(sg * a for a in d))
+    vals = []
+    for (dz, dy, dx) in offs3:
+        v = 0.0
+        for c in range(2 ** dim):
+            cx, cy, cz = c & 1, (c >> 1) & 1, (c >> 2) & 1
+            ex, ey, ez = cx + dx, cy + dy, cz + dz
+            if not (0 <= ex <= 1 and 0 <= ey <= 1 and 0 <= ez <= 1) or (dim == 2 and ez != 0):
+                continue
+            v += Mc[c, ex + 2 * ey + 4 * ez]
+        vals.append(scale * v)
+    shape = (k, k) if dim == 2 else (k, k, k)
+    return _stencil_csr(shape, offs3, vals)
+
+
+def aniso2d_p1(m: int, theta: float, eps: float):
+    """-div(K grad u), K = R(theta) diag(1, eps) R(theta)^T, on the unit square
+    with m x m cells each split along (1,1) into two P1 triangles; 7-point rows
+    on the (m-1)^2 interior nodes (scale-free in 2-D)."""
+    Mc = _p1_cell_matrix(2, anisotropic_tensor(2, theta, eps))
+    return _constant_cell_stencil(2, m, Mc, 1.0)
+
+
+def aniso3d_kuhn(m: int, theta: float, eps: float):
+    """-div(K grad u), K = R_z(theta) diag(1, eps, eps) R_z(theta)^T, on the unit
+    cube with m^3 Kuhn-split cubes; 15-point rows (h-scaled)."""
+    Mc = _p1_cell_matrix(3, anisotropic_tensor(3, theta, eps))
+    return _constant_cell_stencil(3, m, Mc, 1.0 / m)
 
 
 def kuhn_element_stiffness_check():

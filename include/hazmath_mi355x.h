@@ -70,6 +70,7 @@ extern "C" {
 #define HZ_SMOOTHER_JACOBI  1
 #define HZ_SMOOTHER_GS      2
 #define HZ_SMOOTHER_L1DIAG 10
+#define HZ_SMOOTHER_CHEBY  91   /* no counterpart: Chebyshev on D^-1 A (build-owned) */
 #define HZ_UNPT            -1
 #define HZ_MAX_AMG_LVL     20
 
@@ -328,6 +329,7 @@ typedef struct {
     REAL   condest;
     INT    relres_count;      /* entries valid in relres_hist (<= 4096) */
     REAL   relres_hist[4096]; /* relres after each PCG iteration */
+    INT    fallback_used;     /* hz_krylov_amg_learned: 1 when the plain method produced x */
 } hz_solve_info;
 
 void hz_get_last_info(hz_solve_info *info);
@@ -421,6 +423,34 @@ INT hz_mlp_loss_grad(const float *theta, const float *x, const float *t, INT B, 
  * after the last one. */
 INT hz_mlp_train(float *theta, const float *x, const float *t, INT B, INT epochs, INT batch, float lr,
                  unsigned long long seed, float *loss_hist);
+
+/* ---------------------------------------------------------------------------
+ * Learned-parameter AMG (no HAZmath counterpart).
+ * ------------------------------------------------------------------------- */
+/* PCG with smoothed aggregation whose per-aggregate prolongator damping and
+ * per-level Chebyshev bound scale come from the parameter network (weights
+ * theta, HZ_MLP_PARAMS floats), with the HZ_SMOOTHER_CHEBY smoother
+ * (amgparam's aggregation, coarse_dof, cycle and sweep counts are used).  If it
+ * needs more than `budget` iterations, fails, or its FP64 true relative
+ * residual exceeds itparam->linear_tol, x is reset to the caller's guess and
+ * the plain method of itparam/amgparam (linear_solver_dcsr_krylov_amg) solves
+ * instead; hz_solve_info.fallback_used records which.  Returns the iteration
+ * count of the method that produced x (or a negative error code). */
+INT hz_krylov_amg_learned(dCSRmat *A, dvector *b, dvector *x, linear_itsolver_param *itparam,
+                          AMG_param *amgparam, const float *theta, INT budget);
+/* Two-grid error contraction for labelling: smoothed aggregation (uniform
+ * damping omega, filter from amgparam) on two levels with an exact coarse
+ * solve and the Chebyshev smoother (bound scale lscale); the cycle iterated
+ * `iters` times on A x = 0 from a seeded random x_0; *rate =
+ * (||x_K||_A/||x_{K/2}||_A)^(1/(K-K/2)) estimates the asymptotic factor. */
+INT hz_two_grid_rate(dCSRmat *A, AMG_param *amgparam, REAL omega, REAL lscale, INT iters, REAL *rate);
+/* Parity hooks: feat[nagg][12] of the aggregates of level `level`; `steps`
+ * Chebyshev steps on x in place; the smoothed prolongation with strength
+ * threshold theta, damping omega or per-aggregate omega_agg (may be NULL). */
+INT hz_aggregate_features(dCSRmat *A, const INT *aggregates, INT num_aggregates, INT level, float *feat);
+INT hz_cheby_smooth(dCSRmat *A, const REAL *b, REAL *x, INT steps, REAL scale);
+INT hz_sa_prolongation(dCSRmat *A, const INT *aggregates, INT num_aggregates, REAL theta, REAL omega,
+                       const float *omega_agg, INT filter, dCSRmat *P);
 
 /* Runtime information. */
 const char *hz_version(void);

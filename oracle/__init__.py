@@ -44,6 +44,9 @@ def load():
     lib.orc_tridiag_condest.argtypes = [INT, P(REAL), P(REAL)]
     lib.orc_tridiag_condest.restype = REAL
     lib.orc_set_jacobi_weight.argtypes = [REAL]
+    lib.orc_sa_prolongation.argtypes = [P(dCSRmat), P(INT), INT, REAL, REAL, C.POINTER(C.c_float), INT,
+                                        P(dCSRmat)]
+    lib.orc_cheby_smooth.argtypes = [P(dCSRmat), P(REAL), P(REAL), INT, REAL]
     lib.orc_linear_solver_amg.argtypes = [P(dCSRmat), P(dvector), P(dvector), P(AMG_param)]
     lib.orc_linear_solver_amg.restype = INT
     lib.orc_precond_create.argtypes = [P(dCSRmat), P(AMG_param)]

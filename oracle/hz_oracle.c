@@ -678,7 +678,7 @@ static void orc_rap(const dCSRmat *R, const dCSRmat *A, const dCSRmat *P, dCSRma
  * absorbs the dropped off-diagonal row sum of A; N is modified in place), D
  * the first diagonal entry of M (1.0 where |d| < 1e-6); then P = S P_tent. */
 static void orc_smooth_p(const dCSRmat *A, dCSRmat *N, const INT *agg, INT nagg, REAL omega, SHORT filter,
-                         dCSRmat *P) {
+                         dCSRmat *P, const float *omega_agg) {
     const INT n = A->row;
     const dCSRmat *M = A;
     if (filter == 1) {
@@ -697,9 +697,12 @@ static void orc_smooth_p(const dCSRmat *A, dCSRmat *N, const INT *agg, INT nagg,
     for (INT i = 0; i < n; ++i) if (fabs(d[i]) < 1e-6) d[i] = 1.0;
     dCSRmat S;
     csr_copy(M, &S);
-    for (INT i = 0; i < n; ++i)
+    for (INT i = 0; i < n; ++i) {
+        /* per-aggregate damping (learned parameters): row i uses its aggregate's value */
+        const REAL w = (omega_agg && agg[i] > HZ_UNPT) ? (REAL)omega_agg[agg[i]] : omega;
         for (INT k = S.IA[i]; k < S.IA[i + 1]; ++k)
-            S.val[k] = (S.JA[k] == i) ? 1 - omega * M->val[k] / d[i] : -omega * M->val[k] / d[i];
+            S.val[k] = (S.JA[k] == i) ? 1 - w * M->val[k] / d[i] : -w * M->val[k] / d[i];
+    }
     free(d);
     /* tentative P (form_tentative_p, amg_setup_ua.c:251-295): (agg[i], 1.0) */
     dCSRmat T;
@@ -719,6 +722,59 @@ static void orc_smooth_p(const dCSRmat *A, dCSRmat *N, const INT *agg, INT nagg,
     orc_mxm(&S, &T, P);
     csr_free(&S);
     csr_free(&T);
+}
+
+/* Smoothed prolongation of one level for a given aggregation (strength
+ * threshold theta), with a uniform or per-aggregate damping. */
+void orc_sa_prolongation(const dCSRmat *A, const INT *agg, INT nagg, REAL theta, REAL omega,
+                         const float *omega_agg, INT filter, dCSRmat *P) {
+    dCSRmat N;
+    orc_strength(A, theta, &N);
+    orc_smooth_p(A, &N, agg, nagg, omega, (SHORT)filter, P, omega_agg);
+    csr_free(&N);
+}
+
+/* Chebyshev smoother (no reference counterpart; Saad, Iterative Methods for
+ * Sparse Linear Systems, 2nd ed., Alg. 12.1) on D^-1 A, D the first diagonal
+ * entries, interval [lmax/30, lmax], lmax = scale * max_i sum_j |a_ij|/|a_ii|;
+ * `steps` steps on x in place. */
+void orc_cheby_smooth(const dCSRmat *A, const REAL *b, REAL *x, INT steps, REAL scale) {
+    const INT n = A->row;
+    REAL *dg = (REAL *)xcalloc((size_t)n + 1, sizeof(REAL));
+    REAL rho_g = 0.0;
+    for (INT i = 0; i < n; ++i) {
+        REAL di = 0.0, s = 0.0;
+        INT found = 0;
+        for (INT k = A->IA[i]; k < A->IA[i + 1]; ++k) {
+            s += fabs(A->val[k]);
+            if (!found && A->JA[k] == i) { di = A->val[k]; found = 1; }
+        }
+        dg[i] = di;
+        if (di != 0.0 && s / fabs(di) > rho_g) rho_g = s / fabs(di);
+    }
+    const REAL lmax = scale * rho_g, lmin = lmax / 30.0;
+    const REAL theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
+    REAL rho = 1.0 / sigma;
+    REAL *d = (REAL *)xcalloc((size_t)n + 1, sizeof(REAL));
+    REAL *xn = (REAL *)xcalloc((size_t)n + 1, sizeof(REAL));
+    for (INT k = 0; k < steps; ++k) {
+        REAL c1 = 0.0, c2 = 1.0 / theta;
+        if (k > 0) {
+            const REAL rn = 1.0 / (2.0 * sigma - rho);
+            c1 = rn * rho;
+            c2 = 2.0 * rn / delta;
+            rho = rn;
+        }
+        for (INT i = 0; i < n; ++i) {
+            REAL t = b[i];
+            for (INT q = A->IA[i]; q < A->IA[i + 1]; ++q) t = t - A->val[q] * x[A->JA[q]];
+            const REAL dn = (c1 == 0.0 ? 0.0 : c1 * d[i]) + (dg[i] != 0.0 ? c2 * t / dg[i] : 0.0);
+            d[i] = dn;
+            xn[i] = x[i] + dn;
+        }
+        memcpy(x, xn, sizeof(REAL) * (size_t)n);
+    }
+    free(dg); free(d); free(xn);
 }
 
 /* amg_setup_unsmoothP_unsmoothR, amg_setup_ua.c:1228-1443, and for
@@ -755,7 +811,7 @@ static INT orc_setup(orc_amg *m, const dCSRmat *A, AMG_param *param) {
         if (param->AMG_type == HZ_SA_AMG) {
             dCSRmat N;
             orc_strength(&L->A, L->theta, &N);
-            orc_smooth_p(&L->A, &N, L->agg, L->nagg, param->tentative_smooth, param->smooth_filter, &L->P);
+            orc_smooth_p(&L->A, &N, L->agg, L->nagg, param->tentative_smooth, param->smooth_filter, &L->P, NULL);
             csr_free(&N);
             csr_transpose(&L->P, &L->R);
             orc_rap(&L->R, &L->A, &L->P, &m->lv[lvl + 1].A);
@@ -1037,6 +1093,10 @@ typedef struct {
 } orc_prec_data;
 
 static void orc_smooth(const orc_prec_data *pd, orc_level *L, INT sweeps) {
+    if (pd->cyc.smoother == 91) {   /* Chebyshev (build-owned), bound scale 1 */
+        orc_cheby_smooth(&L->A, L->b, L->x, sweeps, 1.0);
+        return;
+    }
     for (INT s = 0; s < sweeps; ++s)
         orc_smoother_sweep(&L->A, L->b, L->x, pd->cyc.smoother, pd->jacobi_w);
 }
@@ -1145,7 +1205,8 @@ INT orc_krylov_amg(dCSRmat *A, dvector *b, dvector *x, linear_itsolver_param *it
     orc_amg *m = (orc_amg *)xcalloc(1, sizeof(orc_amg));
     double t0 = orc_now();
     if (prt > 0) printf("\n Calling UA AMG ...\n");
-    if (amgparam->smoother != HZ_SMOOTHER_JACOBI && amgparam->smoother != HZ_SMOOTHER_L1DIAG) {
+    if (amgparam->smoother != HZ_SMOOTHER_JACOBI && amgparam->smoother != HZ_SMOOTHER_L1DIAG &&
+        amgparam->smoother != 91) {
         printf("### ERROR: Wrong smoother type %lld!\n", (long long)amgparam->smoother);
         free(m);
         return HZ_ERROR_AMG_SMOOTH_TYPE;   /* smoothers outside the restated set */
@@ -1246,7 +1307,9 @@ INT orc_linear_solver_amg(dCSRmat *A, dvector *b, dvector *x, AMG_param *param) 
 
 /* create_precond_amg (helper.c:206-283) / precond_amg (precond.c:164-183). */
 void *orc_precond_create(dCSRmat *A, AMG_param *amgparam) {
-    if (amgparam->smoother != HZ_SMOOTHER_JACOBI && amgparam->smoother != HZ_SMOOTHER_L1DIAG) return NULL;
+    if (amgparam->smoother != HZ_SMOOTHER_JACOBI && amgparam->smoother != HZ_SMOOTHER_L1DIAG &&
+        amgparam->smoother != 91)
+        return NULL;
     orc_amg *m = (orc_amg *)xcalloc(1, sizeof(orc_amg));
     if (orc_setup(m, A, amgparam) < 0) { orc_amg_free(m); free(m); return NULL; }
     orc_prec_data *pd = (orc_prec_data *)xcalloc(1, sizeof(orc_prec_data));

@@ -58,6 +58,11 @@ INT  orc_aggregate(const dCSRmat *A, INT type, REAL theta, INT max_aggregation,
 REAL orc_tridiag_condest(INT n, const REAL *alpha, const REAL *beta);
 /* Jacobi damping used by the V-cycle (reference: fixed 1.0, smoother.c:47). */
 void orc_set_jacobi_weight(REAL w);
+/* smoothed prolongation of one level (uniform or per-aggregate damping) */
+void orc_sa_prolongation(const dCSRmat *A, const INT *agg, INT nagg, REAL theta, REAL omega,
+                         const float *omega_agg, INT filter, dCSRmat *P);
+/* Chebyshev smoother, `steps` steps in place (build-owned method) */
+void orc_cheby_smooth(const dCSRmat *A, const REAL *b, REAL *x, INT steps, REAL scale);
 /* linear_solver_amg + amg_solve (itsolver.c:524-640, amg_solve.c:109-161) */
 INT  orc_linear_solver_amg(dCSRmat *A, dvector *b, dvector *x, AMG_param *param);
 /* create_precond_amg / precond_amg (helper.c:206-283, precond.c:164-183) */

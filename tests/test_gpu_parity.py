@@ -727,3 +727,88 @@ def test_mlp_training_fits_smooth_target(lib):
     assert hist[-1] < hist[0] / 10, hist
     xv, tv = _mlp_data(1024, 2)
     assert mlp.loss(theta.astype(np.float64), xv.astype(np.float64), tv.astype(np.float64)) < 3 * hist[-1] + 1e-3
+
+
+def test_chebyshev_smoother_bitexact(lib, orc):
+    ia, ja, va, _ = problems.poisson3d_kuhn_coef(12)
+    n = ia.size - 1
+    A = capi.HostCSR(ia, ja, va)
+    rng = np.random.default_rng(3)
+    b, x0 = rng.standard_normal(n), rng.standard_normal(n)
+    for steps, scale in ((1, 1.0), (3, 0.8)):
+        xg, xo = x0.copy(), x0.copy()
+        assert lib.hz_cheby_smooth(A.ptr(), capi.dptr(b), capi.dptr(xg), steps, scale) == 0
+        orc.orc_cheby_smooth(A.ptr(), capi.dptr(b), capi.dptr(xo), steps, scale)
+        assert np.array_equal(xg, xo)
+
+
+def test_sa_prolongation_per_aggregate_damping_bitexact(lib, orc, libc):
+    ia, ja, va = problems.aniso2d_p1(40, 0.7, 0.01)
+    n = ia.size - 1
+    A = capi.HostCSR(ia, ja, va)
+    agg = (np.arange(n) // 4).astype(np.int32)
+    agg[::17] = -1
+    nagg = int(agg.max()) + 1
+    om = (0.2 + np.random.default_rng(2).random(nagg)).astype(np.float32)
+    for filt, omp in ((1, om), (0, om), (1, None)):
+        Pg, Po = capi.dCSRmat(), capi.dCSRmat()
+        ptr = capi.fptr(omp) if omp is not None else None
+        assert lib.hz_sa_prolongation(A.ptr(), capi.dptr(agg, capi.INT), nagg, 0.1, 0.67, ptr, filt,
+                                      C.byref(Pg)) == 0
+        orc.orc_sa_prolongation(A.ptr(), capi.dptr(agg, capi.INT), nagg, 0.1, 0.67, ptr, filt, C.byref(Po))
+        for a, b_ in zip(capi.csr_to_numpy(Pg), capi.csr_to_numpy(Po)):
+            assert np.array_equal(a, b_)
+
+
+def test_aggregate_features_vs_oracle(lib):
+    from oracle import learned
+    ia, ja, va, _ = problems.poisson3d_kuhn_coef(10)
+    n = ia.size - 1
+    agg = (np.arange(n) // 5).astype(np.int32)
+    agg[3::11] = -1
+    nagg = int(agg.max()) + 1
+    F = np.zeros((nagg, 12), np.float32)
+    assert lib.hz_aggregate_features(capi.HostCSR(ia, ja, va).ptr(), capi.dptr(agg, capi.INT), nagg, 2,
+                                     capi.fptr(F)) == 0
+    Fo = learned.aggregate_features(ia, ja, va, agg, nagg, 2)
+    assert np.allclose(F, Fo, rtol=1e-5, atol=1e-6)
+
+
+def test_sa_chebyshev_pcg_parity(lib, orc, libc, solvers_fixture):
+    import oracle
+    ia, ja, va, b = solvers_fixture
+    A = capi.HostCSR(ia, ja, va)
+    res = {}
+    for which in ("gpu", "oracle"):
+        fn = (lambda **k: capi.solver_params(lib, **k)) if which == "gpu" else oracle.params
+        it, amg = _sa_param(fn, capi.HEC, 1, tol=1e-10, maxit=300, smoother=capi.SMOOTHER_CHEBY, coarse_dof=100)
+        amg.presmooth_iter = amg.postsmooth_iter = 2
+        x = capi.HostVec(np.zeros(b.size))
+        libc.srand(1)
+        if which == "gpu":
+            r = lib.linear_solver_dcsr_krylov_amg(A.ptr(), capi.HostVec(b).ptr(), x.ptr(), C.byref(it),
+                                                  C.byref(amg))
+        else:
+            r = orc.orc_krylov_amg(A.ptr(), capi.HostVec(b).ptr(), x.ptr(), C.byref(it), C.byref(amg),
+                                   C.byref(capi.orc_trace()))
+        res[which] = (r, x.arr.copy())
+    (ig, xg), (io, xo) = res["gpu"], res["oracle"]
+    assert io > 0 and abs(ig - io) <= 1
+    assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) < 1e-10
+
+
+def test_two_grid_rate_behaves(lib, libc):
+    """Two-grid contraction of SA + Chebyshev(2,2) on a 2-D Poisson problem
+    against the spectral radius of the dense error propagator I - B A built
+    from the oracle's cycle: 0.700 (damping 0) and 0.596 (damping 0.67)."""
+    ia, ja, va = problems.poisson2d_p1(32)
+    A = capi.HostCSR(ia, ja, va)
+    _, amg = capi.solver_params(lib, aggregation=capi.HEC, coarse_dof=10)
+    amg.presmooth_iter = amg.postsmooth_iter = 2
+    rates = {}
+    for om in (0.0, 0.67):
+        r = capi.REAL()
+        libc.srand(1)
+        assert lib.hz_two_grid_rate(A.ptr(), C.byref(amg), om, 1.0, 40, C.byref(r)) == 0
+        rates[om] = r.value
+    assert abs(rates[0.0] - 0.700) < 0.03 and abs(rates[0.67] - 0.596) < 0.03, rates

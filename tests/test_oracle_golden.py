@@ -501,3 +501,61 @@ def test_mlp_oracle_matches_torch_autograd():
     L.backward()
     assert abs(L.item() - loss) < 1e-12
     assert np.allclose(th.grad.numpy(), g, atol=1e-12, rtol=1e-10)
+
+
+def test_oracle_chebyshev_is_the_chebyshev_polynomial(orc):
+    """k Chebyshev steps on A x = 0 map e_0 to p_k(D^-1 A) e_0 with
+    p_k(l) = T_k((theta - l)/delta) / T_k(theta/delta) on [lmax/30, lmax],
+    lmax = scale * Gershgorin bound (Saad, 2nd ed., Alg. 12.1)."""
+    ia, ja, va = problems.aniso2d_p1(10, 0.5, 0.1)
+    n = ia.size - 1
+    A = sp.csr_matrix((va, ja, ia), shape=(n, n)).toarray()
+    d = np.diag(A).copy()
+    gersh = (np.abs(A).sum(1) / np.abs(d)).max()
+    e0 = np.random.default_rng(0).standard_normal(n)
+    for steps, scale in ((1, 1.0), (3, 1.0), (4, 0.7)):
+        x = e0.copy()
+        orc.orc_cheby_smooth(capi.HostCSR(ia, ja, va).ptr(), capi.dptr(np.zeros(n)), capi.dptr(x), steps,
+                             scale)
+        lmax = scale * gersh
+        lmin = lmax / 30
+        th, de = (lmax + lmin) / 2, (lmax - lmin) / 2
+        Ds = np.diag(1 / np.sqrt(d))
+        lam, V = np.linalg.eigh(Ds @ A @ Ds)          # spectrum of D^-1 A
+        T = np.polynomial.chebyshev.Chebyshev.basis(steps)
+        p = T((th - lam) / de) / T(th / de)
+        expected = np.linalg.inv(Ds) @ (V @ (p * (V.T @ (Ds @ e0))))
+        assert np.allclose(x, expected, atol=1e-10 * np.abs(e0).max()), steps
+
+
+def test_oracle_sa_per_aggregate_damping(orc, libc):
+    """P = (I - diag(omega_agg(i)) D^-1 A_F) P_tent with a damping per aggregate."""
+    ia, ja, va = problems.aniso2d_p1(16, 0.3, 0.05)
+    n = ia.size - 1
+    A = capi.HostCSR(ia, ja, va)
+    agg = np.zeros(n, np.int32)
+    nagg = 0
+    agg = (np.arange(n) // 3).astype(np.int32)
+    nagg = int(agg.max()) + 1
+    om = (0.3 + 0.6 * np.random.default_rng(1).random(nagg)).astype(np.float32)
+    P = capi.dCSRmat()
+    orc.orc_sa_prolongation(A.ptr(), capi.dptr(agg, capi.INT), nagg, 0.0, 0.67, capi.fptr(om), 0, C.byref(P))
+    pia, pja, pva = capi.csr_to_numpy(P)
+    Ad = sp.csr_matrix((va, ja, ia), shape=(n, n)).toarray()
+    T = np.zeros((n, nagg))
+    T[np.arange(n), agg] = 1.0
+    W = om[agg].astype(np.float64)
+    Pd = (np.eye(n) - (W / np.diag(Ad))[:, None] * Ad) @ T
+    assert np.allclose(sp.csr_matrix((pva, pja, pia), shape=(n, nagg)).toarray(), Pd, atol=1e-14)
+
+
+def test_feature_oracle_on_laplacian():
+    """Closed forms on the 5-point Laplacian with pairs along x: interior rows
+    have s/a = 1, q/a = -1, mx/s = mn/mx... = 1/4 and 1, in/s = 1/4."""
+    from oracle import learned
+    ia, ja, va = problems.poisson2d_p1(9)   # 8x8 interior
+    n = ia.size - 1
+    agg = (np.arange(n) // 2).astype(np.int32)
+    F = learned.aggregate_features(ia, ja, va, agg, n // 2, 3)
+    c = (3 * 8 + 2) // 2          # pair (26, 27): interior rows
+    assert np.allclose(F[c], [1.0, 1.0, -1.0, 0.25, 1.0, 0.25, 0.0, 5 / 16, 0.3, 0.0, 0.25, 1.0])
