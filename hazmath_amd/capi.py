@@ -294,7 +294,7 @@ def _declare(lib):
     lib.hz_mlp_train.argtypes = [P(F), P(F), P(F), INT, INT, INT, F, C.c_ulonglong, P(F)]
     lib.hz_mlp_train.restype = INT
     lib.hz_krylov_amg_learned.argtypes = [P(dCSRmat), P(dvector), P(dvector), P(linear_itsolver_param),
-                                          P(AMG_param), P(F), INT]
+                                          P(AMG_param), P(F), INT, INT]
     lib.hz_krylov_amg_learned.restype = INT
     lib.hz_two_grid_rate.argtypes = [P(dCSRmat), P(AMG_param), REAL, REAL, INT, P(REAL)]
     lib.hz_two_grid_rate.restype = INT

@@ -917,7 +917,7 @@ INT hz_mlp_train(float *theta, const float *x, const float *t, INT B, INT epochs
 }
 
 INT hz_krylov_amg_learned(dCSRmat *A, dvector *b, dvector *x, linear_itsolver_param *itparam,
-                          AMG_param *amgparam, const float *theta, INT budget) {
+                          AMG_param *amgparam, const float *theta, INT budget, INT cheb_steps) {
     if (!device_ready()) return HZ_ERROR_DEVICE;
     try {
         arena().reset();
@@ -930,7 +930,9 @@ INT hz_krylov_amg_learned(dCSRmat *A, dvector *b, dvector *x, linear_itsolver_pa
         // learned method: SA with network damping + Chebyshev smoother
         AMG_param lp = *amgparam;
         lp.AMG_type = HZ_SA_AMG;
+        lp.aggregation_type = HZ_VMB;   // aggregates of ~3^d keep the smoothed operators sparse
         lp.smoother = HZ_SMOOTHER_CHEBY;
+        lp.presmooth_iter = lp.postsmooth_iter = (SHORT)std::max(1, cheb_steps);
         linear_itsolver_param li = *itparam;
         li.linear_maxit = std::min<int>(itparam->linear_maxit, budget);
         INT st = core_solve(dA, db, dx, &li, &lp, dth);

@@ -429,15 +429,16 @@ INT hz_mlp_train(float *theta, const float *x, const float *t, INT B, INT epochs
  * ------------------------------------------------------------------------- */
 /* PCG with smoothed aggregation whose per-aggregate prolongator damping and
  * per-level Chebyshev bound scale come from the parameter network (weights
- * theta, HZ_MLP_PARAMS floats), with the HZ_SMOOTHER_CHEBY smoother
- * (amgparam's aggregation, coarse_dof, cycle and sweep counts are used).  If it
+ * theta, HZ_MLP_PARAMS floats), VMB aggregation and `cheb_steps` pre- and
+ * post-smoothing steps of the HZ_SMOOTHER_CHEBY smoother (amgparam's strength
+ * threshold, max_aggregation, coarse_dof and cycle are used).  If it
  * needs more than `budget` iterations, fails, or its FP64 true relative
  * residual exceeds itparam->linear_tol, x is reset to the caller's guess and
  * the plain method of itparam/amgparam (linear_solver_dcsr_krylov_amg) solves
  * instead; hz_solve_info.fallback_used records which.  Returns the iteration
  * count of the method that produced x (or a negative error code). */
 INT hz_krylov_amg_learned(dCSRmat *A, dvector *b, dvector *x, linear_itsolver_param *itparam,
-                          AMG_param *amgparam, const float *theta, INT budget);
+                          AMG_param *amgparam, const float *theta, INT budget, INT cheb_steps);
 /* Two-grid error contraction for labelling: smoothed aggregation (uniform
  * damping omega, filter from amgparam) on two levels with an exact coarse
  * solve and the Chebyshev smoother (bound scale lscale); the cycle iterated

@@ -294,3 +294,10 @@ def test_mlp_init_layout():
     assert np.all(a[mlp.OFF_B1:mlp.OFF_W2] == 0) and np.all(a[mlp.OFF_B3:] == 0)
     w1 = a[mlp.OFF_W1:mlp.OFF_B1].reshape(16, 64)[:12]
     assert np.abs(w1).max() <= np.sqrt(6.0 / 76) and np.abs(w1).std() > 0.05
+
+
+def test_committed_model_loads():
+    from hazmath_amd import learned
+    theta = learned.load_model()
+    assert theta.dtype == np.float32 and theta.shape == (capi.MLP_PARAMS,)
+    assert np.isfinite(theta).all()

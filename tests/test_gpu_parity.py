@@ -812,3 +812,52 @@ def test_two_grid_rate_behaves(lib, libc):
         assert lib.hz_two_grid_rate(A.ptr(), C.byref(amg), om, 1.0, 40, C.byref(r)) == 0
         rates[om] = r.value
     assert abs(rates[0.0] - 0.700) < 0.03 and abs(rates[0.67] - 0.596) < 0.03, rates
+
+
+def _aniso_plain(lib, libc, ia, ja, va, b, agg, sm, typ=capi.UA_AMG, steps=1):
+    A = capi.HostCSR(ia, ja, va)
+    x = capi.HostVec(np.zeros(b.size))
+    it, amg = capi.solver_params(lib, tol=1e-10, maxit=1000, aggregation=agg, smoother=sm, coarse_dof=200,
+                                 theta=0.04)
+    amg.AMG_type = typ
+    amg.presmooth_iter = amg.postsmooth_iter = steps
+    libc.srand(1)
+    r = lib.linear_solver_dcsr_krylov_amg(A.ptr(), capi.HostVec(b).ptr(), x.ptr(), C.byref(it), C.byref(amg))
+    return r, x.arr
+
+
+@pytest.mark.parametrize("m,th", [(32, 1.1), (48, 0.4)])
+def test_learned_parameters_gain(lib, libc, m, th):
+    """The trained network (hazmath_amd/models) on held-out 3-D anisotropic
+    problems (eps = 1e-3): the learned SA + Chebyshev(2) method reaches the
+    FP64 tolerance without fallback, in fewer PCG iterations than the same
+    method with the default damping 0.67 / bound scale 1, and in fewer than
+    the plain drop-in (UA, HEM, L1)."""
+    import scipy.sparse as sp
+    from hazmath_amd import learned
+    theta = learned.load_model()
+    ia, ja, va = problems.aniso3d_kuhn(m, th, 1e-3)
+    b = problems.uniform_rhs(ia.size - 1, seed=1)
+    libc.srand(1)
+    il, xl, info = learned.solve_learned(lib, ia, ja, va, b, theta, budget=400)
+    assert info.fallback_used == 0 and il > 0
+    Asp = sp.csr_matrix((va, ja, ia))
+    assert np.linalg.norm(b - Asp @ xl) / np.linalg.norm(b) < 1e-10
+    idef, _ = _aniso_plain(lib, libc, ia, ja, va, b, capi.VMB, capi.SMOOTHER_CHEBY, capi.SA_AMG, 2)
+    iplain, _ = _aniso_plain(lib, libc, ia, ja, va, b, capi.HEM, capi.SMOOTHER_L1DIAG)
+    assert il < idef and il < iplain, (il, idef, iplain)
+
+
+def test_learned_fallback_on_budget(lib, libc):
+    """Over budget, the plain method produces x (fallback flag set) and the
+    result meets the tolerance."""
+    import scipy.sparse as sp
+    from hazmath_amd import learned
+    theta = learned.load_model()
+    ia, ja, va = problems.aniso2d_p1(64, 0.7, 1e-2)
+    b = problems.uniform_rhs(ia.size - 1, seed=2)
+    libc.srand(1)
+    r, x, info = learned.solve_learned(lib, ia, ja, va, b, theta, budget=3)
+    assert info.fallback_used == 1 and r > 3
+    Asp = sp.csr_matrix((va, ja, ia))
+    assert np.linalg.norm(b - Asp @ x) / np.linalg.norm(b) < 1e-10
