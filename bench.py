@@ -43,7 +43,9 @@ def measured_traffic(n, nnz):
     return d.get("traffic_bytes")
 
 AGG = {"hem": capi.HEM, "hec": capi.HEC, "vmb": capi.VMB}
-SMOOTHER = {"jacobi": capi.SMOOTHER_JACOBI, "l1": capi.SMOOTHER_L1DIAG}
+SMOOTHER = {"jacobi": capi.SMOOTHER_JACOBI, "l1": capi.SMOOTHER_L1DIAG, "gs": capi.SMOOTHER_GS,
+            "sgs": capi.SMOOTHER_SGS, "sor": capi.SMOOTHER_SOR, "ssor": capi.SMOOTHER_SSOR,
+            "gsor": capi.SMOOTHER_GSOR, "sgsor": capi.SMOOTHER_SGSOR}
 PRECISION = {"fp64": 0, "fp32": 1, "bf16": 2}
 DTYPE = {"fp64": "f64", "fp32": "f32 cycle, f64 PCG", "bf16": "bf16 values/f32 cycle, f64 PCG"}
 
@@ -65,6 +67,7 @@ def parse():
     p.add_argument("--agg", choices=AGG, default="hem")
     p.add_argument("--smoother", choices=SMOOTHER, default="l1")
     p.add_argument("--jacobi-weight", type=float, default=1.0)
+    p.add_argument("--relaxation", type=float, default=1.0, help="SOR-type relaxation weight")
     p.add_argument("--coarse-dof", type=int, default=200)
     p.add_argument("--theta", type=float, default=0.04)
     p.add_argument("--tol", type=float, default=1e-10)
@@ -84,6 +87,7 @@ def set_params(lib, a):
     it, amg = capi.solver_params(lib, tol=a.tol, maxit=1000, aggregation=AGG[a.agg],
                                  smoother=SMOOTHER[a.smoother], coarse_dof=a.coarse_dof,
                                  coarse_solver=capi.SOLVER_UMFPACK, max_levels=20, theta=a.theta)
+    amg.relaxation = a.relaxation
     return it, amg
 
 
@@ -96,6 +100,7 @@ def cpu_baseline(a, ia, ja, va, b):
     it, amg = oracle.params(tol=a.tol, maxit=1000, aggregation=AGG[a.agg],
                             smoother=SMOOTHER[a.smoother], coarse_dof=a.coarse_dof,
                             coarse_solver=32, max_levels=20, theta=a.theta)
+    amg.relaxation = a.relaxation
     A = capi.HostCSR(ia, ja, va)
     x = capi.HostVec(np.zeros(b.size))
     bb = capi.HostVec(b)

@@ -45,6 +45,7 @@ VMB, HEC, HEM = 1, 4, 5
 V_CYCLE, W_CYCLE = 1, 2
 SMOOTHER_JACOBI, SMOOTHER_GS, SMOOTHER_L1DIAG = 1, 2, 10
 SMOOTHER_CHEBY = 91
+SMOOTHER_GS, SMOOTHER_SGS, SMOOTHER_SOR, SMOOTHER_SSOR, SMOOTHER_GSOR, SMOOTHER_SGSOR = 2, 3, 5, 6, 7, 8
 
 
 class dCSRmat(C.Structure):
@@ -302,6 +303,8 @@ def _declare(lib):
     lib.hz_aggregate_features.restype = INT
     lib.hz_cheby_smooth.argtypes = [P(dCSRmat), P(REAL), P(REAL), INT, REAL]
     lib.hz_cheby_smooth.restype = INT
+    lib.hz_gs_smooth.argtypes = [P(dCSRmat), P(REAL), P(REAL), INT, INT, INT, REAL, P(INT), P(INT), P(INT)]
+    lib.hz_gs_smooth.restype = INT
     lib.hz_sa_prolongation.argtypes = [P(dCSRmat), P(INT), INT, REAL, REAL, P(F), INT, P(dCSRmat)]
     lib.hz_sa_prolongation.restype = INT
     lib.linear_solver_amg.argtypes = [P(dCSRmat), P(dvector), P(dvector), P(AMG_param)]

@@ -242,6 +242,7 @@ void amg_prepare_cycle(Hierarchy &H, const AMG_param *param) {
     cp.postsmooth = param->postsmooth_iter;
     cp.coarse_solver = param->coarse_solver;
     cp.coarse_scaling = param->coarse_scaling;
+    cp.relax = param->relaxation;
     cp.coarse_tol = 1e-6 * 1e-2;
     cp.maxit = param->maxit;
     const int n0 = H.lv[0].A.n;
@@ -270,6 +271,7 @@ void amg_prepare_cycle(Hierarchy &H, const AMG_param *param) {
                 L.cheb = cheby_prepare(L.A, sc);
                 L.cwork = arena().alloc_n<double>(n);
             }
+            if (is_gs_family(cp.smoother)) gs_prepare(L.A, L.gs);
         }
         if (cp.precision > 0) {
             L.bf = arena().alloc_n<float>(n);
@@ -298,11 +300,12 @@ void presmooth(Hierarchy &H, int l, CycleState &cs) {
     Level &L = H.lv[l];
     const CycleParams &cp = H.cp;
     int sweeps = cp.presmooth;
-    if (cp.smoother == HZ_SMOOTHER_CHEBY) {
+    if (cp.smoother == HZ_SMOOTHER_CHEBY || is_gs_family(cp.smoother)) {
         if (cs.zero[l]) fill(L.x, 0.0, L.A.n);
         cs.zero[l] = false;
         cs.first_done[l] = false;
-        cheby_apply(L.S, L.cheb, sweeps, L.b, L.x, L.xt, L.cwork);
+        if (cp.smoother == HZ_SMOOTHER_CHEBY) cheby_apply(L.S, L.cheb, sweeps, L.b, L.x, L.xt, L.cwork);
+        else gs_smooth(L.gs, cp.smoother, false, sweeps, cp.relax, L.b, L.x);
         return;
     }
     if (cs.zero[l]) {
@@ -343,10 +346,11 @@ void prolong_postsmooth(Hierarchy &H, int l, double alpha, const double *alpha_d
     Level &L = H.lv[l];
     Level &C = H.lv[l + 1];
     const CycleParams &cp = H.cp;
-    if (cp.postsmooth <= 0 || cp.smoother == HZ_SMOOTHER_CHEBY) {
+    if (cp.postsmooth <= 0 || cp.smoother == HZ_SMOOTHER_CHEBY || is_gs_family(cp.smoother)) {
         if (L.sa) csr_prolong(L.P, C.x, alpha, alpha_dev, L.x);
         else prolong_agg(L.A.n, L.agg, C.x, alpha, alpha_dev, L.x);
         if (cp.smoother == HZ_SMOOTHER_CHEBY) cheby_apply(L.S, L.cheb, cp.postsmooth, L.b, L.x, L.xt, L.cwork);
+        else if (is_gs_family(cp.smoother)) gs_smooth(L.gs, cp.smoother, true, cp.postsmooth, cp.relax, L.b, L.x);
         return;
     }
     const bool want_dot = dot_out && cp.postsmooth == 1;
@@ -392,7 +396,8 @@ forward:
         cs.num_lvl[l]++;
         presmooth(H, l, cs);
         // the next level restarts from zero; fuse its first sweep into the restriction
-        const bool fuse = (l + 1 < nl - 1) && cp.presmooth > 0 && !L.sa && cp.smoother != HZ_SMOOTHER_CHEBY;
+        const bool fuse = (l + 1 < nl - 1) && cp.presmooth > 0 && !L.sa && cp.smoother != HZ_SMOOTHER_CHEBY &&
+                          !is_gs_family(cp.smoother);
         if (L.sa) {
             residual(L.S, L.b, L.x, L.w);
             csr_mxv(L.R, L.w, C.b);    // dcsr_mxv(R), mgcycle.c:302
@@ -591,8 +596,11 @@ bool amg_precond(Hierarchy &H, const double *r, double *z, double *zr_out) {
         return false;
     }
     // Arrange the ping-pong so that the last sweep of level 0 writes into z.
-    const int swaps = (cp.presmooth > 0 ? cp.presmooth - 1 : 0) + cp.postsmooth +
-                      (cp.maxit > 1 ? (cp.maxit - 1) * (cp.presmooth + cp.postsmooth) : 0);
+    // (Gauss-Seidel-type sweeps work in place: no swaps)
+    const int swaps = is_gs_family(cp.smoother)
+                          ? 0
+                          : (cp.presmooth > 0 ? cp.presmooth - 1 : 0) + cp.postsmooth +
+                                (cp.maxit > 1 ? (cp.maxit - 1) * (cp.presmooth + cp.postsmooth) : 0);
     double *scratch = H.x0buf;
     if (swaps % 2 == 1) {
         L0.x = scratch;

@@ -21,6 +21,7 @@ struct Level {
     DevCSR P, R;
     ChebLevel cheb;           // Chebyshev smoother data
     double *cwork = nullptr;  // Chebyshev direction vector
+    GsLevel gs;               // colouring and colour-sliced matrix (GS family)
     double *Ainv = nullptr;   // coarsest, coarse_solver 32
     double *cgwork = nullptr; // coarsest, coarse_solver 0
     // FP32 cycle (CycleParams::precision > 0)
@@ -37,6 +38,7 @@ struct CycleParams {
     int coarse_scaling = 0;
     double coarse_tol = 1e-8;   // param_amg_init tol 1e-6 times 1e-2 (mgcycle.c:255)
     double jacobi_w = 1.0;
+    double relax = 1.0;         // SOR-type relaxation (AMG_param relaxation)
     int maxit = 1;              // cycles per preconditioner application
     int precision = 0;          // 0 FP64 cycle; 1 FP32 cycle; 2 FP32 cycle, BF16 values below level 0
 };

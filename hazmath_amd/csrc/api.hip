@@ -139,9 +139,13 @@ int fatal(int code) {
 int check_supported(const linear_itsolver_param *it, const AMG_param *amg) {
     (void)it;
     if (amg->smoother != HZ_SMOOTHER_JACOBI && amg->smoother != HZ_SMOOTHER_L1DIAG &&
-        amg->smoother != HZ_SMOOTHER_CHEBY) {
+        amg->smoother != HZ_SMOOTHER_CHEBY && !is_gs_family(amg->smoother)) {
         std::printf("### ERROR: Wrong smoother type %lld!\n", (long long)amg->smoother);
         return HZ_ERROR_AMG_SMOOTH_TYPE;
+    }
+    if (is_gs_family(amg->smoother) && g_opt.vcycle_precision > 0) {
+        std::printf("### ERROR: Gauss-Seidel-type smoothers need the FP64 cycle (vcycle_precision = 0)\n");
+        return HZ_ERROR_INPUT_PAR;
     }
     if (amg->AMG_type != HZ_UA_AMG && amg->AMG_type != HZ_SA_AMG) {
         std::printf("### ERROR: AMG type %lld is not available in this build\n", (long long)amg->AMG_type);
@@ -1029,6 +1033,30 @@ INT hz_cheby_smooth(dCSRmat *A, const REAL *b, REAL *x, INT steps, REAL scale) {
         const ChebLevel c = cheby_prepare(dA, scale);
         cheby_apply(S, c, steps, db, dx, dt, dd);
         HZ_CHECK(memcpy_sync(x, dx, sizeof(double) * A->row, hipMemcpyDeviceToHost));
+        return HZ_SUCCESS;
+    } catch (const DeviceError &) {
+        return HZ_ERROR_DEVICE;
+    }
+}
+
+INT hz_gs_smooth(dCSRmat *A, const REAL *b, REAL *x, INT smoother, INT post, INT nsweeps, REAL relax, INT *order,
+                 INT *cptr, INT *ncolors) {
+    if (!is_gs_family((int)smoother)) return HZ_ERROR_AMG_SMOOTH_TYPE;
+    if (A->row != A->col) return HZ_ERROR_MAT_SIZE;
+    if (!device_ready()) return HZ_ERROR_DEVICE;
+    try {
+        arena().reset();
+        const DevCSR dA = upload_csr(A);
+        const double *db = upload(b, (size_t)A->row);
+        double *dx = upload(x, (size_t)A->row);
+        GsLevel G;
+        gs_prepare(dA, G);
+        gs_smooth(G, (int)smoother, post != 0, (int)nsweeps, relax, db, dx);
+        HZ_CHECK(memcpy_sync(x, dx, sizeof(double) * A->row, hipMemcpyDeviceToHost));
+        if (order && A->row > 0) HZ_CHECK(memcpy_sync(order, G.rows, sizeof(int) * A->row, hipMemcpyDeviceToHost));
+        if (cptr)
+            for (int c = 0; c <= G.ncolors; ++c) cptr[c] = G.cptr.empty() ? 0 : G.cptr[c];
+        if (ncolors) *ncolors = G.ncolors;
         return HZ_SUCCESS;
     } catch (const DeviceError &) {
         return HZ_ERROR_DEVICE;

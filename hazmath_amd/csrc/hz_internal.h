@@ -204,4 +204,25 @@ void coarse_dense_solve_m(int n, const double *Ainv, const float *b, float *x);
 // Turn order of HEC from libc rand() (iarray_shuffle, array.c:302-318).
 void hec_turn_order(int n, int *perm_dev, int rng_source);
 
+// ---- kernels_gs.hip: Gauss-Seidel family as multicolour sweeps -------------
+struct GsLevel {
+    int ncolors = 0, nslices = 0;
+    std::vector<int> cptr;   // host: colour c holds row-list positions [cptr[c], cptr[c+1])
+    std::vector<int> sl0;    // host: colour c holds slices [sl0[c], sl0[c+1])
+    int *rows = nullptr;     // colour-major row list (rows ascending inside a colour)
+    int *spos = nullptr, *scnt = nullptr;   // per slice: first list position, rows
+    long long *sptr = nullptr;               // per slice: entry offset (nslices + 1)
+    int *col = nullptr;
+    double *val = nullptr;   // off-diagonal entries, slice column-major (col: list positions)
+    double *diag = nullptr;  // a_ii per list position
+    int *dsl0 = nullptr;     // sl0 on the device
+    int n = 0;
+    bool in_lds = false;     // small level: a whole smoothing call is one workgroup, x in LDS
+    double *xp = nullptr, *bp = nullptr;   // x and b permuted to the list order (other levels)
+};
+bool is_gs_family(int smoother);
+void gs_prepare(const DevCSR &A, GsLevel &G);
+// pre (post = false) or post smoothing with smoother 2, 3, 5, 6, 7 or 8, in place
+void gs_smooth(const GsLevel &G, int smoother, bool post, int nsweeps, double relax, const double *b, double *x);
+
 }  // namespace hz

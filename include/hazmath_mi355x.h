@@ -68,7 +68,12 @@ extern "C" {
 #define HZ_V_CYCLE          1
 #define HZ_W_CYCLE          2
 #define HZ_SMOOTHER_JACOBI  1
-#define HZ_SMOOTHER_GS      2
+#define HZ_SMOOTHER_GS      2   /* GS family: multicolour sweeps on the GPU */
+#define HZ_SMOOTHER_SGS     3
+#define HZ_SMOOTHER_SOR     5
+#define HZ_SMOOTHER_SSOR    6
+#define HZ_SMOOTHER_GSOR    7
+#define HZ_SMOOTHER_SGSOR   8
 #define HZ_SMOOTHER_L1DIAG 10
 #define HZ_SMOOTHER_CHEBY  91   /* no counterpart: Chebyshev on D^-1 A (build-owned) */
 #define HZ_UNPT            -1
@@ -452,6 +457,13 @@ INT hz_aggregate_features(dCSRmat *A, const INT *aggregates, INT num_aggregates,
 INT hz_cheby_smooth(dCSRmat *A, const REAL *b, REAL *x, INT steps, REAL scale);
 INT hz_sa_prolongation(dCSRmat *A, const INT *aggregates, INT num_aggregates, REAL theta, REAL omega,
                        const float *omega_agg, INT filter, dCSRmat *P);
+/* Parity hook: dcsr_presmoothing (post = 0) or dcsr_postsmoothing (post = 1)
+ * of mgcycle.c:76-223 with a Gauss-Seidel-type smoother (2, 3, 5, 6, 7, 8) as
+ * the library's multicolour sweeps, x updated in place; order[n] and cptr[n+1]
+ * (may be NULL) receive the colour-major row order, colour c at positions
+ * cptr[c] .. cptr[c+1]-1, and *ncolors the colour count. */
+INT hz_gs_smooth(dCSRmat *A, const REAL *b, REAL *x, INT smoother, INT post, INT nsweeps, REAL relax, INT *order,
+                 INT *cptr, INT *ncolors);
 
 /* Runtime information. */
 const char *hz_version(void);

@@ -53,6 +53,9 @@ def load():
     lib.orc_precond_create.restype = C.c_void_p
     lib.orc_precond_apply.argtypes = [C.c_void_p, P(REAL), P(REAL)]
     lib.orc_precond_free.argtypes = [C.c_void_p]
+    lib.orc_gs_smooth_ordered.argtypes = [P(dCSRmat), P(REAL), P(REAL), INT, INT, INT, REAL, P(INT), P(INT),
+                                          INT]
+    lib.orc_gs_smooth_ordered.restype = C.c_int
     _LIB = lib
     return lib
 

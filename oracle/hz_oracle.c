@@ -234,6 +234,119 @@ void orc_smoother_sweep(const dCSRmat *A, const REAL *b, REAL *x, INT smoother, 
     free(t); free(d);
 }
 
+/* Gauss-Seidel-type sweeps, smoother_dcsr_gs (smoother.c:109-190),
+ * smoother_dcsr_sgs (smoother.c:205-250) and smoother_dcsr_sor
+ * (smoother.c:266-307), generalised to a row ordering: `ord` lists the rows
+ * in blocks ord[cptr[c] .. cptr[c+1]), c < nblk.  A forward sweep visits the
+ * blocks in increasing order, a backward sweep in decreasing order, rows
+ * inside a block in list order.  ord == NULL is the natural order with one
+ * row per block, which is exactly the reference's loop i = 0..n-1 (forward)
+ * or n-1..0 (backward).  When the rows of every block are pairwise uncoupled
+ * (a colouring), the order inside a block does not change the result.
+ *   mode 0  (GS)  d = 1/a_ii when |a_ii| > SMALLREAL, x_i = t * d; d keeps its
+ *                 last value across rows and sweeps of one call, as in
+ *                 smoother.c:140-146;
+ *   mode 1  (SGS) x_i = t / a_ii when |a_ii| > SMALLREAL;
+ *   mode 2  (SOR) x_i = w (t / a_ii) + (1 - w) x_i when |a_ii| > SMALLREAL.
+ * skip_first: the backward half of SGS starts one block early (n-2 in the
+ * natural order, smoother.c:236). */
+typedef struct {
+    const INT *ord, *cptr;
+    INT nblk;
+} orc_order;
+
+static void orc_gs_row(const dCSRmat *A, const REAL *b, REAL *x, INT i, int mode, REAL w, REAL *dstate) {
+    REAL t = b[i], d = (mode == 0) ? *dstate : 0.0;
+    for (INT k = A->IA[i]; k < A->IA[i + 1]; ++k) {
+        const INT j = A->JA[k];
+        if (i != j) t -= A->val[k] * x[j];
+        else if (mode == 0) { if (fabs(A->val[k]) > ORC_SMALLREAL) d = 1.e+0 / A->val[k]; }
+        else d = A->val[k];
+    }
+    if (mode == 0) { x[i] = t * d; *dstate = d; }
+    else if (fabs(d) > ORC_SMALLREAL) x[i] = (mode == 1) ? t / d : w * (t / d) + (1 - w) * x[i];
+}
+
+static void orc_gs_pass(const dCSRmat *A, const REAL *b, REAL *x, const orc_order *o, int forward,
+                        int skip_first, int mode, REAL w, REAL *dstate) {
+    const INT nb = o->ord ? o->nblk : A->row;
+    for (INT q = skip_first; q < nb; ++q) {
+        const INT c = forward ? q : nb - 1 - q;
+        if (!o->ord) { orc_gs_row(A, b, x, c, mode, w, dstate); continue; }
+        for (INT p = o->cptr[c]; p < o->cptr[c + 1]; ++p) orc_gs_row(A, b, x, o->ord[p], mode, w, dstate);
+    }
+}
+
+/* smoother_dcsr_gs / smoother_dcsr_sor with L sweeps in one direction */
+static void orc_gs_call(const dCSRmat *A, const REAL *b, REAL *x, const orc_order *o, int forward, int mode,
+                        REAL w, INT L) {
+    REAL dstate = 0.0;   /* smoother.c:124: d = 0.0 at entry */
+    while (L-- > 0) orc_gs_pass(A, b, x, o, forward, 0, mode, w, &dstate);
+}
+
+/* smoother_dcsr_sgs: L times (forward sweep, backward sweep from n-2) */
+static void orc_sgs_call(const dCSRmat *A, const REAL *b, REAL *x, const orc_order *o, INT L) {
+    REAL dstate = 0.0;
+    while (L-- > 0) {
+        orc_gs_pass(A, b, x, o, 1, 0, 1, 1.0, &dstate);
+        orc_gs_pass(A, b, x, o, 0, 1, 1, 1.0, &dstate);
+    }
+}
+
+/* dcsr_presmoothing (mgcycle.c:76-137, called with istart 0, iend n-1,
+ * istep 1) and dcsr_postsmoothing (mgcycle.c:162-223, called with istart 0,
+ * iend n-1, istep -1) for smoothers GS 2, SGS 3, SOR 5, SSOR 6, GSOR 7 and
+ * SGSOR 8.  Returns 0, or -1 for any other smoother. */
+int orc_gs_smooth_ordered(const dCSRmat *A, const REAL *b, REAL *x, INT smoother, INT post, INT nsweeps,
+                          REAL relax, const INT *ord, const INT *cptr, INT nblk) {
+    orc_order o = {ord, cptr, nblk};
+    const int F = 1, B = 0;
+    switch (smoother) {
+        case 2:   /* GS: forward before, backward after */
+            orc_gs_call(A, b, x, &o, post ? B : F, 0, 1.0, nsweeps);
+            break;
+        case 3:
+            orc_sgs_call(A, b, x, &o, nsweeps);
+            break;
+        case 5:
+            orc_gs_call(A, b, x, &o, post ? B : F, 2, relax, nsweeps);
+            break;
+        case 6:   /* SSOR: forward then backward, before and after */
+            orc_gs_call(A, b, x, &o, F, 2, relax, nsweeps);
+            orc_gs_call(A, b, x, &o, B, 2, relax, nsweeps);
+            break;
+        case 7:   /* GSOR */
+            if (!post) {
+                orc_gs_call(A, b, x, &o, F, 0, 1.0, nsweeps);
+                orc_gs_call(A, b, x, &o, B, 2, relax, nsweeps);
+            } else {
+                orc_gs_call(A, b, x, &o, F, 2, relax, nsweeps);
+                orc_gs_call(A, b, x, &o, B, 0, 1.0, nsweeps);
+            }
+            break;
+        case 8:   /* SGSOR */
+            if (!post) {
+                orc_gs_call(A, b, x, &o, F, 0, 1.0, nsweeps);
+                orc_gs_call(A, b, x, &o, B, 0, 1.0, nsweeps);
+                orc_gs_call(A, b, x, &o, F, 2, relax, nsweeps);
+                orc_gs_call(A, b, x, &o, B, 2, relax, nsweeps);
+            } else {
+                orc_gs_call(A, b, x, &o, F, 2, relax, nsweeps);
+                orc_gs_call(A, b, x, &o, B, 2, relax, nsweeps);
+                orc_gs_call(A, b, x, &o, F, 0, 1.0, nsweeps);
+                orc_gs_call(A, b, x, &o, B, 0, 1.0, nsweeps);
+            }
+            break;
+        default:
+            return -1;
+    }
+    return 0;
+}
+
+int orc_is_gs_family(INT smoother) {
+    return smoother == 2 || smoother == 3 || smoother == 5 || smoother == 6 || smoother == 7 || smoother == 8;
+}
+
 /* ------------------------------------------------------------------ */
 /* Strength graph: construct_strongly_coupled, amg_setup_ua.c:375-452  */
 /* ------------------------------------------------------------------ */
@@ -1092,9 +1205,13 @@ typedef struct {
     INT maxit;
 } orc_prec_data;
 
-static void orc_smooth(const orc_prec_data *pd, orc_level *L, INT sweeps) {
+static void orc_smooth(const orc_prec_data *pd, orc_level *L, INT sweeps, int post) {
     if (pd->cyc.smoother == 91) {   /* Chebyshev (build-owned), bound scale 1 */
         orc_cheby_smooth(&L->A, L->b, L->x, sweeps, 1.0);
+        return;
+    }
+    if (orc_is_gs_family(pd->cyc.smoother)) {
+        orc_gs_smooth_ordered(&L->A, L->b, L->x, pd->cyc.smoother, post, sweeps, pd->cyc.relaxation, NULL, NULL, 0);
         return;
     }
     for (INT s = 0; s < sweeps; ++s)
@@ -1113,7 +1230,7 @@ forward:
     while (l < nl - 1) {
         orc_level *L = &m->lv[l];
         num_lvl[l]++;
-        orc_smooth(pd, L, p->presmooth_iter);
+        orc_smooth(pd, L, p->presmooth_iter, 0);
         orc_dcsr_residual(&L->A, L->b, L->x, L->w);
         orc_level *C = &m->lv[l + 1];
         if (L->P.IA) {
@@ -1156,7 +1273,7 @@ forward:
             else if (alpha == -1.0) L->x[i] -= t;
             else L->x[i] += t * alpha;
         }
-        orc_smooth(pd, L, p->postsmooth_iter);
+        orc_smooth(pd, L, p->postsmooth_iter, 1);
         if (num_lvl[l] < p->cycle_type) break;
         num_lvl[l] = 0;
     }
@@ -1206,7 +1323,7 @@ INT orc_krylov_amg(dCSRmat *A, dvector *b, dvector *x, linear_itsolver_param *it
     double t0 = orc_now();
     if (prt > 0) printf("\n Calling UA AMG ...\n");
     if (amgparam->smoother != HZ_SMOOTHER_JACOBI && amgparam->smoother != HZ_SMOOTHER_L1DIAG &&
-        amgparam->smoother != 91) {
+        amgparam->smoother != 91 && !orc_is_gs_family(amgparam->smoother)) {
         printf("### ERROR: Wrong smoother type %lld!\n", (long long)amgparam->smoother);
         free(m);
         return HZ_ERROR_AMG_SMOOTH_TYPE;   /* smoothers outside the restated set */
@@ -1260,7 +1377,8 @@ INT orc_krylov_amg(dCSRmat *A, dvector *b, dvector *x, linear_itsolver_param *it
 INT orc_linear_solver_amg(dCSRmat *A, dvector *b, dvector *x, AMG_param *param) {
     const SHORT prt = param->print_level;
     if (A->row != A->col || A->nnz <= 0) return -15;   /* ERROR_MAT_SIZE (check_error exits) */
-    if (param->smoother != HZ_SMOOTHER_JACOBI && param->smoother != HZ_SMOOTHER_L1DIAG)
+    if (param->smoother != HZ_SMOOTHER_JACOBI && param->smoother != HZ_SMOOTHER_L1DIAG &&
+        !orc_is_gs_family(param->smoother))
         return HZ_ERROR_AMG_SMOOTH_TYPE;
     orc_amg *m = (orc_amg *)xcalloc(1, sizeof(orc_amg));
     if (prt > 0) printf("\n Calling UA AMG ...\n");
@@ -1308,7 +1426,7 @@ INT orc_linear_solver_amg(dCSRmat *A, dvector *b, dvector *x, AMG_param *param) 
 /* create_precond_amg (helper.c:206-283) / precond_amg (precond.c:164-183). */
 void *orc_precond_create(dCSRmat *A, AMG_param *amgparam) {
     if (amgparam->smoother != HZ_SMOOTHER_JACOBI && amgparam->smoother != HZ_SMOOTHER_L1DIAG &&
-        amgparam->smoother != 91)
+        amgparam->smoother != 91 && !orc_is_gs_family(amgparam->smoother))
         return NULL;
     orc_amg *m = (orc_amg *)xcalloc(1, sizeof(orc_amg));
     if (orc_setup(m, A, amgparam) < 0) { orc_amg_free(m); free(m); return NULL; }

@@ -62,6 +62,12 @@ void orc_set_jacobi_weight(REAL w);
 void orc_sa_prolongation(const dCSRmat *A, const INT *agg, INT nagg, REAL theta, REAL omega,
                          const float *omega_agg, INT filter, dCSRmat *P);
 /* Chebyshev smoother, `steps` steps in place (build-owned method) */
+/* dcsr_presmoothing / dcsr_postsmoothing (mgcycle.c:76-223) for GS 2, SGS 3,
+ * SOR 5, SSOR 6, GSOR 7, SGSOR 8 over a row ordering in blocks (NULL: natural
+ * order, the reference's loops).  Returns -1 for other smoothers. */
+int orc_gs_smooth_ordered(const dCSRmat *A, const REAL *b, REAL *x, INT smoother, INT post, INT nsweeps,
+                          REAL relax, const INT *ord, const INT *cptr, INT nblk);
+int orc_is_gs_family(INT smoother);
 void orc_cheby_smooth(const dCSRmat *A, const REAL *b, REAL *x, INT steps, REAL scale);
 /* linear_solver_amg + amg_solve (itsolver.c:524-640, amg_solve.c:109-161) */
 INT  orc_linear_solver_amg(dCSRmat *A, dvector *b, dvector *x, AMG_param *param);

@@ -559,3 +559,135 @@ def test_feature_oracle_on_laplacian():
     F = learned.aggregate_features(ia, ja, va, agg, n // 2, 3)
     c = (3 * 8 + 2) // 2          # pair (26, 27): interior rows
     assert np.allclose(F[c], [1.0, 1.0, -1.0, 0.25, 1.0, 0.25, 0.0, 5 / 16, 0.3, 0.0, 0.25, 1.0])
+
+
+# ---- Gauss-Seidel family (smoother.c:109-307, mgcycle.c:76-223) ----------
+
+def _gs(orc, A, b, x0, smoother, post, sweeps=1, relax=1.0, order=None):
+    x = x0.copy()
+    if order is None:
+        r = orc.orc_gs_smooth_ordered(A.ptr(), capi.dptr(b), capi.dptr(x), smoother, post, sweeps, relax,
+                                      None, None, 0)
+    else:
+        ord_, cptr = order
+        r = orc.orc_gs_smooth_ordered(A.ptr(), capi.dptr(b), capi.dptr(x), smoother, post, sweeps, relax,
+                                      capi.dptr(ord_, capi.INT), capi.dptr(cptr, capi.INT), cptr.size - 1)
+    assert r == 0
+    return x
+
+
+def _tri_parts(ia, ja, va):
+    M = sp.csr_matrix((va, ja, ia)).toarray()
+    return np.tril(M, -1), np.diag(np.diag(M)), np.triu(M, 1), M
+
+
+def _fwd(L, D, U, b, x, w=1.0):   # (D + w L) x1 = w b - (w U + (w - 1) D) x
+    return scipy.linalg.solve_triangular(D + w * L, w * b - (w * U + (w - 1) * D) @ x, lower=True)
+
+
+def _bwd(L, D, U, b, x, w=1.0):
+    return scipy.linalg.solve_triangular(D + w * U, w * b - (w * L + (w - 1) * D) @ x, lower=False)
+
+
+def test_oracle_gauss_seidel_sweeps_are_triangular_solves(orc):
+    n = 150
+    ia, ja, va = _rand_csr(n, 11)
+    A = capi.HostCSR(ia, ja, va)
+    L, D, U, _ = _tri_parts(ia, ja, va)
+    rng = np.random.default_rng(3)
+    x0, b = rng.standard_normal(n), rng.standard_normal(n)
+    tol = dict(rtol=1e-11, atol=1e-11)
+    # GS: forward before the coarse correction, backward after
+    assert np.allclose(_gs(orc, A, b, x0, 2, 0), _fwd(L, D, U, b, x0), **tol)
+    assert np.allclose(_gs(orc, A, b, x0, 2, 1), _bwd(L, D, U, b, x0), **tol)
+    # two sweeps are two solves
+    assert np.allclose(_gs(orc, A, b, x0, 2, 0, sweeps=2), _fwd(L, D, U, b, _fwd(L, D, U, b, x0)), **tol)
+    # SGS: forward, then backward (its row n-1 would repeat the forward value)
+    ref = _bwd(L, D, U, b, _fwd(L, D, U, b, x0))
+    assert np.allclose(_gs(orc, A, b, x0, 3, 0), ref, **tol)
+    assert np.allclose(_gs(orc, A, b, x0, 3, 1), ref, **tol)
+    for w in (0.7, 1.3):
+        f, bk = _fwd(L, D, U, b, x0, w), _bwd(L, D, U, b, x0, w)
+        assert np.allclose(_gs(orc, A, b, x0, 5, 0, relax=w), f, **tol)
+        assert np.allclose(_gs(orc, A, b, x0, 5, 1, relax=w), bk, **tol)
+        # SSOR: forward then backward on both sides; sweeps=2 is 2 forward then 2 backward
+        assert np.allclose(_gs(orc, A, b, x0, 6, 0, relax=w), _bwd(L, D, U, b, f, w), **tol)
+        assert np.allclose(_gs(orc, A, b, x0, 6, 1, relax=w), _bwd(L, D, U, b, f, w), **tol)
+        ff = _fwd(L, D, U, b, f, w)
+        assert np.allclose(_gs(orc, A, b, x0, 6, 0, sweeps=2, relax=w),
+                           _bwd(L, D, U, b, _bwd(L, D, U, b, ff, w), w), **tol)
+        # GSOR: GS forward + SOR backward before; SOR forward + GS backward after
+        assert np.allclose(_gs(orc, A, b, x0, 7, 0, relax=w), _bwd(L, D, U, b, _fwd(L, D, U, b, x0), w), **tol)
+        assert np.allclose(_gs(orc, A, b, x0, 7, 1, relax=w), _bwd(L, D, U, b, f), **tol)
+        # SGSOR: GS fwd, GS bwd, SOR fwd, SOR bwd before; the SOR pair first after
+        g = _bwd(L, D, U, b, _fwd(L, D, U, b, x0))
+        assert np.allclose(_gs(orc, A, b, x0, 8, 0, relax=w),
+                           _bwd(L, D, U, b, _fwd(L, D, U, b, g, w), w), **tol)
+        s2 = _bwd(L, D, U, b, f, w)
+        assert np.allclose(_gs(orc, A, b, x0, 8, 1, relax=w), _bwd(L, D, U, b, _fwd(L, D, U, b, s2)), **tol)
+    # SOR with w = 1 is GS (t / d versus t * (1 / d): equal to rounding)
+    assert np.allclose(_gs(orc, A, b, x0, 5, 0, relax=1.0), _gs(orc, A, b, x0, 2, 0), rtol=1e-13, atol=1e-13)
+    # anything else is rejected
+    x = x0.copy()
+    assert orc.orc_gs_smooth_ordered(A.ptr(), capi.dptr(b), capi.dptr(x), 1, 0, 1, 1.0, None, None, 0) == -1
+
+
+def _greedy_colouring(ia, ja, n, seed):
+    rng = np.random.default_rng(seed)
+    col = -np.ones(n, np.int64)
+    for i in rng.permutation(n):
+        used = {col[j] for j in ja[ia[i]:ia[i + 1]] if j != i}
+        c = 0
+        while c in used:
+            c += 1
+        col[i] = c
+    return col
+
+
+def test_oracle_ordered_sweep_is_gs_of_the_permuted_matrix(orc):
+    """A sweep over colour blocks is natural-order GS of P A P^T, and inside a
+    block of uncoupled rows the visiting order changes no bit."""
+    n = 160
+    ia, ja, va = _rand_csr(n, 12, density=0.03)
+    M = sp.csr_matrix((va, ja, ia))
+    Msym = (abs(M) + abs(M.T)).tocsr()
+    col = _greedy_colouring(Msym.indptr, Msym.indices, n, 4)
+    nc = int(col.max()) + 1
+    ord_ = np.argsort(col, kind="stable").astype(np.int32)
+    cptr = np.zeros(nc + 1, np.int32)
+    cptr[1:] = np.cumsum(np.bincount(col, minlength=nc))
+    A = capi.HostCSR(ia, ja, va)
+    rng = np.random.default_rng(8)
+    x0, b = rng.standard_normal(n), rng.standard_normal(n)
+    Pm = M[ord_][:, ord_].tocsr()
+    Pm.sort_indices()
+    L, D, U, _ = _tri_parts(Pm.indptr, Pm.indices, Pm.data)
+    for sm, post, w in ((2, 0, 1.0), (2, 1, 1.0), (3, 0, 1.0), (6, 1, 1.2)):
+        got = _gs(orc, A, b, x0, sm, post, relax=w, order=(ord_, cptr))
+        if sm == 2:
+            ref = _fwd(L, D, U, b[ord_], x0[ord_]) if not post else _bwd(L, D, U, b[ord_], x0[ord_])
+        elif sm == 3:
+            ref = _bwd(L, D, U, b[ord_], _fwd(L, D, U, b[ord_], x0[ord_]))
+        else:
+            ref = _bwd(L, D, U, b[ord_], _fwd(L, D, U, b[ord_], x0[ord_], w), w)
+        assert np.allclose(got[ord_], ref, rtol=1e-11, atol=1e-11), (sm, post)
+        # reverse the rows inside every block: identical bits
+        ord2 = ord_.copy()
+        for c in range(nc):
+            ord2[cptr[c]:cptr[c + 1]] = ord_[cptr[c]:cptr[c + 1]][::-1]
+        assert np.array_equal(_gs(orc, A, b, x0, sm, post, relax=w, order=(ord2, cptr)), got)
+    # natural order with one row per block is the default loop
+    ident = np.arange(n, dtype=np.int32)
+    assert np.array_equal(_gs(orc, A, b, x0, 8, 0, relax=1.1, order=(ident, np.arange(n + 1, dtype=np.int32))),
+                          _gs(orc, A, b, x0, 8, 0, relax=1.1))
+
+
+@pytest.mark.parametrize("smoother", [2, 3, 6])
+def test_oracle_gs_pcg_beats_jacobi(orc, libc, solvers_fixture, smoother):
+    ia, ja, va, b = solvers_fixture
+    A = capi.HostCSR(ia, ja, va)
+    rj, _, _ = _solve(orc, libc, A, b, tol=1e-8, maxit=200, aggregation=5, smoother=1)
+    rg, _, x = _solve(orc, libc, A, b, tol=1e-8, maxit=200, aggregation=5, smoother=smoother)
+    assert 0 < rg < rj
+    Asp = sp.csr_matrix((va, ja, ia))
+    assert np.linalg.norm(b - Asp @ x) / np.linalg.norm(b) < 1e-7
