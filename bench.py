@@ -78,6 +78,12 @@ def parse():
     p.add_argument("--precision", choices=PRECISION, default="fp64",
                    help="arithmetic of the AMG cycle (PCG and coarsest solve stay FP64)")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--batched", action="store_true",
+                   help="the batched workload instead: 256 anisotropic 2-D P1 systems x 8 RHS")
+    p.add_argument("--batch-count", type=int, default=256)
+    p.add_argument("--batch-cpu-pairs", type=int, default=24,
+                   help="(system, RHS) pairs solved by the serial oracle for the CPU baseline")
+    p.add_argument("--batch-cpu-procs", type=int, default=8)
     p.add_argument("--cpu-sample-m", type=int, default=128,
                    help="cells per direction of the CPU-baseline sample")
     return p.parse_args()
@@ -120,12 +126,209 @@ def cpu_baseline(a, ia, ja, va, b):
     return t, r, tr, x.arr
 
 
+def _batch_pair_worker(job):
+    """Serial oracle solve of one (system, RHS) pair; returns (seconds, iterations)."""
+    side, theta, eps, index, q, tol = job
+    import oracle
+    orc = oracle.load()
+    libc = C.CDLL(None)
+    ia, ja, va = problems.aniso2d_p1(side - 1, theta, eps)
+    b = problems.batch_rhs(ia.size - 1, index)[q]
+    it, amg = oracle.params(tol=tol, maxit=1000, aggregation=capi.HEM, smoother=capi.SMOOTHER_L1DIAG,
+                            coarse_dof=200, coarse_solver=32, max_levels=20, theta=0.04)
+    A = capi.HostCSR(ia, ja, va)
+    x = capi.HostVec(np.zeros(b.size))
+    devnull = os.open(os.devnull, os.O_WRONLY)
+    saved = os.dup(1)
+    sys.stdout.flush()
+    os.dup2(devnull, 1)
+    try:
+        libc.srand(1)
+        t0 = time.perf_counter()
+        r = orc.orc_krylov_amg(A.ptr(), capi.HostVec(b).ptr(), x.ptr(), C.byref(it), C.byref(amg),
+                               C.byref(capi.orc_trace()))
+        t = time.perf_counter() - t0
+    finally:
+        os.dup2(saved, 1)
+        os.close(devnull)
+        os.close(saved)
+    return t, int(r), int(ia.size - 1)
+
+
+def main_batched(a, lib):
+    """Batched workload: 256 independent anisotropic 2-D P1 systems (sides
+    log-U[33,513], theta ~ U[0,pi), eps = 10^U[-4,0]), 8 seeded RHS each, one
+    call of hz_batch_krylov_amg_device (stacked CSR and RHS resident in HBM)."""
+    import torch
+    specs = problems.batch_anisotropic_2d(a.batch_count)
+    t_gen = time.perf_counter()
+    ias, jas, vas, off, bs = [], [], [], [0], []
+    nnz_off = 0
+    for d in specs:
+        ia, ja, va = problems.aniso2d_p1(d["side"] - 1, d["theta"], d["eps"])
+        n_s = ia.size - 1
+        ias.append(ia[:-1].astype(np.int64) + nnz_off)
+        jas.append(ja.astype(np.int64) + off[-1])
+        vas.append(va)
+        bs.append(problems.batch_rhs(n_s, d["index"]).T)
+        nnz_off += int(ia[-1])
+        off.append(off[-1] + n_s)
+    ia = np.concatenate(ias + [np.array([nnz_off])]).astype(np.int32)
+    ja = np.concatenate(jas).astype(np.int32)
+    va = np.concatenate(vas)
+    b = np.ascontiguousarray(np.concatenate(bs))          # [n][8]
+    n, nnz = int(off[-1]), int(nnz_off)
+    t_gen = time.perf_counter() - t_gen
+    print(f"[bench] batched: {len(specs)} systems, n={n}, nnz={nnz}, generated in {t_gen:.1f}s",
+          file=sys.stderr, flush=True)
+    dev = torch.device("cuda:0")
+    dia, dja, dva = (torch.from_numpy(v).to(dev) for v in (ia, ja, va))
+    db = torch.from_numpy(b).to(dev)
+    dx = torch.zeros_like(db)
+    offs = (capi.INT * len(off))(*off)
+    nsys = len(specs)
+    iters = np.zeros(nsys * capi.BATCH_RHS, np.int32)
+    libc = C.CDLL(None)
+
+    def step():
+        dx.zero_()
+        torch.cuda.synchronize()
+        it, amg = capi.solver_params(lib, tol=a.tol, maxit=1000, aggregation=capi.HEM,
+                                     smoother=capi.SMOOTHER_L1DIAG, coarse_dof=200,
+                                     coarse_solver=capi.SOLVER_UMFPACK, max_levels=20, theta=0.04)
+        libc.srand(1)
+        r = lib.hz_batch_krylov_amg_device(nsys, offs, nnz, dia.data_ptr(), dja.data_ptr(), dva.data_ptr(),
+                                           db.data_ptr(), dx.data_ptr(), C.byref(it), C.byref(amg),
+                                           capi.dptr(iters, capi.INT))
+        if r < 0:
+            raise RuntimeError(f"batched solver returned {r}")
+
+    for w in range(a.warmup):
+        step()
+        print(f"[bench] warmup {w}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    stats = []
+    t0 = time.perf_counter()
+    for k in range(a.steps):
+        step()
+        st = capi.hz_batch_stats()
+        lib.hz_batch_get_stats(C.byref(st))
+        stats.append((st.setup_ms, st.solve_ms, st.graph_launches))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = 1e3 * wall / a.steps
+    pairs = nsys * capi.BATCH_RHS
+    print(f"[bench] {a.steps} steps, {ms_per_step:.1f} ms/step", file=sys.stderr, flush=True)
+    # true FP64 residuals of every pair (CPU)
+    import scipy.sparse as sp
+    x = dx.cpu().numpy()
+    A = sp.csr_matrix((va, ja, ia), shape=(n, n))
+    R = b - A @ x
+    worst = 0.0
+    for s_ in range(nsys):
+        rs = np.linalg.norm(R[off[s_]:off[s_ + 1]], axis=0) / np.linalg.norm(b[off[s_]:off[s_ + 1]], axis=0)
+        worst = max(worst, float(rs.max()))
+    # level-0 kernels timed live (HIP events on the library's stream)
+    opts = capi.hz_options()
+    lib.hz_get_options(C.byref(opts))
+    opts.batch_time_kernels = 1
+    lib.hz_set_options(C.byref(opts))
+    step()
+    opts.batch_time_kernels = 0
+    lib.hz_set_options(C.byref(opts))
+    ks = capi.hz_batch_stats()
+    lib.hz_batch_get_stats(C.byref(ks))
+    # algorithmic bytes of y = A x with 8 FP64 RHS: 12 nnz + 4 (n+1) + 64 n (x) + 64 n (y)
+    bytes_spmm = 12 * nnz + 4 * (n + 1) + 128 * n
+    spmm_gbs = bytes_spmm / (ks.spmm_ms * 1e-3) / 1e9
+    # FP32 sweep with 8 RHS: 8 nnz (FP32 value + col) + 32 n each for x, b, xout + n (live)
+    bytes_sweep = 8 * nnz + 96 * n + n
+    sweep_gbs = bytes_sweep / (ks.sweep_ms * 1e-3) / 1e9
+    value = float(np.mean([s_[0] + s_[1] for s_ in stats])) * 1e-3
+    cpu = None
+    if not a.no_cpu_baseline:
+        import multiprocessing as mp
+        rng = np.random.default_rng(11)
+        pick = rng.choice(pairs, size=min(a.batch_cpu_pairs, pairs), replace=False)
+        jobs = [(specs[p // 8]["side"], specs[p // 8]["theta"], specs[p // 8]["eps"], specs[p // 8]["index"],
+                 int(p % 8), a.tol) for p in pick]
+        procs = max(1, min(a.batch_cpu_procs, os.cpu_count() or 1))
+        print(f"[bench] cpu baseline: {len(jobs)} pairs on {procs} processes ...", file=sys.stderr, flush=True)
+        tc = time.perf_counter()
+        with mp.get_context("fork").Pool(procs) as pool:
+            res = pool.map(_batch_pair_worker, jobs, chunksize=1)
+        tc = time.perf_counter() - tc
+        rows_done = sum(r_[2] for r_ in res)
+        # throughput in pairs/s, scaled by the sample's share of the batch's rows
+        mean_rows = n / nsys
+        cpu_pairs_per_s = len(jobs) / tc * (rows_done / len(jobs)) / mean_rows
+        gi = {int(p): r_[1] for p, r_ in zip(pick, res)}
+        cpu = {"value": pairs / cpu_pairs_per_s, "unit": "s", "cores": procs, "kind": "port",
+               "sample": (f"{len(jobs)} seeded (system, RHS) pairs of the batch solved one by one by the serial "
+                          f"C restatement (FP64 cycle) on {procs} processes in {tc:.1f}s; value = time for all "
+                          f"{pairs} pairs at that rate, scaled by rows"),
+               "pairs_per_s": cpu_pairs_per_s,
+               "iterations_vs_gpu": [[int(p // 8), int(p % 8), gi[int(p)], int(iters[int(p)])] for p in pick[:8]],
+               "host_nproc": os.cpu_count()}
+    line = {
+        "metric": "batched AMG-PCG time-to-1e-10 for 256 systems x 8 RHS (s)",
+        "value": value,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 cycle, f64 PCG",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"{nsys} 2-D P1 anisotropic diffusion systems (sides log-U[33,513], theta~U[0,pi), "
+                         f"eps=10^U[-4,0], seed 4), 8 seeded RHS each, n={n}, nnz={nnz}; UA-AMG HEM theta=0.04 "
+                         f"V(1,1) L1, dense coarsest <=200, FP32 cycle / FP64 PCG to {a.tol:g}, one PCG "
+                         f"iteration per hipGraph replay"),
+        },
+        "pairs": pairs,
+        "pairs_per_s": pairs / value,
+        "setup_ms": float(np.mean([s_[0] for s_ in stats])),
+        "solve_ms": float(np.mean([s_[1] for s_ in stats])),
+        "graph_launches": stats[-1][2],
+        "iterations_max": int(iters.max()),
+        "iterations_mean": float(iters.mean()),
+        "worst_true_rel_residual": worst,
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "FP64 SpMM y=Ax, 8 interleaved RHS, level 0 (PCG)",
+            "achieved": spmm_gbs,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": spmm_gbs / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": bytes_spmm,
+            "avg_launch_ms": ks.spmm_ms,
+        },
+        "kernels": {"spmm_ms": ks.spmm_ms, "sweep_f32_ms": ks.sweep_ms, "sweep_f32_gbs": sweep_gbs,
+                    "lanes_per_row_level0": ks.shape_level0, "levels": ks.num_levels,
+                    "op_complexity": ks.op_complexity},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py: this build targets one GPU; --gpus {a.gpus} is not supported "
               "(the single-GPU throughput is the reported value)", file=sys.stderr)
         sys.exit(2)
+    if a.batched:
+        import torch
+        lib = capi.load_library()
+        if lib.hz_device_count() < 1 or not torch.cuda.is_available():
+            print("bench.py: no HIP device", file=sys.stderr)
+            sys.exit(1)
+        return main_batched(a, lib)
     import torch
     lib = capi.load_library()
     if lib.hz_device_count() < 1 or not torch.cuda.is_available():

@@ -82,7 +82,13 @@ class AMG_param(C.Structure):
 
 class hz_options(C.Structure):
     _fields_ = [("jacobi_weight", REAL), ("hec_rng_source", INT), ("verbose_timing", INT),
-                ("return_on_fatal", INT), ("vcycle_precision", INT)]
+                ("return_on_fatal", INT), ("vcycle_precision", INT), ("batch_time_kernels", INT)]
+
+
+class hz_batch_stats(C.Structure):
+    _fields_ = [("setup_ms", REAL), ("solve_ms", REAL), ("op_complexity", REAL), ("num_levels", INT),
+                ("graph_launches", INT), ("shape_level0", INT), ("spmm_ms", REAL), ("sweep_ms", REAL),
+                ("rows", C.c_longlong), ("nnz", C.c_longlong), ("sell_entries", C.c_longlong)]
 
 
 class hz_solve_info(C.Structure):
@@ -247,6 +253,27 @@ def solver_params(lib, *, tol=1e-8, maxit=500, aggregation=HEC, smoother=SMOOTHE
     return it, amg
 
 
+BATCH_RHS = 8
+
+
+def batch_solve(lib, systems, B, X, it, amg):
+    """hz_batch_krylov_amg on host data: systems = [(ia, ja, va)], B[s] and X[s]
+    arrays of shape (nrhs, n_s) (X: initial guesses, overwritten).  Returns
+    (status, iters[nsys, nrhs])."""
+    nsys = len(systems)
+    nrhs = B[0].shape[0]
+    mats = [HostCSR(*sysm) for sysm in systems]
+    Aptr = (C.POINTER(dCSRmat) * nsys)(*[C.pointer(m.mat) for m in mats])
+    Bc = [np.ascontiguousarray(b, dtype=np.float64) for b in B]
+    for x in X:
+        assert x.flags.c_contiguous and x.dtype == np.float64
+    Bptr = (C.POINTER(REAL) * nsys)(*[dptr(b) for b in Bc])
+    Xptr = (C.POINTER(REAL) * nsys)(*[dptr(x) for x in X])
+    iters = np.zeros(nsys * nrhs, np.int32)
+    r = lib.hz_batch_krylov_amg(nsys, Aptr, nrhs, Bptr, Xptr, C.byref(it), C.byref(amg), dptr(iters, INT))
+    return r, iters.reshape(nsys, nrhs)
+
+
 _LIB = None
 
 
@@ -303,6 +330,15 @@ def _declare(lib):
     lib.hz_aggregate_features.restype = INT
     lib.hz_cheby_smooth.argtypes = [P(dCSRmat), P(REAL), P(REAL), INT, REAL]
     lib.hz_cheby_smooth.restype = INT
+    lib.hz_batch_krylov_amg.argtypes = [INT, P(P(dCSRmat)), INT, P(P(REAL)), P(P(REAL)),
+                                        P(linear_itsolver_param), P(AMG_param), P(INT)]
+    lib.hz_batch_krylov_amg.restype = INT
+    lib.hz_batch_get_stats.argtypes = [P(hz_batch_stats)]
+    lib.hz_batch_get_stats.restype = None
+    lib.hz_batch_krylov_amg_device.argtypes = [INT, P(INT), INT, C.c_void_p, C.c_void_p, C.c_void_p,
+                                               C.c_void_p, C.c_void_p, P(linear_itsolver_param), P(AMG_param),
+                                               P(INT)]
+    lib.hz_batch_krylov_amg_device.restype = INT
     lib.hz_gs_smooth.argtypes = [P(dCSRmat), P(REAL), P(REAL), INT, INT, INT, REAL, P(INT), P(INT), P(INT)]
     lib.hz_gs_smooth.restype = INT
     lib.hz_sa_prolongation.argtypes = [P(dCSRmat), P(INT), INT, REAL, REAL, P(F), INT, P(dCSRmat)]

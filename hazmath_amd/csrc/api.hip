@@ -10,7 +10,7 @@
 using namespace hz;
 
 namespace {
-hz_options g_opt = {1.0, 0, 0, 0, 0};
+hz_options g_opt = {1.0, 0, 0, 0, 0, 0};
 hz_solve_info g_info;
 }  // namespace
 
@@ -175,6 +175,63 @@ int check_supported(const linear_itsolver_param *it, const AMG_param *amg) {
         std::printf("### ERROR: Schwarz smoothing is not available in this build\n");
         return HZ_ERROR_INPUT_PAR;
     }
+    return HZ_SUCCESS;
+}
+
+// The batched driver: CG + UA-AMG with HEM pairs, V(pre, post) Jacobi / L1DIAG,
+// dense coarsest solve, one cycle per application, relative-residual stop.
+int batch_check(const linear_itsolver_param *it, const AMG_param *amg) {
+    const char *what = nullptr;
+    if (it->linear_itsolver_type != HZ_SOLVER_CG) what = "the Krylov method must be CG (1)";
+    else if (it->linear_stop_type != HZ_STOP_REL_RES) what = "the stop type must be 1 (relative residual)";
+    else if (amg->AMG_type != HZ_UA_AMG) what = "AMG_type must be UA";
+    else if (amg->aggregation_type != HZ_HEM) what = "the aggregation must be HEM (5)";
+    else if (amg->smoother != HZ_SMOOTHER_JACOBI && amg->smoother != HZ_SMOOTHER_L1DIAG)
+        what = "the smoother must be JACOBI (1) or L1DIAG (10)";
+    else if (amg->cycle_type != HZ_V_CYCLE) what = "the cycle must be V";
+    else if (amg->coarse_solver != HZ_SOLVER_UMFPACK) what = "the coarse solver must be the direct one (32)";
+    else if (amg->coarse_scaling != 0) what = "coarse scaling must be off";
+    else if (amg->maxit != 1) what = "one cycle per preconditioner application (maxit = 1)";
+    else if (amg->Schwarz_levels > 0) what = "Schwarz levels must be 0";
+    if (what) {
+        std::printf("### ERROR: batched AMG-PCG: %s\n", what);
+        return HZ_ERROR_INPUT_PAR;
+    }
+    return HZ_SUCCESS;
+}
+
+hz_batch_stats g_batch_stats{};
+
+INT batch_core(const DevCSR &A, const std::vector<int> &off, const double *db, double *dx,
+               linear_itsolver_param *it, AMG_param *amg, INT *iters) {
+    const int st = batch_check(it, amg);
+    if (st < 0) return st;
+    BatchStats stats;
+    stats.time_kernels = g_opt.batch_time_kernels != 0;
+    std::vector<int> out((off.size() - 1) * kBatchRhs);
+    const int r = batch_solve(A, off, db, dx, it, amg, g_opt.jacobi_weight, out.data(), &stats);
+    if (r <= -1000) return fatal(r);
+    if (r < 0) return r;
+    std::memset(&g_info, 0, sizeof(g_info));
+    int maxit = 0;
+    for (int v : out) maxit = std::max(maxit, v);
+    g_info.iterations = maxit;
+    g_info.num_levels = stats.num_levels;
+    g_info.setup_ms = stats.setup_ms;
+    g_info.solve_ms = stats.solve_ms;
+    g_info.total_ms = stats.setup_ms + stats.solve_ms;
+    g_batch_stats.setup_ms = stats.setup_ms;
+    g_batch_stats.solve_ms = stats.solve_ms;
+    g_batch_stats.op_complexity = stats.op_complexity;
+    g_batch_stats.num_levels = stats.num_levels;
+    g_batch_stats.graph_launches = stats.graph_launches;
+    g_batch_stats.shape_level0 = stats.shape_level0;
+    g_batch_stats.spmm_ms = stats.spmm_ms;
+    g_batch_stats.sweep_ms = stats.sweep_ms;
+    g_batch_stats.rows = stats.rows;
+    g_batch_stats.nnz = stats.nnz;
+    g_batch_stats.sell_entries = stats.sell_entries;
+    if (iters) std::copy(out.begin(), out.end(), iters);
     return HZ_SUCCESS;
 }
 
@@ -1037,6 +1094,94 @@ INT hz_cheby_smooth(dCSRmat *A, const REAL *b, REAL *x, INT steps, REAL scale) {
     } catch (const DeviceError &) {
         return HZ_ERROR_DEVICE;
     }
+}
+
+INT hz_batch_krylov_amg_device(INT nsys, const INT *row_offsets, INT nnz, const INT *dIA, const INT *dJA,
+                               const REAL *dval, const REAL *db, REAL *dx, linear_itsolver_param *itparam,
+                               AMG_param *amgparam, INT *iters) {
+    if (nsys <= 0 || !row_offsets || row_offsets[0] != 0) return HZ_ERROR_INPUT_PAR;
+    if (batch_check(itparam, amgparam) < 0) return HZ_ERROR_INPUT_PAR;
+    for (INT s = 0; s < nsys; ++s)
+        if (row_offsets[s + 1] <= row_offsets[s]) return HZ_ERROR_MAT_SIZE;
+    if (!device_ready()) return HZ_ERROR_DEVICE;
+    try {
+        arena().reset();
+        DevCSR A;
+        A.n = A.ncol = row_offsets[nsys];
+        A.nnz = nnz;
+        A.ia = const_cast<int *>(dIA);
+        A.ja = const_cast<int *>(dJA);
+        A.val = const_cast<double *>(dval);
+        std::vector<int> off(row_offsets, row_offsets + nsys + 1);
+        return batch_core(A, off, db, dx, itparam, amgparam, iters);
+    } catch (const DeviceError &) {
+        return HZ_ERROR_DEVICE;
+    }
+}
+
+INT hz_batch_krylov_amg(INT nsys, dCSRmat *const *A, INT nrhs, REAL *const *b, REAL *const *x,
+                        linear_itsolver_param *itparam, AMG_param *amgparam, INT *iters) {
+    if (nsys <= 0 || nrhs < 1 || nrhs > HZ_BATCH_RHS) return HZ_ERROR_INPUT_PAR;
+    if (batch_check(itparam, amgparam) < 0) return HZ_ERROR_INPUT_PAR;
+    std::vector<int> off(nsys + 1, 0);
+    long long nnz = 0;
+    for (INT s = 0; s < nsys; ++s) {
+        if (A[s]->row != A[s]->col || A[s]->row <= 0 || A[s]->nnz <= 0) return HZ_ERROR_MAT_SIZE;
+        off[s + 1] = off[s] + A[s]->row;
+        nnz += A[s]->IA[A[s]->row];
+    }
+    if (nnz > INT32_MAX) return HZ_ERROR_MAT_SIZE;
+    if (!device_ready()) return HZ_ERROR_DEVICE;
+    try {
+        arena().reset();
+        const int n = off[nsys];
+        // the block-diagonal stack and the interleaved right-hand sides
+        std::vector<int> ia(n + 1), ja(nnz);
+        std::vector<double> va(nnz), hb((size_t)n * HZ_BATCH_RHS, 0.0), hx((size_t)n * HZ_BATCH_RHS, 0.0);
+        long long pos = 0;
+        for (INT s = 0; s < nsys; ++s) {
+            const dCSRmat *M = A[s];
+            const int r0 = off[s];
+            for (int i = 0; i < M->row; ++i) {
+                ia[r0 + i] = (int)pos;
+                for (int k = M->IA[i]; k < M->IA[i + 1]; ++k, ++pos) {
+                    ja[pos] = M->JA[k] + r0;
+                    va[pos] = M->val[k];
+                }
+                for (int q = 0; q < nrhs; ++q) {
+                    hb[(size_t)(r0 + i) * HZ_BATCH_RHS + q] = b[s][(size_t)q * M->row + i];
+                    hx[(size_t)(r0 + i) * HZ_BATCH_RHS + q] = x[s][(size_t)q * M->row + i];
+                }
+            }
+        }
+        ia[n] = (int)pos;
+        DevCSR dA;
+        dA.n = dA.ncol = n;
+        dA.nnz = (int)nnz;
+        dA.ia = upload(ia.data(), ia.size());
+        dA.ja = upload(ja.data(), ja.size());
+        dA.val = upload(va.data(), va.size());
+        const double *db = upload(hb.data(), hb.size());
+        double *dx = upload(hx.data(), hx.size());
+        std::vector<int> all((size_t)nsys * HZ_BATCH_RHS);
+        const INT st = batch_core(dA, off, db, dx, itparam, amgparam, all.data());
+        if (st < 0) return st;
+        HZ_CHECK(memcpy_sync(hx.data(), dx, sizeof(double) * hx.size(), hipMemcpyDeviceToHost));
+        for (INT s = 0; s < nsys; ++s) {
+            const int r0 = off[s], m = A[s]->row;
+            for (int q = 0; q < nrhs; ++q) {
+                for (int i = 0; i < m; ++i) x[s][(size_t)q * m + i] = hx[(size_t)(r0 + i) * HZ_BATCH_RHS + q];
+                if (iters) iters[(size_t)s * nrhs + q] = all[(size_t)s * HZ_BATCH_RHS + q];
+            }
+        }
+        return st;
+    } catch (const DeviceError &) {
+        return HZ_ERROR_DEVICE;
+    }
+}
+
+void hz_batch_get_stats(hz_batch_stats *out) {
+    if (out) *out = g_batch_stats;
 }
 
 INT hz_gs_smooth(dCSRmat *A, const REAL *b, REAL *x, INT smoother, INT post, INT nsweeps, REAL relax, INT *order,

@@ -225,4 +225,22 @@ void gs_prepare(const DevCSR &A, GsLevel &G);
 // pre (post = false) or post smoothing with smoother 2, 3, 5, 6, 7 or 8, in place
 void gs_smooth(const GsLevel &G, int smoother, bool post, int nsweeps, double relax, const double *b, double *x);
 
+// ---- kernels_batch.hip: many systems x kBatchRhs right-hand sides ---------
+constexpr int kBatchRhs = 8;
+struct BatchStats {
+    double setup_ms = 0, solve_ms = 0, op_complexity = 0;
+    int num_levels = 0, graph_launches = 0, shape_level0 = 0;
+    bool time_kernels = false;   // also time the level-0 SpMM and FP32 sweep
+    double spmm_ms = 0, sweep_ms = 0;
+    long long rows = 0, nnz = 0, sell_entries = 0;
+};
+// A0: block-diagonal stack of the systems (device CSR, global column indices),
+// off0: nsys + 1 row offsets; b, u: [n][kBatchRhs] FP64 on the device (u: initial
+// guess in, solution out); iters: nsys * kBatchRhs results (iterations or an
+// error code per column).
+int batch_solve(const DevCSR &A0, const std::vector<int> &off0, const double *b, double *u,
+                const linear_itsolver_param *itp, AMG_param *amg, double jacobi_w, int *iters, BatchStats *stats);
+// strength graph with a per-row squared threshold
+void strength_graph_rows(const DevCSR &A, const double *th2_row, DevCSR &N, int *counts_ws);
+
 }  // namespace hz

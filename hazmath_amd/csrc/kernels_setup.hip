@@ -40,11 +40,13 @@ __global__ void k_first_diag(int n, const int *__restrict__ ia, const int *__res
 
 template <bool kFill>
 __global__ void k_strength(int n, const int *__restrict__ ia, const int *__restrict__ ja,
-                           const double *__restrict__ val, const double *__restrict__ d, double th2,
+                           const double *__restrict__ val, const double *__restrict__ d, double th2c,
                            int *__restrict__ count, const int *__restrict__ nia,
-                           int *__restrict__ nja, double *__restrict__ nval) {
+                           int *__restrict__ nja, double *__restrict__ nval,
+                           const double *__restrict__ th2_row) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const double di = d[i];
+        const double th2 = th2_row ? th2_row[i] : th2c;
         int pos = kFill ? nia[i] : 0;
         int cnt = 0;
         for (int k = ia[i]; k < ia[i + 1]; ++k) {
@@ -329,15 +331,15 @@ long long exclusive_scan_int(const int *in, int *out, int n) {
     return total;
 }
 
-void strength_graph(const DevCSR &A, double theta, DevCSR &N, int *counts_ws) {
+namespace {
+void strength_impl(const DevCSR &A, double th2, const double *th2_row, DevCSR &N, int *counts_ws) {
     hipStream_t st = stream();
     const int n = A.n;
     double *d = arena().alloc_n<double>((size_t)n + 1);
     k_first_diag<<<grid_for(n), kBlock, 0, st>>>(n, A.ia, A.ja, A.val, d);
     HZ_LAUNCH_CHECK();
-    const double th2 = theta * theta;
     k_strength<false><<<grid_for(n), kBlock, 0, st>>>(n, A.ia, A.ja, A.val, d, th2, counts_ws,
-                                                      nullptr, nullptr, nullptr);
+                                                      nullptr, nullptr, nullptr, th2_row);
     HZ_LAUNCH_CHECK();
     N.n = n;
     N.ncol = A.ncol;
@@ -346,8 +348,17 @@ void strength_graph(const DevCSR &A, double theta, DevCSR &N, int *counts_ws) {
     N.ja = arena().alloc_n<int>((size_t)N.nnz + 1);
     N.val = arena().alloc_n<double>((size_t)N.nnz + 1);
     k_strength<true><<<grid_for(n), kBlock, 0, st>>>(n, A.ia, A.ja, A.val, d, th2, nullptr, N.ia,
-                                                     N.ja, N.val);
+                                                     N.ja, N.val, th2_row);
     HZ_LAUNCH_CHECK();
+}
+}  // namespace
+
+void strength_graph(const DevCSR &A, double theta, DevCSR &N, int *counts_ws) {
+    strength_impl(A, theta * theta, nullptr, N, counts_ws);
+}
+
+void strength_graph_rows(const DevCSR &A, const double *th2_row, DevCSR &N, int *counts_ws) {
+    strength_impl(A, 0.0, th2_row, N, counts_ws);
 }
 
 int hec_aggregate(const DevCSR &N, const int *perm, int *agg) {

@@ -267,3 +267,23 @@ def kuhn_element_stiffness_check():
         vol = abs(np.linalg.det(T)) / 6.0
         K[np.ix_(tet, tet)] += vol * G.T @ G
     return K
+
+
+def batch_anisotropic_2d(count: int = 256, seed: int = 4, side_range=(33, 513)):
+    """The batched workload: `count` 2-D P1 anisotropic diffusion systems with
+    grid sides (nodes per side, boundary included) log-uniform in side_range,
+    theta ~ U[0, pi), eps = 10^U[-4, 0].  Returns a list of dicts with the
+    parameters; the matrix of entry d is aniso2d_p1(d["side"] - 1, d["theta"], d["eps"])."""
+    rng = np.random.default_rng(seed)
+    lo, hi = np.log(side_range[0]), np.log(side_range[1])
+    out = []
+    for k in range(count):
+        side = int(np.clip(np.rint(np.exp(rng.uniform(lo, hi))), side_range[0], side_range[1]))
+        out.append(dict(side=side, theta=float(rng.uniform(0.0, np.pi)), eps=float(10.0 ** rng.uniform(-4.0, 0.0)),
+                        index=k))
+    return out
+
+
+def batch_rhs(n: int, system: int, nrhs: int = 8, seed: int = 4):
+    """nrhs seeded U[-1, 1) right-hand sides of system `system`, shape (nrhs, n)."""
+    return np.stack([uniform_rhs(n, seed=(seed * 1000003 + system) * 16 + q) for q in range(nrhs)])

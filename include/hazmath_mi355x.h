@@ -316,6 +316,8 @@ typedef struct {
                               1: FP32 vectors and matrix values, FP32 sums;
                               2: as 1 with BF16 matrix values on levels below the finest.
                               Modes 1 and 2 need coarse_scaling = OFF. */
+    INT  batch_time_kernels; /* >0: the batched solver also times its level-0 SpMM and
+                              sweep after the solve (hz_batch_get_stats) */
 } hz_options;
 
 void hz_get_options(hz_options *opt);
@@ -457,6 +459,35 @@ INT hz_aggregate_features(dCSRmat *A, const INT *aggregates, INT num_aggregates,
 INT hz_cheby_smooth(dCSRmat *A, const REAL *b, REAL *x, INT steps, REAL scale);
 INT hz_sa_prolongation(dCSRmat *A, const INT *aggregates, INT num_aggregates, REAL theta, REAL omega,
                        const float *omega_agg, INT filter, dCSRmat *P);
+/* No counterpart in the reference: many independent systems, up to
+ * HZ_BATCH_RHS right-hand sides each, through one AMG-PCG -- the semantics of
+ * linear_solver_dcsr_krylov_amg (itsolver.c:846-931) per (system, right-hand
+ * side) pair, with the V-cycle in FP32 and CG in FP64.  Settings: CG, stop
+ * type 1, UA with HEM aggregation, V-cycle with JACOBI or L1DIAG, dense
+ * coarsest solve (coarse_solver 32), maxit 1, no coarse scaling; anything
+ * else returns ERROR_INPUT_PAR.  Host form: b[s] and x[s] hold nrhs vectors of
+ * A[s]->row entries one after the other (x: initial guesses in, solutions
+ * out); iters[s * nrhs + q] receives the iteration count of pair (s, q), or
+ * ERROR_SOLVER_MAXIT / ERROR_SOLVER_TOLSMALL.  The caller's
+ * amgparam->strong_coupled is not modified (each system keeps its own
+ * threshold sequence internally).  Device form: the systems stacked into one
+ * block-diagonal CSR (row_offsets[nsys+1], global column indices) and b, x as
+ * n x HZ_BATCH_RHS row-major arrays (entry (i, q) at i * HZ_BATCH_RHS + q);
+ * iters has nsys * HZ_BATCH_RHS entries. */
+#define HZ_BATCH_RHS 8
+/* Measurements of the last batched call (no counterpart in the reference). */
+typedef struct hz_batch_stats {
+    double setup_ms, solve_ms, op_complexity;
+    INT num_levels, graph_launches, shape_level0;
+    double spmm_ms, sweep_ms;      /* level-0 FP64 SpMM and FP32 sweep, when timed */
+    long long rows, nnz, sell_entries;
+} hz_batch_stats;
+void hz_batch_get_stats(hz_batch_stats *out);
+INT hz_batch_krylov_amg(INT nsys, dCSRmat *const *A, INT nrhs, REAL *const *b, REAL *const *x,
+                        linear_itsolver_param *itparam, AMG_param *amgparam, INT *iters);
+INT hz_batch_krylov_amg_device(INT nsys, const INT *row_offsets, INT nnz, const INT *dIA, const INT *dJA,
+                               const REAL *dval, const REAL *db, REAL *dx, linear_itsolver_param *itparam,
+                               AMG_param *amgparam, INT *iters);
 /* Parity hook: dcsr_presmoothing (post = 0) or dcsr_postsmoothing (post = 1)
  * of mgcycle.c:76-223 with a Gauss-Seidel-type smoother (2, 3, 5, 6, 7, 8) as
  * the library's multicolour sweeps, x updated in place; order[n] and cptr[n+1]

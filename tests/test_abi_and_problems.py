@@ -38,7 +38,7 @@ def test_struct_layouts_agree_between_header_and_ctypes(tmp_path):
                "linear_itsolver_param": capi.linear_itsolver_param, "AMG_param": capi.AMG_param,
                "hz_options": capi.hz_options, "hz_solve_info": capi.hz_solve_info,
                "hz_hierarchy": capi.hz_hierarchy, "hz_kernel_times": capi.hz_kernel_times,
-               "precond": capi.precond}
+               "precond": capi.precond, "hz_batch_stats": capi.hz_batch_stats}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "hazmath_mi355x.h"',
              'int main(void){']
     for s, cls in structs.items():
@@ -301,3 +301,26 @@ def test_committed_model_loads():
     theta = learned.load_model()
     assert theta.dtype == np.float32 and theta.shape == (capi.MLP_PARAMS,)
     assert np.isfinite(theta).all()
+
+
+def test_batch_rejects_settings_outside_its_scope():
+    """The batched driver refuses, before touching the device, every setting it
+    does not implement (ERROR_INPUT_PAR = -13)."""
+    import ctypes as C
+    lib = capi.load_library()
+    ia, ja, va = problems.aniso2d_p1(8, 0.2, 0.1)
+    b = np.ones((8, ia.size - 1))
+    x = np.zeros((8, ia.size - 1))
+    for field, value in (("aggregation_type", capi.HEC), ("smoother", capi.SMOOTHER_GS), ("cycle_type", 2),
+                         ("coarse_solver", 0), ("maxit", 2), ("AMG_type", capi.SA_AMG)):
+        it, amg = capi.solver_params(lib, aggregation=capi.HEM, smoother=capi.SMOOTHER_L1DIAG)
+        setattr(amg, field, value)
+        r, _ = capi.batch_solve(lib, [(ia, ja, va)], [b], [x], it, amg)
+        assert r == -13, field
+    it, amg = capi.solver_params(lib, aggregation=capi.HEM, smoother=capi.SMOOTHER_L1DIAG)
+    it.linear_stop_type = 2
+    assert capi.batch_solve(lib, [(ia, ja, va)], [b], [x], it, amg)[0] == -13
+    # nrhs beyond HZ_BATCH_RHS
+    it, amg = capi.solver_params(lib, aggregation=capi.HEM, smoother=capi.SMOOTHER_L1DIAG)
+    r, _ = capi.batch_solve(lib, [(ia, ja, va)], [np.ones((9, ia.size - 1))], [np.zeros((9, ia.size - 1))], it, amg)
+    assert r == -13

@@ -987,3 +987,138 @@ def test_gs_family_linear_solver_amg_vs_oracle(lib, orc, libc, smoother):
     assert abs(ig - io) <= _gs_tolerance(io, 0.35)
     Asp = __import__("scipy.sparse", fromlist=["csr_matrix"]).csr_matrix((va, ja, ia))
     assert np.linalg.norm(b - Asp @ xg) / np.linalg.norm(b) < 1e-8
+
+
+# ---- batched systems x 8 right-hand sides (hz_batch_krylov_amg) -------------
+# Bar: every (system, right-hand side) pair behaves as the single-system
+# solver with the same FP32 cycle (linear_solver_dcsr_krylov_amg with
+# vcycle_precision = 1): iteration counts within +-1 and solutions within
+# 1e-9 relative; against the FP64 oracle on a seeded subset: iteration counts
+# within +-2 (the cycle's rounding differs) and the FP64 true residual below
+# the tolerance (check III recomputes it before a pair is declared converged).
+
+def _batch_case(nrhs=8, tiny=True):
+    specs = problems.batch_anisotropic_2d(7, seed=9, side_range=(33, 97))
+    systems, B = [], []
+    for d in specs:
+        ia, ja, va = problems.aniso2d_p1(d["side"] - 1, d["theta"], d["eps"])
+        systems.append((ia, ja, va))
+        B.append(problems.batch_rhs(ia.size - 1, d["index"], nrhs=nrhs))
+    if tiny:   # 121 rows <= coarse_dof: solved by its dense inverse at level 0
+        ia, ja, va = problems.aniso2d_p1(12, 0.3, 0.01)
+        systems.insert(2, (ia, ja, va))
+        B.insert(2, problems.batch_rhs(ia.size - 1, 99, nrhs=nrhs))
+    return systems, B
+
+
+def _batch_params(lib, tol=1e-10, **kw):
+    it, amg = capi.solver_params(lib, tol=tol, maxit=500, aggregation=capi.HEM, smoother=capi.SMOOTHER_L1DIAG,
+                                 coarse_dof=200, theta=0.04, **kw)
+    return it, amg
+
+
+def test_batch_matches_single_system_fp32_cycle(lib, libc):
+    systems, B = _batch_case(nrhs=5)
+    B[3][1] = 0.0      # a zero right-hand side finishes at once
+    X = [np.zeros_like(b) for b in B]
+    it, amg = _batch_params(lib)
+    r, iters = capi.batch_solve(lib, systems, B, X, it, amg)
+    assert r == 0
+    assert iters[3, 1] == 0 and not X[3][1].any()
+    # dense-inverse-only system: exact up to the cycle's FP32 output, so the
+    # second iteration reaches 1e-10
+    assert iters[2].max() <= 2
+    old = _set_precision(lib, 1)
+    try:
+        for k, (ia, ja, va) in enumerate(systems):
+            A = capi.HostCSR(ia, ja, va)
+            for q in range(B[k].shape[0]):
+                if not B[k][q].any():
+                    continue
+                its, amgs = _batch_params(lib)
+                x = capi.HostVec(np.zeros(ia.size - 1))
+                libc.srand(1)
+                rs = lib.linear_solver_dcsr_krylov_amg(A.ptr(), capi.HostVec(B[k][q]).ptr(), x.ptr(), C.byref(its),
+                                                       C.byref(amgs))
+                print(f"[batch] system {k} rhs {q}: batched {iters[k, q]} single {rs}", flush=True)
+                assert abs(int(iters[k, q]) - rs) <= 1
+                assert np.linalg.norm(X[k][q] - x.arr) <= 1e-9 * np.linalg.norm(x.arr)
+    finally:
+        _set_precision(lib, old)
+
+
+def test_batch_vs_oracle_subset(lib, orc, libc):
+    import oracle
+    import scipy.sparse as sp
+    systems, B = _batch_case()
+    X = [np.zeros_like(b) for b in B]
+    it, amg = _batch_params(lib)
+    r, iters = capi.batch_solve(lib, systems, B, X, it, amg)
+    assert r == 0
+    rng = np.random.default_rng(3)
+    for k, q in zip(rng.integers(0, len(systems), 5), rng.integers(0, 8, 5)):
+        ia, ja, va = systems[k]
+        A = capi.HostCSR(ia, ja, va)
+        ito, amgo = oracle.params(tol=1e-10, maxit=500, aggregation=capi.HEM, smoother=capi.SMOOTHER_L1DIAG,
+                                  coarse_dof=200, theta=0.04)
+        x = capi.HostVec(np.zeros(ia.size - 1))
+        libc.srand(1)
+        io = orc.orc_krylov_amg(A.ptr(), capi.HostVec(B[k][q]).ptr(), x.ptr(), C.byref(ito), C.byref(amgo),
+                                C.byref(capi.orc_trace()))
+        print(f"[batch] system {k} rhs {q}: batched {iters[k, q]} oracle {io}", flush=True)
+        assert io > 0 and abs(int(iters[k, q]) - io) <= 2
+        M = sp.csr_matrix((va, ja, ia))
+        assert np.linalg.norm(B[k][q] - M @ X[k][q]) / np.linalg.norm(B[k][q]) < 1e-10
+        assert np.linalg.norm(X[k][q] - x.arr) <= 1e-8 * np.linalg.norm(x.arr)
+
+
+def test_batch_device_entry_matches_host_entry_and_repeats(lib):
+    """The device form (stacked CSR, interleaved RHS) gives the host form's
+    bits, and a second call (graph rebuilt) repeats them."""
+    systems, B = _batch_case(tiny=False)
+    X = [np.zeros_like(b) for b in B]
+    it, amg = _batch_params(lib)
+    r, iters = capi.batch_solve(lib, systems, B, X, it, amg)
+    assert r == 0
+    off, ias, jas, vas, bs = [0], [], [], [], []
+    nz = 0
+    for (ia, ja, va), b in zip(systems, B):
+        ias.append(ia[:-1] + nz)
+        jas.append(ja + off[-1])
+        vas.append(va)
+        bs.append(b.T)
+        nz += int(ia[-1])
+        off.append(off[-1] + ia.size - 1)
+    host = [np.ascontiguousarray(np.concatenate(ias + [np.array([nz])]).astype(np.int32)),
+            np.ascontiguousarray(np.concatenate(jas).astype(np.int32)),
+            np.ascontiguousarray(np.concatenate(vas)),
+            np.ascontiguousarray(np.concatenate(bs))]
+    hip = _hip_runtime()
+    dev = []
+    try:
+        for h in host + [np.zeros_like(host[3])]:
+            p = C.c_void_p()
+            assert hip.hipMalloc(C.byref(p), C.c_size_t(h.nbytes)) == 0
+            dev.append(p)
+            assert hip.hipMemcpy(p, h.ctypes.data_as(C.c_void_p), C.c_size_t(h.nbytes), 1) == 0
+        outs = []
+        for rep in range(2):
+            zero = np.zeros_like(host[3])
+            assert hip.hipMemcpy(dev[4], zero.ctypes.data_as(C.c_void_p), C.c_size_t(zero.nbytes), 1) == 0
+            its = np.zeros(len(systems) * 8, np.int32)
+            it2, amg2 = _batch_params(lib)
+            r = lib.hz_batch_krylov_amg_device(len(systems), (capi.INT * len(off))(*off), nz, dev[0], dev[1],
+                                               dev[2], dev[3], dev[4], C.byref(it2), C.byref(amg2),
+                                               capi.dptr(its, capi.INT))
+            assert r == 0
+            x = np.zeros_like(host[3])
+            assert hip.hipMemcpy(x.ctypes.data_as(C.c_void_p), dev[4], C.c_size_t(x.nbytes), 2) == 0
+            outs.append((x, its))
+    finally:
+        for p in dev:
+            hip.hipFree(p)
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+    x = outs[0][0]
+    for k in range(len(systems)):
+        assert np.array_equal(x[off[k]:off[k + 1]].T, X[k])
+    assert np.array_equal(outs[0][1].reshape(len(systems), 8), iters)
