@@ -82,7 +82,8 @@ class AMG_param(C.Structure):
 
 class hz_options(C.Structure):
     _fields_ = [("jacobi_weight", REAL), ("hec_rng_source", INT), ("verbose_timing", INT),
-                ("return_on_fatal", INT), ("vcycle_precision", INT), ("batch_time_kernels", INT)]
+                ("return_on_fatal", INT), ("vcycle_precision", INT), ("batch_time_kernels", INT),
+                ("time_pcg_spmv", INT)]
 
 
 class hz_batch_stats(C.Structure):
@@ -96,7 +97,8 @@ class hz_solve_info(C.Structure):
                 ("level_rows", INT * MAX_AMG_LVL), ("level_nnz", INT * MAX_AMG_LVL),
                 ("setup_ms", REAL), ("solve_ms", REAL), ("total_ms", REAL),
                 ("final_relres", REAL), ("condest", REAL), ("relres_count", INT),
-                ("relres_hist", REAL * 4096), ("fallback_used", INT)]
+                ("relres_hist", REAL * 4096), ("fallback_used", INT), ("pcg_spmv_ms", REAL),
+                ("pcg_spmv_count", INT)]
 
 
 class hz_hierarchy(C.Structure):

@@ -56,6 +56,7 @@ struct Hierarchy {
     // and Chebyshev bound scale come from the network with these weights
     const float *mlp_theta = nullptr;
     std::vector<double> cheb_scale;   // per level (default 1.0)
+    LaunchTimer *spmv_timer = nullptr;  // when set: events around the PCG's fine-level SpMV
 };
 
 // Runs the reference's setup loop on the device.  A0 arrays are borrowed.

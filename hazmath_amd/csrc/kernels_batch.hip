@@ -251,8 +251,8 @@ __global__ __launch_bounds__(kBlock) void k_bsweep(int n, int nslices, const lon
         float aa[4], xx[4][kN];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            cc[u] = c[(long long)(k + u) * kSlice];
-            aa[u] = v[(long long)(k + u) * kSlice];
+            cc[u] = ld_nt(c + (long long)(k + u) * kSlice);
+            aa[u] = ld_nt(v + (long long)(k + u) * kSlice);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -262,10 +262,10 @@ __global__ __launch_bounds__(kBlock) void k_bsweep(int n, int nslices, const lon
         for (int u = 0; u < 4; ++u) step(cc[u], aa[u], xx[u]);
     }
     for (; k < wd; ++k) {
-        const int cj = c[(long long)k * kSlice];
+        const int cj = ld_nt(c + (long long)k * kSlice);
         float xj[kN];
         if (cj >= 0) ld_vec<kN>(x + (size_t)cj * kR + m.q0, xj);
-        step(cj, v[(long long)k * kSlice], xj);
+        step(cj, ld_nt(v + (long long)k * kSlice), xj);
     }
     float xn[kN];
 #pragma unroll
@@ -306,8 +306,8 @@ __global__ __launch_bounds__(kBlock) void k_bresidual(int n, int nslices, const 
         float aa[4], xx[4][kN];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            cc[u] = c[(long long)(k + u) * kSlice];
-            aa[u] = v[(long long)(k + u) * kSlice];
+            cc[u] = ld_nt(c + (long long)(k + u) * kSlice);
+            aa[u] = ld_nt(v + (long long)(k + u) * kSlice);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -321,9 +321,9 @@ __global__ __launch_bounds__(kBlock) void k_bresidual(int n, int nslices, const 
         }
     }
     for (; k < wd; ++k) {
-        const int cj = c[(long long)k * kSlice];
+        const int cj = ld_nt(c + (long long)k * kSlice);
         if (cj < 0) continue;
-        const float a = v[(long long)k * kSlice];
+        const float a = ld_nt(v + (long long)k * kSlice);
         float xj[kN];
         ld_vec<kN>(x + (size_t)cj * kR + m.q0, xj);
 #pragma unroll
@@ -488,8 +488,8 @@ __global__ __launch_bounds__(kBlock) void k_spmm64(int n, int nslices, const lon
         double2 x0[4], x1[4];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
-            cc[u] = c[(long long)(k + u) * kSlice];
-            aa[u] = v[(long long)(k + u) * kSlice];
+            cc[u] = ld_nt(c + (long long)(k + u) * kSlice);
+            aa[u] = ld_nt(v + (long long)(k + u) * kSlice);
         }
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
@@ -509,9 +509,9 @@ __global__ __launch_bounds__(kBlock) void k_spmm64(int n, int nslices, const lon
         }
     }
     for (; k < wd; ++k) {
-        const int cj = c[(long long)k * kSlice];
+        const int cj = ld_nt(c + (long long)k * kSlice);
         if (cj < 0) continue;
-        const double a = v[(long long)k * kSlice];
+        const double a = ld_nt(v + (long long)k * kSlice);
         const double2 *xp = reinterpret_cast<const double2 *>(x + (size_t)cj * kR + m.q0);
         const double2 x0 = xp[0], x1 = xp[1];
         acc[0] += a * x0.x;

@@ -185,15 +185,15 @@ __device__ __forceinline__ double slice_row_sub(double t, const int *__restrict_
         double v[8], xv[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            c[u] = cp[(long long)(k + u) * kSlice];
-            v[u] = vp[(long long)(k + u) * kSlice];
+            c[u] = ld_nt(cp + (long long)(k + u) * kSlice);
+            v[u] = ld_nt(vp + (long long)(k + u) * kSlice);
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) xv[u] = x[c[u]];
 #pragma unroll
         for (int u = 0; u < 8; ++u) t -= v[u] * xv[u];
     }
-    for (; k < width; ++k) t -= vp[(long long)k * kSlice] * x[cp[(long long)k * kSlice]];
+    for (; k < width; ++k) t -= ld_nt(vp + (long long)k * kSlice) * x[ld_nt(cp + (long long)k * kSlice)];
     return t;
 }
 

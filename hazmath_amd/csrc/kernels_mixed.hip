@@ -22,6 +22,7 @@ constexpr int kSlicesPerBlock = kBlock / kWave;
 constexpr int kFinBlock = 1024;
 constexpr int kStreamGrid = 2048;
 constexpr float kSmallRealF = 1e-20f;
+constexpr int kU = 8;   // SELL entries per round: their loads are issued before the math
 
 inline int grid_for(long n) {
     long g = (n + kBlock - 1) / kBlock;
@@ -29,9 +30,10 @@ inline int grid_for(long n) {
 }
 
 // bf16 stored as the upper half of the FP32 bit pattern
-__device__ __forceinline__ float ldv(const float *p) { return *p; }
+// matrix values are streamed once per launch: non-temporal loads
+__device__ __forceinline__ float ldv(const float *p) { return ld_nt(p); }
 __device__ __forceinline__ float ldv(const unsigned short *p) {
-    return __uint_as_float(((unsigned int)*p) << 16);
+    return __uint_as_float(((unsigned int)ld_nt(p)) << 16);
 }
 
 __device__ __forceinline__ unsigned short f2bf(float f) {
@@ -105,25 +107,25 @@ __global__ __launch_bounds__(kBlock) void k_sweep_m(int n, int nslices, const lo
         float t = (r < n) ? b[r] : 0.0f;
         float d = 0.0f;
         int k = 0;
-        for (; k + 4 <= wd; k += 4) {
-            int cc[4];
-            float vv[4], xx[4];
+        for (; k + kU <= wd; k += kU) {
+            int cc[kU];
+            float vv[kU], xx[kU];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                cc[u] = c[(k + u) * kSlice];
+            for (int u = 0; u < kU; ++u) {
+                cc[u] = ld_nt(c + (k + u) * kSlice);
                 vv[u] = ldv(v + (k + u) * kSlice);
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) xx[u] = cc[u] >= 0 ? x[cc[u]] : 0.0f;
+            for (int u = 0; u < kU; ++u) xx[u] = cc[u] >= 0 ? x[cc[u]] : 0.0f;
             if (kProlong) {
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
+                for (int u = 0; u < kU; ++u) {
                     const int g = cc[u] >= 0 ? agg[cc[u]] : HZ_UNPT;
                     if (g > HZ_UNPT) xx[u] = xx[u] + xc[g];
                 }
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < kU; ++u) {
                 if (kSmoother == HZ_SMOOTHER_L1DIAG) {
                     if (cc[u] >= 0) {
                         t = t - vv[u] * xx[u];
@@ -136,7 +138,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_m(int n, int nslices, const lo
             }
         }
         for (; k < wd; ++k) {
-            const int cj = c[k * kSlice];
+            const int cj = ld_nt(c + k * kSlice);
             const float a = ldv(v + k * kSlice);
             float xj = cj >= 0 ? x[cj] : 0.0f;
             if (kProlong && cj >= 0) {
@@ -194,22 +196,22 @@ __global__ __launch_bounds__(kBlock) void k_residual_m(int n, int nslices, const
     const MT *v = val + base + lane;
     float acc = 0.0f;
     int k = 0;
-    for (; k + 4 <= wd; k += 4) {
-        int cc[4];
-        float vv[4], xx[4];
+    for (; k + kU <= wd; k += kU) {
+        int cc[kU];
+        float vv[kU], xx[kU];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            cc[u] = c[(k + u) * kSlice];
+        for (int u = 0; u < kU; ++u) {
+            cc[u] = ld_nt(c + (k + u) * kSlice);
             vv[u] = ldv(v + (k + u) * kSlice);
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) xx[u] = cc[u] >= 0 ? x[cc[u]] : 0.0f;
+        for (int u = 0; u < kU; ++u) xx[u] = cc[u] >= 0 ? x[cc[u]] : 0.0f;
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < kU; ++u)
             if (cc[u] >= 0) acc = acc + vv[u] * xx[u];
     }
     for (; k < wd; ++k) {
-        const int cj = c[k * kSlice];
+        const int cj = ld_nt(c + k * kSlice);
         if (cj >= 0) acc = acc + ldv(v + k * kSlice) * x[cj];
     }
     if (r < n) w[r] = b[r] - acc;

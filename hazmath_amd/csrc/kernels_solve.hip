@@ -57,6 +57,36 @@ __global__ void k_sell_fill(int n, const int *__restrict__ ia, const int *__rest
     }
 }
 
+// Row sum from 0.0 in stored order over one lane's SELL entries (padding col
+// -1 skipped).  Entries go eight at a time: their column and value loads
+// (non-temporal) are issued first, then the eight gathers, then the adds in
+// order, so a row costs a few memory round trips instead of one per entry.
+template <bool kNtTail>
+__device__ __forceinline__ double sell_row_dot(const int *c, const double *v, int w, const double *x) {
+    constexpr int U = 8;
+    double acc = 0.0;
+    int k = 0;
+    for (; k + U <= w; k += U) {
+        int cc[U];
+        double vv[U], xx[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            cc[u] = ld_nt(c + (k + u) * kSlice);
+            vv[u] = ld_nt(v + (k + u) * kSlice);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) xx[u] = cc[u] >= 0 ? x[cc[u]] : 0.0;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (cc[u] >= 0) acc = acc + vv[u] * xx[u];
+    }
+    for (; k < w; ++k) {
+        const int c0 = kNtTail ? ld_nt(c + k * kSlice) : c[k * kSlice];
+        if (c0 >= 0) acc = acc + (kNtTail ? ld_nt(v + k * kSlice) : v[k * kSlice]) * x[c0];
+    }
+    return acc;
+}
+
 // y = A x (row sums from 0.0 in CSR order); optional deterministic (y, x).
 template <bool kDot>
 __global__ __launch_bounds__(kBlock) void k_spmv(int n, int nslices, const long long *__restrict__ sptr,
@@ -76,23 +106,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n, int nslices, const long 
         const int w = (int)((sptr[s + 1] - base) / kSlice);
         const int *c = col + base + lane;
         const double *v = val + base + lane;
-        int k = 0;
-        for (; k + 4 <= w; k += 4) {
-            const int c0 = c[(k + 0) * kSlice], c1 = c[(k + 1) * kSlice];
-            const int c2 = c[(k + 2) * kSlice], c3 = c[(k + 3) * kSlice];
-            const double v0 = v[(k + 0) * kSlice], v1 = v[(k + 1) * kSlice];
-            const double v2 = v[(k + 2) * kSlice], v3 = v[(k + 3) * kSlice];
-            const double x0 = c0 >= 0 ? x[c0] : 0.0, x1 = c1 >= 0 ? x[c1] : 0.0;
-            const double x2 = c2 >= 0 ? x[c2] : 0.0, x3 = c3 >= 0 ? x[c3] : 0.0;
-            if (c0 >= 0) acc = acc + v0 * x0;
-            if (c1 >= 0) acc = acc + v1 * x1;
-            if (c2 >= 0) acc = acc + v2 * x2;
-            if (c3 >= 0) acc = acc + v3 * x3;
-        }
-        for (; k < w; ++k) {
-            const int c0 = c[k * kSlice];
-            if (c0 >= 0) acc = acc + v[k * kSlice] * x[c0];
-        }
+        acc = sell_row_dot<true>(c, v, w, x);
         if (r < n) {
             y[r] = acc;
             if (kDot) prod = acc * x[r];
@@ -119,24 +133,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(int n, int nslices, const l
     const int wd = (int)((sptr[s + 1] - base) / kSlice);
     const int *c = col + base + lane;
     const double *v = val + base + lane;
-    double acc = 0.0;
-    int k = 0;
-    for (; k + 4 <= wd; k += 4) {
-        const int c0 = c[(k + 0) * kSlice], c1 = c[(k + 1) * kSlice];
-        const int c2 = c[(k + 2) * kSlice], c3 = c[(k + 3) * kSlice];
-        const double v0 = v[(k + 0) * kSlice], v1 = v[(k + 1) * kSlice];
-        const double v2 = v[(k + 2) * kSlice], v3 = v[(k + 3) * kSlice];
-        const double x0 = c0 >= 0 ? x[c0] : 0.0, x1 = c1 >= 0 ? x[c1] : 0.0;
-        const double x2 = c2 >= 0 ? x[c2] : 0.0, x3 = c3 >= 0 ? x[c3] : 0.0;
-        if (c0 >= 0) acc = acc + v0 * x0;
-        if (c1 >= 0) acc = acc + v1 * x1;
-        if (c2 >= 0) acc = acc + v2 * x2;
-        if (c3 >= 0) acc = acc + v3 * x3;
-    }
-    for (; k < wd; ++k) {
-        const int c0 = c[k * kSlice];
-        if (c0 >= 0) acc = acc + v[k * kSlice] * x[c0];
-    }
+    const double acc = sell_row_dot<false>(c, v, wd, x);
     if (r < n) w[r] = b[r] - acc;
 }
 
@@ -184,29 +181,30 @@ __global__ __launch_bounds__(kBlock) void k_sweep(int n, int nslices, const long
         const double *v = val + base + lane;
         double t = (r < n) ? b[r] : 0.0;
         double d = 0.0;
+        constexpr int U = 8;   // entries per round: loads first (matrix non-temporal), then the math in order
         int k = 0;
-        for (; k + 4 <= wd; k += 4) {
-            int cc[4];
-            double vv[4], xx[4];
+        for (; k + U <= wd; k += U) {
+            int cc[U];
+            double vv[U], xx[U];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                cc[u] = c[(k + u) * kSlice];
-                vv[u] = v[(k + u) * kSlice];
+            for (int u = 0; u < U; ++u) {
+                cc[u] = ld_nt(c + (k + u) * kSlice);
+                vv[u] = ld_nt(v + (k + u) * kSlice);
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) xx[u] = cc[u] >= 0 ? x[cc[u]] : 0.0;
+            for (int u = 0; u < U; ++u) xx[u] = cc[u] >= 0 ? x[cc[u]] : 0.0;
             if (kProlong) {
-                int gg[4];
-                double xg[4];
+                int gg[U];
+                double xg[U];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) gg[u] = cc[u] >= 0 ? agg[cc[u]] : HZ_UNPT;
+                for (int u = 0; u < U; ++u) gg[u] = cc[u] >= 0 ? agg[cc[u]] : HZ_UNPT;
 #pragma unroll
-                for (int u = 0; u < 4; ++u) xg[u] = gg[u] > HZ_UNPT ? xc[gg[u]] : 0.0;
+                for (int u = 0; u < U; ++u) xg[u] = gg[u] > HZ_UNPT ? xc[gg[u]] : 0.0;
 #pragma unroll
-                for (int u = 0; u < 4; ++u) xx[u] = prolong_value(xx[u], gg[u], xg[u], al);
+                for (int u = 0; u < U; ++u) xx[u] = prolong_value(xx[u], gg[u], xg[u], al);
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < U; ++u) {
                 if (kSmoother == HZ_SMOOTHER_L1DIAG) {
                     if (cc[u] >= 0) {
                         t = t - vv[u] * xx[u];
@@ -568,12 +566,15 @@ void sell_from_csr(const DevCSR &A, DevSELL &S) {
     }
 }
 
-void spmv(const DevSELL &S, const double *x, double *y, double *dot_out, const ReduceWs &ws) {
+void spmv(const DevSELL &S, const double *x, double *y, double *dot_out, const ReduceWs &ws,
+          LaunchTimer *timer) {
     if (S.n == 0) return;
     const int grid = div_up(S.nslices, kSlicesPerBlock);
     if (dot_out) {
+        if (timer) timer->begin();
         k_spmv<true><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col, S.val, x, y,
                                                     ws.partials);
+        if (timer) timer->end();
         HZ_LAUNCH_CHECK();
         k_final_sum<<<1, kFinBlock, 0, stream()>>>(ws.partials, grid, dot_out);
     } else {

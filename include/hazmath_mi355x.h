@@ -318,6 +318,8 @@ typedef struct {
                               Modes 1 and 2 need coarse_scaling = OFF. */
     INT  batch_time_kernels; /* >0: the batched solver also times its level-0 SpMM and
                               sweep after the solve (hz_batch_get_stats) */
+    INT  time_pcg_spmv;      /* >0: record HIP events around each PCG SpMV of the solve
+                              (hz_solve_info.pcg_spmv_ms) */
 } hz_options;
 
 void hz_get_options(hz_options *opt);
@@ -337,6 +339,9 @@ typedef struct {
     INT    relres_count;      /* entries valid in relres_hist (<= 4096) */
     REAL   relres_hist[4096]; /* relres after each PCG iteration */
     INT    fallback_used;     /* hz_krylov_amg_learned: 1 when the plain method produced x */
+    REAL   pcg_spmv_ms;       /* with hz_options.time_pcg_spmv: mean duration of the PCG's fine-level
+                                 SpMV launches in this solve (HIP events on the library's stream) */
+    INT    pcg_spmv_count;
 } hz_solve_info;
 
 void hz_get_last_info(hz_solve_info *info);
