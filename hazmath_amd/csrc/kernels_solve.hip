@@ -181,15 +181,21 @@ __global__ __launch_bounds__(kBlock) void k_sweep(int n, int nslices, const long
         const double *v = val + base + lane;
         double t = (r < n) ? b[r] : 0.0;
         double d = 0.0;
-        constexpr int U = 8;   // entries per round: loads first (matrix non-temporal), then the math in order
+        // entries per round: loads first (matrix non-temporal), then the math in
+        // order.  With the prolongation folded in (the latency-bound small
+        // levels) rounds are 16 wide and masked past the slice width, so a
+        // 15-entry row takes one round; otherwise 8 wide plus a tail.
+        constexpr bool kMask = kProlong;
+        constexpr int U = kMask ? 16 : 8;
         int k = 0;
-        for (; k + U <= wd; k += U) {
+        for (; kMask ? (k < wd) : (k + U <= wd); k += U) {
             int cc[U];
             double vv[U], xx[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
-                cc[u] = ld_nt(c + (k + u) * kSlice);
-                vv[u] = ld_nt(v + (k + u) * kSlice);
+                const bool ok = !kMask || k + u < wd;
+                cc[u] = ok ? ld_nt(c + (k + u) * kSlice) : -1;
+                vv[u] = ok ? ld_nt(v + (k + u) * kSlice) : 0.0;
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) xx[u] = cc[u] >= 0 ? x[cc[u]] : 0.0;
@@ -318,35 +324,56 @@ __global__ void k_restrict(int nc, const int *__restrict__ rptr, const int *__re
 // one pass, one thread per coarse row: w_m = b_m - A_m x for each member m
 // (row sum from 0.0 in stored order), bc = 0.0 + w_m0 + w_m1 + ... in member
 // order, then xc from bc.  Same operations and order as residual + restrict.
+// Latency-bound small levels: kLanes lanes share a coarse row, lane j taking
+// members j, j + kLanes, ...; each member row's sum is formed in CSR order with
+// its loads issued 16 at a time (masked past the row end), and lane 0 adds the
+// members' residuals in member order -- the same operations and order as
+// residual + restrict.
+template <int kLanes>
 __global__ void k_down_fused(int nc, const int *__restrict__ rptr, const int *__restrict__ rmem,
                              const int *__restrict__ ia, const int *__restrict__ ja,
                              const double *__restrict__ a, const double *__restrict__ b,
                              const double *__restrict__ x, double *__restrict__ bc, int smoother,
                              double wgt, const double *__restrict__ dc, double *__restrict__ xc) {
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
-        double s = 0.0;
-        const int e = rptr[c + 1];
-        for (int q = rptr[c]; q < e; ++q) {
+    constexpr int U = 16;
+    const long gt = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = (int)(gt / kLanes), j = (int)(gt % kLanes);
+    const int lane = threadIdx.x % kWave, gbase = lane - j;
+    const bool valid = c < nc;
+    const int q0 = valid ? rptr[c] : 0, q1 = valid ? rptr[c + 1] : 0;
+    double s = 0.0;
+    for (int qb = q0; qb < q1; qb += kLanes) {   // trip count uniform within the lane group
+        const int q = qb + j;
+        double wv = 0.0;
+        if (q < q1) {
             const int m = rmem[q];
-            const int k1 = ia[m + 1];
+            const int k0 = ia[m], k1 = ia[m + 1];
+            const double bm = b[m];
             double t = 0.0;
-            int k = ia[m];
-            for (; k + 4 <= k1; k += 4) {
-                int jj[4];
-                double aa[4], xx[4];
+            for (int k = k0; k < k1; k += U) {
+                int jj[U];
+                double aa[U], xx[U];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    jj[u] = ja[k + u];
-                    aa[u] = a[k + u];
+                for (int u = 0; u < U; ++u) {
+                    const bool ok = k + u < k1;
+                    jj[u] = ok ? ja[k + u] : -1;
+                    aa[u] = ok ? a[k + u] : 0.0;
                 }
 #pragma unroll
-                for (int u = 0; u < 4; ++u) xx[u] = x[jj[u]];
+                for (int u = 0; u < U; ++u) xx[u] = jj[u] >= 0 ? x[jj[u]] : 0.0;
 #pragma unroll
-                for (int u = 0; u < 4; ++u) t = t + aa[u] * xx[u];
+                for (int u = 0; u < U; ++u)
+                    if (jj[u] >= 0) t = t + aa[u] * xx[u];
             }
-            for (; k < k1; ++k) t = t + a[k] * x[ja[k]];
-            s = s + (b[m] - t);
+            wv = bm - t;
         }
+#pragma unroll
+        for (int u = 0; u < kLanes; ++u) {
+            const double v = __shfl(wv, gbase + u, kWave);
+            if (j == 0 && qb + u < q1) s = s + v;
+        }
+    }
+    if (valid && j == 0) {
         bc[c] = s;
         if (xc) {
             const double di = dc[c];
@@ -649,8 +676,9 @@ void restrict_agg(int nc, const int *rptr, const int *rmem, const double *w, dou
 void down_fused(int nc, const int *rptr, const int *rmem, const DevCSR &A, const double *b,
                 const double *x, double *bc, int smoother, double wgt, const double *dc, double *xc) {
     if (nc == 0) return;
-    k_down_fused<<<stream_grid(nc), kBlock, 0, stream()>>>(nc, rptr, rmem, A.ia, A.ja, A.val, b, x, bc,
-                                                           smoother, wgt, dc, xc);
+    constexpr int kLanes = 4;
+    k_down_fused<kLanes><<<div_up((long)nc * kLanes, kBlock), kBlock, 0, stream()>>>(
+        nc, rptr, rmem, A.ia, A.ja, A.val, b, x, bc, smoother, wgt, dc, xc);
     HZ_LAUNCH_CHECK();
 }
 
