@@ -102,6 +102,24 @@ hipStream_t stream() {
 // ---------------------------------------------------------------------------
 // Setup
 // ---------------------------------------------------------------------------
+// Levels up to this many rows run HEM's sequential sweep on the host
+// (HZ_HEM_HOST_MAX overrides the default).  Measured on the 3-D Kuhn 128^3
+// hierarchy (setup ms by threshold): 0: 43.2, 16K: 35.4, 64K: 34.0, 256K: 30.8,
+// 600K: 31.9, 1.1M: 50.3 -- the GPU pipeline's dependency wavefront does not
+// shrink with the level, while the host sweep is linear in it.
+int g_hem_host_max = 0;   // hz_options.hem_host_max_rows: 0 = default, < 0 = never on the host
+
+int hem_host_max_rows() {
+    if (g_hem_host_max < 0) return -1;
+    if (g_hem_host_max > 0) return g_hem_host_max;
+    static int v = -1;
+    if (v < 0) {
+        const char *e = std::getenv("HZ_HEM_HOST_MAX");
+        v = e ? std::atoi(e) : 300000;
+    }
+    return v;
+}
+
 int amg_setup(Hierarchy &H, const DevCSR &A0, AMG_param *param, bool) {
     const short min_cdof = (short)std::max(param->coarse_dof, 1);
     int max_levels = std::max<int>(1, param->max_levels);
@@ -140,7 +158,9 @@ int amg_setup(Hierarchy &H, const DevCSR &A0, AMG_param *param, bool) {
             }
             nagg = r;
         } else if (type == HZ_HEM) {
-            st = hem_aggregate_gpu(N, L.agg, &nagg);
+            // small levels: the sequential sweep on the host beats the GPU's
+            // wavefront (whose length does not shrink with the level)
+            st = n <= hem_host_max_rows() ? 1 : hem_aggregate_gpu(N, L.agg, &nagg);
             if (st == 1) st = aggregate_host(N, type, param->max_aggregation, L.agg, &nagg);
             if (st == -1017) return -1017;
         } else {

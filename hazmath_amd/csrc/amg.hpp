@@ -95,6 +95,11 @@ double error_contraction(Hierarchy &H, int iters, unsigned long long seed);
 // Host-staged sequential aggregation for VMB (and HEM on non-symmetric patterns).
 int aggregate_host(const DevCSR &N, int type, int max_aggregation, int *agg_dev, int *nagg);
 
+// Levels up to hem_host_max_rows() rows run HEM on the host (set through
+// hz_options.hem_host_max_rows -> g_hem_host_max).
+extern int g_hem_host_max;
+int hem_host_max_rows();
+
 // HEM on the GPU: SUCCESS, -1017, or 1 when the strength pattern is not
 // symmetric (then the host sweep is used).
 int hem_aggregate_gpu(const DevCSR &N, int *agg, int *nagg_out);

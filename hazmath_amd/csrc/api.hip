@@ -10,7 +10,7 @@
 using namespace hz;
 
 namespace {
-hz_options g_opt = {1.0, 0, 0, 0, 0, 0, 0};
+hz_options g_opt = {1.0, 0, 0, 0, 0, 0, 0, 0};
 hz_solve_info g_info;
 }  // namespace
 
@@ -493,7 +493,10 @@ dvector *dvector_read_p(FILE *fp) {
 }
 
 void hz_get_options(hz_options *opt) { *opt = g_opt; }
-void hz_set_options(const hz_options *opt) { g_opt = *opt; }
+void hz_set_options(const hz_options *opt) {
+    g_opt = *opt;
+    g_hem_host_max = opt->hem_host_max_rows;
+}
 void hz_get_last_info(hz_solve_info *info) { *info = g_info; }
 
 const char *hz_version(void) { return "hazmath_mi355x 0.1 (gfx950)"; }

@@ -320,6 +320,9 @@ typedef struct {
                               sweep after the solve (hz_batch_get_stats) */
     INT  time_pcg_spmv;      /* >0: record HIP events around each PCG SpMV of the solve
                               (hz_solve_info.pcg_spmv_ms) */
+    INT  hem_host_max_rows;  /* HEM levels up to this many rows run the sequential sweep on the
+                              host, larger ones the GPU pipeline (same aggregates); 0: 300000
+                              (or the HZ_HEM_HOST_MAX environment variable), < 0: always GPU */
 } hz_options;
 
 void hz_get_options(hz_options *opt);

@@ -122,12 +122,36 @@ def _hierarchies(lib, orc, libc, A, agg_type, theta, coarse_solver=32, coarse_do
     return out
 
 
-@pytest.mark.parametrize("agg_type", [capi.HEC, capi.HEM, capi.VMB])
+class _HemPath:
+    """HEM levels on the host sweep (the default below 300,000 rows) or all on
+    the GPU pipeline (hz_options.hem_host_max_rows = -1): both must give the
+    reference's aggregates."""
+
+    def __init__(self, lib, path):
+        self.lib, self.path = lib, path
+
+    def __enter__(self):
+        opts = capi.hz_options()
+        self.lib.hz_get_options(C.byref(opts))
+        self.old = opts.hem_host_max_rows
+        opts.hem_host_max_rows = -1 if self.path == "gpu" else 0
+        self.lib.hz_set_options(C.byref(opts))
+
+    def __exit__(self, *exc):
+        opts = capi.hz_options()
+        self.lib.hz_get_options(C.byref(opts))
+        opts.hem_host_max_rows = self.old
+        self.lib.hz_set_options(C.byref(opts))
+
+
+@pytest.mark.parametrize("agg_type,hem_path", [(capi.HEC, "host"), (capi.HEM, "host"), (capi.HEM, "gpu"),
+                                               (capi.VMB, "host")])
 @pytest.mark.parametrize("theta", [0.0, 0.04, 0.25])
-def test_hierarchy_bitexact_solvers_fixture(lib, orc, libc, solvers_fixture, agg_type, theta):
+def test_hierarchy_bitexact_solvers_fixture(lib, orc, libc, solvers_fixture, agg_type, hem_path, theta):
     ia, ja, va, _ = solvers_fixture
     A = capi.HostCSR(ia, ja, va)
-    (lg, thg, rg), (lo, tho, ro) = _hierarchies(lib, orc, libc, A, agg_type, theta)
+    with _HemPath(lib, hem_path):
+        (lg, thg, rg), (lo, tho, ro) = _hierarchies(lib, orc, libc, A, agg_type, theta)
     assert len(lg) == len(lo)
     for g, o in zip(lg, lo):
         for key in ("ia", "ja", "val"):
@@ -141,11 +165,13 @@ def test_hierarchy_bitexact_solvers_fixture(lib, orc, libc, solvers_fixture, agg
 
 
 @pytest.mark.parametrize("m", [16, 24])
-@pytest.mark.parametrize("agg_type", [capi.HEC, capi.HEM, capi.VMB])
-def test_hierarchy_bitexact_kuhn3d(lib, orc, libc, m, agg_type):
+@pytest.mark.parametrize("agg_type,hem_path", [(capi.HEC, "host"), (capi.HEM, "host"), (capi.HEM, "gpu"),
+                                               (capi.VMB, "host")])
+def test_hierarchy_bitexact_kuhn3d(lib, orc, libc, m, agg_type, hem_path):
     ia, ja, va = problems.poisson3d_kuhn(m)
     A = capi.HostCSR(ia, ja, va)
-    (lg, _, rg), (lo, _, ro) = _hierarchies(lib, orc, libc, A, agg_type, 0.04, coarse_dof=50)
+    with _HemPath(lib, hem_path):
+        (lg, _, rg), (lo, _, ro) = _hierarchies(lib, orc, libc, A, agg_type, 0.04, coarse_dof=50)
     assert len(lg) == len(lo)
     for g, o in zip(lg, lo):
         for key in ("ia", "ja", "val"):
