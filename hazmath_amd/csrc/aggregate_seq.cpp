@@ -6,6 +6,7 @@
 // the GPU, copied to the host, swept once, and the aggregate map copied back.
 // HEC, the reference default, is resolved fully on the GPU instead
 // (kernels_setup.hip).
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -15,8 +16,7 @@ namespace hz {
 
 namespace {
 
-int vmb(int n, const std::vector<int> &ia, const std::vector<int> &ja, int maxagg, std::vector<int> &v,
-        int &nagg) {
+int vmb(int n, const int *ia, const int *ja, int maxagg, std::vector<int> &v, int &nagg) {
     int left = n, na = 0;
     v.assign(n, -2);
     for (int i = n - 1; i >= 0; --i) {
@@ -91,8 +91,7 @@ int heaviest(const std::vector<double> &w, const std::vector<int> &id, int n0, i
     return best;
 }
 
-int hem(int n, const std::vector<int> &ia, const std::vector<int> &ja, const std::vector<double> &a,
-        std::vector<int> &v, int &nagg) {
+int hem(int n, const int *ia, const int *ja, const double *a, std::vector<int> &v, int &nagg) {
     v.assign(n, -1);
     int maxdeg = 0;
     for (int k = 0; k < n; ++k) {
@@ -150,21 +149,45 @@ int hem(int n, const std::vector<int> &ia, const std::vector<int> &ja, const std
 
 }  // namespace
 
+namespace {
+// Grow-only pinned staging buffers, kept for the process: copies through
+// pinned memory run at full link speed (pageable copies are staged twice).
+struct PinnedStage {
+    void *p = nullptr;
+    size_t cap = 0;
+    void *get(size_t bytes) {
+        bytes = std::max<size_t>(bytes, 256);
+        if (bytes > cap) {
+            if (p) (void)hipHostFree(p);
+            p = nullptr;
+            HZ_CHECK(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+            cap = bytes;
+        }
+        return p;
+    }
+};
+PinnedStage g_ia, g_ja, g_a, g_v;
+}  // namespace
+
 int aggregate_host(const DevCSR &N, int type, int max_aggregation, int *agg_dev, int *nagg) {
     const int n = N.n;
-    std::vector<int> ia(n + 1), ja(N.nnz);
-    std::vector<double> a(N.nnz);
+    int *ia = static_cast<int *>(g_ia.get(sizeof(int) * ((size_t)n + 1)));
+    int *ja = static_cast<int *>(g_ja.get(sizeof(int) * (size_t)N.nnz));
+    double *a = static_cast<double *>(g_a.get(sizeof(double) * (size_t)N.nnz));
     hipStream_t st = stream();
-    HZ_CHECK(hipMemcpyAsync(ia.data(), N.ia, sizeof(int) * (n + 1), hipMemcpyDeviceToHost, st));
+    HZ_CHECK(hipMemcpyAsync(ia, N.ia, sizeof(int) * (n + 1), hipMemcpyDeviceToHost, st));
     if (N.nnz) {
-        HZ_CHECK(hipMemcpyAsync(ja.data(), N.ja, sizeof(int) * N.nnz, hipMemcpyDeviceToHost, st));
-        HZ_CHECK(hipMemcpyAsync(a.data(), N.val, sizeof(double) * N.nnz, hipMemcpyDeviceToHost, st));
+        HZ_CHECK(hipMemcpyAsync(ja, N.ja, sizeof(int) * N.nnz, hipMemcpyDeviceToHost, st));
+        if (type != HZ_VMB)
+            HZ_CHECK(hipMemcpyAsync(a, N.val, sizeof(double) * N.nnz, hipMemcpyDeviceToHost, st));
     }
     HZ_CHECK(hipStreamSynchronize(st));
     std::vector<int> v;
     int st_code = (type == HZ_VMB) ? vmb(n, ia, ja, max_aggregation, v, *nagg) : hem(n, ia, ja, a, v, *nagg);
     if (st_code == -1017) return st_code;
-    HZ_CHECK(hipMemcpyAsync(agg_dev, v.data(), sizeof(int) * n, hipMemcpyHostToDevice, st));
+    int *pv = static_cast<int *>(g_v.get(sizeof(int) * (size_t)n));
+    std::copy(v.begin(), v.end(), pv);
+    HZ_CHECK(hipMemcpyAsync(agg_dev, pv, sizeof(int) * n, hipMemcpyHostToDevice, st));
     HZ_CHECK(hipStreamSynchronize(st));
     return st_code;
 }

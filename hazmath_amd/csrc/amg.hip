@@ -104,8 +104,8 @@ hipStream_t stream() {
 // ---------------------------------------------------------------------------
 // Levels up to this many rows run HEM's sequential sweep on the host
 // (HZ_HEM_HOST_MAX overrides the default).  Measured on the 3-D Kuhn 128^3
-// hierarchy (setup ms by threshold): 0: 43.2, 16K: 35.4, 64K: 34.0, 256K: 30.8,
-// 600K: 31.9, 1.1M: 50.3 -- the GPU pipeline's dependency wavefront does not
+// hierarchy with pinned staging (setup ms by threshold): 0: 43.2, 300K: 29.5,
+// 600K: 27.9, 1.1M: 31.3 -- the GPU pipeline's dependency wavefront does not
 // shrink with the level, while the host sweep is linear in it.
 int g_hem_host_max = 0;   // hz_options.hem_host_max_rows: 0 = default, < 0 = never on the host
 
@@ -115,7 +115,7 @@ int hem_host_max_rows() {
     static int v = -1;
     if (v < 0) {
         const char *e = std::getenv("HZ_HEM_HOST_MAX");
-        v = e ? std::atoi(e) : 300000;
+        v = e ? std::atoi(e) : 600000;
     }
     return v;
 }

@@ -321,7 +321,7 @@ typedef struct {
     INT  time_pcg_spmv;      /* >0: record HIP events around each PCG SpMV of the solve
                               (hz_solve_info.pcg_spmv_ms) */
     INT  hem_host_max_rows;  /* HEM levels up to this many rows run the sequential sweep on the
-                              host, larger ones the GPU pipeline (same aggregates); 0: 300000
+                              host, larger ones the GPU pipeline (same aggregates); 0: 600000
                               (or the HZ_HEM_HOST_MAX environment variable), < 0: always GPU */
 } hz_options;
 

@@ -123,7 +123,7 @@ def _hierarchies(lib, orc, libc, A, agg_type, theta, coarse_solver=32, coarse_do
 
 
 class _HemPath:
-    """HEM levels on the host sweep (the default below 300,000 rows) or all on
+    """HEM levels on the host sweep (the default up to 600,000 rows) or all on
     the GPU pipeline (hz_options.hem_host_max_rows = -1): both must give the
     reference's aggregates."""
 
