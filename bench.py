@@ -309,7 +309,8 @@ def main_batched(a, lib):
             "avg_launch_ms": ks.spmm_ms,
         },
         "kernels": {"spmm_ms": ks.spmm_ms, "sweep_f32_ms": ks.sweep_ms, "sweep_f32_gbs": sweep_gbs,
-                    "lanes_per_row_level0": ks.shape_level0, "levels": ks.num_levels,
+                    "lanes_per_row_sweep_level0": ks.shape_level0 // 10,
+                    "lanes_per_row_spmm_level0": ks.shape_level0 % 10, "levels": ks.num_levels,
                     "op_complexity": ks.op_complexity},
         "cpu_baseline": cpu,
     }

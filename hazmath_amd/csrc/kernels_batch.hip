@@ -196,7 +196,9 @@ template <int kH>
 struct LaneMap {
     int s, r, q0;
     __device__ __forceinline__ LaneMap() {
-        const int wave = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+        // XCD-aware: each XCD gets a contiguous range of slices, so the
+        // neighbouring rows a slice gathers stay in that XCD's L2
+        const int wave = xcd_block_id() * (blockDim.x / kWave) + threadIdx.x / kWave;
         s = wave / kH;
         r = s * kSlice + threadIdx.x % kWave;
         q0 = (wave % kH) * (kR / kH);
@@ -421,11 +423,50 @@ __global__ __launch_bounds__(256) void k_bcoarse(const long long *__restrict__ t
     }
 }
 
-__global__ void k_transpose_sq(int nb, const double *__restrict__ M, double *__restrict__ T) {
+// Explicit inverses of many small dense matrices at once: workgroup k inverts
+// block k (size tab[4k+1], stored row-major at W + woff[k]) in place by
+// Gauss-Jordan elimination without pivoting (the coarsest Galerkin matrices are
+// SPD), the elimination of the single-system coarsest inverse applied to the
+// whole block.  Writes the transpose of the inverse to T + tab[4k+2].  A zero
+// pivot sets status[k] = 1 and leaves the block.
+__global__ __launch_bounds__(1024) void k_batch_gj_inverse(const long long *__restrict__ tab,
+                                                           const long long *__restrict__ woff,
+                                                           double *__restrict__ W, double *__restrict__ T,
+                                                           int *__restrict__ status) {
+    __shared__ double colk[kCoarseMax];
+    __shared__ double rowk[kCoarseMax];
+    const int blk = blockIdx.x;
+    const int nb = (int)tab[4 * blk + 1];
+    double *M = W + woff[blk];
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int k = 0; k < nb; ++k) {
+        const double piv = M[(size_t)k * nb + k];
+        if (piv == 0.0) {
+            if (tid == 0) status[blk] = 1;
+            return;   // uniform: every thread read the same pivot
+        }
+        for (int i = tid; i < nb; i += nt) colk[i] = M[(size_t)i * nb + k];
+        __syncthreads();
+        for (int j = tid; j < nb; j += nt) {
+            const double v = (j == k) ? 1.0 / piv : M[(size_t)k * nb + j] / piv;
+            rowk[j] = v;
+            M[(size_t)k * nb + j] = v;
+        }
+        __syncthreads();
+        const long total = (long)nb * nb;
+        for (long e = tid; e < total; e += nt) {
+            const int i = (int)(e / nb), j = (int)(e % nb);
+            if (i == k) continue;
+            if (j != k) M[e] -= colk[i] * rowk[j];
+            else M[e] = -colk[i] * rowk[k];
+        }
+        __syncthreads();
+    }
+    double *Tb = T + tab[4 * blk + 2];
     const long total = (long)nb * nb;
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+    for (long e = tid; e < total; e += nt) {
         const long i = e / nb, j = e % nb;
-        T[j * nb + i] = M[e];
+        Tb[j * nb + i] = M[e];
     }
 }
 
@@ -462,15 +503,16 @@ __global__ void k_exit_b(int n, const float *__restrict__ x, double *__restrict_
 // ---------------------------------------------------------------------------
 // y = A x (kSub = 1: y = A x; kSub = 2: y = b - A x); two lanes per row
 // (four right-hand sides each)
-template <int kSub>
+template <int kSub, int kH>
 __global__ __launch_bounds__(kBlock) void k_spmm64(int n, int nslices, const long long *__restrict__ sptr,
                                                    const int *__restrict__ col, const double *__restrict__ val,
                                                    const double *__restrict__ x, const double *__restrict__ b,
                                                    double *__restrict__ y, const int *__restrict__ gate,
                                                    const int *__restrict__ rsys, const int *__restrict__ act) {
     if (gate && *gate == 0) return;
-    constexpr int kN = kR / 2;
-    const LaneMap<2> m;
+    constexpr int kN = kR / kH;      // right-hand sides per lane
+    constexpr int kV = kN / 2;       // double2 loads per gathered row
+    const LaneMap<kH> m;
     if (m.s >= nslices || m.r >= n) return;
     if (skip_row(rsys, act, m.r)) return;
     double acc[kN];
@@ -481,11 +523,18 @@ __global__ __launch_bounds__(kBlock) void k_spmm64(int n, int nslices, const lon
     const int lane = threadIdx.x % kWave;
     const int *c = col + base + lane;
     const double *v = val + base + lane;
+    auto madd = [&](double a, const double2 *xv) {
+#pragma unroll
+        for (int h = 0; h < kV; ++h) {
+            acc[2 * h] += a * xv[h].x;
+            acc[2 * h + 1] += a * xv[h].y;
+        }
+    };
     int k = 0;
     for (; k + 4 <= wd; k += 4) {
         int cc[4];
         double aa[4];
-        double2 x0[4], x1[4];
+        double2 xx[4][kV];
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             cc[u] = ld_nt(c + (long long)(k + u) * kSlice);
@@ -495,29 +544,23 @@ __global__ __launch_bounds__(kBlock) void k_spmm64(int n, int nslices, const lon
         for (int u = 0; u < 4; ++u) {
             if (cc[u] >= 0) {
                 const double2 *xp = reinterpret_cast<const double2 *>(x + (size_t)cc[u] * kR + m.q0);
-                x0[u] = xp[0];
-                x1[u] = xp[1];
+#pragma unroll
+                for (int h = 0; h < kV; ++h) xx[u][h] = xp[h];
             }
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            if (cc[u] < 0) continue;
-            acc[0] += aa[u] * x0[u].x;
-            acc[1] += aa[u] * x0[u].y;
-            acc[2] += aa[u] * x1[u].x;
-            acc[3] += aa[u] * x1[u].y;
-        }
+        for (int u = 0; u < 4; ++u)
+            if (cc[u] >= 0) madd(aa[u], xx[u]);
     }
     for (; k < wd; ++k) {
         const int cj = ld_nt(c + (long long)k * kSlice);
         if (cj < 0) continue;
         const double a = ld_nt(v + (long long)k * kSlice);
         const double2 *xp = reinterpret_cast<const double2 *>(x + (size_t)cj * kR + m.q0);
-        const double2 x0 = xp[0], x1 = xp[1];
-        acc[0] += a * x0.x;
-        acc[1] += a * x0.y;
-        acc[2] += a * x1.x;
-        acc[3] += a * x1.y;
+        double2 xv[kV];
+#pragma unroll
+        for (int h = 0; h < kV; ++h) xv[h] = xp[h];
+        madd(a, xv);
     }
     const size_t o = (size_t)m.r * kR + m.q0;
 #pragma unroll
@@ -703,10 +746,11 @@ __global__ __launch_bounds__(1024) void k_cfinal(int ncols, const double *__rest
                     cs.restart = 0;
                 }
             }
-            open = !cs.done;
+            open = kKind == 2 ? cs.cand : !cs.done;
         }
-        if (kKind == 0 || kKind == 3) {
-            // a system stays active while any of its kR columns iterates
+        if (kKind == 0 || kKind == 2 || kKind == 3) {
+            // per system: active while any of its kR columns iterates (kinds 0, 3),
+            // or holding a check-III candidate (kind 2)
 #pragma unroll
             for (int o = 1; o < kR; o <<= 1) open |= __shfl_xor(open, o, kWave);
             if (valid && col % kR == 0) sysact[col / kR] = open;
@@ -765,6 +809,7 @@ struct BatchLevel {
     double *Ainv = nullptr;
     float *b = nullptr, *x = nullptr, *xt = nullptr, *w = nullptr;
     int shape = 1;                   // lanes per row chosen by the tuner (1 or 2)
+    int spmm_shape = 2;              // level 0: lanes per row of the FP64 PCG SpMM (tuned)
 };
 
 struct BatchHierarchy {
@@ -918,8 +963,19 @@ int batch_setup(BatchHierarchy &H, const DevCSR &A0, const std::vector<int> &off
             strength_graph_rows(L.A, dth2, N, counts);
             L.agg = arena().alloc_n<int>((size_t)n + 1);
             int nagg = 0;
-            int sth = hem_aggregate_gpu(N, L.agg, &nagg);
+            const bool tr = trace_enabled();
+            double t0 = 0;
+            if (tr) {
+                HZ_CHECK(hipStreamSynchronize(st));
+                t0 = wall_ms();
+            }
+            int sth = n <= hem_host_max_rows() ? 1 : hem_aggregate_gpu(N, L.agg, &nagg);
             if (sth == 1) sth = aggregate_host(N, HZ_HEM, param->max_aggregation, L.agg, &nagg);
+            if (tr) {
+                HZ_CHECK(hipStreamSynchronize(st));
+                std::fprintf(stderr, "[hz-batch] level %d n=%d nnz=%d blocks=%d: aggregate %.2f ms\n", lvl, n,
+                             L.A.nnz, nblk, wall_ms() - t0);
+            }
             if (sth == -1017) return -1017;
             int *damin = arena().alloc_n<int>((size_t)nblk + 1);
             int *damax = arena().alloc_n<int>((size_t)nblk + 1);
@@ -962,16 +1018,26 @@ int batch_setup(BatchHierarchy &H, const DevCSR &A0, const std::vector<int> &off
             L.Ainv = arena().alloc_n<double>((size_t)inv_total + 1);
             L.ctab = arena().alloc_n<long long>(ctab.size() + 1);
             HZ_CHECK(memcpy_sync(L.ctab, ctab.data(), sizeof(long long) * ctab.size(), hipMemcpyHostToDevice));
+            // dense copies of the blocks (work area at the inverses' offsets),
+            // then all inverses in one launch
+            double *W = arena().alloc_n<double>((size_t)inv_total + 1);
+            std::vector<long long> woff(L.ncoarse);
             for (int c = 0; c < L.ncoarse; ++c) {
                 const int r0 = (int)ctab[4 * c], nb = (int)ctab[4 * c + 1];
-                double *M = L.Ainv + ctab[4 * c + 2];
-                double *W = arena().alloc_n<double>((size_t)nb * nb + 1);
-                k_dense_block<<<div_up(nb, 64), 64, 0, st>>>(L.A.ia, L.A.ja, L.A.val, r0, nb, W);
-                HZ_LAUNCH_CHECK();
-                if (dense_inverse_inplace(nb, W) != 0) return HZ_ERROR_SOLVER_MISC;
-                k_transpose_sq<<<grid_for((long)nb * nb), kBlock, 0, st>>>(nb, W, M);
-                HZ_LAUNCH_CHECK();
+                woff[c] = ctab[4 * c + 2];
+                k_dense_block<<<div_up(nb, 64), 64, 0, st>>>(L.A.ia, L.A.ja, L.A.val, r0, nb, W + woff[c]);
             }
+            HZ_LAUNCH_CHECK();
+            long long *dwoff = arena().alloc_n<long long>((size_t)L.ncoarse + 1);
+            int *dstat = arena().alloc_n<int>((size_t)L.ncoarse + 1);
+            HZ_CHECK(memcpy_sync(dwoff, woff.data(), sizeof(long long) * L.ncoarse, hipMemcpyHostToDevice));
+            HZ_CHECK(hipMemsetAsync(dstat, 0, sizeof(int) * L.ncoarse, st));
+            k_batch_gj_inverse<<<L.ncoarse, 1024, 0, st>>>(L.ctab, dwoff, W, L.Ainv, dstat);
+            HZ_LAUNCH_CHECK();
+            std::vector<int> hstat(L.ncoarse);
+            HZ_CHECK(memcpy_sync(hstat.data(), dstat, sizeof(int) * L.ncoarse, hipMemcpyDeviceToHost));
+            for (int v : hstat)
+                if (v) return HZ_ERROR_SOLVER_MISC;
         }
         // live rows, renumbered aggregates, next level
         L.live = arena().alloc_n<unsigned char>((size_t)n + 1);
@@ -1083,6 +1149,7 @@ struct BatchPcg {
     const double *b = nullptr;
     ColState *cs = nullptr;
     PcgCtl *ctl = nullptr;
+    int *scand = nullptr;   // per system: some column is a check-III candidate this iteration
     hipGraphExec_t exec = nullptr;
 };
 
@@ -1115,20 +1182,55 @@ void pcg_alloc(BatchPcg &P, const BatchHierarchy &H, const std::vector<int> &off
     P.p = arena().alloc_n<double>(nv);
     P.t = arena().alloc_n<double>(nv);
     P.cs = arena().alloc_n<ColState>((size_t)P.ncols + 1);
+    P.scand = arena().alloc_n<int>((size_t)nsys + 1);
     P.ctl = arena().alloc_n<PcgCtl>(1);
 }
 
+template <int kSub, int kH>
+void launch_spmm(const BatchLevel &L0, const double *x, const double *b, double *y, const int *gate,
+                 const int *act) {
+    const int grid = std::max(1, div_up(L0.S.nslices * kH, kSlicesPerBlock));
+    k_spmm64<kSub, kH><<<grid, kBlock, 0, stream()>>>(L0.S.n, L0.S.nslices, L0.S.sptr, L0.S.col, L0.S.val, x, b,
+                                                      y, gate, L0.rsys, act);
+}
+
+// y = A x (sub = 0) or y = b - A x (sub = 1) on level 0 in FP64, in the
+// lanes-per-row shape the tuner chose for this matrix
 void spmm(const BatchLevel &L0, const double *x, const double *b, double *y, int sub, const int *gate,
           const int *act) {
-    const int waves = L0.S.nslices * 2;
-    const int grid = std::max(1, div_up(waves, kSlicesPerBlock));
-    if (sub)
-        k_spmm64<2><<<grid, kBlock, 0, stream()>>>(L0.S.n, L0.S.nslices, L0.S.sptr, L0.S.col, L0.S.val, x, b, y,
-                                                   gate, L0.rsys, act);
-    else
-        k_spmm64<1><<<grid, kBlock, 0, stream()>>>(L0.S.n, L0.S.nslices, L0.S.sptr, L0.S.col, L0.S.val, x, b, y,
-                                                   gate, L0.rsys, act);
+    if (L0.spmm_shape == 1) {
+        if (sub) launch_spmm<2, 1>(L0, x, b, y, gate, act);
+        else launch_spmm<1, 1>(L0, x, b, y, gate, act);
+    } else {
+        if (sub) launch_spmm<2, 2>(L0, x, b, y, gate, act);
+        else launch_spmm<1, 2>(L0, x, b, y, gate, act);
+    }
     HZ_LAUNCH_CHECK();
+}
+
+// The tuner's choice of the FP64 SpMM shape for this level-0 matrix: both
+// shapes timed (HIP events) on x = p, cached per (rows, entries per row) class
+// like the cycle kernels' (HZ_BATCH_SPMM_SHAPE overrides).
+int tune_spmm_shape(BatchLevel &L0, const double *x, double *y) {
+    const char *env = std::getenv("HZ_BATCH_SPMM_SHAPE");
+    if (env && (env[0] == '1' || env[0] == '2')) return env[0] - '0';
+    static std::map<std::pair<int, int>, int> cache;
+    const auto key = std::make_pair((int)std::lround(std::log2(std::max(1, L0.S.n))),
+                                    L0.S.n ? (int)std::lround(4.0 * (double)L0.A.nnz / L0.S.n) : 0);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int best = 2;
+    float tbest = 1e30f;
+    for (int shape : {1, 2}) {
+        L0.spmm_shape = shape;
+        const float t = time_ms([&] { spmm(L0, x, nullptr, y, 0, nullptr, nullptr); }, 3);
+        if (t < tbest) {
+            tbest = t;
+            best = shape;
+        }
+    }
+    cache[key] = best;
+    return best;
 }
 
 // one PCG iteration (graph body)
@@ -1142,9 +1244,10 @@ void pcg_iteration(BatchHierarchy &H, BatchPcg &P, double *u) {
     k_cdot<<<nch, kBlock, 0, st>>>(P.crow, P.p, P.t, P.part, P.csys, act);                   // (t, p)
     k_cfinal<1><<<1, 1024, 0, st>>>(nc, P.part, P.schunk, P.cs, P.ctl, sact);              // alpha
     k_cupdate<<<nch, kBlock, 0, st>>>(P.crow, P.csys, P.cs, P.p, P.t, u, P.r, P.part, act);  // u, r, ||r||
-    k_cfinal<2><<<1, 1024, 0, st>>>(nc, P.part, P.schunk, P.cs, P.ctl, sact);              // relres, candidates
-    spmm(L0, u, P.b, P.t, 1, &P.ctl->anycand, act);                                         // check III: b - A u
-    k_creplace<<<nch, kBlock, 0, st>>>(P.crow, P.csys, P.cs, P.t, P.r, P.part, &P.ctl->anycand, act);
+    k_cfinal<2><<<1, 1024, 0, st>>>(nc, P.part, P.schunk, P.cs, P.ctl, P.scand);           // relres, candidates
+    // check III: b - A u, only on the systems holding a candidate column
+    spmm(L0, u, P.b, P.t, 1, &P.ctl->anycand, P.scand);
+    k_creplace<<<nch, kBlock, 0, st>>>(P.crow, P.csys, P.cs, P.t, P.r, P.part, &P.ctl->anycand, P.scand);
     k_cfinal<3><<<1, 1024, 0, st>>>(nc, P.part, P.schunk, P.cs, P.ctl, sact);              // check III, maxit
     HZ_LAUNCH_CHECK();
     batch_vcycle(H, P.r, P.z);                                                               // z = B r
@@ -1179,6 +1282,7 @@ int batch_solve(const DevCSR &A0, const std::vector<int> &off0, const double *b,
     // (every system active here: H.act was set to 1 at setup)
     spmm(L0, u, b, P.r, 1, nullptr, nullptr);
     k_fill_d<<<grid_for(tot0), kBlock, 0, st>>>(tot0 + kR, 0.0, P.p);
+    H.lv[0].spmm_shape = tune_spmm_shape(H.lv[0], P.p, P.t);
     batch_vcycle(H, P.r, P.z);
     k_cdot<<<nch, kBlock, 0, st>>>(P.crow, P.r, P.r, P.part, P.csys, nullptr);
     k_cfinal<0><<<1, 1024, 0, st>>>(nc, P.part, P.schunk, P.cs, P.ctl, H.act);
@@ -1228,7 +1332,7 @@ int batch_solve(const DevCSR &A0, const std::vector<int> &off0, const double *b,
         long long nnz = 0;
         for (const auto &L : H.lv) nnz += L.A.nnz;
         stats->op_complexity = L0.A.nnz ? (double)nnz / L0.A.nnz : 0.0;
-        stats->shape_level0 = L0.shape;
+        stats->shape_level0 = L0.shape * 10 + L0.spmm_shape;
     }
     return HZ_SUCCESS;
 }

@@ -486,7 +486,8 @@ INT hz_sa_prolongation(dCSRmat *A, const INT *aggregates, INT num_aggregates, RE
 /* Measurements of the last batched call (no counterpart in the reference). */
 typedef struct hz_batch_stats {
     double setup_ms, solve_ms, op_complexity;
-    INT num_levels, graph_launches, shape_level0;
+    INT num_levels, graph_launches, shape_level0;   /* shape_level0 = 10 x (lanes per row of the FP32
+                                                       sweep) + (lanes per row of the FP64 SpMM) */
     double spmm_ms, sweep_ms;      /* level-0 FP64 SpMM and FP32 sweep, when timed */
     long long rows, nnz, sell_entries;
 } hz_batch_stats;
