@@ -84,6 +84,9 @@ def parse():
     p.add_argument("--batch-cpu-pairs", type=int, default=24,
                    help="(system, RHS) pairs solved by the serial oracle for the CPU baseline")
     p.add_argument("--batch-cpu-procs", type=int, default=8)
+    p.add_argument("--learned", action="store_true",
+                   help="the learned-parameter evaluation: 3-D Kuhn 128^3 anisotropic (eps=1e-3, seeded theta), "
+                        "learned vs plain AMG-PCG")
     p.add_argument("--cpu-sample-m", type=int, default=128,
                    help="cells per direction of the CPU-baseline sample")
     return p.parse_args()
@@ -317,6 +320,102 @@ def main_batched(a, lib):
     print(json.dumps(line))
 
 
+def main_learned(a, lib):
+    """Learned per-aggregate SA parameters against the plain drop-in on the
+    3-D Kuhn P1 anisotropic problem (K = R_z(theta) diag(1, eps, eps) R_z^T,
+    eps = 1e-3, theta = U[0, pi) seed 5), m^3 cells, seeded RHS, tol 1e-10."""
+    from hazmath_amd import learned
+    theta_dir = float(np.random.default_rng(5).uniform(0.0, np.pi))
+    eps = 1e-3
+    ia, ja, va = problems.aniso3d_kuhn(a.m, theta_dir, eps)
+    n = ia.size - 1
+    b = problems.uniform_rhs(n, seed=1)
+    weights = learned.load_model()
+    A = capi.HostCSR(ia, ja, va)
+    libc = C.CDLL(None)
+    import scipy.sparse as sp
+    Asp = sp.csr_matrix((va, ja, ia), shape=(n, n))
+    devnull = os.open(os.devnull, os.O_WRONLY)
+    saved = os.dup(1)
+
+    def quiet(fn):
+        sys.stdout.flush()
+        os.dup2(devnull, 1)
+        try:
+            return fn()
+        finally:
+            os.dup2(saved, 1)
+
+    def plain_params():
+        it, amg = capi.solver_params(lib, tol=a.tol, maxit=1000, aggregation=capi.HEM,
+                                     smoother=capi.SMOOTHER_L1DIAG, coarse_dof=200,
+                                     coarse_solver=capi.SOLVER_UMFPACK, max_levels=20, theta=0.04)
+        return it, amg
+
+    def run(kind):
+        x = capi.HostVec(np.zeros(n))
+        it, amg = plain_params()
+        libc.srand(1)
+        if kind == "plain":
+            r = lib.linear_solver_dcsr_krylov_amg(A.ptr(), capi.HostVec(b).ptr(), x.ptr(), C.byref(it), C.byref(amg))
+        elif kind == "sa":
+            amg.AMG_type = capi.SA_AMG
+            amg.aggregation_type = capi.VMB
+            amg.smoother = capi.SMOOTHER_CHEBY
+            amg.presmooth_iter = amg.postsmooth_iter = 2
+            r = lib.linear_solver_dcsr_krylov_amg(A.ptr(), capi.HostVec(b).ptr(), x.ptr(), C.byref(it), C.byref(amg))
+        else:
+            r = lib.hz_krylov_amg_learned(A.ptr(), capi.HostVec(b).ptr(), x.ptr(), C.byref(it), C.byref(amg),
+                                          capi.fptr(weights), 400, 2)
+        info = capi.hz_solve_info()
+        lib.hz_get_last_info(C.byref(info))
+        rel = float(np.linalg.norm(b - Asp @ x.arr) / np.linalg.norm(b))
+        out = dict(iterations=int(r), setup_ms=info.setup_ms, solve_ms=info.solve_ms, total_ms=info.total_ms,
+                   fallback=int(info.fallback_used) if kind == "learned" else None, true_rel_residual=rel)
+        if kind == "learned":
+            # the whole call: the learned attempt plus, when it fell back, the plain solve
+            out["learned_attempt"] = dict(iterations=int(info.learned_iterations),
+                                          total_ms=info.learned_total_ms)
+            if info.fallback_used:
+                out["total_ms"] = info.learned_total_ms + info.total_ms
+        return out
+
+    res = {}
+    for kind in ("plain", "sa", "learned"):
+        for w in range(a.warmup):
+            quiet(lambda: run(kind))
+        runs = [quiet(lambda: run(kind)) for _ in range(a.steps)]
+        best = dict(runs[-1])
+        best["total_ms"] = float(np.mean([r_["total_ms"] for r_ in runs]))
+        best["setup_ms"] = float(np.mean([r_["setup_ms"] for r_ in runs]))
+        best["solve_ms"] = float(np.mean([r_["solve_ms"] for r_ in runs]))
+        res[kind] = best
+        print(f"[bench] {kind}: {best}", file=sys.stderr, flush=True)
+    line = {
+        "metric": "learned vs plain AMG-PCG time-to-1e-10 (s), 3-D Kuhn P1 anisotropic eps=1e-3",
+        "value": res["learned"]["total_ms"] * 1e-3,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": res["learned"]["total_ms"],
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 (learned network f32)",
+        "data": "synthetic",
+        "config": {"workload": (f"3-D Kuhn P1 -div(K grad u), K = R_z(theta) diag(1, eps, eps) R_z^T, eps={eps}, "
+                                f"theta={theta_dir:.4f}, {a.m}^3 cells, n={n}, seeded RHS, PCG to {a.tol:g}; "
+                                "plain = UA HEM L1 V(1,1) drop-in; sa = SA on VMB with Chebyshev(2), constant "
+                                "parameters; learned = the same with per-aggregate damping and bound scale from "
+                                "the committed network, plain method as fallback")},
+        "plain": res["plain"], "sa_constant": res["sa"], "learned": res["learned"],
+        "speedup_learned_vs_plain": res["plain"]["total_ms"] / res["learned"]["total_ms"],
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -330,6 +429,12 @@ def main():
             print("bench.py: no HIP device", file=sys.stderr)
             sys.exit(1)
         return main_batched(a, lib)
+    if a.learned:
+        lib = capi.load_library()
+        if lib.hz_device_count() < 1:
+            print("bench.py: no HIP device", file=sys.stderr)
+            sys.exit(1)
+        return main_learned(a, lib)
     import torch
     lib = capi.load_library()
     if lib.hz_device_count() < 1 or not torch.cuda.is_available():

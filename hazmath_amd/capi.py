@@ -98,7 +98,7 @@ class hz_solve_info(C.Structure):
                 ("setup_ms", REAL), ("solve_ms", REAL), ("total_ms", REAL),
                 ("final_relres", REAL), ("condest", REAL), ("relres_count", INT),
                 ("relres_hist", REAL * 4096), ("fallback_used", INT), ("pcg_spmv_ms", REAL),
-                ("pcg_spmv_count", INT)]
+                ("pcg_spmv_count", INT), ("learned_iterations", INT), ("learned_total_ms", REAL)]
 
 
 class hz_hierarchy(C.Structure):

@@ -1029,6 +1029,7 @@ INT hz_krylov_amg_learned(dCSRmat *A, dvector *b, dvector *x, linear_itsolver_pa
             ok = std::sqrt(rr) <= itparam->linear_tol * std::max(std::sqrt(bb), 1e-300) * 1.0000001;
         }
         const hz_solve_info learned_info = g_info;
+        const INT learned_st = st;
         if (!ok) {
             HZ_CHECK(hipMemcpyAsync(dx, dx0, sizeof(double) * n, hipMemcpyDeviceToDevice, stream()));
             AMG_param pp = *amgparam;
@@ -1038,6 +1039,8 @@ INT hz_krylov_amg_learned(dCSRmat *A, dvector *b, dvector *x, linear_itsolver_pa
             g_info = learned_info;
             g_info.fallback_used = 0;
         }
+        g_info.learned_iterations = learned_st;
+        g_info.learned_total_ms = learned_info.total_ms;
         HZ_CHECK(memcpy_sync(x->val, dx, sizeof(double) * n, hipMemcpyDeviceToHost));
         return st;
     } catch (const DeviceError &) {

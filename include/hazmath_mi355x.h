@@ -345,6 +345,9 @@ typedef struct {
     REAL   pcg_spmv_ms;       /* with hz_options.time_pcg_spmv: mean duration of the PCG's fine-level
                                  SpMV launches in this solve (HIP events on the library's stream) */
     INT    pcg_spmv_count;
+    INT    learned_iterations;  /* hz_krylov_amg_learned: the learned attempt's result (iterations or
+                                   error code) and its device time, also when it fell back */
+    REAL   learned_total_ms;
 } hz_solve_info;
 
 void hz_get_last_info(hz_solve_info *info);
