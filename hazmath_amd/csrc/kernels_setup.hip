@@ -179,8 +179,12 @@ __global__ void k_agg_count(int n, const int *__restrict__ agg, int *__restrict_
 // then one slot per other column in order of first appearance, whose value is
 // the in-order sum of its candidates starting from the first one.
 // ---------------------------------------------------------------------------
-constexpr int kRapCap = 512;    // candidates held in LDS per row (larger rows use global scratch)
-constexpr int kRapHash = 1024;  // LDS hash slots (>= 2 * kRapCap)
+// Candidates held in LDS per row: rows up to kRapSmall candidates run in a
+// small-LDS kernel (high occupancy; HEM pairs of 15-point rows have ~30), rows
+// up to kRapCap in a large-LDS one, longer rows in global scratch.  The hash
+// table has twice the slots of the candidate capacity.
+constexpr int kRapSmall = 64;
+constexpr int kRapCap = 512;
 
 __device__ __forceinline__ unsigned rap_hash(int key, unsigned mask) {
     unsigned h = (unsigned)key * 2654435761u;
@@ -194,7 +198,7 @@ __device__ __forceinline__ unsigned rap_find(const int *keys, int key, unsigned 
     return slot;
 }
 
-template <bool kFill, bool kGlobal>
+template <bool kFill, bool kGlobal, int kCap>
 __global__ __launch_bounds__(kWave) void k_rap_row(
     int nrows, const int *__restrict__ rowlist, const int *__restrict__ rptr,
     const int *__restrict__ rmem, const int *__restrict__ aia, const int *__restrict__ aja,
@@ -203,10 +207,11 @@ __global__ __launch_bounds__(kWave) void k_rap_row(
     double *__restrict__ cval, int *__restrict__ gcol, double *__restrict__ gval,
     const long long *__restrict__ goff, int *__restrict__ gkeys, int *__restrict__ gminp,
     const long long *__restrict__ hoff) {
-    __shared__ int s_col[kGlobal ? 1 : kRapCap];
-    __shared__ double s_val[(kGlobal || !kFill) ? 1 : kRapCap];
-    __shared__ int s_keys[kGlobal ? 1 : kRapHash];
-    __shared__ int s_minp[kGlobal ? 1 : kRapHash];
+    constexpr int kHash = 2 * kCap;
+    __shared__ int s_col[kGlobal ? 1 : kCap];
+    __shared__ double s_val[(kGlobal || !kFill) ? 1 : kCap];
+    __shared__ int s_keys[kGlobal ? 1 : kHash];
+    __shared__ int s_minp[kGlobal ? 1 : kHash];
     const int lane = threadIdx.x;
     const int idx = blockIdx.x;
     if (idx >= nrows) return;
@@ -221,14 +226,14 @@ __global__ __launch_bounds__(kWave) void k_rap_row(
         const int b = aia[i1], len = aia[i1 + 1] - b;
         for (int k = lane; k < len; k += kWave) {
             const int p = L + k;
-            if (kGlobal || p < kRapCap) {
+            if (kGlobal || p < kCap) {
                 col[p] = agg[aja[b + k]];
                 if (kFill) val[p] = aval[b + k];
             }
         }
         L += len;
     }
-    if (!kGlobal && L > kRapCap) {   // handled by the global-memory variant
+    if (!kGlobal && L > kCap) {   // handled by a larger variant
         if (!kFill && lane == 0) { rowcount[c] = -1; rowlen[c] = L; }
         return;
     }
@@ -241,7 +246,7 @@ __global__ __launch_bounds__(kWave) void k_rap_row(
         keys = gkeys + h0;
         minp = gminp + h0;
     } else {
-        mask = kRapHash - 1u;
+        mask = kHash - 1u;
         keys = s_keys;
         minp = s_minp;
     }
@@ -418,35 +423,53 @@ void rap_agg(const DevCSR &A, const int *agg, int nagg, const int *rptr, const i
     hipStream_t st = stream();
     int *rowcount = arena().alloc_n<int>((size_t)nagg + 1);
     int *rowlen = arena().alloc_n<int>((size_t)nagg + 1);
-    if (nagg > 0) {
-        k_rap_row<false, false><<<nagg, kWave, 0, st>>>(nagg, nullptr, rptr, rmem, A.ia, A.ja, A.val,
-                                                        agg, rowcount, rowlen, nullptr, nullptr,
-                                                        nullptr, nullptr, nullptr, nullptr, nullptr,
-                                                        nullptr, nullptr);
-        HZ_LAUNCH_CHECK();
-    }
-    // rows whose candidate list exceeds the LDS capacity
     int *olist = arena().alloc_n<int>((size_t)nagg + 1);
     int *nol = arena().alloc_n<int>(1);
-    HZ_CHECK(hipMemsetAsync(nol, 0, sizeof(int), st));
-    if (nagg > 0) {
+    // rows whose candidate list exceeds the capacity of the pass just run
+    auto overflow = [&]() {
+        HZ_CHECK(hipMemsetAsync(nol, 0, sizeof(int), st));
         k_collect_overflow<<<grid_for(nagg), kBlock, 0, st>>>(nagg, rowcount, olist, nol);
         HZ_LAUNCH_CHECK();
+        int h = 0;
+        HZ_CHECK(hipMemcpyAsync(&h, nol, sizeof(int), hipMemcpyDeviceToHost, st));
+        HZ_CHECK(hipStreamSynchronize(st));
+        return h;
+    };
+    // count pass: small-LDS kernel over all rows, the large-LDS one over its
+    // overflow; what still overflows goes to global scratch
+    int n1 = 0, n2 = 0;
+    int *list1 = nullptr, *list2 = nullptr;
+    if (nagg > 0) {
+        k_rap_row<false, false, kRapSmall><<<nagg, kWave, 0, st>>>(nagg, nullptr, rptr, rmem, A.ia, A.ja, A.val,
+                                                                    agg, rowcount, rowlen, nullptr, nullptr,
+                                                                    nullptr, nullptr, nullptr, nullptr, nullptr,
+                                                                    nullptr, nullptr);
+        HZ_LAUNCH_CHECK();
+        n1 = overflow();
     }
-    int hnol = 0;
-    HZ_CHECK(hipMemcpyAsync(&hnol, nol, sizeof(int), hipMemcpyDeviceToHost, st));
-    HZ_CHECK(hipStreamSynchronize(st));
+    if (n1 > 0) {
+        list1 = arena().alloc_n<int>((size_t)n1 + 1);
+        HZ_CHECK(hipMemcpyAsync(list1, olist, sizeof(int) * n1, hipMemcpyDeviceToDevice, st));
+        k_rap_row<false, false, kRapCap><<<n1, kWave, 0, st>>>(n1, list1, rptr, rmem, A.ia, A.ja, A.val, agg,
+                                                                rowcount, rowlen, nullptr, nullptr, nullptr,
+                                                                nullptr, nullptr, nullptr, nullptr, nullptr,
+                                                                nullptr);
+        HZ_LAUNCH_CHECK();
+        n2 = overflow();
+    }
     int *gcol = nullptr;
     double *gval = nullptr;
     long long *goff = nullptr;
     int *gkeys = nullptr, *gminp = nullptr;
     long long *hoffd = nullptr;
-    if (hnol > 0) {
-        // deterministic order of the overflow list does not matter: each row is independent
-        std::vector<int> hl(hnol), hlen(hnol);
-        HZ_CHECK(memcpy_sync(hl.data(), olist, sizeof(int) * hnol, hipMemcpyDeviceToHost));
-        std::vector<long long> hoff(hnol + 1, 0), hh(hnol + 1, 0);
-        for (int i = 0; i < hnol; ++i) {
+    if (n2 > 0) {
+        list2 = arena().alloc_n<int>((size_t)n2 + 1);
+        HZ_CHECK(hipMemcpyAsync(list2, olist, sizeof(int) * n2, hipMemcpyDeviceToDevice, st));
+        // each overflow row is independent: the order of the list does not matter
+        std::vector<int> hl(n2);
+        HZ_CHECK(memcpy_sync(hl.data(), list2, sizeof(int) * n2, hipMemcpyDeviceToHost));
+        std::vector<long long> hoff(n2 + 1, 0), hh(n2 + 1, 0);
+        for (int i = 0; i < n2; ++i) {
             int L = 0;
             HZ_CHECK(memcpy_sync(&L, rowlen + hl[i], sizeof(int), hipMemcpyDeviceToHost));
             hoff[i + 1] = hoff[i] + L;
@@ -454,17 +477,17 @@ void rap_agg(const DevCSR &A, const int *agg, int nagg, const int *rptr, const i
             while (H < 2LL * L) H <<= 1;
             hh[i + 1] = hh[i] + H;
         }
-        hoffd = arena().alloc_n<long long>((size_t)hnol + 1);
-        HZ_CHECK(memcpy_sync(hoffd, hh.data(), sizeof(long long) * (hnol + 1), hipMemcpyHostToDevice));
-        gkeys = arena().alloc_n<int>((size_t)hh[hnol] + 1);
-        gminp = arena().alloc_n<int>((size_t)hh[hnol] + 1);
-        goff = arena().alloc_n<long long>((size_t)hnol + 1);
-        HZ_CHECK(memcpy_sync(goff, hoff.data(), sizeof(long long) * (hnol + 1), hipMemcpyHostToDevice));
-        gcol = arena().alloc_n<int>((size_t)hoff[hnol] + 1);
-        gval = arena().alloc_n<double>((size_t)hoff[hnol] + 1);
-        k_rap_row<false, true><<<hnol, kWave, 0, st>>>(hnol, olist, rptr, rmem, A.ia, A.ja, A.val, agg,
-                                                       rowcount, rowlen, nullptr, nullptr, nullptr,
-                                                       gcol, gval, goff, gkeys, gminp, hoffd);
+        hoffd = arena().alloc_n<long long>((size_t)n2 + 1);
+        HZ_CHECK(memcpy_sync(hoffd, hh.data(), sizeof(long long) * (n2 + 1), hipMemcpyHostToDevice));
+        gkeys = arena().alloc_n<int>((size_t)hh[n2] + 1);
+        gminp = arena().alloc_n<int>((size_t)hh[n2] + 1);
+        goff = arena().alloc_n<long long>((size_t)n2 + 1);
+        HZ_CHECK(memcpy_sync(goff, hoff.data(), sizeof(long long) * (n2 + 1), hipMemcpyHostToDevice));
+        gcol = arena().alloc_n<int>((size_t)hoff[n2] + 1);
+        gval = arena().alloc_n<double>((size_t)hoff[n2] + 1);
+        k_rap_row<false, true, 1><<<n2, kWave, 0, st>>>(n2, list2, rptr, rmem, A.ia, A.ja, A.val, agg, rowcount,
+                                                         rowlen, nullptr, nullptr, nullptr, gcol, gval, goff, gkeys,
+                                                         gminp, hoffd);
         HZ_LAUNCH_CHECK();
     }
     Ac.n = nagg;
@@ -474,16 +497,21 @@ void rap_agg(const DevCSR &A, const int *agg, int nagg, const int *rptr, const i
     Ac.ja = arena().alloc_n<int>((size_t)Ac.nnz + 1);
     Ac.val = arena().alloc_n<double>((size_t)Ac.nnz + 1);
     if (nagg > 0) {
-        k_rap_row<true, false><<<nagg, kWave, 0, st>>>(nagg, nullptr, rptr, rmem, A.ia, A.ja, A.val, agg,
-                                                       nullptr, nullptr, Ac.ia, Ac.ja, Ac.val,
-                                                       nullptr, nullptr, nullptr, nullptr, nullptr,
-                                                       nullptr);
+        k_rap_row<true, false, kRapSmall><<<nagg, kWave, 0, st>>>(nagg, nullptr, rptr, rmem, A.ia, A.ja, A.val, agg,
+                                                                   nullptr, nullptr, Ac.ia, Ac.ja, Ac.val, nullptr,
+                                                                   nullptr, nullptr, nullptr, nullptr, nullptr);
         HZ_LAUNCH_CHECK();
     }
-    if (hnol > 0) {
-        k_rap_row<true, true><<<hnol, kWave, 0, st>>>(hnol, olist, rptr, rmem, A.ia, A.ja, A.val, agg,
-                                                      nullptr, nullptr, Ac.ia, Ac.ja, Ac.val, gcol,
-                                                      gval, goff, gkeys, gminp, hoffd);
+    if (n1 > 0) {
+        k_rap_row<true, false, kRapCap><<<n1, kWave, 0, st>>>(n1, list1, rptr, rmem, A.ia, A.ja, A.val, agg,
+                                                               nullptr, nullptr, Ac.ia, Ac.ja, Ac.val, nullptr,
+                                                               nullptr, nullptr, nullptr, nullptr, nullptr);
+        HZ_LAUNCH_CHECK();
+    }
+    if (n2 > 0) {
+        k_rap_row<true, true, 1><<<n2, kWave, 0, st>>>(n2, list2, rptr, rmem, A.ia, A.ja, A.val, agg, nullptr,
+                                                        nullptr, Ac.ia, Ac.ja, Ac.val, gcol, gval, goff, gkeys,
+                                                        gminp, hoffd);
         HZ_LAUNCH_CHECK();
     }
 }
