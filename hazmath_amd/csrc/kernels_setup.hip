@@ -49,16 +49,33 @@ __global__ void k_strength(int n, const int *__restrict__ ia, const int *__restr
         const double th2 = th2_row ? th2_row[i] : th2c;
         int pos = kFill ? nia[i] : 0;
         int cnt = 0;
-        for (int k = ia[i]; k < ia[i + 1]; ++k) {
-            const int j = ja[k];
-            const double a = val[k];
-            if (j == i || ((a * a) >= th2 * fabs(di * d[j]) && a < 0e0)) {
-                if (kFill) {
-                    nja[pos] = j;
-                    nval[pos] = a;
-                    ++pos;
+        // entries in rounds of 8: column, value and d[j] loads issued first
+        constexpr int U = 8;
+        const int k0 = ia[i], k1 = ia[i + 1];
+        for (int k = k0; k < k1; k += U) {
+            int jj[U];
+            double aa[U], dj[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const bool ok = k + u < k1;
+                jj[u] = ok ? ja[k + u] : -1;
+                aa[u] = ok ? val[k + u] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) dj[u] = jj[u] >= 0 ? d[jj[u]] : 0.0;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (jj[u] < 0) continue;
+                const int j = jj[u];
+                const double a = aa[u];
+                if (j == i || ((a * a) >= th2 * fabs(di * dj[u]) && a < 0e0)) {
+                    if (kFill) {
+                        nja[pos] = j;
+                        nval[pos] = a;
+                        ++pos;
+                    }
+                    ++cnt;
                 }
-                ++cnt;
             }
         }
         if (!kFill) count[i] = cnt;
