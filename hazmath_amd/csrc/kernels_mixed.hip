@@ -106,26 +106,31 @@ __global__ __launch_bounds__(kBlock) void k_sweep_m(int n, int nslices, const lo
         const MT *v = val + base + lane;
         float t = (r < n) ? b[r] : 0.0f;
         float d = 0.0f;
+        // with the prolongation folded in (latency-bound small levels): rounds of
+        // 16 masked past the slice width; otherwise kU wide plus a tail
+        constexpr bool kMask = kProlong;
+        constexpr int U = kMask ? 16 : kU;
         int k = 0;
-        for (; k + kU <= wd; k += kU) {
-            int cc[kU];
-            float vv[kU], xx[kU];
+        for (; kMask ? (k < wd) : (k + U <= wd); k += U) {
+            int cc[U];
+            float vv[U], xx[U];
 #pragma unroll
-            for (int u = 0; u < kU; ++u) {
-                cc[u] = ld_nt(c + (k + u) * kSlice);
-                vv[u] = ldv(v + (k + u) * kSlice);
+            for (int u = 0; u < U; ++u) {
+                const bool ok = !kMask || k + u < wd;
+                cc[u] = ok ? ld_nt(c + (k + u) * kSlice) : -1;
+                vv[u] = ok ? ldv(v + (k + u) * kSlice) : 0.0f;
             }
 #pragma unroll
-            for (int u = 0; u < kU; ++u) xx[u] = cc[u] >= 0 ? x[cc[u]] : 0.0f;
+            for (int u = 0; u < U; ++u) xx[u] = cc[u] >= 0 ? x[cc[u]] : 0.0f;
             if (kProlong) {
 #pragma unroll
-                for (int u = 0; u < kU; ++u) {
+                for (int u = 0; u < U; ++u) {
                     const int g = cc[u] >= 0 ? agg[cc[u]] : HZ_UNPT;
                     if (g > HZ_UNPT) xx[u] = xx[u] + xc[g];
                 }
             }
 #pragma unroll
-            for (int u = 0; u < kU; ++u) {
+            for (int u = 0; u < U; ++u) {
                 if (kSmoother == HZ_SMOOTHER_L1DIAG) {
                     if (cc[u] >= 0) {
                         t = t - vv[u] * xx[u];
@@ -230,19 +235,53 @@ __global__ void k_restrict_m(int nc, const int *__restrict__ rptr, const int *__
 
 // residual + restriction (+ coarse zero-iterate sweep), one thread per coarse
 // row, for latency-bound levels (values read from the CSR copy).
-template <class MT>
+// kLanes lanes share a coarse row (members j, j + kLanes, ...), member rows
+// summed in CSR order with their loads issued 16 at a time, lane 0 adding the
+// members' residuals in member order (as k_down_fused in FP64).
+template <class MT, int kLanes>
 __global__ void k_down_fused_m(int nc, const int *__restrict__ rptr, const int *__restrict__ rmem,
                                const int *__restrict__ ia, const int *__restrict__ ja, const MT *__restrict__ a,
                                const float *__restrict__ b, const float *__restrict__ x, float *__restrict__ bc,
                                int smoother, float wgt, const float *__restrict__ dc, float *__restrict__ xc) {
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
-        float s = 0.0f;
-        for (int q = rptr[c]; q < rptr[c + 1]; ++q) {
+    constexpr int U = 16;
+    const long gt = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = (int)(gt / kLanes), j = (int)(gt % kLanes);
+    const int lane = threadIdx.x % kWave, gbase = lane - j;
+    const bool valid = c < nc;
+    const int q0 = valid ? rptr[c] : 0, q1 = valid ? rptr[c + 1] : 0;
+    float s = 0.0f;
+    for (int qb = q0; qb < q1; qb += kLanes) {   // trip count uniform within the lane group
+        const int q = qb + j;
+        float wv = 0.0f;
+        if (q < q1) {
             const int m = rmem[q];
+            const int k0 = ia[m], k1 = ia[m + 1];
+            const float bm = b[m];
             float t = 0.0f;
-            for (int k = ia[m]; k < ia[m + 1]; ++k) t = t + ldv(a + k) * x[ja[k]];
-            s = s + (b[m] - t);
+            for (int k = k0; k < k1; k += U) {
+                int jj[U];
+                float aa[U], xx[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const bool ok = k + u < k1;
+                    jj[u] = ok ? ja[k + u] : -1;
+                    aa[u] = ok ? ldv(a + k + u) : 0.0f;
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) xx[u] = jj[u] >= 0 ? x[jj[u]] : 0.0f;
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    if (jj[u] >= 0) t = t + aa[u] * xx[u];
+            }
+            wv = bm - t;
         }
+#pragma unroll
+        for (int u = 0; u < kLanes; ++u) {
+            const float v = __shfl(wv, gbase + u, kWave);
+            if (j == 0 && qb + u < q1) s = s + v;
+        }
+    }
+    if (valid && j == 0) {
         bc[c] = s;
         if (xc) xc[c] = zero_sweep(smoother, wgt, s, dc[c]);
     }
@@ -395,10 +434,10 @@ void down_fused_m(int nc, const int *rptr, const int *rmem, const DevCSR &A, con
                   const float *x, float *bc, int smoother, float wgt, const float *dc, float *xc) {
     if (nc <= 0) return;
     if (M.kind == kLowF32)
-        k_down_fused_m<float><<<grid_for(nc), kBlock, 0, stream()>>>(
+        k_down_fused_m<float, 4><<<div_up((long)nc * 4, kBlock), kBlock, 0, stream()>>>(
             nc, rptr, rmem, A.ia, A.ja, static_cast<const float *>(M.csr), b, x, bc, smoother, wgt, dc, xc);
     else
-        k_down_fused_m<unsigned short><<<grid_for(nc), kBlock, 0, stream()>>>(
+        k_down_fused_m<unsigned short, 4><<<div_up((long)nc * 4, kBlock), kBlock, 0, stream()>>>(
             nc, rptr, rmem, A.ia, A.ja, static_cast<const unsigned short *>(M.csr), b, x, bc, smoother, wgt, dc,
             xc);
     HZ_LAUNCH_CHECK();
