@@ -166,7 +166,7 @@ struct PinnedStage {
         return p;
     }
 };
-PinnedStage g_ia, g_ja, g_a, g_v;
+PinnedStage g_ia, g_ja, g_a, g_v, g_small;
 }  // namespace
 
 int aggregate_host(const DevCSR &N, int type, int max_aggregation, int *agg_dev, int *nagg) {
@@ -190,6 +190,101 @@ int aggregate_host(const DevCSR &N, int type, int max_aggregation, int *agg_dev,
     HZ_CHECK(hipMemcpyAsync(agg_dev, pv, sizeof(int) * n, hipMemcpyHostToDevice, st));
     HZ_CHECK(hipStreamSynchronize(st));
     return st_code;
+}
+
+// HEM with the sequential pass 1 on the host and only the data it reads on
+// the link.  With a symmetric strength pattern every lower neighbour of a
+// still-unmatched k is matched by time k, so pass 1 at k takes the first
+// unmatched entry of k's higher neighbours in preference order (|a|
+// descending, then row position descending: heavy_edg's scan).  The GPU sorts
+// those lists (hem_pref_lists); their copy to the host runs in row chunks
+// that the sweep consumes as they land; pass 2 (the leftovers) runs on the
+// GPU.  Returns 1 when the lists cannot express the pattern (the caller then
+// runs the full host sweep).
+int aggregate_hem_staged(const DevCSR &N, int *agg_dev, int *nagg) {
+    constexpr int kChunks = 8;
+    static hipEvent_t ev[kChunks];
+    static bool ev_ready = false;
+    if (!ev_ready) {
+        for (auto &e : ev) HZ_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        ev_ready = true;
+    }
+    const int n = N.n;
+    hipStream_t st = stream();
+    HemPrefLists pl;
+    hem_pref_lists(N, pl);
+    int *small = static_cast<int *>(g_small.get(4 * sizeof(int)));
+    int *info = static_cast<int *>(g_ia.get(sizeof(int) * ((size_t)n + 1)));
+    HZ_CHECK(hipMemcpyAsync(small, pl.flags, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    HZ_CHECK(hipMemcpyAsync(small + 2, pl.pp + n, sizeof(int), hipMemcpyDeviceToHost, st));
+    HZ_CHECK(hipMemcpyAsync(info, pl.info, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+    HZ_CHECK(hipStreamSynchronize(st));
+    if (small[0] || small[1]) return 1;
+    const long long total = small[2];
+    int *cols = static_cast<int *>(g_ja.get(sizeof(int) * ((size_t)total + 1)));
+    // row chunks of about equal entry counts; the sweep waits for each in turn
+    int row_end[kChunks];
+    {
+        long long off = 0;
+        int c = 0;
+        for (int k = 0; k < n && c < kChunks - 1; ++k) {
+            off += info[k] > 0 ? info[k] : 0;
+            if (off * kChunks >= total * (c + 1)) row_end[c++] = k + 1;
+        }
+        while (c < kChunks) row_end[c++] = n;
+        long long start = 0, acc = 0;
+        int k = 0;
+        for (int q = 0; q < kChunks; ++q) {
+            for (; k < row_end[q]; ++k) acc += info[k] > 0 ? info[k] : 0;
+            if (acc > start)
+                HZ_CHECK(hipMemcpyAsync(cols + start, pl.cols + start, sizeof(int) * (acc - start),
+                                        hipMemcpyDeviceToHost, st));
+            HZ_CHECK(hipEventRecord(ev[q], st));
+            start = acc;
+        }
+    }
+    int *v = static_cast<int *>(g_v.get(sizeof(int) * (size_t)n));
+    std::fill(v, v + n, -1);
+    std::vector<int> left;
+    long long off = 0;
+    int nc = 0, k = 0;
+    for (int q = 0; q < kChunks; ++q) {
+        HZ_CHECK(hipEventSynchronize(ev[q]));
+        for (; k < row_end[q]; ++k) {
+            const int c = info[k];
+            if (c < 0) continue;   // isolated (fewer than two entries): no aggregate
+            const int *pk = cols + off;
+            off += c;
+            if (v[k] != -1) continue;   // matched by a lower vertex
+            int pick = -1;
+            for (int r = 0; r < c; ++r)
+                if (v[pk[r]] == -1) {
+                    pick = pk[r];
+                    break;
+                }
+            if (pick >= 0) {
+                v[k] = nc;
+                v[pick] = nc;
+                ++nc;
+            } else {
+                left.push_back(k);
+            }
+        }
+    }
+    HZ_CHECK(hipMemcpyAsync(agg_dev, v, sizeof(int) * n, hipMemcpyHostToDevice, st));
+    int rounds = 0;
+    const int nl = (int)left.size();
+    if (nl > 0) {
+        const int p2 = hem_pass2_from_host(N, agg_dev, nc, left.data(), nl, &rounds);
+        if (p2 != HZ_SUCCESS) return p2;
+    } else {
+        HZ_CHECK(hipStreamSynchronize(st));   // v is reused by the next level
+    }
+    *nagg = nc;
+    if (trace_enabled())
+        std::fprintf(stderr, "[hz]   HEM n=%d: host pass 1 over %lld preference entries, leftovers %d, "
+                     "pass-2 rounds %d\n", n, total, nl, rounds);
+    return HZ_SUCCESS;
 }
 
 }  // namespace hz

@@ -107,7 +107,8 @@ hipStream_t stream() {
 // hierarchy with pinned staging (setup ms by threshold): 0: 43.2, 300K: 29.5,
 // 600K: 27.9, 1.1M: 31.3 -- the GPU pipeline's dependency wavefront does not
 // shrink with the level, while the host sweep is linear in it.
-int g_hem_host_max = 0;   // hz_options.hem_host_max_rows: 0 = default, < 0 = never on the host
+int g_hem_host_max = 0;   // hz_options.hem_host_max_rows: 0 = default, -1 = GPU pipeline,
+                          // -2 = full host sweep, -3 = staged host sweep (every level)
 
 int hem_host_max_rows() {
     if (g_hem_host_max < 0) return -1;
@@ -115,10 +116,32 @@ int hem_host_max_rows() {
     static int v = -1;
     if (v < 0) {
         const char *e = std::getenv("HZ_HEM_HOST_MAX");
-        v = e ? std::atoi(e) : 600000;
+        v = e ? std::atoi(e) : 1100000;
     }
     return v;
 }
+
+namespace {
+// HEM for one level.  Small levels copy the whole strength graph and sweep it
+// on the host (two transfers, nothing else to wait for); mid-size levels copy
+// only GPU-sorted preference lists; the largest run the GPU pipeline, whose
+// cost follows the length of the sweep's dependency chains rather than n.
+constexpr int kHemFullHostRows = 32768;
+int hem_level(const DevCSR &N, AMG_param *param, int *agg, int *nagg) {
+    const int n = N.n;
+    int st;
+    if (g_hem_host_max == -1) st = hem_aggregate_gpu(N, agg, nagg);
+    else if (g_hem_host_max == -2) st = 1;
+    else if (g_hem_host_max == -3) st = aggregate_hem_staged(N, agg, nagg);
+    else if (n > hem_host_max_rows()) st = hem_aggregate_gpu(N, agg, nagg);
+    else if (n > kHemFullHostRows) st = aggregate_hem_staged(N, agg, nagg);
+    else st = 1;
+    // the full host sweep: small levels, patterns the other forms cannot
+    // express, and the reference's message for a failed pass 2
+    if (st == 1 || st == -1017) st = aggregate_host(N, HZ_HEM, param->max_aggregation, agg, nagg);
+    return st;
+}
+}  // namespace
 
 int amg_setup(Hierarchy &H, const DevCSR &A0, AMG_param *param, bool) {
     const short min_cdof = (short)std::max(param->coarse_dof, 1);
@@ -160,8 +183,7 @@ int amg_setup(Hierarchy &H, const DevCSR &A0, AMG_param *param, bool) {
         } else if (type == HZ_HEM) {
             // small levels: the sequential sweep on the host beats the GPU's
             // wavefront (whose length does not shrink with the level)
-            st = n <= hem_host_max_rows() ? 1 : hem_aggregate_gpu(N, L.agg, &nagg);
-            if (st == 1) st = aggregate_host(N, type, param->max_aggregation, L.agg, &nagg);
+            st = hem_level(N, param, L.agg, &nagg);
             if (st == -1017) return -1017;
         } else {
             st = aggregate_host(N, type, param->max_aggregation, L.agg, &nagg);

@@ -104,4 +104,15 @@ int hem_host_max_rows();
 // symmetric (then the host sweep is used).
 int hem_aggregate_gpu(const DevCSR &N, int *agg, int *nagg_out);
 
+// HEM pass 1 on the host over GPU-sorted preference lists, pass 2 on the GPU:
+// SUCCESS, -1017, or 1 when the pattern needs the full host sweep.
+int aggregate_hem_staged(const DevCSR &N, int *agg_dev, int *nagg);
+
+// Pieces shared by the two HEM paths (kernels_hem.hip).
+struct HemPrefLists {
+    int *info = nullptr, *pp = nullptr, *cols = nullptr, *flags = nullptr;
+};
+void hem_pref_lists(const DevCSR &N, HemPrefLists &out);
+int hem_pass2_from_host(const DevCSR &N, int *agg_dev, int nc, const int *list_host, int nl, int *rounds);
+
 }  // namespace hz

@@ -133,6 +133,8 @@ void coarse_scaling_alpha(const DevSELL &S, const double *x, const double *b, Pc
 void strength_graph(const DevCSR &A, double theta, DevCSR &N, int *counts_ws);
 // Exclusive scan of n ints into out (n+1 entries, out[n] = total); returns total.
 long long exclusive_scan_int(const int *in, int *out, int n);
+// The same scan left on the stream (no host wait); the total stays in out[n].
+void exclusive_scan_int_device(const int *in, int *out, int n);
 // HEC aggregation (exact restatement of the sequential turn-order algorithm).
 // perm: turn order (perm[t] = vertex); returns number of aggregates or a negative code.
 int hec_aggregate(const DevCSR &N, const int *perm, int *agg);

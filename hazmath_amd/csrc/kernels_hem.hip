@@ -65,6 +65,21 @@ __global__ void k_sym_check(int n, const int *__restrict__ ia, const int *__rest
     }
 }
 
+// Inserts position q (weight |a[q]|) into the m sorted positions at out[0..m):
+// |a| descending, then position descending (heavy_edg's tie rule).
+__device__ __forceinline__ void pref_insert(int *out, int m, int q, const double *__restrict__ a) {
+    const double w = fabs(a[q]);
+    int pos = m;
+    while (pos > 0) {
+        const int qq = out[pos - 1];
+        const double wq = fabs(a[qq]);
+        if (wq > w || (wq == w && qq > q)) break;
+        out[pos] = qq;
+        --pos;
+    }
+    out[pos] = q;
+}
+
 // Preference list of each row: higher neighbours sorted by |a| descending,
 // then by row position descending (heavy_edg's tie rule).
 __global__ void k_prefs(int n, const int *__restrict__ ia, const int *__restrict__ ja,
@@ -75,23 +90,55 @@ __global__ void k_prefs(int n, const int *__restrict__ ia, const int *__restrict
         fstate[k] = (e - b > 1) ? kFreeUnk : kFreeF;   // isolated: never processed
         int m = 0;
         for (int q = b; q < e; ++q) {
-            const int j = ja[q];
-            if (j <= k) continue;
-            // insertion: keep (|a| desc, position desc)
-            const double w = fabs(a[q]);
-            int pos = m;
-            while (pos > 0) {
-                const int qq = pref[b + pos - 1];
-                const double wq = fabs(a[qq]);
-                if (wq > w || (wq == w && qq > q)) break;
-                pref[b + pos] = qq;
-                --pos;
-            }
-            pref[b + pos] = q;
+            if (ja[q] <= k) continue;
+            pref_insert(pref + b, m, q, a);
             ++m;
         }
         npref[k] = m;
     }
+}
+
+// Compact preference lists for the host sweep.  cnt[k]: number of higher
+// neighbours (0 for an isolated row, fewer than two entries); info[k]: the
+// same count, or -1 for an isolated row.  A one-entry row whose entry is not
+// its diagonal would be an isolated vertex that others can still pick, which
+// the higher-neighbour lists cannot express: it raises *unsupported.
+__global__ void k_pref_count(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                             int *__restrict__ cnt, int *__restrict__ info, int *__restrict__ unsupported) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int b = ia[k], e = ia[k + 1];
+        if (e - b <= 1) {
+            if (e - b == 1 && ja[b] != k) atomicExch(unsupported, 1);
+            cnt[k] = 0;
+            info[k] = -1;
+            continue;
+        }
+        int m = 0;
+        for (int q = b; q < e; ++q) m += ja[q] > k;
+        cnt[k] = m;
+        info[k] = m;
+    }
+}
+
+// Row k's higher neighbours as column indices at out[pp[k] ..), in preference order.
+__global__ void k_pref_fill(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                            const double *__restrict__ a, const int *__restrict__ pp, int *__restrict__ out) {
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int b = ia[k], e = ia[k + 1];
+        if (e - b <= 1) continue;
+        int *o = out + pp[k];
+        int m = 0;
+        for (int q = b; q < e; ++q) {
+            if (ja[q] <= k) continue;
+            pref_insert(o, m, q, a);
+            ++m;
+        }
+        for (int r = 0; r < m; ++r) o[r] = ja[o[r]];
+    }
+}
+
+__global__ void k_scatter_one(int nl, const int *__restrict__ list, int *__restrict__ flag) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nl; t += gridDim.x * blockDim.x) flag[list[t]] = 1;
 }
 
 // taken(j, t): 1 true, 0 false, -1 unknown
@@ -405,6 +452,40 @@ __global__ void k_compact_leftovers(int n, const int *__restrict__ leftover, con
 
 }  // namespace
 
+// Pass 2 of HEM (amg_setup_ua.c:1018-1046) on the GPU: the nl leftovers in
+// list (ascending) join the smallest neighbouring aggregate, first in row
+// order on ties, with sizes growing as they join.  agg holds the pass-1
+// aggregates (-1 elsewhere), size their sizes, lstate[v] = 1 exactly for the
+// leftovers (cleared as they decide).  flagbuf: 2 device ints of scratch.
+// Returns SUCCESS, -1017 (a leftover without aggregated neighbours) or 1.
+static int hem_pass2(const DevCSR &N, int *agg, int *size, int nc, const int *list, int nl, int *lstate,
+                     int *flagbuf, int *rounds_out) {
+    hipStream_t st = stream();
+    int *done = arena().alloc_n<int>((size_t)nl + 1);
+    int *owner = arena().alloc_n<int>((size_t)nc + 1);
+    HZ_CHECK(hipMemsetAsync(done, 0, sizeof(int) * nl, st));
+    k_fill_i<<<grid_for(nc), kBlock, 0, st>>>(owner, INT_MAX, nc);
+    int *err = flagbuf, *remaining = flagbuf + 1;
+    HZ_CHECK(hipMemsetAsync(err, 0, sizeof(int), st));
+    const int gl = grid_for(nl);
+    for (int round = 0;; ++round) {
+        HZ_CHECK(hipMemsetAsync(remaining, 0, sizeof(int), st));
+        k_left_claim<<<gl, kBlock, 0, st>>>(nl, list, done, N.ia, N.ja, agg, owner);
+        k_left_decide<<<gl, kBlock, 0, st>>>(nl, list, done, N.ia, N.ja, agg, size, owner, lstate, err,
+                                             remaining);
+        k_reset_owner<<<gl, kBlock, 0, st>>>(nl, list, N.ia, N.ja, agg, owner);
+        k_left_commit<<<gl, kBlock, 0, st>>>(nl, list, done, agg, size, lstate);
+        HZ_LAUNCH_CHECK();
+        int hr[2] = {0, 0};
+        HZ_CHECK(memcpy_sync(hr, err, 2 * sizeof(int), hipMemcpyDeviceToHost));
+        *rounds_out = round + 1;
+        if (hr[0]) return -1017;
+        if (hr[1] == 0) break;
+        if (round > nl + 8) return 1;
+    }
+    return HZ_SUCCESS;
+}
+
 // Returns SUCCESS, -1017 (reference exit(17)), or 1 when the strength pattern
 // is not symmetric (caller falls back to the host sweep).
 int hem_aggregate_gpu(const DevCSR &N, int *agg, int *nagg_out) {
@@ -475,37 +556,59 @@ int hem_aggregate_gpu(const DevCSR &N, int *agg, int *nagg_out) {
     const int nl = (int)exclusive_scan_int(leftover, lpos, n);
     if (nl > 0) {
         int *list = arena().alloc_n<int>((size_t)nl + 1);
-        int *done = arena().alloc_n<int>((size_t)nl + 1);
-        int *owner = arena().alloc_n<int>((size_t)nc + 1);
         k_compact_leftovers<<<g, kBlock, 0, st>>>(n, leftover, lpos, list);
-        HZ_CHECK(hipMemsetAsync(done, 0, sizeof(int) * nl, st));
-        k_fill_i<<<grid_for(nc), kBlock, 0, st>>>(owner, INT_MAX, nc);
-        int *err = flagbuf + 2, *remaining = flagbuf + 3;
-        HZ_CHECK(hipMemsetAsync(err, 0, sizeof(int), st));
-        // lstate[v] = 1 while v is an undecided leftover (the leftover flags themselves)
-        int *lstate = leftover;
-        const int gl = grid_for(nl);
-        for (int round = 0;; ++round) {
-            HZ_CHECK(hipMemsetAsync(remaining, 0, sizeof(int), st));
-            k_left_claim<<<gl, kBlock, 0, st>>>(nl, list, done, N.ia, N.ja, agg, owner);
-            k_left_decide<<<gl, kBlock, 0, st>>>(nl, list, done, N.ia, N.ja, agg, size, owner, lstate,
-                                                 err, remaining);
-            k_reset_owner<<<gl, kBlock, 0, st>>>(nl, list, N.ia, N.ja, agg, owner);
-            k_left_commit<<<gl, kBlock, 0, st>>>(nl, list, done, agg, size, lstate);
-            HZ_LAUNCH_CHECK();
-            int hr[2] = {0, 0};
-            HZ_CHECK(memcpy_sync(hr, err, 2 * sizeof(int), hipMemcpyDeviceToHost));
-            rounds2 = round + 1;
-            if (hr[0]) return -1017;
-            if (hr[1] == 0) break;
-            if (round > nl + 8) return 1;
-        }
+        HZ_LAUNCH_CHECK();
+        // the leftover flags double as lstate (1 while undecided)
+        const int r = hem_pass2(N, agg, size, nc, list, nl, leftover, flagbuf + 2, &rounds2);
+        if (r != HZ_SUCCESS) return r;
     }
     *nagg_out = nc;
     if (trace_enabled())
         std::fprintf(stderr, "[hz]   HEM n=%d: pass-1 %s, fallback rounds %d, leftovers %d, pass-2 rounds %d\n",
                      n, pipe_status ? "pipeline gave up" : "pipeline", rounds1, nl, rounds2);
     return HZ_SUCCESS;
+}
+
+// Preference lists of HEM's pass 1 for the host sweep, queued on the stream
+// without a host wait: info[k] (count of higher neighbours, -1 isolated),
+// their column indices in preference order concatenated in row order at
+// cols[pp[k] ..) (pp[n]: the total; cols holds up to nnz entries), and
+// flags[0..1] nonzero when the lists cannot express the pattern (not
+// symmetric, or a one-entry row without its diagonal).  Device arena memory.
+void hem_pref_lists(const DevCSR &N, HemPrefLists &out) {
+    hipStream_t st = stream();
+    const int n = N.n;
+    const int g = grid_for(n);
+    out.flags = arena().alloc_n<int>(2);
+    HZ_CHECK(hipMemsetAsync(out.flags, 0, 2 * sizeof(int), st));
+    k_sym_check<<<g, kBlock, 0, st>>>(n, N.ia, N.ja, out.flags);
+    int *cnt = arena().alloc_n<int>((size_t)n + 1);
+    out.info = arena().alloc_n<int>((size_t)n + 1);
+    out.pp = arena().alloc_n<int>((size_t)n + 1);
+    k_pref_count<<<g, kBlock, 0, st>>>(n, N.ia, N.ja, cnt, out.info, out.flags + 1);
+    HZ_LAUNCH_CHECK();
+    exclusive_scan_int_device(cnt, out.pp, n);
+    out.cols = arena().alloc_n<int>((size_t)N.nnz + 1);
+    k_pref_fill<<<g, kBlock, 0, st>>>(n, N.ia, N.ja, N.val, out.pp, out.cols);
+    HZ_LAUNCH_CHECK();
+}
+
+// Pass 2 after a host pass 1: agg_dev holds the pass-1 map (-1 for leftovers
+// and isolated vertices), every pass-1 aggregate has two members, and
+// list_host the nl leftovers in ascending order.
+int hem_pass2_from_host(const DevCSR &N, int *agg_dev, int nc, const int *list_host, int nl, int *rounds) {
+    hipStream_t st = stream();
+    const int n = N.n;
+    int *list = arena().alloc_n<int>((size_t)nl + 1);
+    HZ_CHECK(hipMemcpyAsync(list, list_host, sizeof(int) * nl, hipMemcpyHostToDevice, st));
+    int *size = arena().alloc_n<int>((size_t)nc + 1);
+    k_fill_i<<<grid_for(nc), kBlock, 0, st>>>(size, 2, nc);
+    int *lstate = arena().alloc_n<int>((size_t)n + 1);
+    k_fill_i<<<grid_for(n), kBlock, 0, st>>>(lstate, 0, n);
+    k_scatter_one<<<grid_for(nl), kBlock, 0, st>>>(nl, list, lstate);
+    HZ_LAUNCH_CHECK();
+    int *flagbuf = arena().alloc_n<int>(2);
+    return hem_pass2(N, agg_dev, size, nc, list, nl, lstate, flagbuf, rounds);
 }
 
 }  // namespace hz

@@ -335,7 +335,7 @@ inline int grid_for(long n) {
 
 }  // namespace
 
-long long exclusive_scan_int(const int *in, int *out, int n) {
+void exclusive_scan_int_device(const int *in, int *out, int n) {
     hipStream_t st = stream();
     size_t tmp_bytes = 0;
     // out has n+1 entries; scan n+1 inputs where the last input is treated as 0
@@ -347,6 +347,11 @@ long long exclusive_scan_int(const int *in, int *out, int n) {
     void *tmp = arena().alloc(tmp_bytes);
     HZ_CHECK(rocprim::exclusive_scan(tmp, tmp_bytes, in_ext, out, 0, (size_t)n + 1,
                                      rocprim::plus<int>(), st));
+}
+
+long long exclusive_scan_int(const int *in, int *out, int n) {
+    hipStream_t st = stream();
+    exclusive_scan_int_device(in, out, n);
     int total = 0;
     HZ_CHECK(hipMemcpyAsync(&total, out + n, sizeof(int), hipMemcpyDeviceToHost, st));
     HZ_CHECK(hipStreamSynchronize(st));

@@ -320,9 +320,12 @@ typedef struct {
                               sweep after the solve (hz_batch_get_stats) */
     INT  time_pcg_spmv;      /* >0: record HIP events around each PCG SpMV of the solve
                               (hz_solve_info.pcg_spmv_ms) */
-    INT  hem_host_max_rows;  /* HEM levels up to this many rows run the sequential sweep on the
-                              host, larger ones the GPU pipeline (same aggregates); 0: 600000
-                              (or the HZ_HEM_HOST_MAX environment variable), < 0: always GPU */
+    INT  hem_host_max_rows;  /* HEM levels up to this many rows run pass 1 as a sequential sweep
+                              on the host (up to 32768 rows over the whole strength graph, above
+                              over GPU-sorted preference lists), larger ones the GPU pipeline;
+                              all give the same aggregates.  0: 1100000 (or the HZ_HEM_HOST_MAX
+                              environment variable); every level through one form: -1 the GPU
+                              pipeline, -2 the full host sweep, -3 the preference lists */
 } hz_options;
 
 void hz_get_options(hz_options *opt);

@@ -123,9 +123,12 @@ def _hierarchies(lib, orc, libc, A, agg_type, theta, coarse_solver=32, coarse_do
 
 
 class _HemPath:
-    """HEM levels on the host sweep (the default up to 600,000 rows) or all on
-    the GPU pipeline (hz_options.hem_host_max_rows = -1): both must give the
-    reference's aggregates."""
+    """HEM levels through the default choice by level size ("host": these
+    problems are small enough for the full host sweep), or every level through
+    one form (hz_options.hem_host_max_rows): the GPU pipeline (-1), the full
+    host sweep over the whole strength graph (-2), or the host sweep over
+    GPU-sorted preference lists with pass 2 on the GPU (-3).  All must give
+    the reference's aggregates."""
 
     def __init__(self, lib, path):
         self.lib, self.path = lib, path
@@ -134,7 +137,7 @@ class _HemPath:
         opts = capi.hz_options()
         self.lib.hz_get_options(C.byref(opts))
         self.old = opts.hem_host_max_rows
-        opts.hem_host_max_rows = -1 if self.path == "gpu" else 0
+        opts.hem_host_max_rows = {"gpu": -1, "full": -2, "staged": -3}.get(self.path, 0)
         self.lib.hz_set_options(C.byref(opts))
 
     def __exit__(self, *exc):
@@ -145,7 +148,7 @@ class _HemPath:
 
 
 @pytest.mark.parametrize("agg_type,hem_path", [(capi.HEC, "host"), (capi.HEM, "host"), (capi.HEM, "gpu"),
-                                               (capi.VMB, "host")])
+                                               (capi.HEM, "full"), (capi.HEM, "staged"), (capi.VMB, "host")])
 @pytest.mark.parametrize("theta", [0.0, 0.04, 0.25])
 def test_hierarchy_bitexact_solvers_fixture(lib, orc, libc, solvers_fixture, agg_type, hem_path, theta):
     ia, ja, va, _ = solvers_fixture
@@ -166,7 +169,7 @@ def test_hierarchy_bitexact_solvers_fixture(lib, orc, libc, solvers_fixture, agg
 
 @pytest.mark.parametrize("m", [16, 24])
 @pytest.mark.parametrize("agg_type,hem_path", [(capi.HEC, "host"), (capi.HEM, "host"), (capi.HEM, "gpu"),
-                                               (capi.VMB, "host")])
+                                               (capi.HEM, "full"), (capi.HEM, "staged"), (capi.VMB, "host")])
 def test_hierarchy_bitexact_kuhn3d(lib, orc, libc, m, agg_type, hem_path):
     ia, ja, va = problems.poisson3d_kuhn(m)
     A = capi.HostCSR(ia, ja, va)
@@ -179,6 +182,25 @@ def test_hierarchy_bitexact_kuhn3d(lib, orc, libc, m, agg_type, hem_path):
         if "agg" in o:
             assert np.array_equal(g["agg"], o["agg"])
     assert rg == ro
+
+
+@pytest.mark.parametrize("m", [64, 106])
+def test_hierarchy_bitexact_kuhn3d_hem_by_size(lib, orc, libc, m):
+    """HEM with the default form per level size: at m = 106 (1,157,625 rows)
+    level 0 runs the GPU pipeline, the levels down to 32,768 rows the host
+    sweep over GPU-sorted preference lists, the rest the full host sweep."""
+    ia, ja, va = problems.poisson3d_kuhn(m)
+    A = capi.HostCSR(ia, ja, va)
+    with _HemPath(lib, "host"):
+        (lg, thg, _), (lo, tho, _) = _hierarchies(lib, orc, libc, A, capi.HEM, 0.04, coarse_dof=100)
+    assert len(lg) == len(lo)
+    for g, o in zip(lg, lo):
+        if "agg" in o:
+            assert g["nagg"] == o["nagg"]
+            assert np.array_equal(g["agg"], o["agg"])
+        for key in ("ia", "ja", "val"):
+            assert np.array_equal(g[key], o[key]), key
+    assert thg == tho
 
 
 def _solve_both(lib, orc, libc, A, b, **kw):
