@@ -245,6 +245,10 @@ int aggregate_hem_staged(const DevCSR &N, int *agg_dev, int *nagg) {
     }
     int *v = static_cast<int *>(g_v.get(sizeof(int) * (size_t)n));
     std::fill(v, v + n, -1);
+    // the sweep's working set: one byte per vertex (matched), cache resident
+    static std::vector<unsigned char> matched;
+    matched.assign(n, 0);
+    unsigned char *mt = matched.data();
     std::vector<int> left;
     long long off = 0;
     int nc = 0, k = 0;
@@ -255,14 +259,16 @@ int aggregate_hem_staged(const DevCSR &N, int *agg_dev, int *nagg) {
             if (c < 0) continue;   // isolated (fewer than two entries): no aggregate
             const int *pk = cols + off;
             off += c;
-            if (v[k] != -1) continue;   // matched by a lower vertex
+            if (mt[k]) continue;   // matched by a lower vertex
             int pick = -1;
             for (int r = 0; r < c; ++r)
-                if (v[pk[r]] == -1) {
+                if (!mt[pk[r]]) {
                     pick = pk[r];
                     break;
                 }
             if (pick >= 0) {
+                mt[k] = 1;
+                mt[pick] = 1;
                 v[k] = nc;
                 v[pick] = nc;
                 ++nc;

@@ -215,7 +215,10 @@ __device__ __forceinline__ unsigned rap_find(const int *keys, int key, unsigned 
     return slot;
 }
 
-template <bool kFill, bool kGlobal, int kCap>
+// kTemp (with kFill): one pass -- the row goes to fixed-capacity slots
+// cja/cval + c * kCap (moved to its CSR place by k_rap_compact after the
+// scan) and its count and candidate length to rowcount/rowlen.
+template <bool kFill, bool kGlobal, int kCap, bool kTemp = false>
 __global__ __launch_bounds__(kWave) void k_rap_row(
     int nrows, const int *__restrict__ rowlist, const int *__restrict__ rptr,
     const int *__restrict__ rmem, const int *__restrict__ aia, const int *__restrict__ aja,
@@ -229,6 +232,8 @@ __global__ __launch_bounds__(kWave) void k_rap_row(
     __shared__ double s_val[(kGlobal || !kFill) ? 1 : kCap];
     __shared__ int s_keys[kGlobal ? 1 : kHash];
     __shared__ int s_minp[kGlobal ? 1 : kHash];
+    __shared__ int s_maxp[kGlobal ? 1 : kHash];
+    __shared__ int s_diag;
     const int lane = threadIdx.x;
     const int idx = blockIdx.x;
     if (idx >= nrows) return;
@@ -251,10 +256,12 @@ __global__ __launch_bounds__(kWave) void k_rap_row(
         L += len;
     }
     if (!kGlobal && L > kCap) {   // handled by a larger variant
-        if (!kFill && lane == 0) { rowcount[c] = -1; rowlen[c] = L; }
+        if ((!kFill || kTemp) && lane == 0) { rowcount[c] = -1; rowlen[c] = L; }
         return;
     }
-    // 2) hash table of distinct coarse columns -> first position
+    const long long obase = kTemp ? (long long)c * kCap : (kFill ? (long long)cia[c] : 0);
+    // 2) hash table of distinct coarse columns -> first position (LDS
+    //    variants: also the last position, and the diagonal column c)
     unsigned mask;
     int *keys, *minp;
     if (kGlobal) {
@@ -270,11 +277,13 @@ __global__ __launch_bounds__(kWave) void k_rap_row(
     for (unsigned t = lane; t <= mask; t += kWave) {
         keys[t] = -1;
         minp[t] = INT_MAX;
+        if (!kGlobal) s_maxp[t] = -1;
     }
+    if (lane == 0) s_diag = 0;
     __syncthreads();
     for (int p = lane; p < L; p += kWave) {
         const int key = col[p];
-        if (key <= HZ_UNPT || key == c) continue;
+        if (key <= HZ_UNPT || (kGlobal && key == c)) continue;
         unsigned slot = rap_hash(key, mask);
         while (true) {
             const int prev = atomicCAS(&keys[slot], -1, key);
@@ -282,41 +291,72 @@ __global__ __launch_bounds__(kWave) void k_rap_row(
             slot = (slot + 1) & mask;
         }
         atomicMin(&minp[slot], p);
+        if (!kGlobal) atomicMax(&s_maxp[slot], p);
     }
     __syncthreads();
-    // 3) first appearances, their ranks and (fill) in-order sums
+    // 3) first appearances, their ranks and (fill) in-order sums; a sum
+    //    scans from the column's first to its last position
     int nfirst_before = 0;
     for (int base = 0; base < L; base += kWave) {
         const int p = base + lane;
-        bool first = false;
-        int cp = -1;
+        bool first = false, dfirst = false;
+        int cp = -1, last = L - 1;
         if (p < L) {
             cp = col[p];
-            if (cp > HZ_UNPT && cp != c) first = (minp[rap_find(keys, cp, mask)] == p);
+            if (cp > HZ_UNPT && (!kGlobal || cp != c)) {
+                const unsigned slot = rap_find(keys, cp, mask);
+                const bool f = minp[slot] == p;
+                if (!kGlobal) last = s_maxp[slot];
+                first = f && cp != c;
+                dfirst = f && cp == c;
+            }
         }
         const unsigned long long m = __ballot(first);
-        if (kFill && first) {
-            const int rank = nfirst_before + __popcll(m & ((1ull << lane) - 1ull));
-            double v = val[p];
-            for (int q = p + 1; q < L; ++q)
+        if (kFill && (first || dfirst)) {
+            // the diagonal's sum starts from 0.0, the others from the first value
+            double v = dfirst ? 0.0 + val[p] : val[p];
+            for (int q = p + 1; q <= last; ++q)
                 if (col[q] == cp) v = v + val[q];
-            const int pos = cia[c] + 1 + rank;
+            const long long pos = dfirst ? obase : obase + 1 + nfirst_before + __popcll(m & ((1ull << lane) - 1ull));
             cja[pos] = cp;
             cval[pos] = v;
+            if (dfirst) s_diag = 1;
         }
         nfirst_before += __popcll(m);
     }
+    if (kFill && !kGlobal) __syncthreads();
     if (lane == 0) {
-        if (kFill) {
+        if (kFill && kGlobal) {
             double v = 0.0;
             for (int q = 0; q < L; ++q)
                 if (col[q] == c) v = v + val[q];
-            cja[cia[c]] = c;
-            cval[cia[c]] = v;
-        } else {
+            cja[obase] = c;
+            cval[obase] = v;
+        } else if (kFill && !s_diag) {   // no candidate in column c
+            cja[obase] = c;
+            cval[obase] = 0.0;
+        }
+        if (!kFill || kTemp) {
             rowcount[c] = 1 + nfirst_before;
             if (!kGlobal) rowlen[c] = L;
         }
+    }
+}
+
+// Rows built by the one-pass kernel (rowlen <= kCap) to their CSR places,
+// kGroup lanes per row.
+template <int kCap, int kGroup>
+__global__ void k_rap_compact(int nc, const int *__restrict__ rowlen, const int *__restrict__ cia,
+                              const int *__restrict__ tcol, const double *__restrict__ tval,
+                              int *__restrict__ cja, double *__restrict__ cval) {
+    const long gt = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = (int)(gt / kGroup), j = (int)(gt % kGroup);
+    if (c >= nc || rowlen[c] > kCap) return;
+    const int b = cia[c], cnt = cia[c + 1] - b;
+    const long long t = (long long)c * kCap;
+    for (int k = j; k < cnt; k += kGroup) {
+        cja[b + k] = tcol[t + k];
+        cval[b + k] = tval[t + k];
     }
 }
 
@@ -421,7 +461,7 @@ void build_restriction(int n, const int *agg, int nagg, int *rptr, int *rmem) {
     HZ_CHECK(hipMemsetAsync(cnt, 0, sizeof(int) * ((size_t)nagg + 1), st));
     k_agg_count<<<grid_for(n), kBlock, 0, st>>>(n, agg, cnt);
     HZ_LAUNCH_CHECK();
-    exclusive_scan_int(cnt, rptr, nagg);
+    exclusive_scan_int_device(cnt, rptr, nagg);
     // stable radix sort of (aggregate, vertex) pairs: members come out ascending
     unsigned int *keys = arena().alloc_n<unsigned int>((size_t)n + 1);
     unsigned int *keys_out = arena().alloc_n<unsigned int>((size_t)n + 1);
@@ -457,15 +497,19 @@ void rap_agg(const DevCSR &A, const int *agg, int nagg, const int *rptr, const i
         HZ_CHECK(hipStreamSynchronize(st));
         return h;
     };
-    // count pass: small-LDS kernel over all rows, the large-LDS one over its
-    // overflow; what still overflows goes to global scratch
+    // the small-LDS kernel over all rows (one pass), the large-LDS one over
+    // its overflow (count, then fill); what still overflows goes to global
+    // scratch
     int n1 = 0, n2 = 0;
     int *list1 = nullptr, *list2 = nullptr;
+    // rows up to kRapSmall candidates are built in one pass into slots of
+    // kRapSmall entries and moved to their CSR places after the scan
+    int *tcol = arena().alloc_n<int>((size_t)nagg * kRapSmall + 1);
+    double *tval = arena().alloc_n<double>((size_t)nagg * kRapSmall + 1);
     if (nagg > 0) {
-        k_rap_row<false, false, kRapSmall><<<nagg, kWave, 0, st>>>(nagg, nullptr, rptr, rmem, A.ia, A.ja, A.val,
-                                                                    agg, rowcount, rowlen, nullptr, nullptr,
-                                                                    nullptr, nullptr, nullptr, nullptr, nullptr,
-                                                                    nullptr, nullptr);
+        k_rap_row<true, false, kRapSmall, true><<<nagg, kWave, 0, st>>>(
+            nagg, nullptr, rptr, rmem, A.ia, A.ja, A.val, agg, rowcount, rowlen, nullptr, tcol, tval, nullptr,
+            nullptr, nullptr, nullptr, nullptr, nullptr);
         HZ_LAUNCH_CHECK();
         n1 = overflow();
     }
@@ -519,9 +563,9 @@ void rap_agg(const DevCSR &A, const int *agg, int nagg, const int *rptr, const i
     Ac.ja = arena().alloc_n<int>((size_t)Ac.nnz + 1);
     Ac.val = arena().alloc_n<double>((size_t)Ac.nnz + 1);
     if (nagg > 0) {
-        k_rap_row<true, false, kRapSmall><<<nagg, kWave, 0, st>>>(nagg, nullptr, rptr, rmem, A.ia, A.ja, A.val, agg,
-                                                                   nullptr, nullptr, Ac.ia, Ac.ja, Ac.val, nullptr,
-                                                                   nullptr, nullptr, nullptr, nullptr, nullptr);
+        constexpr int kGroup = 16;
+        k_rap_compact<kRapSmall, kGroup><<<div_up((long)nagg * kGroup, kBlock), kBlock, 0, st>>>(
+            nagg, rowlen, Ac.ia, tcol, tval, Ac.ja, Ac.val);
         HZ_LAUNCH_CHECK();
     }
     if (n1 > 0) {
