@@ -243,9 +243,10 @@ int aggregate_hem_staged(const DevCSR &N, int *agg_dev, int *nagg) {
             start = acc;
         }
     }
-    int *v = static_cast<int *>(g_v.get(sizeof(int) * (size_t)n));
-    std::fill(v, v + n, -1);
-    // the sweep's working set: one byte per vertex (matched), cache resident
+    // the sweep appends its pairs (initiator, partner) in order, the GPU
+    // turns them into the aggregate map; its working set is one byte per
+    // vertex (matched), cache resident
+    int *pairs = static_cast<int *>(g_v.get(sizeof(int) * ((size_t)n + 2)));
     static std::vector<unsigned char> matched;
     matched.assign(n, 0);
     unsigned char *mt = matched.data();
@@ -269,22 +270,22 @@ int aggregate_hem_staged(const DevCSR &N, int *agg_dev, int *nagg) {
             if (pick >= 0) {
                 mt[k] = 1;
                 mt[pick] = 1;
-                v[k] = nc;
-                v[pick] = nc;
+                pairs[2 * nc] = k;
+                pairs[2 * nc + 1] = pick;
                 ++nc;
             } else {
                 left.push_back(k);
             }
         }
     }
-    HZ_CHECK(hipMemcpyAsync(agg_dev, v, sizeof(int) * n, hipMemcpyHostToDevice, st));
+    hem_agg_from_pairs(pairs, nc, n, agg_dev);
     int rounds = 0;
     const int nl = (int)left.size();
     if (nl > 0) {
         const int p2 = hem_pass2_from_host(N, agg_dev, nc, left.data(), nl, &rounds);
         if (p2 != HZ_SUCCESS) return p2;
     } else {
-        HZ_CHECK(hipStreamSynchronize(st));   // v is reused by the next level
+        HZ_CHECK(hipStreamSynchronize(st));   // the pairs buffer is reused by the next level
     }
     *nagg = nc;
     if (trace_enabled())

@@ -113,6 +113,7 @@ struct HemPrefLists {
     int *info = nullptr, *pp = nullptr, *cols = nullptr, *flags = nullptr;
 };
 void hem_pref_lists(const DevCSR &N, HemPrefLists &out);
+void hem_agg_from_pairs(const int *pairs_host, int nc, int n, int *agg);
 int hem_pass2_from_host(const DevCSR &N, int *agg_dev, int nc, const int *list_host, int nl, int *rounds);
 
 }  // namespace hz

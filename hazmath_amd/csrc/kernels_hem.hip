@@ -137,6 +137,13 @@ __global__ void k_pref_fill(int n, const int *__restrict__ ia, const int *__rest
     }
 }
 
+__global__ void k_pairs_to_agg(int nc, const int *__restrict__ pairs, int *__restrict__ agg) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x) {
+        agg[pairs[2 * c]] = c;
+        agg[pairs[2 * c + 1]] = c;
+    }
+}
+
 __global__ void k_scatter_one(int nl, const int *__restrict__ list, int *__restrict__ flag) {
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nl; t += gridDim.x * blockDim.x) flag[list[t]] = 1;
 }
@@ -590,6 +597,18 @@ void hem_pref_lists(const DevCSR &N, HemPrefLists &out) {
     exclusive_scan_int_device(cnt, out.pp, n);
     out.cols = arena().alloc_n<int>((size_t)N.nnz + 1);
     k_pref_fill<<<g, kBlock, 0, st>>>(n, N.ia, N.ja, N.val, out.pp, out.cols);
+    HZ_LAUNCH_CHECK();
+}
+
+// Aggregate map of a host pass 1 from its nc pairs (host array, pair c =
+// aggregate c): -1 elsewhere.  Queued on the stream; pairs_host must stay
+// untouched until the stream has passed this point.
+void hem_agg_from_pairs(const int *pairs_host, int nc, int n, int *agg) {
+    hipStream_t st = stream();
+    int *pd = arena().alloc_n<int>(2 * (size_t)nc + 2);
+    if (nc > 0) HZ_CHECK(hipMemcpyAsync(pd, pairs_host, sizeof(int) * 2 * (size_t)nc, hipMemcpyHostToDevice, st));
+    k_fill_i<<<grid_for(n), kBlock, 0, st>>>(agg, -1, n);
+    if (nc > 0) k_pairs_to_agg<<<grid_for(nc), kBlock, 0, st>>>(nc, pd, agg);
     HZ_LAUNCH_CHECK();
 }
 
