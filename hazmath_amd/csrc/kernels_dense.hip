@@ -98,37 +98,48 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const double 
 }
 
 // In-place Gauss-Jordan inverse of the b x b block at A[k0][k0] (ld n) into P.
-__global__ __launch_bounds__(256) void k_block_inverse(int n, int k0, int b, const double *__restrict__ A,
-                                                       double *__restrict__ P, int *__restrict__ status) {
+// 1024 threads: thread (i, t) owns row i, columns t, t+16, t+32, t+48.  Per
+// pivot k: the column is saved, the pivot row scaled (s[k][k] = 1/piv), then
+// the rank-1 update of the other rows together with column k.
+__global__ __launch_bounds__(1024) void k_block_inverse(int n, int k0, int b, const double *__restrict__ A,
+                                                        double *__restrict__ P, int *__restrict__ status) {
+    static_assert(kTile == 64, "thread layout assumes 64 x 64 blocks");
+    constexpr int kCols = 16;
     __shared__ double s[kTile][kTile + 1];
     __shared__ double colk[kTile];
     const int tid = threadIdx.x;
-    for (int idx = tid; idx < b * b; idx += 256) s[idx / b][idx % b] = A[(size_t)(k0 + idx / b) * n + k0 + idx % b];
+    const int ti = tid / kCols, tj = tid % kCols;
+    if (ti < b)
+        for (int q = 0; q < kTile / kCols; ++q) {
+            const int j = tj + kCols * q;
+            if (j < b) s[ti][j] = A[(size_t)(k0 + ti) * n + k0 + j];
+        }
     __syncthreads();
     for (int k = 0; k < b; ++k) {
         const double piv = s[k][k];
-        if (piv == 0.0) {
+        if (piv == 0.0) {   // the same value for every thread: all leave together
             if (tid == 0) status[0] = 1;
             return;
         }
-        for (int i = tid; i < b; i += 256) colk[i] = s[i][k];
+        if (tid < b) colk[tid] = s[tid][k];
         __syncthreads();
-        for (int j = tid; j < b; j += 256)
-            if (j != k) s[k][j] = s[k][j] / piv;
+        if (tid < b) s[k][tid] = tid == k ? 1.0 / piv : s[k][tid] / piv;
         __syncthreads();
-        if (tid == 0) s[k][k] = 1.0 / piv;
-        __syncthreads();
-        for (int idx = tid; idx < b * b; idx += 256) {
-            const int i = idx / b, j = idx % b;
-            if (i == k) continue;
-            if (j != k) s[i][j] -= colk[i] * s[k][j];
+        if (ti < b && ti != k) {
+            const double ci = colk[ti];
+            for (int q = 0; q < kTile / kCols; ++q) {
+                const int j = tj + kCols * q;
+                if (j < b && j != k) s[ti][j] -= ci * s[k][j];
+            }
+            if (tj == 0) s[ti][k] = -ci * s[k][k];
         }
         __syncthreads();
-        for (int i = tid; i < b; i += 256)
-            if (i != k) s[i][k] = -colk[i] * s[k][k];
-        __syncthreads();
     }
-    for (int idx = tid; idx < b * b; idx += 256) P[idx] = s[idx / b][idx % b];
+    if (ti < b)
+        for (int q = 0; q < kTile / kCols; ++q) {
+            const int j = tj + kCols * q;
+            if (j < b) P[ti * b + j] = s[ti][j];
+        }
 }
 
 // Copy a rows x cols block between row-major buffers.
@@ -346,7 +357,7 @@ int blocked_inverse(int n, double *M) {
     for (int k0 = 0; k0 < n; k0 += kTile) {
         const int b = std::min(kTile, n - k0);
         const int below = n - k0 - b;
-        k_block_inverse<<<1, 256, 0, stream()>>>(n, k0, b, M, Pinv, status);
+        k_block_inverse<<<1, 1024, 0, stream()>>>(n, k0, b, M, Pinv, status);
         HZ_LAUNCH_CHECK();
         launch_gemm(mfma, b, n, b, Pinv, b, M + (size_t)k0 * n, n, R, n, 1.0, 0.0);           // R = P^-1 A_K,:
         k_copy_block<<<g, 256, 0, stream()>>>(n, b, M + k0, n, Cb, b, 1.0);                   // Cb = A_:,K
