@@ -38,48 +38,49 @@ __global__ void k_first_diag(int n, const int *__restrict__ ia, const int *__res
     }
 }
 
+// Rows by groups of kStrLanes lanes: a round's entries sit on consecutive
+// lanes (coalesced loads), kept entries are ranked with a ballot and written
+// in stored order.  count pass (kFill false), then fill at nia.
+constexpr int kStrLanes = 16;
 template <bool kFill>
-__global__ void k_strength(int n, const int *__restrict__ ia, const int *__restrict__ ja,
-                           const double *__restrict__ val, const double *__restrict__ d, double th2c,
-                           int *__restrict__ count, const int *__restrict__ nia,
-                           int *__restrict__ nja, double *__restrict__ nval,
-                           const double *__restrict__ th2_row) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const double di = d[i];
-        const double th2 = th2_row ? th2_row[i] : th2c;
-        int pos = kFill ? nia[i] : 0;
-        int cnt = 0;
-        // entries in rounds of 8: column, value and d[j] loads issued first
-        constexpr int U = 8;
-        const int k0 = ia[i], k1 = ia[i + 1];
-        for (int k = k0; k < k1; k += U) {
-            int jj[U];
-            double aa[U], dj[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const bool ok = k + u < k1;
-                jj[u] = ok ? ja[k + u] : -1;
-                aa[u] = ok ? val[k + u] : 0.0;
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u) dj[u] = jj[u] >= 0 ? d[jj[u]] : 0.0;
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                if (jj[u] < 0) continue;
-                const int j = jj[u];
-                const double a = aa[u];
-                if (j == i || ((a * a) >= th2 * fabs(di * dj[u]) && a < 0e0)) {
-                    if (kFill) {
-                        nja[pos] = j;
-                        nval[pos] = a;
-                        ++pos;
-                    }
-                    ++cnt;
-                }
-            }
+__global__ __launch_bounds__(kBlock) void k_strength_grp(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                                                         const double *__restrict__ val, const double *__restrict__ d,
+                                                         double th2c, int *__restrict__ count,
+                                                         const int *__restrict__ nia, int *__restrict__ nja,
+                                                         double *__restrict__ nval, const double *__restrict__ th2_row) {
+    const long gt = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = (int)(gt / kStrLanes), j = (int)(gt % kStrLanes);
+    const int lane = threadIdx.x % kWave, gbase = lane - j;
+    if (i >= n) return;   // the whole group
+    const double di = d[i];
+    const double th2 = th2_row ? th2_row[i] : th2c;
+    const int k0 = ia[i], k1 = ia[i + 1];
+    const unsigned long long gmask = ((1ull << kStrLanes) - 1ull) << gbase;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    int pos = kFill ? nia[i] : 0;
+    int cnt = 0;
+    for (int base = k0; base < k1; base += kStrLanes) {   // trip count uniform within the group
+        const int k = base + j;
+        bool keep = false;
+        int col = -1;
+        double a = 0.0;
+        if (k < k1) {
+            col = ja[k];
+            a = val[k];
+            const double dj = d[col];
+            keep = col == i || ((a * a) >= th2 * fabs(di * dj) && a < 0e0);
         }
-        if (!kFill) count[i] = cnt;
+        const unsigned long long m = __ballot(keep) & gmask;
+        if (kFill && keep) {
+            const int r = __popcll(m & below);
+            nja[pos + r] = col;
+            nval[pos + r] = a;
+        }
+        const int c = __popcll(m);
+        pos += c;
+        cnt += c;
     }
+    if (!kFill && j == 0) count[i] = cnt;
 }
 
 // ---------------------------------------------------------------------------
@@ -405,8 +406,9 @@ void strength_impl(const DevCSR &A, double th2, const double *th2_row, DevCSR &N
     double *d = arena().alloc_n<double>((size_t)n + 1);
     k_first_diag<<<grid_for(n), kBlock, 0, st>>>(n, A.ia, A.ja, A.val, d);
     HZ_LAUNCH_CHECK();
-    k_strength<false><<<grid_for(n), kBlock, 0, st>>>(n, A.ia, A.ja, A.val, d, th2, counts_ws,
-                                                      nullptr, nullptr, nullptr, th2_row);
+    const int gg = div_up((long)n * kStrLanes, kBlock);
+    k_strength_grp<false><<<gg, kBlock, 0, st>>>(n, A.ia, A.ja, A.val, d, th2, counts_ws, nullptr, nullptr, nullptr,
+                                                 th2_row);
     HZ_LAUNCH_CHECK();
     N.n = n;
     N.ncol = A.ncol;
@@ -414,8 +416,7 @@ void strength_impl(const DevCSR &A, double th2, const double *th2_row, DevCSR &N
     N.nnz = (int)exclusive_scan_int(counts_ws, N.ia, n);
     N.ja = arena().alloc_n<int>((size_t)N.nnz + 1);
     N.val = arena().alloc_n<double>((size_t)N.nnz + 1);
-    k_strength<true><<<grid_for(n), kBlock, 0, st>>>(n, A.ia, A.ja, A.val, d, th2, nullptr, N.ia,
-                                                     N.ja, N.val, th2_row);
+    k_strength_grp<true><<<gg, kBlock, 0, st>>>(n, A.ia, A.ja, A.val, d, th2, nullptr, N.ia, N.ja, N.val, th2_row);
     HZ_LAUNCH_CHECK();
 }
 }  // namespace
