@@ -23,6 +23,8 @@ struct DevSELL {
     int *len = nullptr;        // row lengths
     int *col = nullptr;
     double *val = nullptr;
+    double *dl1 = nullptr;     // row l1 norms (L1DIAG diagonal), formed by sell_from_csr
+    double *djac = nullptr;    // last stored diagonal entry of each row (Jacobi diagonal)
 };
 
 // Device scalars of the PCG (one 256-byte block; host copies it back once per

@@ -34,26 +34,61 @@ __global__ void k_slice_width(int n, const int *__restrict__ ia, long long *__re
     if (lane == 0) wcount[s] = (long long)len * kSlice;
 }
 
-__global__ void k_sell_fill(int n, const int *__restrict__ ia, const int *__restrict__ ja,
-                            const double *__restrict__ val, const long long *__restrict__ sptr,
-                            int *__restrict__ scol, double *__restrict__ sval, int nslices) {
-    const int s = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+// One wavefront per slice.  The slice's rows are contiguous in the CSR arrays:
+// up to kSellStage of their entries are first loaded coalesced into LDS, then
+// each lane writes its row down the slice's columns (col -1 / 0.0 padding)
+// and forms its row's smoother diagonals on the way: the l1 norm (sum of
+// |a| from 0.0 in stored order, L1DIAG) and the last stored diagonal entry
+// (Jacobi and the rest) -- what k_smoother_diag computes from the SELL copy.
+constexpr int kSellStage = 1024;
+__global__ __launch_bounds__(kBlock) void k_sell_fill(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                                                      const double *__restrict__ val,
+                                                      const long long *__restrict__ sptr, int *__restrict__ scol,
+                                                      double *__restrict__ sval, int nslices, double *__restrict__ dl1,
+                                                      double *__restrict__ djac) {
+    __shared__ int s_col[kSlicesPerBlock][kSellStage];
+    __shared__ double s_val[kSlicesPerBlock][kSellStage];
+    const int wv = threadIdx.x / kWave;
+    const int s = blockIdx.x * kSlicesPerBlock + wv;
     const int lane = threadIdx.x % kWave;
-    if (s >= nslices) return;
+    const bool active = s < nslices;
     const int r = s * kSlice + lane;
+    int e0 = 0, e1 = 0;
+    if (active) {
+        e0 = ia[s * kSlice];
+        e1 = ia[min(n, (s + 1) * kSlice)];
+    }
+    const bool staged = active && e1 - e0 <= kSellStage;
+    if (staged)
+        for (int e = e0 + lane; e < e1; e += kWave) {
+            s_col[wv][e - e0] = ja[e];
+            s_val[wv][e - e0] = val[e];
+        }
+    __syncthreads();
+    if (!active) return;
     const long long base = sptr[s];
     const int w = (int)((sptr[s + 1] - base) / kSlice);
     const int beg = (r < n) ? ia[r] : 0;
     const int len = (r < n) ? ia[r + 1] - beg : 0;
+    double l1 = 0.0, dj = 0.0;
     for (int k = 0; k < w; ++k) {
         const long long e = base + (long long)k * kSlice + lane;
         if (k < len) {
-            scol[e] = ja[beg + k];
-            sval[e] = val[beg + k];
+            const int q = beg + k;
+            const int j = staged ? s_col[wv][q - e0] : ja[q];
+            const double a = staged ? s_val[wv][q - e0] : val[q];
+            scol[e] = j;
+            sval[e] = a;
+            l1 = l1 + fabs(a);
+            if (j == r) dj = a;
         } else {
             scol[e] = -1;
             sval[e] = 0.0;
         }
+    }
+    if (r < n) {
+        dl1[r] = l1;
+        djac[r] = dj;
     }
 }
 
@@ -586,9 +621,11 @@ void sell_from_csr(const DevCSR &A, DevSELL &S) {
     S.nent = nent;
     S.col = arena().alloc_n<int>((size_t)nent + 1);
     S.val = arena().alloc_n<double>((size_t)nent + 1);
+    S.dl1 = arena().alloc_n<double>((size_t)A.n + 1);
+    S.djac = arena().alloc_n<double>((size_t)A.n + 1);
     if (ns > 0) {
-        k_sell_fill<<<div_up(ns, kSlicesPerBlock), kBlock, 0, st>>>(A.n, A.ia, A.ja, A.val, S.sptr,
-                                                                  S.col, S.val, ns);
+        k_sell_fill<<<div_up(ns, kSlicesPerBlock), kBlock, 0, st>>>(A.n, A.ia, A.ja, A.val, S.sptr, S.col, S.val,
+                                                                  ns, S.dl1, S.djac);
         HZ_LAUNCH_CHECK();
     }
 }
@@ -650,6 +687,11 @@ void smooth_sweep_dot(const DevSELL &S, int smoother, double w, const double *b,
 
 void smoother_diag(const DevSELL &S, int smoother, double *d) {
     if (S.n == 0) return;
+    if (S.dl1) {   // formed with the SELL copy
+        HZ_CHECK(hipMemcpyAsync(d, smoother == HZ_SMOOTHER_L1DIAG ? S.dl1 : S.djac, sizeof(double) * S.n,
+                                hipMemcpyDeviceToDevice, stream()));
+        return;
+    }
     const int grid = div_up(S.nslices, kSlicesPerBlock);
     if (smoother == HZ_SMOOTHER_L1DIAG)
         k_smoother_diag<HZ_SMOOTHER_L1DIAG><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr,
