@@ -166,7 +166,7 @@ int amg_setup(Hierarchy &H, const DevCSR &A0, AMG_param *param, bool) {
         double t0 = 0, t1 = 0, t2 = 0, t3 = 0;
         if (tr) { HZ_CHECK(hipStreamSynchronize(stream())); t0 = wall_ms(); }
         DevCSR N;
-        strength_graph(L.A, param->strong_coupled, N, counts);
+        strength_graph(L.A, param->strong_coupled, N, counts, L.S.dfirst);
         if (tr) { HZ_CHECK(hipStreamSynchronize(stream())); t1 = wall_ms(); }
         L.agg = arena().alloc_n<int>((size_t)n + 1);
         int nagg = 0, st = HZ_SUCCESS;

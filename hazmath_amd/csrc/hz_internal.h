@@ -25,6 +25,7 @@ struct DevSELL {
     double *val = nullptr;
     double *dl1 = nullptr;     // row l1 norms (L1DIAG diagonal), formed by sell_from_csr
     double *djac = nullptr;    // last stored diagonal entry of each row (Jacobi diagonal)
+    double *dfirst = nullptr;  // first stored diagonal entry of each row (0.0 if none)
 };
 
 // Device scalars of the PCG (one 256-byte block; host copies it back once per
@@ -132,7 +133,8 @@ void coarse_scaling_alpha(const DevSELL &S, const double *x, const double *b, Pc
 
 // ---- kernels_setup.hip ----------------------------------------------------
 // Strength graph N (construct_strongly_coupled, amg_setup_ua.c:375-452).
-void strength_graph(const DevCSR &A, double theta, DevCSR &N, int *counts_ws);
+// dfirst (optional): the rows' first stored diagonal entries, when already formed.
+void strength_graph(const DevCSR &A, double theta, DevCSR &N, int *counts_ws, const double *dfirst = nullptr);
 // Exclusive scan of n ints into out (n+1 entries, out[n] = total); returns total.
 long long exclusive_scan_int(const int *in, int *out, int n);
 // The same scan left on the stream (no host wait); the total stays in out[n].

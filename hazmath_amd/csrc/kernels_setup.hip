@@ -400,12 +400,17 @@ long long exclusive_scan_int(const int *in, int *out, int n) {
 }
 
 namespace {
-void strength_impl(const DevCSR &A, double th2, const double *th2_row, DevCSR &N, int *counts_ws) {
+void strength_impl(const DevCSR &A, double th2, const double *th2_row, DevCSR &N, int *counts_ws,
+                   const double *dfirst) {
     hipStream_t st = stream();
     const int n = A.n;
-    double *d = arena().alloc_n<double>((size_t)n + 1);
-    k_first_diag<<<grid_for(n), kBlock, 0, st>>>(n, A.ia, A.ja, A.val, d);
-    HZ_LAUNCH_CHECK();
+    const double *d = dfirst;
+    if (!d) {
+        double *dd = arena().alloc_n<double>((size_t)n + 1);
+        k_first_diag<<<grid_for(n), kBlock, 0, st>>>(n, A.ia, A.ja, A.val, dd);
+        HZ_LAUNCH_CHECK();
+        d = dd;
+    }
     const int gg = div_up((long)n * kStrLanes, kBlock);
     k_strength_grp<false><<<gg, kBlock, 0, st>>>(n, A.ia, A.ja, A.val, d, th2, counts_ws, nullptr, nullptr, nullptr,
                                                  th2_row);
@@ -421,12 +426,12 @@ void strength_impl(const DevCSR &A, double th2, const double *th2_row, DevCSR &N
 }
 }  // namespace
 
-void strength_graph(const DevCSR &A, double theta, DevCSR &N, int *counts_ws) {
-    strength_impl(A, theta * theta, nullptr, N, counts_ws);
+void strength_graph(const DevCSR &A, double theta, DevCSR &N, int *counts_ws, const double *dfirst) {
+    strength_impl(A, theta * theta, nullptr, N, counts_ws, dfirst);
 }
 
 void strength_graph_rows(const DevCSR &A, const double *th2_row, DevCSR &N, int *counts_ws) {
-    strength_impl(A, 0.0, th2_row, N, counts_ws);
+    strength_impl(A, 0.0, th2_row, N, counts_ws, nullptr);
 }
 
 int hec_aggregate(const DevCSR &N, const int *perm, int *agg) {

@@ -37,15 +37,16 @@ __global__ void k_slice_width(int n, const int *__restrict__ ia, long long *__re
 // One wavefront per slice.  The slice's rows are contiguous in the CSR arrays:
 // up to kSellStage of their entries are first loaded coalesced into LDS, then
 // each lane writes its row down the slice's columns (col -1 / 0.0 padding)
-// and forms its row's smoother diagonals on the way: the l1 norm (sum of
-// |a| from 0.0 in stored order, L1DIAG) and the last stored diagonal entry
-// (Jacobi and the rest) -- what k_smoother_diag computes from the SELL copy.
+// and forms its row's diagonals on the way: the l1 norm (sum of |a| from 0.0
+// in stored order, L1DIAG) and the last stored diagonal entry (Jacobi and the
+// rest) -- what k_smoother_diag computes from the SELL copy -- and the first
+// stored diagonal entry (dcsr_getdiag, which the strength graph reads).
 constexpr int kSellStage = 1024;
 __global__ __launch_bounds__(kBlock) void k_sell_fill(int n, const int *__restrict__ ia, const int *__restrict__ ja,
                                                       const double *__restrict__ val,
                                                       const long long *__restrict__ sptr, int *__restrict__ scol,
                                                       double *__restrict__ sval, int nslices, double *__restrict__ dl1,
-                                                      double *__restrict__ djac) {
+                                                      double *__restrict__ djac, double *__restrict__ dfirst) {
     __shared__ int s_col[kSlicesPerBlock][kSellStage];
     __shared__ double s_val[kSlicesPerBlock][kSellStage];
     const int wv = threadIdx.x / kWave;
@@ -70,7 +71,8 @@ __global__ __launch_bounds__(kBlock) void k_sell_fill(int n, const int *__restri
     const int w = (int)((sptr[s + 1] - base) / kSlice);
     const int beg = (r < n) ? ia[r] : 0;
     const int len = (r < n) ? ia[r + 1] - beg : 0;
-    double l1 = 0.0, dj = 0.0;
+    double l1 = 0.0, dj = 0.0, df = 0.0;
+    bool seen = false;
     for (int k = 0; k < w; ++k) {
         const long long e = base + (long long)k * kSlice + lane;
         if (k < len) {
@@ -80,7 +82,11 @@ __global__ __launch_bounds__(kBlock) void k_sell_fill(int n, const int *__restri
             scol[e] = j;
             sval[e] = a;
             l1 = l1 + fabs(a);
-            if (j == r) dj = a;
+            if (j == r) {
+                dj = a;
+                if (!seen) df = a;
+                seen = true;
+            }
         } else {
             scol[e] = -1;
             sval[e] = 0.0;
@@ -89,6 +95,7 @@ __global__ __launch_bounds__(kBlock) void k_sell_fill(int n, const int *__restri
     if (r < n) {
         dl1[r] = l1;
         djac[r] = dj;
+        dfirst[r] = df;
     }
 }
 
@@ -623,9 +630,10 @@ void sell_from_csr(const DevCSR &A, DevSELL &S) {
     S.val = arena().alloc_n<double>((size_t)nent + 1);
     S.dl1 = arena().alloc_n<double>((size_t)A.n + 1);
     S.djac = arena().alloc_n<double>((size_t)A.n + 1);
+    S.dfirst = arena().alloc_n<double>((size_t)A.n + 1);
     if (ns > 0) {
         k_sell_fill<<<div_up(ns, kSlicesPerBlock), kBlock, 0, st>>>(A.n, A.ia, A.ja, A.val, S.sptr, S.col, S.val,
-                                                                  ns, S.dl1, S.djac);
+                                                                  ns, S.dl1, S.djac, S.dfirst);
         HZ_LAUNCH_CHECK();
     }
 }
