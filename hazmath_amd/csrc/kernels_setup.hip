@@ -512,12 +512,28 @@ void rap_agg(const DevCSR &A, const int *agg, int nagg, const int *rptr, const i
     // kRapSmall entries and moved to their CSR places after the scan
     int *tcol = arena().alloc_n<int>((size_t)nagg * kRapSmall + 1);
     double *tval = arena().alloc_n<double>((size_t)nagg * kRapSmall + 1);
+    Ac.n = nagg;
+    Ac.ncol = nagg;
+    Ac.ia = arena().alloc_n<int>((size_t)nagg + 1);
+    long long total = 0;
     if (nagg > 0) {
         k_rap_row<true, false, kRapSmall, true><<<nagg, kWave, 0, st>>>(
             nagg, nullptr, rptr, rmem, A.ia, A.ja, A.val, agg, rowcount, rowlen, nullptr, tcol, tval, nullptr,
             nullptr, nullptr, nullptr, nullptr, nullptr);
         HZ_LAUNCH_CHECK();
-        n1 = overflow();
+        HZ_CHECK(hipMemsetAsync(nol, 0, sizeof(int), st));
+        k_collect_overflow<<<grid_for(nagg), kBlock, 0, st>>>(nagg, rowcount, olist, nol);
+        HZ_LAUNCH_CHECK();
+        // the row pointers are final unless a row overflowed: one host wait for both
+        exclusive_scan_int_device(rowcount, Ac.ia, nagg);
+        int h[2] = {0, 0};
+        HZ_CHECK(hipMemcpyAsync(&h[0], nol, sizeof(int), hipMemcpyDeviceToHost, st));
+        HZ_CHECK(hipMemcpyAsync(&h[1], Ac.ia + nagg, sizeof(int), hipMemcpyDeviceToHost, st));
+        HZ_CHECK(hipStreamSynchronize(st));
+        n1 = h[0];
+        total = h[1];
+    } else {
+        HZ_CHECK(hipMemsetAsync(Ac.ia, 0, sizeof(int), st));
     }
     if (n1 > 0) {
         list1 = arena().alloc_n<int>((size_t)n1 + 1);
@@ -562,10 +578,8 @@ void rap_agg(const DevCSR &A, const int *agg, int nagg, const int *rptr, const i
                                                          gminp, hoffd);
         HZ_LAUNCH_CHECK();
     }
-    Ac.n = nagg;
-    Ac.ncol = nagg;
-    Ac.ia = arena().alloc_n<int>((size_t)nagg + 1);
-    Ac.nnz = (int)exclusive_scan_int(rowcount, Ac.ia, nagg);
+    if (n1 > 0) total = exclusive_scan_int(rowcount, Ac.ia, nagg);
+    Ac.nnz = (int)total;
     Ac.ja = arena().alloc_n<int>((size_t)Ac.nnz + 1);
     Ac.val = arena().alloc_n<double>((size_t)Ac.nnz + 1);
     if (nagg > 0) {
