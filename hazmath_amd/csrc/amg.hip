@@ -121,13 +121,14 @@ int hem_host_max_rows() {
     return v;
 }
 
-namespace {
 // HEM for one level.  Small levels copy the whole strength graph and sweep it
 // on the host (two transfers, nothing else to wait for); mid-size levels copy
 // only GPU-sorted preference lists; the largest run the GPU pipeline, whose
 // cost follows the length of the sweep's dependency chains rather than n.
+namespace {
 constexpr int kHemFullHostRows = 32768;
-int hem_level(const DevCSR &N, AMG_param *param, int *agg, int *nagg) {
+}  // namespace
+int hem_level(const DevCSR &N, const AMG_param *param, int *agg, int *nagg) {
     const int n = N.n;
     int st;
     if (g_hem_host_max == -1) st = hem_aggregate_gpu(N, agg, nagg);
@@ -141,7 +142,6 @@ int hem_level(const DevCSR &N, AMG_param *param, int *agg, int *nagg) {
     if (st == 1 || st == -1017) st = aggregate_host(N, HZ_HEM, param->max_aggregation, agg, nagg);
     return st;
 }
-}  // namespace
 
 int amg_setup(Hierarchy &H, const DevCSR &A0, AMG_param *param, bool) {
     const short min_cdof = (short)std::max(param->coarse_dof, 1);

@@ -104,6 +104,11 @@ int hem_host_max_rows();
 // symmetric (then the host sweep is used).
 int hem_aggregate_gpu(const DevCSR &N, int *agg, int *nagg_out);
 
+// HEM aggregation of one level in the form its size calls for (full host
+// sweep, preference lists, or the GPU pipeline; hz_options.hem_host_max_rows):
+// SUCCESS or -1017.
+int hem_level(const DevCSR &N, const AMG_param *param, int *agg, int *nagg);
+
 // HEM pass 1 on the host over GPU-sorted preference lists, pass 2 on the GPU:
 // SUCCESS, -1017, or 1 when the pattern needs the full host sweep.
 int aggregate_hem_staged(const DevCSR &N, int *agg_dev, int *nagg);

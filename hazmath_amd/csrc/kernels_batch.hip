@@ -969,8 +969,7 @@ int batch_setup(BatchHierarchy &H, const DevCSR &A0, const std::vector<int> &off
                 HZ_CHECK(hipStreamSynchronize(st));
                 t0 = wall_ms();
             }
-            int sth = n <= hem_host_max_rows() ? 1 : hem_aggregate_gpu(N, L.agg, &nagg);
-            if (sth == 1) sth = aggregate_host(N, HZ_HEM, param->max_aggregation, L.agg, &nagg);
+            int sth = hem_level(N, param, L.agg, &nagg);
             if (tr) {
                 HZ_CHECK(hipStreamSynchronize(st));
                 std::fprintf(stderr, "[hz-batch] level %d n=%d nnz=%d blocks=%d: aggregate %.2f ms\n", lvl, n,
