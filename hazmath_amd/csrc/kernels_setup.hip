@@ -197,10 +197,11 @@ __global__ void k_agg_count(int n, const int *__restrict__ agg, int *__restrict_
 // then one slot per other column in order of first appearance, whose value is
 // the in-order sum of its candidates starting from the first one.
 // ---------------------------------------------------------------------------
-// Candidates held in LDS per row: rows up to kRapSmall candidates run in a
-// small-LDS kernel (high occupancy; HEM pairs of 15-point rows have ~30), rows
-// up to kRapCap in a large-LDS one, longer rows in global scratch.  The hash
-// table has twice the slots of the candidate capacity.
+// Rows up to kRapSmall candidates (HEM pairs of 15-point rows have ~30) are
+// built in one pass, two per wavefront, into fixed slots (k_rap_pair) and
+// moved to their CSR places after the scan; longer rows up to kRapCap run in
+// a large-LDS wavefront per row (count, then fill), longer ones in global
+// scratch.  The hash tables have twice the slots of the candidate capacity.
 constexpr int kRapSmall = 64;
 constexpr int kRapCap = 512;
 
@@ -216,10 +217,7 @@ __device__ __forceinline__ unsigned rap_find(const int *keys, int key, unsigned 
     return slot;
 }
 
-// kTemp (with kFill): one pass -- the row goes to fixed-capacity slots
-// cja/cval + c * kCap (moved to its CSR place by k_rap_compact after the
-// scan) and its count and candidate length to rowcount/rowlen.
-template <bool kFill, bool kGlobal, int kCap, bool kTemp = false>
+template <bool kFill, bool kGlobal, int kCap>
 __global__ __launch_bounds__(kWave) void k_rap_row(
     int nrows, const int *__restrict__ rowlist, const int *__restrict__ rptr,
     const int *__restrict__ rmem, const int *__restrict__ aia, const int *__restrict__ aja,
@@ -257,10 +255,10 @@ __global__ __launch_bounds__(kWave) void k_rap_row(
         L += len;
     }
     if (!kGlobal && L > kCap) {   // handled by a larger variant
-        if ((!kFill || kTemp) && lane == 0) { rowcount[c] = -1; rowlen[c] = L; }
+        if (!kFill && lane == 0) { rowcount[c] = -1; rowlen[c] = L; }
         return;
     }
-    const long long obase = kTemp ? (long long)c * kCap : (kFill ? (long long)cia[c] : 0);
+    const long long obase = kFill ? (long long)cia[c] : 0;
     // 2) hash table of distinct coarse columns -> first position (LDS
     //    variants: also the last position, and the diagonal column c)
     unsigned mask;
@@ -337,10 +335,125 @@ __global__ __launch_bounds__(kWave) void k_rap_row(
             cja[obase] = c;
             cval[obase] = 0.0;
         }
-        if (!kFill || kTemp) {
+        if (!kFill) {
             rowcount[c] = 1 + nfirst_before;
             if (!kGlobal) rowlen[c] = L;
         }
+    }
+}
+
+// Rows of at most kRapSmall candidates in one pass, two coarse rows per
+// wavefront (32 lanes and their own LDS tables each): these rows are short
+// and a wavefront's time is its chain of dependent accesses, so two rows
+// share it.  Same candidate order, hash, ranks and in-order sums as
+// k_rap_row; each row goes to its slots tcol/tval + c * kRapSmall (moved to
+// its CSR place by k_rap_compact after the scan) and its count and candidate
+// length to rowcount/rowlen (-1 and the length when it is longer).
+__global__ __launch_bounds__(kWave) void k_rap_pair(int nrows, const int *__restrict__ rptr,
+                                                    const int *__restrict__ rmem, const int *__restrict__ aia,
+                                                    const int *__restrict__ aja, const double *__restrict__ aval,
+                                                    const int *__restrict__ agg, int *__restrict__ rowcount,
+                                                    int *__restrict__ rowlen, int *__restrict__ tcol,
+                                                    double *__restrict__ tval) {
+    constexpr int kH = kWave / 2, kCap = kRapSmall, kHash = 2 * kRapSmall;
+    __shared__ int s_col[2][kCap];
+    __shared__ double s_val[2][kCap];
+    __shared__ int s_keys[2][kHash], s_minp[2][kHash], s_maxp[2][kHash];
+    __shared__ int s_diag[2];
+    const int half = threadIdx.x / kH, hl = threadIdx.x % kH;
+    const int c = blockIdx.x * 2 + half;
+    const bool valid = c < nrows;
+    int *col = s_col[half];
+    double *val = s_val[half];
+    int *keys = s_keys[half], *minp = s_minp[half], *maxp = s_maxp[half];
+    // 1) enumerate candidates (members ascending x stored row order)
+    int L = 0;
+    if (valid) {
+        const int q1 = rptr[c + 1];
+        for (int q = rptr[c]; q < q1; ++q) {
+            const int i1 = rmem[q];
+            const int b = aia[i1], len = aia[i1 + 1] - b;
+            for (int k = hl; k < len; k += kH) {
+                const int p = L + k;
+                if (p < kCap) {
+                    col[p] = agg[aja[b + k]];
+                    val[p] = aval[b + k];
+                }
+            }
+            L += len;
+        }
+    }
+    const bool over = L > kCap;   // handled by the larger variants
+    if (valid && over && hl == 0) {
+        rowcount[c] = -1;
+        rowlen[c] = L;
+    }
+    const bool act = valid && !over;
+    const int La = act ? L : 0;
+    const long long obase = (long long)c * kCap;
+    // 2) hash of distinct coarse columns (the diagonal included) -> first and last position
+    for (int t = hl; t < kHash; t += kH) {
+        keys[t] = -1;
+        minp[t] = INT_MAX;
+        maxp[t] = -1;
+    }
+    if (hl == 0) s_diag[half] = 0;
+    __syncthreads();
+    const unsigned mask = kHash - 1u;
+    for (int p = hl; p < La; p += kH) {
+        const int key = col[p];
+        if (key <= HZ_UNPT) continue;
+        unsigned slot = rap_hash(key, mask);
+        while (true) {
+            const int prev = atomicCAS(&keys[slot], -1, key);
+            if (prev == -1 || prev == key) break;
+            slot = (slot + 1) & mask;
+        }
+        atomicMin(&minp[slot], p);
+        atomicMax(&maxp[slot], p);
+    }
+    __syncthreads();
+    // 3) first appearances in rounds of kH (trip count uniform over the wavefront)
+    int lw = La;
+    lw = max(lw, __shfl_xor(lw, kH, kWave));
+    int nfirst_before = 0;
+    const unsigned long long hmask = 0xffffffffull << (half * kH);
+    for (int base = 0; base < lw; base += kH) {
+        const int p = base + hl;
+        bool first = false, dfirst = false;
+        int cp = -1, last = -1;
+        if (p < La) {
+            cp = col[p];
+            if (cp > HZ_UNPT) {
+                const unsigned slot = rap_find(keys, cp, mask);
+                const bool f = minp[slot] == p;
+                last = maxp[slot];
+                first = f && cp != c;
+                dfirst = f && cp == c;
+            }
+        }
+        const unsigned long long m = __ballot(first) & hmask;
+        if (first || dfirst) {
+            double v = dfirst ? 0.0 + val[p] : val[p];
+            for (int q = p + 1; q <= last; ++q)
+                if (col[q] == cp) v = v + val[q];
+            const long long pos = dfirst ? obase
+                                         : obase + 1 + nfirst_before +
+                                               __popcll(m & ((1ull << (half * kH + hl)) - 1ull) & hmask);
+            tcol[pos] = cp;
+            tval[pos] = v;
+            if (dfirst) s_diag[half] = 1;
+        }
+        nfirst_before += __popcll(m);
+    }
+    __syncthreads();
+    if (act && hl == 0) {
+        if (!s_diag[half]) {   // no candidate in column c
+            tcol[obase] = c;
+            tval[obase] = 0.0;
+        }
+        rowcount[c] = 1 + nfirst_before;
+        rowlen[c] = L;
     }
 }
 
@@ -517,9 +630,8 @@ void rap_agg(const DevCSR &A, const int *agg, int nagg, const int *rptr, const i
     Ac.ia = arena().alloc_n<int>((size_t)nagg + 1);
     long long total = 0;
     if (nagg > 0) {
-        k_rap_row<true, false, kRapSmall, true><<<nagg, kWave, 0, st>>>(
-            nagg, nullptr, rptr, rmem, A.ia, A.ja, A.val, agg, rowcount, rowlen, nullptr, tcol, tval, nullptr,
-            nullptr, nullptr, nullptr, nullptr, nullptr);
+        k_rap_pair<<<div_up(nagg, 2), kWave, 0, st>>>(nagg, rptr, rmem, A.ia, A.ja, A.val, agg, rowcount, rowlen,
+                                                      tcol, tval);
         HZ_LAUNCH_CHECK();
         HZ_CHECK(hipMemsetAsync(nol, 0, sizeof(int), st));
         k_collect_overflow<<<grid_for(nagg), kBlock, 0, st>>>(nagg, rowcount, olist, nol);
