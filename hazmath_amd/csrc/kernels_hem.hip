@@ -475,11 +475,12 @@ static int hem_pass2(const DevCSR &N, int *agg, int *size, int nc, const int *li
     int *err = flagbuf, *remaining = flagbuf + 1;
     HZ_CHECK(hipMemsetAsync(err, 0, sizeof(int), st));
     const int gl = grid_for(nl);
-    // a few rounds per host check: a round after the last decision changes
-    // nothing, and `remaining` counts the undecided leftovers of the last one
+    // one round, then a few per host check: a round after the last decision
+    // changes nothing, and `remaining` counts the undecided leftovers of the
+    // last one (most levels need a single round)
     constexpr int kRoundsPerCheck = 4;
-    for (int round = 0;; round += kRoundsPerCheck) {
-        for (int q = 0; q < kRoundsPerCheck; ++q) {
+    for (int round = 0, batch = 1;; round += batch, batch = kRoundsPerCheck) {
+        for (int q = 0; q < batch; ++q) {
             HZ_CHECK(hipMemsetAsync(remaining, 0, sizeof(int), st));
             k_left_claim<<<gl, kBlock, 0, st>>>(nl, list, done, N.ia, N.ja, agg, owner);
             k_left_decide<<<gl, kBlock, 0, st>>>(nl, list, done, N.ia, N.ja, agg, size, owner, lstate, err,
@@ -490,7 +491,7 @@ static int hem_pass2(const DevCSR &N, int *agg, int *size, int nc, const int *li
         }
         int hr[2] = {0, 0};
         HZ_CHECK(memcpy_sync(hr, err, 2 * sizeof(int), hipMemcpyDeviceToHost));
-        *rounds_out = round + kRoundsPerCheck;
+        *rounds_out = round + batch;
         if (hr[0]) return -1017;
         if (hr[1] == 0) break;
         if (round > nl + 8) return 1;
