@@ -136,10 +136,13 @@ __global__ void k_trans_fill(int nnz, const int *__restrict__ order, const int *
 // coarse row with an open-addressing table (coarse column -> output slot) in
 // global scratch, sized per row from the number of candidate products.
 // ---------------------------------------------------------------------------
-__global__ void k_rap_candidates(int nc, const int *__restrict__ ria, const int *__restrict__ rja,
-                                 const int *__restrict__ aia, const int *__restrict__ aja,
-                                 const int *__restrict__ pia, long long *__restrict__ tsize) {
-    for (int ic = blockIdx.x * blockDim.x + threadIdx.x; ic < nc; ic += gridDim.x * blockDim.x) {
+// rows (optional): the coarse rows to take, entry t of the list at tsize[t].
+__global__ void k_rap_candidates(int nrows, const int *__restrict__ rows, int nc, const int *__restrict__ ria,
+                                 const int *__restrict__ rja, const int *__restrict__ aia,
+                                 const int *__restrict__ aja, const int *__restrict__ pia,
+                                 long long *__restrict__ tsize) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < nrows; t += gridDim.x * blockDim.x) {
+        const int ic = rows ? rows[t] : t;
         long long L = 1;
         for (int q1 = ria[ic]; q1 < ria[ic + 1]; ++q1) {
             const int i1 = rja[q1];
@@ -151,7 +154,7 @@ __global__ void k_rap_candidates(int nc, const int *__restrict__ ria, const int 
         const long long bound = L < (long long)nc ? L : (long long)nc;
         long long h = 16;
         while (h < 2 * bound) h <<= 1;
-        tsize[ic] = h;
+        tsize[t] = h;
     }
 }
 
@@ -161,7 +164,8 @@ __device__ __forceinline__ unsigned long long slot_hash(int key, unsigned long l
 }
 
 template <bool kFill>
-__global__ void k_rap_rows(int r0, int r1, const long long *__restrict__ toff, long long tbase,
+__global__ void k_rap_rows(int r0, int r1, const int *__restrict__ rows, const long long *__restrict__ toff,
+                           long long tbase,
                            int *__restrict__ tkey, int *__restrict__ tpos, const int *__restrict__ ria,
                            const int *__restrict__ rja, const double *__restrict__ rval,
                            const int *__restrict__ aia, const int *__restrict__ aja,
@@ -169,12 +173,13 @@ __global__ void k_rap_rows(int r0, int r1, const long long *__restrict__ toff, l
                            const int *__restrict__ pja, const double *__restrict__ pval,
                            int *__restrict__ rowcount, const int *__restrict__ cia, int *__restrict__ cja,
                            double *__restrict__ cval) {
-    for (int ic = r0 + blockIdx.x * blockDim.x + threadIdx.x; ic < r1; ic += gridDim.x * blockDim.x) {
-        const long long t0 = toff[ic] - tbase;
-        const unsigned long long mask = (unsigned long long)(toff[ic + 1] - toff[ic]) - 1ull;
+    for (int t = r0 + blockIdx.x * blockDim.x + threadIdx.x; t < r1; t += gridDim.x * blockDim.x) {
+        const int ic = rows ? rows[t] : t;
+        const long long t0 = toff[t] - tbase;
+        const unsigned long long mask = (unsigned long long)(toff[t + 1] - toff[t]) - 1ull;
         int *key = tkey + t0;
         int *pos = tpos + t0;
-        for (unsigned long long t = 0; t <= mask; ++t) key[t] = -1;
+        for (unsigned long long u = 0; u <= mask; ++u) key[u] = -1;
         int next = kFill ? cia[ic] : 0;
         // the diagonal comes first
         {
@@ -213,6 +218,173 @@ __global__ void k_rap_rows(int r0, int r1, const long long *__restrict__ toff, l
         }
         if (!kFill) rowcount[ic] = next;
     }
+}
+
+// The same product with one wavefront per coarse row, for rows of at most
+// kRapwPairs (R entry, A entry) pairs and kRapwCols distinct columns.  The
+// pairs are staged in LDS in traversal order with the prefix of their P-row
+// lengths; the candidates (q1, q2, q3) are then taken 64 at a time in that
+// order.  A column's first lane in a round folds the round's values of that
+// column into the column's running sum in lane order, so every sum keeps the
+// sequential order; new columns take output slots in order of first
+// appearance, after the diagonal (slot 0, summed from 0.0).  The count pass
+// marks the rows it cannot take with rowcount -1 (the per-thread kernel takes
+// them); the fill pass skips them.
+constexpr int kRapwPairs = 2048;
+constexpr int kRapwCols = 512;
+constexpr int kRapwHash = 2 * kRapwCols;   // at most half full: probing ends
+
+__device__ __forceinline__ unsigned rapw_hash(int key) {
+    unsigned h = (unsigned)key * 2654435761u;
+    return (h ^ (h >> 15)) & (kRapwHash - 1u);
+}
+
+template <bool kFill>
+__global__ __launch_bounds__(kWave) void k_rapw(int nc, const int *__restrict__ ria, const int *__restrict__ rja,
+                                                const double *__restrict__ rval, const int *__restrict__ aia,
+                                                const int *__restrict__ aja, const double *__restrict__ aval,
+                                                const int *__restrict__ pia, const int *__restrict__ pja,
+                                                const double *__restrict__ pval, int *__restrict__ rowcount,
+                                                const int *__restrict__ cia, int *__restrict__ cja,
+                                                double *__restrict__ cval) {
+    __shared__ int s_pst[kRapwPairs];          // P-row start of each pair
+    __shared__ int s_off[kRapwPairs + 1];      // its P-row length, then its first candidate
+    __shared__ double s_ra[kFill ? kRapwPairs : 1];
+    __shared__ int s_key[kRapwHash], s_pos[kRapwHash], s_lead[kRapwHash];
+    __shared__ double s_val[kFill ? kRapwHash : 1];
+    __shared__ int s_rc[kWave];
+    __shared__ double s_rv[kFill ? kWave : 1];
+    const int lane = threadIdx.x;
+    const int ic = blockIdx.x;
+    if (ic >= nc) return;   // the whole (single-wavefront) block
+    if (kFill && rowcount[ic] < 0) return;
+    // 1) (R entry, A entry) pairs in traversal order
+    const int r0 = ria[ic], r1 = ria[ic + 1];
+    int np = 0;
+    bool big = false;
+    for (int b = r0; b < r1; b += kWave) {   // uniform trip count
+        const int q1 = b + lane;
+        int a0 = 0, alen = 0;
+        double r = 0.0;
+        if (q1 < r1) {
+            const int i1 = rja[q1];
+            a0 = aia[i1];
+            alen = aia[i1 + 1] - a0;
+            if (kFill) r = rval[q1];
+        }
+        int incl = alen;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int tt = __shfl_up(incl, o, kWave);
+            if (lane >= o) incl += tt;
+        }
+        const int total = __shfl(incl, kWave - 1, kWave);
+        if (np + total > kRapwPairs) {
+            big = true;
+            break;
+        }
+        const int first = np + incl - alen;
+        for (int k = 0; k < alen; ++k) {
+            const int q2 = a0 + k;
+            const int i2 = aja[q2];
+            const int ps = pia[i2];
+            s_pst[first + k] = ps;
+            s_off[first + k] = pia[i2 + 1] - ps;
+            if (kFill) s_ra[first + k] = r * aval[q2];
+        }
+        np += total;
+    }
+    __syncthreads();
+    // prefix of the P-row lengths: s_off[i] = first candidate of pair i
+    int L = 0;
+    if (!big) {
+        for (int b = 0; b < np; b += kWave) {
+            const int idx = b + lane;
+            const int len = idx < np ? s_off[idx] : 0;
+            int incl = len;
+#pragma unroll
+            for (int o = 1; o < kWave; o <<= 1) {
+                const int tt = __shfl_up(incl, o, kWave);
+                if (lane >= o) incl += tt;
+            }
+            if (idx < np) s_off[idx] = L + incl - len;
+            L += __shfl(incl, kWave - 1, kWave);
+        }
+    }
+    // 2) column table: the diagonal first, from 0.0
+    for (int t = lane; t < kRapwHash; t += kWave) {
+        s_key[t] = -1;
+        s_pos[t] = -1;
+        s_lead[t] = kWave;
+    }
+    __syncthreads();
+    if (lane == 0 && !big) {
+        const unsigned d = rapw_hash(ic);
+        s_key[d] = ic;
+        s_pos[d] = 0;
+        if (kFill) s_val[d] = 0.0;
+    }
+    __syncthreads();
+    int next = 1;
+    // 3) candidates in rounds of a wavefront
+    for (int base = 0; !big && base < L; base += kWave) {   // uniform
+        const int p = base + lane;
+        const bool has = p < L;
+        int col = -1;
+        double v = 0.0;
+        unsigned slot = 0;
+        if (has) {
+            int lo = 0, hi = np - 1;   // last pair whose first candidate is <= p
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (s_off[mid] <= p) lo = mid;
+                else hi = mid - 1;
+            }
+            const int q3 = s_pst[lo] + (p - s_off[lo]);
+            col = pja[q3];
+            if (kFill) v = s_ra[lo] * pval[q3];
+            slot = rapw_hash(col);
+            while (true) {
+                const int prev = atomicCAS(&s_key[slot], -1, col);
+                if (prev == -1 || prev == col) break;
+                slot = (slot + 1) & (kRapwHash - 1u);
+            }
+            atomicMin(&s_lead[slot], lane);
+        }
+        s_rc[lane] = col;
+        if (kFill) s_rv[lane] = v;
+        __syncthreads();
+        const bool lead = has && s_lead[slot] == lane;
+        const bool fresh = lead && s_pos[slot] < 0;
+        const unsigned long long mf = __ballot(fresh);
+        if (fresh) s_pos[slot] = next + __popcll(mf & ((1ull << lane) - 1ull));
+        next += __popcll(mf);
+        if (kFill && lead) {
+            double acc = fresh ? v : s_val[slot] + v;
+            for (int u = lane + 1; u < kWave; ++u)
+                if (s_rc[u] == col) acc = acc + s_rv[u];
+            s_val[slot] = acc;
+        }
+        __syncthreads();
+        if (lead) s_lead[slot] = kWave;
+        if (next > kRapwCols) big = true;   // uniform; the table is still at most half full
+        __syncthreads();
+    }
+    if (!kFill) {
+        if (lane == 0) rowcount[ic] = big ? -1 : next;
+        return;
+    }
+    const int o = cia[ic];
+    for (int t = lane; t < kRapwHash; t += kWave)
+        if (s_key[t] >= 0) {
+            cja[o + s_pos[t]] = s_key[t];
+            cval[o + s_pos[t]] = s_val[t];
+        }
+}
+
+__global__ void k_rapw_big(int nc, const int *__restrict__ rowcount, int *__restrict__ list, int *__restrict__ nlist) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < nc; c += gridDim.x * blockDim.x)
+        if (rowcount[c] < 0) list[atomicAdd(nlist, 1)] = c;
 }
 
 // y = A x, row sums from 0.0 in CSR order (dcsr_mxv)
@@ -315,48 +487,63 @@ void rap_general(const DevCSR &R, const DevCSR &A, const DevCSR &P, DevCSR &C) {
         C.val = arena().alloc_n<double>(1);
         return;
     }
-    long long *tsize = arena().alloc_n<long long>((size_t)nc + 1);
-    long long *toff = arena().alloc_n<long long>((size_t)nc + 1);
-    k_rap_candidates<<<grid_for(nc), kBlock, 0, st>>>(nc, R.ia, R.ja, A.ia, A.ja, P.ia, tsize);
+    int *rowcount = arena().alloc_n<int>((size_t)nc + 1);
+    k_rapw<false><<<nc, kWave, 0, st>>>(nc, R.ia, R.ja, R.val, A.ia, A.ja, A.val, P.ia, P.ja, P.val, rowcount,
+                                        nullptr, nullptr, nullptr);
     HZ_LAUNCH_CHECK();
-    {
+    // rows beyond the wavefront kernel's LDS: one thread per row, global tables
+    int *list = arena().alloc_n<int>((size_t)nc + 1);
+    int *nlist = arena().alloc_n<int>(1);
+    HZ_CHECK(hipMemsetAsync(nlist, 0, sizeof(int), st));
+    k_rapw_big<<<grid_for(nc), kBlock, 0, st>>>(nc, rowcount, list, nlist);
+    HZ_LAUNCH_CHECK();
+    int nbig = 0;
+    HZ_CHECK(memcpy_sync(&nbig, nlist, sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<long long> hoff;
+    std::vector<int> cuts;
+    long long *toff = nullptr;
+    int *tkey = nullptr, *tpos = nullptr;
+    if (nbig > 0) {
+        long long *tsize = arena().alloc_n<long long>((size_t)nbig + 1);
+        toff = arena().alloc_n<long long>((size_t)nbig + 1);
+        k_rap_candidates<<<grid_for(nbig), kBlock, 0, st>>>(nbig, list, nc, R.ia, R.ja, A.ia, A.ja, P.ia, tsize);
+        HZ_LAUNCH_CHECK();
         size_t tmp_bytes = 0;
-        HZ_CHECK(rocprim::exclusive_scan(nullptr, tmp_bytes, tsize, toff, 0LL, (size_t)nc + 1,
+        HZ_CHECK(rocprim::exclusive_scan(nullptr, tmp_bytes, tsize, toff, 0LL, (size_t)nbig + 1,
                                          rocprim::plus<long long>(), st));
         void *tmp = arena().alloc(tmp_bytes);
-        HZ_CHECK(hipMemsetAsync(tsize + nc, 0, sizeof(long long), st));
-        HZ_CHECK(rocprim::exclusive_scan(tmp, tmp_bytes, tsize, toff, 0LL, (size_t)nc + 1,
+        HZ_CHECK(hipMemsetAsync(tsize + nbig, 0, sizeof(long long), st));
+        HZ_CHECK(rocprim::exclusive_scan(tmp, tmp_bytes, tsize, toff, 0LL, (size_t)nbig + 1,
                                          rocprim::plus<long long>(), st));
-    }
-    std::vector<long long> hoff((size_t)nc + 1);
-    HZ_CHECK(memcpy_sync(hoff.data(), toff, sizeof(long long) * ((size_t)nc + 1), hipMemcpyDeviceToHost));
-    // rows are processed in chunks whose tables fit the scratch budget
-    const long long budget = std::max<long long>(1LL << 26, [&] {
+        hoff.resize((size_t)nbig + 1);
+        HZ_CHECK(memcpy_sync(hoff.data(), toff, sizeof(long long) * ((size_t)nbig + 1), hipMemcpyDeviceToHost));
+        // the rows go in chunks whose tables fit the scratch budget
         long long mx = 0;
-        for (int i = 0; i < nc; ++i) mx = std::max(mx, hoff[i + 1] - hoff[i]);
-        return mx;
-    }());
-    const long long tables = std::min<long long>(budget, hoff[nc]);
-    int *tkey = arena().alloc_n<int>((size_t)tables + 1);
-    int *tpos = arena().alloc_n<int>((size_t)tables + 1);
-    std::vector<int> cuts{0};
-    for (int i = 0; i < nc; ++i)
-        if (hoff[i + 1] - hoff[cuts.back()] > tables) cuts.push_back(i);
-    cuts.push_back(nc);
-    int *rowcount = arena().alloc_n<int>((size_t)nc + 1);
-    for (size_t c = 0; c + 1 < cuts.size(); ++c) {
-        const int r0 = cuts[c], r1 = cuts[c + 1];
-        k_rap_rows<false><<<grid_for(r1 - r0), kBlock, 0, st>>>(r0, r1, toff, hoff[r0], tkey, tpos, R.ia, R.ja,
-                                                                R.val, A.ia, A.ja, A.val, P.ia, P.ja, P.val,
-                                                                rowcount, nullptr, nullptr, nullptr);
-        HZ_LAUNCH_CHECK();
+        for (int i = 0; i < nbig; ++i) mx = std::max(mx, hoff[i + 1] - hoff[i]);
+        const long long tables = std::min<long long>(std::max<long long>(1LL << 26, mx), hoff[nbig]);
+        tkey = arena().alloc_n<int>((size_t)tables + 1);
+        tpos = arena().alloc_n<int>((size_t)tables + 1);
+        cuts.push_back(0);
+        for (int i = 0; i < nbig; ++i)
+            if (hoff[i + 1] - hoff[cuts.back()] > tables) cuts.push_back(i);
+        cuts.push_back(nbig);
+        for (size_t c = 0; c + 1 < cuts.size(); ++c) {
+            const int r0 = cuts[c], r1 = cuts[c + 1];
+            k_rap_rows<false><<<grid_for(r1 - r0), kBlock, 0, st>>>(r0, r1, list, toff, hoff[r0], tkey, tpos, R.ia,
+                                                                    R.ja, R.val, A.ia, A.ja, A.val, P.ia, P.ja,
+                                                                    P.val, rowcount, nullptr, nullptr, nullptr);
+            HZ_LAUNCH_CHECK();
+        }
     }
     C.nnz = (int)exclusive_scan_int(rowcount, C.ia, nc);
     C.ja = arena().alloc_n<int>((size_t)C.nnz + 1);
     C.val = arena().alloc_n<double>((size_t)C.nnz + 1);
+    k_rapw<true><<<nc, kWave, 0, st>>>(nc, R.ia, R.ja, R.val, A.ia, A.ja, A.val, P.ia, P.ja, P.val, rowcount, C.ia,
+                                       C.ja, C.val);
+    HZ_LAUNCH_CHECK();
     for (size_t c = 0; c + 1 < cuts.size(); ++c) {
         const int r0 = cuts[c], r1 = cuts[c + 1];
-        k_rap_rows<true><<<grid_for(r1 - r0), kBlock, 0, st>>>(r0, r1, toff, hoff[r0], tkey, tpos, R.ia, R.ja,
+        k_rap_rows<true><<<grid_for(r1 - r0), kBlock, 0, st>>>(r0, r1, list, toff, hoff[r0], tkey, tpos, R.ia, R.ja,
                                                                R.val, A.ia, A.ja, A.val, P.ia, P.ja, P.val,
                                                                nullptr, C.ia, C.ja, C.val);
         HZ_LAUNCH_CHECK();
