@@ -83,7 +83,8 @@ class AMG_param(C.Structure):
 class hz_options(C.Structure):
     _fields_ = [("jacobi_weight", REAL), ("hec_rng_source", INT), ("verbose_timing", INT),
                 ("return_on_fatal", INT), ("vcycle_precision", INT), ("batch_time_kernels", INT),
-                ("time_pcg_spmv", INT), ("hem_host_max_rows", INT)]
+                ("time_pcg_spmv", INT), ("hem_host_max_rows", INT),
+                ("rap_wave_stage_pairs", INT)]
 
 
 class hz_batch_stats(C.Structure):

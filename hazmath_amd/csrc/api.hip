@@ -10,7 +10,7 @@
 using namespace hz;
 
 namespace {
-hz_options g_opt = {1.0, 0, 0, 0, 0, 0, 0, 0};
+hz_options g_opt = {1.0, 0, 0, 0, 0, 0, 0, 0, 0};
 hz_solve_info g_info;
 }  // namespace
 
@@ -496,6 +496,7 @@ void hz_get_options(hz_options *opt) { *opt = g_opt; }
 void hz_set_options(const hz_options *opt) {
     g_opt = *opt;
     g_hem_host_max = opt->hem_host_max_rows;
+    g_rap_wave_pairs = opt->rap_wave_stage_pairs;
 }
 void hz_get_last_info(hz_solve_info *info) { *info = g_info; }
 

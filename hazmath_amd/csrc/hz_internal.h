@@ -173,6 +173,11 @@ void sa_prolongation(const DevCSR &A, DevCSR &N, const int *agg, int nagg, doubl
 void csr_transpose_gpu(const DevCSR &A, DevCSR &AT);
 // C = R A P with the layout and summation order of dcsr_rap.
 void rap_general(const DevCSR &R, const DevCSR &A, const DevCSR &P, DevCSR &C);
+// hz_options.rap_wave_stage_pairs: (R entry, A entry) pairs the wavefront
+// Galerkin kernel stages in LDS at a time; 0 = its capacity, -1 = every row
+// through the one-thread-per-row kernel, -2 = the default with a 16-column
+// limit.
+extern int g_rap_wave_pairs;
 // y = A x (row sums from 0.0, CSR order) and x += alpha P xc (dcsr_aAxpy).
 void csr_mxv(const DevCSR &A, const double *x, double *y);
 void csr_prolong(const DevCSR &P, const double *xc, double alpha, const double *alpha_dev, double *x);

@@ -220,23 +220,35 @@ __global__ void k_rap_rows(int r0, int r1, const int *__restrict__ rows, const l
     }
 }
 
-// The same product with one wavefront per coarse row, for rows of at most
-// kRapwPairs (R entry, A entry) pairs and kRapwCols distinct columns.  The
-// pairs are staged in LDS in traversal order with the prefix of their P-row
-// lengths; the candidates (q1, q2, q3) are then taken 64 at a time in that
-// order.  A column's first lane in a round folds the round's values of that
+// The same product with one wavefront per coarse row.  The row's (R entry,
+// A entry) pairs are staged in LDS in traversal order, whole A rows at a
+// time and at most max_pairs pairs per stage, with the prefix of their P-row
+// lengths; each stage's candidates (q1, q2, q3) are then taken 64 at a time
+// in that order.  The lanes of a round are grouped by column (one ballot per
+// distinct column); a column's first lane folds the round's values of that
 // column into the column's running sum in lane order, so every sum keeps the
 // sequential order; new columns take output slots in order of first
-// appearance, after the diagonal (slot 0, summed from 0.0).  The count pass
-// marks the rows it cannot take with rowcount -1 (the per-thread kernel takes
-// them); the fill pass skips them.
+// appearance, after the diagonal (slot 0, summed from 0.0).  The column table
+// and the sums persist across stages.  The count pass marks with rowcount -1
+// the rows it cannot take (more than max_cols columns, or one A row longer
+// than a stage); the per-thread kernel takes those and the fill pass skips
+// them.
 constexpr int kRapwPairs = 2048;
 constexpr int kRapwCols = 512;
-constexpr int kRapwHash = 2 * kRapwCols;   // at most half full: probing ends
+constexpr int kRapwHash = 2 * kRapwCols;   // at most kRapwCols + 64 keys: probing ends
 
 __device__ __forceinline__ unsigned rapw_hash(int key) {
     unsigned h = (unsigned)key * 2654435761u;
     return (h ^ (h >> 15)) & (kRapwHash - 1u);
+}
+
+__device__ __forceinline__ int wave_incl_scan(int v, int lane) {
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const int t = __shfl_up(v, o, kWave);
+        if (lane >= o) v += t;
+    }
+    return v;
 }
 
 template <bool kFill>
@@ -244,136 +256,140 @@ __global__ __launch_bounds__(kWave) void k_rapw(int nc, const int *__restrict__ 
                                                 const double *__restrict__ rval, const int *__restrict__ aia,
                                                 const int *__restrict__ aja, const double *__restrict__ aval,
                                                 const int *__restrict__ pia, const int *__restrict__ pja,
-                                                const double *__restrict__ pval, int *__restrict__ rowcount,
-                                                const int *__restrict__ cia, int *__restrict__ cja,
-                                                double *__restrict__ cval) {
+                                                const double *__restrict__ pval, int max_pairs, int max_cols,
+                                                int *__restrict__ rowcount, const int *__restrict__ cia,
+                                                int *__restrict__ cja, double *__restrict__ cval) {
     __shared__ int s_pst[kRapwPairs];          // P-row start of each pair
-    __shared__ int s_off[kRapwPairs + 1];      // its P-row length, then its first candidate
+    __shared__ int s_off[kRapwPairs];          // its P-row length, then its first candidate
     __shared__ double s_ra[kFill ? kRapwPairs : 1];
-    __shared__ int s_key[kRapwHash], s_pos[kRapwHash], s_lead[kRapwHash];
+    __shared__ int s_key[kRapwHash], s_pos[kRapwHash];
     __shared__ double s_val[kFill ? kRapwHash : 1];
-    __shared__ int s_rc[kWave];
     __shared__ double s_rv[kFill ? kWave : 1];
     const int lane = threadIdx.x;
     const int ic = blockIdx.x;
     if (ic >= nc) return;   // the whole (single-wavefront) block
     if (kFill && rowcount[ic] < 0) return;
-    // 1) (R entry, A entry) pairs in traversal order
-    const int r0 = ria[ic], r1 = ria[ic + 1];
-    int np = 0;
-    bool big = false;
-    for (int b = r0; b < r1; b += kWave) {   // uniform trip count
-        const int q1 = b + lane;
-        int a0 = 0, alen = 0;
-        double r = 0.0;
-        if (q1 < r1) {
-            const int i1 = rja[q1];
-            a0 = aia[i1];
-            alen = aia[i1 + 1] - a0;
-            if (kFill) r = rval[q1];
-        }
-        int incl = alen;
-#pragma unroll
-        for (int o = 1; o < kWave; o <<= 1) {
-            const int tt = __shfl_up(incl, o, kWave);
-            if (lane >= o) incl += tt;
-        }
-        const int total = __shfl(incl, kWave - 1, kWave);
-        if (np + total > kRapwPairs) {
-            big = true;
-            break;
-        }
-        const int first = np + incl - alen;
-        for (int k = 0; k < alen; ++k) {
-            const int q2 = a0 + k;
-            const int i2 = aja[q2];
-            const int ps = pia[i2];
-            s_pst[first + k] = ps;
-            s_off[first + k] = pia[i2 + 1] - ps;
-            if (kFill) s_ra[first + k] = r * aval[q2];
-        }
-        np += total;
-    }
-    __syncthreads();
-    // prefix of the P-row lengths: s_off[i] = first candidate of pair i
-    int L = 0;
-    if (!big) {
-        for (int b = 0; b < np; b += kWave) {
-            const int idx = b + lane;
-            const int len = idx < np ? s_off[idx] : 0;
-            int incl = len;
-#pragma unroll
-            for (int o = 1; o < kWave; o <<= 1) {
-                const int tt = __shfl_up(incl, o, kWave);
-                if (lane >= o) incl += tt;
-            }
-            if (idx < np) s_off[idx] = L + incl - len;
-            L += __shfl(incl, kWave - 1, kWave);
-        }
-    }
-    // 2) column table: the diagonal first, from 0.0
+    // column table: the diagonal first, from 0.0
     for (int t = lane; t < kRapwHash; t += kWave) {
         s_key[t] = -1;
         s_pos[t] = -1;
-        s_lead[t] = kWave;
     }
     __syncthreads();
-    if (lane == 0 && !big) {
+    if (lane == 0) {
         const unsigned d = rapw_hash(ic);
         s_key[d] = ic;
         s_pos[d] = 0;
         if (kFill) s_val[d] = 0.0;
     }
-    __syncthreads();
     int next = 1;
-    // 3) candidates in rounds of a wavefront
-    for (int base = 0; !big && base < L; base += kWave) {   // uniform
-        const int p = base + lane;
-        const bool has = p < L;
-        int col = -1;
-        double v = 0.0;
-        unsigned slot = 0;
-        if (has) {
-            int lo = 0, hi = np - 1;   // last pair whose first candidate is <= p
-            while (lo < hi) {
-                const int mid = (lo + hi + 1) >> 1;
-                if (s_off[mid] <= p) lo = mid;
-                else hi = mid - 1;
+    bool big = false;
+    const int r1 = ria[ic + 1];
+    int q1 = ria[ic];
+    while (q1 < r1 && !big) {   // uniform
+        __syncthreads();
+        // 1) a stage of (R entry, A entry) pairs: whole A rows while they fit
+        int np = 0;
+        while (q1 < r1) {
+            const int q = q1 + lane;
+            int a0 = 0, alen = 0;
+            double r = 0.0;
+            if (q < r1) {
+                const int i1 = rja[q];
+                a0 = aia[i1];
+                alen = aia[i1 + 1] - a0;
+                if (kFill) r = rval[q];
             }
-            const int q3 = s_pst[lo] + (p - s_off[lo]);
-            col = pja[q3];
-            if (kFill) v = s_ra[lo] * pval[q3];
-            slot = rapw_hash(col);
-            while (true) {
-                const int prev = atomicCAS(&s_key[slot], -1, col);
-                if (prev == -1 || prev == col) break;
-                slot = (slot + 1) & (kRapwHash - 1u);
+            const int incl = wave_incl_scan(alen, lane);
+            // a prefix of the lanes: incl does not decrease with the lane
+            const int nfit = __popcll(__ballot(q < r1 && incl <= max_pairs - np));
+            if (nfit == 0) {
+                if (np == 0) big = true;   // one A row longer than a stage
+                break;
             }
-            atomicMin(&s_lead[slot], lane);
+            if (lane < nfit) {
+                const int first = np + incl - alen;
+                for (int k = 0; k < alen; ++k) {
+                    const int q2 = a0 + k;
+                    const int i2 = aja[q2];
+                    const int ps = pia[i2];
+                    s_pst[first + k] = ps;
+                    s_off[first + k] = pia[i2 + 1] - ps;
+                    if (kFill) s_ra[first + k] = r * aval[q2];
+                }
+            }
+            np += __shfl(incl, nfit - 1, kWave);
+            q1 += nfit;
+            if (nfit < kWave) break;   // the stage is full or the row is done
         }
-        s_rc[lane] = col;
-        if (kFill) s_rv[lane] = v;
+        if (big) break;
         __syncthreads();
-        const bool lead = has && s_lead[slot] == lane;
-        const bool fresh = lead && s_pos[slot] < 0;
-        const unsigned long long mf = __ballot(fresh);
-        if (fresh) s_pos[slot] = next + __popcll(mf & ((1ull << lane) - 1ull));
-        next += __popcll(mf);
-        if (kFill && lead) {
-            double acc = fresh ? v : s_val[slot] + v;
-            for (int u = lane + 1; u < kWave; ++u)
-                if (s_rc[u] == col) acc = acc + s_rv[u];
-            s_val[slot] = acc;
+        // prefix of the P-row lengths: s_off[i] = first candidate of pair i
+        int L = 0;
+        for (int b = 0; b < np; b += kWave) {
+            const int idx = b + lane;
+            const int len = idx < np ? s_off[idx] : 0;
+            const int incl = wave_incl_scan(len, lane);
+            if (idx < np) s_off[idx] = L + incl - len;
+            L += __shfl(incl, kWave - 1, kWave);
         }
         __syncthreads();
-        if (lead) s_lead[slot] = kWave;
-        if (next > kRapwCols) big = true;   // uniform; the table is still at most half full
-        __syncthreads();
+        // 2) the stage's candidates in rounds of a wavefront
+        for (int base = 0; !big && base < L; base += kWave) {   // uniform
+            const int p = base + lane;
+            const bool has = p < L;
+            int col = -1;
+            double v = 0.0;
+            if (has) {
+                int lo = 0, hi = np - 1;   // last pair whose first candidate is <= p
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (s_off[mid] <= p) lo = mid;
+                    else hi = mid - 1;
+                }
+                const int q3 = s_pst[lo] + (p - s_off[lo]);
+                col = pja[q3];
+                if (kFill) v = s_ra[lo] * pval[q3];
+            }
+            if (kFill) s_rv[lane] = v;
+            // the lanes of each distinct column (one ballot per column)
+            unsigned long long mine = 0;
+            for (unsigned long long todo = __ballot(has); todo;) {   // uniform
+                const int l = __builtin_ctzll(todo);
+                const int c = __builtin_amdgcn_readlane(col, l);
+                const unsigned long long m = __ballot(has && col == c);
+                if (has && col == c) mine = m;
+                todo &= ~m;
+            }
+            const bool lead = has && __builtin_ctzll(mine) == lane;
+            unsigned slot = 0;
+            if (lead) {   // distinct columns: only the first lane of each probes
+                slot = rapw_hash(col);
+                while (true) {
+                    const int prev = atomicCAS(&s_key[slot], -1, col);
+                    if (prev == -1 || prev == col) break;
+                    slot = (slot + 1) & (kRapwHash - 1u);
+                }
+            }
+            const bool fresh = lead && s_pos[slot] < 0;
+            const unsigned long long mf = __ballot(fresh);
+            if (fresh) s_pos[slot] = next + __popcll(mf & ((1ull << lane) - 1ull));
+            next += __popcll(mf);
+            __syncthreads();
+            if (kFill && lead) {
+                double acc = fresh ? v : s_val[slot] + v;
+                for (unsigned long long rest = mine & (mine - 1); rest; rest &= rest - 1)
+                    acc = acc + s_rv[__builtin_ctzll(rest)];
+                s_val[slot] = acc;
+            }
+            if (next > max_cols) big = true;   // uniform; at most max_cols + 64 keys so far
+            __syncthreads();
+        }
     }
     if (!kFill) {
         if (lane == 0) rowcount[ic] = big ? -1 : next;
         return;
     }
+    __syncthreads();
     const int o = cia[ic];
     for (int t = lane; t < kRapwHash; t += kWave)
         if (s_key[t] >= 0) {
@@ -387,14 +403,29 @@ __global__ void k_rapw_big(int nc, const int *__restrict__ rowcount, int *__rest
         if (rowcount[c] < 0) list[atomicAdd(nlist, 1)] = c;
 }
 
-// y = A x, row sums from 0.0 in CSR order (dcsr_mxv)
+// y = A x, row sums from 0.0 in CSR order (dcsr_mxv, sparse.c:1456-1590).
+// kLanes lanes share a row: each round they load kLanes consecutive entries
+// and form the products val*x, and the group's first lane adds them to the
+// row sum in CSR order -- the same operations and order as one thread per row.
+template <int kLanes>
 __global__ void k_csr_mxv(int n, const int *__restrict__ ia, const int *__restrict__ ja,
                           const double *__restrict__ val, const double *__restrict__ x, double *__restrict__ y) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        double t = 0.0;
-        for (int k = ia[i]; k < ia[i + 1]; ++k) t += val[k] * x[ja[k]];
-        y[i] = t;
+    const long gt = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = (int)(gt / kLanes), j = (int)(gt % kLanes);
+    const int gbase = threadIdx.x % kWave - j;
+    const bool valid = i < n;
+    const int k0 = valid ? ia[i] : 0, k1 = valid ? ia[i + 1] : 0;
+    double t = 0.0;
+    for (int kb = k0; kb < k1; kb += kLanes) {   // trip count uniform within the lane group
+        const int k = kb + j;
+        const double pr = k < k1 ? val[k] * x[ja[k]] : 0.0;
+#pragma unroll
+        for (int u = 0; u < kLanes; ++u) {
+            const double v = __shfl(pr, gbase + u, kWave);
+            if (j == 0 && kb + u < k1) t = t + v;
+        }
     }
+    if (valid && j == 0) y[i] = t;
 }
 
 // x += alpha P xc (dcsr_aAxpy, sparse.c:1693-1730)
@@ -474,6 +505,8 @@ void csr_transpose_gpu(const DevCSR &A, DevCSR &AT) {
     HZ_LAUNCH_CHECK();
 }
 
+int g_rap_wave_pairs = 0;
+
 void rap_general(const DevCSR &R, const DevCSR &A, const DevCSR &P, DevCSR &C) {
     hipStream_t st = stream();
     const int nc = R.n;
@@ -488,8 +521,15 @@ void rap_general(const DevCSR &R, const DevCSR &A, const DevCSR &P, DevCSR &C) {
         return;
     }
     int *rowcount = arena().alloc_n<int>((size_t)nc + 1);
-    k_rapw<false><<<nc, kWave, 0, st>>>(nc, R.ia, R.ja, R.val, A.ia, A.ja, A.val, P.ia, P.ja, P.val, rowcount,
-                                        nullptr, nullptr, nullptr);
+    const bool wave = g_rap_wave_pairs >= 0 || g_rap_wave_pairs == -2;
+    const int max_pairs = g_rap_wave_pairs > 0 ? std::min(g_rap_wave_pairs, kRapwPairs) : kRapwPairs;
+    const int max_cols = g_rap_wave_pairs == -2 ? 16 : kRapwCols;
+    if (wave) {
+        k_rapw<false><<<nc, kWave, 0, st>>>(nc, R.ia, R.ja, R.val, A.ia, A.ja, A.val, P.ia, P.ja, P.val, max_pairs,
+                                            max_cols, rowcount, nullptr, nullptr, nullptr);
+    } else {
+        HZ_CHECK(hipMemsetAsync(rowcount, 0xff, sizeof(int) * (size_t)nc, st));   // every row -1
+    }
     HZ_LAUNCH_CHECK();
     // rows beyond the wavefront kernel's LDS: one thread per row, global tables
     int *list = arena().alloc_n<int>((size_t)nc + 1);
@@ -538,9 +578,11 @@ void rap_general(const DevCSR &R, const DevCSR &A, const DevCSR &P, DevCSR &C) {
     C.nnz = (int)exclusive_scan_int(rowcount, C.ia, nc);
     C.ja = arena().alloc_n<int>((size_t)C.nnz + 1);
     C.val = arena().alloc_n<double>((size_t)C.nnz + 1);
-    k_rapw<true><<<nc, kWave, 0, st>>>(nc, R.ia, R.ja, R.val, A.ia, A.ja, A.val, P.ia, P.ja, P.val, rowcount, C.ia,
-                                       C.ja, C.val);
-    HZ_LAUNCH_CHECK();
+    if (wave) {
+        k_rapw<true><<<nc, kWave, 0, st>>>(nc, R.ia, R.ja, R.val, A.ia, A.ja, A.val, P.ia, P.ja, P.val, max_pairs,
+                                           max_cols, rowcount, C.ia, C.ja, C.val);
+        HZ_LAUNCH_CHECK();
+    }
     for (size_t c = 0; c + 1 < cuts.size(); ++c) {
         const int r0 = cuts[c], r1 = cuts[c + 1];
         k_rap_rows<true><<<grid_for(r1 - r0), kBlock, 0, st>>>(r0, r1, list, toff, hoff[r0], tkey, tpos, R.ia, R.ja,
@@ -552,7 +594,7 @@ void rap_general(const DevCSR &R, const DevCSR &A, const DevCSR &P, DevCSR &C) {
 
 void csr_mxv(const DevCSR &A, const double *x, double *y) {
     if (A.n == 0) return;
-    k_csr_mxv<<<grid_for(A.n), kBlock, 0, stream()>>>(A.n, A.ia, A.ja, A.val, x, y);
+    k_csr_mxv<16><<<div_up((long)A.n * 16, kBlock), kBlock, 0, stream()>>>(A.n, A.ia, A.ja, A.val, x, y);
     HZ_LAUNCH_CHECK();
 }
 

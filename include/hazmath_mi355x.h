@@ -326,6 +326,13 @@ typedef struct {
                               all give the same aggregates.  0: 1100000 (or the HZ_HEM_HOST_MAX
                               environment variable); every level through one form: -1 the GPU
                               pipeline, -2 the full host sweep, -3 the preference lists */
+    INT  rap_wave_stage_pairs; /* SA's general Galerkin product (dcsr_rap) runs one wavefront per
+                              coarse row, staging the row's (R entry, A entry) pairs in LDS this
+                              many at a time; rows of more than 512 columns, or with one A row
+                              longer than a stage, run one thread per row.  All forms give the
+                              same matrix.  0: 2048 (the LDS capacity; larger values are capped
+                              there), -1: every row one thread per row, -2: the default with a
+                              16-column limit */
 } hz_options;
 
 void hz_get_options(hz_options *opt);

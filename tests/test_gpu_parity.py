@@ -715,6 +715,48 @@ def test_sa_kuhn3d_hierarchy_and_solve(lib, orc, libc):
     assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) < 1e-10
 
 
+@pytest.mark.parametrize("problem", ["kuhn16_hec", "p2d40_hem"])
+@pytest.mark.parametrize("form", [-1, -2, 100])
+def test_sa_galerkin_product_forms_bitexact(lib, orc, libc, problem, form):
+    """dcsr_rap (sparse.c:2401-2538) through each form of the GPU product
+    (hz_options.rap_wave_stage_pairs): every row one thread per row (-1), the
+    wavefront kernel limited to 16 columns so that rows overflow its column
+    table and fall back (-2), and stages of 100 (R entry, A entry) pairs, so
+    that rows span many stages and rows with a longer A row fall back (100).
+    Every coarse matrix bit-identical to the oracle's."""
+    import oracle
+    if problem == "kuhn16_hec":
+        (ia, ja, va), agg, cdof = problems.poisson3d_kuhn(16), capi.HEC, 60
+    else:
+        (ia, ja, va), agg, cdof = problems.poisson2d_p1(40), capi.HEM, 100
+    A = capi.HostCSR(ia, ja, va)
+    opts = capi.hz_options()
+    lib.hz_get_options(C.byref(opts))
+    old = opts.rap_wave_stage_pairs
+    out = []
+    try:
+        opts.rap_wave_stage_pairs = form
+        lib.hz_set_options(C.byref(opts))
+        for which in ("gpu", "oracle"):
+            fn = (lambda **k: capi.solver_params(lib, **k)) if which == "gpu" else oracle.params
+            _, amg = _sa_param(fn, agg, 1, theta=0.04, coarse_dof=cdof)
+            h = capi.hz_hierarchy()
+            libc.srand(1)
+            st = (lib.hz_setup_hierarchy if which == "gpu" else orc.orc_setup_hierarchy)(A.ptr(), C.byref(amg),
+                                                                                          C.byref(h))
+            assert st == 0
+            out.append(capi.hierarchy_to_numpy(h))
+            (lib.hz_hierarchy_free if which == "gpu" else orc.orc_hierarchy_free)(C.byref(h))
+    finally:
+        opts.rap_wave_stage_pairs = old
+        lib.hz_set_options(C.byref(opts))
+    g, o = out
+    assert len(g) == len(o) >= 3
+    for lg, lo in zip(g, o):
+        for key in ("ia", "ja", "val"):
+            assert np.array_equal(lg[key], lo[key]), key
+
+
 def test_first_call_in_process_keeps_caller_rand_stream():
     """The first library call of a process brings up the HIP runtime, which
     consumes libc random() values; the caller's stream must be untouched so
