@@ -233,7 +233,7 @@ __global__ void k_rap_rows(int r0, int r1, const int *__restrict__ rows, const l
 // the rows it cannot take (more than max_cols columns, or one A row longer
 // than a stage); the per-thread kernel takes those and the fill pass skips
 // them.
-constexpr int kRapwPairs = 2048;
+constexpr int kRapwPairs = 512;
 constexpr int kRapwCols = 512;
 constexpr int kRapwHash = 2 * kRapwCols;   // at most kRapwCols + 64 keys: probing ends
 

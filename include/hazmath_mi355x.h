@@ -330,7 +330,7 @@ typedef struct {
                               coarse row, staging the row's (R entry, A entry) pairs in LDS this
                               many at a time; rows of more than 512 columns, or with one A row
                               longer than a stage, run one thread per row.  All forms give the
-                              same matrix.  0: 2048 (the LDS capacity; larger values are capped
+                              same matrix.  0: 512 (the LDS capacity; larger values are capped
                               there), -1: every row one thread per row, -2: the default with a
                               16-column limit */
 } hz_options;
