@@ -346,7 +346,7 @@ void presmooth(Hierarchy &H, int l, CycleState &cs) {
         if (cs.zero[l]) fill(L.x, 0.0, L.A.n);
         cs.zero[l] = false;
         cs.first_done[l] = false;
-        if (cp.smoother == HZ_SMOOTHER_CHEBY) cheby_apply(L.S, L.cheb, sweeps, L.b, L.x, L.xt, L.cwork);
+        if (cp.smoother == HZ_SMOOTHER_CHEBY) cheby_apply(L.S, L.cheb, sweeps, L.b, L.x, L.xt, L.cwork, &L.A);
         else gs_smooth(L.gs, cp.smoother, false, sweeps, cp.relax, L.b, L.x);
         return;
     }
@@ -391,7 +391,7 @@ void prolong_postsmooth(Hierarchy &H, int l, double alpha, const double *alpha_d
     if (cp.postsmooth <= 0 || cp.smoother == HZ_SMOOTHER_CHEBY || is_gs_family(cp.smoother)) {
         if (L.sa) csr_prolong(L.P, C.x, alpha, alpha_dev, L.x);
         else prolong_agg(L.A.n, L.agg, C.x, alpha, alpha_dev, L.x);
-        if (cp.smoother == HZ_SMOOTHER_CHEBY) cheby_apply(L.S, L.cheb, cp.postsmooth, L.b, L.x, L.xt, L.cwork);
+        if (cp.smoother == HZ_SMOOTHER_CHEBY) cheby_apply(L.S, L.cheb, cp.postsmooth, L.b, L.x, L.xt, L.cwork, &L.A);
         else if (is_gs_family(cp.smoother)) gs_smooth(L.gs, cp.smoother, true, cp.postsmooth, cp.relax, L.b, L.x);
         return;
     }

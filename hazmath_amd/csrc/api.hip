@@ -1107,7 +1107,7 @@ INT hz_cheby_smooth(dCSRmat *A, const REAL *b, REAL *x, INT steps, REAL scale) {
         double *dt = arena().alloc_n<double>((size_t)A->row + 1);
         double *dd = arena().alloc_n<double>((size_t)A->row + 1);
         const ChebLevel c = cheby_prepare(dA, scale);
-        cheby_apply(S, c, steps, db, dx, dt, dd);
+        cheby_apply(S, c, steps, db, dx, dt, dd, &dA);
         HZ_CHECK(memcpy_sync(x, dx, sizeof(double) * A->row, hipMemcpyDeviceToHost));
         return HZ_SUCCESS;
     } catch (const DeviceError &) {

@@ -190,8 +190,9 @@ struct ChebLevel {
 // diag and lmax = scale * Gershgorin bound of D^-1 A, lmin = lmax / 30
 ChebLevel cheby_prepare(const DevCSR &A, double scale);
 // `steps` Chebyshev steps from x (x and xt swap roles each step; x ends on the result)
+// A (optional): the level's CSR, used on small levels with long rows.
 void cheby_apply(const DevSELL &S, const ChebLevel &c, int steps, const double *b, double *&x, double *&xt,
-                 double *dwork);
+                 double *dwork, const DevCSR *A = nullptr);
 // network outputs [nagg][2] -> omega[c] = clamp(out[c][0], 0, 1.6); returns
 // the level's Chebyshev bound scale clamp(mean_c out[c][1], 0.2, 1.5)
 double learned_parameters(int nagg, const float *out, float *omega);

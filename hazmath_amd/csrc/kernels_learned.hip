@@ -101,6 +101,38 @@ __global__ __launch_bounds__(kBlock) void k_cheb_step(int n, int nslices, const 
     }
 }
 
+// The same step over CSR with kLanes lanes per row, for small levels with long
+// rows: each round the lanes load kLanes consecutive entries and form the
+// products a*x, and the group's first lane subtracts them from t = b in CSR
+// (= stored) order -- the same operations and order as one thread per row.
+template <int kLanes>
+__global__ void k_cheb_step_grp(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                                const double *__restrict__ a, const double *__restrict__ diag, double c1, double c2,
+                                const double *__restrict__ b, const double *__restrict__ x,
+                                double *__restrict__ xout, double *__restrict__ d) {
+    const long gt = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = (int)(gt / kLanes), j = (int)(gt % kLanes);
+    const int gbase = threadIdx.x % kWave - j;
+    const bool valid = r < n;
+    const int k0 = valid ? ia[r] : 0, k1 = valid ? ia[r + 1] : 0;
+    double t = valid ? b[r] : 0.0;
+    for (int kb = k0; kb < k1; kb += kLanes) {   // trip count uniform within the lane group
+        const int k = kb + j;
+        const double pr = k < k1 ? a[k] * x[ja[k]] : 0.0;
+#pragma unroll
+        for (int u = 0; u < kLanes; ++u) {
+            const double v = __shfl(pr, gbase + u, kWave);
+            if (j == 0 && kb + u < k1) t = t - v;
+        }
+    }
+    if (valid && j == 0) {
+        const double di = diag[r];
+        const double dn = (c1 == 0.0 ? 0.0 : c1 * d[r]) + (di != 0.0 ? c2 * t / di : 0.0);
+        d[r] = dn;
+        xout[r] = x[r] + dn;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Per-aggregate features (12, FP32), averaged over the aggregate's members
 // (R rows).  Per row i with a_ii the first diagonal entry, s = sum_{j!=i}|a_ij|,
@@ -195,12 +227,15 @@ ChebLevel cheby_prepare(const DevCSR &A, double scale) {
 }
 
 void cheby_apply(const DevSELL &S, const ChebLevel &c, int steps, const double *b, double *&x, double *&xt,
-                 double *dwork) {
+                 double *dwork, const DevCSR *A) {
     if (S.n == 0 || steps <= 0) return;
     const double theta = 0.5 * (c.lmax + c.lmin), delta = 0.5 * (c.lmax - c.lmin);
     const double sigma = theta / delta;
     double rho = 1.0 / sigma;
     const int grid = div_up(S.nslices, kSlicesPerBlock);
+    // small levels with long rows (latency-bound): 16 lanes per CSR row
+    constexpr int G = 16;
+    const bool grp = A && A->ia && A->n == S.n && S.n <= 65536 && (long long)A->nnz >= 16LL * A->n;
     for (int k = 0; k < steps; ++k) {
         double c1 = 0.0, c2 = 1.0 / theta;
         if (k > 0) {
@@ -209,8 +244,13 @@ void cheby_apply(const DevSELL &S, const ChebLevel &c, int steps, const double *
             c2 = 2.0 * rn / delta;
             rho = rn;
         }
-        k_cheb_step<<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col, S.val, c.diag, c1, c2, b, x, xt,
-                                                   dwork);
+        if (grp)
+            k_cheb_step_grp<G><<<div_up((long)S.n * G, kBlock), kBlock, 0, stream()>>>(S.n, A->ia, A->ja, A->val,
+                                                                                       c.diag, c1, c2, b, x, xt,
+                                                                                       dwork);
+        else
+            k_cheb_step<<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col, S.val, c.diag, c1, c2, b, x,
+                                                       xt, dwork);
         HZ_LAUNCH_CHECK();
         std::swap(x, xt);
     }

@@ -819,8 +819,38 @@ def test_mlp_training_fits_smooth_target(lib):
     assert mlp.loss(theta.astype(np.float64), xv.astype(np.float64), tv.astype(np.float64)) < 3 * hist[-1] + 1e-3
 
 
-def test_chebyshev_smoother_bitexact(lib, orc):
-    ia, ja, va, _ = problems.poisson3d_kuhn_coef(12)
+def _band_spd(n, half, seed):
+    """Symmetric banded matrix with 2*half+1 entries a row (fewer near the
+    ends), strictly diagonally dominant: long rows for the 16-lane kernels."""
+    rng = np.random.default_rng(seed)
+    off = {k: -rng.uniform(0.1, 1.0, n - k) for k in range(1, half + 1)}
+    rows, cols, vals = [], [], []
+    for i in range(n):
+        s = 0.0
+        for j in range(max(0, i - half), min(n, i + half + 1)):
+            if j == i:
+                continue
+            v = off[abs(i - j)][min(i, j)]
+            rows.append(i), cols.append(j), vals.append(v)
+            s += abs(v)
+        rows.append(i), cols.append(i), vals.append(s + 1.0)
+    order = np.lexsort((np.array(cols), np.array(rows)))
+    ja = np.array(cols, np.int32)[order]
+    va = np.array(vals)[order]
+    ia = np.zeros(n + 1, np.int32)
+    np.add.at(ia, np.array(rows)[order] + 1, 1)
+    return np.cumsum(ia).astype(np.int32), ja, va
+
+
+@pytest.mark.parametrize("problem", ["kuhn_coef12", "band41"])
+def test_chebyshev_smoother_bitexact(lib, orc, problem):
+    """Chebyshev steps against the oracle, bit-exact: on the SELL kernel
+    (15-entry rows) and on the 16-lanes-per-row CSR kernel that small levels
+    with long rows take (41-entry band)."""
+    if problem == "kuhn_coef12":
+        ia, ja, va, _ = problems.poisson3d_kuhn_coef(12)
+    else:
+        ia, ja, va = _band_spd(3000, 20, 5)
     n = ia.size - 1
     A = capi.HostCSR(ia, ja, va)
     rng = np.random.default_rng(3)
