@@ -213,7 +213,15 @@ int amg_setup(Hierarchy &H, const DevCSR &A0, AMG_param *param, bool) {
                 mlp_forward(H.mlp_theta, w, nagg);
                 mlp_outputs(nagg, w, out);
                 float *om = arena().alloc_n<float>((size_t)nagg + 1);
-                const double scale = learned_parameters(nagg, out, om);
+                double scale = learned_parameters(nagg, out, om);
+                // floor: the Chebyshev interval must reach the level's lambda_max(D^-1 A).  The power
+                // estimate (60 iterations) is from below, hence 1.1x; the Gershgorin bound (scale 1)
+                // always covers lambda_max, so the floor never exceeds 1.
+                const ChebLevel g1 = cheby_prepare(L.A, 1.0);
+                if (g1.lmax > 0.0) {
+                    const double floor = std::min(1.0, 1.1 * power_lambda_dinv(L.A, g1.diag, 60) / g1.lmax);
+                    scale = std::min(1.5, std::max(scale, floor));
+                }
                 if ((int)H.cheb_scale.size() < max_levels) H.cheb_scale.assign(max_levels, 1.0);
                 H.cheb_scale[lvl] = scale;
                 omega_agg = om;

@@ -196,6 +196,9 @@ void cheby_apply(const DevSELL &S, const ChebLevel &c, int steps, const double *
 // network outputs [nagg][2] -> omega[c] = clamp(out[c][0], 0, 1.6); returns
 // the level's Chebyshev bound scale clamp(mean_c out[c][1], 0.2, 1.5)
 double learned_parameters(int nagg, const float *out, float *omega);
+// lambda_max(D^-1 A) by `iters` normalized power iterations (d: the diagonal
+// the Chebyshev smoother divides by); an estimate from below
+double power_lambda_dinv(const DevCSR &A, const double *d, int iters);
 // feat [nagg][12] (FP32) of level `level`
 void aggregate_features(const DevCSR &A, const int *agg, int nagg, const int *rptr, const int *rmem, int level,
                         float *feat);

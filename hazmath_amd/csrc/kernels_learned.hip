@@ -284,6 +284,88 @@ __global__ __launch_bounds__(kBlock) void k_learned_params(int nagg, const float
 
 }  // namespace
 
+namespace {
+
+// y = s D^-1 A v with s = 1/|v| from the previous step's sum of squares
+// (qprev; s = 1 when null or zero; d: first diagonal entries, rows with d = 0
+// give 0), and per-block partial sums of y_i^2.  16 lanes share a row (the
+// loads coalesce); the group's first lane adds the products in row order.
+constexpr int kPowLanes = 16;
+__global__ __launch_bounds__(kBlock) void k_dinv_av(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                                                    const double *__restrict__ a, const double *__restrict__ d,
+                                                    const double *__restrict__ v, const double *__restrict__ qprev,
+                                                    double *__restrict__ y, double *__restrict__ part) {
+    __shared__ double sh[kBlock / kWave];
+    const double s = (qprev && *qprev > 0.0) ? 1.0 / sqrt(*qprev) : 1.0;
+    const long gt = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = (int)(gt / kPowLanes), j = (int)(gt % kPowLanes);
+    const int gbase = threadIdx.x % kWave - j;
+    const bool valid = i < n;
+    const int k0 = valid ? ia[i] : 0, k1 = valid ? ia[i + 1] : 0;
+    double t = 0.0;
+    for (int kb = k0; kb < k1; kb += kPowLanes) {   // trip count uniform within the lane group
+        const int k = kb + j;
+        const double pr = k < k1 ? a[k] * v[ja[k]] : 0.0;
+#pragma unroll
+        for (int u = 0; u < kPowLanes; ++u) {
+            const double w = __shfl(pr, gbase + u, kWave);
+            if (j == 0 && kb + u < k1) t += w;
+        }
+    }
+    double q = 0.0;
+    if (valid && j == 0) {
+        const double yi = d[i] != 0.0 ? s * t / d[i] : 0.0;
+        y[i] = yi;
+        q = yi * yi;
+    }
+    const double bs = block_sum<kBlock>(q, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
+// the block partials summed in one block (fixed order)
+__global__ __launch_bounds__(kBlock) void k_sum_partials(const double *__restrict__ part, int g,
+                                                         double *__restrict__ out) {
+    __shared__ double sh[kBlock / kWave];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < g; i += kBlock) s += part[i];
+    const double t = block_sum<kBlock>(s, sh);
+    if (threadIdx.x == 0) *out = t;
+}
+
+// a fixed start vector with every entry nonzero and no smooth structure
+__global__ void k_power_start(int n, double *__restrict__ v) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const unsigned h = (unsigned)i * 2654435761u;
+        v[i] = 0.25 + (double)((h >> 8) & 0xffffu) / 65536.0;
+    }
+}
+
+}  // namespace
+
+double power_lambda_dinv(const DevCSR &A, const double *d, int iters) {
+    const int n = A.n;
+    if (n == 0) return 0.0;
+    const int g = div_up((long)n * kPowLanes, kBlock);
+    double *v = arena().alloc_n<double>((size_t)n + 1);
+    double *y = arena().alloc_n<double>((size_t)n + 1);
+    double *part = arena().alloc_n<double>((size_t)g + 1);
+    double *q = arena().alloc_n<double>(2);   // sum of squares of the last two steps
+    k_power_start<<<grid_for(n), kBlock, 0, stream()>>>(n, v);
+    HZ_LAUNCH_CHECK();
+    // step k > 0 applies D^-1 A to the unit vector v / |v|, so |y| estimates lambda_max;
+    // the norms stay on the device and the host reads the last one
+    for (int k = 0; k <= iters; ++k) {
+        k_dinv_av<<<g, kBlock, 0, stream()>>>(n, A.ia, A.ja, A.val, d, v, k > 0 ? q + ((k - 1) & 1) : nullptr, y,
+                                              part);
+        k_sum_partials<<<1, kBlock, 0, stream()>>>(part, g, q + (k & 1));
+        HZ_LAUNCH_CHECK();
+        std::swap(v, y);
+    }
+    double last = 0.0;
+    HZ_CHECK(memcpy_sync(&last, q + (iters & 1), sizeof(double), hipMemcpyDeviceToHost));
+    return iters > 0 && last > 0.0 ? std::sqrt(last) : 0.0;
+}
+
 double learned_parameters(int nagg, const float *out, float *omega) {
     if (nagg <= 0) return 1.0;
     const int g = grid_for(nagg);

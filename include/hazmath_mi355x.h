@@ -457,7 +457,9 @@ INT hz_mlp_train(float *theta, const float *x, const float *t, INT B, INT epochs
  * ------------------------------------------------------------------------- */
 /* PCG with smoothed aggregation whose per-aggregate prolongator damping and
  * per-level Chebyshev bound scale come from the parameter network (weights
- * theta, HZ_MLP_PARAMS floats), VMB aggregation and `cheb_steps` pre- and
+ * theta, HZ_MLP_PARAMS floats; the scale is raised to at least
+ * min(1, 1.1 lambda_est / Gershgorin bound), lambda_est from 60 power
+ * iterations on D^-1 A, so the interval covers lambda_max), VMB aggregation and `cheb_steps` pre- and
  * post-smoothing steps of the HZ_SMOOTHER_CHEBY smoother (amgparam's strength
  * threshold, max_aggregation, coarse_dof and cycle are used).  If it
  * needs more than `budget` iterations, fails, or its FP64 true relative
