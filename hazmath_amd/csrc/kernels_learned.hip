@@ -289,7 +289,8 @@ namespace {
 // y = s D^-1 A v with s = 1/|v| from the previous step's sum of squares
 // (qprev; s = 1 when null or zero; d: first diagonal entries, rows with d = 0
 // give 0), and per-block partial sums of y_i^2.  16 lanes share a row (the
-// loads coalesce); the group's first lane adds the products in row order.
+// loads coalesce) and combine their products by a butterfly: an estimate,
+// so the summation order is free.
 constexpr int kPowLanes = 16;
 __global__ __launch_bounds__(kBlock) void k_dinv_av(int n, const int *__restrict__ ia, const int *__restrict__ ja,
                                                     const double *__restrict__ a, const double *__restrict__ d,
@@ -299,19 +300,12 @@ __global__ __launch_bounds__(kBlock) void k_dinv_av(int n, const int *__restrict
     const double s = (qprev && *qprev > 0.0) ? 1.0 / sqrt(*qprev) : 1.0;
     const long gt = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int i = (int)(gt / kPowLanes), j = (int)(gt % kPowLanes);
-    const int gbase = threadIdx.x % kWave - j;
     const bool valid = i < n;
     const int k0 = valid ? ia[i] : 0, k1 = valid ? ia[i + 1] : 0;
     double t = 0.0;
-    for (int kb = k0; kb < k1; kb += kPowLanes) {   // trip count uniform within the lane group
-        const int k = kb + j;
-        const double pr = k < k1 ? a[k] * v[ja[k]] : 0.0;
+    for (int k = k0 + j; k < k1; k += kPowLanes) t += a[k] * v[ja[k]];
 #pragma unroll
-        for (int u = 0; u < kPowLanes; ++u) {
-            const double w = __shfl(pr, gbase + u, kWave);
-            if (j == 0 && kb + u < k1) t += w;
-        }
-    }
+    for (int o = kPowLanes / 2; o > 0; o >>= 1) t += __shfl_xor(t, o, kWave);   // within the 16-lane group
     double q = 0.0;
     if (valid && j == 0) {
         const double yi = d[i] != 0.0 ? s * t / d[i] : 0.0;
