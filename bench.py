@@ -27,20 +27,89 @@ from hazmath_amd import capi, problems  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, 8 TB/s
 COPY_CAL_DOUBLES = 8 << 20
-TRAFFIC_FILE = ROOT / "profiles" / "spmv_traffic_kuhn128.json"
+CAL_COPIES = 3
+SPMV_KERNEL = "k_spmv<true>"
 
 
-def measured_traffic(n, nnz):
-    """HBM bytes per fine-level SpMV launch from the committed rocprofv3 PMC
-    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy calibration), when it
-    was taken on this exact matrix; None otherwise."""
+def host_cpu_model():
     try:
-        d = json.loads(TRAFFIC_FILE.read_text())
-    except (OSError, ValueError):
-        return None
-    if d.get("n") != n or d.get("nnz") != nnz:
-        return None
-    return d.get("traffic_bytes")
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return os.uname().machine
+
+
+def _counter_rows(outdir):
+    """(dispatch id, kernel name, counter name, value) rows of every
+    *counter_collection.csv that rocprofv3 wrote under outdir."""
+    import csv
+    rows = []
+    for f in sorted(Path(outdir).rglob("*counter_collection.csv")):
+        with open(f, newline="") as fh:
+            for r in csv.DictReader(fh):
+                try:
+                    rows.append((int(r["Dispatch_Id"]), r["Kernel_Name"], r["Counter_Name"],
+                                 float(r["Counter_Value"])))
+                except (KeyError, ValueError):
+                    continue
+    return rows
+
+
+def measure_traffic(a):
+    """HBM bytes per fine-level PCG SpMV launch, measured now: two rocprofv3
+    passes (FETCH_SIZE, then WRITE_SIZE) over a child run of this script in
+    --traffic-probe mode (one solve, then CAL_COPIES device copies of 64 MiB as
+    its last dispatches).  Each counter is scaled by what it reads for the copy
+    (known bytes read and written).  Runs before this process touches the GPU.
+    Returns (dict, None) or (None, reason)."""
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if not prof:
+        return None, "rocprofv3 not found"
+    per = {}
+    cal = {}
+    with tempfile.TemporaryDirectory(dir="/tmp") as td:
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            out = Path(td) / counter
+            cmd = [prof, "--pmc", counter, "-f", "csv", "-d", str(out), "-o", "c", "--",
+                   sys.executable, str(ROOT / "bench.py"), "--traffic-probe", "--m", str(a.m),
+                   "--agg", a.agg, "--smoother", a.smoother, "--theta", str(a.theta),
+                   "--coarse-dof", str(a.coarse_dof), "--tol", str(a.tol), "--problem", a.problem,
+                   "--precision", a.precision, "--jacobi-weight", str(a.jacobi_weight)]
+            env = dict(os.environ, TMPDIR="/tmp")
+            try:
+                r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.DEVNULL,
+                                   stderr=subprocess.PIPE, timeout=240, start_new_session=True)
+            except subprocess.TimeoutExpired:
+                return None, f"rocprofv3 --pmc {counter} timed out"
+            rows = _counter_rows(out)
+            if not rows:
+                tail = r.stderr.decode(errors="replace")[-300:] if r.stderr else ""
+                return None, f"rocprofv3 --pmc {counter} wrote no counters (rc {r.returncode}): {tail}"
+            vals = {}
+            names = {}
+            for d, k, c, v in rows:
+                if c == counter:
+                    vals[d] = vals.get(d, 0.0) + v
+                    names[d] = k
+            ids = sorted(vals)
+            cal_ids = ids[-CAL_COPIES:]
+            spmv = [vals[d] for d in ids if names[d].startswith(SPMV_KERNEL)]
+            if len(cal_ids) < CAL_COPIES or not spmv:
+                return None, f"{counter}: calibration copies or {SPMV_KERNEL} dispatches not found"
+            cal[counter] = float(np.mean([vals[d] for d in cal_ids]))
+            per[counter] = (float(np.mean(spmv)), len(spmv))
+    cal_bytes = COPY_CAL_DOUBLES * 8
+    f_fetch = cal_bytes / (cal["FETCH_SIZE"] * 1024.0)
+    f_write = cal_bytes / (cal["WRITE_SIZE"] * 1024.0)
+    traffic = per["FETCH_SIZE"][0] * 1024.0 * f_fetch + per["WRITE_SIZE"][0] * 1024.0 * f_write
+    return {"traffic_bytes": traffic, "fetch_kib": per["FETCH_SIZE"][0], "write_kib": per["WRITE_SIZE"][0],
+            "dispatches": per["FETCH_SIZE"][1], "copy_fetch_kib": cal["FETCH_SIZE"],
+            "copy_write_kib": cal["WRITE_SIZE"], "fetch_factor": f_fetch, "write_factor": f_write}, None
 
 AGG = {"hem": capi.HEM, "hec": capi.HEC, "vmb": capi.VMB}
 SMOOTHER = {"jacobi": capi.SMOOTHER_JACOBI, "l1": capi.SMOOTHER_L1DIAG, "gs": capi.SMOOTHER_GS,
@@ -89,6 +158,11 @@ def parse():
                         "learned vs plain AMG-PCG")
     p.add_argument("--cpu-sample-m", type=int, default=128,
                    help="cells per direction of the CPU-baseline sample")
+    p.add_argument("--cpu-runs", type=int, default=5, help="CPU-baseline solves (median reported)")
+    p.add_argument("--no-traffic", action="store_true",
+                   help="skip the two rocprofv3 counter passes that measure the SpMV's HBM traffic")
+    p.add_argument("--traffic-probe", action="store_true",
+                   help="(internal) one solve, then the counter calibration copies, nothing printed")
     return p.parse_args()
 
 
@@ -258,7 +332,8 @@ def main_batched(a, lib):
         procs = max(1, min(a.batch_cpu_procs, os.cpu_count() or 1))
         print(f"[bench] cpu baseline: {len(jobs)} pairs on {procs} processes ...", file=sys.stderr, flush=True)
         tc = time.perf_counter()
-        with mp.get_context("fork").Pool(procs) as pool:
+        # spawned workers: a forked child would inherit this process's GPU state
+        with mp.get_context("spawn").Pool(procs) as pool:
             res = pool.map(_batch_pair_worker, jobs, chunksize=1)
         tc = time.perf_counter() - tc
         rows_done = sum(r_[2] for r_ in res)
@@ -435,6 +510,13 @@ def main():
             print("bench.py: no HIP device", file=sys.stderr)
             sys.exit(1)
         return main_learned(a, lib)
+    traffic, traffic_note = None, "not measured (--no-traffic)"
+    if not a.traffic_probe and not a.no_traffic:
+        # before this process initialises the GPU: the counter passes run in children
+        t_tr = time.perf_counter()
+        traffic, traffic_note = measure_traffic(a)
+        print(f"[bench] traffic passes {time.perf_counter() - t_tr:.1f}s: "
+              f"{traffic['traffic_bytes'] / 1e6 if traffic else traffic_note}", file=sys.stderr, flush=True)
     import torch
     lib = capi.load_library()
     if lib.hz_device_count() < 1 or not torch.cuda.is_available():
@@ -485,6 +567,17 @@ def main():
         finally:
             os.dup2(saved, 1)
 
+    if a.traffic_probe:
+        # one solve, then the calibration copies as the run's last dispatches
+        quiet(step)
+        cal_src = torch.ones(COPY_CAL_DOUBLES, dtype=torch.float64, device=dev)
+        cal_dst = torch.empty_like(cal_src)
+        torch.cuda.synchronize()
+        for _ in range(CAL_COPIES):
+            cal_dst.copy_(cal_src)
+        torch.cuda.synchronize()
+        return
+
     for w in range(a.warmup):
         r = quiet(step)
         print(f"[bench] warmup {w}: iterations {r}", file=sys.stderr, flush=True)
@@ -527,14 +620,6 @@ def main():
                                       a.kernel_reps, C.byref(kt))
         if rc != 0:
             raise RuntimeError("hz_time_fine_kernels failed")
-    # a plain device copy of known size (64 MiB read + 64 MiB written): the
-    # reference point that scales the FETCH_SIZE / WRITE_SIZE counters
-    cal_src = torch.ones(COPY_CAL_DOUBLES, dtype=torch.float64, device=dev)
-    cal_dst = torch.empty_like(cal_src)
-    for _ in range(3):
-        cal_dst.copy_(cal_src)
-    torch.cuda.synchronize()
-    del cal_src, cal_dst
     # algorithmic bytes: 12 nnz (val + col) + 4 (n+1) (row pointers) + 8 n (x) + 8 n (y)
     bytes_spmv = 12 * nnz + 4 * (n + 1) + 16 * n
     bytes_jacobi = 12 * nnz + 4 * (n + 1) + 24 * n
@@ -550,16 +635,22 @@ def main():
         else:
             ci, cj, cv = make_problem(a, a.cpu_sample_m)
             cb = problems.uniform_rhs(ci.size - 1, seed=1)
-        print("[bench] cpu baseline ...", file=sys.stderr, flush=True)
-        t_cpu, r_cpu, tr, x_cpu = cpu_baseline(a, ci, cj, cv, cb)
-        print(f"[bench] cpu baseline {t_cpu:.2f}s, iterations {r_cpu}", file=sys.stderr, flush=True)
-        sample = (f"one full solve (setup + PCG to {a.tol:g}) of the same workload with "
+        runs = []
+        for k in range(max(1, a.cpu_runs)):
+            print(f"[bench] cpu baseline run {k} ...", file=sys.stderr, flush=True)
+            t_cpu, r_cpu, tr, x_cpu = cpu_baseline(a, ci, cj, cv, cb)
+            runs.append((t_cpu, tr.setup_s, tr.solve_s))
+            print(f"[bench] cpu baseline {t_cpu:.2f}s, iterations {r_cpu}", file=sys.stderr, flush=True)
+        t_med = float(np.median([r_[0] for r_ in runs]))
+        sample = (f"median of {len(runs)} full solves (setup + PCG to {a.tol:g}) of the same workload with "
                   f"{a.cpu_sample_m}^3 cells, serial C restatement of the reference path, "
-                  f"{r_cpu} iterations")
+                  f"{r_cpu} iterations each")
         scale = 1.0 if a.cpu_sample_m == a.m else (a.m / a.cpu_sample_m) ** 3
-        cpu = {"value": t_cpu * scale, "unit": "s", "cores": 1, "kind": "port", "sample": sample,
-               "iterations": int(r_cpu), "setup_s": tr.setup_s, "solve_s": tr.solve_s,
-               "host_cpu": os.uname().machine, "host_nproc": os.cpu_count()}
+        cpu = {"value": t_med * scale, "unit": "s", "cores": 1, "kind": "port", "sample": sample,
+               "iterations": int(r_cpu), "runs_s": [r_[0] for r_ in runs],
+               "setup_s": float(np.median([r_[1] for r_ in runs])),
+               "solve_s": float(np.median([r_[2] for r_ in runs])),
+               "host_cpu": host_cpu_model(), "host_nproc": os.cpu_count()}
         if a.cpu_sample_m == a.m:
             # error of the GPU solution against the FP64 oracle's
             cpu["x_rel_diff_vs_gpu"] = float(np.linalg.norm(x_cpu - x) / np.linalg.norm(x_cpu))
@@ -601,7 +692,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": spmv_gbs / HBM_PEAK_GBS,
-            "traffic": measured_traffic(n, nnz),
+            "traffic": traffic["traffic_bytes"] if traffic else None,
+            "traffic_detail": traffic if traffic else traffic_note,
             "algorithmic_bytes_per_launch": bytes_spmv,
             "avg_launch_ms": pcg_spmv_ms,
             "launches_timed": pcg_spmv_count,

@@ -84,7 +84,7 @@ class hz_options(C.Structure):
     _fields_ = [("jacobi_weight", REAL), ("hec_rng_source", INT), ("verbose_timing", INT),
                 ("return_on_fatal", INT), ("vcycle_precision", INT), ("batch_time_kernels", INT),
                 ("time_pcg_spmv", INT), ("hem_host_max_rows", INT),
-                ("rap_wave_stage_pairs", INT)]
+                ("rap_wave_stage_pairs", INT), ("hem_pipeline_max_sweeps", INT)]
 
 
 class hz_batch_stats(C.Structure):
@@ -99,7 +99,9 @@ class hz_solve_info(C.Structure):
                 ("setup_ms", REAL), ("solve_ms", REAL), ("total_ms", REAL),
                 ("final_relres", REAL), ("condest", REAL), ("relres_count", INT),
                 ("relres_hist", REAL * 4096), ("fallback_used", INT), ("pcg_spmv_ms", REAL),
-                ("pcg_spmv_count", INT), ("learned_iterations", INT), ("learned_total_ms", REAL)]
+                ("pcg_spmv_count", INT), ("learned_iterations", INT), ("learned_total_ms", REAL),
+                ("hem_pipeline_giveups", INT), ("cheb_scale", REAL * MAX_AMG_LVL),
+                ("lambda_ratio", REAL * MAX_AMG_LVL)]
 
 
 class hz_hierarchy(C.Structure):

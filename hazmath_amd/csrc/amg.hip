@@ -143,7 +143,13 @@ int hem_level(const DevCSR &N, const AMG_param *param, int *agg, int *nagg) {
     return st;
 }
 
+// print_cputime, message.c:73-77 (wall seconds here).
+void print_cputime(const char *message, double seconds) {
+    std::printf("%s costs %.4f seconds.\n", message, seconds);
+}
+
 int amg_setup(Hierarchy &H, const DevCSR &A0, AMG_param *param, bool) {
+    const double setup_start = wall_ms();
     const short min_cdof = (short)std::max(param->coarse_dof, 1);
     int max_levels = std::max<int>(1, param->max_levels);
     if (max_levels > HZ_MAX_AMG_LVL) max_levels = HZ_MAX_AMG_LVL;
@@ -218,11 +224,14 @@ int amg_setup(Hierarchy &H, const DevCSR &A0, AMG_param *param, bool) {
                 // estimate (60 iterations) is from below, hence 1.1x; the Gershgorin bound (scale 1)
                 // always covers lambda_max, so the floor never exceeds 1.
                 const ChebLevel g1 = cheby_prepare(L.A, 1.0);
-                if (g1.lmax > 0.0) {
-                    const double floor = std::min(1.0, 1.1 * power_lambda_dinv(L.A, g1.diag, 60) / g1.lmax);
-                    scale = std::min(1.5, std::max(scale, floor));
-                }
                 if ((int)H.cheb_scale.size() < max_levels) H.cheb_scale.assign(max_levels, 1.0);
+                if ((int)H.lambda_ratio.size() < max_levels) H.lambda_ratio.assign(max_levels, 0.0);
+                if (g1.lmax > 0.0) {
+                    const double ratio = power_lambda_dinv(L.A, g1.diag, 60) / g1.lmax;
+                    const double floor = std::min(1.0, 1.1 * ratio);
+                    scale = std::min(1.5, std::max(scale, floor));
+                    H.lambda_ratio[lvl] = ratio;
+                }
                 H.cheb_scale[lvl] = scale;
                 omega_agg = om;
             }
@@ -256,10 +265,13 @@ int amg_setup(Hierarchy &H, const DevCSR &A0, AMG_param *param, bool) {
         sell_from_csr(Cst.A, Cst.S);
         Cst.Ainv = arena().alloc_n<double>((size_t)Cst.A.n * Cst.A.n + 1);
         if (coarse_dense_setup(Cst.A, Cst.Ainv) != 0)
-            std::fprintf(stderr, "hazmath_mi355x: zero pivot in the coarsest L D L^T factorization\n");
+            std::fprintf(stderr, "hazmath_mi355x: zero pivot in the coarsest-level Gauss-Jordan inverse\n");
     } else {
         Cst.cgwork = arena().alloc_n<double>((size_t)Cst.A.n * 4 + 4);
     }
+    // amg_setup_ua.c:1426-1431 / 1668-1673: print_amg_complexity (message.c:105-136)
+    // and the setup time
+    if (param->print_level > 0) HZ_CHECK(hipStreamSynchronize(stream()));
     if (param->print_level >= 2) {
         double gc = 0.0, oc = 0.0;
         std::printf("-----------------------------------------------------------\n");
@@ -279,6 +291,9 @@ int amg_setup(Hierarchy &H, const DevCSR &A0, AMG_param *param, bool) {
         std::printf("  Operator complexity = %.3f\n", oc);
         std::printf("-----------------------------------------------------------\n");
     }
+    if (param->print_level > 0)
+        print_cputime(param->AMG_type == HZ_SA_AMG ? "Smoothed aggregation setup" : "Unsmoothed aggregation setup",
+                      (wall_ms() - setup_start) * 1e-3);
     return HZ_SUCCESS;
 }
 
@@ -967,6 +982,7 @@ FINISHED:
 // ---------------------------------------------------------------------------
 AmgSolveResult amg_solve(Hierarchy &H, const double *b, double *x, double tol, int MaxIt, int prt) {
     AmgSolveResult res;
+    const double solve_start = wall_ms();
     Level &L0 = H.lv[0];
     const int n = L0.A.n;
     double *r = arena().alloc_n<double>((size_t)n + 1);
@@ -1017,6 +1033,8 @@ AmgSolveResult amg_solve(Hierarchy &H, const double *b, double *x, double tol, i
                         relres1);
         else if (iter >= 0)
             std::printf("Num_iter(amg_solve.c) = %lld with relative residual %e.\n", (long long)iter, relres1);
+        HZ_CHECK(hipStreamSynchronize(stream()));
+        print_cputime("AMG solve", (wall_ms() - solve_start) * 1e-3);
     }
     HZ_CHECK(hipStreamSynchronize(stream()));
     res.iter = iter;

@@ -56,8 +56,12 @@ struct Hierarchy {
     // and Chebyshev bound scale come from the network with these weights
     const float *mlp_theta = nullptr;
     std::vector<double> cheb_scale;   // per level (default 1.0)
+    std::vector<double> lambda_ratio; // learned: lambda_est(D^-1 A) / Gershgorin bound per level
     LaunchTimer *spmv_timer = nullptr;  // when set: events around the PCG's fine-level SpMV
 };
+
+// print_cputime (message.c:73-77): "<message> costs <seconds> seconds."
+void print_cputime(const char *message, double seconds);
 
 // Runs the reference's setup loop on the device.  A0 arrays are borrowed.
 // Returns SUCCESS, a negative HAZmath code, or a negative exit code

@@ -10,7 +10,7 @@
 using namespace hz;
 
 namespace {
-hz_options g_opt = {1.0, 0, 0, 0, 0, 0, 0, 0, 0};
+hz_options g_opt = {1.0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 hz_solve_info g_info;
 }  // namespace
 
@@ -235,19 +235,33 @@ INT batch_core(const DevCSR &A, const std::vector<int> &off, const double *db, d
     return HZ_SUCCESS;
 }
 
+// Three timing events, destroyed on every path out of core_solve.
+struct SolveEvents {
+    hipEvent_t e[3] = {nullptr, nullptr, nullptr};
+    SolveEvents() {
+        for (auto &x : e) HZ_CHECK(hipEventCreate(&x));
+    }
+    ~SolveEvents() {
+        for (auto &x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+    SolveEvents(const SolveEvents &) = delete;
+    SolveEvents &operator=(const SolveEvents &) = delete;
+};
+
 // linear_solver_dcsr_krylov_amg (itsolver.c:846-931) on device-resident data.
 INT core_solve(const DevCSR &A, const double *db, double *dx, linear_itsolver_param *it,
                AMG_param *amg, const float *mlp_theta = nullptr) {
     std::memset(&g_info, 0, sizeof(g_info));
     const short prt = it->linear_print_level;
-    hipEvent_t e0, e1, e2;
-    HZ_CHECK(hipEventCreate(&e0));
-    HZ_CHECK(hipEventCreate(&e1));
-    HZ_CHECK(hipEventCreate(&e2));
+    SolveEvents ev;
+    hipEvent_t e0 = ev.e[0], e1 = ev.e[1], e2 = ev.e[2];
     HZ_CHECK(hipEventRecord(e0, stream()));
-    if (prt > 0) std::printf("\n Calling UA AMG ...\n");
+    const double wall0 = wall_ms();
+    if (prt > 0) std::printf(amg->AMG_type == HZ_SA_AMG ? "\n Calling SA AMG ...\n" : "\n Calling UA AMG ...\n");
     int st = check_supported(it, amg);
     if (st < 0) return st;
+    const int giveups0 = g_hem_pipe_giveups;
     Hierarchy H;
     H.cp.jacobi_w = g_opt.jacobi_weight;
     H.hec_rng = g_opt.hec_rng_source;
@@ -259,6 +273,7 @@ INT core_solve(const DevCSR &A, const double *db, double *dx, linear_itsolver_pa
     amg_prepare_cycle(H, amg);
     H.cp.jacobi_w = g_opt.jacobi_weight;
     HZ_CHECK(hipEventRecord(e1, stream()));
+    const double wall_solve0 = wall_ms();
     INT iter;
     double condest = 0.0;
     PcgResult pr;
@@ -293,13 +308,24 @@ INT core_solve(const DevCSR &A, const double *db, double *dx, linear_itsolver_pa
     }
     HZ_CHECK(hipEventRecord(e2, stream()));
     HZ_CHECK(hipEventSynchronize(e2));
+    // itsolver.c:201-206 and 917-922
+    if (prt >= 2 && iter >= 0) {
+        print_cputime("Iterative method", (wall_ms() - wall_solve0) * 1e-3);
+        std::printf("**********************************************************\n");
+    }
+    if (prt >= 1) {
+        print_cputime("AMG_Krylov method totally", (wall_ms() - wall0) * 1e-3);
+        std::printf("**********************************************************\n");
+    }
     float ms_setup = 0.f, ms_solve = 0.f;
     HZ_CHECK(hipEventElapsedTime(&ms_setup, e0, e1));
     HZ_CHECK(hipEventElapsedTime(&ms_solve, e1, e2));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipEventDestroy(e2);
     g_info.iterations = iter;
+    g_info.hem_pipeline_giveups = g_hem_pipe_giveups - giveups0;
+    for (int l = 0; l < H.nl && l < HZ_MAX_AMG_LVL; ++l) {
+        if (l < (int)H.cheb_scale.size() && mlp_theta) g_info.cheb_scale[l] = H.cheb_scale[l];
+        if (l < (int)H.lambda_ratio.size()) g_info.lambda_ratio[l] = H.lambda_ratio[l];
+    }
     g_info.num_levels = H.nl;
     for (int l = 0; l < H.nl && l < HZ_MAX_AMG_LVL; ++l) {
         g_info.level_rows[l] = H.lv[l].A.n;
@@ -497,6 +523,7 @@ void hz_set_options(const hz_options *opt) {
     g_opt = *opt;
     g_hem_host_max = opt->hem_host_max_rows;
     g_rap_wave_pairs = opt->rap_wave_stage_pairs;
+    g_hem_pipe_max_sweeps = opt->hem_pipeline_max_sweeps;
 }
 void hz_get_last_info(hz_solve_info *info) { *info = g_info; }
 
@@ -554,7 +581,9 @@ INT hz_setup_hierarchy(dCSRmat *A, AMG_param *amgparam, hz_hierarchy *out) {
         const DevCSR dA = upload_csr(A);
         Hierarchy H;
         H.hec_rng = g_opt.hec_rng_source;
+        const int giveups0 = g_hem_pipe_giveups;
         int st = amg_setup(H, dA, amgparam, true);
+        g_info.hem_pipeline_giveups = g_hem_pipe_giveups - giveups0;
         if (st <= -1000) return fatal(st);
         if (st < 0) return st;
         out->num_levels = H.nl;
@@ -758,7 +787,7 @@ INT linear_solver_amg(dCSRmat *A, dvector *b, dvector *x, AMG_param *param) {
         const DevCSR dA = upload_csr(A);
         const double *db = upload(b->val, (size_t)A->row);
         double *dx = upload(x->val, (size_t)A->row);
-        if (prt > 0) std::printf("\n Calling UA AMG ...\n");
+        if (prt > 0) std::printf(param->AMG_type == HZ_SA_AMG ? "\n Calling SA AMG ...\n" : "\n Calling UA AMG ...\n");
         Hierarchy H;
         H.cp.jacobi_w = g_opt.jacobi_weight;
         H.hec_rng = g_opt.hec_rng_source;
@@ -781,7 +810,7 @@ INT linear_solver_amg(dCSRmat *A, dvector *b, dvector *x, AMG_param *param) {
         g_info.total_ms = ms;
         g_info.final_relres = r.relres;
         if (prt > 0) {
-            std::printf("AMG totally costs %f seconds\n", ms * 1e-3);
+            print_cputime("AMG totally", ms * 1e-3);
             std::printf("**********************************************************\n");
         }
         std::fflush(stdout);
@@ -995,6 +1024,10 @@ INT hz_mlp_train(float *theta, const float *x, const float *t, INT B, INT epochs
 
 INT hz_krylov_amg_learned(dCSRmat *A, dvector *b, dvector *x, linear_itsolver_param *itparam,
                           AMG_param *amgparam, const float *theta, INT budget, INT cheb_steps) {
+    if (itparam->linear_stop_type != HZ_STOP_REL_RES && itparam->linear_stop_type != HZ_STOP_MOD_REL_RES) {
+        std::printf("### ERROR: the learned AMG-PCG judges its result by the true residual: stop type 1 or 3\n");
+        return HZ_ERROR_INPUT_PAR;
+    }
     if (!device_ready()) return HZ_ERROR_DEVICE;
     try {
         arena().reset();
@@ -1012,22 +1045,34 @@ INT hz_krylov_amg_learned(dCSRmat *A, dvector *b, dvector *x, linear_itsolver_pa
         lp.presmooth_iter = lp.postsmooth_iter = (SHORT)std::max(1, cheb_steps);
         linear_itsolver_param li = *itparam;
         li.linear_maxit = std::min<int>(itparam->linear_maxit, budget);
+        // The FP64 true residual is the judge, normalised as the caller's stop
+        // type normalises it (krylov.c:441-466): by max(1e-20, ||b - A x0||)
+        // for the relative residual, by ||x|| for the modified one.
+        DevSELL S;
+        sell_from_csr(dA, S);
+        double *w = arena().alloc_n<double>(n + 1);
+        auto residual_norm = [&](const double *xv) {
+            residual(S, db, xv, w);
+            std::vector<double> hw(n);
+            HZ_CHECK(memcpy_sync(hw.data(), w, sizeof(double) * n, hipMemcpyDeviceToHost));
+            double rr = 0.0;
+            for (size_t i = 0; i < n; ++i) rr += hw[i] * hw[i];
+            return std::sqrt(rr);
+        };
+        const double normr0 = std::max(kSmallReal, residual_norm(dx0));
         INT st = core_solve(dA, db, dx, &li, &lp, dth);
         bool ok = st >= 0 && st <= budget;
         if (ok) {
-            // the FP64 true relative residual is the judge
-            DevSELL S;
-            sell_from_csr(dA, S);
-            double *w = arena().alloc_n<double>(n + 1);
-            residual(S, db, dx, w);
-            std::vector<double> hw(n), hb(n);
-            HZ_CHECK(memcpy_sync(hw.data(), w, sizeof(double) * n, hipMemcpyDeviceToHost));
-            double rr = 0.0, bb = 0.0;
-            for (size_t i = 0; i < n; ++i) {
-                rr += hw[i] * hw[i];
-                bb += b->val[i] * b->val[i];
+            const double rn = residual_norm(dx);
+            double denom = normr0;
+            if (itparam->linear_stop_type == HZ_STOP_MOD_REL_RES) {
+                std::vector<double> hx(n);
+                HZ_CHECK(memcpy_sync(hx.data(), dx, sizeof(double) * n, hipMemcpyDeviceToHost));
+                double xx = 0.0;
+                for (double v : hx) xx += v * v;
+                denom = std::max(kSmallReal, std::sqrt(xx));
             }
-            ok = std::sqrt(rr) <= itparam->linear_tol * std::max(std::sqrt(bb), 1e-300) * 1.0000001;
+            ok = rn <= itparam->linear_tol * denom * 1.0000001;
         }
         const hz_solve_info learned_info = g_info;
         const INT learned_st = st;

@@ -178,6 +178,11 @@ void rap_general(const DevCSR &R, const DevCSR &A, const DevCSR &P, DevCSR &C);
 // through the one-thread-per-row kernel, -2 = the default with a 16-column
 // limit.
 extern int g_rap_wave_pairs;
+// hz_options.hem_pipeline_max_sweeps: sweeps after which HEM's one-launch
+// pipeline hands a chunk to the per-round kernel (0 = 8192).
+extern int g_hem_pipe_max_sweeps;
+// HEM levels whose pipeline gave up (process-wide count).
+extern int g_hem_pipe_giveups;
 // y = A x (row sums from 0.0, CSR order) and x += alpha P xc (dcsr_aAxpy).
 void csr_mxv(const DevCSR &A, const double *x, double *y);
 void csr_prolong(const DevCSR &P, const double *xc, double alpha, const double *alpha_dev, double *x);

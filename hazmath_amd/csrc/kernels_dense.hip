@@ -341,13 +341,23 @@ bool mfma_layout_ok() {
         mag = std::max(mag, std::fabs(r2[i]));
     }
     ok = (err <= 1e-12 * mag) ? 1 : 0;
-    if (!ok) std::fprintf(stderr, "hazmath_mi355x: MFMA layout check failed (%g); using FMA GEMM\n", err);
     return ok == 1;
+}
+
+// The coarsest inverse runs on the FP64 matrix cores only: a failed lane-layout
+// self-check is a device error, never a silent switch to another kernel.
+void require_mfma() {
+    if (!mfma_layout_ok()) {
+        std::fprintf(stderr, "hazmath_mi355x: the FP64 MFMA GEMM disagrees with the FMA GEMM on its "
+                             "self-check; refusing to build the coarsest inverse\n");
+        throw DeviceError{hipErrorUnknown, "MFMA lane-layout self-check"};
+    }
 }
 
 // In-place inverse of the dense n x n matrix M by 64-column block Gauss-Jordan.
 int blocked_inverse(int n, double *M) {
-    const bool mfma = mfma_layout_ok();
+    require_mfma();
+    const bool mfma = true;
     double *Pinv = arena().alloc_n<double>((size_t)kTile * kTile);
     double *R = arena().alloc_n<double>((size_t)kTile * n);
     double *Cb = arena().alloc_n<double>((size_t)n * kTile);

@@ -319,7 +319,7 @@ __global__ __launch_bounds__(kBlock) void k_hem_pipeline(int n, const int *__res
                                                          const int *__restrict__ pref,
                                                          const int *__restrict__ npref, int *fstate, int *pick,
                                                          int *linkpar, int *__restrict__ ticket,
-                                                         int *__restrict__ status) {
+                                                         int *__restrict__ status, int max_sweeps) {
     __shared__ int chunk;
     const int nchunks = (n + kHemChunk - 1) / kHemChunk;
     for (;;) {
@@ -340,7 +340,7 @@ __global__ __launch_bounds__(kBlock) void k_hem_pipeline(int n, const int *__res
             if ((it & 15) == 15) {
                 int stop = 0;
                 if (threadIdx.x == 0) {
-                    if (it >= kPipeMaxSweeps) st(status, 1);
+                    if (it >= max_sweeps) st(status, 1);
                     stop = ld(status);
                 }
                 if (__syncthreads_or(stop)) return;
@@ -459,6 +459,9 @@ __global__ void k_compact_leftovers(int n, const int *__restrict__ leftover, con
 
 }  // namespace
 
+int g_hem_pipe_max_sweeps = 0;
+int g_hem_pipe_giveups = 0;
+
 // Pass 2 of HEM (amg_setup_ua.c:1018-1046) on the GPU: the nl leftovers in
 // list (ascending) join the smallest neighbouring aggregate, first in row
 // order on ties, with sizes growing as they join.  agg holds the pass-1
@@ -530,13 +533,15 @@ int hem_aggregate_gpu(const DevCSR &N, int *agg, int *nagg_out) {
     const int gr = std::max(1, std::min(8192, nchunks));
     int *pipe = arena().alloc_n<int>(2);   // ticket, status
     HZ_CHECK(hipMemsetAsync(pipe, 0, 2 * sizeof(int), st));
+    const int max_sweeps = g_hem_pipe_max_sweeps > 0 ? g_hem_pipe_max_sweeps : kPipeMaxSweeps;
     k_hem_pipeline<<<std::min(nchunks, 2048), kBlock, 0, st>>>(n, N.ia, N.ja, pref, npref, fstate, pick,
-                                                              linkpar, pipe, pipe + 1);
+                                                              linkpar, pipe, pipe + 1, max_sweeps);
     HZ_LAUNCH_CHECK();
     int pipe_status = 0;
     HZ_CHECK(memcpy_sync(&pipe_status, pipe + 1, sizeof(int), hipMemcpyDeviceToHost));
     const int kRoundsPerCheck = 2;
     int rounds1 = 0, rounds2 = 0;
+    if (pipe_status != 0) ++g_hem_pipe_giveups;
     for (int round = 0; pipe_status != 0; round += kRoundsPerCheck) {
         if (round + kRoundsPerCheck > kCounterSlots) {
             HZ_CHECK(hipMemsetAsync(undecided, 0, sizeof(int) * kCounterSlots, st));

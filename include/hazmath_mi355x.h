@@ -333,6 +333,10 @@ typedef struct {
                               same matrix.  0: 512 (the LDS capacity; larger values are capped
                               there), -1: every row one thread per row, -2: the default with a
                               16-column limit */
+    INT  hem_pipeline_max_sweeps; /* HEM's one-launch GPU pipeline gives up on a chunk of vertices
+                              after this many sweeps (checked every 16) and the remaining pass-1
+                              decisions run as one launch per round; both give the same
+                              aggregates.  0: 8192 */
 } hz_options;
 
 void hz_get_options(hz_options *opt);
@@ -358,6 +362,12 @@ typedef struct {
     INT    learned_iterations;  /* hz_krylov_amg_learned: the learned attempt's result (iterations or
                                    error code) and its device time, also when it fell back */
     REAL   learned_total_ms;
+    INT    hem_pipeline_giveups;   /* HEM levels whose one-launch GPU pipeline handed its remaining
+                                      decisions to the per-round kernel (hem_pipeline_max_sweeps) */
+    REAL   cheb_scale[HZ_MAX_AMG_LVL];   /* learned SA: Chebyshev bound scale in force per level
+                                            (after the floor), 0 on levels without one */
+    REAL   lambda_ratio[HZ_MAX_AMG_LVL]; /* learned SA: estimated lambda_max(D^-1 A) over the
+                                            Gershgorin bound per level, 0 where not estimated */
 } hz_solve_info;
 
 void hz_get_last_info(hz_solve_info *info);
@@ -401,8 +411,9 @@ INT hz_rap_agg(dCSRmat *A, const INT *aggregates, INT num_aggregates, dCSRmat *A
  * U^T D U factorization, hazmath_direct.c:633-854).  Returns
  * HZ_ERROR_SOLVER_MISC on a zero pivot. */
 INT hz_dense_inverse(INT n, REAL *M);
-/* 1 when the dense inverse runs its rank-64 updates on the FP64 matrix cores
- * (the one-time lane-layout self-check passed), 0 when it uses the FMA kernel. */
+/* 1 when the one-time lane-layout self-check of the FP64 MFMA GEMM passed (the
+ * dense inverse runs its rank-64 updates on the matrix cores), 0 when it
+ * failed: then every dense inverse returns HZ_ERROR_DEVICE. */
 INT hz_dense_mfma_active(void);
 
 /* Average device duration of the fine-level solve kernels on the library's
@@ -462,10 +473,13 @@ INT hz_mlp_train(float *theta, const float *x, const float *t, INT B, INT epochs
  * iterations on D^-1 A, so the interval covers lambda_max), VMB aggregation and `cheb_steps` pre- and
  * post-smoothing steps of the HZ_SMOOTHER_CHEBY smoother (amgparam's strength
  * threshold, max_aggregation, coarse_dof and cycle are used).  If it
- * needs more than `budget` iterations, fails, or its FP64 true relative
- * residual exceeds itparam->linear_tol, x is reset to the caller's guess and
+ * needs more than `budget` iterations, fails, or its FP64 true residual
+ * ||b - A x|| exceeds itparam->linear_tol times the stop type's normaliser
+ * (max(1e-20, ||b - A x0||) for stop type 1, ||x|| for stop type 3; stop
+ * type 2 returns ERROR_INPUT_PAR), x is reset to the caller's guess and
  * the plain method of itparam/amgparam (linear_solver_dcsr_krylov_amg) solves
- * instead; hz_solve_info.fallback_used records which.  Returns the iteration
+ * instead, its result returned as that solver returns it;
+ * hz_solve_info.fallback_used records which.  Returns the iteration
  * count of the method that produced x (or a negative error code). */
 INT hz_krylov_amg_learned(dCSRmat *A, dvector *b, dvector *x, linear_itsolver_param *itparam,
                           AMG_param *amgparam, const float *theta, INT budget, INT cheb_steps);

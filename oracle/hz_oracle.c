@@ -892,7 +892,13 @@ void orc_cheby_smooth(const dCSRmat *A, const REAL *b, REAL *x, INT steps, REAL 
 
 /* amg_setup_unsmoothP_unsmoothR, amg_setup_ua.c:1228-1443, and for
  * AMG_type = SA amg_setup_smoothP_smoothR, amg_setup_ua.c:1461-1687. */
+/* print_cputime, message.c:73-77 (wall seconds) */
+static void orc_print_cputime(const char *message, double seconds) {
+    printf("%s costs %.4f seconds.\n", message, seconds);
+}
+
 static INT orc_setup(orc_amg *m, const dCSRmat *A, AMG_param *param) {
+    const double setup_start = orc_now();
     memset(m, 0, sizeof(*m));
     const SHORT min_cdof = (SHORT)(param->coarse_dof > 1 ? param->coarse_dof : 1);
     SHORT max_levels = param->max_levels;
@@ -977,6 +983,10 @@ static INT orc_setup(orc_amg *m, const dCSRmat *A, AMG_param *param) {
         printf("  Operator complexity = %.3f\n", oc);
         printf("-----------------------------------------------------------\n");
     }
+    /* amg_setup_ua.c:1426-1431, 1668-1673 */
+    if (param->print_level > 0)
+        orc_print_cputime(param->AMG_type == HZ_SA_AMG ? "Smoothed aggregation setup" : "Unsmoothed aggregation setup",
+                          orc_now() - setup_start);
     return status;
 }
 
@@ -1321,7 +1331,8 @@ INT orc_krylov_amg(dCSRmat *A, dvector *b, dvector *x, linear_itsolver_param *it
     if (trace) memset(trace, 0, sizeof(*trace));
     orc_amg *m = (orc_amg *)xcalloc(1, sizeof(orc_amg));
     double t0 = orc_now();
-    if (prt > 0) printf("\n Calling UA AMG ...\n");
+    /* itsolver.c:866-883 */
+    if (prt > 0) printf(amgparam->AMG_type == HZ_SA_AMG ? "\n Calling SA AMG ...\n" : "\n Calling UA AMG ...\n");
     if (amgparam->smoother != HZ_SMOOTHER_JACOBI && amgparam->smoother != HZ_SMOOTHER_L1DIAG &&
         amgparam->smoother != 91 && !orc_is_gs_family(amgparam->smoother)) {
         printf("### ERROR: Wrong smoother type %lld!\n", (long long)amgparam->smoother);
@@ -1361,6 +1372,15 @@ INT orc_krylov_amg(dCSRmat *A, dvector *b, dvector *x, linear_itsolver_param *it
         printf("Estimated condition number = %13.6e\n", condest);
     }
     double t2 = orc_now();
+    /* itsolver.c:201-206, 917-922 */
+    if (prt >= 2 && iter >= 0) {
+        orc_print_cputime("Iterative method", t2 - t1);
+        printf("**********************************************************\n");
+    }
+    if (prt >= 1) {
+        orc_print_cputime("AMG_Krylov method totally", t2 - t0);
+        printf("**********************************************************\n");
+    }
     if (trace) {
         trace->iterations = iter;
         trace->condest = condest;
@@ -1381,8 +1401,11 @@ INT orc_linear_solver_amg(dCSRmat *A, dvector *b, dvector *x, AMG_param *param) 
         !orc_is_gs_family(param->smoother))
         return HZ_ERROR_AMG_SMOOTH_TYPE;
     orc_amg *m = (orc_amg *)xcalloc(1, sizeof(orc_amg));
-    if (prt > 0) printf("\n Calling UA AMG ...\n");
+    const double t0 = orc_now();
+    /* itsolver.c:545-560 */
+    if (prt > 0) printf(param->AMG_type == HZ_SA_AMG ? "\n Calling SA AMG ...\n" : "\n Calling UA AMG ...\n");
     INT status = orc_setup(m, A, param);
+    const double t_solve = orc_now();
     if (status < 0) { orc_amg_free(m); free(m); return status; }
     /* amg_solve runs mgcycle with the caller's param directly */
     orc_prec_data pd;
@@ -1415,8 +1438,13 @@ INT orc_linear_solver_amg(dCSRmat *A, dvector *b, dvector *x, AMG_param *param) 
             printf("### HAZMATH WARNING: Max iter %lld reached with rel. resid. %e.\n", (long long)MaxIt, relres1);
         else if (iter >= 0)
             printf("Num_iter(amg_solve.c) = %lld with relative residual %e.\n", (long long)iter, relres1);
+        orc_print_cputime("AMG solve", orc_now() - t_solve);   /* amg_solve.c:153-157 */
     }
     memcpy(x->val, L0->x, sizeof(REAL) * (size_t)n);
+    if (prt > 0) {   /* itsolver.c:631-636 */
+        orc_print_cputime("AMG totally", orc_now() - t0);
+        printf("**********************************************************\n");
+    }
     fflush(stdout);
     orc_amg_free(m);
     free(m);
