@@ -14,6 +14,7 @@ import argparse
 import ctypes as C
 import json
 import os
+import re
 import sys
 import time
 from pathlib import Path
@@ -98,9 +99,12 @@ def measure_traffic(a):
                     names[d] = k
             ids = sorted(vals)
             cal_ids = ids[-CAL_COPIES:]
-            spmv = [vals[d] for d in ids if names[d].startswith(SPMV_KERNEL)]
-            if len(cal_ids) < CAL_COPIES or not spmv:
-                return None, f"{counter}: calibration copies or {SPMV_KERNEL} dispatches not found"
+            # demangled names: "void hz::(anonymous namespace)::k_spmv<true>(int, ...)"
+            spmv = [vals[d] for d in ids if re.search(r"\bk_spmv<true>\(", names[d])]
+            if len(cal_ids) < CAL_COPIES or not spmv or not all("opy" in names[d] for d in cal_ids):
+                seen = sorted({names[d][:60] for d in ids})[:8]
+                return None, (f"{counter}: calibration copies or {SPMV_KERNEL} dispatches not found "
+                              f"({len(ids)} dispatches; last {[names[d][:40] for d in cal_ids]}; seen {seen})")
             cal[counter] = float(np.mean([vals[d] for d in cal_ids]))
             per[counter] = (float(np.mean(spmv)), len(spmv))
     cal_bytes = COPY_CAL_DOUBLES * 8

@@ -67,10 +67,13 @@ def _code_case(case):
     if case == "stag":
         # singular A with A (1e22 * 1) = 0 exactly: r0 = b, and every PCG
         # update (far below the 2^21 spacing of doubles near 1e22) is lost,
-        # so check II restarts with the true residual unchanged until MAX_STAG
+        # so check II restarts with the true residual unchanged until MAX_STAG.
+        # b has mean zero, so every coarse system stays consistent and the
+        # coarse CG returns corrections of the size of b
         ia, ja, va = _neumann2d(40)
         n = ia.size - 1
         b = problems.uniform_rhs(n, seed=5)
+        b -= b.mean()
         return ia, ja, va, b, np.full(n, 1e22), dict(tol=1e-6, maxit=100, coarse_solver=0), -42
     ia, ja, va = problems.poisson2d_p1(64)
     n = ia.size - 1
