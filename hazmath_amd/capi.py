@@ -24,6 +24,7 @@ LIB_PATH = PKG_DIR / "lib" / "libhazmath_mi355x.so"
 
 # Error codes (include/macro.h:38-75)
 SUCCESS = 0
+ERROR_INPUT_PAR = -13
 ERROR_AMG_SMOOTH_TYPE = -31
 ERROR_AMG_AGG_TYPE = -32
 ERROR_AMG_COARSEING = -33
@@ -322,6 +323,8 @@ def _declare(lib):
     lib.hz_mlp_init.argtypes = [P(F), C.c_ulonglong]
     lib.hz_mlp_forward.argtypes = [P(F), P(F), INT, P(F)]
     lib.hz_mlp_forward.restype = INT
+    lib.hz_mlp_forward_noalloc.argtypes = [C.c_void_p, C.c_void_p, INT, C.c_void_p, C.c_void_p]
+    lib.hz_mlp_forward_noalloc.restype = INT
     lib.hz_mlp_loss_grad.argtypes = [P(F), P(F), P(F), INT, P(F), P(F)]
     lib.hz_mlp_loss_grad.restype = INT
     lib.hz_mlp_train.argtypes = [P(F), P(F), P(F), INT, INT, INT, F, C.c_ulonglong, P(F)]
@@ -380,6 +383,7 @@ def load_library(path: os.PathLike | None = None):
 
 
 MLP_INPUTS, MLP_OUTPUTS, MLP_PARAMS = 12, 2, 6288
+MLP_NOALLOC_MAX_ROWS = 16384   # HZ_MLP_NOALLOC_MAX_ROWS
 
 
 def fptr(arr: np.ndarray):

@@ -85,6 +85,24 @@ bool runtime_up() {
     return ready == 1;
 }
 
+}  // namespace
+
+namespace hz {
+bool runtime_visible() {
+    static int ready = -1;
+    if (ready < 0) {
+        RandomSnapshot snap;
+        const bool saved = save_random(snap);
+        int n = 0;
+        ready = (hipGetDeviceCount(&n) == hipSuccess && n >= 1) ? 1 : 0;
+        if (saved) restore_random(snap);
+    }
+    return ready == 1;
+}
+}  // namespace hz
+
+namespace {
+
 bool device_ready() {
     if (!runtime_up()) {
         std::fprintf(stderr, "hazmath_mi355x: no HIP device available\n");
@@ -937,6 +955,18 @@ INT hz_mlp_forward(const float *theta, const float *x, INT B, float *y) {
     } catch (const DeviceError &) {
         return HZ_ERROR_DEVICE;
     }
+}
+
+INT hz_mlp_forward_noalloc(const float *theta, const float *x, INT B, float *y, void *stream) {
+    if (B < 0 || B > HZ_MLP_NOALLOC_MAX_ROWS) {
+        std::fprintf(stderr, "hazmath_mi355x: hz_mlp_forward_noalloc: %lld rows is outside 0..%d\n", (long long)B,
+                     HZ_MLP_NOALLOC_MAX_ROWS);
+        return HZ_ERROR_INPUT_PAR;
+    }
+    if (B == 0) return HZ_SUCCESS;
+    if (!theta || !x || !y) return HZ_ERROR_INPUT_PAR;
+    if (!runtime_visible()) return HZ_ERROR_DEVICE;
+    return mlp_forward_noalloc(theta, x, B, y, static_cast<hipStream_t>(stream));
 }
 
 INT hz_mlp_loss_grad(const float *theta, const float *x, const float *t, INT B, float *grad, float *loss) {

@@ -225,6 +225,12 @@ void mlp_outputs(int B, const MlpWork &w, float *y);   // y [B][2]
 // gradient of the loss w.r.t. theta (after mlp_forward with the same inputs) and the loss
 void mlp_backward(const float *theta, const MlpWork &w, int B, float *grad, float *loss);
 void mlp_adam(float *theta, const float *grad, float *m, float *v, float lr, int step);
+// Forward pass with device x, y and theta through the static scratch array,
+// queued on st; no device memory is allocated.  B <= HZ_MLP_NOALLOC_MAX_ROWS.
+int mlp_forward_noalloc(const float *theta, const float *x, int B, float *y, hipStream_t st);
+// The HIP runtime is up with a device visible (no allocation; the caller's
+// libc random() state is kept).
+bool runtime_visible();
 
 // ---- kernels_mixed.hip: the V-cycle in FP32 / BF16 ------------------------
 constexpr int kLowF32 = 1, kLowBF16 = 2;

@@ -13,7 +13,18 @@
 #include <cstring>
 #include <vector>
 
+#include <mutex>
+
 #include "hz_internal.h"
+
+// Scratch memory of hz_mlp_forward_noalloc: the padded inputs and the two
+// hidden layers and padded outputs of up to kNoAllocRows rows (160 floats a
+// row, 10 MiB).  Each GPU holds its own copy for the life of the process; host
+// code reaches it only through the address hipGetSymbolAddress returns for the
+// current device, never through this name.
+constexpr int kNoAllocRows = HZ_MLP_NOALLOC_MAX_ROWS;
+constexpr size_t kNoAllocFloats = (size_t)kNoAllocRows * (16 + 64 + 64 + 16);
+static __device__ __align__(256) float scratch[kNoAllocFloats];
 
 namespace hz {
 
@@ -87,7 +98,7 @@ __global__ __launch_bounds__(256) void k_mlp_gemm(int M, int N, int K, const flo
             } else {
                 if (bias) v += bias[col];
                 if (kEpi == kEpiGelu) {
-                    Z[o] = v;
+                    if (Z) Z[o] = v;   // kept for the backward pass; null in a forward-only pass
                     C[o] = gelu(v);
                 } else {
                     C[o] = v;
@@ -295,6 +306,51 @@ void mlp_backward(const float *theta, const MlpWork &w, int B, float *grad, floa
     // layer 1
     wgrad(kIn, kHid, w.X, w.dH, grad + kOffW1);
     HZ_LAUNCH_CHECK();
+}
+
+// The device address of `scratch` on the current device, looked up once per
+// device (nullptr when the lookup fails).
+static float *noalloc_scratch() {
+    static std::mutex mu;
+    static float *addr[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) {
+            std::fprintf(stderr, "hazmath_mi355x: hipGetSymbolAddress(scratch) failed: %s\n", hipGetErrorString(e));
+            return nullptr;
+        }
+        addr[dev] = static_cast<float *>(p);
+    }
+    return addr[dev];
+}
+
+int mlp_forward_noalloc(const float *theta, const float *x, int B, float *y, hipStream_t st) {
+    float *base = noalloc_scratch();
+    if (!base) return HZ_ERROR_DEVICE;
+    float *X = base;
+    float *H1 = X + (size_t)kNoAllocRows * kIn;
+    float *H2 = H1 + (size_t)kNoAllocRows * kHid;
+    float *Y = H2 + (size_t)kNoAllocRows * kHid;
+    const int M = (B + kRows - 1) / kRows * kRows;
+    const int g = M / kRows;
+    k_mlp_pad_inputs<<<gsz((long)M * kIn), 256, 0, st>>>(B, M, x, nullptr, X);
+    k_mlp_gemm<false, kEpiGelu><<<g, 256, 0, st>>>(M, kHid, kIn, X, theta + kOffW1, theta + kOffB1, H1, nullptr,
+                                                   nullptr);
+    k_mlp_gemm<false, kEpiGelu><<<g, 256, 0, st>>>(M, kHid, kHid, H1, theta + kOffW2, theta + kOffB2, H2, nullptr,
+                                                   nullptr);
+    k_mlp_gemm<false, kEpiNone><<<g, 256, 0, st>>>(M, kOut, kHid, H2, theta + kOffW3, theta + kOffB3, Y, nullptr,
+                                                   nullptr);
+    k_mlp_take_outputs<<<gsz((long)B * 2), 256, 0, st>>>(B, Y, y);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        std::fprintf(stderr, "hazmath_mi355x: hz_mlp_forward_noalloc launch failed: %s\n", hipGetErrorString(e));
+        return HZ_ERROR_DEVICE;
+    }
+    return HZ_SUCCESS;
 }
 
 void mlp_adam(float *theta, const float *grad, float *m, float *v, float lr, int step) {

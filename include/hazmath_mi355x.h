@@ -462,6 +462,18 @@ INT hz_mlp_loss_grad(const float *theta, const float *x, const float *t, INT B, 
  * after the last one. */
 INT hz_mlp_train(float *theta, const float *x, const float *t, INT B, INT epochs, INT batch, float lr,
                  unsigned long long seed, float *loss_hist);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): hz_mlp_forward with DEVICE pointers theta, x [B][12] and
+ * y [B][2], queued on `stream` (a hipStream_t; NULL = the null stream), returning
+ * without waiting.  It allocates nothing: its intermediate rows live in a
+ * static device array of HZ_MLP_NOALLOC_MAX_ROWS rows (10 MiB, held on every
+ * GPU the library runs on until the process ends).  B above that maximum
+ * returns HZ_ERROR_INPUT_PAR.  All calls on one GPU share that array, so calls
+ * that overlap on one GPU (different streams, or threads) race; calls queued
+ * on one stream do not.  Same results as hz_mlp_forward, bit for bit. */
+#define HZ_MLP_NOALLOC_MAX_ROWS 16384
+INT hz_mlp_forward_noalloc(const float *theta, const float *x, INT B, float *y, void *stream);
 
 /* ---------------------------------------------------------------------------
  * Learned-parameter AMG (no HAZmath counterpart).
