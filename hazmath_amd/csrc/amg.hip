@@ -220,14 +220,14 @@ int amg_setup(Hierarchy &H, const DevCSR &A0, AMG_param *param, bool) {
                 mlp_outputs(nagg, w, out);
                 float *om = arena().alloc_n<float>((size_t)nagg + 1);
                 double scale = learned_parameters(nagg, out, om);
-                // floor: the Chebyshev interval must reach the level's lambda_max(D^-1 A).  The power
-                // estimate (60 iterations) is from below, hence 1.1x; the Gershgorin bound (scale 1)
-                // always covers lambda_max, so the floor never exceeds 1.
+                // floor: the Chebyshev interval must reach the level's lambda_max(D^-1 A).  The Lanczos
+                // estimate (30 steps) is from below, hence 1.1x; the Gershgorin bound (scale 1) always
+                // covers lambda_max, so the floor never exceeds 1.
                 const ChebLevel g1 = cheby_prepare(L.A, 1.0);
                 if ((int)H.cheb_scale.size() < max_levels) H.cheb_scale.assign(max_levels, 1.0);
                 if ((int)H.lambda_ratio.size() < max_levels) H.lambda_ratio.assign(max_levels, 0.0);
                 if (g1.lmax > 0.0) {
-                    const double ratio = power_lambda_dinv(L.A, g1.diag, 60) / g1.lmax;
+                    const double ratio = lanczos_lambda_dinv(L.A, g1.diag, 30, nullptr) / g1.lmax;
                     const double floor = std::min(1.0, 1.1 * ratio);
                     scale = std::min(1.5, std::max(scale, floor));
                     H.lambda_ratio[lvl] = ratio;
@@ -347,6 +347,15 @@ void amg_prepare_cycle(Hierarchy &H, const AMG_param *param) {
                 L.dsmf = arena().alloc_n<float>(n);
                 d2f(L.dsm, L.dsmf, L.A.n);
                 low_values(L.S, L.A, (cp.precision == 2 && l > 0) ? kLowBF16 : kLowF32, L.low);
+                if (L.sa) {
+                    L.Pf = values_f32(L.P);
+                    L.Rf = values_f32(L.R);
+                }
+                if (cp.smoother == HZ_SMOOTHER_CHEBY) {
+                    L.cdiagf = arena().alloc_n<float>(n);
+                    d2f(L.cheb.diag, L.cdiagf, L.A.n);
+                    L.cworkf = arena().alloc_n<float>(n);
+                }
             }
         }
     }
@@ -519,6 +528,14 @@ void presmooth_m(Hierarchy &H, int l, CycleState &cs) {
     const CycleParams &cp = H.cp;
     const float w = (float)cp.jacobi_w;
     int sweeps = cp.presmooth;
+    if (cp.smoother == HZ_SMOOTHER_CHEBY) {
+        if (cs.zero[l]) fill_f(L.xf, L.A.n);
+        cs.zero[l] = false;
+        cs.first_done[l] = false;
+        cheby_apply_m(L.S, L.A, L.low, L.cheb, L.cdiagf, sweeps, L.bf, L.xf, L.xtf, L.cworkf, nullptr, nullptr,
+                      nullptr, H.ws);
+        return;
+    }
     if (cs.zero[l]) {
         if (sweeps == 0) {
             fill_f(L.xf, L.A.n);
@@ -546,15 +563,23 @@ bool prolong_postsmooth_m(Hierarchy &H, int l, bool final, double *z, const doub
     Level &C = H.lv[l + 1];
     const CycleParams &cp = H.cp;
     const float w = (float)cp.jacobi_w;
-    if (cp.postsmooth <= 0) {
-        prolong_m(L.A.n, L.agg, C.xf, L.xf);
-        return false;
+    const bool cheb = cp.smoother == HZ_SMOOTHER_CHEBY;
+    if (cp.postsmooth <= 0 || L.sa || cheb) {
+        // separate prolongation: x += P xc (mgcycle.c:344-347)
+        if (L.sa) csr_prolong_m(L.P, L.Pf, C.xf, L.xf);
+        else prolong_m(L.A.n, L.agg, C.xf, L.xf);
+        if (cp.postsmooth <= 0) return false;
+        if (cheb) {
+            cheby_apply_m(L.S, L.A, L.low, L.cheb, L.cdiagf, cp.postsmooth, L.bf, L.xf, L.xtf, L.cworkf,
+                          final ? z : nullptr, r, final ? dot_out : nullptr, H.ws);
+            return final;
+        }
     }
     for (int s = 0; s < cp.postsmooth; ++s) {
         const bool last = final && s == cp.postsmooth - 1;
         const int *agg = nullptr;
         const float *xc = nullptr;
-        if (s == 0) {
+        if (s == 0 && !L.sa) {
             if (L.A.n <= fused_down_max_rows()) {
                 agg = L.agg;
                 xc = C.xf;
@@ -583,8 +608,12 @@ forward:
         Level &C = H.lv[l + 1];
         cs.num_lvl[l]++;
         presmooth_m(H, l, cs);
-        const bool fuse = (l + 1 < nl - 1) && cp.presmooth > 0;
-        if (L.A.n <= fused_down_max_rows()) {
+        // the next level's first sweep from zero rides on the restriction (UA, Jacobi / L1)
+        const bool fuse = (l + 1 < nl - 1) && cp.presmooth > 0 && cp.smoother != HZ_SMOOTHER_CHEBY && !L.sa;
+        if (L.sa) {
+            residual_m(L.S, L.low, L.bf, L.xf, L.wf);
+            csr_mxv_m(L.R, L.Rf, L.wf, C.bf);   // dcsr_mxv(R), mgcycle.c:302
+        } else if (L.A.n <= fused_down_max_rows()) {
             down_fused_m(L.nagg, L.rptr, L.rmem, L.A, L.low, L.bf, L.xf, C.bf, cp.smoother, w,
                          fuse ? C.dsmf : nullptr, fuse ? C.xf : nullptr);
         } else {
@@ -626,14 +655,15 @@ bool amg_precond_m(Hierarchy &H, const double *r, double *z, double *zr_out) {
         fill(z, 0.0, n);
         return false;
     }
-    enter_m(n, r, cp.smoother, (float)cp.jacobi_w, L0.dsmf, L0.bf, cp.presmooth > 0 ? L0.xf : nullptr);
+    const bool first_fused = cp.presmooth > 0 && cp.smoother != HZ_SMOOTHER_CHEBY;
+    enter_m(n, r, cp.smoother, (float)cp.jacobi_w, L0.dsmf, L0.bf, first_fused ? L0.xf : nullptr);
     CycleState cs;
     for (int i = 0; i < HZ_MAX_AMG_LVL; ++i) {
         cs.zero[i] = false;
         cs.first_done[i] = false;
     }
     cs.zero[0] = true;
-    cs.first_done[0] = cp.presmooth > 0;
+    cs.first_done[0] = first_fused;
     bool z_written = false;
     for (int it = 0; it < cp.maxit; ++it) z_written = mgcycle_m(H, cs, it == cp.maxit - 1, z, r, zr_out);
     if (!z_written) {

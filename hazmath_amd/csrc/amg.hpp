@@ -28,6 +28,8 @@ struct Level {
     LowMatrix low;            // values in FP32 or BF16 (non-coarsest levels)
     float *dsmf = nullptr;    // smoother diagonal rounded to FP32
     float *bf = nullptr, *xf = nullptr, *xtf = nullptr, *wf = nullptr;
+    float *Pf = nullptr, *Rf = nullptr;           // smoothed aggregation: P and R values in FP32
+    float *cdiagf = nullptr, *cworkf = nullptr;   // Chebyshev: diagonal and direction vector in FP32
 };
 
 struct CycleParams {

@@ -169,14 +169,6 @@ int check_supported(const linear_itsolver_param *it, const AMG_param *amg) {
         std::printf("### ERROR: AMG type %lld is not available in this build\n", (long long)amg->AMG_type);
         return HZ_ERROR_INPUT_PAR;
     }
-    if (amg->smoother == HZ_SMOOTHER_CHEBY && g_opt.vcycle_precision > 0) {
-        std::printf("### ERROR: the Chebyshev smoother needs the FP64 cycle (vcycle_precision = 0)\n");
-        return HZ_ERROR_INPUT_PAR;
-    }
-    if (amg->AMG_type == HZ_SA_AMG && g_opt.vcycle_precision > 0) {
-        std::printf("### ERROR: smoothed aggregation needs the FP64 cycle (vcycle_precision = 0)\n");
-        return HZ_ERROR_INPUT_PAR;
-    }
     if (amg->cycle_type != HZ_V_CYCLE && amg->cycle_type != HZ_W_CYCLE) {
         std::printf("### ERROR: cycle type %lld is not available in this build\n", (long long)amg->cycle_type);
         return HZ_ERROR_INPUT_PAR;
@@ -1164,6 +1156,23 @@ INT hz_aggregate_features(dCSRmat *A, const INT *aggregates, INT num_aggregates,
         float *df = arena().alloc_n<float>((size_t)num_aggregates * HZ_MLP_INPUTS + 1);
         aggregate_features(dA, dagg, num_aggregates, rp, rm, level, df);
         HZ_CHECK(memcpy_sync(feat, df, sizeof(float) * num_aggregates * HZ_MLP_INPUTS, hipMemcpyDeviceToHost));
+        return HZ_SUCCESS;
+    } catch (const DeviceError &) {
+        return HZ_ERROR_DEVICE;
+    }
+}
+
+INT hz_lambda_max_dinv(dCSRmat *A, INT steps, REAL *lambda, REAL *gershgorin) {
+    *lambda = 0.0;
+    *gershgorin = 0.0;
+    if (steps <= 0) return HZ_ERROR_INPUT_PAR;
+    if (!device_ready()) return HZ_ERROR_DEVICE;
+    try {
+        arena().reset();
+        const DevCSR dA = upload_csr(A);
+        const ChebLevel g1 = cheby_prepare(dA, 1.0);
+        *gershgorin = g1.lmax;
+        *lambda = lanczos_lambda_dinv(dA, g1.diag, steps, nullptr);
         return HZ_SUCCESS;
     } catch (const DeviceError &) {
         return HZ_ERROR_DEVICE;

@@ -201,9 +201,11 @@ void cheby_apply(const DevSELL &S, const ChebLevel &c, int steps, const double *
 // network outputs [nagg][2] -> omega[c] = clamp(out[c][0], 0, 1.6); returns
 // the level's Chebyshev bound scale clamp(mean_c out[c][1], 0.2, 1.5)
 double learned_parameters(int nagg, const float *out, float *omega);
-// lambda_max(D^-1 A) by `iters` normalized power iterations (d: the diagonal
-// the Chebyshev smoother divides by); an estimate from below
-double power_lambda_dinv(const DevCSR &A, const double *d, int iters);
+// lambda_max(D^-1 A) as the largest Ritz value of `steps` Lanczos steps on
+// D^-1/2 A D^-1/2 (d: the diagonal the Chebyshev smoother divides by); an
+// estimate from below.  *steps_done (optional): steps before the Krylov space
+// became invariant.
+double lanczos_lambda_dinv(const DevCSR &A, const double *d, int steps, int *steps_done);
 // feat [nagg][12] (FP32) of level `level`
 void aggregate_features(const DevCSR &A, const int *agg, int nagg, const int *rptr, const int *rmem, int level,
                         float *feat);
@@ -242,6 +244,17 @@ struct LowMatrix {
     void *csr = nullptr;
 };
 void low_values(const DevSELL &S, const DevCSR &A, int kind, LowMatrix &out);
+// FP32 copy of a CSR matrix's values (smoothed aggregation's P and R).
+float *values_f32(const DevCSR &A);
+// `steps` Chebyshev steps in FP32 on the level's low-precision values (x and
+// xt swap roles each step); with out64 the last step writes x + d to out64 in
+// FP64 instead, with (out64, r64) summed into *dot_out when dot_out is set.
+void cheby_apply_m(const DevSELL &S, const DevCSR &A, const LowMatrix &M, const ChebLevel &c, const float *diag,
+                   int steps, const float *b, float *&x, float *&xt, float *dwork, double *out64, const double *r64,
+                   double *dot_out, const ReduceWs &ws);
+// y = A x and x += P xc in FP32 with FP32 values (smoothed aggregation transfers)
+void csr_mxv_m(const DevCSR &A, const float *val, const float *x, float *y);
+void csr_prolong_m(const DevCSR &P, const float *val, const float *xc, float *x);
 void d2f(const double *a, float *b, long n);
 void f2d(const float *a, double *b, long n);
 void fill_f(float *x, long n);

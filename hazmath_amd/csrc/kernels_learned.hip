@@ -286,78 +286,165 @@ __global__ __launch_bounds__(kBlock) void k_learned_params(int nagg, const float
 
 namespace {
 
-// y = s D^-1 A v with s = 1/|v| from the previous step's sum of squares
-// (qprev; s = 1 when null or zero; d: first diagonal entries, rows with d = 0
-// give 0), and per-block partial sums of y_i^2.  16 lanes share a row (the
-// loads coalesce) and combine their products by a butterfly: an estimate,
-// so the summation order is free.
-constexpr int kPowLanes = 16;
-__global__ __launch_bounds__(kBlock) void k_dinv_av(int n, const int *__restrict__ ia, const int *__restrict__ ja,
-                                                    const double *__restrict__ a, const double *__restrict__ d,
-                                                    const double *__restrict__ v, const double *__restrict__ qprev,
-                                                    double *__restrict__ y, double *__restrict__ part) {
+// Lanczos on B = D^-1/2 A D^-1/2 (similar to D^-1 A, symmetric for SPD A;
+// d: first diagonal entries, rows with d <= 0 drop out).  The scalars of the
+// recurrence stay on the device (alpha[k], beta[k]); the host reads them once.
+// 16 lanes share a row of A (coalesced loads) and combine their products by a
+// butterfly: an estimate, so the summation order is free.
+constexpr int kLzLanes = 16;
+
+// w = B v (v already divided by beta of the previous step when inv_beta is set),
+// block partials of (w, v)
+__global__ __launch_bounds__(kBlock) void k_lz_apply(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                                                     const double *__restrict__ a, const double *__restrict__ isd,
+                                                     const double *__restrict__ v, double *__restrict__ w,
+                                                     double *__restrict__ part) {
     __shared__ double sh[kBlock / kWave];
-    const double s = (qprev && *qprev > 0.0) ? 1.0 / sqrt(*qprev) : 1.0;
     const long gt = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = (int)(gt / kPowLanes), j = (int)(gt % kPowLanes);
+    const int i = (int)(gt / kLzLanes), j = (int)(gt % kLzLanes);
     const bool valid = i < n;
     const int k0 = valid ? ia[i] : 0, k1 = valid ? ia[i + 1] : 0;
     double t = 0.0;
-    for (int k = k0 + j; k < k1; k += kPowLanes) t += a[k] * v[ja[k]];
+    for (int k = k0 + j; k < k1; k += kLzLanes) t += a[k] * (isd[ja[k]] * v[ja[k]]);
 #pragma unroll
-    for (int o = kPowLanes / 2; o > 0; o >>= 1) t += __shfl_xor(t, o, kWave);   // within the 16-lane group
+    for (int o = kLzLanes / 2; o > 0; o >>= 1) t += __shfl_xor(t, o, kWave);
     double q = 0.0;
     if (valid && j == 0) {
-        const double yi = d[i] != 0.0 ? s * t / d[i] : 0.0;
-        y[i] = yi;
-        q = yi * yi;
+        const double wi = isd[i] * t;
+        w[i] = wi;
+        q = wi * v[i];
     }
     const double bs = block_sum<kBlock>(q, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = bs;
 }
 
-// the block partials summed in one block (fixed order)
-__global__ __launch_bounds__(kBlock) void k_sum_partials(const double *__restrict__ part, int g,
+// w -= alpha_k v + beta_{k-1} vprev; block partials of |w|^2
+__global__ __launch_bounds__(kBlock) void k_lz_orth(int n, const double *__restrict__ alpha,
+                                                    const double *__restrict__ beta_prev,
+                                                    const double *__restrict__ v, const double *__restrict__ vprev,
+                                                    double *__restrict__ w, double *__restrict__ part) {
+    __shared__ double sh[kBlock / kWave];
+    const double al = *alpha, be = beta_prev ? *beta_prev : 0.0;
+    double q = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double wi = w[i] - al * v[i] - (beta_prev ? be * vprev[i] : 0.0);
+        w[i] = wi;
+        q += wi * wi;
+    }
+    const double bs = block_sum<kBlock>(q, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
+// vprev = v, v = w / beta (beta = 0: the Krylov space is invariant; v = 0)
+__global__ void k_lz_next(int n, const double *__restrict__ beta, const double *__restrict__ w,
+                          double *__restrict__ v, double *__restrict__ vprev) {
+    const double be = *beta;
+    const double s = be > 0.0 ? 1.0 / be : 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        vprev[i] = v[i];
+        v[i] = s * w[i];
+    }
+}
+
+// the block partials summed in one block (fixed order); sqrt when asked
+__global__ __launch_bounds__(kBlock) void k_sum_partials(const double *__restrict__ part, int g, int take_sqrt,
                                                          double *__restrict__ out) {
     __shared__ double sh[kBlock / kWave];
     double s = 0.0;
     for (int i = threadIdx.x; i < g; i += kBlock) s += part[i];
     const double t = block_sum<kBlock>(s, sh);
-    if (threadIdx.x == 0) *out = t;
+    if (threadIdx.x == 0) *out = take_sqrt ? sqrt(fmax(t, 0.0)) : t;
 }
 
-// a fixed start vector with every entry nonzero and no smooth structure
-__global__ void k_power_start(int n, double *__restrict__ v) {
+// 1/sqrt(d_i) (0 where d_i <= 0) and a fixed start vector with every entry
+// nonzero and no smooth structure
+__global__ void k_lz_start(int n, const double *__restrict__ d, double *__restrict__ isd, double *__restrict__ v) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        isd[i] = d[i] > 0.0 ? 1.0 / sqrt(d[i]) : 0.0;
         const unsigned h = (unsigned)i * 2654435761u;
         v[i] = 0.25 + (double)((h >> 8) & 0xffffu) / 65536.0;
     }
 }
 
+__global__ __launch_bounds__(kBlock) void k_sumsq(int n, const double *__restrict__ v, double *__restrict__ part) {
+    __shared__ double sh[kBlock / kWave];
+    double q = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) q += v[i] * v[i];
+    const double bs = block_sum<kBlock>(q, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = bs;
+}
+
+// Number of eigenvalues of the symmetric tridiagonal (diag a, off-diagonal b)
+// below x (Sturm sequence).
+int sturm_below(int m, const double *a, const double *b, double x) {
+    int c = 0;
+    double q = a[0] - x;
+    if (q < 0.0) ++c;
+    for (int i = 1; i < m; ++i) {
+        const double qq = (q != 0.0) ? q : 1e-300;
+        q = a[i] - x - b[i - 1] * b[i - 1] / qq;
+        if (q < 0.0) ++c;
+    }
+    return c;
+}
+
 }  // namespace
 
-double power_lambda_dinv(const DevCSR &A, const double *d, int iters) {
+double lanczos_lambda_dinv(const DevCSR &A, const double *d, int steps, int *steps_done) {
     const int n = A.n;
-    if (n == 0) return 0.0;
-    const int g = div_up((long)n * kPowLanes, kBlock);
+    if (steps_done) *steps_done = 0;
+    if (n == 0 || steps <= 0) return 0.0;
+    hipStream_t st = stream();
+    const int g = div_up((long)n * kLzLanes, kBlock);
+    const int gv = grid_for(n);
+    const int gp = std::max(g, gv);
+    double *isd = arena().alloc_n<double>((size_t)n + 1);
     double *v = arena().alloc_n<double>((size_t)n + 1);
-    double *y = arena().alloc_n<double>((size_t)n + 1);
-    double *part = arena().alloc_n<double>((size_t)g + 1);
-    double *q = arena().alloc_n<double>(2);   // sum of squares of the last two steps
-    k_power_start<<<grid_for(n), kBlock, 0, stream()>>>(n, v);
+    double *vprev = arena().alloc_n<double>((size_t)n + 1);
+    double *w = arena().alloc_n<double>((size_t)n + 1);
+    double *part = arena().alloc_n<double>((size_t)gp + 1);
+    double *ab = arena().alloc_n<double>(2 * (size_t)steps + 2);   // alpha[0..steps), beta[0..steps), norm
+    double *alpha = ab, *beta = ab + steps, *nrm = ab + 2 * steps;
+    k_lz_start<<<gv, kBlock, 0, st>>>(n, d, isd, v);
+    k_sumsq<<<gv, kBlock, 0, st>>>(n, v, part);
+    k_sum_partials<<<1, kBlock, 0, st>>>(part, gv, 1, nrm);
+    k_lz_next<<<gv, kBlock, 0, st>>>(n, nrm, v, w, vprev);   // w = v / |v| (vprev: scratch)
+    std::swap(v, w);
     HZ_LAUNCH_CHECK();
-    // step k > 0 applies D^-1 A to the unit vector v / |v|, so |y| estimates lambda_max;
-    // the norms stay on the device and the host reads the last one
-    for (int k = 0; k <= iters; ++k) {
-        k_dinv_av<<<g, kBlock, 0, stream()>>>(n, A.ia, A.ja, A.val, d, v, k > 0 ? q + ((k - 1) & 1) : nullptr, y,
-                                              part);
-        k_sum_partials<<<1, kBlock, 0, stream()>>>(part, g, q + (k & 1));
+    for (int k = 0; k < steps; ++k) {
+        k_lz_apply<<<g, kBlock, 0, st>>>(n, A.ia, A.ja, A.val, isd, v, w, part);
+        k_sum_partials<<<1, kBlock, 0, st>>>(part, g, 0, alpha + k);
+        k_lz_orth<<<gv, kBlock, 0, st>>>(n, alpha + k, k > 0 ? beta + k - 1 : nullptr, v, vprev, w, part);
+        k_sum_partials<<<1, kBlock, 0, st>>>(part, gv, 1, beta + k);
+        if (k + 1 < steps) k_lz_next<<<gv, kBlock, 0, st>>>(n, beta + k, w, v, vprev);
         HZ_LAUNCH_CHECK();
-        std::swap(v, y);
     }
-    double last = 0.0;
-    HZ_CHECK(memcpy_sync(&last, q + (iters & 1), sizeof(double), hipMemcpyDeviceToHost));
-    return iters > 0 && last > 0.0 ? std::sqrt(last) : 0.0;
+    std::vector<double> h(2 * (size_t)steps);
+    HZ_CHECK(memcpy_sync(h.data(), ab, sizeof(double) * 2 * steps, hipMemcpyDeviceToHost));
+    // stop at the first (near-)zero beta: the Krylov space is invariant there
+    const double *ha = h.data(), *hb = h.data() + steps;
+    int m = steps;
+    double scale = 0.0;
+    for (int k = 0; k < steps; ++k) scale = std::max(scale, std::fabs(ha[k]));
+    for (int k = 0; k + 1 < steps; ++k)
+        if (hb[k] <= 1e-14 * std::max(scale, 1e-300)) {
+            m = k + 1;
+            break;
+        }
+    if (steps_done) *steps_done = m;
+    // largest eigenvalue of the m x m tridiagonal by bisection on Gershgorin bounds
+    double lo = ha[0], hi = ha[0];
+    for (int i = 0; i < m; ++i) {
+        const double r = (i > 0 ? std::fabs(hb[i - 1]) : 0.0) + (i + 1 < m ? std::fabs(hb[i]) : 0.0);
+        lo = std::min(lo, ha[i] - r);
+        hi = std::max(hi, ha[i] + r);
+    }
+    for (int it = 0; it < 200 && hi - lo > 1e-15 * std::max(std::fabs(hi), 1e-300); ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (sturm_below(m, ha, hb, mid) >= m) hi = mid;   // every eigenvalue below mid
+        else lo = mid;
+    }
+    return hi;
 }
 
 double learned_parameters(int nagg, const float *out, float *omega) {

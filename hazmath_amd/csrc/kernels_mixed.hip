@@ -295,6 +295,118 @@ __global__ void k_prolong_m(int n, const int *__restrict__ agg, const float *__r
     }
 }
 
+// One Chebyshev step (the FP64 k_cheb_step's recurrence) in FP32:
+// dnew = c1 d + c2 (b - A x) / diag, x + dnew to xout, or on the last step of
+// the cycle to out64 (FP64) with block partials of sum_i out64_i r64_i.
+template <bool kOut64, class MT>
+__global__ __launch_bounds__(kBlock) void k_cheb_step_m(int n, int nslices, const long long *__restrict__ sptr,
+                                                        const int *__restrict__ col, const MT *__restrict__ val,
+                                                        const float *__restrict__ diag, float c1, float c2,
+                                                        const float *__restrict__ b, const float *__restrict__ x,
+                                                        float *__restrict__ xout, float *__restrict__ d,
+                                                        double *__restrict__ out64, const double *__restrict__ r64,
+                                                        double *__restrict__ partials) {
+    __shared__ double sh[kBlock / kWave];
+    const int blk = xcd_block_id();
+    const int s = blk * kSlicesPerBlock + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    const int r = s * kSlice + lane;
+    double prod = 0.0;
+    if (s < nslices) {
+        const long long base = sptr[s];
+        const int wd = (int)((sptr[s + 1] - base) / kSlice);
+        const int *c = col + base + lane;
+        const MT *v = val + base + lane;
+        float t = (r < n) ? b[r] : 0.0f;
+        int k = 0;
+        for (; k + kU <= wd; k += kU) {
+            int cc[kU];
+            float vv[kU], xx[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                cc[u] = ld_nt(c + (k + u) * kSlice);
+                vv[u] = ldv(v + (k + u) * kSlice);
+            }
+#pragma unroll
+            for (int u = 0; u < kU; ++u) xx[u] = cc[u] >= 0 ? x[cc[u]] : 0.0f;
+#pragma unroll
+            for (int u = 0; u < kU; ++u)
+                if (cc[u] >= 0) t = t - vv[u] * xx[u];
+        }
+        for (; k < wd; ++k) {
+            const int cj = ld_nt(c + k * kSlice);
+            if (cj >= 0) t = t - ldv(v + k * kSlice) * x[cj];
+        }
+        if (r < n) {
+            const float di = diag[r];
+            const float dn = (c1 == 0.0f ? 0.0f : c1 * d[r]) + (di != 0.0f ? c2 * t / di : 0.0f);
+            d[r] = dn;
+            const float xn = x[r] + dn;
+            if (kOut64) {
+                const double z = (double)xn;
+                out64[r] = z;
+                prod = z * r64[r];
+            } else {
+                xout[r] = xn;
+            }
+        }
+    }
+    if (kOut64) {
+        const double bs = block_sum<kBlock>(prod, sh);
+        if (threadIdx.x == 0) partials[blk] = bs;
+    }
+}
+
+// The same step over the CSR copy with kLanes lanes per row (small levels with
+// long rows, latency-bound); the products are combined by a butterfly.
+template <int kLanes, class MT>
+__global__ void k_cheb_step_grp_m(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                                  const MT *__restrict__ a, const float *__restrict__ diag, float c1, float c2,
+                                  const float *__restrict__ b, const float *__restrict__ x, float *__restrict__ xout,
+                                  float *__restrict__ d) {
+    const long gt = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = (int)(gt / kLanes), j = (int)(gt % kLanes);
+    const bool valid = r < n;
+    const int k0 = valid ? ia[r] : 0, k1 = valid ? ia[r + 1] : 0;
+    float acc = 0.0f;
+    for (int k = k0 + j; k < k1; k += kLanes) acc = acc + ldv(a + k) * x[ja[k]];
+#pragma unroll
+    for (int o = kLanes / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kWave);
+    if (valid && j == 0) {
+        const float t = b[r] - acc;
+        const float di = diag[r];
+        const float dn = (c1 == 0.0f ? 0.0f : c1 * d[r]) + (di != 0.0f ? c2 * t / di : 0.0f);
+        d[r] = dn;
+        xout[r] = x[r] + dn;
+    }
+}
+
+// y = A x in FP32 over a CSR matrix with FP32 values (smoothed aggregation's
+// restriction with R), kLanes lanes per row combined by a butterfly.
+template <int kLanes>
+__global__ void k_csr_mxv_m(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                            const float *__restrict__ a, const float *__restrict__ x, float *__restrict__ y) {
+    const long gt = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = (int)(gt / kLanes), j = (int)(gt % kLanes);
+    const bool valid = r < n;
+    const int k0 = valid ? ia[r] : 0, k1 = valid ? ia[r + 1] : 0;
+    float acc = 0.0f;
+    for (int k = k0 + j; k < k1; k += kLanes) acc = acc + ld_nt(a + k) * x[ja[k]];
+#pragma unroll
+    for (int o = kLanes / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kWave);
+    if (valid && j == 0) y[r] = acc;
+}
+
+// x += P xc in FP32 (smoothed aggregation's prolongation), one thread per row.
+__global__ void k_csr_prolong_m(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                                const float *__restrict__ a, const float *__restrict__ xc, float *__restrict__ x) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        float s = 0.0f;
+        for (int k = ia[i]; k < ia[i + 1]; ++k) s = s + ld_nt(a + k) * xc[ja[k]];
+        x[i] = x[i] + s;
+    }
+}
+
 // x = Ainv b with FP64 Ainv and accumulation, FP32 b and x; a wavefront per row
 __global__ __launch_bounds__(256) void k_gemv_m(int n, const double *__restrict__ Ainv,
                                                 const float *__restrict__ b, float *__restrict__ x) {
@@ -447,6 +559,86 @@ void prolong_m(int n, const int *agg, const float *xc, float *x) {
     if (n <= 0) return;
     k_prolong_m<<<grid_for(n), kBlock, 0, stream()>>>(n, agg, xc, x);
     HZ_LAUNCH_CHECK();
+}
+
+void cheby_apply_m(const DevSELL &S, const DevCSR &A, const LowMatrix &M, const ChebLevel &c, const float *diag,
+                   int steps, const float *b, float *&x, float *&xt, float *dwork, double *out64, const double *r64,
+                   double *dot_out, const ReduceWs &ws) {
+    if (S.n == 0 || steps <= 0) return;
+    const double theta = 0.5 * (c.lmax + c.lmin), delta = 0.5 * (c.lmax - c.lmin);
+    const double sigma = theta / delta;
+    double rho = 1.0 / sigma;
+    const int grid = div_up(S.nslices, kSlicesPerBlock);
+    constexpr int G = 16;
+    const bool grp = A.ia && A.n == S.n && S.n <= 65536 && (long long)A.nnz >= 16LL * A.n;
+    for (int k = 0; k < steps; ++k) {
+        double c1 = 0.0, c2 = 1.0 / theta;
+        if (k > 0) {
+            const double rn = 1.0 / (2.0 * sigma - rho);
+            c1 = rn * rho;
+            c2 = 2.0 * rn / delta;
+            rho = rn;
+        }
+        const bool last64 = out64 && k == steps - 1;
+        if (last64) {
+            if (M.kind == kLowF32)
+                k_cheb_step_m<true, float><<<grid, kBlock, 0, stream()>>>(
+                    S.n, S.nslices, S.sptr, S.col, static_cast<const float *>(M.sell), diag, (float)c1, (float)c2, b,
+                    x, nullptr, dwork, out64, r64, ws.partials);
+            else
+                k_cheb_step_m<true, unsigned short><<<grid, kBlock, 0, stream()>>>(
+                    S.n, S.nslices, S.sptr, S.col, static_cast<const unsigned short *>(M.sell), diag, (float)c1,
+                    (float)c2, b, x, nullptr, dwork, out64, r64, ws.partials);
+            HZ_LAUNCH_CHECK();
+            if (dot_out) {
+                k_final_sum_m<<<1, kFinBlock, 0, stream()>>>(ws.partials, grid, dot_out);
+                HZ_LAUNCH_CHECK();
+            }
+            return;
+        }
+        if (grp) {
+            if (M.kind == kLowF32)
+                k_cheb_step_grp_m<G, float><<<div_up((long)S.n * G, kBlock), kBlock, 0, stream()>>>(
+                    S.n, A.ia, A.ja, static_cast<const float *>(M.csr), diag, (float)c1, (float)c2, b, x, xt, dwork);
+            else
+                k_cheb_step_grp_m<G, unsigned short><<<div_up((long)S.n * G, kBlock), kBlock, 0, stream()>>>(
+                    S.n, A.ia, A.ja, static_cast<const unsigned short *>(M.csr), diag, (float)c1, (float)c2, b, x, xt,
+                    dwork);
+        } else if (M.kind == kLowF32) {
+            k_cheb_step_m<false, float><<<grid, kBlock, 0, stream()>>>(
+                S.n, S.nslices, S.sptr, S.col, static_cast<const float *>(M.sell), diag, (float)c1, (float)c2, b, x,
+                xt, dwork, nullptr, nullptr, nullptr);
+        } else {
+            k_cheb_step_m<false, unsigned short><<<grid, kBlock, 0, stream()>>>(
+                S.n, S.nslices, S.sptr, S.col, static_cast<const unsigned short *>(M.sell), diag, (float)c1,
+                (float)c2, b, x, xt, dwork, nullptr, nullptr, nullptr);
+        }
+        HZ_LAUNCH_CHECK();
+        std::swap(x, xt);
+    }
+}
+
+void csr_mxv_m(const DevCSR &A, const float *val, const float *x, float *y) {
+    if (A.n <= 0) return;
+    constexpr int G = 16;
+    k_csr_mxv_m<G><<<div_up((long)A.n * G, kBlock), kBlock, 0, stream()>>>(A.n, A.ia, A.ja, val, x, y);
+    HZ_LAUNCH_CHECK();
+}
+
+void csr_prolong_m(const DevCSR &P, const float *val, const float *xc, float *x) {
+    if (P.n <= 0) return;
+    k_csr_prolong_m<<<grid_for(P.n), kBlock, 0, stream()>>>(P.n, P.ia, P.ja, val, xc, x);
+    HZ_LAUNCH_CHECK();
+}
+
+float *values_f32(const DevCSR &A) {
+    float *v = arena().alloc_n<float>((size_t)A.nnz + 1);
+    if (A.nnz) {
+        const int g = (int)std::max(1LL, std::min(8192LL, ((long long)A.nnz + kBlock - 1) / kBlock));
+        k_vals_f32<<<g, kBlock, 0, stream()>>>(A.nnz, A.val, v);
+        HZ_LAUNCH_CHECK();
+    }
+    return v;
 }
 
 void coarse_dense_solve_m(int n, const double *Ainv, const float *b, float *x) {

@@ -506,6 +506,11 @@ INT hz_two_grid_rate(dCSRmat *A, AMG_param *amgparam, REAL omega, REAL lscale, I
  * threshold theta, damping omega or per-aggregate omega_agg (may be NULL). */
 INT hz_aggregate_features(dCSRmat *A, const INT *aggregates, INT num_aggregates, INT level, float *feat);
 INT hz_cheby_smooth(dCSRmat *A, const REAL *b, REAL *x, INT steps, REAL scale);
+/* The estimate behind the learned method's Chebyshev floor: *lambda = the
+ * largest Ritz value of `steps` Lanczos steps on D^-1/2 A D^-1/2 (D the first
+ * diagonal entries; a lower bound of lambda_max(D^-1 A) for SPD A) and
+ * *gershgorin = max_i sum_j |a_ij| / |a_ii| (an upper bound). */
+INT hz_lambda_max_dinv(dCSRmat *A, INT steps, REAL *lambda, REAL *gershgorin);
 INT hz_sa_prolongation(dCSRmat *A, const INT *aggregates, INT num_aggregates, REAL theta, REAL omega,
                        const float *omega_agg, INT filter, dCSRmat *P);
 /* No counterpart in the reference: many independent systems, up to

@@ -17,13 +17,16 @@ LIB_PATH = HERE / "liboracle.so"
 _LIB = None
 
 
-def load():
+def load(path=None):
+    """The oracle library (oracle/liboracle.so), or another build of the same
+    source at `path` (the sanitizer build of tests/sanitize), declared alike."""
     global _LIB
-    if _LIB is not None:
+    if _LIB is not None and path is None:
         return _LIB
-    if not LIB_PATH.exists():
-        raise RuntimeError(f"{LIB_PATH} missing: run `make -C oracle`")
-    lib = C.CDLL(str(LIB_PATH), mode=C.RTLD_LOCAL)
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise RuntimeError(f"{p} missing: run `make -C oracle`")
+    lib = C.CDLL(str(p), mode=C.RTLD_LOCAL)
     P = C.POINTER
     lib.orc_krylov_amg.argtypes = [P(dCSRmat), P(dvector), P(dvector), P(linear_itsolver_param),
                                    P(AMG_param), P(orc_trace)]
@@ -56,8 +59,15 @@ def load():
     lib.orc_gs_smooth_ordered.argtypes = [P(dCSRmat), P(REAL), P(REAL), INT, INT, INT, REAL, P(INT), P(INT),
                                           INT]
     lib.orc_gs_smooth_ordered.restype = C.c_int
-    _LIB = lib
+    lib.orc_set_learned.argtypes = [ORC_OMEGA_FN, P(REAL)]
+    lib.orc_set_learned.restype = None
+    if path is None:
+        _LIB = lib
     return lib
+
+
+# fn(level, A_l, agg_l, nagg_l, omega[nagg_l]) of orc_set_learned
+ORC_OMEGA_FN = C.CFUNCTYPE(None, INT, C.POINTER(dCSRmat), C.POINTER(INT), INT, C.POINTER(C.c_float))
 
 
 def params(*, tol=1e-8, maxit=500, aggregation=4, smoother=1, coarse_dof=100, coarse_solver=32,

@@ -890,6 +890,18 @@ void orc_cheby_smooth(const dCSRmat *A, const REAL *b, REAL *x, INT steps, REAL 
     free(dg); free(d); free(xn);
 }
 
+/* Learned-parameter composition (no reference counterpart): per-aggregate
+ * prolongator damping supplied level by level by a callback, and per-level
+ * Chebyshev bound scales (instead of the uniform tentative_smooth and scale 1). */
+static orc_omega_fn g_orc_omega_fn = NULL;
+static REAL g_orc_cheb_scale[HZ_MAX_AMG_LVL];
+static int g_orc_cheb_scale_set = 0;
+void orc_set_learned(orc_omega_fn fn, const REAL *scales) {
+    g_orc_omega_fn = fn;
+    g_orc_cheb_scale_set = scales != NULL;
+    for (INT l = 0; l < HZ_MAX_AMG_LVL; ++l) g_orc_cheb_scale[l] = scales ? scales[l] : 1.0;
+}
+
 /* amg_setup_unsmoothP_unsmoothR, amg_setup_ua.c:1228-1443, and for
  * AMG_type = SA amg_setup_smoothP_smoothR, amg_setup_ua.c:1461-1687. */
 /* print_cputime, message.c:73-77 (wall seconds) */
@@ -930,7 +942,13 @@ static INT orc_setup(orc_amg *m, const dCSRmat *A, AMG_param *param) {
         if (param->AMG_type == HZ_SA_AMG) {
             dCSRmat N;
             orc_strength(&L->A, L->theta, &N);
-            orc_smooth_p(&L->A, &N, L->agg, L->nagg, param->tentative_smooth, param->smooth_filter, &L->P, NULL);
+            float *om = NULL;
+            if (g_orc_omega_fn) {
+                om = (float *)xcalloc((size_t)L->nagg + 1, sizeof(float));
+                g_orc_omega_fn(lvl, &L->A, L->agg, L->nagg, om);
+            }
+            orc_smooth_p(&L->A, &N, L->agg, L->nagg, param->tentative_smooth, param->smooth_filter, &L->P, om);
+            free(om);
             csr_free(&N);
             csr_transpose(&L->P, &L->R);
             orc_rap(&L->R, &L->A, &L->P, &m->lv[lvl + 1].A);
@@ -1216,8 +1234,9 @@ typedef struct {
 } orc_prec_data;
 
 static void orc_smooth(const orc_prec_data *pd, orc_level *L, INT sweeps, int post) {
-    if (pd->cyc.smoother == 91) {   /* Chebyshev (build-owned), bound scale 1 */
-        orc_cheby_smooth(&L->A, L->b, L->x, sweeps, 1.0);
+    if (pd->cyc.smoother == 91) {   /* Chebyshev (build-owned): bound scale 1 or the level's set scale */
+        const INT l = (INT)(L - pd->m->lv);
+        orc_cheby_smooth(&L->A, L->b, L->x, sweeps, g_orc_cheb_scale_set ? g_orc_cheb_scale[l] : 1.0);
         return;
     }
     if (orc_is_gs_family(pd->cyc.smoother)) {

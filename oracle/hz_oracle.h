@@ -76,6 +76,13 @@ void *orc_precond_create(dCSRmat *A, AMG_param *amgparam);
 void orc_precond_apply(void *h, const REAL *r, REAL *z);
 void orc_precond_free(void *h);
 
+/* Learned-parameter composition (no reference counterpart): fn(level, A_l,
+ * agg_l, nagg_l, omega[nagg_l]) fills the per-aggregate damping of every SA
+ * level; scales[HZ_MAX_AMG_LVL] (may be NULL) the Chebyshev bound scale per
+ * level.  orc_set_learned(NULL, NULL) restores the uniform parameters. */
+typedef void (*orc_omega_fn)(INT level, const dCSRmat *A, const INT *agg, INT nagg, float *omega);
+void orc_set_learned(orc_omega_fn fn, const REAL *scales);
+
 #ifdef __cplusplus
 }
 #endif

@@ -106,6 +106,40 @@ def test_coo_reader_matches_fixture(tmp_path):
     assert np.array_equal(gia, ia) and np.array_equal(gja, ja) and np.array_equal(gva, va)
 
 
+def test_coo_reader_unsorted_input_is_row_stable(tmp_path):
+    """dcoo_2_dcsr_p (format.c:785-827) is a counting sort by row: entries of a
+    row keep the order they have in the file (duplicates stay separate).  The
+    expected CSR is numpy's stable argsort of the row indices."""
+    lib = capi.load_library()
+    rng = np.random.default_rng(3)
+    n, nnz = 57, 400
+    rows = rng.integers(0, n, nnz)
+    cols = rng.integers(0, n, nnz)
+    rows[:5] = 7
+    cols[:5] = [3, 3, 9, 1, 3]          # a repeated entry and a descending run in one row
+    vals = rng.standard_normal(nnz)
+    p = tmp_path / "A.dat"
+    with open(p, "w") as f:
+        f.write(f"{n} {n} {nnz}\n")
+        for r, c, v in zip(rows, cols, vals):
+            f.write(f"{r} {c} {v:.17e}\n")
+    libc = C.CDLL(None)
+    libc.fopen.restype = C.c_void_p
+    libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+    libc.fclose.argtypes = [C.c_void_p]
+    lib.dcoo_read_dcsr_p.restype = C.POINTER(capi.dCSRmat)
+    lib.dcoo_read_dcsr_p.argtypes = [C.c_void_p]
+    fp = libc.fopen(str(p).encode(), b"r")
+    A = lib.dcoo_read_dcsr_p(fp)
+    libc.fclose(fp)
+    gia, gja, gva = capi.csr_to_numpy(A.contents)
+    order = np.argsort(rows, kind="stable")
+    eia = np.zeros(n + 1, np.int64)
+    eia[1:] = np.cumsum(np.bincount(rows, minlength=n))
+    assert np.array_equal(gia, eia)
+    assert np.array_equal(gja, cols[order]) and np.array_equal(gva, vals[order])
+
+
 def test_device_entry_fails_loudly_without_gpu():
     lib = capi.load_library()
     if lib.hz_device_count() > 0:

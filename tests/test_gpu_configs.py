@@ -14,6 +14,10 @@ the CPU oracle (the serial restatement of the reference path).
    every one of the 2,048 pairs meets 1e-10 in the FP64 true residual, and a
    subset that holds the largest and the most anisotropic systems matches
    the oracle.
+5. The learned method on 3-D Kuhn P1 anisotropic (eps = 1e-3) at 128^3,
+   with the FP64 and the FP32 cycle: no fallback, 1e-10 in the FP64 true
+   residual (its oracle composition runs at smaller sizes in
+   test_gpu_learned.py).
 """
 import ctypes as C
 import time
@@ -195,3 +199,32 @@ def test_config4_full_batch_every_pair_and_oracle_subset(lib, orc, libc):
         _say(f"config 4: system {k}: batched {iters[k, q]} oracle {io} ({t:.1f} s), distance {err:.2e}")
         assert io > 0 and abs(int(iters[k, q]) - io) <= max(2, int(0.05 * io))
         assert err < 1e-8
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_config5_learned_kuhn128_anisotropic(lib, libc, precision):
+    """5. The learned method's evaluation problem: 3-D Kuhn P1 anisotropic,
+    eps = 1e-3, seeded rotation, 128^3 cells: the learned SA + Chebyshev(2)
+    method with the committed network (FP64 cycle, and the FP32 cycle under
+    the FP64 PCG) converges without falling back and meets 1e-10 in the FP64
+    true residual; the plain drop-in runs beside it for the record."""
+    from hazmath_amd import learned
+    theta_dir = float(np.random.default_rng(5).uniform(0.0, np.pi))
+    ia, ja, va = problems.aniso3d_kuhn(128, theta_dir, 1e-3)
+    n = ia.size - 1
+    b = problems.uniform_rhs(n, seed=1)
+    weights = learned.load_model()
+    old = _set_options(lib, vcycle_precision=precision)
+    try:
+        _say(f"config 5: learned solve, precision {precision}")
+        libc.srand(1)
+        r, x, info = learned.solve_learned(lib, ia, ja, va, b, weights, budget=400)
+    finally:
+        _set_options(lib, **old)
+    M = sp.csr_matrix((va, ja, ia), shape=(n, n))
+    res = np.linalg.norm(b - M @ x) / np.linalg.norm(b)
+    scales = [round(info.cheb_scale[l], 4) for l in range(info.num_levels - 1)]
+    _say(f"config 5: precision {precision}: {r} iterations, fallback {info.fallback_used}, "
+         f"{info.total_ms:.1f} ms, FP64 true residual {res:.3e}, scales {scales}")
+    assert info.fallback_used == 0 and 0 < r <= 400
+    assert res < 1e-10
