@@ -163,6 +163,8 @@ def parse():
     p.add_argument("--cpu-sample-m", type=int, default=128,
                    help="cells per direction of the CPU-baseline sample")
     p.add_argument("--cpu-runs", type=int, default=5, help="CPU-baseline solves (median reported)")
+    p.add_argument("--no-learned", action="store_true",
+                   help="skip the learned-vs-plain comparison on the anisotropic problem in the default line")
     p.add_argument("--no-traffic", action="store_true",
                    help="skip the two rocprofv3 counter passes that measure the SpMV's HBM traffic")
     p.add_argument("--traffic-probe", action="store_true",
@@ -399,14 +401,19 @@ def main_batched(a, lib):
     print(json.dumps(line))
 
 
-def main_learned(a, lib):
+LEARNED_EPS = 1e-3
+
+
+def learned_comparison(a, lib, steps, warmup, kinds=("plain", "sa", "learned", "learned_fp32")):
     """Learned per-aggregate SA parameters against the plain drop-in on the
     3-D Kuhn P1 anisotropic problem (K = R_z(theta) diag(1, eps, eps) R_z^T,
-    eps = 1e-3, theta = U[0, pi) seed 5), m^3 cells, seeded RHS, tol 1e-10."""
+    eps = 1e-3, theta = U[0, pi) seed 5), m^3 cells, seeded RHS, tol 1e-10.
+    Times are device times of the whole call (setup + PCG; for the learned
+    method the attempt plus, if it fell back, the plain solve), inputs
+    uploaded before the timed region."""
     from hazmath_amd import learned
     theta_dir = float(np.random.default_rng(5).uniform(0.0, np.pi))
-    eps = 1e-3
-    ia, ja, va = problems.aniso3d_kuhn(a.m, theta_dir, eps)
+    ia, ja, va = problems.aniso3d_kuhn(a.m, theta_dir, LEARNED_EPS)
     n = ia.size - 1
     b = problems.uniform_rhs(n, seed=1)
     weights = learned.load_model()
@@ -425,71 +432,88 @@ def main_learned(a, lib):
         finally:
             os.dup2(saved, 1)
 
-    def plain_params():
+    def run(kind):
+        x = capi.HostVec(np.zeros(n))
         it, amg = capi.solver_params(lib, tol=a.tol, maxit=1000, aggregation=capi.HEM,
                                      smoother=capi.SMOOTHER_L1DIAG, coarse_dof=200,
                                      coarse_solver=capi.SOLVER_UMFPACK, max_levels=20, theta=0.04)
-        return it, amg
-
-    def run(kind):
-        x = capi.HostVec(np.zeros(n))
-        it, amg = plain_params()
+        opts = capi.hz_options()
+        lib.hz_get_options(C.byref(opts))
+        old = opts.vcycle_precision
+        opts.vcycle_precision = 1 if kind == "learned_fp32" else 0
+        lib.hz_set_options(C.byref(opts))
         libc.srand(1)
-        if kind == "plain":
-            r = lib.linear_solver_dcsr_krylov_amg(A.ptr(), capi.HostVec(b).ptr(), x.ptr(), C.byref(it), C.byref(amg))
-        elif kind == "sa":
-            amg.AMG_type = capi.SA_AMG
-            amg.aggregation_type = capi.VMB
-            amg.smoother = capi.SMOOTHER_CHEBY
-            amg.presmooth_iter = amg.postsmooth_iter = 2
-            r = lib.linear_solver_dcsr_krylov_amg(A.ptr(), capi.HostVec(b).ptr(), x.ptr(), C.byref(it), C.byref(amg))
-        else:
-            r = lib.hz_krylov_amg_learned(A.ptr(), capi.HostVec(b).ptr(), x.ptr(), C.byref(it), C.byref(amg),
-                                          capi.fptr(weights), 400, 2)
+        try:
+            if kind == "plain":
+                r = lib.linear_solver_dcsr_krylov_amg(A.ptr(), capi.HostVec(b).ptr(), x.ptr(), C.byref(it),
+                                                      C.byref(amg))
+            elif kind == "sa":
+                amg.AMG_type = capi.SA_AMG
+                amg.aggregation_type = capi.VMB
+                amg.smoother = capi.SMOOTHER_CHEBY
+                amg.presmooth_iter = amg.postsmooth_iter = 2
+                r = lib.linear_solver_dcsr_krylov_amg(A.ptr(), capi.HostVec(b).ptr(), x.ptr(), C.byref(it),
+                                                      C.byref(amg))
+            else:
+                r = lib.hz_krylov_amg_learned(A.ptr(), capi.HostVec(b).ptr(), x.ptr(), C.byref(it), C.byref(amg),
+                                              capi.fptr(weights), 400, 2)
+        finally:
+            opts.vcycle_precision = old
+            lib.hz_set_options(C.byref(opts))
         info = capi.hz_solve_info()
         lib.hz_get_last_info(C.byref(info))
         rel = float(np.linalg.norm(b - Asp @ x.arr) / np.linalg.norm(b))
         out = dict(iterations=int(r), setup_ms=info.setup_ms, solve_ms=info.solve_ms, total_ms=info.total_ms,
-                   fallback=int(info.fallback_used) if kind == "learned" else None, true_rel_residual=rel)
-        if kind == "learned":
-            # the whole call: the learned attempt plus, when it fell back, the plain solve
-            out["learned_attempt"] = dict(iterations=int(info.learned_iterations),
-                                          total_ms=info.learned_total_ms)
+                   true_rel_residual=rel)
+        if kind.startswith("learned"):
+            out["fallback"] = int(info.fallback_used)
+            out["learned_attempt"] = dict(iterations=int(info.learned_iterations), total_ms=info.learned_total_ms)
+            out["cheb_scale"] = [round(info.cheb_scale[l], 4) for l in range(max(0, info.num_levels - 1))]
             if info.fallback_used:
                 out["total_ms"] = info.learned_total_ms + info.total_ms
         return out
 
     res = {}
-    for kind in ("plain", "sa", "learned"):
-        for w in range(a.warmup):
+    for kind in kinds:
+        for _ in range(warmup):
             quiet(lambda: run(kind))
-        runs = [quiet(lambda: run(kind)) for _ in range(a.steps)]
+        runs = [quiet(lambda: run(kind)) for _ in range(steps)]
         best = dict(runs[-1])
-        best["total_ms"] = float(np.mean([r_["total_ms"] for r_ in runs]))
-        best["setup_ms"] = float(np.mean([r_["setup_ms"] for r_ in runs]))
-        best["solve_ms"] = float(np.mean([r_["solve_ms"] for r_ in runs]))
+        for key in ("total_ms", "setup_ms", "solve_ms"):
+            best[key] = float(np.mean([r_[key] for r_ in runs]))
         res[kind] = best
         print(f"[bench] {kind}: {best}", file=sys.stderr, flush=True)
+    os.close(devnull)
+    os.close(saved)
+    workload = (f"3-D Kuhn P1 -div(K grad u), K = R_z(theta) diag(1, eps, eps) R_z^T, eps={LEARNED_EPS}, "
+                f"theta={theta_dir:.4f}, {a.m}^3 cells, n={n}, seeded RHS, PCG to {a.tol:g}; "
+                "plain = UA HEM L1 V(1,1) FP64 drop-in; sa = SA on VMB with Chebyshev(2), constant "
+                "parameters; learned = the same with per-aggregate damping and bound scale from "
+                "the committed network, plain method as fallback; learned_fp32 = learned with the FP32 cycle "
+                "under the FP64 PCG")
+    return res, workload
+
+
+def main_learned(a, lib):
+    res, workload = learned_comparison(a, lib, a.steps, a.warmup, kinds=("plain", "sa", "learned", "learned_fp32"))
     line = {
         "metric": "learned vs plain AMG-PCG time-to-1e-10 (s), 3-D Kuhn P1 anisotropic eps=1e-3",
-        "value": res["learned"]["total_ms"] * 1e-3,
+        "value": res["learned_fp32"]["total_ms"] * 1e-3,
         "unit": "s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": res["learned"]["total_ms"],
+        "ms_per_step": res["learned_fp32"]["total_ms"],
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 (learned network f32)",
+        "dtype": "f32 cycle, f64 PCG (learned network f32)",
         "data": "synthetic",
-        "config": {"workload": (f"3-D Kuhn P1 -div(K grad u), K = R_z(theta) diag(1, eps, eps) R_z^T, eps={eps}, "
-                                f"theta={theta_dir:.4f}, {a.m}^3 cells, n={n}, seeded RHS, PCG to {a.tol:g}; "
-                                "plain = UA HEM L1 V(1,1) drop-in; sa = SA on VMB with Chebyshev(2), constant "
-                                "parameters; learned = the same with per-aggregate damping and bound scale from "
-                                "the committed network, plain method as fallback")},
+        "config": {"workload": workload},
         "plain": res["plain"], "sa_constant": res["sa"], "learned": res["learned"],
+        "learned_fp32": res["learned_fp32"],
         "speedup_learned_vs_plain": res["plain"]["total_ms"] / res["learned"]["total_ms"],
+        "speedup_learned_fp32_vs_plain": res["plain"]["total_ms"] / res["learned_fp32"]["total_ms"],
         "cpu_baseline": None,
     }
     print(json.dumps(line))
@@ -594,7 +618,7 @@ def main():
         lib.hz_get_last_info(C.byref(info))
         infos.append((r, info.setup_ms, info.solve_ms, info.total_ms, info.num_levels,
                       info.final_relres, list(info.level_rows[:info.num_levels]),
-                      list(info.level_nnz[:info.num_levels])))
+                      list(info.level_nnz[:info.num_levels]), info.solve_alg_bytes))
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     ms_per_step = 1e3 * wall / a.steps
@@ -632,6 +656,14 @@ def main():
 
     iters = [i[0] for i in infos]
     value = float(np.mean([i[3] for i in infos])) * 1e-3   # device time per solve (s)
+    lvp = None
+    if not a.no_learned and a.problem == "poisson" and a.precision == "fp64":
+        # the metric's second half: learned vs plain on the anisotropic 128^3 problem
+        res, workload = learned_comparison(a, lib, steps=3, warmup=1, kinds=("plain", "learned", "learned_fp32"))
+        lvp = {"workload": workload, "plain": res["plain"], "learned": res["learned"],
+               "learned_fp32": res["learned_fp32"],
+               "speedup_learned_vs_plain": res["plain"]["total_ms"] / res["learned"]["total_ms"],
+               "speedup_learned_fp32_vs_plain": res["plain"]["total_ms"] / res["learned_fp32"]["total_ms"]}
     cpu = None
     if not a.no_cpu_baseline:
         if a.cpu_sample_m == a.m:
@@ -688,6 +720,14 @@ def main():
         "setup_ms": float(np.mean([i[1] for i in infos])),
         "solve_ms": float(np.mean([i[2] for i in infos])),
         "true_rel_residual": true_rel,
+        "whole_solve": {
+            "what": "algorithmic bytes of every PCG kernel launch (counted by the library) over the PCG's "
+                    "device time (gaps between launches included)",
+            "bytes": infos[-1][8],
+            "solve_ms": infos[-1][2],
+            "gbs": infos[-1][8] / (infos[-1][2] * 1e-3) / 1e9 if infos[-1][2] > 0 else None,
+            "frac": infos[-1][8] / (infos[-1][2] * 1e-3) / 1e9 / HBM_PEAK_GBS if infos[-1][2] > 0 else None,
+        },
         "spmv_gbs": spmv_gbs,
         "roofline": {
             "bound": "hbm",
@@ -709,6 +749,7 @@ def main():
             "jacobi_ms": kt.jacobi_ms, "jacobi_gbs": jac_gbs, "sell_entries": kt.sell_entries,
         },
         "cpu_baseline": cpu,
+        "learned_vs_plain": lvp,
     }
     print(json.dumps(line))
 

@@ -480,7 +480,7 @@ forward:
                        fuse ? C.dsm : nullptr, fuse ? C.x : nullptr);
         } else {
             residual(L.S, L.b, L.x, L.w);
-            restrict_agg(L.nagg, L.rptr, L.rmem, L.w, C.b, cp.smoother, cp.jacobi_w,
+            restrict_agg(L.nagg, L.A.n, L.rptr, L.rmem, L.w, C.b, cp.smoother, cp.jacobi_w,
                          fuse ? C.dsm : nullptr, fuse ? C.x : nullptr);
         }
         ++l;
@@ -618,7 +618,7 @@ forward:
                          fuse ? C.dsmf : nullptr, fuse ? C.xf : nullptr);
         } else {
             residual_m(L.S, L.low, L.bf, L.xf, L.wf);
-            restrict_m(L.nagg, L.rptr, L.rmem, L.wf, C.bf, cp.smoother, w, fuse ? C.dsmf : nullptr,
+            restrict_m(L.nagg, L.A.n, L.rptr, L.rmem, L.wf, C.bf, cp.smoother, w, fuse ? C.dsmf : nullptr,
                        fuse ? C.xf : nullptr);
         }
         ++l;
@@ -816,6 +816,7 @@ void fetch(const Hierarchy &H, PcgScalars &out) {
 PcgResult pcg_solve(Hierarchy &H, const DevSELL &A, const double *b, double *u, double tol,
                     int MaxIt, int stop, int prt) {
     PcgResult res;
+    g_alg_bytes = 0.0;
     const int m = A.n;
     const double maxdiff = tol * 1e-4;
     int iter = 0, stag = 1, more_step = 1, restart_step = 1;
@@ -1001,6 +1002,7 @@ FINISHED:
             std::printf("Num_iter(krylov.c) = %lld with relative residual %e.\n", (long long)iter, relres);
     }
     HZ_CHECK(hipStreamSynchronize(stream()));
+    res.alg_bytes = g_alg_bytes;
     res.relres = relres;
     res.iter = (iter > MaxIt) ? HZ_ERROR_SOLVER_MAXIT : iter;
     return res;

@@ -78,6 +78,7 @@ struct PcgResult {
     int iter = 0;
     double relres = 0.0;
     double condest = 0.0;
+    double alg_bytes = 0.0;   // algorithmic bytes of the solve's kernels (g_alg_bytes)
     std::vector<double> relres_hist;
 };
 

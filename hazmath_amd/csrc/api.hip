@@ -345,6 +345,7 @@ INT core_solve(const DevCSR &A, const double *db, double *dx, linear_itsolver_pa
     g_info.solve_ms = ms_solve;
     g_info.total_ms = ms_setup + ms_solve;
     g_info.final_relres = pr.relres;
+    g_info.solve_alg_bytes = pr.alg_bytes;
     g_info.condest = condest;
     g_info.relres_count = (int)std::min<size_t>(pr.relres_hist.size(), 4096);
     for (int i = 0; i < g_info.relres_count; ++i) g_info.relres_hist[i] = pr.relres_hist[i];

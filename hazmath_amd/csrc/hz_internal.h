@@ -18,6 +18,7 @@ struct DevCSR {
 // same order as the CSR loops of the reference.
 struct DevSELL {
     int n = 0, nslices = 0;
+    long long nnz = 0;         // stored CSR entries (algorithmic bytes count these, not the padding)
     long long nent = 0;        // padded entries
     long long *sptr = nullptr; // nslices+1 offsets
     int *len = nullptr;        // row lengths
@@ -44,6 +45,16 @@ struct PcgScalars {
 };
 
 // Reduction workspace (partials + tickets), sized for the largest grid used.
+// Algorithmic bytes of the solve kernels launched since the counter was
+// last cleared (each distinct input array read once, each output written
+// once; gathered coarse corrections not counted): the numerator of the
+// whole-solve bandwidth.  The PCG clears it when it starts.
+extern double g_alg_bytes;
+inline void count_bytes(double b) { g_alg_bytes += b; }
+// matrix stream of a CSR / SELL launch: values (vb bytes each) and int32
+// columns of nnz entries, int32 row pointers of n rows
+inline double matrix_bytes(long long nnz, long n, int vb = 8) { return (double)nnz * (vb + 4) + 4.0 * (n + 1); }
+
 struct ReduceWs {
     double *partials = nullptr;   // kMaxBlocks * kSlots
     unsigned int *tickets = nullptr;
@@ -103,7 +114,7 @@ void smooth_from_zero(int n, int smoother, double w, const double *d, const doub
 // (smoother.c:57-60), L1DIAG uses the row's l1 norm (smoother.c:352-357).
 void smoother_diag(const DevSELL &S, int smoother, double *d);
 // bc = R w (member sums, ascending members);  optional fused zero-iterate sweep of the coarse level.
-void restrict_agg(int nc, const int *rptr, const int *rmem, const double *w, double *bc,
+void restrict_agg(int nc, int n, const int *rptr, const int *rmem, const double *w, double *bc,
                   int smoother, double wgt, const double *dc, double *xc);
 // Fused residual + restriction (+ coarse zero-iterate sweep when xc != null),
 // one thread per coarse row reading the member rows of the CSR.
@@ -266,7 +277,7 @@ void sweep_m(const DevSELL &S, const LowMatrix &M, int smoother, float w, const 
              const float *b, const float *x, float *xout, double *out64, const double *r64, double *dot_out,
              const ReduceWs &ws);
 void residual_m(const DevSELL &S, const LowMatrix &M, const float *b, const float *x, float *w);
-void restrict_m(int nc, const int *rptr, const int *rmem, const float *w, float *bc, int smoother, float wgt,
+void restrict_m(int nc, int n, const int *rptr, const int *rmem, const float *w, float *bc, int smoother, float wgt,
                 const float *dc, float *xc);
 void down_fused_m(int nc, const int *rptr, const int *rmem, const DevCSR &A, const LowMatrix &M, const float *b,
                   const float *x, float *bc, int smoother, float wgt, const float *dc, float *xc);

@@ -404,6 +404,7 @@ int coarse_dense_setup(const DevCSR &A, double *Ainv) {
 
 void coarse_dense_solve(int n, const double *Ainv, const double *b, double *x) {
     if (n == 0) return;
+    count_bytes(8.0 * n * (double)n + 16.0 * n);
     k_gemv<<<div_up(n, 4), 256, 0, stream()>>>(n, Ainv, b, x);
     HZ_LAUNCH_CHECK();
 }

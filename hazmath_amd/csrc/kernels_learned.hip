@@ -244,6 +244,7 @@ void cheby_apply(const DevSELL &S, const ChebLevel &c, int steps, const double *
             c2 = 2.0 * rn / delta;
             rho = rn;
         }
+        count_bytes(matrix_bytes(S.nnz, S.n) + 48.0 * S.n);
         if (grp)
             k_cheb_step_grp<G><<<div_up((long)S.n * G, kBlock), kBlock, 0, stream()>>>(S.n, A->ia, A->ja, A->val,
                                                                                        c.diag, c1, c2, b, x, xt,

@@ -495,6 +495,7 @@ void fill_f(float *x, long n) {
 
 void enter_m(int n, const double *r, int smoother, float w, const float *d, float *b, float *x) {
     if (n <= 0) return;
+    count_bytes(8.0 * n + 4.0 * n + (x ? 8.0 * n : 0.0));
     k_enter<<<grid_for(n), kBlock, 0, stream()>>>(n, r, smoother, w, d, b, x);
     HZ_LAUNCH_CHECK();
 }
@@ -503,6 +504,8 @@ void sweep_m(const DevSELL &S, const LowMatrix &M, int smoother, float w, const 
              const float *b, const float *x, float *xout, double *out64, const double *r64, double *dot_out,
              const ReduceWs &ws) {
     if (S.n == 0) return;
+    count_bytes(matrix_bytes(S.nnz, S.n, M.kind == kLowF32 ? 4 : 2) + 12.0 * S.n + (agg ? 4.0 * S.n : 0.0) +
+                (out64 ? 12.0 * S.n : 0.0));
     double *partials = out64 ? ws.partials : nullptr;
     if (M.kind == kLowF32) {
         const float *v = static_cast<const float *>(M.sell);
@@ -525,6 +528,7 @@ void sweep_m(const DevSELL &S, const LowMatrix &M, int smoother, float w, const 
 
 void residual_m(const DevSELL &S, const LowMatrix &M, const float *b, const float *x, float *w) {
     if (S.n == 0) return;
+    count_bytes(matrix_bytes(S.nnz, S.n, M.kind == kLowF32 ? 4 : 2) + 12.0 * S.n);
     const int grid = div_up(S.nslices, kSlicesPerBlock);
     if (M.kind == kLowF32)
         k_residual_m<float><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col,
@@ -535,9 +539,10 @@ void residual_m(const DevSELL &S, const LowMatrix &M, const float *b, const floa
     HZ_LAUNCH_CHECK();
 }
 
-void restrict_m(int nc, const int *rptr, const int *rmem, const float *w, float *bc, int smoother, float wgt,
+void restrict_m(int nc, int n, const int *rptr, const int *rmem, const float *w, float *bc, int smoother, float wgt,
                 const float *dc, float *xc) {
     if (nc <= 0) return;
+    count_bytes(8.0 * n + 4.0 * (nc + 1) + 4.0 * nc + (xc ? 8.0 * nc : 0.0));
     k_restrict_m<<<grid_for(nc), kBlock, 0, stream()>>>(nc, rptr, rmem, w, bc, smoother, wgt, dc, xc);
     HZ_LAUNCH_CHECK();
 }
@@ -545,6 +550,8 @@ void restrict_m(int nc, const int *rptr, const int *rmem, const float *w, float 
 void down_fused_m(int nc, const int *rptr, const int *rmem, const DevCSR &A, const LowMatrix &M, const float *b,
                   const float *x, float *bc, int smoother, float wgt, const float *dc, float *xc) {
     if (nc <= 0) return;
+    count_bytes(matrix_bytes(A.nnz, A.n, M.kind == kLowF32 ? 4 : 2) + 8.0 * A.n + 4.0 * A.n + 4.0 * (nc + 1) +
+                4.0 * nc + (xc ? 8.0 * nc : 0.0));
     if (M.kind == kLowF32)
         k_down_fused_m<float, 4><<<div_up((long)nc * 4, kBlock), kBlock, 0, stream()>>>(
             nc, rptr, rmem, A.ia, A.ja, static_cast<const float *>(M.csr), b, x, bc, smoother, wgt, dc, xc);
@@ -557,6 +564,7 @@ void down_fused_m(int nc, const int *rptr, const int *rmem, const DevCSR &A, con
 
 void prolong_m(int n, const int *agg, const float *xc, float *x) {
     if (n <= 0) return;
+    count_bytes(12.0 * n);
     k_prolong_m<<<grid_for(n), kBlock, 0, stream()>>>(n, agg, xc, x);
     HZ_LAUNCH_CHECK();
 }
@@ -580,6 +588,7 @@ void cheby_apply_m(const DevSELL &S, const DevCSR &A, const LowMatrix &M, const 
             rho = rn;
         }
         const bool last64 = out64 && k == steps - 1;
+        count_bytes(matrix_bytes(S.nnz, S.n, M.kind == kLowF32 ? 4 : 2) + 24.0 * S.n + (last64 ? 16.0 * S.n : 0.0));
         if (last64) {
             if (M.kind == kLowF32)
                 k_cheb_step_m<true, float><<<grid, kBlock, 0, stream()>>>(
@@ -620,6 +629,7 @@ void cheby_apply_m(const DevSELL &S, const DevCSR &A, const LowMatrix &M, const 
 
 void csr_mxv_m(const DevCSR &A, const float *val, const float *x, float *y) {
     if (A.n <= 0) return;
+    count_bytes(matrix_bytes(A.nnz, A.n, 4) + 4.0 * A.ncol + 4.0 * A.n);
     constexpr int G = 16;
     k_csr_mxv_m<G><<<div_up((long)A.n * G, kBlock), kBlock, 0, stream()>>>(A.n, A.ia, A.ja, val, x, y);
     HZ_LAUNCH_CHECK();
@@ -627,6 +637,7 @@ void csr_mxv_m(const DevCSR &A, const float *val, const float *x, float *y) {
 
 void csr_prolong_m(const DevCSR &P, const float *val, const float *xc, float *x) {
     if (P.n <= 0) return;
+    count_bytes(matrix_bytes(P.nnz, P.n, 4) + 4.0 * P.ncol + 8.0 * P.n);
     k_csr_prolong_m<<<grid_for(P.n), kBlock, 0, stream()>>>(P.n, P.ia, P.ja, val, xc, x);
     HZ_LAUNCH_CHECK();
 }
@@ -643,6 +654,7 @@ float *values_f32(const DevCSR &A) {
 
 void coarse_dense_solve_m(int n, const double *Ainv, const float *b, float *x) {
     if (n <= 0) return;
+    count_bytes(8.0 * n * (double)n + 8.0 * n);
     k_gemv_m<<<div_up(n, 4), 256, 0, stream()>>>(n, Ainv, b, x);
     HZ_LAUNCH_CHECK();
 }

@@ -593,12 +593,14 @@ void rap_general(const DevCSR &R, const DevCSR &A, const DevCSR &P, DevCSR &C) {
 }
 
 void csr_mxv(const DevCSR &A, const double *x, double *y) {
+    count_bytes(matrix_bytes(A.nnz, A.n) + 8.0 * A.ncol + 8.0 * A.n);
     if (A.n == 0) return;
     k_csr_mxv<16><<<div_up((long)A.n * 16, kBlock), kBlock, 0, stream()>>>(A.n, A.ia, A.ja, A.val, x, y);
     HZ_LAUNCH_CHECK();
 }
 
 void csr_prolong(const DevCSR &P, const double *xc, double alpha, const double *alpha_dev, double *x) {
+    count_bytes(matrix_bytes(P.nnz, P.n) + 8.0 * P.ncol + 16.0 * P.n);
     if (P.n == 0) return;
     k_csr_aAxpy<<<grid_for(P.n), kBlock, 0, stream()>>>(P.n, P.ia, P.ja, P.val, xc, alpha, alpha_dev, x);
     HZ_LAUNCH_CHECK();

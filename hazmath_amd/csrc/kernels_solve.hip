@@ -603,8 +603,11 @@ inline int stream_grid(long n) {
 
 }  // namespace
 
+double g_alg_bytes = 0.0;
+
 void sell_from_csr(const DevCSR &A, DevSELL &S) {
     S.n = A.n;
+    S.nnz = A.nnz;
     S.nslices = div_up(A.n, kSlice);
     const int ns = S.nslices;
     long long *wcount = arena().alloc_n<long long>((size_t)ns + 1);
@@ -641,6 +644,7 @@ void sell_from_csr(const DevCSR &A, DevSELL &S) {
 void spmv(const DevSELL &S, const double *x, double *y, double *dot_out, const ReduceWs &ws,
           LaunchTimer *timer) {
     if (S.n == 0) return;
+    count_bytes(matrix_bytes(S.nnz, S.n) + 16.0 * S.n);
     const int grid = div_up(S.nslices, kSlicesPerBlock);
     if (dot_out) {
         if (timer) timer->begin();
@@ -658,6 +662,7 @@ void spmv(const DevSELL &S, const double *x, double *y, double *dot_out, const R
 
 void residual(const DevSELL &S, const double *b, const double *x, double *w) {
     if (S.n == 0) return;
+    count_bytes(matrix_bytes(S.nnz, S.n) + 24.0 * S.n);
     k_residual<<<div_up(S.nslices, kSlicesPerBlock), kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr,
                                                                            S.col, S.val, b, x, w);
     HZ_LAUNCH_CHECK();
@@ -668,6 +673,7 @@ template <bool kProlong, bool kDot>
 void launch_sweep(const DevSELL &S, int smoother, double w, const int *agg, const double *xc,
                   double alpha, const double *alpha_dev, const double *b, const double *x, double *xout,
                   double *partials) {
+    count_bytes(matrix_bytes(S.nnz, S.n) + 24.0 * S.n + (kProlong ? 4.0 * S.n : 0.0));
     const int grid = div_up(S.nslices, kSlicesPerBlock);
     if (smoother == HZ_SMOOTHER_L1DIAG)
         k_sweep<HZ_SMOOTHER_L1DIAG, kProlong, kDot><<<grid, kBlock, 0, stream()>>>(
@@ -712,13 +718,15 @@ void smoother_diag(const DevSELL &S, int smoother, double *d) {
 
 void smooth_from_zero(int n, int smoother, double w, const double *d, const double *b, double *x) {
     if (n == 0) return;
+    count_bytes(24.0 * n);
     k_smooth_from_zero<<<stream_grid(n), kBlock, 0, stream()>>>(n, smoother, w, d, b, x);
     HZ_LAUNCH_CHECK();
 }
 
-void restrict_agg(int nc, const int *rptr, const int *rmem, const double *w, double *bc,
+void restrict_agg(int nc, int n, const int *rptr, const int *rmem, const double *w, double *bc,
                   int smoother, double wgt, const double *dc, double *xc) {
     if (nc == 0) return;
+    count_bytes(12.0 * n + 4.0 * (nc + 1) + 8.0 * nc + (xc ? 16.0 * nc : 0.0));
     k_restrict<<<stream_grid(nc), kBlock, 0, stream()>>>(nc, rptr, rmem, w, bc, smoother, wgt, dc, xc);
     HZ_LAUNCH_CHECK();
 }
@@ -726,6 +734,8 @@ void restrict_agg(int nc, const int *rptr, const int *rmem, const double *w, dou
 void down_fused(int nc, const int *rptr, const int *rmem, const DevCSR &A, const double *b,
                 const double *x, double *bc, int smoother, double wgt, const double *dc, double *xc) {
     if (nc == 0) return;
+    count_bytes(matrix_bytes(A.nnz, A.n) + 16.0 * A.n + 4.0 * A.n + 4.0 * (nc + 1) + 8.0 * nc +
+                (xc ? 16.0 * nc : 0.0));
     constexpr int kLanes = 4;
     k_down_fused<kLanes><<<div_up((long)nc * kLanes, kBlock), kBlock, 0, stream()>>>(
         nc, rptr, rmem, A.ia, A.ja, A.val, b, x, bc, smoother, wgt, dc, xc);
@@ -748,18 +758,21 @@ void up_fused(const DevSELL &S, int smoother, double w, const int *agg, const do
 void prolong_agg(int n, const int *agg, const double *xc, double alpha, const double *alpha_dev,
                  double *x) {
     if (n == 0) return;
+    count_bytes(20.0 * n);
     k_prolong<<<stream_grid(n), kBlock, 0, stream()>>>(n, agg, xc, alpha, alpha_dev, x);
     HZ_LAUNCH_CHECK();
 }
 
 void fill(double *x, double v, long n) {
     if (n == 0) return;
+    count_bytes(8.0 * n);
     k_fill<<<stream_grid(n), kBlock, 0, stream()>>>(x, v, n);
     HZ_LAUNCH_CHECK();
 }
 
 void copy(double *dst, const double *src, long n) {
     if (n == 0) return;
+    count_bytes(16.0 * n);
     k_copy<<<stream_grid(n), kBlock, 0, stream()>>>(dst, src, n);
     HZ_LAUNCH_CHECK();
 }
@@ -767,6 +780,7 @@ void copy(double *dst, const double *src, long n) {
 void pcg_update(int n, PcgScalars *s, const double *p, const double *t, double *u, double *r,
                 double *alpha_hist, int iter, const ReduceWs &ws) {
     const int G = stream_grid(n);
+    count_bytes(48.0 * n);
     k_pcg_update<<<G, kBlock, 0, stream()>>>(n, s, p, t, u, r, ws.partials);
     HZ_LAUNCH_CHECK();
     k_final_update<<<1, kFinBlock, 0, stream()>>>(s, ws.partials, G, alpha_hist, iter);
@@ -775,6 +789,7 @@ void pcg_update(int n, PcgScalars *s, const double *p, const double *t, double *
 
 void dot(int n, const double *x, const double *y, double *out, const ReduceWs &ws) {
     const int G = stream_grid(n);
+    count_bytes(x == y ? 8.0 * n : 16.0 * n);
     k_dot<<<G, kBlock, 0, stream()>>>(n, x, y, ws.partials);
     HZ_LAUNCH_CHECK();
     k_final_sum<<<1, kFinBlock, 0, stream()>>>(ws.partials, G, out);
@@ -784,6 +799,7 @@ void dot(int n, const double *x, const double *y, double *out, const ReduceWs &w
 void norm2sq(int n, const double *x, double *out, const ReduceWs &ws) { dot(n, x, x, out, ws); }
 
 void pcg_direction(int n, PcgScalars *s, const double *z, double *p, double *beta_hist, int iter) {
+    count_bytes(24.0 * n);
     k_pcg_direction<<<stream_grid(n), kBlock, 0, stream()>>>(n, s, z, p, beta_hist, iter);
     HZ_LAUNCH_CHECK();
 }

@@ -368,6 +368,10 @@ typedef struct {
                                             (after the floor), 0 on levels without one */
     REAL   lambda_ratio[HZ_MAX_AMG_LVL]; /* learned SA: estimated lambda_max(D^-1 A) over the
                                             Gershgorin bound per level, 0 where not estimated */
+    REAL   solve_alg_bytes;  /* algorithmic bytes of every kernel the PCG launched (each distinct
+                                input read once, each output written once; the gathered coarse
+                                corrections of the transfers not counted): over solve_ms, the
+                                whole-solve bandwidth */
 } hz_solve_info;
 
 void hz_get_last_info(hz_solve_info *info);
