@@ -141,7 +141,8 @@ def parse():
     p.add_argument("--smoother", choices=SMOOTHER, default="l1")
     p.add_argument("--jacobi-weight", type=float, default=1.0)
     p.add_argument("--relaxation", type=float, default=1.0, help="SOR-type relaxation weight")
-    p.add_argument("--coarse-dof", type=int, default=200)
+    p.add_argument("--coarse-dof", type=int, default=1000,
+                   help="rows at or below which the coarsest level is solved by its dense inverse")
     p.add_argument("--theta", type=float, default=0.04)
     p.add_argument("--tol", type=float, default=1e-10)
     p.add_argument("--kernel-reps", type=int, default=50)
@@ -435,7 +436,7 @@ def learned_comparison(a, lib, steps, warmup, kinds=("plain", "sa", "learned", "
     def run(kind):
         x = capi.HostVec(np.zeros(n))
         it, amg = capi.solver_params(lib, tol=a.tol, maxit=1000, aggregation=capi.HEM,
-                                     smoother=capi.SMOOTHER_L1DIAG, coarse_dof=200,
+                                     smoother=capi.SMOOTHER_L1DIAG, coarse_dof=a.coarse_dof,
                                      coarse_solver=capi.SOLVER_UMFPACK, max_levels=20, theta=0.04)
         opts = capi.hz_options()
         lib.hz_get_options(C.byref(opts))
@@ -487,7 +488,7 @@ def learned_comparison(a, lib, steps, warmup, kinds=("plain", "sa", "learned", "
     os.close(saved)
     workload = (f"3-D Kuhn P1 -div(K grad u), K = R_z(theta) diag(1, eps, eps) R_z^T, eps={LEARNED_EPS}, "
                 f"theta={theta_dir:.4f}, {a.m}^3 cells, n={n}, seeded RHS, PCG to {a.tol:g}; "
-                "plain = UA HEM L1 V(1,1) FP64 drop-in; sa = SA on VMB with Chebyshev(2), constant "
+                f"coarse_dof={a.coarse_dof}; plain = UA HEM L1 V(1,1) FP64 drop-in; sa = SA on VMB with Chebyshev(2), constant "
                 "parameters; learned = the same with per-aggregate damping and bound scale from "
                 "the committed network, plain method as fallback; learned_fp32 = learned with the FP32 cycle "
                 "under the FP64 PCG")

@@ -235,9 +235,20 @@ int amg_setup(Hierarchy &H, const DevCSR &A0, AMG_param *param, bool) {
                 H.cheb_scale[lvl] = scale;
                 omega_agg = om;
             }
+            double ts[4] = {0, 0, 0, 0};
+            if (tr) { HZ_CHECK(hipStreamSynchronize(stream())); ts[0] = wall_ms(); }
             sa_prolongation(L.A, N, L.agg, nagg, param->tentative_smooth, param->smooth_filter, L.P, omega_agg);
+            if (tr) { HZ_CHECK(hipStreamSynchronize(stream())); ts[1] = wall_ms(); }
             csr_transpose_gpu(L.P, L.R);
+            if (tr) { HZ_CHECK(hipStreamSynchronize(stream())); ts[2] = wall_ms(); }
             rap_general(L.R, L.A, L.P, C.A);
+            if (tr) {
+                HZ_CHECK(hipStreamSynchronize(stream()));
+                ts[3] = wall_ms();
+                std::fprintf(stderr, "[hz]   SA level %d: learned parameters %.2f ms, P %.2f ms (nnz %d), R %.2f ms, "
+                             "RAP %.2f ms (nnz %d)\n", lvl, ts[0] - t2, ts[1] - ts[0], L.P.nnz, ts[2] - ts[1],
+                             ts[3] - ts[2], C.A.nnz);
+            }
             L.sa = true;
         } else {
             L.rptr = arena().alloc_n<int>((size_t)nagg + 1);

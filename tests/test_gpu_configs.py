@@ -120,14 +120,17 @@ def test_config2_kuhn128_hierarchy_bitexact_and_pcg(lib, orc, libc):
     assert thetas["gpu"] == thetas["oracle"]
     del levels, g, o
     b = problems.uniform_rhs(n, seed=1)
-    kw = dict(tol=1e-10, maxit=1000, **_KUHN_KW)
-    _say("config 2: GPU solve")
-    ig, xg = _gpu_solve(lib, libc, A, b, **kw)
-    _say("config 2: oracle solve")
-    io, xo, t = _oracle_solve(orc, libc, A, b, **kw)
-    _say(f"config 2: GPU {ig} iterations, oracle {io} ({t:.1f} s)")
-    assert io > 0 and abs(ig - io) <= 1
-    assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) < 1e-10
+    # coarse_dof 200 (param_amg_init's default) and 1000 (the benchmark's: the
+    # same hierarchy cut three levels earlier, a dense coarsest inverse of 992 rows)
+    for coarse_dof in (200, 1000):
+        kw = dict(tol=1e-10, maxit=1000, **dict(_KUHN_KW, coarse_dof=coarse_dof))
+        _say(f"config 2: GPU solve, coarse_dof {coarse_dof}")
+        ig, xg = _gpu_solve(lib, libc, A, b, **kw)
+        _say(f"config 2: oracle solve, coarse_dof {coarse_dof}")
+        io, xo, t = _oracle_solve(orc, libc, A, b, **kw)
+        _say(f"config 2: coarse_dof {coarse_dof}: GPU {ig} iterations, oracle {io} ({t:.1f} s)")
+        assert io > 0 and abs(ig - io) <= 1
+        assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) < 1e-10
 
 
 @pytest.mark.parametrize("precision,bound", [(1, 1e-8), (2, 1e-8)])
