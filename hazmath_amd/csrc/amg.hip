@@ -233,6 +233,7 @@ int amg_setup(Hierarchy &H, const DevCSR &A0, AMG_param *param, bool) {
                     H.lambda_ratio[lvl] = ratio;
                 }
                 H.cheb_scale[lvl] = scale;
+                L.cheb = g1;   // the diagonal and bound, rescaled when the cycle is prepared
                 omega_agg = om;
             }
             double ts[4] = {0, 0, 0, 0};
@@ -344,7 +345,12 @@ void amg_prepare_cycle(Hierarchy &H, const AMG_param *param) {
             smoother_diag(L.S, cp.smoother, L.dsm);
             if (cp.smoother == HZ_SMOOTHER_CHEBY) {
                 const double sc = l < (int)H.cheb_scale.size() ? H.cheb_scale[l] : 1.0;
-                L.cheb = cheby_prepare(L.A, sc);
+                if (L.cheb.diag) {   // formed during the setup (learned parameters)
+                    L.cheb.lmax = sc * L.cheb.gersh;
+                    L.cheb.lmin = L.cheb.lmax / 30.0;
+                } else {
+                    L.cheb = cheby_prepare(L.A, sc);
+                }
                 L.cwork = arena().alloc_n<double>(n);
             }
             if (is_gs_family(cp.smoother)) gs_prepare(L.A, L.gs);

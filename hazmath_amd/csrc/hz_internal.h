@@ -202,6 +202,7 @@ void csr_prolong(const DevCSR &P, const double *xc, double alpha, const double *
 struct ChebLevel {
     double *diag = nullptr;   // first diagonal entry per row
     double lmax = 0.0, lmin = 0.0;
+    double gersh = 0.0;       // max_i sum_j |a_ij| / |a_ii| (lmax = scale * gersh)
 };
 // diag and lmax = scale * Gershgorin bound of D^-1 A, lmin = lmax / 30
 ChebLevel cheby_prepare(const DevCSR &A, double scale);

@@ -221,6 +221,7 @@ ChebLevel cheby_prepare(const DevCSR &A, double scale) {
     HZ_LAUNCH_CHECK();
     double hr = 0.0;
     HZ_CHECK(memcpy_sync(&hr, rho, sizeof(double), hipMemcpyDeviceToHost));
+    c.gersh = hr;
     c.lmax = scale * hr;
     c.lmin = c.lmax / 30.0;
     return c;
@@ -290,61 +291,63 @@ namespace {
 // Lanczos on B = D^-1/2 A D^-1/2 (similar to D^-1 A, symmetric for SPD A;
 // d: first diagonal entries, rows with d <= 0 drop out).  The scalars of the
 // recurrence stay on the device (alpha[k], beta[k]); the host reads them once.
-// 16 lanes share a row of A (coalesced loads) and combine their products by a
-// butterfly: an estimate, so the summation order is free.
+// Every kernel runs a fixed grid of kLzGrid blocks (grid-stride loops), so
+// each reduction has at most kLzGrid partials.  16 lanes share a row of A
+// (coalesced loads) and combine their products by a butterfly: an estimate,
+// so the summation order is free.
 constexpr int kLzLanes = 16;
+constexpr int kLzGrid = 2048;
 
-// w = B v (v already divided by beta of the previous step when inv_beta is set),
-// block partials of (w, v)
+// v_k = s wprev (s = 1 / *norm; written out) and y = B v_k, block partials of
+// (y, v_k); v_k's gathered entries are formed from wprev on the fly
 __global__ __launch_bounds__(kBlock) void k_lz_apply(int n, const int *__restrict__ ia, const int *__restrict__ ja,
                                                      const double *__restrict__ a, const double *__restrict__ isd,
-                                                     const double *__restrict__ v, double *__restrict__ w,
-                                                     double *__restrict__ part) {
+                                                     const double *__restrict__ wprev,
+                                                     const double *__restrict__ norm, double *__restrict__ v,
+                                                     double *__restrict__ y, double *__restrict__ part) {
     __shared__ double sh[kBlock / kWave];
-    const long gt = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = (int)(gt / kLzLanes), j = (int)(gt % kLzLanes);
-    const bool valid = i < n;
-    const int k0 = valid ? ia[i] : 0, k1 = valid ? ia[i + 1] : 0;
-    double t = 0.0;
-    for (int k = k0 + j; k < k1; k += kLzLanes) t += a[k] * (isd[ja[k]] * v[ja[k]]);
-#pragma unroll
-    for (int o = kLzLanes / 2; o > 0; o >>= 1) t += __shfl_xor(t, o, kWave);
+    const double nv = *norm;
+    const double sc = nv > 0.0 ? 1.0 / nv : 0.0;
+    const int groups = gridDim.x * (kBlock / kLzLanes);
+    const int g0 = blockIdx.x * (kBlock / kLzLanes) + threadIdx.x / kLzLanes;
+    const int j = threadIdx.x % kLzLanes;
     double q = 0.0;
-    if (valid && j == 0) {
-        const double wi = isd[i] * t;
-        w[i] = wi;
-        q = wi * v[i];
+    for (int i = g0; i < n; i += groups) {   // trip count uniform within the lane group
+        const int k0 = ia[i], k1 = ia[i + 1];
+        double t = 0.0;
+        for (int k = k0 + j; k < k1; k += kLzLanes) {
+            const int c = ja[k];
+            t += a[k] * (isd[c] * (sc * wprev[c]));
+        }
+#pragma unroll
+        for (int o = kLzLanes / 2; o > 0; o >>= 1) t += __shfl_xor(t, o, kWave);
+        if (j == 0) {
+            const double vi = sc * wprev[i];
+            const double yi = isd[i] * t;
+            v[i] = vi;
+            y[i] = yi;
+            q += yi * vi;
+        }
     }
     const double bs = block_sum<kBlock>(q, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = bs;
 }
 
-// w -= alpha_k v + beta_{k-1} vprev; block partials of |w|^2
+// y -= alpha_k v + beta_{k-1} vprev; block partials of |y|^2
 __global__ __launch_bounds__(kBlock) void k_lz_orth(int n, const double *__restrict__ alpha,
                                                     const double *__restrict__ beta_prev,
                                                     const double *__restrict__ v, const double *__restrict__ vprev,
-                                                    double *__restrict__ w, double *__restrict__ part) {
+                                                    double *__restrict__ y, double *__restrict__ part) {
     __shared__ double sh[kBlock / kWave];
     const double al = *alpha, be = beta_prev ? *beta_prev : 0.0;
     double q = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const double wi = w[i] - al * v[i] - (beta_prev ? be * vprev[i] : 0.0);
-        w[i] = wi;
-        q += wi * wi;
+        const double yi = y[i] - al * v[i] - (beta_prev ? be * vprev[i] : 0.0);
+        y[i] = yi;
+        q += yi * yi;
     }
     const double bs = block_sum<kBlock>(q, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = bs;
-}
-
-// vprev = v, v = w / beta (beta = 0: the Krylov space is invariant; v = 0)
-__global__ void k_lz_next(int n, const double *__restrict__ beta, const double *__restrict__ w,
-                          double *__restrict__ v, double *__restrict__ vprev) {
-    const double be = *beta;
-    const double s = be > 0.0 ? 1.0 / be : 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        vprev[i] = v[i];
-        v[i] = s * w[i];
-    }
 }
 
 // the block partials summed in one block (fixed order); sqrt when asked
@@ -357,20 +360,19 @@ __global__ __launch_bounds__(kBlock) void k_sum_partials(const double *__restric
     if (threadIdx.x == 0) *out = take_sqrt ? sqrt(fmax(t, 0.0)) : t;
 }
 
-// 1/sqrt(d_i) (0 where d_i <= 0) and a fixed start vector with every entry
-// nonzero and no smooth structure
-__global__ void k_lz_start(int n, const double *__restrict__ d, double *__restrict__ isd, double *__restrict__ v) {
+// 1/sqrt(d_i) (0 where d_i <= 0), a fixed start vector with every entry
+// nonzero and no smooth structure, and block partials of its |.|^2
+__global__ __launch_bounds__(kBlock) void k_lz_start(int n, const double *__restrict__ d, double *__restrict__ isd,
+                                                     double *__restrict__ w, double *__restrict__ part) {
+    __shared__ double sh[kBlock / kWave];
+    double q = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         isd[i] = d[i] > 0.0 ? 1.0 / sqrt(d[i]) : 0.0;
         const unsigned h = (unsigned)i * 2654435761u;
-        v[i] = 0.25 + (double)((h >> 8) & 0xffffu) / 65536.0;
+        const double wi = 0.25 + (double)((h >> 8) & 0xffffu) / 65536.0;
+        w[i] = wi;
+        q += wi * wi;
     }
-}
-
-__global__ __launch_bounds__(kBlock) void k_sumsq(int n, const double *__restrict__ v, double *__restrict__ part) {
-    __shared__ double sh[kBlock / kWave];
-    double q = 0.0;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) q += v[i] * v[i];
     const double bs = block_sum<kBlock>(q, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = bs;
 }
@@ -396,28 +398,25 @@ double lanczos_lambda_dinv(const DevCSR &A, const double *d, int steps, int *ste
     if (steps_done) *steps_done = 0;
     if (n == 0 || steps <= 0) return 0.0;
     hipStream_t st = stream();
-    const int g = div_up((long)n * kLzLanes, kBlock);
-    const int gv = grid_for(n);
-    const int gp = std::max(g, gv);
+    const int G = std::max(1, std::min(kLzGrid, div_up((long)n * kLzLanes, kBlock)));
     double *isd = arena().alloc_n<double>((size_t)n + 1);
-    double *v = arena().alloc_n<double>((size_t)n + 1);
-    double *vprev = arena().alloc_n<double>((size_t)n + 1);
-    double *w = arena().alloc_n<double>((size_t)n + 1);
-    double *part = arena().alloc_n<double>((size_t)gp + 1);
-    double *ab = arena().alloc_n<double>(2 * (size_t)steps + 2);   // alpha[0..steps), beta[0..steps), norm
-    double *alpha = ab, *beta = ab + steps, *nrm = ab + 2 * steps;
-    k_lz_start<<<gv, kBlock, 0, st>>>(n, d, isd, v);
-    k_sumsq<<<gv, kBlock, 0, st>>>(n, v, part);
-    k_sum_partials<<<1, kBlock, 0, st>>>(part, gv, 1, nrm);
-    k_lz_next<<<gv, kBlock, 0, st>>>(n, nrm, v, w, vprev);   // w = v / |v| (vprev: scratch)
-    std::swap(v, w);
+    double *vb[2] = {arena().alloc_n<double>((size_t)n + 1), arena().alloc_n<double>((size_t)n + 1)};
+    double *yb[2] = {arena().alloc_n<double>((size_t)n + 1), arena().alloc_n<double>((size_t)n + 1)};
+    double *part = arena().alloc_n<double>((size_t)G + 1);
+    double *ab = arena().alloc_n<double>(2 * (size_t)steps + 2);   // alpha[0..steps), beta[0..steps), |w0|
+    double *alpha = ab, *beta = ab + steps, *nrm0 = ab + 2 * steps;
+    // y[1] holds the start vector: step 0 reads it as "wprev" with norm |w0|
+    k_lz_start<<<G, kBlock, 0, st>>>(n, d, isd, yb[1], part);
+    k_sum_partials<<<1, kBlock, 0, st>>>(part, G, 1, nrm0);
     HZ_LAUNCH_CHECK();
     for (int k = 0; k < steps; ++k) {
-        k_lz_apply<<<g, kBlock, 0, st>>>(n, A.ia, A.ja, A.val, isd, v, w, part);
-        k_sum_partials<<<1, kBlock, 0, st>>>(part, g, 0, alpha + k);
-        k_lz_orth<<<gv, kBlock, 0, st>>>(n, alpha + k, k > 0 ? beta + k - 1 : nullptr, v, vprev, w, part);
-        k_sum_partials<<<1, kBlock, 0, st>>>(part, gv, 1, beta + k);
-        if (k + 1 < steps) k_lz_next<<<gv, kBlock, 0, st>>>(n, beta + k, w, v, vprev);
+        double *v = vb[k & 1], *vprev = vb[(k + 1) & 1];
+        double *y = yb[k & 1];
+        const double *wprev = yb[(k + 1) & 1];
+        k_lz_apply<<<G, kBlock, 0, st>>>(n, A.ia, A.ja, A.val, isd, wprev, k > 0 ? beta + k - 1 : nrm0, v, y, part);
+        k_sum_partials<<<1, kBlock, 0, st>>>(part, G, 0, alpha + k);
+        k_lz_orth<<<G, kBlock, 0, st>>>(n, alpha + k, k > 0 ? beta + k - 1 : nullptr, v, vprev, y, part);
+        k_sum_partials<<<1, kBlock, 0, st>>>(part, G, 1, beta + k);
         HZ_LAUNCH_CHECK();
     }
     std::vector<double> h(2 * (size_t)steps);
