@@ -323,6 +323,8 @@ def _declare(lib):
     lib.hz_mlp_init.argtypes = [P(F), C.c_ulonglong]
     lib.hz_mlp_forward.argtypes = [P(F), P(F), INT, P(F)]
     lib.hz_mlp_forward.restype = INT
+    lib.hz_cycle_rate_agg.argtypes = [P(dCSRmat), P(AMG_param), P(F), INT, REAL, INT, P(REAL)]
+    lib.hz_cycle_rate_agg.restype = INT
     lib.hz_lambda_max_dinv.argtypes = [P(dCSRmat), INT, P(REAL), P(REAL)]
     lib.hz_lambda_max_dinv.restype = INT
     lib.hz_mlp_forward_noalloc.argtypes = [C.c_void_p, C.c_void_p, INT, C.c_void_p, C.c_void_p]

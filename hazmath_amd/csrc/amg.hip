@@ -205,7 +205,10 @@ int amg_setup(Hierarchy &H, const DevCSR &A0, AMG_param *param, bool) {
         if (param->AMG_type == HZ_SA_AMG) {
             // amg_setup_smoothP_smoothR: smoothed P, R = P^T, C = R A P
             const float *omega_agg = nullptr;
-            if (H.mlp_theta) {
+            if (lvl == 0 && H.omega_level0) {
+                if (nagg != H.omega_level0_n) return HZ_ERROR_INPUT_PAR;
+                omega_agg = H.omega_level0;
+            } else if (H.mlp_theta) {
                 // learned parameters: features of the aggregates -> network ->
                 // per-aggregate damping and this level's Chebyshev bound scale
                 int *rp = arena().alloc_n<int>((size_t)nagg + 1);

@@ -57,6 +57,10 @@ struct Hierarchy {
     // learned parameters: when set (SA only), each level's per-aggregate damping
     // and Chebyshev bound scale come from the network with these weights
     const float *mlp_theta = nullptr;
+    // when set (SA only): level 0's per-aggregate prolongator damping (device,
+    // omega_level0_n entries; the setup fails when level 0 has another count)
+    const float *omega_level0 = nullptr;
+    int omega_level0_n = 0;
     std::vector<double> cheb_scale;   // per level (default 1.0)
     std::vector<double> lambda_ratio; // learned: lambda_est(D^-1 A) / Gershgorin bound per level
     LaunchTimer *spmv_timer = nullptr;  // when set: events around the PCG's fine-level SpMV

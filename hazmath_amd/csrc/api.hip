@@ -1145,6 +1145,34 @@ INT hz_two_grid_rate(dCSRmat *A, AMG_param *amgparam, REAL omega, REAL lscale, I
     }
 }
 
+INT hz_cycle_rate_agg(dCSRmat *A, AMG_param *amgparam, const float *omega_agg, INT nagg, REAL lscale, INT iters,
+                      REAL *rate) {
+    *rate = 0.0;
+    if (!omega_agg || nagg <= 0 || iters < 2) return HZ_ERROR_INPUT_PAR;
+    if (!device_ready()) return HZ_ERROR_DEVICE;
+    try {
+        arena().reset();
+        const DevCSR dA = upload_csr(A);
+        AMG_param p = *amgparam;
+        p.AMG_type = HZ_SA_AMG;
+        p.smoother = HZ_SMOOTHER_CHEBY;
+        p.cycle_type = HZ_V_CYCLE;
+        Hierarchy H;
+        H.hec_rng = g_opt.hec_rng_source;
+        H.cheb_scale.assign(HZ_MAX_AMG_LVL, lscale);
+        H.omega_level0 = upload(omega_agg, (size_t)nagg);
+        H.omega_level0_n = nagg;
+        const int st = amg_setup(H, dA, &p, false);
+        if (st <= -1000) return fatal(st);
+        if (st < 0) return st;
+        amg_prepare_cycle(H, &p);
+        *rate = error_contraction(H, iters, 0x5eedULL);
+        return HZ_SUCCESS;
+    } catch (const DeviceError &) {
+        return HZ_ERROR_DEVICE;
+    }
+}
+
 INT hz_aggregate_features(dCSRmat *A, const INT *aggregates, INT num_aggregates, INT level, float *feat) {
     if (!device_ready()) return HZ_ERROR_DEVICE;
     try {

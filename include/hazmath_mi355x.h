@@ -505,6 +505,13 @@ INT hz_krylov_amg_learned(dCSRmat *A, dvector *b, dvector *x, linear_itsolver_pa
  * `iters` times on A x = 0 from a seeded random x_0; *rate =
  * (||x_K||_A/||x_{K/2}||_A)^(1/(K-K/2)) estimates the asymptotic factor. */
 INT hz_two_grid_rate(dCSRmat *A, AMG_param *amgparam, REAL omega, REAL lscale, INT iters, REAL *rate);
+/* Labelling with per-aggregate damping: the same contraction estimate for the
+ * cycle of amgparam's whole SA hierarchy (Chebyshev smoother with bound scale
+ * lscale on every level, V-cycle), with level 0's prolongator damped per
+ * aggregate by omega_agg[nagg] (nagg must equal level 0's aggregate count,
+ * else HZ_ERROR_INPUT_PAR) and coarser levels by amgparam->tentative_smooth. */
+INT hz_cycle_rate_agg(dCSRmat *A, AMG_param *amgparam, const float *omega_agg, INT nagg, REAL lscale, INT iters,
+                      REAL *rate);
 /* Parity hooks: feat[nagg][12] of the aggregates of level `level`; `steps`
  * Chebyshev steps on x in place; the smoothed prolongation with strength
  * threshold theta, damping omega or per-aggregate omega_agg (may be NULL). */
