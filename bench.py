@@ -43,7 +43,7 @@ def host_cpu_model():
 
 
 def _counter_rows(outdir):
-    """(dispatch id, kernel name, counter name, value) rows of every
+    """(dispatch id, kernel name, counter name, value, grid size) rows of every
     *counter_collection.csv that rocprofv3 wrote under outdir."""
     import csv
     rows = []
@@ -52,10 +52,15 @@ def _counter_rows(outdir):
             for r in csv.DictReader(fh):
                 try:
                     rows.append((int(r["Dispatch_Id"]), r["Kernel_Name"], r["Counter_Name"],
-                                 float(r["Counter_Value"])))
+                                 float(r["Counter_Value"]), int(r.get("Grid_Size", 0) or 0)))
                 except (KeyError, ValueError):
                     continue
     return rows
+
+
+# fine-level kernels whose HBM traffic the counter passes report: the graded
+# SpMV, and the V-cycle's fine-level residual and last sweep (largest grid)
+TRAFFIC_KERNELS = {"spmv": r"\bk_spmv<true", "residual": r"\bk_residual<", "sweep_dot": r"\bk_sweep<10, false, true"}
 
 
 def measure_traffic(a):
@@ -93,27 +98,38 @@ def measure_traffic(a):
                 return None, f"rocprofv3 --pmc {counter} wrote no counters (rc {r.returncode}): {tail}"
             vals = {}
             names = {}
-            for d, k, c, v in rows:
+            grids = {}
+            for d, k, c, v, g in rows:
                 if c == counter:
                     vals[d] = vals.get(d, 0.0) + v
                     names[d] = k
+                    grids[d] = g
             ids = sorted(vals)
             cal_ids = ids[-CAL_COPIES:]
-            # demangled names: "void hz::(anonymous namespace)::k_spmv<true>(int, ...)"
-            spmv = [vals[d] for d in ids if re.search(r"\bk_spmv<true>\(", names[d])]
-            if len(cal_ids) < CAL_COPIES or not spmv or not all("opy" in names[d] for d in cal_ids):
+            if len(cal_ids) < CAL_COPIES or not all("opy" in names[d] for d in cal_ids):
                 seen = sorted({names[d][:60] for d in ids})[:8]
-                return None, (f"{counter}: calibration copies or {SPMV_KERNEL} dispatches not found "
+                return None, (f"{counter}: calibration copies not found "
                               f"({len(ids)} dispatches; last {[names[d][:40] for d in cal_ids]}; seen {seen})")
             cal[counter] = float(np.mean([vals[d] for d in cal_ids]))
-            per[counter] = (float(np.mean(spmv)), len(spmv))
+            for key, pat in TRAFFIC_KERNELS.items():
+                # demangled names: "void hz::(anonymous namespace)::k_spmv<true>(int, ...)"
+                ds = [d for d in ids if re.search(pat, names[d])]
+                if not ds:
+                    return None, f"{counter}: no {key} dispatches"
+                gmax = max(grids[d] for d in ds)
+                sel = [vals[d] for d in ds if grids[d] == gmax]
+                per.setdefault(key, {})[counter] = (float(np.mean(sel)), len(sel))
     cal_bytes = COPY_CAL_DOUBLES * 8
     f_fetch = cal_bytes / (cal["FETCH_SIZE"] * 1024.0)
     f_write = cal_bytes / (cal["WRITE_SIZE"] * 1024.0)
-    traffic = per["FETCH_SIZE"][0] * 1024.0 * f_fetch + per["WRITE_SIZE"][0] * 1024.0 * f_write
-    return {"traffic_bytes": traffic, "fetch_kib": per["FETCH_SIZE"][0], "write_kib": per["WRITE_SIZE"][0],
-            "dispatches": per["FETCH_SIZE"][1], "copy_fetch_kib": cal["FETCH_SIZE"],
-            "copy_write_kib": cal["WRITE_SIZE"], "fetch_factor": f_fetch, "write_factor": f_write}, None
+    out = {"copy_fetch_kib": cal["FETCH_SIZE"], "copy_write_kib": cal["WRITE_SIZE"], "fetch_factor": f_fetch,
+           "write_factor": f_write}
+    for key, d in per.items():
+        out[key] = {"traffic_bytes": d["FETCH_SIZE"][0] * 1024.0 * f_fetch + d["WRITE_SIZE"][0] * 1024.0 * f_write,
+                    "fetch_kib": d["FETCH_SIZE"][0], "write_kib": d["WRITE_SIZE"][0],
+                    "dispatches": d["FETCH_SIZE"][1]}
+    out["traffic_bytes"] = out["spmv"]["traffic_bytes"]
+    return out, None
 
 AGG = {"hem": capi.HEM, "hec": capi.HEC, "vmb": capi.VMB}
 SMOOTHER = {"jacobi": capi.SMOOTHER_JACOBI, "l1": capi.SMOOTHER_L1DIAG, "gs": capi.SMOOTHER_GS,
@@ -641,6 +657,7 @@ def main():
     pinfo = capi.hz_solve_info()
     lib.hz_get_last_info(C.byref(pinfo))
     pcg_spmv_ms, pcg_spmv_count = pinfo.pcg_spmv_ms, pinfo.pcg_spmv_count
+    res_ms, sweep_ms = pinfo.cycle_residual_ms, pinfo.cycle_sweep_ms
     # fine-level kernels back to back (the matrix partly stays in the 256 MB
     # last-level cache between repetitions, so these read faster than in a solve)
     kt = capi.hz_kernel_times()
@@ -652,6 +669,10 @@ def main():
     # algorithmic bytes: 12 nnz (val + col) + 4 (n+1) (row pointers) + 8 n (x) + 8 n (y)
     bytes_spmv = 12 * nnz + 4 * (n + 1) + 16 * n
     bytes_jacobi = 12 * nnz + 4 * (n + 1) + 24 * n
+    # residual w = b - A x: matrix, x and b read, w written; last sweep: matrix, b (= r) and x
+    # read, x' written ((x', r) needs no other array)
+    bytes_res = 12 * nnz + 4 * (n + 1) + 24 * n
+    bytes_sweep_dot = 12 * nnz + 4 * (n + 1) + 24 * n
     spmv_gbs = bytes_spmv / (pcg_spmv_ms * 1e-3) / 1e9
     jac_gbs = bytes_jacobi / (kt.jacobi_ms * 1e-3) / 1e9 if kt.jacobi_ms > 0 else None
 
@@ -742,6 +763,19 @@ def main():
             "algorithmic_bytes_per_launch": bytes_spmv,
             "avg_launch_ms": pcg_spmv_ms,
             "launches_timed": pcg_spmv_count,
+        },
+        "fine_kernels_in_solve": {
+            "what": "fine-level kernels timed with HIP events inside one solve; algorithmic bytes per launch; "
+                    "traffic from the FETCH_SIZE / WRITE_SIZE passes",
+            "spmv": {"us": pcg_spmv_ms * 1e3, "bytes": bytes_spmv,
+                     "frac": bytes_spmv / (pcg_spmv_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if pcg_spmv_ms else None,
+                     "traffic": traffic["spmv"]["traffic_bytes"] if traffic else None},
+            "residual": {"us": res_ms * 1e3, "bytes": bytes_res,
+                         "frac": bytes_res / (res_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if res_ms else None,
+                         "traffic": traffic["residual"]["traffic_bytes"] if traffic else None},
+            "last_sweep": {"us": sweep_ms * 1e3, "bytes": bytes_sweep_dot,
+                           "frac": bytes_sweep_dot / (sweep_ms * 1e-3) / 1e9 / HBM_PEAK_GBS if sweep_ms else None,
+                           "traffic": traffic["sweep_dot"]["traffic_bytes"] if traffic else None},
         },
         "kernels": {
             "back_to_back_spmv_ms": kt.spmv_ms,

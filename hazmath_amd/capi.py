@@ -102,7 +102,8 @@ class hz_solve_info(C.Structure):
                 ("relres_hist", REAL * 4096), ("fallback_used", INT), ("pcg_spmv_ms", REAL),
                 ("pcg_spmv_count", INT), ("learned_iterations", INT), ("learned_total_ms", REAL),
                 ("hem_pipeline_giveups", INT), ("cheb_scale", REAL * MAX_AMG_LVL),
-                ("lambda_ratio", REAL * MAX_AMG_LVL), ("solve_alg_bytes", REAL)]
+                ("lambda_ratio", REAL * MAX_AMG_LVL), ("solve_alg_bytes", REAL), ("cycle_residual_ms", REAL),
+                ("cycle_sweep_ms", REAL)]
 
 
 class hz_hierarchy(C.Structure):

@@ -462,7 +462,8 @@ void prolong_postsmooth(Hierarchy &H, int l, double alpha, const double *alpha_d
         // bandwidth-bound level: gathering P xc per matrix entry costs more than one pass
         prolong_agg(L.A.n, L.agg, C.x, alpha, alpha_dev, L.x);
         if (want_dot)
-            smooth_sweep_dot(L.S, cp.smoother, cp.jacobi_w, L.b, L.x, L.xt, dot_out, H.ws);
+            smooth_sweep_dot(L.S, cp.smoother, cp.jacobi_w, L.b, L.x, L.xt, dot_out, H.ws,
+                             l == 0 ? H.sweep_timer : nullptr);
         else
             smooth_sweep(L.S, cp.smoother, cp.jacobi_w, L.b, L.x, L.xt);
     }
@@ -499,7 +500,7 @@ forward:
             down_fused(L.nagg, L.rptr, L.rmem, L.A, L.b, L.x, C.b, cp.smoother, cp.jacobi_w,
                        fuse ? C.dsm : nullptr, fuse ? C.x : nullptr);
         } else {
-            residual(L.S, L.b, L.x, L.w);
+            residual(L.S, L.b, L.x, L.w, l == 0 ? H.res_timer : nullptr);
             restrict_agg(L.nagg, L.A.n, L.rptr, L.rmem, L.w, C.b, cp.smoother, cp.jacobi_w,
                          fuse ? C.dsm : nullptr, fuse ? C.x : nullptr);
         }

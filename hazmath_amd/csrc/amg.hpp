@@ -64,6 +64,8 @@ struct Hierarchy {
     std::vector<double> cheb_scale;   // per level (default 1.0)
     std::vector<double> lambda_ratio; // learned: lambda_est(D^-1 A) / Gershgorin bound per level
     LaunchTimer *spmv_timer = nullptr;  // when set: events around the PCG's fine-level SpMV
+    LaunchTimer *res_timer = nullptr;   // ... the cycle's fine-level residual (SELL form)
+    LaunchTimer *sweep_timer = nullptr; // ... the cycle's last fine-level sweep (with (z, r))
 };
 
 // print_cputime (message.c:73-77): "<message> costs <seconds> seconds."

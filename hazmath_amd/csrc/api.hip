@@ -299,15 +299,23 @@ INT core_solve(const DevCSR &A, const double *db, double *dx, linear_itsolver_pa
             std::printf("**********************************************************\n");
             std::printf(" --> using Conjugate Gradient Method:\n");
         }
-        LaunchTimer spmv_timer;
-        if (g_opt.time_pcg_spmv) H.spmv_timer = &spmv_timer;
+        LaunchTimer spmv_timer, res_timer, sweep_timer;
+        if (g_opt.time_pcg_spmv) {
+            H.spmv_timer = &spmv_timer;
+            H.res_timer = &res_timer;
+            H.sweep_timer = &sweep_timer;
+        }
         pr = pcg_solve(H, H.lv[0].S, db, dx, it->linear_tol, it->linear_maxit, it->linear_stop_type, prt);
         if (H.spmv_timer) {
             int cnt = 0;
             const double tot = spmv_timer.total_ms(&cnt);
             g_info.pcg_spmv_count = cnt;
             g_info.pcg_spmv_ms = cnt ? tot / cnt : 0.0;
-            H.spmv_timer = nullptr;
+            const double tr = res_timer.total_ms(&cnt);
+            g_info.cycle_residual_ms = cnt ? tr / cnt : 0.0;
+            const double ts = sweep_timer.total_ms(&cnt);
+            g_info.cycle_sweep_ms = cnt ? ts / cnt : 0.0;
+            H.spmv_timer = H.res_timer = H.sweep_timer = nullptr;
         } else {
             g_info.pcg_spmv_count = 0;
             g_info.pcg_spmv_ms = 0.0;

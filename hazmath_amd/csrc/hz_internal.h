@@ -100,14 +100,14 @@ void sell_from_csr(const DevCSR &A, DevSELL &S);
 // events around the SpMV kernel itself (not the partials' finalize).
 void spmv(const DevSELL &S, const double *x, double *y, double *dot_out, const ReduceWs &ws,
           LaunchTimer *timer = nullptr);
-// w = b - A x
-void residual(const DevSELL &S, const double *b, const double *x, double *w);
+// w = b - A x (timer: events around the kernel)
+void residual(const DevSELL &S, const double *b, const double *x, double *w, LaunchTimer *timer = nullptr);
 // One smoothing sweep out of place: xout = sweep(x).  smoother: 1 Jacobi (weight w), 10 L1DIAG.
 void smooth_sweep(const DevSELL &S, int smoother, double w, const double *b, const double *x,
                   double *xout);
 // Same sweep, also writing sum_i xout_i * b_i (deterministic) to dot_out.
 void smooth_sweep_dot(const DevSELL &S, int smoother, double w, const double *b, const double *x,
-                      double *xout, double *dot_out, const ReduceWs &ws);
+                      double *xout, double *dot_out, const ReduceWs &ws, LaunchTimer *timer = nullptr);
 // First sweep from a zero iterate: x = w*b/d (Jacobi) or b/d_l1 (L1DIAG); d precomputed.
 void smooth_from_zero(int n, int smoother, double w, const double *d, const double *b, double *x);
 // Smoother diagonal: Jacobi uses the last stored diagonal entry of each row

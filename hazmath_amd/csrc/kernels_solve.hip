@@ -103,18 +103,20 @@ __global__ __launch_bounds__(kBlock) void k_sell_fill(int n, const int *__restri
 // -1 skipped).  Entries go eight at a time: their column and value loads
 // (non-temporal) are issued first, then the eight gathers, then the adds in
 // order, so a row costs a few memory round trips instead of one per entry.
-template <bool kNtTail>
+template <bool kNtTail, bool kWide = false>
 __device__ __forceinline__ double sell_row_dot(const int *c, const double *v, int w, const double *x) {
-    constexpr int U = 8;
+    // kWide: rounds of 16 masked past the slice width (a 15-entry row in one round)
+    constexpr int U = kWide ? 16 : 8;
     double acc = 0.0;
     int k = 0;
-    for (; k + U <= w; k += U) {
+    for (; kWide ? (k < w) : (k + U <= w); k += U) {
         int cc[U];
         double vv[U], xx[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            cc[u] = ld_nt(c + (k + u) * kSlice);
-            vv[u] = ld_nt(v + (k + u) * kSlice);
+            const bool ok = !kWide || k + u < w;
+            cc[u] = ok ? ld_nt(c + (k + u) * kSlice) : -1;
+            vv[u] = ok ? ld_nt(v + (k + u) * kSlice) : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) xx[u] = cc[u] >= 0 ? x[cc[u]] : 0.0;
@@ -130,7 +132,7 @@ __device__ __forceinline__ double sell_row_dot(const int *c, const double *v, in
 }
 
 // y = A x (row sums from 0.0 in CSR order); optional deterministic (y, x).
-template <bool kDot>
+template <bool kDot, bool kWide = false>
 __global__ __launch_bounds__(kBlock) void k_spmv(int n, int nslices, const long long *__restrict__ sptr,
                                                  const int *__restrict__ col,
                                                  const double *__restrict__ val,
@@ -148,7 +150,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n, int nslices, const long 
         const int w = (int)((sptr[s + 1] - base) / kSlice);
         const int *c = col + base + lane;
         const double *v = val + base + lane;
-        acc = sell_row_dot<true>(c, v, w, x);
+        acc = sell_row_dot<true, kWide>(c, v, w, x);
         if (r < n) {
             y[r] = acc;
             if (kDot) prod = acc * x[r];
@@ -161,6 +163,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n, int nslices, const long 
 }
 
 // w = b - A x  (array_cp + dcsr_aAxpy(-1.0): row sum t from 0.0, then b - t)
+template <bool kWide = false>
 __global__ __launch_bounds__(kBlock) void k_residual(int n, int nslices, const long long *__restrict__ sptr,
                                                      const int *__restrict__ col,
                                                      const double *__restrict__ val,
@@ -175,7 +178,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(int n, int nslices, const l
     const int wd = (int)((sptr[s + 1] - base) / kSlice);
     const int *c = col + base + lane;
     const double *v = val + base + lane;
-    const double acc = sell_row_dot<false>(c, v, wd, x);
+    const double acc = sell_row_dot<false, kWide>(c, v, wd, x);
     if (r < n) w[r] = b[r] - acc;
 }
 
@@ -198,7 +201,7 @@ __device__ __forceinline__ double prolong_value(double xj, int g, double xcg, do
 // L1DIAG (smoother.c:324-381): t = b - sum_j a_ij x_j, d = sum_j |a_ij|; x_i += t/d.
 // Entries are consumed four at a time with all their loads issued first, so a
 // row costs a few dependent memory round trips instead of one per entry.
-template <int kSmoother, bool kProlong, bool kDot>
+template <int kSmoother, bool kProlong, bool kDot, bool kWide = false>
 __global__ __launch_bounds__(kBlock) void k_sweep(int n, int nslices, const long long *__restrict__ sptr,
                                                   const int *__restrict__ col,
                                                   const double *__restrict__ val, double wgt,
@@ -227,7 +230,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(int n, int nslices, const long
         // order.  With the prolongation folded in (the latency-bound small
         // levels) rounds are 16 wide and masked past the slice width, so a
         // 15-entry row takes one round; otherwise 8 wide plus a tail.
-        constexpr bool kMask = kProlong;
+        constexpr bool kMask = kProlong || kWide;
         constexpr int U = kMask ? 16 : 8;
         int k = 0;
         for (; kMask ? (k < wd) : (k + U <= wd); k += U) {
@@ -236,8 +239,12 @@ __global__ __launch_bounds__(kBlock) void k_sweep(int n, int nslices, const long
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const bool ok = !kMask || k + u < wd;
-                cc[u] = ok ? ld_nt(c + (k + u) * kSlice) : -1;
-                vv[u] = ok ? ld_nt(v + (k + u) * kSlice) : 0.0;
+                // the fine level's last sweep (kDot) precedes the PCG SpMV: the
+                // second half of each round uses ordinary loads, which stay in the
+                // last-level cache for that SpMV
+                const bool keep = kWide && kDot && u >= 8;
+                cc[u] = ok ? (keep ? c[(k + u) * kSlice] : ld_nt(c + (k + u) * kSlice)) : -1;
+                vv[u] = ok ? (keep ? v[(k + u) * kSlice] : ld_nt(v + (k + u) * kSlice)) : 0.0;
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) xx[u] = cc[u] >= 0 ? x[cc[u]] : 0.0;
@@ -605,6 +612,7 @@ inline int stream_grid(long n) {
 
 double g_alg_bytes = 0.0;
 
+
 void sell_from_csr(const DevCSR &A, DevSELL &S) {
     S.n = A.n;
     S.nnz = A.nnz;
@@ -660,11 +668,14 @@ void spmv(const DevSELL &S, const double *x, double *y, double *dot_out, const R
     HZ_LAUNCH_CHECK();
 }
 
-void residual(const DevSELL &S, const double *b, const double *x, double *w) {
+void residual(const DevSELL &S, const double *b, const double *x, double *w, LaunchTimer *timer) {
     if (S.n == 0) return;
     count_bytes(matrix_bytes(S.nnz, S.n) + 24.0 * S.n);
-    k_residual<<<div_up(S.nslices, kSlicesPerBlock), kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr,
-                                                                           S.col, S.val, b, x, w);
+    // rounds of 16 masked loads: a 15-entry row in one round of loads
+    if (timer) timer->begin();
+    k_residual<true><<<div_up(S.nslices, kSlicesPerBlock), kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col,
+                                                                                 S.val, b, x, w);
+    if (timer) timer->end();
     HZ_LAUNCH_CHECK();
 }
 
@@ -675,12 +686,22 @@ void launch_sweep(const DevSELL &S, int smoother, double w, const int *agg, cons
                   double *partials) {
     count_bytes(matrix_bytes(S.nnz, S.n) + 24.0 * S.n + (kProlong ? 4.0 * S.n : 0.0));
     const int grid = div_up(S.nslices, kSlicesPerBlock);
-    if (smoother == HZ_SMOOTHER_L1DIAG)
-        k_sweep<HZ_SMOOTHER_L1DIAG, kProlong, kDot><<<grid, kBlock, 0, stream()>>>(
-            S.n, S.nslices, S.sptr, S.col, S.val, w, agg, xc, alpha, alpha_dev, b, x, xout, partials);
-    else
-        k_sweep<HZ_SMOOTHER_JACOBI, kProlong, kDot><<<grid, kBlock, 0, stream()>>>(
-            S.n, S.nslices, S.sptr, S.col, S.val, w, agg, xc, alpha, alpha_dev, b, x, xout, partials);
+    const bool wide = !kProlong;   // rounds of 16 masked loads (the prolongation form has them anyway)
+    if (smoother == HZ_SMOOTHER_L1DIAG) {
+        if (wide)
+            k_sweep<HZ_SMOOTHER_L1DIAG, kProlong, kDot, true><<<grid, kBlock, 0, stream()>>>(
+                S.n, S.nslices, S.sptr, S.col, S.val, w, agg, xc, alpha, alpha_dev, b, x, xout, partials);
+        else
+            k_sweep<HZ_SMOOTHER_L1DIAG, kProlong, kDot><<<grid, kBlock, 0, stream()>>>(
+                S.n, S.nslices, S.sptr, S.col, S.val, w, agg, xc, alpha, alpha_dev, b, x, xout, partials);
+    } else {
+        if (wide)
+            k_sweep<HZ_SMOOTHER_JACOBI, kProlong, kDot, true><<<grid, kBlock, 0, stream()>>>(
+                S.n, S.nslices, S.sptr, S.col, S.val, w, agg, xc, alpha, alpha_dev, b, x, xout, partials);
+        else
+            k_sweep<HZ_SMOOTHER_JACOBI, kProlong, kDot><<<grid, kBlock, 0, stream()>>>(
+                S.n, S.nslices, S.sptr, S.col, S.val, w, agg, xc, alpha, alpha_dev, b, x, xout, partials);
+    }
     HZ_LAUNCH_CHECK();
 }
 }  // namespace
@@ -692,9 +713,11 @@ void smooth_sweep(const DevSELL &S, int smoother, double w, const double *b, con
 }
 
 void smooth_sweep_dot(const DevSELL &S, int smoother, double w, const double *b, const double *x,
-                      double *xout, double *dot_out, const ReduceWs &ws) {
+                      double *xout, double *dot_out, const ReduceWs &ws, LaunchTimer *timer) {
     if (S.n == 0) return;
+    if (timer) timer->begin();
     launch_sweep<false, true>(S, smoother, w, nullptr, nullptr, 1.0, nullptr, b, x, xout, ws.partials);
+    if (timer) timer->end();
     k_final_sum<<<1, kFinBlock, 0, stream()>>>(ws.partials, div_up(S.nslices, kSlicesPerBlock), dot_out);
     HZ_LAUNCH_CHECK();
 }

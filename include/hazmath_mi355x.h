@@ -372,6 +372,9 @@ typedef struct {
                                 input read once, each output written once; the gathered coarse
                                 corrections of the transfers not counted): over solve_ms, the
                                 whole-solve bandwidth */
+    REAL   cycle_residual_ms;  /* with hz_options.time_pcg_spmv: mean duration of the cycle's
+                                  fine-level residual and of its last fine-level sweep (the one that */
+    REAL   cycle_sweep_ms;     /* also forms (z, r)) in this solve; 0 where not launched */
 } hz_solve_info;
 
 void hz_get_last_info(hz_solve_info *info);
