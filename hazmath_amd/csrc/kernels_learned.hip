@@ -74,24 +74,23 @@ __global__ __launch_bounds__(kBlock) void k_cheb_step(int n, int nslices, const 
     const int *c = col + base + lane;
     const double *v = val + base + lane;
     double t = (r < n) ? b[r] : 0.0;
-    int k = 0;
-    for (; k + 4 <= wd; k += 4) {
-        int cc[4];
-        double vv[4], xx[4];
+    // rounds of 16 entries masked past the slice width: loads first, then the
+    // products in stored order
+    constexpr int U = 16;
+    for (int k = 0; k < wd; k += U) {
+        int cc[U];
+        double vv[U], xx[U];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            cc[u] = c[(k + u) * kSlice];
-            vv[u] = v[(k + u) * kSlice];
+        for (int u = 0; u < U; ++u) {
+            const bool ok = k + u < wd;
+            cc[u] = ok ? c[(k + u) * kSlice] : -1;
+            vv[u] = ok ? v[(k + u) * kSlice] : 0.0;
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) xx[u] = cc[u] >= 0 ? x[cc[u]] : 0.0;
+        for (int u = 0; u < U; ++u) xx[u] = cc[u] >= 0 ? x[cc[u]] : 0.0;
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < U; ++u)
             if (cc[u] >= 0) t = t - vv[u] * xx[u];
-    }
-    for (; k < wd; ++k) {
-        const int cj = c[k * kSlice];
-        if (cj >= 0) t = t - v[k * kSlice] * x[cj];
     }
     if (r < n) {
         const double di = diag[r];

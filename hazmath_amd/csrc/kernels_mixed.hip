@@ -22,7 +22,6 @@ constexpr int kSlicesPerBlock = kBlock / kWave;
 constexpr int kFinBlock = 1024;
 constexpr int kStreamGrid = 2048;
 constexpr float kSmallRealF = 1e-20f;
-constexpr int kU = 8;   // SELL entries per round: their loads are issued before the math
 
 inline int grid_for(long n) {
     long g = (n + kBlock - 1) / kBlock;
@@ -106,10 +105,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep_m(int n, int nslices, const lo
         const MT *v = val + base + lane;
         float t = (r < n) ? b[r] : 0.0f;
         float d = 0.0f;
-        // with the prolongation folded in (latency-bound small levels): rounds of
-        // 16 masked past the slice width; otherwise kU wide plus a tail
-        constexpr bool kMask = kProlong;
-        constexpr int U = kMask ? 16 : kU;
+        // rounds of 16 entries masked past the slice width (a 15-entry row in one
+        // round): all loads of a round issued before its math
+        constexpr bool kMask = true;
+        constexpr int U = 16;
         int k = 0;
         for (; kMask ? (k < wd) : (k + U <= wd); k += U) {
             int cc[U];
@@ -200,24 +199,21 @@ __global__ __launch_bounds__(kBlock) void k_residual_m(int n, int nslices, const
     const int *c = col + base + lane;
     const MT *v = val + base + lane;
     float acc = 0.0f;
-    int k = 0;
-    for (; k + kU <= wd; k += kU) {
-        int cc[kU];
-        float vv[kU], xx[kU];
+    constexpr int U = 16;   // rounds of 16 masked past the slice width
+    for (int k = 0; k < wd; k += U) {
+        int cc[U];
+        float vv[U], xx[U];
 #pragma unroll
-        for (int u = 0; u < kU; ++u) {
-            cc[u] = ld_nt(c + (k + u) * kSlice);
-            vv[u] = ldv(v + (k + u) * kSlice);
+        for (int u = 0; u < U; ++u) {
+            const bool ok = k + u < wd;
+            cc[u] = ok ? ld_nt(c + (k + u) * kSlice) : -1;
+            vv[u] = ok ? ldv(v + (k + u) * kSlice) : 0.0f;
         }
 #pragma unroll
-        for (int u = 0; u < kU; ++u) xx[u] = cc[u] >= 0 ? x[cc[u]] : 0.0f;
+        for (int u = 0; u < U; ++u) xx[u] = cc[u] >= 0 ? x[cc[u]] : 0.0f;
 #pragma unroll
-        for (int u = 0; u < kU; ++u)
+        for (int u = 0; u < U; ++u)
             if (cc[u] >= 0) acc = acc + vv[u] * xx[u];
-    }
-    for (; k < wd; ++k) {
-        const int cj = ld_nt(c + k * kSlice);
-        if (cj >= 0) acc = acc + ldv(v + k * kSlice) * x[cj];
     }
     if (r < n) w[r] = b[r] - acc;
 }
@@ -318,24 +314,21 @@ __global__ __launch_bounds__(kBlock) void k_cheb_step_m(int n, int nslices, cons
         const int *c = col + base + lane;
         const MT *v = val + base + lane;
         float t = (r < n) ? b[r] : 0.0f;
-        int k = 0;
-        for (; k + kU <= wd; k += kU) {
-            int cc[kU];
-            float vv[kU], xx[kU];
+        constexpr int U = 16;   // rounds of 16 masked past the slice width
+        for (int k = 0; k < wd; k += U) {
+            int cc[U];
+            float vv[U], xx[U];
 #pragma unroll
-            for (int u = 0; u < kU; ++u) {
-                cc[u] = ld_nt(c + (k + u) * kSlice);
-                vv[u] = ldv(v + (k + u) * kSlice);
+            for (int u = 0; u < U; ++u) {
+                const bool ok = k + u < wd;
+                cc[u] = ok ? ld_nt(c + (k + u) * kSlice) : -1;
+                vv[u] = ok ? ldv(v + (k + u) * kSlice) : 0.0f;
             }
 #pragma unroll
-            for (int u = 0; u < kU; ++u) xx[u] = cc[u] >= 0 ? x[cc[u]] : 0.0f;
+            for (int u = 0; u < U; ++u) xx[u] = cc[u] >= 0 ? x[cc[u]] : 0.0f;
 #pragma unroll
-            for (int u = 0; u < kU; ++u)
+            for (int u = 0; u < U; ++u)
                 if (cc[u] >= 0) t = t - vv[u] * xx[u];
-        }
-        for (; k < wd; ++k) {
-            const int cj = ld_nt(c + k * kSlice);
-            if (cj >= 0) t = t - ldv(v + k * kSlice) * x[cj];
         }
         if (r < n) {
             const float di = diag[r];
