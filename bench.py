@@ -177,6 +177,8 @@ def parse():
     p.add_argument("--learned", action="store_true",
                    help="the learned-parameter evaluation: 3-D Kuhn 128^3 anisotropic (eps=1e-3, seeded theta), "
                         "learned vs plain AMG-PCG")
+    p.add_argument("--learned-kinds", default="plain,sa,learned,learned_fp32",
+                   help="methods of --learned, comma separated (all four print the full line)")
     p.add_argument("--cpu-sample-m", type=int, default=128,
                    help="cells per direction of the CPU-baseline sample")
     p.add_argument("--cpu-runs", type=int, default=5, help="CPU-baseline solves (median reported)")
@@ -512,7 +514,11 @@ def learned_comparison(a, lib, steps, warmup, kinds=("plain", "sa", "learned", "
 
 
 def main_learned(a, lib):
-    res, workload = learned_comparison(a, lib, a.steps, a.warmup, kinds=("plain", "sa", "learned", "learned_fp32"))
+    kinds = tuple(a.learned_kinds.split(","))
+    res, workload = learned_comparison(a, lib, a.steps, a.warmup, kinds=kinds)
+    if len(kinds) < 4:
+        print(json.dumps({"partial_learned_comparison": res}))
+        return
     line = {
         "metric": "learned vs plain AMG-PCG time-to-1e-10 (s), 3-D Kuhn P1 anisotropic eps=1e-3",
         "value": res["learned_fp32"]["total_ms"] * 1e-3,
