@@ -25,6 +25,7 @@ LIB_PATH = PKG_DIR / "lib" / "libhazmath_mi355x.so"
 # Error codes (include/macro.h:38-75)
 SUCCESS = 0
 ERROR_INPUT_PAR = -13
+ERROR_ALLOC_MEM = -20
 ERROR_AMG_SMOOTH_TYPE = -31
 ERROR_AMG_AGG_TYPE = -32
 ERROR_AMG_COARSEING = -33

@@ -134,9 +134,11 @@ int hem_level(const DevCSR &N, const AMG_param *param, int *agg, int *nagg) {
     if (g_hem_host_max == -1) st = hem_aggregate_gpu(N, agg, nagg);
     else if (g_hem_host_max == -2) st = 1;
     else if (g_hem_host_max == -3) st = aggregate_hem_staged(N, agg, nagg);
-    else if (n > hem_host_max_rows()) st = hem_aggregate_gpu(N, agg, nagg);
+    else if (n > hem_host_max_rows()) st = hem_aggregate_gpu(N, agg, nagg, true);
     else if (n > kHemFullHostRows) st = aggregate_hem_staged(N, agg, nagg);
     else st = 1;
+    // the pipeline met chains longer than its cap: the host sweep over preference lists
+    if (st == 2) st = aggregate_hem_staged(N, agg, nagg);
     // the full host sweep: small levels, patterns the other forms cannot
     // express, and the reference's message for a failed pass 2
     if (st == 1 || st == -1017) st = aggregate_host(N, HZ_HEM, param->max_aggregation, agg, nagg);
@@ -273,6 +275,12 @@ int amg_setup(Hierarchy &H, const DevCSR &A0, AMG_param *param, bool) {
     H.lv[lvl].theta = param->strong_coupled;
     Level &Cst = H.lv[lvl];
     if (param->coarse_solver == HZ_SOLVER_UMFPACK) {
+        // the dense inverse holds n^2 values: refused beyond kDenseCoarsestMaxRows
+        if (Cst.A.n > kDenseCoarsestMaxRows) {
+            std::fprintf(stderr, "hazmath_mi355x: the coarsest level has %d rows; the dense coarsest solve takes at "
+                         "most %d (raise max_levels or lower coarse_dof)\n", Cst.A.n, kDenseCoarsestMaxRows);
+            return HZ_ERROR_ALLOC_MEM;
+        }
         // amg_setup_ua.c:1390-1401: the coarsest matrix is replaced by its sorted transpose
         DevCSR T;
         csr_transpose(Cst.A, T);

@@ -105,6 +105,11 @@ AmgSolveResult amg_solve(Hierarchy &H, const double *b, double *x, double tol, i
 // (||x_K||_A / ||x_{K/2}||_A)^(1/(K-K/2)) of the cycle iterated on A x = 0 from a random x_0.
 double error_contraction(Hierarchy &H, int iters, unsigned long long seed);
 
+// The dense coarsest solve (coarse_solver SOLVER_UMFPACK) inverts the level
+// explicitly; larger coarsest levels are refused with HZ_ERROR_ALLOC_MEM
+// (32768^2 doubles = 8 GiB, about a second of FP64 MFMA work).
+constexpr int kDenseCoarsestMaxRows = 32768;
+
 // Host-staged sequential aggregation for VMB (and HEM on non-symmetric patterns).
 int aggregate_host(const DevCSR &N, int type, int max_aggregation, int *agg_dev, int *nagg);
 
@@ -114,8 +119,12 @@ extern int g_hem_host_max;
 int hem_host_max_rows();
 
 // HEM on the GPU: SUCCESS, -1017, or 1 when the strength pattern is not
-// symmetric (then the host sweep is used).
-int hem_aggregate_gpu(const DevCSR &N, int *agg, int *nagg_out);
+// symmetric (then the host sweep is used).  With host_on_giveup (and no sweep
+// cap set through hz_options), a pass 1 whose dependency chains need more
+// than kPipeProbeSweeps sweeps of one chunk stops early and returns 2: the
+// caller then runs the host sweep, which is linear in n whatever the chains
+// (a negative hz_options.hem_pipeline_max_sweeps sets that cap).
+int hem_aggregate_gpu(const DevCSR &N, int *agg, int *nagg_out, bool host_on_giveup = false);
 
 // HEM aggregation of one level in the form its size calls for (full host
 // sweep, preference lists, or the GPU pipeline; hz_options.hem_host_max_rows):

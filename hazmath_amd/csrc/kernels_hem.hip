@@ -313,6 +313,12 @@ __global__ __launch_bounds__(kBlock) void k_hem_round(int n, const int *__restri
 // kPipeMaxSweeps sweeps raises *status and every workgroup stops (the host
 // then finishes with k_hem_round); the kernel therefore always terminates.
 constexpr int kPipeMaxSweeps = 8192;
+// The cap of the default form, which hands a level with long dependency chains
+// to the host sweep instead.  Measured on 128^3 Kuhn levels (3-D): the Poisson
+// level 0 needs fewer than 16 sweeps a chunk, the anisotropic one (eps = 1e-3)
+// 513-1024, where the whole pipeline takes 17 ms against about 7 ms for the
+// host sweep over preference lists.
+constexpr int kPipeProbeSweeps = 128;
 
 __global__ __launch_bounds__(kBlock) void k_hem_pipeline(int n, const int *__restrict__ ia,
                                                          const int *__restrict__ ja,
@@ -504,7 +510,7 @@ static int hem_pass2(const DevCSR &N, int *agg, int *size, int nc, const int *li
 
 // Returns SUCCESS, -1017 (reference exit(17)), or 1 when the strength pattern
 // is not symmetric (caller falls back to the host sweep).
-int hem_aggregate_gpu(const DevCSR &N, int *agg, int *nagg_out) {
+int hem_aggregate_gpu(const DevCSR &N, int *agg, int *nagg_out, bool host_on_giveup) {
     hipStream_t st = stream();
     const int n = N.n;
     const int g = grid_for(n);
@@ -533,7 +539,14 @@ int hem_aggregate_gpu(const DevCSR &N, int *agg, int *nagg_out) {
     const int gr = std::max(1, std::min(8192, nchunks));
     int *pipe = arena().alloc_n<int>(2);   // ticket, status
     HZ_CHECK(hipMemsetAsync(pipe, 0, 2 * sizeof(int), st));
-    const int max_sweeps = g_hem_pipe_max_sweeps > 0 ? g_hem_pipe_max_sweeps : kPipeMaxSweeps;
+    // hz_options.hem_pipeline_max_sweeps: > 0 a cap finished by the round
+    // kernel, < 0 a cap that hands the level to the host sweep, 0 the default
+    bool probe = host_on_giveup;
+    int max_sweeps = probe ? kPipeProbeSweeps : kPipeMaxSweeps;
+    if (g_hem_pipe_max_sweeps != 0) {
+        probe = g_hem_pipe_max_sweeps < 0;
+        max_sweeps = probe ? -g_hem_pipe_max_sweeps : g_hem_pipe_max_sweeps;
+    }
     k_hem_pipeline<<<std::min(nchunks, 2048), kBlock, 0, st>>>(n, N.ia, N.ja, pref, npref, fstate, pick,
                                                               linkpar, pipe, pipe + 1, max_sweeps);
     HZ_LAUNCH_CHECK();
@@ -542,6 +555,12 @@ int hem_aggregate_gpu(const DevCSR &N, int *agg, int *nagg_out) {
     const int kRoundsPerCheck = 2;
     int rounds1 = 0, rounds2 = 0;
     if (pipe_status != 0) ++g_hem_pipe_giveups;
+    if (pipe_status != 0 && probe) {
+        if (trace_enabled())
+            std::fprintf(stderr, "[hz]   HEM n=%d: pipeline reached %d sweeps in a chunk, host sweep instead\n", n,
+                         max_sweeps);
+        return 2;
+    }
     for (int round = 0; pipe_status != 0; round += kRoundsPerCheck) {
         if (round + kRoundsPerCheck > kCounterSlots) {
             HZ_CHECK(hipMemsetAsync(undecided, 0, sizeof(int) * kCounterSlots, st));

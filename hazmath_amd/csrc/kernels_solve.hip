@@ -103,7 +103,9 @@ __global__ __launch_bounds__(kBlock) void k_sell_fill(int n, const int *__restri
 // -1 skipped).  Entries go eight at a time: their column and value loads
 // (non-temporal) are issued first, then the eight gathers, then the adds in
 // order, so a row costs a few memory round trips instead of one per entry.
-template <bool kNtTail, bool kWide = false>
+// kKeep: the second half of each round uses ordinary loads, whose lines stay
+// in the last-level cache for the next kernel that streams the same matrix.
+template <bool kNtTail, bool kWide = false, bool kKeep = false>
 __device__ __forceinline__ double sell_row_dot(const int *c, const double *v, int w, const double *x) {
     // kWide: rounds of 16 masked past the slice width (a 15-entry row in one round)
     constexpr int U = kWide ? 16 : 8;
@@ -115,8 +117,9 @@ __device__ __forceinline__ double sell_row_dot(const int *c, const double *v, in
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const bool ok = !kWide || k + u < w;
-            cc[u] = ok ? ld_nt(c + (k + u) * kSlice) : -1;
-            vv[u] = ok ? ld_nt(v + (k + u) * kSlice) : 0.0;
+            const bool keep = kKeep && u >= U / 2;
+            cc[u] = ok ? (keep ? c[(k + u) * kSlice] : ld_nt(c + (k + u) * kSlice)) : -1;
+            vv[u] = ok ? (keep ? v[(k + u) * kSlice] : ld_nt(v + (k + u) * kSlice)) : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) xx[u] = cc[u] >= 0 ? x[cc[u]] : 0.0;
@@ -132,7 +135,7 @@ __device__ __forceinline__ double sell_row_dot(const int *c, const double *v, in
 }
 
 // y = A x (row sums from 0.0 in CSR order); optional deterministic (y, x).
-template <bool kDot, bool kWide = false>
+template <bool kDot, bool kWide = false, bool kKeep = false>
 __global__ __launch_bounds__(kBlock) void k_spmv(int n, int nslices, const long long *__restrict__ sptr,
                                                  const int *__restrict__ col,
                                                  const double *__restrict__ val,
@@ -150,7 +153,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n, int nslices, const long 
         const int w = (int)((sptr[s + 1] - base) / kSlice);
         const int *c = col + base + lane;
         const double *v = val + base + lane;
-        acc = sell_row_dot<true, kWide>(c, v, w, x);
+        acc = sell_row_dot<true, kWide, kKeep>(c, v, w, x);
         if (r < n) {
             y[r] = acc;
             if (kDot) prod = acc * x[r];
@@ -656,8 +659,10 @@ void spmv(const DevSELL &S, const double *x, double *y, double *dot_out, const R
     const int grid = div_up(S.nslices, kSlicesPerBlock);
     if (dot_out) {
         if (timer) timer->begin();
-        k_spmv<true><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col, S.val, x, y,
-                                                    ws.partials);
+        // the PCG's SpMV: rounds of 16, the second half of each with ordinary
+        // loads that stay in the last-level cache for the cycle's residual
+        k_spmv<true, true, true><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col, S.val, x, y,
+                                                                ws.partials);
         if (timer) timer->end();
         HZ_LAUNCH_CHECK();
         k_final_sum<<<1, kFinBlock, 0, stream()>>>(ws.partials, grid, dot_out);

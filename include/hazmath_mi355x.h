@@ -334,9 +334,13 @@ typedef struct {
                               there), -1: every row one thread per row, -2: the default with a
                               16-column limit */
     INT  hem_pipeline_max_sweeps; /* HEM's one-launch GPU pipeline gives up on a chunk of vertices
-                              after this many sweeps (checked every 16) and the remaining pass-1
-                              decisions run as one launch per round; both give the same
-                              aggregates.  0: 8192 */
+                              after this many sweeps (checked every 16).  > 0: the remaining
+                              pass-1 decisions run as one launch per round; < 0: after -value
+                              sweeps the level goes to the host sweep over preference lists.
+                              All forms give the same aggregates.  0: levels the pipeline takes
+                              by size go to the host sweep after 128 sweeps (long dependency
+                              chains, e.g. strongly anisotropic problems); a pipeline forced on
+                              every level (hem_host_max_rows -1) finishes by rounds after 8192 */
 } hz_options;
 
 void hz_get_options(hz_options *opt);
@@ -362,8 +366,9 @@ typedef struct {
     INT    learned_iterations;  /* hz_krylov_amg_learned: the learned attempt's result (iterations or
                                    error code) and its device time, also when it fell back */
     REAL   learned_total_ms;
-    INT    hem_pipeline_giveups;   /* HEM levels whose one-launch GPU pipeline handed its remaining
-                                      decisions to the per-round kernel (hem_pipeline_max_sweeps) */
+    INT    hem_pipeline_giveups;   /* HEM levels whose one-launch GPU pipeline stopped at its sweep
+                                      cap and handed the rest to the per-round kernel or the host
+                                      sweep (hem_pipeline_max_sweeps) */
     REAL   cheb_scale[HZ_MAX_AMG_LVL];   /* learned SA: Chebyshev bound scale in force per level
                                             (after the floor), 0 on levels without one */
     REAL   lambda_ratio[HZ_MAX_AMG_LVL]; /* learned SA: estimated lambda_max(D^-1 A) over the

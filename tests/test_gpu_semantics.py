@@ -22,7 +22,7 @@ import pytest
 
 import oracle
 from hazmath_amd import capi, problems
-from test_gpu_parity import _HemPath, _hierarchies
+from test_gpu_parity import _hierarchies
 
 pytestmark = pytest.mark.gpu
 
@@ -284,27 +284,33 @@ def test_one_entry_rows(lib, orc, libc, agg):
     assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) < 1e-10
 
 
-def test_hem_pipeline_give_up_path_bitexact(lib, orc, libc):
-    """hem_pipeline_max_sweeps = 1: every chunk of the one-launch pipeline
+@pytest.mark.parametrize("cap,host_max", [(1, -1), (-1, -1), (-1, 2000)])
+def test_hem_pipeline_give_up_path_bitexact(lib, orc, libc, cap, host_max):
+    """hem_pipeline_max_sweeps = +-1: every chunk of the one-launch pipeline
     that is not decided within its first 16 sweeps raises the give-up flag
-    (the kernel's own bounded wait), the workgroups stop, and the per-round
-    kernel finishes pass 1.  The aggregates stay the reference's."""
+    (the kernel's own bounded wait) and the workgroups stop.  With cap 1 the
+    per-round kernel finishes pass 1; with cap -1 the level goes to the host
+    sweep over preference lists -- on every level (hem_host_max_rows -1) or
+    on the levels the size rule gives the pipeline (hem_host_max_rows 2000).
+    The aggregates stay the reference's."""
     ia, ja, va = problems.poisson3d_kuhn(24)
     A = capi.HostCSR(ia, ja, va)
     opts = capi.hz_options()
     lib.hz_get_options(C.byref(opts))
-    old = opts.hem_pipeline_max_sweeps
-    opts.hem_pipeline_max_sweeps = 1
+    old = (opts.hem_pipeline_max_sweeps, opts.hem_host_max_rows)
+    opts.hem_pipeline_max_sweeps = cap
+    opts.hem_host_max_rows = host_max
     lib.hz_set_options(C.byref(opts))
     try:
-        with _HemPath(lib, "gpu"):
-            (g, thg, _), (o, tho, _) = _hierarchies(lib, orc, libc, A, capi.HEM, 0.04)
-            info = capi.hz_solve_info()
-            lib.hz_get_last_info(C.byref(info))
+        (g, thg, _), (o, tho, _) = _hierarchies(lib, orc, libc, A, capi.HEM, 0.04)
+        info = capi.hz_solve_info()
+        lib.hz_get_last_info(C.byref(info))
     finally:
         lib.hz_get_options(C.byref(opts))
-        opts.hem_pipeline_max_sweeps = old
+        opts.hem_pipeline_max_sweeps, opts.hem_host_max_rows = old
         lib.hz_set_options(C.byref(opts))
+    print(f"[semantics] cap {cap}, hem_host_max_rows {host_max}: {info.hem_pipeline_giveups} levels gave up",
+          flush=True)
     assert info.hem_pipeline_giveups >= 1
     assert len(g) == len(o)
     for gl, ol in zip(g, o):
@@ -313,3 +319,22 @@ def test_hem_pipeline_give_up_path_bitexact(lib, orc, libc):
         for key in ("ia", "ja", "val"):
             assert np.array_equal(gl[key], ol[key]), key
     assert thg == tho
+
+
+def test_dense_coarsest_size_limit(lib, libc, capfd):
+    """A coarsest level beyond the dense solve's 32,768 rows (here max_levels
+    2 on a 3-D problem: level 1 has about 55,000 rows) is refused with
+    ERROR_ALLOC_MEM and a message, instead of a device allocation failure."""
+    ia, ja, va = problems.poisson3d_kuhn(48)
+    n = ia.size - 1
+    A = capi.HostCSR(ia, ja, va)
+    b = capi.HostVec(problems.uniform_rhs(n, seed=3))
+    x = capi.HostVec(np.zeros(n))
+    it, amg = capi.solver_params(lib, tol=1e-8, maxit=50, aggregation=capi.HEM, max_levels=2)
+    libc.srand(1)
+    r = lib.linear_solver_dcsr_krylov_amg(A.ptr(), b.ptr(), x.ptr(), C.byref(it), C.byref(amg))
+    err = capfd.readouterr().err
+    print(f"[semantics] max_levels 2, dense coarsest: {r}; {err.strip()}", flush=True)
+    assert r == capi.ERROR_ALLOC_MEM
+    assert "the dense coarsest solve takes at most 32768" in err
+    assert not np.any(x.arr)
