@@ -305,8 +305,13 @@ def main_batched(a, lib):
         if r < 0:
             raise RuntimeError(f"batched solver returned {r}")
 
-    for w in range(a.warmup):
+    tune_first = None
+    for w in range(max(1, a.warmup)):
         step()
+        if tune_first is None:
+            st = capi.hz_batch_stats()
+            lib.hz_batch_get_stats(C.byref(st))
+            tune_first = (st.tuned, st.tune_ms)
         print(f"[bench] warmup {w}", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     stats = []
@@ -412,9 +417,12 @@ def main_batched(a, lib):
             "avg_launch_ms": ks.spmm_ms,
         },
         "kernels": {"spmm_ms": ks.spmm_ms, "sweep_f32_ms": ks.sweep_ms, "sweep_f32_gbs": sweep_gbs,
-                    "lanes_per_row_sweep_level0": ks.shape_level0 // 10,
-                    "lanes_per_row_spmm_level0": ks.shape_level0 % 10, "levels": ks.num_levels,
+                    "shape_sweep_level0": ks.sweep_shape0, "shape_spmm_level0": ks.spmm_shape0,
+                    "shape_code": "100 x lanes per row + entries per load round", "levels": ks.num_levels,
                     "op_complexity": ks.op_complexity},
+        "autotuner": {"first_call_timed_decisions": tune_first[0], "first_call_tune_ms": tune_first[1],
+                      "timed_call_cached_decisions": ks.tune_cached, "timed_call_timed_decisions": ks.tuned,
+                      "cache_file": os.environ.get("HZ_TUNE_CACHE", "default (~/.cache/hazmath_mi355x/tune.txt)")},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

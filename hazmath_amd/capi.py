@@ -92,7 +92,9 @@ class hz_options(C.Structure):
 class hz_batch_stats(C.Structure):
     _fields_ = [("setup_ms", REAL), ("solve_ms", REAL), ("op_complexity", REAL), ("num_levels", INT),
                 ("graph_launches", INT), ("shape_level0", INT), ("spmm_ms", REAL), ("sweep_ms", REAL),
-                ("rows", C.c_longlong), ("nnz", C.c_longlong), ("sell_entries", C.c_longlong)]
+                ("rows", C.c_longlong), ("nnz", C.c_longlong), ("sell_entries", C.c_longlong),
+                ("sweep_shape0", INT), ("spmm_shape0", INT), ("tuned", INT), ("tune_cached", INT),
+                ("tune_ms", REAL)]
 
 
 class hz_solve_info(C.Structure):
@@ -349,6 +351,8 @@ def _declare(lib):
     lib.hz_batch_krylov_amg.restype = INT
     lib.hz_batch_get_stats.argtypes = [P(hz_batch_stats)]
     lib.hz_batch_get_stats.restype = None
+    lib.hz_batch_tune_cache.argtypes = [C.c_char_p]
+    lib.hz_batch_tune_cache.restype = INT
     lib.hz_batch_krylov_amg_device.argtypes = [INT, P(INT), INT, C.c_void_p, C.c_void_p, C.c_void_p,
                                                C.c_void_p, C.c_void_p, P(linear_itsolver_param), P(AMG_param),
                                                P(INT)]

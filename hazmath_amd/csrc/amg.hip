@@ -702,7 +702,32 @@ bool amg_precond_m(Hierarchy &H, const double *r, double *z, double *zr_out) {
     return zr_out != nullptr;
 }
 
-bool amg_precond(Hierarchy &H, const double *r, double *z, double *zr_out) {
+namespace {
+// Level-0 ping-pong of the preconditioner: the buffer its iterate starts in,
+// chosen so that the last sweep of level 0 writes into z.  (Gauss-Seidel-type
+// sweeps work in place: no swaps.)
+double *precond_x0(Hierarchy &H, double *z) {
+    const CycleParams &cp = H.cp;
+    const int swaps = is_gs_family(cp.smoother)
+                          ? 0
+                          : (cp.presmooth > 0 ? cp.presmooth - 1 : 0) + cp.postsmooth +
+                                (cp.maxit > 1 ? (cp.maxit - 1) * (cp.presmooth + cp.postsmooth) : 0);
+    return swaps % 2 == 1 ? H.x0buf : z;
+}
+}  // namespace
+
+bool amg_precond_first_sweep(Hierarchy &H, double *z, FirstSweep &out) {
+    const CycleParams &cp = H.cp;
+    if (H.cp.precision > 0 || H.nl < 2 || cp.maxit < 1 || cp.presmooth < 1) return false;
+    if (cp.smoother != HZ_SMOOTHER_JACOBI && cp.smoother != HZ_SMOOTHER_L1DIAG) return false;
+    out.smoother = cp.smoother;
+    out.w = cp.jacobi_w;
+    out.d = H.lv[0].dsm;
+    out.x0 = precond_x0(H, z);
+    return true;
+}
+
+bool amg_precond(Hierarchy &H, const double *r, double *z, double *zr_out, bool first_done) {
     if (H.cp.precision > 0 && H.nl > 1) return amg_precond_m(H, r, z, zr_out);
     Level &L0 = H.lv[0];
     const CycleParams &cp = H.cp;
@@ -719,26 +744,16 @@ bool amg_precond(Hierarchy &H, const double *r, double *z, double *zr_out) {
         if (cp.maxit <= 0) fill(z, 0.0, n);
         return false;
     }
-    // Arrange the ping-pong so that the last sweep of level 0 writes into z.
-    // (Gauss-Seidel-type sweeps work in place: no swaps)
-    const int swaps = is_gs_family(cp.smoother)
-                          ? 0
-                          : (cp.presmooth > 0 ? cp.presmooth - 1 : 0) + cp.postsmooth +
-                                (cp.maxit > 1 ? (cp.maxit - 1) * (cp.presmooth + cp.postsmooth) : 0);
-    double *scratch = H.x0buf;
-    if (swaps % 2 == 1) {
-        L0.x = scratch;
-        L0.xt = z;
-    } else {
-        L0.x = z;
-        L0.xt = scratch;
-    }
+    // the ping-pong is arranged so that the last sweep of level 0 writes into z
+    L0.x = precond_x0(H, z);
+    L0.xt = L0.x == z ? H.x0buf : z;
     CycleState cs;
     for (int i = 0; i < HZ_MAX_AMG_LVL; ++i) {
         cs.zero[i] = false;
         cs.first_done[i] = false;
     }
     cs.zero[0] = true;
+    cs.first_done[0] = first_done;
     bool zr_done = false;
     for (int it = 0; it < cp.maxit; ++it)
         mgcycle(H, cs, it == cp.maxit - 1 ? zr_out : nullptr, &zr_done);
@@ -862,9 +877,13 @@ PcgResult pcg_solve(Hierarchy &H, const DevSELL &A, const double *b, double *u, 
     PcgScalars hs{};
     double *sc_rr = &H.sc->rr, *sc_zr = &H.sc->zr;
 
-    auto precond_zr = [&](const double *rr_, double *zz_) {
-        if (!amg_precond(H, rr_, zz_, sc_zr)) dot(m, zz_, rr_, sc_zr, H.ws);
+    auto precond_zr = [&](const double *rr_, double *zz_, bool first_done = false) {
+        if (!amg_precond(H, rr_, zz_, sc_zr, first_done)) dot(m, zz_, rr_, sc_zr, H.ws);
     };
+    // the PCG update forms the preconditioner's first level-0 sweep when the
+    // preconditioner is queued right behind it
+    FirstSweep first;
+    const bool fuse_first = stop != HZ_STOP_REL_PRECRES && amg_precond_first_sweep(H, z, first);
     auto measure = [&](double &absr) -> double {
         // r = b - A u; then the stop-type residual
         residual(A, b, u, r);
@@ -920,10 +939,10 @@ PcgResult pcg_solve(Hierarchy &H, const DevSELL &A, const double *b, double *u, 
     // r); the rare restart / convergence paths below redo or drop that work.
     while (iter++ < MaxIt) {
         spmv(A, p, t, &H.sc->temp2, H.ws, H.spmv_timer);
-        pcg_update(m, H.sc, p, t, u, r, alpha_hist, iter, H.ws);
+        pcg_update(m, H.sc, p, t, u, r, alpha_hist, iter, H.ws, fuse_first ? &first : nullptr);
         fetch_async(H);
         if (speculate) {
-            precond_zr(r, z);
+            precond_zr(r, z, fuse_first);
             pcg_direction(m, H.sc, z, p, beta_hist, iter);
         }
         fetch_wait(hs);

@@ -77,8 +77,14 @@ void print_cputime(const char *message, double seconds);
 int amg_setup(Hierarchy &H, const DevCSR &A0, AMG_param *param, bool keep_for_export);
 void amg_prepare_cycle(Hierarchy &H, const AMG_param *param);
 // z = B(r): cp.maxit cycles from a zero iterate.  Writes (z, r) to zr_out and
-// returns true when the last fine sweep produced it.
-bool amg_precond(Hierarchy &H, const double *r, double *z, double *zr_out);
+// returns true when the last fine sweep produced it.  first_done: the first
+// level-0 sweep from zero is already in amg_precond_first_sweep(H, z)->x0.
+bool amg_precond(Hierarchy &H, const double *r, double *z, double *zr_out, bool first_done = false);
+// Where and how the preconditioner's first level-0 sweep from zero writes, so
+// the PCG update can form it; false when the cycle does not start with such a
+// sweep (FP32 cycle, one level, Chebyshev or Gauss-Seidel-type smoothers, no
+// pre-smoothing).
+bool amg_precond_first_sweep(Hierarchy &H, double *z, FirstSweep &out);
 
 struct PcgResult {
     int iter = 0;

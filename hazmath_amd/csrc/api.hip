@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "amg.hpp"
+#include "tune_cache.h"
 
 using namespace hz;
 
@@ -241,6 +242,11 @@ INT batch_core(const DevCSR &A, const std::vector<int> &off, const double *db, d
     g_batch_stats.rows = stats.rows;
     g_batch_stats.nnz = stats.nnz;
     g_batch_stats.sell_entries = stats.sell_entries;
+    g_batch_stats.sweep_shape0 = stats.sweep_shape0;
+    g_batch_stats.spmm_shape0 = stats.spmm_shape0;
+    g_batch_stats.tuned = stats.tuned;
+    g_batch_stats.tune_cached = stats.tune_cached;
+    g_batch_stats.tune_ms = stats.tune_ms;
     if (iters) std::copy(out.begin(), out.end(), iters);
     return HZ_SUCCESS;
 }
@@ -1323,6 +1329,8 @@ INT hz_batch_krylov_amg(INT nsys, dCSRmat *const *A, INT nrhs, REAL *const *b, R
 void hz_batch_get_stats(hz_batch_stats *out) {
     if (out) *out = g_batch_stats;
 }
+
+INT hz_batch_tune_cache(const char *path) { return TuneCache::get().reset(path, false); }
 
 INT hz_gs_smooth(dCSRmat *A, const REAL *b, REAL *x, INT smoother, INT post, INT nsweeps, REAL relax, INT *order,
                  INT *cptr, INT *ncolors) {

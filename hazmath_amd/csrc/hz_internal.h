@@ -132,8 +132,17 @@ void prolong_agg(int n, const int *agg, const double *xc, double alpha, const do
 void fill(double *x, double v, long n);
 void copy(double *dst, const double *src, long n);
 // PCG fused vector kernels (scalars live on the device)
+// The preconditioner's first level-0 sweep from a zero iterate, x0 = sweep(0)
+// on the updated residual, when pcg_update is to form it (smooth_from_zero's
+// arithmetic).
+struct FirstSweep {
+    int smoother = 0;
+    double w = 1.0;
+    const double *d = nullptr;
+    double *x0 = nullptr;
+};
 void pcg_update(int n, PcgScalars *s, const double *p, const double *t, double *u, double *r,
-                double *alpha_hist, int iter, const ReduceWs &ws);
+                double *alpha_hist, int iter, const ReduceWs &ws, const FirstSweep *first = nullptr);
 void dot(int n, const double *x, const double *y, double *out, const ReduceWs &ws);
 void norm2sq(int n, const double *x, double *out, const ReduceWs &ws);
 // beta = zr/temp1; temp1 = zr; p = z + beta p ; beta_hist[iter-1] = beta
@@ -318,6 +327,9 @@ struct BatchStats {
     bool time_kernels = false;   // also time the level-0 SpMM and FP32 sweep
     double spmm_ms = 0, sweep_ms = 0;
     long long rows = 0, nnz = 0, sell_entries = 0;
+    int sweep_shape0 = 0, spmm_shape0 = 0;   // level-0 kernel shape codes
+    int tuned = 0, tune_cached = 0;          // tuner decisions timed / taken from the cache
+    double tune_ms = 0;                      // host wall time of the timed decisions
 };
 // A0: block-diagonal stack of the systems (device CSR, global column indices),
 // off0: nsys + 1 row offsets; b, u: [n][kBatchRhs] FP64 on the device (u: initial

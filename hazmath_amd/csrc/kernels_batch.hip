@@ -41,9 +41,11 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "amg.hpp"
+#include "tune_cache.h"
 
 namespace hz {
 
@@ -169,9 +171,13 @@ __device__ __forceinline__ void ld_vec(const float *p, float *v) {
         const float4 b = *reinterpret_cast<const float4 *>(p + 4);
         v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
         v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
-    } else {
+    } else if constexpr (kN == 4) {
         const float4 a = *reinterpret_cast<const float4 *>(p);
         v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    } else {
+        static_assert(kN == 2, "2, 4 or 8 right-hand sides per lane");
+        const float2 a = *reinterpret_cast<const float2 *>(p);
+        v[0] = a.x; v[1] = a.y;
     }
 }
 
@@ -180,8 +186,10 @@ __device__ __forceinline__ void st_vec(float *p, const float *v) {
     if constexpr (kN == 8) {
         *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
         *reinterpret_cast<float4 *>(p + 4) = make_float4(v[4], v[5], v[6], v[7]);
-    } else {
+    } else if constexpr (kN == 4) {
         *reinterpret_cast<float4 *>(p) = make_float4(v[0], v[1], v[2], v[3]);
+    } else {
+        *reinterpret_cast<float2 *>(p) = make_float2(v[0], v[1]);
     }
 }
 
@@ -207,7 +215,7 @@ struct LaneMap {
 
 // One sweep xout = sweep(x) for live rows (Jacobi weight w / L1DIAG as in
 // the FP32 single-system cycle); dead rows copy x.
-template <int kSm, int kH>
+template <int kSm, int kH, int kU>
 __global__ __launch_bounds__(kBlock) void k_bsweep(int n, int nslices, const long long *__restrict__ sptr,
                                                    const int *__restrict__ col, const float *__restrict__ val,
                                                    float wgt, const unsigned char *__restrict__ live,
@@ -233,7 +241,7 @@ __global__ __launch_bounds__(kBlock) void k_bsweep(int n, int nslices, const lon
     const int lane = threadIdx.x % kWave;
     const int *c = col + base + lane;
     const float *v = val + base + lane;
-    // entries in row order; the loads of 4 entries are issued before their use
+    // entries in row order; the loads of kU entries are issued before their use
     auto step = [&](int cj, float a, const float *xj) {
         if (cj < 0) return;
         if (kSm == HZ_SMOOTHER_L1DIAG) {
@@ -248,20 +256,20 @@ __global__ __launch_bounds__(kBlock) void k_bsweep(int n, int nslices, const lon
         }
     };
     int k = 0;
-    for (; k + 4 <= wd; k += 4) {
-        int cc[4];
-        float aa[4], xx[4][kN];
+    for (; k + kU <= wd; k += kU) {
+        int cc[kU];
+        float aa[kU], xx[kU][kN];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kU; ++u) {
             cc[u] = ld_nt(c + (long long)(k + u) * kSlice);
             aa[u] = ld_nt(v + (long long)(k + u) * kSlice);
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kU; ++u) {
             if (cc[u] >= 0) ld_vec<kN>(x + (size_t)cc[u] * kR + m.q0, xx[u]);
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) step(cc[u], aa[u], xx[u]);
+        for (int u = 0; u < kU; ++u) step(cc[u], aa[u], xx[u]);
     }
     for (; k < wd; ++k) {
         const int cj = ld_nt(c + (long long)k * kSlice);
@@ -282,7 +290,7 @@ __global__ __launch_bounds__(kBlock) void k_bsweep(int n, int nslices, const lon
 }
 
 // w = b - A x for live rows
-template <int kH>
+template <int kH, int kU>
 __global__ __launch_bounds__(kBlock) void k_bresidual(int n, int nslices, const long long *__restrict__ sptr,
                                                       const int *__restrict__ col, const float *__restrict__ val,
                                                       const unsigned char *__restrict__ live,
@@ -303,20 +311,20 @@ __global__ __launch_bounds__(kBlock) void k_bresidual(int n, int nslices, const 
     const int *c = col + base + lane;
     const float *v = val + base + lane;
     int k = 0;
-    for (; k + 4 <= wd; k += 4) {
-        int cc[4];
-        float aa[4], xx[4][kN];
+    for (; k + kU <= wd; k += kU) {
+        int cc[kU];
+        float aa[kU], xx[kU][kN];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kU; ++u) {
             cc[u] = ld_nt(c + (long long)(k + u) * kSlice);
             aa[u] = ld_nt(v + (long long)(k + u) * kSlice);
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kU; ++u) {
             if (cc[u] >= 0) ld_vec<kN>(x + (size_t)cc[u] * kR + m.q0, xx[u]);
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kU; ++u) {
             if (cc[u] < 0) continue;
 #pragma unroll
             for (int q = 0; q < kN; ++q) acc[q] = acc[q] + aa[u] * xx[u][q];
@@ -503,7 +511,7 @@ __global__ void k_exit_b(int n, const float *__restrict__ x, double *__restrict_
 // ---------------------------------------------------------------------------
 // y = A x (kSub = 1: y = A x; kSub = 2: y = b - A x); two lanes per row
 // (four right-hand sides each)
-template <int kSub, int kH>
+template <int kSub, int kH, int kU>
 __global__ __launch_bounds__(kBlock) void k_spmm64(int n, int nslices, const long long *__restrict__ sptr,
                                                    const int *__restrict__ col, const double *__restrict__ val,
                                                    const double *__restrict__ x, const double *__restrict__ b,
@@ -531,17 +539,17 @@ __global__ __launch_bounds__(kBlock) void k_spmm64(int n, int nslices, const lon
         }
     };
     int k = 0;
-    for (; k + 4 <= wd; k += 4) {
-        int cc[4];
-        double aa[4];
-        double2 xx[4][kV];
+    for (; k + kU <= wd; k += kU) {
+        int cc[kU];
+        double aa[kU];
+        double2 xx[kU][kV];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kU; ++u) {
             cc[u] = ld_nt(c + (long long)(k + u) * kSlice);
             aa[u] = ld_nt(v + (long long)(k + u) * kSlice);
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < kU; ++u) {
             if (cc[u] >= 0) {
                 const double2 *xp = reinterpret_cast<const double2 *>(x + (size_t)cc[u] * kR + m.q0);
 #pragma unroll
@@ -549,7 +557,7 @@ __global__ __launch_bounds__(kBlock) void k_spmm64(int n, int nslices, const lon
             }
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
+        for (int u = 0; u < kU; ++u)
             if (cc[u] >= 0) madd(aa[u], xx[u]);
     }
     for (; k < wd; ++k) {
@@ -808,8 +816,8 @@ struct BatchLevel {
     long long *ctab = nullptr;       // (row offset, size, inverse offset, system) per such system
     double *Ainv = nullptr;
     float *b = nullptr, *x = nullptr, *xt = nullptr, *w = nullptr;
-    int shape = 1;                   // lanes per row chosen by the tuner (1 or 2)
-    int spmm_shape = 2;              // level 0: lanes per row of the FP64 PCG SpMM (tuned)
+    int shape = 104;                 // FP32 cycle kernels' shape chosen by the tuner (shape code)
+    int spmm_shape = 204;            // level 0: shape of the FP64 PCG SpMM (tuned)
 };
 
 struct BatchHierarchy {
@@ -823,37 +831,57 @@ struct BatchHierarchy {
 
 namespace {
 
-template <int kH>
-void launch_bsweep(const BatchLevel &L, int sm, float w, const float *b, const float *x, float *xo,
-                   const int *act) {
-    const int waves = L.S.nslices * kH;
-    const int grid = div_up(waves, kSlicesPerBlock);
-    if (sm == HZ_SMOOTHER_L1DIAG)
-        k_bsweep<HZ_SMOOTHER_L1DIAG, kH><<<grid, kBlock, 0, stream()>>>(L.S.n, L.S.nslices, L.S.sptr, L.S.col,
-                                                                       L.valf, w, L.live, b, x, xo, L.rsys, act);
-    else
-        k_bsweep<HZ_SMOOTHER_JACOBI, kH><<<grid, kBlock, 0, stream()>>>(L.S.n, L.S.nslices, L.S.sptr, L.S.col,
-                                                                       L.valf, w, L.live, b, x, xo, L.rsys, act);
+// Kernel shapes: code = 100 x (lanes per row, each holding 8 / lanes of the
+// right-hand sides) + (entries whose loads are issued together per round).
+constexpr int kShapes[] = {104, 108, 204, 208, 404, 408};
+
+template <int V>
+using IC = std::integral_constant<int, V>;
+
+// f(IC<lanes>, IC<round>) for a shape code
+template <class F>
+void with_shape(int code, F &&f) {
+    switch (code) {
+        case 104: f(IC<1>{}, IC<4>{}); break;
+        case 108: f(IC<1>{}, IC<8>{}); break;
+        case 204: f(IC<2>{}, IC<4>{}); break;
+        case 208: f(IC<2>{}, IC<8>{}); break;
+        case 404: f(IC<4>{}, IC<4>{}); break;
+        case 408: f(IC<4>{}, IC<8>{}); break;
+        default: throw DeviceError{hipErrorInvalidValue, "batched AMG: unknown kernel shape"};
+    }
+}
+
+bool valid_shape(int code) {
+    for (int c : kShapes)
+        if (c == code) return true;
+    return false;
 }
 
 void bsweep(const BatchLevel &L, int sm, float w, const float *b, const float *x, float *xo,
             const int *act = nullptr) {
     if (L.S.n == 0) return;
-    if (L.shape == 2) launch_bsweep<2>(L, sm, w, b, x, xo, act);
-    else launch_bsweep<1>(L, sm, w, b, x, xo, act);
+    with_shape(L.shape, [&](auto H, auto U) {
+        constexpr int kH = decltype(H)::value, kU = decltype(U)::value;
+        const int grid = div_up(L.S.nslices * kH, kSlicesPerBlock);
+        if (sm == HZ_SMOOTHER_L1DIAG)
+            k_bsweep<HZ_SMOOTHER_L1DIAG, kH, kU><<<grid, kBlock, 0, stream()>>>(
+                L.S.n, L.S.nslices, L.S.sptr, L.S.col, L.valf, w, L.live, b, x, xo, L.rsys, act);
+        else
+            k_bsweep<HZ_SMOOTHER_JACOBI, kH, kU><<<grid, kBlock, 0, stream()>>>(
+                L.S.n, L.S.nslices, L.S.sptr, L.S.col, L.valf, w, L.live, b, x, xo, L.rsys, act);
+    });
     HZ_LAUNCH_CHECK();
 }
 
 void bresidual(const BatchLevel &L, const int *act) {
     if (L.S.n == 0) return;
-    const int waves = L.S.nslices * L.shape;
-    const int grid = div_up(waves, kSlicesPerBlock);
-    if (L.shape == 2)
-        k_bresidual<2><<<grid, kBlock, 0, stream()>>>(L.S.n, L.S.nslices, L.S.sptr, L.S.col, L.valf, L.live, L.b,
-                                                       L.x, L.w, L.rsys, act);
-    else
-        k_bresidual<1><<<grid, kBlock, 0, stream()>>>(L.S.n, L.S.nslices, L.S.sptr, L.S.col, L.valf, L.live, L.b,
-                                                       L.x, L.w, L.rsys, act);
+    with_shape(L.shape, [&](auto H, auto U) {
+        constexpr int kH = decltype(H)::value, kU = decltype(U)::value;
+        const int grid = div_up(L.S.nslices * kH, kSlicesPerBlock);
+        k_bresidual<kH, kU><<<grid, kBlock, 0, stream()>>>(L.S.n, L.S.nslices, L.S.sptr, L.S.col, L.valf, L.live,
+                                                          L.b, L.x, L.w, L.rsys, act);
+    });
     HZ_LAUNCH_CHECK();
 }
 
@@ -875,33 +903,67 @@ float time_ms(F fn, int reps) {
     return ms / reps;
 }
 
-// Shape per level: the tuner's choice cached by (rows, entries per row) class
-std::map<std::pair<int, int>, int> &shape_cache() {
-    static std::map<std::pair<int, int>, int> c;
-    return c;
+// The tuner's bookkeeping of the current batched call (reported in hz_batch_stats).
+struct TuneStats {
+    int timed = 0, cached = 0;
+    double ms = 0.0;
+};
+TuneStats g_tune;
+
+// Cache key of a decision: device, kernel, and the matrix's class (rounded
+// log2 of the rows, 4 x the mean entries per row).
+std::string tune_key(const char *kernel, const BatchLevel &L) {
+    static std::string dev;
+    if (dev.empty()) {
+        hipDeviceProp_t prop{};
+        int d = 0;
+        HZ_CHECK(hipGetDevice(&d));
+        HZ_CHECK(hipGetDeviceProperties(&prop, d));
+        std::string arch = prop.gcnArchName;
+        arch = arch.substr(0, arch.find(':'));
+        dev = arch + "-" + std::to_string(prop.multiProcessorCount) + "cu";
+    }
+    const int nclass = (int)std::lround(std::log2(std::max(1, L.S.n)));
+    const int wclass = L.S.n ? (int)std::lround(4.0 * (double)L.A.nnz / L.S.n) : 0;
+    return dev + "/" + kernel + "/" + std::to_string(nclass) + "/" + std::to_string(wclass);
+}
+
+// The shape for one kernel on one matrix: an override from the environment,
+// else the cached decision, else every shape timed with HIP events (3 runs
+// after one warm-up) and the fastest kept, in memory and in the cache file.
+// All shapes add every row's entries in stored order, so they give
+// bit-identical results.
+template <class Run>
+int tune(const char *kernel, const char *env_name, const BatchLevel &L, int &field, Run run) {
+    if (const char *env = std::getenv(env_name)) {
+        const int forced = std::atoi(env);
+        if (valid_shape(forced)) return forced;
+    }
+    const std::string key = tune_key(kernel, L);
+    const int hit = TuneCache::get().lookup(key);
+    if (valid_shape(hit)) {
+        ++g_tune.cached;
+        return hit;
+    }
+    const double t0 = wall_ms();
+    int best = kShapes[0];
+    float tbest = 1e30f;
+    for (int code : kShapes) {
+        field = code;
+        const float t = time_ms(run, 3);
+        if (t < tbest) {
+            tbest = t;
+            best = code;
+        }
+    }
+    ++g_tune.timed;
+    g_tune.ms += wall_ms() - t0;
+    TuneCache::get().store(key, best);
+    return best;
 }
 
 int tune_shape(BatchLevel &L, int sm, float w) {
-    const int nclass = (int)std::lround(std::log2(std::max(1, L.S.n)));
-    const int wclass = L.S.n ? (int)std::lround(4.0 * (double)L.A.nnz / L.S.n) : 0;
-    const auto key = std::make_pair(nclass, wclass);
-    auto &cache = shape_cache();
-    auto it = cache.find(key);
-    if (it != cache.end()) return it->second;
-    const char *env = std::getenv("HZ_BATCH_SHAPE");
-    if (env && (env[0] == '1' || env[0] == '2')) return env[0] - '0';
-    int best = 1;
-    float tbest = 1e30f;
-    for (int shape : {1, 2}) {
-        L.shape = shape;
-        const float t = time_ms([&] { bsweep(L, sm, w, L.b, L.x, L.xt); }, 3);
-        if (t < tbest) {
-            tbest = t;
-            best = shape;
-        }
-    }
-    cache[key] = best;
-    return best;
+    return tune("bsweep", "HZ_BATCH_SHAPE", L, L.shape, [&] { bsweep(L, sm, w, L.b, L.x, L.xt); });
 }
 
 }  // namespace
@@ -1185,51 +1247,28 @@ void pcg_alloc(BatchPcg &P, const BatchHierarchy &H, const std::vector<int> &off
     P.ctl = arena().alloc_n<PcgCtl>(1);
 }
 
-template <int kSub, int kH>
-void launch_spmm(const BatchLevel &L0, const double *x, const double *b, double *y, const int *gate,
-                 const int *act) {
-    const int grid = std::max(1, div_up(L0.S.nslices * kH, kSlicesPerBlock));
-    k_spmm64<kSub, kH><<<grid, kBlock, 0, stream()>>>(L0.S.n, L0.S.nslices, L0.S.sptr, L0.S.col, L0.S.val, x, b,
-                                                      y, gate, L0.rsys, act);
-}
-
 // y = A x (sub = 0) or y = b - A x (sub = 1) on level 0 in FP64, in the
-// lanes-per-row shape the tuner chose for this matrix
+// shape the tuner chose for this matrix
 void spmm(const BatchLevel &L0, const double *x, const double *b, double *y, int sub, const int *gate,
           const int *act) {
-    if (L0.spmm_shape == 1) {
-        if (sub) launch_spmm<2, 1>(L0, x, b, y, gate, act);
-        else launch_spmm<1, 1>(L0, x, b, y, gate, act);
-    } else {
-        if (sub) launch_spmm<2, 2>(L0, x, b, y, gate, act);
-        else launch_spmm<1, 2>(L0, x, b, y, gate, act);
-    }
+    with_shape(L0.spmm_shape, [&](auto H, auto U) {
+        constexpr int kH = decltype(H)::value, kU = decltype(U)::value;
+        const int grid = std::max(1, div_up(L0.S.nslices * kH, kSlicesPerBlock));
+        if (sub)
+            k_spmm64<2, kH, kU><<<grid, kBlock, 0, stream()>>>(L0.S.n, L0.S.nslices, L0.S.sptr, L0.S.col, L0.S.val,
+                                                              x, b, y, gate, L0.rsys, act);
+        else
+            k_spmm64<1, kH, kU><<<grid, kBlock, 0, stream()>>>(L0.S.n, L0.S.nslices, L0.S.sptr, L0.S.col, L0.S.val,
+                                                              x, b, y, gate, L0.rsys, act);
+    });
     HZ_LAUNCH_CHECK();
 }
 
-// The tuner's choice of the FP64 SpMM shape for this level-0 matrix: both
-// shapes timed (HIP events) on x = p, cached per (rows, entries per row) class
-// like the cycle kernels' (HZ_BATCH_SPMM_SHAPE overrides).
+// The tuner's choice of the FP64 SpMM shape for this level-0 matrix, timed on
+// y = A x (HZ_BATCH_SPMM_SHAPE overrides).
 int tune_spmm_shape(BatchLevel &L0, const double *x, double *y) {
-    const char *env = std::getenv("HZ_BATCH_SPMM_SHAPE");
-    if (env && (env[0] == '1' || env[0] == '2')) return env[0] - '0';
-    static std::map<std::pair<int, int>, int> cache;
-    const auto key = std::make_pair((int)std::lround(std::log2(std::max(1, L0.S.n))),
-                                    L0.S.n ? (int)std::lround(4.0 * (double)L0.A.nnz / L0.S.n) : 0);
-    auto it = cache.find(key);
-    if (it != cache.end()) return it->second;
-    int best = 2;
-    float tbest = 1e30f;
-    for (int shape : {1, 2}) {
-        L0.spmm_shape = shape;
-        const float t = time_ms([&] { spmm(L0, x, nullptr, y, 0, nullptr, nullptr); }, 3);
-        if (t < tbest) {
-            tbest = t;
-            best = shape;
-        }
-    }
-    cache[key] = best;
-    return best;
+    return tune("spmm64", "HZ_BATCH_SPMM_SHAPE", L0, L0.spmm_shape,
+                [&] { spmm(L0, x, nullptr, y, 0, nullptr, nullptr); });
 }
 
 // one PCG iteration (graph body)
@@ -1262,6 +1301,7 @@ int batch_solve(const DevCSR &A0, const std::vector<int> &off0, const double *b,
                 const linear_itsolver_param *itp, AMG_param *amg, double jacobi_w, int *iters, BatchStats *stats) {
     hipStream_t st = stream();
     const double t0 = wall_ms();
+    g_tune = TuneStats{};
     BatchHierarchy H;
     int status = batch_setup(H, A0, off0, amg, jacobi_w);
     if (status < 0) return status;
@@ -1331,7 +1371,12 @@ int batch_solve(const DevCSR &A0, const std::vector<int> &off0, const double *b,
         long long nnz = 0;
         for (const auto &L : H.lv) nnz += L.A.nnz;
         stats->op_complexity = L0.A.nnz ? (double)nnz / L0.A.nnz : 0.0;
-        stats->shape_level0 = L0.shape * 10 + L0.spmm_shape;
+        stats->shape_level0 = (L0.shape / 100) * 10 + L0.spmm_shape / 100;
+        stats->sweep_shape0 = L0.shape;
+        stats->spmm_shape0 = L0.spmm_shape;
+        stats->tuned = g_tune.timed;
+        stats->tune_cached = g_tune.cached;
+        stats->tune_ms = g_tune.ms;
     }
     return HZ_SUCCESS;
 }

@@ -21,6 +21,7 @@ namespace {
 constexpr int kBlock = 256;               // 4 wavefronts
 constexpr int kSlicesPerBlock = kBlock / kWave;
 constexpr int kStreamGrid = 2048;         // fixed grid of grid-stride vector kernels
+constexpr double kCacheResidentBytes = 192e6;   // three quarters of the last-level cache
 
 __global__ void k_slice_width(int n, const int *__restrict__ ia, long long *__restrict__ wcount,
                               int nslices) {
@@ -103,9 +104,10 @@ __global__ __launch_bounds__(kBlock) void k_sell_fill(int n, const int *__restri
 // -1 skipped).  Entries go eight at a time: their column and value loads
 // (non-temporal) are issued first, then the eight gathers, then the adds in
 // order, so a row costs a few memory round trips instead of one per entry.
-// kKeep: the second half of each round uses ordinary loads, whose lines stay
-// in the last-level cache for the next kernel that streams the same matrix.
-template <bool kNtTail, bool kWide = false, bool kKeep = false>
+// kKeep 1: the second half of each round uses ordinary loads, whose lines
+// stay in the last-level cache for the next kernel that streams the same
+// matrix; kKeep 2: every load is ordinary.
+template <bool kNtTail, bool kWide = false, int kKeep = 0>
 __device__ __forceinline__ double sell_row_dot(const int *c, const double *v, int w, const double *x) {
     // kWide: rounds of 16 masked past the slice width (a 15-entry row in one round)
     constexpr int U = kWide ? 16 : 8;
@@ -117,7 +119,7 @@ __device__ __forceinline__ double sell_row_dot(const int *c, const double *v, in
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const bool ok = !kWide || k + u < w;
-            const bool keep = kKeep && u >= U / 2;
+            const bool keep = kKeep == 2 || (kKeep == 1 && u >= U / 2);
             cc[u] = ok ? (keep ? c[(k + u) * kSlice] : ld_nt(c + (k + u) * kSlice)) : -1;
             vv[u] = ok ? (keep ? v[(k + u) * kSlice] : ld_nt(v + (k + u) * kSlice)) : 0.0;
         }
@@ -135,7 +137,7 @@ __device__ __forceinline__ double sell_row_dot(const int *c, const double *v, in
 }
 
 // y = A x (row sums from 0.0 in CSR order); optional deterministic (y, x).
-template <bool kDot, bool kWide = false, bool kKeep = false>
+template <bool kDot, bool kWide = false, int kKeep = 0>
 __global__ __launch_bounds__(kBlock) void k_spmv(int n, int nslices, const long long *__restrict__ sptr,
                                                  const int *__restrict__ col,
                                                  const double *__restrict__ val,
@@ -166,7 +168,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n, int nslices, const long 
 }
 
 // w = b - A x  (array_cp + dcsr_aAxpy(-1.0): row sum t from 0.0, then b - t)
-template <bool kWide = false>
+template <bool kWide = false, int kKeep = 0>
 __global__ __launch_bounds__(kBlock) void k_residual(int n, int nslices, const long long *__restrict__ sptr,
                                                      const int *__restrict__ col,
                                                      const double *__restrict__ val,
@@ -181,7 +183,7 @@ __global__ __launch_bounds__(kBlock) void k_residual(int n, int nslices, const l
     const int wd = (int)((sptr[s + 1] - base) / kSlice);
     const int *c = col + base + lane;
     const double *v = val + base + lane;
-    const double acc = sell_row_dot<false, kWide>(c, v, wd, x);
+    const double acc = sell_row_dot<false, kWide, kKeep>(c, v, wd, x);
     if (r < n) w[r] = b[r] - acc;
 }
 
@@ -338,17 +340,20 @@ __global__ void k_smoother_diag(int n, int nslices, const long long *__restrict_
 
 // First sweep from x = 0: t = b exactly (up to the sign of zero), so
 // Jacobi gives (1-w)*0 + w*b/d and L1DIAG gives 0 + b/d.
+// One sweep from a zero iterate at one row (Jacobi / L1DIAG with x = 0).
+__device__ __forceinline__ double sweep_from_zero(double bi, double di, int smoother, double wgt) {
+    double xi = 0.0;
+    if (fabs(di) > kSmallReal) {
+        if (smoother == HZ_SMOOTHER_L1DIAG) xi = 0.0 + bi / di;
+        else xi = (1 - wgt) * 0.0 + wgt * bi / di;
+    }
+    return xi;
+}
+
 __global__ void k_smooth_from_zero(int n, int smoother, double wgt, const double *__restrict__ d,
                                    const double *__restrict__ b, double *__restrict__ x) {
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const double di = d[i];
-        double xi = 0.0;
-        if (fabs(di) > kSmallReal) {
-            if (smoother == HZ_SMOOTHER_L1DIAG) xi = 0.0 + b[i] / di;
-            else xi = (1 - wgt) * 0.0 + wgt * b[i] / di;
-        }
-        x[i] = xi;
-    }
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        x[i] = sweep_from_zero(b[i], d[i], smoother, wgt);
 }
 
 // bc[c] = sum over members (ascending) of w, from 0.0 (dcsr_mxv_agg, sparse.c:1663-1679)
@@ -467,12 +472,17 @@ __global__ void k_copy(double *__restrict__ d, const double *__restrict__ s, lon
 
 // u += alpha p ; r -= alpha t ; then per-block partials of ||r||^2, ||u||_inf,
 // ||u||^2, ||p||^2 (finalized by k_final_update).  alpha = zr / temp2.
+// kFirst: also x0 = the preconditioner's first level-0 sweep from zero on the
+// new r (the V-cycle then starts after it).
+template <bool kFirst>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, const PcgScalars *__restrict__ s,
                                                        const double *__restrict__ p,
                                                        const double *__restrict__ t,
                                                        double *__restrict__ u,
                                                        double *__restrict__ r,
-                                                       double *__restrict__ partials) {
+                                                       double *__restrict__ partials, int smoother,
+                                                       double wgt, const double *__restrict__ d,
+                                                       double *__restrict__ x0) {
     __shared__ double sh[kBlock / kWave];
     const double alpha = s->zr / s->temp2;
     const double malpha = -alpha;
@@ -490,6 +500,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int n, const PcgScalars *
         else ri = ri + malpha * ti;
         u[i] = ui_;
         r[i] = ri;
+        if (kFirst) x0[i] = sweep_from_zero(ri, d[i], smoother, wgt);
         rr += ri * ri;
         ui = fmax(ui, fabs(ui_));
         uu += ui_ * ui_;
@@ -661,7 +672,7 @@ void spmv(const DevSELL &S, const double *x, double *y, double *dot_out, const R
         if (timer) timer->begin();
         // the PCG's SpMV: rounds of 16, the second half of each with ordinary
         // loads that stay in the last-level cache for the cycle's residual
-        k_spmv<true, true, true><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col, S.val, x, y,
+        k_spmv<true, true, 1><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col, S.val, x, y,
                                                                 ws.partials);
         if (timer) timer->end();
         HZ_LAUNCH_CHECK();
@@ -678,8 +689,15 @@ void residual(const DevSELL &S, const double *b, const double *x, double *w, Lau
     count_bytes(matrix_bytes(S.nnz, S.n) + 24.0 * S.n);
     // rounds of 16 masked loads: a 15-entry row in one round of loads
     if (timer) timer->begin();
-    k_residual<true><<<div_up(S.nslices, kSlicesPerBlock), kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col,
-                                                                                 S.val, b, x, w);
+    // a matrix that fits the 256 MB last-level cache with room to spare is
+    // loaded with ordinary loads: the same level's post-sweep, after the
+    // coarser levels' work, still finds part of it there
+    if (matrix_bytes(S.nnz, S.n) <= kCacheResidentBytes)
+        k_residual<true, 2><<<div_up(S.nslices, kSlicesPerBlock), kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr,
+                                                                                         S.col, S.val, b, x, w);
+    else
+        k_residual<true><<<div_up(S.nslices, kSlicesPerBlock), kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr,
+                                                                                     S.col, S.val, b, x, w);
     if (timer) timer->end();
     HZ_LAUNCH_CHECK();
 }
@@ -806,10 +824,16 @@ void copy(double *dst, const double *src, long n) {
 }
 
 void pcg_update(int n, PcgScalars *s, const double *p, const double *t, double *u, double *r,
-                double *alpha_hist, int iter, const ReduceWs &ws) {
+                double *alpha_hist, int iter, const ReduceWs &ws, const FirstSweep *first) {
     const int G = stream_grid(n);
-    count_bytes(48.0 * n);
-    k_pcg_update<<<G, kBlock, 0, stream()>>>(n, s, p, t, u, r, ws.partials);
+    if (first) {
+        count_bytes(64.0 * n);
+        k_pcg_update<true><<<G, kBlock, 0, stream()>>>(n, s, p, t, u, r, ws.partials, first->smoother, first->w,
+                                                       first->d, first->x0);
+    } else {
+        count_bytes(48.0 * n);
+        k_pcg_update<false><<<G, kBlock, 0, stream()>>>(n, s, p, t, u, r, ws.partials, 0, 1.0, nullptr, nullptr);
+    }
     HZ_LAUNCH_CHECK();
     k_final_update<<<1, kFinBlock, 0, stream()>>>(s, ws.partials, G, alpha_hist, iter);
     HZ_LAUNCH_CHECK();

@@ -555,8 +555,23 @@ typedef struct hz_batch_stats {
                                                        sweep) + (lanes per row of the FP64 SpMM) */
     double spmm_ms, sweep_ms;      /* level-0 FP64 SpMM and FP32 sweep, when timed */
     long long rows, nnz, sell_entries;
+    INT sweep_shape0, spmm_shape0; /* level-0 kernel shapes: 100 x lanes per row + entries per
+                                      load round (104, 108, 204, 208, 404 or 408) */
+    INT tuned, tune_cached;        /* autotuner decisions timed in this call / read from its cache */
+    double tune_ms;                /* host wall time of the timed decisions */
 } hz_batch_stats;
 void hz_batch_get_stats(hz_batch_stats *out);
+/* The batched kernels' autotuner (no counterpart in the reference) times the
+ * six shapes of each kernel on each level's matrix class (device, kernel,
+ * rounded log2 rows, mean row length) once and keeps the fastest in a cache
+ * that persists in a text file: this call's path, else the HZ_TUNE_CACHE
+ * environment variable ("" or "off": memory only), else
+ * ${XDG_CACHE_HOME:-$HOME/.cache}/hazmath_mi355x/tune.txt.  The call drops
+ * the decisions in memory, reads those of the file at path (NULL: memory
+ * only from now on) and returns how many it read.  HZ_BATCH_SHAPE and
+ * HZ_BATCH_SPMM_SHAPE force a shape code.  Every shape gives bit-identical
+ * results. */
+INT hz_batch_tune_cache(const char *path);
 INT hz_batch_krylov_amg(INT nsys, dCSRmat *const *A, INT nrhs, REAL *const *b, REAL *const *x,
                         linear_itsolver_param *itparam, AMG_param *amgparam, INT *iters);
 INT hz_batch_krylov_amg_device(INT nsys, const INT *row_offsets, INT nnz, const INT *dIA, const INT *dJA,

@@ -1,7 +1,8 @@
 """Runs the sanitizer build (TEST INFRASTRUCTURE; started by
 tests/test_sanitized_host.py with the ASan runtime preloaded): the oracle's
 solver paths, failure paths, the library's host VMB / HEM sweeps against the
-oracle's aggregates, and input.dat parsing plus the raw-pointer wrapper.
+oracle's aggregates, input.dat parsing plus the raw-pointer wrapper, and the
+batched autotuner's cache file.
 Prints "SANITIZED OK" at the end; any sanitizer report aborts before that."""
 import ctypes as C
 import os
@@ -28,6 +29,9 @@ LIB.param_linear_solver_set.argtypes = [P(capi.linear_itsolver_param), P(capi.in
 LIB.param_amg_set.argtypes = [P(capi.AMG_param), P(capi.input_param)]
 LIB.python_wrapper_krylov_amg.argtypes = [P(capi.INT), P(capi.INT), P(capi.INT), P(capi.INT), P(capi.REAL),
                                           P(capi.REAL), P(capi.REAL), P(capi.REAL), P(capi.INT), P(capi.INT)]
+LIB.san_tune_reset.argtypes = [C.c_char_p]
+LIB.san_tune_lookup.argtypes = [C.c_char_p]
+LIB.san_tune_store.argtypes = [C.c_char_p, C.c_int]
 LIBC = C.CDLL(None)
 LIBC.free.argtypes = [C.c_void_p]
 
@@ -127,6 +131,23 @@ def main():
             os.chdir(old)
         import scipy.sparse as sp
         assert np.linalg.norm(rhs - sp.csr_matrix((va, ja, ia)) @ u) < 1e-9 * np.linalg.norm(rhs)
+    # the autotuner's cache file: malformed lines skipped, decisions appended
+    # (directories created), read back by a fresh reset
+    with tempfile.TemporaryDirectory() as td:
+        f = Path(td, "seed.txt")
+        f.write_text("gfx950-256cu/bsweep/14/60 208\n# comment 1\nbad\ngfx950-256cu/spmm64/14/60 404 x\n"
+                     "gfx950-256cu/spmm64/9/56 -3\n" + "k" * 5000 + " 104\ngfx950-256cu/spmm64/14/60 104\n")
+        assert LIB.san_tune_reset(str(f).encode()) == 3
+        assert LIB.san_tune_lookup(b"gfx950-256cu/bsweep/14/60") == 208
+        assert LIB.san_tune_lookup(b"gfx950-256cu/spmm64/14/60") == 104
+        assert LIB.san_tune_lookup(b"gfx950-256cu/spmm64/9/56") == -1
+        g = Path(td, "a", "b", "tune.txt")
+        assert LIB.san_tune_reset(str(g).encode()) == 0
+        LIB.san_tune_store(b"dev/bresidual/12/58", 408)
+        LIB.san_tune_store(b"dev/bresidual/12/58", 204)
+        assert LIB.san_tune_reset(str(g).encode()) == 2
+        assert LIB.san_tune_lookup(b"dev/bresidual/12/58") == 204
+        assert LIB.san_tune_reset(None) == 0 and LIB.san_tune_lookup(b"dev/bresidual/12/58") == -1
     print("SANITIZED OK", flush=True)
 
 

@@ -7,6 +7,7 @@
 
 #include "aggregate_sweeps.h"
 #include "hz_oracle.h"
+#include "tune_cache.h"
 
 namespace hz {
 int fatal_status(int code) { return code; }
@@ -33,5 +34,9 @@ int san_hem(int n, const int *ia, const int *ja, const double *a, int *agg, int 
     if (st == 0) std::copy(v.begin(), v.end(), agg);
     return st;
 }
+
+int san_tune_reset(const char *path) { return hz::TuneCache::get().reset(path, false); }
+int san_tune_lookup(const char *key) { return hz::TuneCache::get().lookup(key); }
+void san_tune_store(const char *key, int shape) { hz::TuneCache::get().store(key, shape); }
 
 }  // extern "C"

@@ -1242,3 +1242,65 @@ def test_batch_device_entry_matches_host_entry_and_repeats(lib):
     for k in range(len(systems)):
         assert np.array_equal(x[off[k]:off[k + 1]].T, X[k])
     assert np.array_equal(outs[0][1].reshape(len(systems), 8), iters)
+
+
+_SHAPES = (104, 108, 204, 208, 404, 408)
+
+
+def test_batch_every_kernel_shape_gives_identical_bits(lib, monkeypatch):
+    """Each shape the autotuner chooses from (1, 2 or 4 lanes per row, loads
+    issued 4 or 8 entries at a time), forced on the FP32 cycle kernels and the
+    FP64 SpMM alike, adds every row's entries in stored order: the same
+    iterations and solution bits."""
+    systems, B = _batch_case(tiny=False)
+    ref = None
+    for code in _SHAPES:
+        monkeypatch.setenv("HZ_BATCH_SHAPE", str(code))
+        monkeypatch.setenv("HZ_BATCH_SPMM_SHAPE", str(code))
+        X = [np.zeros_like(b) for b in B]
+        it, amg = _batch_params(lib)
+        r, iters = capi.batch_solve(lib, systems, B, X, it, amg)
+        st = capi.hz_batch_stats()
+        lib.hz_batch_get_stats(C.byref(st))
+        print(f"[batch] shape {code}: max iterations {iters.max()}, shapes {st.sweep_shape0} / {st.spmm_shape0}",
+              flush=True)
+        assert r == 0 and st.sweep_shape0 == code and st.spmm_shape0 == code
+        if ref is None:
+            ref = (iters.copy(), [x.copy() for x in X])
+        else:
+            assert np.array_equal(iters, ref[0])
+            assert all(np.array_equal(a, b) for a, b in zip(X, ref[1]))
+
+
+def test_batch_tuner_cache_file_round_trip(lib, tmp_path):
+    """The tuner times every shape once per (device, kernel, matrix class),
+    appends its choices to the cache file, and a fresh process state read from
+    that file (hz_batch_tune_cache) takes them without timing anything and
+    reproduces the solution bits."""
+    path = tmp_path / "cache" / "tune.txt"
+    systems, B = _batch_case(tiny=False)
+    runs = []
+    try:
+        assert lib.hz_batch_tune_cache(str(path).encode()) == 0
+        for rep in range(2):
+            X = [np.zeros_like(b) for b in B]
+            it, amg = _batch_params(lib)
+            r, iters = capi.batch_solve(lib, systems, B, X, it, amg)
+            st = capi.hz_batch_stats()
+            lib.hz_batch_get_stats(C.byref(st))
+            print(f"[batch] cache pass {rep}: timed {st.tuned} ({st.tune_ms:.1f} ms), cached {st.tune_cached}, "
+                  f"level-0 shapes {st.sweep_shape0} / {st.spmm_shape0}", flush=True)
+            assert r == 0
+            runs.append((st.tuned, st.tune_cached, iters.copy(), X))
+            if rep == 0:
+                lines = path.read_text().splitlines()
+                assert len(lines) == st.tuned > 0
+                for ln in lines:
+                    key, code = ln.split()
+                    assert key.count("/") == 3 and int(code) in _SHAPES
+                assert lib.hz_batch_tune_cache(str(path).encode()) == len(lines)
+    finally:
+        lib.hz_batch_tune_cache(None)
+    assert runs[1][0] == 0 and runs[1][1] == runs[0][0] + runs[0][1]
+    assert np.array_equal(runs[0][2], runs[1][2])
+    assert all(np.array_equal(a, b) for a, b in zip(runs[0][3], runs[1][3]))
