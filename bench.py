@@ -29,7 +29,7 @@ from hazmath_amd import capi, problems  # noqa: E402
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E, 8 TB/s
 COPY_CAL_DOUBLES = 8 << 20
 CAL_COPIES = 3
-SPMV_KERNEL = "k_spmv<true>"
+SPMV_KERNEL = "k_spmv<true, true, 1>"
 
 
 def host_cpu_model():
@@ -767,7 +767,7 @@ def main():
         "spmv_gbs": spmv_gbs,
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_spmv<true>: SELL-64 SpMV t=Ap with (t,p) partials, fine level, in the PCG",
+            "kernel": "k_spmv<true, true, 1>: SELL-64 SpMV t=Ap with (t,p) partials, fine level, in the PCG",
             "achieved": spmv_gbs,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -351,6 +351,8 @@ def _declare(lib):
     lib.hz_batch_krylov_amg.restype = INT
     lib.hz_batch_get_stats.argtypes = [P(hz_batch_stats)]
     lib.hz_batch_get_stats.restype = None
+    lib.hz_coarse_itsolver.argtypes = [P(dCSRmat), P(REAL), P(REAL), REAL, P(INT), P(REAL)]
+    lib.hz_coarse_itsolver.restype = INT
     lib.hz_batch_tune_cache.argtypes = [C.c_char_p]
     lib.hz_batch_tune_cache.restype = INT
     lib.hz_batch_krylov_amg_device.argtypes = [INT, P(INT), INT, C.c_void_p, C.c_void_p, C.c_void_p,

@@ -290,7 +290,7 @@ int amg_setup(Hierarchy &H, const DevCSR &A0, AMG_param *param, bool) {
         if (coarse_dense_setup(Cst.A, Cst.Ainv) != 0)
             std::fprintf(stderr, "hazmath_mi355x: zero pivot in the coarsest-level Gauss-Jordan inverse\n");
     } else {
-        Cst.cgwork = arena().alloc_n<double>((size_t)Cst.A.n * 4 + 4);
+        Cst.cgwork = arena().alloc_n<double>(coarse_itsolver_work(Cst.A.n));
     }
     // amg_setup_ua.c:1426-1431 / 1668-1673: print_amg_complexity (message.c:105-136)
     // and the setup time

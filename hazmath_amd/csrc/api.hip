@@ -1242,6 +1242,32 @@ INT hz_cheby_smooth(dCSRmat *A, const REAL *b, REAL *x, INT steps, REAL scale) {
     }
 }
 
+INT hz_coarse_itsolver(dCSRmat *A, const REAL *b, REAL *x, REAL ctol, INT *cg_status, REAL *x_cg) {
+    if (A->row != A->col) return HZ_ERROR_MAT_SIZE;
+    if (!device_ready()) return HZ_ERROR_DEVICE;
+    try {
+        arena().reset();
+        const DevCSR dA = upload_csr(A);
+        DevSELL S;
+        sell_from_csr(dA, S);
+        const double *db = upload(b, (size_t)A->row);
+        double *dx = upload(x, (size_t)A->row);
+        double *dxcg = x_cg ? arena().alloc_n<double>((size_t)A->row + 1) : nullptr;
+        double *work = arena().alloc_n<double>(coarse_itsolver_work(A->row));
+        int *dst = arena().alloc_n<int>(2);
+        HZ_CHECK(hipMemsetAsync(dst, 0, 2 * sizeof(int), stream()));
+        coarse_cg(S, db, dx, ctol, 20 * A->row, work, dst, dxcg);
+        int st[2] = {0, 0};
+        HZ_CHECK(memcpy_sync(st, dst, 2 * sizeof(int), hipMemcpyDeviceToHost));
+        HZ_CHECK(memcpy_sync(x, dx, sizeof(double) * A->row, hipMemcpyDeviceToHost));
+        if (x_cg) HZ_CHECK(memcpy_sync(x_cg, dxcg, sizeof(double) * A->row, hipMemcpyDeviceToHost));
+        if (cg_status) *cg_status = st[1];
+        return st[0];
+    } catch (const DeviceError &) {
+        return HZ_ERROR_DEVICE;
+    }
+}
+
 INT hz_batch_krylov_amg_device(INT nsys, const INT *row_offsets, INT nnz, const INT *dIA, const INT *dJA,
                                const REAL *dval, const REAL *db, REAL *dx, linear_itsolver_param *itparam,
                                AMG_param *amgparam, INT *iters) {

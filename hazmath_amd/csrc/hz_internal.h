@@ -181,7 +181,13 @@ void coarse_dense_solve(int n, const double *Ainv, const double *b, double *x);
 int dense_inverse_inplace(int n, double *M);
 bool dense_mfma_active();
 // Single-workgroup CG for coarse_solver = 0 (coarse_itsolver, mgcycle.c:34-53).
-void coarse_cg(const DevSELL &S, const double *b, double *x, double tol, int maxit, double *work);
+// coarse_itsolver (mgcycle.c:34-53) in one workgroup: CG, then GMRES(30) from
+// CG's last iterate when CG fails.  Optional device outputs: status[0] the
+// result code, status[1] CG's; x_cg CG's last iterate.
+constexpr int kCoarseGmresRestart = 30;
+inline size_t coarse_itsolver_work(int n) { return (size_t)(kCoarseGmresRestart + 3) * (size_t)n + 4; }
+void coarse_cg(const DevSELL &S, const double *b, double *x, double tol, int maxit, double *work,
+               int *status = nullptr, double *x_cg = nullptr);
 
 // ---- kernels_sa.hip: smoothed aggregation ---------------------------------
 // P = (I - omega D^-1 M) P_tent with M = A (filter 0) or the filtered strength

@@ -167,9 +167,10 @@ __global__ __launch_bounds__(256) void k_gemv(int n, const double *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// One-workgroup CG without preconditioner, stop type 1 (||r||/||r0||), with the
-// reference's safeguards (dcsr_pcg, krylov.c:122-372).  x holds the initial
-// guess (zero in the cycle).  work: 4n doubles.
+// The coarsest level's iterative solve in one workgroup: CG without
+// preconditioner, stop type 1 (||r||/||r0||), with the reference's safeguards
+// (dcsr_pcg, krylov.c:122-372), and GMRES(30) when it fails.  x holds the
+// initial guess (zero in the cycle).  work: coarse_itsolver_work(n) doubles.
 // ---------------------------------------------------------------------------
 constexpr int kCgBlock = 1024;
 
@@ -216,13 +217,11 @@ __device__ void cg_matvec(int n, int nslices, const long long *sptr, const int *
     }
 }
 
-__global__ __launch_bounds__(kCgBlock) void k_coarse_cg(int n, int nslices, const long long *__restrict__ sptr,
-                                                        const int *__restrict__ col,
-                                                        const double *__restrict__ val,
-                                                        const double *__restrict__ b,
-                                                        double *__restrict__ u, double tol, int maxit,
-                                                        double *__restrict__ work) {
-    __shared__ double sh[kCgBlock / kWave];
+// dcsr_pcg's result: iterations, or ERROR_SOLVER_SOLSTAG / _STAG / _TOLSMALL /
+// _MAXIT (krylov.c:122-372 as coarse_itsolver calls it: no preconditioner).
+__device__ int coarse_cg_body(int n, int nslices, const long long *__restrict__ sptr, const int *__restrict__ col,
+                              const double *__restrict__ val, const double *__restrict__ b, double *__restrict__ u,
+                              double tol, int maxit, double *__restrict__ work, double *sh) {
     double *p = work, *r = work + n, *t = work + 2 * n;
     const int tid = threadIdx.x;
     const double maxdiff = tol * 1e-4;
@@ -234,7 +233,7 @@ __global__ __launch_bounds__(kCgBlock) void k_coarse_cg(int n, int nslices, cons
     double absres0 = sqrt(cg_block_sum(acc, sh));
     const double normr0 = absres0 > kSmallReal ? absres0 : kSmallReal;
     double relres = absres0 / normr0, absres = absres0;
-    if (relres < tol || absres0 < 1e-3 * tol) return;
+    if (relres < tol || absres0 < 1e-3 * tol) return 0;
     for (int i = tid; i < n; i += kCgBlock) p[i] = r[i];
     acc = 0.0;
     for (int i = tid; i < n; i += kCgBlock) acc += r[i] * r[i];
@@ -261,7 +260,7 @@ __global__ __launch_bounds__(kCgBlock) void k_coarse_cg(int n, int nslices, cons
         absres = sqrt(cg_block_sum(rr, sh));
         relres = absres / normr0;
         const double infnormu = cg_block_max(ui, sh);
-        if (infnormu <= kSmallReal) return;                       // Check I
+        if (infnormu <= kSmallReal) return HZ_ERROR_SOLVER_SOLSTAG;   // Check I
         const double normu = sqrt(cg_block_sum(uu, sh));
         const double reldiff = fabs(alpha) * sqrt(cg_block_sum(pp, sh)) / normu;
         if ((stag <= 20) & (reldiff < maxdiff)) {                  // Check II
@@ -272,8 +271,8 @@ __global__ __launch_bounds__(kCgBlock) void k_coarse_cg(int n, int nslices, cons
             for (int i = tid; i < n; i += kCgBlock) acc += r[i] * r[i];
             absres = sqrt(cg_block_sum(acc, sh));
             relres = absres / normr0;
-            if (relres < tol) return;
-            if (stag >= 20) return;
+            if (relres < tol) return iter;
+            if (stag >= 20) return HZ_ERROR_SOLVER_STAG;
             for (int i = tid; i < n; i += kCgBlock) p[i] = 0.0;
             ++stag;
         }
@@ -285,8 +284,8 @@ __global__ __launch_bounds__(kCgBlock) void k_coarse_cg(int n, int nslices, cons
             for (int i = tid; i < n; i += kCgBlock) acc += r[i] * r[i];
             absres = sqrt(cg_block_sum(acc, sh));
             relres = absres / normr0;
-            if (relres < tol) return;
-            if (more_step >= 20) return;
+            if (relres < tol) return iter;
+            if (more_step >= 20) return HZ_ERROR_SOLVER_TOLSMALL;
             for (int i = tid; i < n; i += kCgBlock) p[i] = 0.0;
             ++more_step;
         }
@@ -298,6 +297,167 @@ __global__ __launch_bounds__(kCgBlock) void k_coarse_cg(int n, int nslices, cons
         const double beta = zr / temp1;
         temp1 = zr;
         for (int i = tid; i < n; i += kCgBlock) p[i] = 1.0 * r[i] + beta * p[i];
+    }
+    return HZ_ERROR_SOLVER_MAXIT;
+}
+
+
+// Block-wide vector helpers of the coarse GMRES: element i always belongs to
+// thread i mod kCgBlock, so consecutive element-wise steps need no barrier.
+__device__ double blk_dot(int n, const double *x, const double *y, double *sh) {
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += kCgBlock) acc += x[i] * y[i];
+    return cg_block_sum(acc, sh);
+}
+__device__ void blk_axpy(int n, double a, const double *x, double *y) {   // array_axpy (array.c:357-373)
+    for (int i = threadIdx.x; i < n; i += kCgBlock) {
+        if (a == 1.0) y[i] += x[i];
+        else if (a == -1.0) y[i] -= x[i];
+        else y[i] += a * x[i];
+    }
+}
+__device__ void blk_ax(int n, double a, double *x) {   // array_ax (array.c:332-341)
+    if (a != 1.0)
+        for (int i = threadIdx.x; i < n; i += kCgBlock) x[i] *= a;
+}
+
+// dcsr_pvgmres (krylov.c:4612-4931) without preconditioner, stop type 1, from
+// the iterate in x: GMRES(restart) with modified Gram-Schmidt and Givens
+// rotations, the restart length adapted per cycle from the last cycle's
+// convergence rate, the true residual checked before convergence is accepted.
+// The Hessenberg matrix and the rotations live in LDS and are updated by
+// thread 0; every branch depends only on block-wide sums or LDS values read
+// after a barrier, so all threads take it together.  work: (restart + 3) n.
+__device__ int coarse_gmres_body(int n, int nslices, const long long *__restrict__ sptr, const int *__restrict__ col,
+                                 const double *__restrict__ val, const double *__restrict__ b, double *__restrict__ x,
+                                 double tol, int MaxIt, double *__restrict__ work, double *sh, double *hh, double *cs,
+                                 double *sn, double *rs) {
+    constexpr int R = kCoarseGmresRestart;
+    const int tid = threadIdx.x;
+    double *r = work, *w = work + n, *P = work + 2 * (size_t)n;
+    auto Pk = [&](int k) { return P + (size_t)k * n; };
+    const double cr_max = 0.99, cr_min = 0.174;
+    const int d = 3, restart_max = R, restart_min = 3;
+    int iter = 0, i = 0, Restart = R;
+    double cr = 1.0, r_norm, r_norm_old, absres0, absres, relres;
+    __syncthreads();
+    cg_matvec(n, nslices, sptr, col, val, x, Pk(0), 1.0, b);
+    __syncthreads();
+    r_norm = sqrt(blk_dot(n, Pk(0), Pk(0), sh));
+    absres0 = r_norm > kSmallReal ? r_norm : kSmallReal;
+    relres = r_norm / absres0;
+    if (relres < tol || absres0 < 1e-3 * tol) return 0;
+    while (iter < MaxIt) {
+        r_norm_old = r_norm;
+        if (tid == 0) rs[0] = r_norm;
+        blk_ax(n, 1.0 / r_norm, Pk(0));
+        if (cr > cr_max || iter == 0) Restart = restart_max;
+        else if (cr < cr_min) {
+        } else if (Restart - d > restart_min) Restart -= d;
+        else Restart = restart_max;
+        i = 0;
+        while (i < Restart && iter < MaxIt) {
+            ++i;
+            ++iter;
+            __syncthreads();
+            cg_matvec(n, nslices, sptr, col, val, Pk(i - 1), Pk(i), 0.0, nullptr);
+            __syncthreads();
+            for (int j = 0; j < i; ++j) {
+                const double h = blk_dot(n, Pk(j), Pk(i), sh);
+                if (tid == 0) hh[j * R + i - 1] = h;
+                blk_axpy(n, -h, Pk(j), Pk(i));
+            }
+            double t = sqrt(blk_dot(n, Pk(i), Pk(i), sh));
+            if (tid == 0) hh[i * R + i - 1] = t;
+            if (t != 0.0) blk_ax(n, 1.0 / t, Pk(i));
+            __syncthreads();
+            if (tid == 0) {
+                for (int j = 1; j < i; ++j) {
+                    t = hh[(j - 1) * R + i - 1];
+                    hh[(j - 1) * R + i - 1] = sn[j - 1] * hh[j * R + i - 1] + cs[j - 1] * t;
+                    hh[j * R + i - 1] = -sn[j - 1] * t + cs[j - 1] * hh[j * R + i - 1];
+                }
+                t = hh[i * R + i - 1] * hh[i * R + i - 1];
+                t += hh[(i - 1) * R + i - 1] * hh[(i - 1) * R + i - 1];
+                double gamma = sqrt(t);
+                if (gamma == 0.0) gamma = kSmallReal;
+                cs[i - 1] = hh[(i - 1) * R + i - 1] / gamma;
+                sn[i - 1] = hh[i * R + i - 1] / gamma;
+                rs[i] = -sn[i - 1] * rs[i - 1];
+                rs[i - 1] = cs[i - 1] * rs[i - 1];
+                hh[(i - 1) * R + i - 1] = sn[i - 1] * hh[i * R + i - 1] + cs[i - 1] * hh[(i - 1) * R + i - 1];
+            }
+            __syncthreads();
+            absres = r_norm = fabs(rs[i]);
+            relres = absres / absres0;
+            if (relres < tol) break;
+        }
+        // back substitution, then x += w with w = sum_j rs_j p_j (j = i-1 first)
+        __syncthreads();
+        if (tid == 0) {
+            rs[i - 1] = rs[i - 1] / hh[(i - 1) * R + i - 1];
+            for (int k = i - 2; k >= 0; --k) {
+                double t = 0.0;
+                for (int j = k + 1; j < i; ++j) t -= hh[k * R + j] * rs[j];
+                t += rs[k];
+                rs[k] = t / hh[k * R + k];
+            }
+        }
+        __syncthreads();
+        for (int e = tid; e < n; e += kCgBlock) w[e] = Pk(i - 1)[e];
+        blk_ax(n, rs[i - 1], w);
+        for (int j = i - 2; j >= 0; --j) blk_axpy(n, rs[j], Pk(j), w);
+        blk_axpy(n, 1.0, w, x);
+        if (relres < tol) {
+            __syncthreads();
+            cg_matvec(n, nslices, sptr, col, val, x, r, 1.0, b);
+            __syncthreads();
+            r_norm = sqrt(blk_dot(n, r, r, sh));
+            absres = r_norm;
+            relres = absres / absres0;
+            if (relres < tol) break;
+            for (int e = tid; e < n; e += kCgBlock) Pk(0)[e] = r[e];
+            i = 0;
+        }
+        // the cycle's residual vector, the next cycle's first basis vector
+        __syncthreads();
+        if (tid == 0)
+            for (int j = i; j > 0; --j) {
+                rs[j - 1] = -sn[j - 1] * rs[j];
+                rs[j] = cs[j - 1] * rs[j];
+            }
+        __syncthreads();
+        if (i) blk_axpy(n, rs[i] - 1.0, Pk(i), Pk(i));
+        for (int j = i - 1; j > 0; --j) blk_axpy(n, rs[j], Pk(j), Pk(i));
+        if (i) {
+            blk_axpy(n, rs[0] - 1.0, Pk(0), Pk(0));
+            blk_axpy(n, 1.0, Pk(i), Pk(0));
+        }
+        cr = r_norm / r_norm_old;
+    }
+    return iter >= MaxIt ? HZ_ERROR_SOLVER_MAXIT : iter;
+}
+
+// coarse_itsolver (mgcycle.c:34-53): CG from x, and GMRES(30) from CG's last
+// iterate when CG fails.  status (optional): [0] the final result code, [1]
+// CG's; u_cg (optional): CG's last iterate.
+__global__ __launch_bounds__(kCgBlock) void k_coarse_itsolver(int n, int nslices, const long long *__restrict__ sptr,
+                                                              const int *__restrict__ col,
+                                                              const double *__restrict__ val,
+                                                              const double *__restrict__ b, double *__restrict__ u,
+                                                              double tol, int maxit, double *__restrict__ work,
+                                                              int *__restrict__ status, double *__restrict__ u_cg) {
+    __shared__ double sh[kCgBlock / kWave];
+    __shared__ double hh[(kCoarseGmresRestart + 1) * kCoarseGmresRestart];
+    __shared__ double cs[kCoarseGmresRestart + 1], sn[kCoarseGmresRestart + 1], rs[kCoarseGmresRestart + 1];
+    const int st_cg = coarse_cg_body(n, nslices, sptr, col, val, b, u, tol, maxit, work, sh);
+    if (u_cg)
+        for (int i = threadIdx.x; i < n; i += kCgBlock) u_cg[i] = u[i];
+    int st = st_cg;
+    if (st < 0) st = coarse_gmres_body(n, nslices, sptr, col, val, b, u, tol, maxit, work, sh, hh, cs, sn, rs);
+    if (status && threadIdx.x == 0) {
+        status[0] = st;
+        status[1] = st_cg;
     }
 }
 
@@ -409,10 +569,11 @@ void coarse_dense_solve(int n, const double *Ainv, const double *b, double *x) {
     HZ_LAUNCH_CHECK();
 }
 
-void coarse_cg(const DevSELL &S, const double *b, double *x, double tol, int maxit, double *work) {
+void coarse_cg(const DevSELL &S, const double *b, double *x, double tol, int maxit, double *work, int *status,
+               double *x_cg) {
     if (S.n == 0) return;
-    k_coarse_cg<<<1, kCgBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col, S.val, b, x, tol, maxit,
-                                              work);
+    k_coarse_itsolver<<<1, kCgBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col, S.val, b, x, tol, maxit, work,
+                                                    status, x_cg);
     HZ_LAUNCH_CHECK();
 }
 

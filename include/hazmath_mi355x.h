@@ -525,6 +525,14 @@ INT hz_cycle_rate_agg(dCSRmat *A, AMG_param *amgparam, const float *omega_agg, I
  * threshold theta, damping omega or per-aggregate omega_agg (may be NULL). */
 INT hz_aggregate_features(dCSRmat *A, const INT *aggregates, INT num_aggregates, INT level, float *feat);
 INT hz_cheby_smooth(dCSRmat *A, const REAL *b, REAL *x, INT steps, REAL scale);
+/* Parity hook of the coarsest level's iterative solve (coarse_solver other
+ * than 32; replaces coarse_itsolver, mgcycle.c:34-53): CG from x to the
+ * relative tolerance ctol, and when it fails GMRES(30) with the variable
+ * restart of dcsr_pvgmres (krylov.c:4612-4931) from CG's last iterate, both
+ * with 20 n iterations at most, in one workgroup on the device.  Returns the
+ * last method's result (iterations, or a negative code); cg_status and x_cg
+ * (either may be NULL) receive CG's result and last iterate. */
+INT hz_coarse_itsolver(dCSRmat *A, const REAL *b, REAL *x, REAL ctol, INT *cg_status, REAL *x_cg);
 /* The estimate behind the learned method's Chebyshev floor: *lambda = the
  * largest Ritz value of `steps` Lanczos steps on D^-1/2 A D^-1/2 (D the first
  * diagonal entries; a lower bound of lambda_max(D^-1 A) for SPD A) and

@@ -45,6 +45,12 @@ def load(path=None):
     lib.orc_aggregate.argtypes = [P(dCSRmat), INT, REAL, INT, P(INT), P(INT)]
     lib.orc_aggregate.restype = INT
     lib.orc_tridiag_condest.argtypes = [INT, P(REAL), P(REAL)]
+    lib.orc_pvgmres.argtypes = [P(dCSRmat), P(REAL), P(REAL), REAL, INT, INT]
+    lib.orc_pvgmres.restype = INT
+    lib.orc_coarse_itsolver.argtypes = [P(dCSRmat), P(REAL), P(REAL), REAL]
+    lib.orc_coarse_itsolver.restype = INT
+    lib.orc_cg.argtypes = [P(dCSRmat), P(REAL), P(REAL), REAL, INT]
+    lib.orc_cg.restype = INT
     lib.orc_tridiag_condest.restype = REAL
     lib.orc_set_jacobi_weight.argtypes = [REAL]
     lib.orc_sa_prolongation.argtypes = [P(dCSRmat), P(INT), INT, REAL, REAL, C.POINTER(C.c_float), INT,

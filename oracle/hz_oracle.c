@@ -14,8 +14,10 @@
  *  - the coarsest UᵀDU solve (hazmath_direct.c:633-854) is restated as a dense
  *    LDLᵀ solve: equal to rounding, not bit-identical (parity unpinned at the
  *    last-bit level, pinned by residual checks);
- *  - the GMRES fallback of the coarse CG (mgcycle.c:45-47) is not restated:
- *    parity unpinned for that branch (never taken on SPD inputs).
+ *  - the GMRES fallback of the coarse CG (mgcycle.c:45-47, dcsr_pvgmres
+ *    krylov.c:4612-4931) is pinned by the minimal-residual property of its
+ *    first restart cycle (least squares over the Krylov space, numpy) and by
+ *    convergence on nonsymmetric systems where the CG fails.
  */
 #include "hz_oracle.h"
 
@@ -1087,6 +1089,139 @@ static REAL orc_resid_measure(INT stop, INT m, const dCSRmat *A, const REAL *b, 
 
 static INT orc_pcg(const dCSRmat *A, const REAL *b, REAL *u, orc_prec_fn pc, void *pcd,
                    REAL tol, INT MaxIt, INT stop, INT prt, int want_cond, REAL *condest,
+                   orc_trace *tr);
+
+/* x = a x (array.c:332-341) */
+static void orc_ax(INT n, REAL a, REAL *x) {
+    if (a != 1.0)
+        for (INT i = 0; i < n; ++i) x[i] *= a;
+}
+
+/* dcsr_pvgmres (krylov.c:4612-4931) without preconditioner, stop type
+ * STOP_REL_RES: GMRES(m) with modified Gram-Schmidt and Givens rotations, the
+ * restart length adapted per cycle from the convergence rate cr of the last
+ * one (Baker, Jessup and Kolev 2009: back to the maximum above cos 8 deg,
+ * kept below cos 80 deg, otherwise shortened by 3 down to 3), and the true
+ * residual checked before convergence is accepted.  Returns the iteration
+ * count or ERROR_SOLVER_MAXIT. */
+INT orc_pvgmres(const dCSRmat *A, const REAL *b, REAL *x, REAL tol, INT MaxIt, INT restart) {
+    const INT n = A->row;
+    const REAL cr_max = 0.99, cr_min = 0.174;
+    const INT d = 3, restart_max = restart, restart_min = 3;
+    INT iter = 0, i = 0;
+    REAL r_norm, absres0, absres, relres, cr = 1.0, r_norm_old, t, gamma;
+    INT Restart = restart;
+    const INT R1 = restart + 1;
+    REAL *work = (REAL *)xcalloc((size_t)(R1 + 2) * n + (size_t)R1 * restart + 3 * (size_t)R1, sizeof(REAL));
+    REAL *r = work, *w = r + n, *P = w + n, *hh = P + (size_t)R1 * n, *c = hh + (size_t)R1 * restart,
+         *sn = c + R1, *rs = sn + R1;
+#define ORC_P(k) (P + (size_t)(k) * n)
+#define ORC_H(a, b) hh[(size_t)(a) * restart + (b)]
+    orc_dcsr_residual(A, b, x, ORC_P(0));
+    r_norm = orc_norm2(n, ORC_P(0));
+    absres0 = r_norm > ORC_SMALLREAL ? r_norm : ORC_SMALLREAL;
+    relres = r_norm / absres0;
+    if (relres < tol || absres0 < 1e-3 * tol) goto FINISHED;
+    while (iter < MaxIt) {
+        rs[0] = r_norm_old = r_norm;
+        t = 1.0 / r_norm;
+        orc_ax(n, t, ORC_P(0));
+        if (cr > cr_max || iter == 0) Restart = restart_max;
+        else if (cr < cr_min) { /* keep */
+        } else if (Restart - d > restart_min) Restart -= d;
+        else Restart = restart_max;
+        i = 0;
+        while (i < Restart && iter < MaxIt) {
+            ++i;
+            ++iter;
+            memcpy(r, ORC_P(i - 1), sizeof(REAL) * (size_t)n);
+            orc_dcsr_mxv(A, r, ORC_P(i));
+            for (INT j = 0; j < i; ++j) {
+                ORC_H(j, i - 1) = orc_dot(n, ORC_P(j), ORC_P(i));
+                orc_axpy(n, -ORC_H(j, i - 1), ORC_P(j), ORC_P(i));
+            }
+            t = orc_norm2(n, ORC_P(i));
+            ORC_H(i, i - 1) = t;
+            if (t != 0.0) {
+                t = 1.0 / t;
+                orc_ax(n, t, ORC_P(i));
+            }
+            for (INT j = 1; j < i; ++j) {
+                t = ORC_H(j - 1, i - 1);
+                ORC_H(j - 1, i - 1) = sn[j - 1] * ORC_H(j, i - 1) + c[j - 1] * t;
+                ORC_H(j, i - 1) = -sn[j - 1] * t + c[j - 1] * ORC_H(j, i - 1);
+            }
+            t = ORC_H(i, i - 1) * ORC_H(i, i - 1);
+            t += ORC_H(i - 1, i - 1) * ORC_H(i - 1, i - 1);
+            gamma = sqrt(t);
+            if (gamma == 0.0) gamma = ORC_SMALLREAL;
+            c[i - 1] = ORC_H(i - 1, i - 1) / gamma;
+            sn[i - 1] = ORC_H(i, i - 1) / gamma;
+            rs[i] = -sn[i - 1] * rs[i - 1];
+            rs[i - 1] = c[i - 1] * rs[i - 1];
+            ORC_H(i - 1, i - 1) = sn[i - 1] * ORC_H(i, i - 1) + c[i - 1] * ORC_H(i - 1, i - 1);
+            absres = r_norm = fabs(rs[i]);
+            relres = absres / absres0;
+            if (relres < tol) break;
+        }
+        /* solution: back substitution, then x += sum_j rs_j p_j */
+        rs[i - 1] = rs[i - 1] / ORC_H(i - 1, i - 1);
+        for (INT k = i - 2; k >= 0; --k) {
+            t = 0.0;
+            for (INT j = k + 1; j < i; ++j) t -= ORC_H(k, j) * rs[j];
+            t += rs[k];
+            rs[k] = t / ORC_H(k, k);
+        }
+        memcpy(w, ORC_P(i - 1), sizeof(REAL) * (size_t)n);
+        orc_ax(n, rs[i - 1], w);
+        for (INT j = i - 2; j >= 0; --j) orc_axpy(n, rs[j], ORC_P(j), w);
+        memcpy(r, w, sizeof(REAL) * (size_t)n);
+        orc_axpy(n, 1.0, r, x);
+        if (relres < tol) {
+            orc_dcsr_residual(A, b, x, r);
+            r_norm = orc_norm2(n, r);
+            absres = r_norm;
+            relres = absres / absres0;
+            if (relres < tol) break;
+            memcpy(ORC_P(0), r, sizeof(REAL) * (size_t)n);
+            i = 0;
+        }
+        /* residual vector of the cycle for the next one */
+        for (INT j = i; j > 0; --j) {
+            rs[j - 1] = -sn[j - 1] * rs[j];
+            rs[j] = c[j - 1] * rs[j];
+        }
+        if (i) orc_axpy(n, rs[i] - 1.0, ORC_P(i), ORC_P(i));
+        for (INT j = i - 1; j > 0; --j) orc_axpy(n, rs[j], ORC_P(j), ORC_P(i));
+        if (i) {
+            orc_axpy(n, rs[0] - 1.0, ORC_P(0), ORC_P(0));
+            orc_axpy(n, 1.0, ORC_P(i), ORC_P(0));
+        }
+        cr = r_norm / r_norm_old;
+    }
+FINISHED:
+#undef ORC_P
+#undef ORC_H
+    free(work);
+    return iter >= MaxIt ? HZ_ERROR_SOLVER_MAXIT : iter;
+}
+
+/* dcsr_pcg without preconditioner (krylov.c:122-372 as the coarse solve calls it). */
+INT orc_cg(const dCSRmat *A, const REAL *b, REAL *x, REAL tol, INT MaxIt) {
+    return orc_pcg(A, b, x, NULL, NULL, tol, MaxIt, HZ_STOP_REL_RES, 0, 0, NULL, NULL);
+}
+
+/* coarse_itsolver (mgcycle.c:34-53): CG from x, GMRES(30) from CG's last
+ * iterate when CG fails. */
+INT orc_coarse_itsolver(const dCSRmat *A, const REAL *b, REAL *x, REAL ctol) {
+    const INT n = A->row;
+    INT status = orc_pcg(A, b, x, NULL, NULL, ctol, 20 * n, HZ_STOP_REL_RES, 0, 0, NULL, NULL);
+    if (status < 0) status = orc_pvgmres(A, b, x, ctol, 20 * n, 30);
+    return status;
+}
+
+static INT orc_pcg(const dCSRmat *A, const REAL *b, REAL *u, orc_prec_fn pc, void *pcd,
+                   REAL tol, INT MaxIt, INT stop, INT prt, int want_cond, REAL *condest,
                    orc_trace *tr) {
     const INT m = A->row;
     const REAL maxdiff = tol * ORC_STAG_RATIO;
@@ -1279,9 +1414,7 @@ forward:
         if (p->coarse_solver == HZ_SOLVER_UMFPACK) {
             orc_coarse_solve(C, C->b, C->x);
         } else {
-            /* coarse_itsolver, mgcycle.c:34-53 */
-            INT n = C->A.row;
-            orc_pcg(&C->A, C->b, C->x, NULL, NULL, ctol, 20 * n, HZ_STOP_REL_RES, 0, 0, NULL, NULL);
+            orc_coarse_itsolver(&C->A, C->b, C->x, ctol);
         }
     }
     while (l > 0) {

@@ -56,6 +56,11 @@ void orc_strength(const dCSRmat *A, REAL theta, dCSRmat *N);
 INT  orc_aggregate(const dCSRmat *A, INT type, REAL theta, INT max_aggregation,
                    INT *agg, INT *nagg);
 REAL orc_tridiag_condest(INT n, const REAL *alpha, const REAL *beta);
+/* dcsr_pvgmres without preconditioner, stop type 1 (krylov.c:4612-4931), and
+ * coarse_itsolver: CG, then GMRES(30) when CG fails (mgcycle.c:34-53). */
+INT  orc_pvgmres(const dCSRmat *A, const REAL *b, REAL *x, REAL tol, INT MaxIt, INT restart);
+INT  orc_coarse_itsolver(const dCSRmat *A, const REAL *b, REAL *x, REAL ctol);
+INT  orc_cg(const dCSRmat *A, const REAL *b, REAL *x, REAL tol, INT MaxIt);
 /* Jacobi damping used by the V-cycle (reference: fixed 1.0, smoother.c:47). */
 void orc_set_jacobi_weight(REAL w);
 /* smoothed prolongation of one level (uniform or per-aggregate damping) */

@@ -338,3 +338,68 @@ def test_dense_coarsest_size_limit(lib, libc, capfd):
     assert r == capi.ERROR_ALLOC_MEM
     assert "the dense coarsest solve takes at most 32768" in err
     assert not np.any(x.arr)
+
+
+def _convdiff(m, bx, by):
+    """Upwind convection-diffusion on an m x m grid: a nonsymmetric M-matrix."""
+    import scipy.sparse as sp
+    n = m * m
+    rows, cols, vals = [], [], []
+    for j in range(m):
+        for i in range(m):
+            k = j * m + i
+            for q, v, ok in ((k - m, -1.0 - by, j > 0), (k - 1, -1.0 - bx, i > 0), (k, 4.0 + bx + by, True),
+                             (k + 1, -1.0, i < m - 1), (k + m, -1.0, j < m - 1)):
+                if ok:
+                    rows.append(k), cols.append(q), vals.append(v)
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+    A.sort_indices()
+    return A
+
+
+def _coarse_gpu(lib, A, b):
+    Ah = capi.HostCSR(A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float64))
+    x = np.zeros(A.shape[0])
+    xcg = np.zeros(A.shape[0])
+    st_cg = capi.INT()
+    st = lib.hz_coarse_itsolver(Ah.ptr(), capi.dptr(b), capi.dptr(x), 1e-8, C.byref(st_cg), capi.dptr(xcg))
+    return st, st_cg.value, x, xcg, Ah
+
+
+def test_coarse_itsolver_cg_on_spd_matches_oracle(lib, orc):
+    """coarse_itsolver on an SPD level (mgcycle.c:34-53): CG alone, as the
+    oracle's dcsr_pcg (iterations within one, solution within 1e-8)."""
+    import scipy.sparse as sp
+    ia, ja, va = problems.poisson2d_p1(24)
+    A = sp.csr_matrix((va, ja, ia))
+    b = problems.uniform_rhs(A.shape[0], seed=2)
+    st, st_cg, x, xcg, Ah = _coarse_gpu(lib, A, b)
+    xo = np.zeros(A.shape[0])
+    so = orc.orc_coarse_itsolver(Ah.ptr(), capi.dptr(b), capi.dptr(xo), 1e-8)
+    print(f"[semantics] coarse CG on SPD: GPU {st} (CG {st_cg}), oracle {so}", flush=True)
+    assert st == st_cg > 0 and abs(st - so) <= 1
+    assert np.array_equal(x, xcg)
+    assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+
+
+def test_coarse_itsolver_gmres_fallback_vs_oracle(lib, orc):
+    """On a nonsymmetric level the coarse CG fails (as the oracle's does) and
+    GMRES(30) with dcsr_pvgmres's variable restart continues from CG's last
+    iterate: the GPU result reaches 1e-8 relative to that iterate's residual,
+    and the oracle's GMRES from the same iterate takes the same number of
+    iterations (within two) to a solution within 1e-6."""
+    A = _convdiff(10, 6.0, 4.0)
+    n = A.shape[0]
+    b = problems.uniform_rhs(n, seed=3)
+    st, st_cg, x, xcg, Ah = _coarse_gpu(lib, A, b)
+    xo_cg = np.zeros(n)
+    so_cg = orc.orc_cg(Ah.ptr(), capi.dptr(b), capi.dptr(xo_cg), 1e-8, 20 * n)
+    xo = xcg.copy()
+    so = orc.orc_pvgmres(Ah.ptr(), capi.dptr(b), capi.dptr(xo), 1e-8, 20 * n, 30)
+    r_cg = np.linalg.norm(b - A @ xcg)
+    print(f"[semantics] coarse solve, nonsymmetric: GPU CG {st_cg} then GMRES {st}; oracle CG {so_cg}, "
+          f"oracle GMRES from the GPU's CG iterate {so}; CG residual {r_cg:.3e}", flush=True)
+    assert st_cg < 0 and so_cg < 0
+    assert st > 0 and abs(st - so) <= 2
+    assert np.linalg.norm(b - A @ x) < 1e-8 * r_cg
+    assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)

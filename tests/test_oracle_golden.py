@@ -691,3 +691,95 @@ def test_oracle_gs_pcg_beats_jacobi(orc, libc, solvers_fixture, smoother):
     assert 0 < rg < rj
     Asp = sp.csr_matrix((va, ja, ia))
     assert np.linalg.norm(b - Asp @ x) / np.linalg.norm(b) < 1e-7
+
+
+def _convdiff(m, bx, by):
+    """Upwind convection-diffusion on an m x m grid (Dirichlet): row
+    (4 + bx + by) u_P - (1 + bx) u_W - u_E - (1 + by) u_S - u_N, a
+    nonsymmetric M-matrix."""
+    n = m * m
+    A = sp.lil_matrix((n, n))
+    for j in range(m):
+        for i in range(m):
+            k = j * m + i
+            A[k, k] = 4.0 + bx + by
+            if i > 0:
+                A[k, k - 1] = -1.0 - bx
+            if i < m - 1:
+                A[k, k + 1] = -1.0
+            if j > 0:
+                A[k, k - m] = -1.0 - by
+            if j < m - 1:
+                A[k, k + m] = -1.0
+    A = A.tocsr()
+    A.sort_indices()
+    return A
+
+
+def _hostcsr(A):
+    return capi.HostCSR(A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float64))
+
+
+@pytest.mark.parametrize("k", [1, 2, 5, 12])
+def test_oracle_gmres_is_minimal_residual_over_krylov_space(orc, k):
+    """dcsr_pvgmres (krylov.c:4612-4931) stopped after k < 30 steps (one
+    restart cycle) returns x0 + z with z minimising ||b - A (x0 + z)|| over
+    the Krylov space K_k(A, r0): compared with a least-squares solve on an
+    orthonormal basis of that space."""
+    A = _convdiff(9, 2.0, 0.5)
+    n = A.shape[0]
+    rng = np.random.default_rng(4)
+    b = rng.uniform(-1, 1, n)
+    x0 = rng.uniform(-1, 1, n)
+    x = x0.copy()
+    Ah = _hostcsr(A)
+    r = orc.orc_pvgmres(Ah.ptr(), capi.dptr(b), capi.dptr(x), 1e-30, k, 30)
+    assert r == capi.ERROR_SOLVER_MAXIT
+    r0 = b - A @ x0
+    K = np.empty((n, k))
+    v = r0 / np.linalg.norm(r0)
+    for j in range(k):
+        K[:, j] = v
+        v = A @ v
+        v = v / np.linalg.norm(v)
+    Q, _ = np.linalg.qr(K)
+    y, *_ = np.linalg.lstsq(A @ Q, r0, rcond=None)
+    best = np.linalg.norm(r0 - A @ Q @ y)
+    got = np.linalg.norm(b - A @ x)
+    assert got == pytest.approx(best, rel=1e-9, abs=1e-14 * np.linalg.norm(r0))
+    assert np.linalg.norm(x - (x0 + Q @ y)) <= 1e-8 * np.linalg.norm(x0 + Q @ y)
+
+
+@pytest.mark.parametrize("restart", [30, 4])
+def test_oracle_gmres_converges_with_restarts(orc, restart):
+    """Restarted (and, with restart 30, adaptively shortened) GMRES reaches
+    the tolerance in the true residual on a nonsymmetric system."""
+    A = _convdiff(20, 3.0, 1.0)
+    n = A.shape[0]
+    b = problems.uniform_rhs(n, seed=8)
+    x = np.zeros(n)
+    Ah = _hostcsr(A)
+    it = orc.orc_pvgmres(Ah.ptr(), capi.dptr(b), capi.dptr(x), 1e-10, 20 * n, restart)
+    assert 0 < it < 20 * n
+    assert np.linalg.norm(b - A @ x) < 1e-10 * np.linalg.norm(b)
+    if restart == 4:
+        assert it > 4   # several restart cycles were needed
+
+
+def test_oracle_coarse_itsolver_falls_back_to_gmres(orc):
+    """coarse_itsolver (mgcycle.c:34-53): on a nonsymmetric system CG fails
+    (20 n iterations without convergence, or a safeguard) and GMRES(30),
+    started from CG's last iterate, reaches the tolerance."""
+    A = _convdiff(10, 6.0, 4.0)
+    n = A.shape[0]
+    b = problems.uniform_rhs(n, seed=3)
+    Ah = _hostcsr(A)
+    xcg = np.zeros(n)
+    assert orc.orc_cg(Ah.ptr(), capi.dptr(b), capi.dptr(xcg), 1e-8, 20 * n) < 0
+    xg = xcg.copy()
+    itg = orc.orc_pvgmres(Ah.ptr(), capi.dptr(b), capi.dptr(xg), 1e-8, 20 * n, 30)
+    x = np.zeros(n)
+    it = orc.orc_coarse_itsolver(Ah.ptr(), capi.dptr(b), capi.dptr(x), 1e-8)
+    assert it == itg > 0 and np.array_equal(x, xg)
+    # GMRES's tolerance is relative to its own starting residual, CG's last one
+    assert np.linalg.norm(b - A @ x) < 1e-8 * np.linalg.norm(b - A @ xcg)
