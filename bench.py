@@ -496,6 +496,7 @@ def learned_comparison(a, lib, steps, warmup, kinds=("plain", "sa", "learned", "
             out["fallback"] = int(info.fallback_used)
             out["learned_attempt"] = dict(iterations=int(info.learned_iterations), total_ms=info.learned_total_ms)
             out["cheb_scale"] = [round(info.cheb_scale[l], 4) for l in range(max(0, info.num_levels - 1))]
+            out["lambda_ratio"] = [round(info.lambda_ratio[l], 4) for l in range(max(0, info.num_levels - 1))]
             if info.fallback_used:
                 out["total_ms"] = info.learned_total_ms + info.total_ms
         return out

@@ -86,7 +86,7 @@ class hz_options(C.Structure):
     _fields_ = [("jacobi_weight", REAL), ("hec_rng_source", INT), ("verbose_timing", INT),
                 ("return_on_fatal", INT), ("vcycle_precision", INT), ("batch_time_kernels", INT),
                 ("time_pcg_spmv", INT), ("hem_host_max_rows", INT),
-                ("rap_wave_stage_pairs", INT), ("hem_pipeline_max_sweeps", INT)]
+                ("rap_wave_stage_pairs", INT), ("hem_pipeline_max_sweeps", INT), ("release_after_call", INT)]
 
 
 class hz_batch_stats(C.Structure):
@@ -351,6 +351,10 @@ def _declare(lib):
     lib.hz_batch_krylov_amg.restype = INT
     lib.hz_batch_get_stats.argtypes = [P(hz_batch_stats)]
     lib.hz_batch_get_stats.restype = None
+    lib.hz_release_device_memory.argtypes = []
+    lib.hz_release_device_memory.restype = None
+    lib.hz_device_pool_bytes.argtypes = []
+    lib.hz_device_pool_bytes.restype = C.c_longlong
     lib.hz_coarse_itsolver.argtypes = [P(dCSRmat), P(REAL), P(REAL), REAL, P(INT), P(REAL)]
     lib.hz_coarse_itsolver.restype = INT
     lib.hz_batch_tune_cache.argtypes = [C.c_char_p]

@@ -43,15 +43,53 @@ int aggregate_host(const DevCSR &N, int type, int max_aggregation, int *agg_dev,
     double *a = static_cast<double *>(g_a.get(sizeof(double) * (size_t)N.nnz));
     hipStream_t st = stream();
     HZ_CHECK(hipMemcpyAsync(ia, N.ia, sizeof(int) * (n + 1), hipMemcpyDeviceToHost, st));
-    if (N.nnz) {
-        HZ_CHECK(hipMemcpyAsync(ja, N.ja, sizeof(int) * N.nnz, hipMemcpyDeviceToHost, st));
-        if (type != HZ_VMB)
-            HZ_CHECK(hipMemcpyAsync(a, N.val, sizeof(double) * N.nnz, hipMemcpyDeviceToHost, st));
-    }
-    HZ_CHECK(hipStreamSynchronize(st));
     std::vector<int> v;
-    int st_code = (type == HZ_VMB) ? vmb_sweep(n, ia, ja, max_aggregation, v, *nagg)
-                                       : hem_sweep(n, ia, ja, a, v, *nagg);
+    int st_code;
+    if (type == HZ_VMB) {
+        // VMB's first pass walks the rows downwards: the column indices come
+        // in blocks of about equal size from the last rows down, and the pass
+        // starts on the first block while the others are on the link
+        constexpr int kBlocks = 8;
+        static hipEvent_t ev[kBlocks];
+        static bool ev_ready = false;
+        if (!ev_ready) {
+            for (auto &e : ev) HZ_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+            ev_ready = true;
+        }
+        HZ_CHECK(hipStreamSynchronize(st));
+        const long long total = ia[n] - ia[0];
+        int lo[kBlocks];
+        int nb = 0;
+        for (int b = 0, hi = n; b < kBlocks && hi > 0; ++b) {
+            int r = hi;
+            if (b == kBlocks - 1) r = 0;
+            else {
+                const long long target = ia[hi] - total / kBlocks;   // rows [r, hi) hold about total / kBlocks
+                while (r > 0 && ia[r - 1] >= target) --r;
+                if (r == hi) --r;
+            }
+            const long long e0 = ia[r], e1 = ia[hi];
+            if (e1 > e0)
+                HZ_CHECK(hipMemcpyAsync(ja + e0, N.ja + e0, sizeof(int) * (size_t)(e1 - e0), hipMemcpyDeviceToHost,
+                                        st));
+            HZ_CHECK(hipEventRecord(ev[nb], st));
+            lo[nb++] = r;
+            hi = r;
+        }
+        RowBlocksDown arrival;
+        arrival.nblocks = nb;
+        arrival.lo = lo;
+        arrival.wait = [](int b) { HZ_CHECK(hipEventSynchronize(ev[b])); };
+        st_code = vmb_sweep(n, ia, ja, max_aggregation, v, *nagg, &arrival);
+        HZ_CHECK(hipStreamSynchronize(st));
+    } else {
+        if (N.nnz) {
+            HZ_CHECK(hipMemcpyAsync(ja, N.ja, sizeof(int) * N.nnz, hipMemcpyDeviceToHost, st));
+            HZ_CHECK(hipMemcpyAsync(a, N.val, sizeof(double) * N.nnz, hipMemcpyDeviceToHost, st));
+        }
+        HZ_CHECK(hipStreamSynchronize(st));
+        st_code = hem_sweep(n, ia, ja, a, v, *nagg);
+    }
     if (st_code == -1017) return st_code;
     int *pv = static_cast<int *>(g_v.get(sizeof(int) * (size_t)n));
     std::copy(v.begin(), v.end(), pv);

@@ -14,10 +14,16 @@
 
 namespace hz {
 
-int vmb_sweep(int n, const int *ia, const int *ja, int maxagg, std::vector<int> &v, int &nagg) {
+int vmb_sweep(int n, const int *ia, const int *ja, int maxagg, std::vector<int> &v, int &nagg,
+              const RowBlocksDown *arrival) {
     int left = n, na = 0;
     v.assign(n, -2);
+    int blk = 0, here = n;   // rows >= here have arrived
     for (int i = n - 1; i >= 0; --i) {
+        while (arrival && i < here && blk < arrival->nblocks) {
+            arrival->wait(blk);
+            here = arrival->lo[blk++];
+        }
         if (ia[i + 1] - ia[i] == 1) {
             v[i] = HZ_UNPT;
             --left;

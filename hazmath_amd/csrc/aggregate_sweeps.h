@@ -1,13 +1,26 @@
 // aggregate_sweeps.h -- the sequential VMB and HEM sweeps on host arrays.
 #pragma once
 
+#include <functional>
 #include <vector>
 
 namespace hz {
 
+// Column indices that arrive in row blocks from the last rows down: block b
+// holds rows [lo[b], lo[b - 1]) (lo[-1] = n), and wait(b) returns once it is
+// there.  VMB's first pass walks the rows downwards and reads only the row it
+// is at, so it can start before the whole graph has arrived.
+struct RowBlocksDown {
+    int nblocks = 0;
+    const int *lo = nullptr;
+    std::function<void(int)> wait;
+};
+
 // aggregation_vmb (amg_setup_ua.c:596-727) on the strength graph (ia, ja) of n
-// rows: v[i] = aggregate of i or -1 (UNPT); returns SUCCESS.
-int vmb_sweep(int n, const int *ia, const int *ja, int maxagg, std::vector<int> &v, int &nagg);
+// rows: v[i] = aggregate of i or -1 (UNPT); returns SUCCESS.  arrival: ja
+// arrives in row blocks (null: all there).
+int vmb_sweep(int n, const int *ia, const int *ja, int maxagg, std::vector<int> &v, int &nagg,
+              const RowBlocksDown *arrival = nullptr);
 // aggregation_hem (amg_setup_ua.c:913-1051) on the strength graph (ia, ja, a):
 // SUCCESS, or -1017 (the reference's exit(17), its message printed) when a
 // leftover has no aggregated neighbour.

@@ -51,6 +51,12 @@ void Arena::release() {
     cur_ = 0;
 }
 
+size_t Arena::bytes_reserved() const {
+    size_t s = 0;
+    for (auto &c : chunks_) s += c.size;
+    return s;
+}
+
 size_t Arena::bytes_in_use() const {
     size_t s = 0;
     for (auto &c : chunks_) s += c.used;
@@ -63,6 +69,9 @@ Arena *g_current_arena = &g_default_arena;
 }  // namespace
 
 Arena &arena() { return *g_current_arena; }
+
+void release_library_arena() { g_default_arena.release(); }
+size_t library_arena_bytes() { return g_default_arena.bytes_reserved(); }
 
 ArenaScope::ArenaScope(Arena *a) : prev_(g_current_arena) { g_current_arena = a; }
 ArenaScope::~ArenaScope() { g_current_arena = prev_; }
@@ -227,11 +236,12 @@ int amg_setup(Hierarchy &H, const DevCSR &A0, AMG_param *param, bool) {
                 double scale = learned_parameters(nagg, out, om);
                 // floor: the Chebyshev interval must reach the level's lambda_max(D^-1 A).  The Lanczos
                 // estimate (30 steps) is from below, hence 1.1x; the Gershgorin bound (scale 1) always
-                // covers lambda_max, so the floor never exceeds 1.
+                // covers lambda_max, so the floor never exceeds 1 and is not needed (nor estimated)
+                // when the network asks for 1 or more.
                 const ChebLevel g1 = cheby_prepare(L.A, 1.0);
                 if ((int)H.cheb_scale.size() < max_levels) H.cheb_scale.assign(max_levels, 1.0);
                 if ((int)H.lambda_ratio.size() < max_levels) H.lambda_ratio.assign(max_levels, 0.0);
-                if (g1.lmax > 0.0) {
+                if (g1.lmax > 0.0 && scale < 1.0) {
                     const double ratio = lanczos_lambda_dinv(L.A, g1.diag, 30, nullptr) / g1.lmax;
                     const double floor = std::min(1.0, 1.1 * ratio);
                     scale = std::min(1.5, std::max(scale, floor));

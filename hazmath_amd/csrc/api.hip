@@ -11,7 +11,22 @@
 using namespace hz;
 
 namespace {
-hz_options g_opt = {1.0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+hz_options g_opt = {1.0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+
+// The library's device memory for one top-level call: the arena is reset on
+// entry, and with hz_options.release_after_call its chunks go back to HIP when
+// the call returns (the reference frees its work arrays before returning).
+struct CallArena {
+    CallArena() { arena().reset(); }
+    ~CallArena() {
+        if (g_opt.release_after_call) {
+            (void)hipStreamSynchronize(stream());
+            release_library_arena();
+        }
+    }
+    CallArena(const CallArena &) = delete;
+    CallArena &operator=(const CallArena &) = delete;
+};
 hz_solve_info g_info;
 }  // namespace
 
@@ -552,6 +567,14 @@ void hz_set_options(const hz_options *opt) {
 }
 void hz_get_last_info(hz_solve_info *info) { *info = g_info; }
 
+void hz_release_device_memory(void) {
+    if (!runtime_up()) return;
+    (void)hipStreamSynchronize(stream());
+    release_library_arena();
+}
+
+long long hz_device_pool_bytes(void) { return (long long)library_arena_bytes(); }
+
 const char *hz_version(void) { return "hazmath_mi355x 0.1 (gfx950)"; }
 
 INT hz_device_count(void) {
@@ -565,7 +588,7 @@ INT hz_krylov_amg_device(INT n, INT nnz, const INT *dIA, const INT *dJA, const R
                          AMG_param *amgparam) {
     if (!device_ready()) return HZ_ERROR_DEVICE;
     try {
-        arena().reset();
+        CallArena call_arena;
         DevCSR A;
         A.n = n;
         A.ncol = n;
@@ -584,7 +607,7 @@ INT linear_solver_dcsr_krylov_amg(dCSRmat *A, dvector *b, dvector *x, linear_its
                                   AMG_param *amgparam) {
     if (!device_ready()) return HZ_ERROR_DEVICE;
     try {
-        arena().reset();
+        CallArena call_arena;
         const DevCSR dA = upload_csr(A);
         const double *db = upload(b->val, (size_t)A->row);
         double *dx = upload(x->val, (size_t)A->row);
@@ -602,7 +625,7 @@ INT hz_setup_hierarchy(dCSRmat *A, AMG_param *amgparam, hz_hierarchy *out) {
     std::memset(out, 0, sizeof(*out));
     if (!device_ready()) return HZ_ERROR_DEVICE;
     try {
-        arena().reset();
+        CallArena call_arena;
         const DevCSR dA = upload_csr(A);
         Hierarchy H;
         H.hec_rng = g_opt.hec_rng_source;
@@ -642,7 +665,7 @@ void hz_hierarchy_free(hz_hierarchy *h) {
 INT hz_dcsr_mxv(dCSRmat *A, const REAL *x, REAL *y) {
     if (!device_ready()) return HZ_ERROR_DEVICE;
     try {
-        arena().reset();
+        CallArena call_arena;
         DevCSR dA = upload_csr(A);
         DevSELL S;
         sell_from_csr(dA, S);
@@ -661,7 +684,7 @@ INT hz_dcsr_mxv(dCSRmat *A, const REAL *x, REAL *y) {
 INT hz_dcsr_residual(dCSRmat *A, const REAL *b, const REAL *x, REAL *w) {
     if (!device_ready()) return HZ_ERROR_DEVICE;
     try {
-        arena().reset();
+        CallArena call_arena;
         DevCSR dA = upload_csr(A);
         DevSELL S;
         sell_from_csr(dA, S);
@@ -680,7 +703,7 @@ INT hz_dcsr_residual(dCSRmat *A, const REAL *b, const REAL *x, REAL *w) {
 INT hz_smoother_sweep(dCSRmat *A, const REAL *b, REAL *x, INT smoother, REAL w) {
     if (!device_ready()) return HZ_ERROR_DEVICE;
     try {
-        arena().reset();
+        CallArena call_arena;
         DevCSR dA = upload_csr(A);
         DevSELL S;
         sell_from_csr(dA, S);
@@ -701,7 +724,7 @@ INT hz_time_fine_kernels(INT n, INT nnz, const INT *dIA, const INT *dJA, const R
     std::memset(out, 0, sizeof(*out));
     if (!device_ready()) return HZ_ERROR_DEVICE;
     try {
-        arena().reset();
+        CallArena call_arena;
         DevCSR A;
         A.n = n;
         A.ncol = n;
@@ -751,7 +774,7 @@ INT hz_time_fine_kernels(INT n, INT nnz, const INT *dIA, const INT *dJA, const R
 INT hz_rap_agg(dCSRmat *A, const INT *aggregates, INT num_aggregates, dCSRmat *Ac) {
     if (!device_ready()) return HZ_ERROR_DEVICE;
     try {
-        arena().reset();
+        CallArena call_arena;
         DevCSR dA = upload_csr(A);
         int *dagg = upload(aggregates, (size_t)A->row);
         int *rptr = arena().alloc_n<int>((size_t)num_aggregates + 1);
@@ -771,7 +794,7 @@ INT hz_dense_inverse(INT n, REAL *M) {
     if (n == 0) return HZ_SUCCESS;
     if (!device_ready()) return HZ_ERROR_DEVICE;
     try {
-        arena().reset();
+        CallArena call_arena;
         double *dM = upload(M, (size_t)n * n);
         const int st = dense_inverse_inplace(n, dM);
         if (st != 0) return HZ_ERROR_SOLVER_MISC;
@@ -805,7 +828,7 @@ INT linear_solver_amg(dCSRmat *A, dvector *b, dvector *x, AMG_param *param) {
     }
     if (!device_ready()) return HZ_ERROR_DEVICE;
     try {
-        arena().reset();
+        CallArena call_arena;
         const auto t0 = std::chrono::steady_clock::now();
         int st = check_supported(nullptr, param);
         if (st < 0) return st;
@@ -949,7 +972,7 @@ INT hz_mlp_forward(const float *theta, const float *x, INT B, float *y) {
     if (B == 0) return HZ_SUCCESS;
     if (!device_ready()) return HZ_ERROR_DEVICE;
     try {
-        arena().reset();
+        CallArena call_arena;
         const float *dth = upload(theta, HZ_MLP_PARAMS);
         const float *dx = upload(x, (size_t)B * HZ_MLP_INPUTS);
         MlpWork w = mlp_work(B);
@@ -980,7 +1003,7 @@ INT hz_mlp_loss_grad(const float *theta, const float *x, const float *t, INT B, 
     if (B <= 0) return HZ_ERROR_INPUT_PAR;
     if (!device_ready()) return HZ_ERROR_DEVICE;
     try {
-        arena().reset();
+        CallArena call_arena;
         const float *dth = upload(theta, HZ_MLP_PARAMS);
         const float *dx = upload(x, (size_t)B * HZ_MLP_INPUTS);
         const float *dt = upload(t, (size_t)B * HZ_MLP_OUTPUTS);
@@ -1005,7 +1028,7 @@ INT hz_mlp_train(float *theta, const float *x, const float *t, INT B, INT epochs
     if (B <= 0 || epochs < 0 || batch <= 0) return HZ_ERROR_INPUT_PAR;
     if (!device_ready()) return HZ_ERROR_DEVICE;
     try {
-        arena().reset();
+        CallArena call_arena;
         float *dth = upload(theta, HZ_MLP_PARAMS);
         const float *dx = upload(x, (size_t)B * HZ_MLP_INPUTS);
         const float *dt = upload(t, (size_t)B * HZ_MLP_OUTPUTS);
@@ -1067,7 +1090,7 @@ INT hz_krylov_amg_learned(dCSRmat *A, dvector *b, dvector *x, linear_itsolver_pa
     }
     if (!device_ready()) return HZ_ERROR_DEVICE;
     try {
-        arena().reset();
+        CallArena call_arena;
         const size_t n = (size_t)A->row;
         const DevCSR dA = upload_csr(A);
         const double *db = upload(b->val, n);
@@ -1135,7 +1158,7 @@ INT hz_two_grid_rate(dCSRmat *A, AMG_param *amgparam, REAL omega, REAL lscale, I
     *rate = 0.0;
     if (!device_ready()) return HZ_ERROR_DEVICE;
     try {
-        arena().reset();
+        CallArena call_arena;
         const DevCSR dA = upload_csr(A);
         AMG_param p = *amgparam;
         p.AMG_type = HZ_SA_AMG;
@@ -1165,7 +1188,7 @@ INT hz_cycle_rate_agg(dCSRmat *A, AMG_param *amgparam, const float *omega_agg, I
     if (!omega_agg || nagg <= 0 || iters < 2) return HZ_ERROR_INPUT_PAR;
     if (!device_ready()) return HZ_ERROR_DEVICE;
     try {
-        arena().reset();
+        CallArena call_arena;
         const DevCSR dA = upload_csr(A);
         AMG_param p = *amgparam;
         p.AMG_type = HZ_SA_AMG;
@@ -1190,7 +1213,7 @@ INT hz_cycle_rate_agg(dCSRmat *A, AMG_param *amgparam, const float *omega_agg, I
 INT hz_aggregate_features(dCSRmat *A, const INT *aggregates, INT num_aggregates, INT level, float *feat) {
     if (!device_ready()) return HZ_ERROR_DEVICE;
     try {
-        arena().reset();
+        CallArena call_arena;
         const DevCSR dA = upload_csr(A);
         int *dagg = upload(aggregates, (size_t)A->row);
         int *rp = arena().alloc_n<int>((size_t)num_aggregates + 1);
@@ -1211,7 +1234,7 @@ INT hz_lambda_max_dinv(dCSRmat *A, INT steps, REAL *lambda, REAL *gershgorin) {
     if (steps <= 0) return HZ_ERROR_INPUT_PAR;
     if (!device_ready()) return HZ_ERROR_DEVICE;
     try {
-        arena().reset();
+        CallArena call_arena;
         const DevCSR dA = upload_csr(A);
         const ChebLevel g1 = cheby_prepare(dA, 1.0);
         *gershgorin = g1.lmax;
@@ -1225,7 +1248,7 @@ INT hz_lambda_max_dinv(dCSRmat *A, INT steps, REAL *lambda, REAL *gershgorin) {
 INT hz_cheby_smooth(dCSRmat *A, const REAL *b, REAL *x, INT steps, REAL scale) {
     if (!device_ready()) return HZ_ERROR_DEVICE;
     try {
-        arena().reset();
+        CallArena call_arena;
         const DevCSR dA = upload_csr(A);
         DevSELL S;
         sell_from_csr(dA, S);
@@ -1246,7 +1269,7 @@ INT hz_coarse_itsolver(dCSRmat *A, const REAL *b, REAL *x, REAL ctol, INT *cg_st
     if (A->row != A->col) return HZ_ERROR_MAT_SIZE;
     if (!device_ready()) return HZ_ERROR_DEVICE;
     try {
-        arena().reset();
+        CallArena call_arena;
         const DevCSR dA = upload_csr(A);
         DevSELL S;
         sell_from_csr(dA, S);
@@ -1277,7 +1300,7 @@ INT hz_batch_krylov_amg_device(INT nsys, const INT *row_offsets, INT nnz, const 
         if (row_offsets[s + 1] <= row_offsets[s]) return HZ_ERROR_MAT_SIZE;
     if (!device_ready()) return HZ_ERROR_DEVICE;
     try {
-        arena().reset();
+        CallArena call_arena;
         DevCSR A;
         A.n = A.ncol = row_offsets[nsys];
         A.nnz = nnz;
@@ -1305,7 +1328,7 @@ INT hz_batch_krylov_amg(INT nsys, dCSRmat *const *A, INT nrhs, REAL *const *b, R
     if (nnz > INT32_MAX) return HZ_ERROR_MAT_SIZE;
     if (!device_ready()) return HZ_ERROR_DEVICE;
     try {
-        arena().reset();
+        CallArena call_arena;
         const int n = off[nsys];
         // the block-diagonal stack and the interleaved right-hand sides
         std::vector<int> ia(n + 1), ja(nnz);
@@ -1364,7 +1387,7 @@ INT hz_gs_smooth(dCSRmat *A, const REAL *b, REAL *x, INT smoother, INT post, INT
     if (A->row != A->col) return HZ_ERROR_MAT_SIZE;
     if (!device_ready()) return HZ_ERROR_DEVICE;
     try {
-        arena().reset();
+        CallArena call_arena;
         const DevCSR dA = upload_csr(A);
         const double *db = upload(b, (size_t)A->row);
         double *dx = upload(x, (size_t)A->row);
@@ -1386,7 +1409,7 @@ INT hz_sa_prolongation(dCSRmat *A, const INT *aggregates, INT num_aggregates, RE
                        const float *omega_agg, INT filter, dCSRmat *P) {
     if (!device_ready()) return HZ_ERROR_DEVICE;
     try {
-        arena().reset();
+        CallArena call_arena;
         const DevCSR dA = upload_csr(A);
         int *dagg = upload(aggregates, (size_t)A->row);
         int *counts = arena().alloc_n<int>((size_t)A->row + 1);

@@ -47,6 +47,7 @@ class Arena {
     void reset();            // forget all allocations, keep chunks
     void release();          // hipFree every chunk
     size_t bytes_in_use() const;
+    size_t bytes_reserved() const;   // sum of the chunks' sizes
     struct Mark { size_t chunk, used; };
     Mark mark() const;               // current fill position
     void rewind(const Mark &m);      // forget allocations made after m
@@ -61,6 +62,9 @@ class Arena {
 // (a preconditioner keeps its hierarchy across calls) while an ArenaScope for
 // it is alive.
 Arena &arena();
+// Returns the library-wide arena's chunks to HIP (its next use allocates again).
+void release_library_arena();
+size_t library_arena_bytes();   // device bytes the library-wide arena holds
 class ArenaScope {
   public:
     explicit ArenaScope(Arena *a);

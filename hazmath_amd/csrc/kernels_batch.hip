@@ -913,11 +913,12 @@ TuneStats g_tune;
 // Cache key of a decision: device, kernel, and the matrix's class (rounded
 // log2 of the rows, 4 x the mean entries per row).
 std::string tune_key(const char *kernel, const BatchLevel &L) {
-    static std::string dev;
+    static std::map<int, std::string> names;   // per device ordinal
+    int d = 0;
+    HZ_CHECK(hipGetDevice(&d));
+    std::string &dev = names[d];
     if (dev.empty()) {
         hipDeviceProp_t prop{};
-        int d = 0;
-        HZ_CHECK(hipGetDevice(&d));
         HZ_CHECK(hipGetDeviceProperties(&prop, d));
         std::string arch = prop.gcnArchName;
         arch = arch.substr(0, arch.find(':'));

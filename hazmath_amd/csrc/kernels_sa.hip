@@ -234,6 +234,7 @@ __global__ void k_rap_rows(int r0, int r1, const int *__restrict__ rows, const l
 // than a stage); the per-thread kernel takes those and the fill pass skips
 // them.
 constexpr int kRapwPairs = 512;
+constexpr double kRapSlotBytes = 4e9;   // slots of the one-pass form: at most 4 GB
 constexpr int kRapwCols = 512;
 constexpr int kRapwHash = 2 * kRapwCols;   // at most kRapwCols + 64 keys: probing ends
 
@@ -251,7 +252,7 @@ __device__ __forceinline__ int wave_incl_scan(int v, int lane) {
     return v;
 }
 
-template <bool kFill>
+template <bool kFill, bool kSlots = false>
 __global__ __launch_bounds__(kWave) void k_rapw(int nc, const int *__restrict__ ria, const int *__restrict__ rja,
                                                 const double *__restrict__ rval, const int *__restrict__ aia,
                                                 const int *__restrict__ aja, const double *__restrict__ aval,
@@ -268,7 +269,7 @@ __global__ __launch_bounds__(kWave) void k_rapw(int nc, const int *__restrict__ 
     const int lane = threadIdx.x;
     const int ic = blockIdx.x;
     if (ic >= nc) return;   // the whole (single-wavefront) block
-    if (kFill && rowcount[ic] < 0) return;
+    if (kFill && !kSlots && rowcount[ic] < 0) return;
     // column table: the diagonal first, from 0.0
     for (int t = lane; t < kRapwHash; t += kWave) {
         s_key[t] = -1;
@@ -385,17 +386,37 @@ __global__ __launch_bounds__(kWave) void k_rapw(int nc, const int *__restrict__ 
             __syncthreads();
         }
     }
-    if (!kFill) {
+    if (!kFill || kSlots) {
         if (lane == 0) rowcount[ic] = big ? -1 : next;
-        return;
+        if (!kFill || big) return;
     }
     __syncthreads();
-    const int o = cia[ic];
+    const long long o = kSlots ? (long long)ic * max_cols : (long long)cia[ic];
     for (int t = lane; t < kRapwHash; t += kWave)
         if (s_key[t] >= 0) {
             cja[o + s_pos[t]] = s_key[t];
             cval[o + s_pos[t]] = s_val[t];
         }
+}
+
+// Rows the one-pass form left in slots of max_cols (count >= 0) move to their
+// CSR place, one wavefront per row.
+__global__ __launch_bounds__(kBlock) void k_rap_compact_slots(int nc, const int *__restrict__ slotcount, int max_cols,
+                                                             const int *__restrict__ sja,
+                                                             const double *__restrict__ sval,
+                                                             const int *__restrict__ cia, int *__restrict__ cja,
+                                                             double *__restrict__ cval) {
+    const int row = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (row >= nc) return;
+    const int cnt = slotcount[row];
+    if (cnt < 0) return;
+    const long long src = (long long)row * max_cols;
+    const int dst = cia[row];
+    for (int k = lane; k < cnt; k += kWave) {
+        cja[dst + k] = sja[src + k];
+        cval[dst + k] = sval[src + k];
+    }
 }
 
 __global__ void k_rapw_big(int nc, const int *__restrict__ rowcount, int *__restrict__ list, int *__restrict__ nlist) {
@@ -524,7 +545,19 @@ void rap_general(const DevCSR &R, const DevCSR &A, const DevCSR &P, DevCSR &C) {
     const bool wave = g_rap_wave_pairs >= 0 || g_rap_wave_pairs == -2;
     const int max_pairs = g_rap_wave_pairs > 0 ? std::min(g_rap_wave_pairs, kRapwPairs) : kRapwPairs;
     const int max_cols = g_rap_wave_pairs == -2 ? 16 : kRapwCols;
-    if (wave) {
+    // one pass into fixed slots of max_cols per row (then compacted) while the
+    // slots fit kRapSlotBytes; otherwise a count pass, then the fill
+    const bool slots = wave && (double)nc * max_cols * 12.0 <= kRapSlotBytes;
+    int *slotcount = nullptr, *sja = nullptr;
+    double *sval = nullptr;
+    if (slots) {
+        slotcount = arena().alloc_n<int>((size_t)nc + 1);
+        sja = arena().alloc_n<int>((size_t)nc * max_cols);
+        sval = arena().alloc_n<double>((size_t)nc * max_cols);
+        k_rapw<true, true><<<nc, kWave, 0, st>>>(nc, R.ia, R.ja, R.val, A.ia, A.ja, A.val, P.ia, P.ja, P.val,
+                                                 max_pairs, max_cols, slotcount, nullptr, sja, sval);
+        HZ_CHECK(hipMemcpyAsync(rowcount, slotcount, sizeof(int) * (size_t)nc, hipMemcpyDeviceToDevice, st));
+    } else if (wave) {
         k_rapw<false><<<nc, kWave, 0, st>>>(nc, R.ia, R.ja, R.val, A.ia, A.ja, A.val, P.ia, P.ja, P.val, max_pairs,
                                             max_cols, rowcount, nullptr, nullptr, nullptr);
     } else {
@@ -578,7 +611,11 @@ void rap_general(const DevCSR &R, const DevCSR &A, const DevCSR &P, DevCSR &C) {
     C.nnz = (int)exclusive_scan_int(rowcount, C.ia, nc);
     C.ja = arena().alloc_n<int>((size_t)C.nnz + 1);
     C.val = arena().alloc_n<double>((size_t)C.nnz + 1);
-    if (wave) {
+    if (slots) {
+        k_rap_compact_slots<<<div_up(nc, kBlock / kWave), kBlock, 0, st>>>(nc, slotcount, max_cols, sja, sval, C.ia,
+                                                                           C.ja, C.val);
+        HZ_LAUNCH_CHECK();
+    } else if (wave) {
         k_rapw<true><<<nc, kWave, 0, st>>>(nc, R.ia, R.ja, R.val, A.ia, A.ja, A.val, P.ia, P.ja, P.val, max_pairs,
                                            max_cols, rowcount, C.ia, C.ja, C.val);
         HZ_LAUNCH_CHECK();

@@ -341,10 +341,19 @@ typedef struct {
                               by size go to the host sweep after 128 sweeps (long dependency
                               chains, e.g. strongly anisotropic problems); a pipeline forced on
                               every level (hem_host_max_rows -1) finishes by rounds after 8192 */
+    INT  release_after_call;  /* 1: every solver call returns the library's device memory pool to
+                              HIP before it returns (the reference frees its work arrays); 0
+                              (default): the pool is kept, so a warm call performs no hipMalloc.
+                              Preconditioner objects keep their own memory until hz_precond_free */
 } hz_options;
 
 void hz_get_options(hz_options *opt);
 void hz_set_options(const hz_options *opt);
+/* Returns the library's device memory pool to HIP now (no counterpart in the
+ * reference, which allocates per call); the next call allocates again.
+ * hz_device_pool_bytes: device bytes the pool holds now. */
+void hz_release_device_memory(void);
+long long hz_device_pool_bytes(void);
 
 /* Per-solve statistics of the last call on this thread. */
 typedef struct {
@@ -372,7 +381,9 @@ typedef struct {
     REAL   cheb_scale[HZ_MAX_AMG_LVL];   /* learned SA: Chebyshev bound scale in force per level
                                             (after the floor), 0 on levels without one */
     REAL   lambda_ratio[HZ_MAX_AMG_LVL]; /* learned SA: estimated lambda_max(D^-1 A) over the
-                                            Gershgorin bound per level, 0 where not estimated */
+                                            Gershgorin bound per level, 0 where not estimated
+                                            (levels whose network scale is 1 or more: the
+                                            floor, at most 1, cannot bind there) */
     REAL   solve_alg_bytes;  /* algorithmic bytes of every kernel the PCG launched (each distinct
                                 input read once, each output written once; the gathered coarse
                                 corrections of the transfers not counted): over solve_ms, the

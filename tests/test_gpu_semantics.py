@@ -403,3 +403,42 @@ def test_coarse_itsolver_gmres_fallback_vs_oracle(lib, orc):
     assert st > 0 and abs(st - so) <= 2
     assert np.linalg.norm(b - A @ x) < 1e-8 * r_cg
     assert np.linalg.norm(x - xo) <= 1e-6 * np.linalg.norm(xo)
+
+
+def test_release_after_call_frees_the_pool(lib, libc):
+    """hz_options.release_after_call returns the library's device pool to HIP
+    when each call returns, and hz_release_device_memory does it on demand
+    (hz_device_pool_bytes: the pool's chunks); results are the same bits as
+    with the pool kept."""
+    ia, ja, va = problems.poisson3d_kuhn(40)
+    A = capi.HostCSR(ia, ja, va)
+    b = problems.uniform_rhs(ia.size - 1, seed=4)
+
+    def solve():
+        x = capi.HostVec(np.zeros(ia.size - 1))
+        it, amg = capi.solver_params(lib, tol=1e-10, maxit=200, aggregation=capi.HEM, smoother=capi.SMOOTHER_L1DIAG)
+        libc.srand(1)
+        r = lib.linear_solver_dcsr_krylov_amg(A.ptr(), capi.HostVec(b).ptr(), x.ptr(), C.byref(it), C.byref(amg))
+        assert r > 0
+        return x.arr.copy()
+
+    x_kept = solve()
+    kept = lib.hz_device_pool_bytes()
+    lib.hz_release_device_memory()
+    released = lib.hz_device_pool_bytes()
+    x_again = solve()
+    opts = capi.hz_options()
+    lib.hz_get_options(C.byref(opts))
+    old = opts.release_after_call
+    opts.release_after_call = 1
+    lib.hz_set_options(C.byref(opts))
+    try:
+        x_rel = solve()
+        after = lib.hz_device_pool_bytes()
+    finally:
+        opts.release_after_call = old
+        lib.hz_set_options(C.byref(opts))
+    print(f"[semantics] pool bytes: kept {kept >> 20} MiB, after release {released}, after a releasing call {after}",
+          flush=True)
+    assert kept >= 256 << 20 and released == 0 and after == 0
+    assert np.array_equal(x_kept, x_again) and np.array_equal(x_kept, x_rel)
