@@ -255,14 +255,15 @@ __global__ __launch_bounds__(kBlock) void k_bsweep(int n, int nslices, const lon
             for (int q = 0; q < kN; ++q) t[q] = t[q] - a * xj[q];
         }
     };
-    int k = 0;
-    for (; k + kU <= wd; k += kU) {
+    // rounds of kU entries, masked past the slice width (no one-entry tail)
+    for (int k = 0; k < wd; k += kU) {
         int cc[kU];
         float aa[kU], xx[kU][kN];
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
-            cc[u] = ld_nt(c + (long long)(k + u) * kSlice);
-            aa[u] = ld_nt(v + (long long)(k + u) * kSlice);
+            const bool ok = k + u < wd;
+            cc[u] = ok ? ld_nt(c + (long long)(k + u) * kSlice) : -1;
+            aa[u] = ok ? ld_nt(v + (long long)(k + u) * kSlice) : 0.0f;
         }
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
@@ -270,12 +271,6 @@ __global__ __launch_bounds__(kBlock) void k_bsweep(int n, int nslices, const lon
         }
 #pragma unroll
         for (int u = 0; u < kU; ++u) step(cc[u], aa[u], xx[u]);
-    }
-    for (; k < wd; ++k) {
-        const int cj = ld_nt(c + (long long)k * kSlice);
-        float xj[kN];
-        if (cj >= 0) ld_vec<kN>(x + (size_t)cj * kR + m.q0, xj);
-        step(cj, ld_nt(v + (long long)k * kSlice), xj);
     }
     float xn[kN];
 #pragma unroll
@@ -310,14 +305,15 @@ __global__ __launch_bounds__(kBlock) void k_bresidual(int n, int nslices, const 
     const int lane = threadIdx.x % kWave;
     const int *c = col + base + lane;
     const float *v = val + base + lane;
-    int k = 0;
-    for (; k + kU <= wd; k += kU) {
+    // rounds of kU entries, masked past the slice width (no one-entry tail)
+    for (int k = 0; k < wd; k += kU) {
         int cc[kU];
         float aa[kU], xx[kU][kN];
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
-            cc[u] = ld_nt(c + (long long)(k + u) * kSlice);
-            aa[u] = ld_nt(v + (long long)(k + u) * kSlice);
+            const bool ok = k + u < wd;
+            cc[u] = ok ? ld_nt(c + (long long)(k + u) * kSlice) : -1;
+            aa[u] = ok ? ld_nt(v + (long long)(k + u) * kSlice) : 0.0f;
         }
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
@@ -329,15 +325,6 @@ __global__ __launch_bounds__(kBlock) void k_bresidual(int n, int nslices, const 
 #pragma unroll
             for (int q = 0; q < kN; ++q) acc[q] = acc[q] + aa[u] * xx[u][q];
         }
-    }
-    for (; k < wd; ++k) {
-        const int cj = ld_nt(c + (long long)k * kSlice);
-        if (cj < 0) continue;
-        const float a = ld_nt(v + (long long)k * kSlice);
-        float xj[kN];
-        ld_vec<kN>(x + (size_t)cj * kR + m.q0, xj);
-#pragma unroll
-        for (int q = 0; q < kN; ++q) acc[q] = acc[q] + a * xj[q];
     }
     float bi[kN];
     ld_vec<kN>(b + (size_t)r * kR + m.q0, bi);
@@ -538,15 +525,16 @@ __global__ __launch_bounds__(kBlock) void k_spmm64(int n, int nslices, const lon
             acc[2 * h + 1] += a * xv[h].y;
         }
     };
-    int k = 0;
-    for (; k + kU <= wd; k += kU) {
+    // rounds of kU entries, masked past the slice width (no one-entry tail)
+    for (int k = 0; k < wd; k += kU) {
         int cc[kU];
         double aa[kU];
         double2 xx[kU][kV];
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
-            cc[u] = ld_nt(c + (long long)(k + u) * kSlice);
-            aa[u] = ld_nt(v + (long long)(k + u) * kSlice);
+            const bool ok = k + u < wd;
+            cc[u] = ok ? ld_nt(c + (long long)(k + u) * kSlice) : -1;
+            aa[u] = ok ? ld_nt(v + (long long)(k + u) * kSlice) : 0.0;
         }
 #pragma unroll
         for (int u = 0; u < kU; ++u) {
@@ -559,16 +547,6 @@ __global__ __launch_bounds__(kBlock) void k_spmm64(int n, int nslices, const lon
 #pragma unroll
         for (int u = 0; u < kU; ++u)
             if (cc[u] >= 0) madd(aa[u], xx[u]);
-    }
-    for (; k < wd; ++k) {
-        const int cj = ld_nt(c + (long long)k * kSlice);
-        if (cj < 0) continue;
-        const double a = ld_nt(v + (long long)k * kSlice);
-        const double2 *xp = reinterpret_cast<const double2 *>(x + (size_t)cj * kR + m.q0);
-        double2 xv[kV];
-#pragma unroll
-        for (int h = 0; h < kV; ++h) xv[h] = xp[h];
-        madd(a, xv);
     }
     const size_t o = (size_t)m.r * kR + m.q0;
 #pragma unroll
