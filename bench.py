@@ -179,6 +179,8 @@ def parse():
                         "learned vs plain AMG-PCG")
     p.add_argument("--learned-kinds", default="plain,sa,learned,learned_fp32",
                    help="methods of --learned, comma separated (all four print the full line)")
+    p.add_argument("--learned-problem", choices=("aniso", "poisson"), default="aniso",
+                   help="--learned on the anisotropic problem (eps=1e-3) or on the Poisson problem")
     p.add_argument("--cpu-sample-m", type=int, default=128,
                    help="cells per direction of the CPU-baseline sample")
     p.add_argument("--cpu-runs", type=int, default=5, help="CPU-baseline solves (median reported)")
@@ -440,7 +442,11 @@ def learned_comparison(a, lib, steps, warmup, kinds=("plain", "sa", "learned", "
     uploaded before the timed region."""
     from hazmath_amd import learned
     theta_dir = float(np.random.default_rng(5).uniform(0.0, np.pi))
-    ia, ja, va = problems.aniso3d_kuhn(a.m, theta_dir, LEARNED_EPS)
+    poisson = getattr(a, "learned_problem", "aniso") == "poisson"
+    if poisson:
+        ia, ja, va = problems.poisson3d_kuhn(a.m)
+    else:
+        ia, ja, va = problems.aniso3d_kuhn(a.m, theta_dir, LEARNED_EPS)
     n = ia.size - 1
     b = problems.uniform_rhs(n, seed=1)
     weights = learned.load_model()
@@ -513,8 +519,10 @@ def learned_comparison(a, lib, steps, warmup, kinds=("plain", "sa", "learned", "
         print(f"[bench] {kind}: {best}", file=sys.stderr, flush=True)
     os.close(devnull)
     os.close(saved)
-    workload = (f"3-D Kuhn P1 -div(K grad u), K = R_z(theta) diag(1, eps, eps) R_z^T, eps={LEARNED_EPS}, "
-                f"theta={theta_dir:.4f}, {a.m}^3 cells, n={n}, seeded RHS, PCG to {a.tol:g}; "
+    problem = (f"3-D Kuhn P1 Poisson, {a.m}^3 cells" if poisson else
+               f"3-D Kuhn P1 -div(K grad u), K = R_z(theta) diag(1, eps, eps) R_z^T, eps={LEARNED_EPS}, "
+               f"theta={theta_dir:.4f}, {a.m}^3 cells")
+    workload = (f"{problem}, n={n}, seeded RHS, PCG to {a.tol:g}; "
                 f"coarse_dof={a.coarse_dof}; plain = UA HEM L1 V(1,1) FP64 drop-in; sa = SA on VMB with Chebyshev(2), constant "
                 "parameters; learned = the same with per-aggregate damping and bound scale from "
                 "the committed network, plain method as fallback; learned_fp32 = learned with the FP32 cycle "
@@ -529,7 +537,8 @@ def main_learned(a, lib):
         print(json.dumps({"partial_learned_comparison": res}))
         return
     line = {
-        "metric": "learned vs plain AMG-PCG time-to-1e-10 (s), 3-D Kuhn P1 anisotropic eps=1e-3",
+        "metric": ("learned vs plain AMG-PCG time-to-1e-10 (s), 3-D Kuhn P1 "
+                   + ("Poisson" if a.learned_problem == "poisson" else "anisotropic eps=1e-3")),
         "value": res["learned_fp32"]["total_ms"] * 1e-3,
         "unit": "s",
         "n_gpus": 1,
