@@ -948,7 +948,7 @@ PcgResult pcg_solve(Hierarchy &H, const DevSELL &A, const double *b, double *u, 
     // before the host has read that iteration's scalars (they do not touch u or
     // r); the rare restart / convergence paths below redo or drop that work.
     while (iter++ < MaxIt) {
-        spmv(A, p, t, &H.sc->temp2, H.ws, H.spmv_timer);
+        spmv(A, p, t, &H.sc->temp2, H.ws, H.spmv_timer, H.cp.precision > 0 && H.nl > 1);
         pcg_update(m, H.sc, p, t, u, r, alpha_hist, iter, H.ws, fuse_first ? &first : nullptr);
         fetch_async(H);
         if (speculate) {

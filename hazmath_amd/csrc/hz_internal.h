@@ -98,8 +98,10 @@ struct LaunchTimer {
 void sell_from_csr(const DevCSR &A, DevSELL &S);
 // y = A x ; if dot_out: *dot_out = sum_i y_i * x_i (deterministic).  timer:
 // events around the SpMV kernel itself (not the partials' finalize).
+// cols_only: the PCG's SpMV ahead of an FP32 cycle, which shares only the
+// column indices (those stay in the last-level cache, the values stream).
 void spmv(const DevSELL &S, const double *x, double *y, double *dot_out, const ReduceWs &ws,
-          LaunchTimer *timer = nullptr);
+          LaunchTimer *timer = nullptr, bool cols_only = false);
 // w = b - A x (timer: events around the kernel)
 void residual(const DevSELL &S, const double *b, const double *x, double *w, LaunchTimer *timer = nullptr);
 // One smoothing sweep out of place: xout = sweep(x).  smoother: 1 Jacobi (weight w), 10 L1DIAG.

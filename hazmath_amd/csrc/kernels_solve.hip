@@ -106,7 +106,7 @@ __global__ __launch_bounds__(kBlock) void k_sell_fill(int n, const int *__restri
 // order, so a row costs a few memory round trips instead of one per entry.
 // kKeep 1: the second half of each round uses ordinary loads, whose lines
 // stay in the last-level cache for the next kernel that streams the same
-// matrix; kKeep 2: every load is ordinary.
+// matrix; kKeep 2: every load is ordinary; kKeep 4: the column indices only.
 template <bool kNtTail, bool kWide = false, int kKeep = 0>
 __device__ __forceinline__ double sell_row_dot(const int *c, const double *v, int w, const double *x) {
     // kWide: rounds of 16 masked past the slice width (a 15-entry row in one round)
@@ -120,7 +120,8 @@ __device__ __forceinline__ double sell_row_dot(const int *c, const double *v, in
         for (int u = 0; u < U; ++u) {
             const bool ok = !kWide || k + u < w;
             const bool keep = kKeep == 2 || (kKeep == 1 && u >= U / 2);
-            cc[u] = ok ? (keep ? c[(k + u) * kSlice] : ld_nt(c + (k + u) * kSlice)) : -1;
+            const bool keep_col = keep || kKeep == 4;
+            cc[u] = ok ? (keep_col ? c[(k + u) * kSlice] : ld_nt(c + (k + u) * kSlice)) : -1;
             vv[u] = ok ? (keep ? v[(k + u) * kSlice] : ld_nt(v + (k + u) * kSlice)) : 0.0;
         }
 #pragma unroll
@@ -667,7 +668,7 @@ void sell_from_csr(const DevCSR &A, DevSELL &S) {
 }
 
 void spmv(const DevSELL &S, const double *x, double *y, double *dot_out, const ReduceWs &ws,
-          LaunchTimer *timer) {
+          LaunchTimer *timer, bool cols_only) {
     if (S.n == 0) return;
     count_bytes(matrix_bytes(S.nnz, S.n) + 16.0 * S.n);
     const int grid = div_up(S.nslices, kSlicesPerBlock);
@@ -675,8 +676,13 @@ void spmv(const DevSELL &S, const double *x, double *y, double *dot_out, const R
         if (timer) timer->begin();
         // the PCG's SpMV: rounds of 16, the second half of each with ordinary
         // loads that stay in the last-level cache for the cycle's residual
-        k_spmv<true, true, 1><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col, S.val, x, y,
-                                                                ws.partials);
+        // (ahead of an FP32 cycle: every column index, no value)
+        if (cols_only)
+            k_spmv<true, true, 4><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col, S.val, x, y,
+                                                                    ws.partials);
+        else
+            k_spmv<true, true, 1><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col, S.val, x, y,
+                                                                    ws.partials);
         if (timer) timer->end();
         HZ_LAUNCH_CHECK();
         k_final_sum<<<1, kFinBlock, 0, stream()>>>(ws.partials, grid, dot_out);
