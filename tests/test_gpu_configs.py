@@ -207,7 +207,7 @@ def test_config4_full_batch_every_pair_and_oracle_subset(lib, orc, libc):
 @pytest.mark.parametrize("precision", [0, 1])
 def test_config5_learned_kuhn128_anisotropic(lib, libc, precision):
     """5. The learned method's evaluation problem: 3-D Kuhn P1 anisotropic,
-    eps = 1e-3, seeded rotation, 128^3 cells: the learned SA + Chebyshev(2)
+    eps = 1e-3, seeded rotation, 128^3 cells: the learned SA + Chebyshev(4)
     method with the committed network (FP64 cycle, and the FP32 cycle under
     the FP64 PCG) converges without falling back and meets 1e-10 in the FP64
     true residual; the plain drop-in runs beside it for the record."""
@@ -221,7 +221,7 @@ def test_config5_learned_kuhn128_anisotropic(lib, libc, precision):
     try:
         _say(f"config 5: learned solve, precision {precision}")
         libc.srand(1)
-        r, x, info = learned.solve_learned(lib, ia, ja, va, b, weights, budget=400)
+        r, x, info = learned.solve_learned(lib, ia, ja, va, b, weights, budget=400, cheb_steps=4)
     finally:
         _set_options(lib, **old)
     M = sp.csr_matrix((va, ja, ia), shape=(n, n))
