@@ -235,14 +235,15 @@ int amg_setup(Hierarchy &H, const DevCSR &A0, AMG_param *param, bool) {
                 float *om = arena().alloc_n<float>((size_t)nagg + 1);
                 double scale = learned_parameters(nagg, out, om);
                 // floor: the Chebyshev interval must reach the level's lambda_max(D^-1 A).  The Lanczos
-                // estimate (30 steps) is from below, hence 1.1x; the Gershgorin bound (scale 1) always
-                // covers lambda_max, so the floor never exceeds 1 and is not needed (nor estimated)
-                // when the network asks for 1 or more.
+                // estimate (up to 30 steps) is from below, hence 1.1x; the Gershgorin bound (scale 1)
+                // always covers lambda_max, so the floor never exceeds 1 and is not needed (nor estimated)
+                // when the network asks for 1 or more.  The estimate stops once 1.1x its Ritz value
+                // reaches the Gershgorin bound: the floor is then 1 whatever the later steps give.
                 const ChebLevel g1 = cheby_prepare(L.A, 1.0);
                 if ((int)H.cheb_scale.size() < max_levels) H.cheb_scale.assign(max_levels, 1.0);
                 if ((int)H.lambda_ratio.size() < max_levels) H.lambda_ratio.assign(max_levels, 0.0);
                 if (g1.lmax > 0.0 && scale < 1.0) {
-                    const double ratio = lanczos_lambda_dinv(L.A, g1.diag, 30, nullptr) / g1.lmax;
+                    const double ratio = lanczos_lambda_dinv(L.A, g1.diag, 30, nullptr, g1.lmax / 1.1) / g1.lmax;
                     const double floor = std::min(1.0, 1.1 * ratio);
                     scale = std::min(1.5, std::max(scale, floor));
                     H.lambda_ratio[lvl] = ratio;

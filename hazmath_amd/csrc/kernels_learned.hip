@@ -376,6 +376,9 @@ __global__ __launch_bounds__(kBlock) void k_lz_start(int n, const double *__rest
     if (threadIdx.x == 0) part[blockIdx.x] = bs;
 }
 
+// An early-stopping estimate looks at its Ritz values every this many steps.
+constexpr int kLzCheckSteps = 8;
+
 // Number of eigenvalues of the symmetric tridiagonal (diag a, off-diagonal b)
 // below x (Sturm sequence).
 int sturm_below(int m, const double *a, const double *b, double x) {
@@ -392,7 +395,34 @@ int sturm_below(int m, const double *a, const double *b, double x) {
 
 }  // namespace
 
-double lanczos_lambda_dinv(const DevCSR &A, const double *d, int steps, int *steps_done) {
+// Largest eigenvalue of the Lanczos tridiagonal (diagonal ha[0..m), off-diagonal
+// hb[0..m-1)), cut at the first (near-)zero off-diagonal entry, where the
+// Krylov space is invariant; *used: the order kept.
+static double ritz_max(const double *ha, const double *hb, int m, int *used) {
+    double scale = 0.0;
+    for (int k = 0; k < m; ++k) scale = std::max(scale, std::fabs(ha[k]));
+    for (int k = 0; k + 1 < m; ++k)
+        if (hb[k] <= 1e-14 * std::max(scale, 1e-300)) {
+            m = k + 1;
+            break;
+        }
+    if (used) *used = m;
+    // bisection on the Gershgorin interval of the tridiagonal
+    double lo = ha[0], hi = ha[0];
+    for (int i = 0; i < m; ++i) {
+        const double r = (i > 0 ? std::fabs(hb[i - 1]) : 0.0) + (i + 1 < m ? std::fabs(hb[i]) : 0.0);
+        lo = std::min(lo, ha[i] - r);
+        hi = std::max(hi, ha[i] + r);
+    }
+    for (int it = 0; it < 200 && hi - lo > 1e-15 * std::max(std::fabs(hi), 1e-300); ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (sturm_below(m, ha, hb, mid) >= m) hi = mid;   // every eigenvalue below mid
+        else lo = mid;
+    }
+    return hi;
+}
+
+double lanczos_lambda_dinv(const DevCSR &A, const double *d, int steps, int *steps_done, double stop_at) {
     const int n = A.n;
     if (steps_done) *steps_done = 0;
     if (n == 0 || steps <= 0) return 0.0;
@@ -404,6 +434,8 @@ double lanczos_lambda_dinv(const DevCSR &A, const double *d, int steps, int *ste
     double *part = arena().alloc_n<double>((size_t)G + 1);
     double *ab = arena().alloc_n<double>(2 * (size_t)steps + 2);   // alpha[0..steps), beta[0..steps), |w0|
     double *alpha = ab, *beta = ab + steps, *nrm0 = ab + 2 * steps;
+    std::vector<double> h(2 * (size_t)steps);
+    const double *ha = h.data(), *hb = h.data() + steps;
     // y[1] holds the start vector: step 0 reads it as "wprev" with norm |w0|
     k_lz_start<<<G, kBlock, 0, st>>>(n, d, isd, yb[1], part);
     k_sum_partials<<<1, kBlock, 0, st>>>(part, G, 1, nrm0);
@@ -417,33 +449,25 @@ double lanczos_lambda_dinv(const DevCSR &A, const double *d, int steps, int *ste
         k_lz_orth<<<G, kBlock, 0, st>>>(n, alpha + k, k > 0 ? beta + k - 1 : nullptr, v, vprev, y, part);
         k_sum_partials<<<1, kBlock, 0, st>>>(part, G, 1, beta + k);
         HZ_LAUNCH_CHECK();
-    }
-    std::vector<double> h(2 * (size_t)steps);
-    HZ_CHECK(memcpy_sync(h.data(), ab, sizeof(double) * 2 * steps, hipMemcpyDeviceToHost));
-    // stop at the first (near-)zero beta: the Krylov space is invariant there
-    const double *ha = h.data(), *hb = h.data() + steps;
-    int m = steps;
-    double scale = 0.0;
-    for (int k = 0; k < steps; ++k) scale = std::max(scale, std::fabs(ha[k]));
-    for (int k = 0; k + 1 < steps; ++k)
-        if (hb[k] <= 1e-14 * std::max(scale, 1e-300)) {
-            m = k + 1;
-            break;
+        // Ritz values only grow with the step count: once the largest reaches
+        // stop_at, more steps cannot bring it back below
+        const int m = k + 1;
+        if (stop_at > 0.0 && m < steps && m % kLzCheckSteps == 0) {
+            HZ_CHECK(memcpy_sync(h.data(), alpha, sizeof(double) * m, hipMemcpyDeviceToHost));
+            HZ_CHECK(memcpy_sync(h.data() + steps, beta, sizeof(double) * m, hipMemcpyDeviceToHost));
+            int used = 0;
+            const double theta = ritz_max(ha, hb, m, &used);
+            if (theta >= stop_at) {
+                if (steps_done) *steps_done = used;
+                return theta;
+            }
         }
-    if (steps_done) *steps_done = m;
-    // largest eigenvalue of the m x m tridiagonal by bisection on Gershgorin bounds
-    double lo = ha[0], hi = ha[0];
-    for (int i = 0; i < m; ++i) {
-        const double r = (i > 0 ? std::fabs(hb[i - 1]) : 0.0) + (i + 1 < m ? std::fabs(hb[i]) : 0.0);
-        lo = std::min(lo, ha[i] - r);
-        hi = std::max(hi, ha[i] + r);
     }
-    for (int it = 0; it < 200 && hi - lo > 1e-15 * std::max(std::fabs(hi), 1e-300); ++it) {
-        const double mid = 0.5 * (lo + hi);
-        if (sturm_below(m, ha, hb, mid) >= m) hi = mid;   // every eigenvalue below mid
-        else lo = mid;
-    }
-    return hi;
+    HZ_CHECK(memcpy_sync(h.data(), ab, sizeof(double) * 2 * steps, hipMemcpyDeviceToHost));
+    int used = 0;
+    const double theta = ritz_max(ha, hb, steps, &used);
+    if (steps_done) *steps_done = used;
+    return theta;
 }
 
 double learned_parameters(int nagg, const float *out, float *omega) {

@@ -84,7 +84,9 @@ def test_learned_scale_floor_binds(lib, libc):
     """A network that asks for bound scale 0.5 everywhere: each level's scale
     is raised to min(1, 1.1 lambda_est / Gershgorin) (the library reports both
     per level), the solve needs no fallback and meets the tolerance, and the
-    level-0 estimate is within 0.2 % of the dense eigenvalue."""
+    level-0 estimate is within 0.2 % of the dense eigenvalue.  Where the floor
+    reaches 1 the estimate may stop early: it is then a Ritz value of fewer
+    steps, at most the 30-step one and at least 1/1.1 of the bound."""
     ia, ja, va = problems.aniso3d_kuhn(12, 0.7, 1e-3)
     n = ia.size - 1
     b = problems.uniform_rhs(n, seed=1)
@@ -106,8 +108,13 @@ def test_learned_scale_floor_binds(lib, libc):
     lam, ger = capi.REAL(), capi.REAL()
     assert lib.hz_lambda_max_dinv(capi.HostCSR(ia, ja, va).ptr(), 30, C.byref(lam), C.byref(ger)) == 0
     exact = _dense_lambda_dinv(ia, ja, va) / ger.value
-    assert info.lambda_ratio[0] == pytest.approx(lam.value / ger.value, rel=1e-12)
-    assert abs(info.lambda_ratio[0] - exact) <= 2e-3 * exact
+    full = lam.value / ger.value
+    _say(f"level 0: 30-step ratio {full:.6f}, in the solve {info.lambda_ratio[0]:.6f}, dense {exact:.6f}")
+    assert abs(full - exact) <= 2e-3 * exact
+    if info.cheb_scale[0] == 1.0:
+        assert 1.0 / 1.1 * (1 - 1e-12) <= info.lambda_ratio[0] <= full * (1 + 1e-12)
+    else:
+        assert info.lambda_ratio[0] == pytest.approx(full, rel=1e-12)
 
 
 def _set_precision(lib, p):
