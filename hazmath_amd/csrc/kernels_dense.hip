@@ -142,6 +142,18 @@ __global__ __launch_bounds__(1024) void k_block_inverse(int n, int k0, int b, co
         }
 }
 
+// Row block K of the matrix after a pivot step: M[k0 + r][c] = R[r][c], the
+// pivot block's inverse over columns [k0, k0 + b).
+__global__ void k_put_pivot_rows(int n, int k0, int b, const double *__restrict__ R,
+                                 const double *__restrict__ Pinv, double *__restrict__ M) {
+    const long total = (long)b * n;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+        const int r = (int)(e / n), c = (int)(e % n);
+        const int cc = c - k0;
+        M[(size_t)(k0 + r) * n + c] = (cc >= 0 && cc < b) ? Pinv[r * b + cc] : R[(size_t)r * n + c];
+    }
+}
+
 // Copy a rows x cols block between row-major buffers.
 __global__ void k_copy_block(int rows, int cols, const double *__restrict__ src, int lds,
                              double *__restrict__ dst, int ldd, double scale) {
@@ -526,19 +538,16 @@ int blocked_inverse(int n, double *M) {
     const int g = std::min(4096, div_up((long)n * kTile, 256));
     for (int k0 = 0; k0 < n; k0 += kTile) {
         const int b = std::min(kTile, n - k0);
-        const int below = n - k0 - b;
         k_block_inverse<<<1, 1024, 0, stream()>>>(n, k0, b, M, Pinv, status);
         HZ_LAUNCH_CHECK();
         launch_gemm(mfma, b, n, b, Pinv, b, M + (size_t)k0 * n, n, R, n, 1.0, 0.0);           // R = P^-1 A_K,:
         k_copy_block<<<g, 256, 0, stream()>>>(n, b, M + k0, n, Cb, b, 1.0);                   // Cb = A_:,K
         HZ_LAUNCH_CHECK();
-        launch_gemm(mfma, k0, n, b, Cb, b, R, n, M, n, -1.0, 1.0);                            // rows above K
-        launch_gemm(mfma, below, n, b, Cb + (size_t)(k0 + b) * b, b, R, n, M + (size_t)(k0 + b) * n, n, -1.0, 1.0);
-        launch_gemm(mfma, k0, b, b, Cb, b, Pinv, b, M + k0, n, -1.0, 0.0);                    // A_i,K = -A_i,K P^-1
-        launch_gemm(mfma, below, b, b, Cb + (size_t)(k0 + b) * b, b, Pinv, b, M + (size_t)(k0 + b) * n + k0, n,
-                    -1.0, 0.0);
-        k_copy_block<<<g, 256, 0, stream()>>>(b, n, R, n, M + (size_t)k0 * n, n, 1.0);        // A_K,: = R
-        k_copy_block<<<1, 256, 0, stream()>>>(b, b, Pinv, b, M + (size_t)k0 * n + k0, n, 1.0); // A_KK = P^-1
+        // every row: A_i,: -= A_i,K R, then A_i,K = -A_i,K P^-1 (row block K is
+        // overwritten after, so it takes the same two products as the others)
+        launch_gemm(mfma, n, n, b, Cb, b, R, n, M, n, -1.0, 1.0);
+        launch_gemm(mfma, n, b, b, Cb, b, Pinv, b, M + k0, n, -1.0, 0.0);
+        k_put_pivot_rows<<<std::min(4096, div_up((long)b * n, 256)), 256, 0, stream()>>>(n, k0, b, R, Pinv, M);
         HZ_LAUNCH_CHECK();
     }
     int hs = 0;
