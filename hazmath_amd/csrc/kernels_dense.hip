@@ -142,25 +142,125 @@ __global__ __launch_bounds__(1024) void k_block_inverse(int n, int k0, int b, co
         }
 }
 
-// Row block K of the matrix after a pivot step: M[k0 + r][c] = R[r][c], the
-// pivot block's inverse over columns [k0, k0 + b).
-__global__ void k_put_pivot_rows(int n, int k0, int b, const double *__restrict__ R,
-                                 const double *__restrict__ Pinv, double *__restrict__ M) {
-    const long total = (long)b * n;
-    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
-        const int r = (int)(e / n), c = (int)(e % n);
-        const int cc = c - k0;
-        M[(size_t)(k0 + r) * n + c] = (cc >= 0 && cc < b) ? Pinv[r * b + cc] : R[(size_t)r * n + c];
+// One 64 x 64 product on the FP64 matrix cores, operands already in LDS:
+// acc (per wavefront a 32 x 32 quarter as 2 x 2 blocks of 16 x 16) = X Y over
+// k = 0..63 in steps of 4, the same order as k_gemm.
+__device__ inline void tile_mfma(const double (*X)[kTile + 1], const double (*Y)[kTile + 1], int wm, int wn,
+                                 int lane, d4 (&acc)[2][2]) {
+    for (int mi = 0; mi < 2; ++mi)
+        for (int ni = 0; ni < 2; ++ni) acc[mi][ni] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int ks = 0; ks < kTile; ks += 4) {
+        const int kq = ks + lane / 16;
+        const double a0 = X[wm + 0 + lane % 16][kq], a1 = X[wm + 16 + lane % 16][kq];
+        const double b0 = Y[kq][wn + 0 + lane % 16], b1 = Y[kq][wn + 16 + lane % 16];
+        acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
     }
 }
 
-// Copy a rows x cols block between row-major buffers.
-__global__ void k_copy_block(int rows, int cols, const double *__restrict__ src, int lds,
-                             double *__restrict__ dst, int ldd, double scale) {
-    const long total = (long)rows * cols;
+// The rest of a Gauss-Jordan panel step in one launch, after the pivot block
+// K = [k0, k0 + b) has been inverted into Pinv (b x b).  Workgroup (tm, tn)
+// owns a 64 x 64 tile:
+//   tn = K:            Ck[tm rows] = -(A_tm,K Pinv)               (new column block)
+//   tn != K, tm = K:   R[:, tn] = Pinv A_K,tn                     (new row block)
+//   tn != K, tm != K:  A_tm,tn -= A_tm,K (Pinv A_K,tn)            (in place)
+// Column block K and row block K are only read here (their new values go to
+// Ck and R), so no workgroup reads what another writes.  Each value is formed
+// as the separate products formed it: R = 0 + 1 (Pinv A_K,:), then
+// C = 1 C + (-1) acc, Ck = 0 + (-1) acc.
+__global__ __launch_bounds__(256) void k_gj_panel(int n, int k0, int b, double *__restrict__ M,
+                                                  const double *__restrict__ Pinv, double *__restrict__ R,
+                                                  double *__restrict__ Ck) {
+    __shared__ double S1[kTile][kTile + 1];
+    __shared__ double S2[kTile][kTile + 1];
+    const int tm = blockIdx.y * kTile, tn = blockIdx.x * kTile;
+    const bool colK = tn == k0, rowK = tm == k0;
+    if (colK && rowK) return;   // the pivot block: Pinv itself
+    const int tid = threadIdx.x;
+    const int w = tid / kWave, lane = tid % kWave;
+    const int wm = (w / 2) * 32, wn = (w % 2) * 32;
+    d4 acc[2][2];
+    if (colK) {
+        // S1 = A_tm,K (rows tm.., columns K), S2 = Pinv
+        for (int idx = tid; idx < kTile * kTile; idx += 256) {
+            const int r = idx / kTile, c = idx % kTile;
+            S1[r][c] = (tm + r < n && c < b) ? M[(size_t)(tm + r) * n + k0 + c] : 0.0;
+            S2[r][c] = (r < b && c < b) ? Pinv[r * b + c] : 0.0;
+        }
+        __syncthreads();
+        tile_mfma(S1, S2, wm, wn, lane, acc);
+        for (int mi = 0; mi < 2; ++mi)
+            for (int ni = 0; ni < 2; ++ni)
+                for (int q = 0; q < 4; ++q) {
+                    const int row = tm + wm + mi * 16 + lane / 16 + 4 * q;
+                    const int col = wn + ni * 16 + lane % 16;
+                    if (row < n && col < b) Ck[(size_t)row * b + col] = 0.0 + -1.0 * acc[mi][ni][q];
+                }
+        return;
+    }
+    // S1 = Pinv, S2 = A_K,tn
+    for (int idx = tid; idx < kTile * kTile; idx += 256) {
+        const int r = idx / kTile, c = idx % kTile;
+        S1[r][c] = (r < b && c < b) ? Pinv[r * b + c] : 0.0;
+        S2[r][c] = (r < b && tn + c < n) ? M[(size_t)(k0 + r) * n + tn + c] : 0.0;
+    }
+    __syncthreads();
+    tile_mfma(S1, S2, wm, wn, lane, acc);
+    if (rowK) {
+        for (int mi = 0; mi < 2; ++mi)
+            for (int ni = 0; ni < 2; ++ni)
+                for (int q = 0; q < 4; ++q) {
+                    const int row = wm + mi * 16 + lane / 16 + 4 * q;
+                    const int col = tn + wn + ni * 16 + lane % 16;
+                    if (row < b && col < n) R[(size_t)row * n + col] = 0.0 + 1.0 * acc[mi][ni][q];
+                }
+        return;
+    }
+    __syncthreads();   // every wavefront is done with S1 and S2
+    // S2 = the R tile (rows k < b, zero beyond), S1 = A_tm,K
+    for (int mi = 0; mi < 2; ++mi)
+        for (int ni = 0; ni < 2; ++ni)
+            for (int q = 0; q < 4; ++q) {
+                const int row = wm + mi * 16 + lane / 16 + 4 * q;
+                const int col = wn + ni * 16 + lane % 16;
+                S2[row][col] = row < b ? 0.0 + 1.0 * acc[mi][ni][q] : 0.0;
+            }
+    for (int idx = tid; idx < kTile * kTile; idx += 256) {
+        const int r = idx / kTile, c = idx % kTile;
+        S1[r][c] = (tm + r < n && c < b) ? M[(size_t)(tm + r) * n + k0 + c] : 0.0;
+    }
+    __syncthreads();
+    tile_mfma(S1, S2, wm, wn, lane, acc);
+    for (int mi = 0; mi < 2; ++mi)
+        for (int ni = 0; ni < 2; ++ni)
+            for (int q = 0; q < 4; ++q) {
+                const int row = tm + wm + mi * 16 + lane / 16 + 4 * q;
+                const int col = tn + wn + ni * 16 + lane % 16;
+                if (row < n && col < n) {
+                    double *c = &M[(size_t)row * n + col];
+                    *c = 1.0 * *c + -1.0 * acc[mi][ni][q];
+                }
+            }
+}
+
+// Row and column block K after a panel step: M[k0 + r][c] = R[r][c] outside
+// K and Pinv inside it; M[i][k0 + c] = Ck[i][c] for the rows outside K.
+__global__ void k_put_pivot_blocks(int n, int k0, int b, const double *__restrict__ R,
+                                   const double *__restrict__ Pinv, const double *__restrict__ Ck,
+                                   double *__restrict__ M) {
+    const long rows_part = (long)b * n, total = 2 * rows_part;
     for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
-        const int r = (int)(e / cols), c = (int)(e % cols);
-        dst[(size_t)r * ldd + c] = scale * src[(size_t)r * lds + c];
+        if (e < rows_part) {
+            const int r = (int)(e / n), c = (int)(e % n);
+            const int cc = c - k0;
+            M[(size_t)(k0 + r) * n + c] = (cc >= 0 && cc < b) ? Pinv[r * b + cc] : R[(size_t)r * n + c];
+        } else {
+            const long f = e - rows_part;
+            const int i = (int)(f / b), c = (int)(f % b);
+            if (i < k0 || i >= k0 + b) M[(size_t)i * n + k0 + c] = Ck[(size_t)i * b + c];
+        }
     }
 }
 
@@ -529,25 +629,18 @@ void require_mfma() {
 // In-place inverse of the dense n x n matrix M by 64-column block Gauss-Jordan.
 int blocked_inverse(int n, double *M) {
     require_mfma();
-    const bool mfma = true;
     double *Pinv = arena().alloc_n<double>((size_t)kTile * kTile);
     double *R = arena().alloc_n<double>((size_t)kTile * n);
-    double *Cb = arena().alloc_n<double>((size_t)n * kTile);
+    double *Ck = arena().alloc_n<double>((size_t)n * kTile);
     int *status = arena().alloc_n<int>(1);
     HZ_CHECK(hipMemsetAsync(status, 0, sizeof(int), stream()));
-    const int g = std::min(4096, div_up((long)n * kTile, 256));
     for (int k0 = 0; k0 < n; k0 += kTile) {
         const int b = std::min(kTile, n - k0);
         k_block_inverse<<<1, 1024, 0, stream()>>>(n, k0, b, M, Pinv, status);
         HZ_LAUNCH_CHECK();
-        launch_gemm(mfma, b, n, b, Pinv, b, M + (size_t)k0 * n, n, R, n, 1.0, 0.0);           // R = P^-1 A_K,:
-        k_copy_block<<<g, 256, 0, stream()>>>(n, b, M + k0, n, Cb, b, 1.0);                   // Cb = A_:,K
-        HZ_LAUNCH_CHECK();
-        // every row: A_i,: -= A_i,K R, then A_i,K = -A_i,K P^-1 (row block K is
-        // overwritten after, so it takes the same two products as the others)
-        launch_gemm(mfma, n, n, b, Cb, b, R, n, M, n, -1.0, 1.0);
-        launch_gemm(mfma, n, b, b, Cb, b, Pinv, b, M + k0, n, -1.0, 0.0);
-        k_put_pivot_rows<<<std::min(4096, div_up((long)b * n, 256)), 256, 0, stream()>>>(n, k0, b, R, Pinv, M);
+        // R = P^-1 A_K,:, A_i,: -= A_i,K R and A_i,K = -A_i,K P^-1 in one launch, then rows and columns K
+        k_gj_panel<<<dim3(div_up(n, kTile), div_up(n, kTile)), 256, 0, stream()>>>(n, k0, b, M, Pinv, R, Ck);
+        k_put_pivot_blocks<<<std::min(4096, div_up(2L * b * n, 256)), 256, 0, stream()>>>(n, k0, b, R, Pinv, Ck, M);
         HZ_LAUNCH_CHECK();
     }
     int hs = 0;
