@@ -651,7 +651,7 @@ forward:
         // the next level's first sweep from zero rides on the restriction (UA, Jacobi / L1)
         const bool fuse = (l + 1 < nl - 1) && cp.presmooth > 0 && cp.smoother != HZ_SMOOTHER_CHEBY && !L.sa;
         if (L.sa) {
-            residual_m(L.S, L.low, L.bf, L.xf, L.wf);
+            residual_m(L.S, L.low, L.bf, L.xf, L.wf, &L.A);
             csr_mxv_m(L.R, L.Rf, L.wf, C.bf);   // dcsr_mxv(R), mgcycle.c:302
         } else if (L.A.n <= fused_down_max_rows()) {
             down_fused_m(L.nagg, L.rptr, L.rmem, L.A, L.low, L.bf, L.xf, C.bf, cp.smoother, w,

@@ -296,7 +296,9 @@ void enter_m(int n, const double *r, int smoother, float w, const float *d, floa
 void sweep_m(const DevSELL &S, const LowMatrix &M, int smoother, float w, const int *agg, const float *xc,
              const float *b, const float *x, float *xout, double *out64, const double *r64, double *dot_out,
              const ReduceWs &ws);
-void residual_m(const DevSELL &S, const LowMatrix &M, const float *b, const float *x, float *w);
+// A (optional): the level's CSR; small levels with long rows then use it.
+void residual_m(const DevSELL &S, const LowMatrix &M, const float *b, const float *x, float *w,
+                const DevCSR *A = nullptr);
 void restrict_m(int nc, int n, const int *rptr, const int *rmem, const float *w, float *bc, int smoother, float wgt,
                 const float *dc, float *xc);
 void down_fused_m(int nc, const int *rptr, const int *rmem, const DevCSR &A, const LowMatrix &M, const float *b,

@@ -431,19 +431,34 @@ __global__ void k_rapw_big(int nc, const int *__restrict__ rowcount, int *__rest
 template <int kLanes>
 __global__ void k_csr_mxv(int n, const int *__restrict__ ia, const int *__restrict__ ja,
                           const double *__restrict__ val, const double *__restrict__ x, double *__restrict__ y) {
+    constexpr int kPass = 4;   // rounds whose loads are issued together
     const long gt = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int i = (int)(gt / kLanes), j = (int)(gt % kLanes);
     const int gbase = threadIdx.x % kWave - j;
     const bool valid = i < n;
     const int k0 = valid ? ia[i] : 0, k1 = valid ? ia[i + 1] : 0;
     double t = 0.0;
-    for (int kb = k0; kb < k1; kb += kLanes) {   // trip count uniform within the lane group
-        const int k = kb + j;
-        const double pr = k < k1 ? val[k] * x[ja[k]] : 0.0;
+    for (int kp = k0; kp < k1; kp += kLanes * kPass) {   // trip count uniform within the lane group
+        double av[kPass];
+        int cj[kPass];
 #pragma unroll
-        for (int u = 0; u < kLanes; ++u) {
-            const double v = __shfl(pr, gbase + u, kWave);
-            if (j == 0 && kb + u < k1) t = t + v;
+        for (int q = 0; q < kPass; ++q) {
+            const int k = kp + q * kLanes + j;
+            av[q] = k < k1 ? val[k] : 0.0;
+            cj[q] = k < k1 ? ja[k] : -1;
+        }
+        double pr[kPass];
+#pragma unroll
+        for (int q = 0; q < kPass; ++q) pr[q] = cj[q] >= 0 ? av[q] * x[cj[q]] : 0.0;
+#pragma unroll
+        for (int q = 0; q < kPass; ++q) {
+            const int kb = kp + q * kLanes;
+            if (kb >= k1) break;   // uniform within the lane group
+#pragma unroll
+            for (int u = 0; u < kLanes; ++u) {
+                const double v = __shfl(pr[q], gbase + u, kWave);
+                if (j == 0 && kb + u < k1) t = t + v;
+            }
         }
     }
     if (valid && j == 0) y[i] = t;
@@ -453,10 +468,23 @@ __global__ void k_csr_mxv(int n, const int *__restrict__ ia, const int *__restri
 __global__ void k_csr_aAxpy(int n, const int *__restrict__ ia, const int *__restrict__ ja,
                             const double *__restrict__ val, const double *__restrict__ xc, double alpha,
                             const double *__restrict__ alpha_dev, double *__restrict__ x) {
+    constexpr int kPass = 8;   // entries whose loads are issued together
     const double a = alpha_dev ? *alpha_dev : alpha;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int k0 = ia[i], k1 = ia[i + 1];
         double t = 0.0;
-        for (int k = ia[i]; k < ia[i + 1]; ++k) t += val[k] * xc[ja[k]];
+        for (int kb = k0; kb < k1; kb += kPass) {
+            double av[kPass];
+            int cj[kPass];
+#pragma unroll
+            for (int u = 0; u < kPass; ++u) {
+                av[u] = kb + u < k1 ? val[kb + u] : 0.0;
+                cj[u] = kb + u < k1 ? ja[kb + u] : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < kPass; ++u)
+                if (cj[u] >= 0) t += av[u] * xc[cj[u]];
+        }
         if (a == 1.0) x[i] += t;
         else if (a == -1.0) x[i] -= t;
         else x[i] += t * a;
@@ -639,7 +667,7 @@ void csr_mxv(const DevCSR &A, const double *x, double *y) {
 void csr_prolong(const DevCSR &P, const double *xc, double alpha, const double *alpha_dev, double *x) {
     count_bytes(matrix_bytes(P.nnz, P.n) + 8.0 * P.ncol + 16.0 * P.n);
     if (P.n == 0) return;
-    k_csr_aAxpy<<<grid_for(P.n), kBlock, 0, stream()>>>(P.n, P.ia, P.ja, P.val, xc, alpha, alpha_dev, x);
+    k_csr_aAxpy<<<div_up(P.n, kBlock), kBlock, 0, stream()>>>(P.n, P.ia, P.ja, P.val, xc, alpha, alpha_dev, x);
     HZ_LAUNCH_CHECK();
 }
 
