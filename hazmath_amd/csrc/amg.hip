@@ -414,10 +414,13 @@ void presmooth(Hierarchy &H, int l, CycleState &cs) {
     const CycleParams &cp = H.cp;
     int sweeps = cp.presmooth;
     if (cp.smoother == HZ_SMOOTHER_CHEBY || is_gs_family(cp.smoother)) {
-        if (cs.zero[l]) fill(L.x, 0.0, L.A.n);
+        // Chebyshev from zero: its first step reads no x (nor the matrix)
+        const bool cheb_from_zero = cp.smoother == HZ_SMOOTHER_CHEBY && cs.zero[l] && sweeps > 0;
+        if (cs.zero[l] && !cheb_from_zero) fill(L.x, 0.0, L.A.n);
         cs.zero[l] = false;
         cs.first_done[l] = false;
-        if (cp.smoother == HZ_SMOOTHER_CHEBY) cheby_apply(L.S, L.cheb, sweeps, L.b, L.x, L.xt, L.cwork, &L.A);
+        if (cp.smoother == HZ_SMOOTHER_CHEBY)
+            cheby_apply(L.S, L.cheb, sweeps, L.b, L.x, L.xt, L.cwork, &L.A, cheb_from_zero);
         else gs_smooth(L.gs, cp.smoother, false, sweeps, cp.relax, L.b, L.x);
         return;
     }
@@ -569,11 +572,13 @@ void presmooth_m(Hierarchy &H, int l, CycleState &cs) {
     const float w = (float)cp.jacobi_w;
     int sweeps = cp.presmooth;
     if (cp.smoother == HZ_SMOOTHER_CHEBY) {
-        if (cs.zero[l]) fill_f(L.xf, L.A.n);
+        // from zero, the first step reads no x (nor the matrix)
+        const bool from_zero = cs.zero[l] && sweeps > 0;
+        if (cs.zero[l] && !from_zero) fill_f(L.xf, L.A.n);
         cs.zero[l] = false;
         cs.first_done[l] = false;
         cheby_apply_m(L.S, L.A, L.low, L.cheb, L.cdiagf, sweeps, L.bf, L.xf, L.xtf, L.cworkf, nullptr, nullptr,
-                      nullptr, H.ws);
+                      nullptr, H.ws, from_zero);
         return;
     }
     if (cs.zero[l]) {

@@ -225,8 +225,9 @@ struct ChebLevel {
 ChebLevel cheby_prepare(const DevCSR &A, double scale);
 // `steps` Chebyshev steps from x (x and xt swap roles each step; x ends on the result)
 // A (optional): the level's CSR, used on small levels with long rows.
+// x_zero: x is zero (its memory need not hold it); the first step then reads no matrix.
 void cheby_apply(const DevSELL &S, const ChebLevel &c, int steps, const double *b, double *&x, double *&xt,
-                 double *dwork, const DevCSR *A = nullptr);
+                 double *dwork, const DevCSR *A = nullptr, bool x_zero = false);
 // network outputs [nagg][2] -> omega[c] = clamp(out[c][0], 0, 1.6); returns
 // the level's Chebyshev bound scale clamp(mean_c out[c][1], 0.2, 1.5)
 double learned_parameters(int nagg, const float *out, float *omega);
@@ -280,9 +281,10 @@ float *values_f32(const DevCSR &A);
 // `steps` Chebyshev steps in FP32 on the level's low-precision values (x and
 // xt swap roles each step); with out64 the last step writes x + d to out64 in
 // FP64 instead, with (out64, r64) summed into *dot_out when dot_out is set.
+// x_zero: as for cheby_apply.
 void cheby_apply_m(const DevSELL &S, const DevCSR &A, const LowMatrix &M, const ChebLevel &c, const float *diag,
                    int steps, const float *b, float *&x, float *&xt, float *dwork, double *out64, const double *r64,
-                   double *dot_out, const ReduceWs &ws);
+                   double *dot_out, const ReduceWs &ws, bool x_zero = false);
 // y = A x and x += P xc in FP32 with FP32 values (smoothed aggregation transfers)
 void csr_mxv_m(const DevCSR &A, const float *val, const float *x, float *y);
 void csr_prolong_m(const DevCSR &P, const float *val, const float *xc, float *x);

@@ -100,6 +100,19 @@ __global__ __launch_bounds__(kBlock) void k_cheb_step(int n, int nslices, const 
     }
 }
 
+// The first step from x = 0: b - A x is b, so d = 0 + c2 b / diag and
+// xout = 0 + d, the values the full step stores when it reads x = 0 (the
+// leading 0 + turns a -0 into +0 whatever the products were).
+__global__ void k_cheb_first(int n, const double *__restrict__ diag, double c2, const double *__restrict__ b,
+                             double *__restrict__ xout, double *__restrict__ d) {
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+        const double di = diag[r];
+        const double dn = 0.0 + (di != 0.0 ? c2 * b[r] / di : 0.0);
+        d[r] = dn;
+        xout[r] = 0.0 + dn;
+    }
+}
+
 // The same step over CSR with kLanes lanes per row, for small levels with long
 // rows: each round the lanes load kLanes consecutive entries and form the
 // products a*x, and the group's first lane subtracts them from t = b in CSR
@@ -227,7 +240,7 @@ ChebLevel cheby_prepare(const DevCSR &A, double scale) {
 }
 
 void cheby_apply(const DevSELL &S, const ChebLevel &c, int steps, const double *b, double *&x, double *&xt,
-                 double *dwork, const DevCSR *A) {
+                 double *dwork, const DevCSR *A, bool x_zero) {
     if (S.n == 0 || steps <= 0) return;
     const double theta = 0.5 * (c.lmax + c.lmin), delta = 0.5 * (c.lmax - c.lmin);
     const double sigma = theta / delta;
@@ -243,6 +256,13 @@ void cheby_apply(const DevSELL &S, const ChebLevel &c, int steps, const double *
             c1 = rn * rho;
             c2 = 2.0 * rn / delta;
             rho = rn;
+        }
+        if (k == 0 && x_zero) {
+            count_bytes(32.0 * S.n);
+            k_cheb_first<<<div_up(S.n, kBlock), kBlock, 0, stream()>>>(S.n, c.diag, c2, b, xt, dwork);
+            HZ_LAUNCH_CHECK();
+            std::swap(x, xt);
+            continue;
         }
         count_bytes(matrix_bytes(S.nnz, S.n) + 48.0 * S.n);
         if (grp)

@@ -393,6 +393,18 @@ __global__ void k_residual_grp_m(int n, const int *__restrict__ ia, const int *_
     if (valid && j == 0) w[r] = b[r] - acc;
 }
 
+// The first Chebyshev step from x = 0: b - A x is b, so d = 0 + c2 b / diag
+// and xout = 0 + d, the values the full step stores when it reads x = 0.
+__global__ void k_cheb_first_m(int n, const float *__restrict__ diag, float c2, const float *__restrict__ b,
+                               float *__restrict__ xout, float *__restrict__ d) {
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+        const float di = diag[r];
+        const float dn = 0.0f + (di != 0.0f ? c2 * b[r] / di : 0.0f);
+        d[r] = dn;
+        xout[r] = 0.0f + dn;
+    }
+}
+
 // The Chebyshev step over the CSR copy with kLanes lanes per row (small levels
 // with long rows, latency-bound).
 template <int kLanes, class MT>
@@ -657,7 +669,7 @@ void prolong_m(int n, const int *agg, const float *xc, float *x) {
 
 void cheby_apply_m(const DevSELL &S, const DevCSR &A, const LowMatrix &M, const ChebLevel &c, const float *diag,
                    int steps, const float *b, float *&x, float *&xt, float *dwork, double *out64, const double *r64,
-                   double *dot_out, const ReduceWs &ws) {
+                   double *dot_out, const ReduceWs &ws, bool x_zero) {
     if (S.n == 0 || steps <= 0) return;
     const double theta = 0.5 * (c.lmax + c.lmin), delta = 0.5 * (c.lmax - c.lmin);
     const double sigma = theta / delta;
@@ -674,6 +686,13 @@ void cheby_apply_m(const DevSELL &S, const DevCSR &A, const LowMatrix &M, const 
             rho = rn;
         }
         const bool last64 = out64 && k == steps - 1;
+        if (k == 0 && x_zero && !last64) {
+            count_bytes(16.0 * S.n);
+            k_cheb_first_m<<<div_up(S.n, kBlock), kBlock, 0, stream()>>>(S.n, diag, (float)c2, b, xt, dwork);
+            HZ_LAUNCH_CHECK();
+            std::swap(x, xt);
+            continue;
+        }
         count_bytes(matrix_bytes(S.nnz, S.n, M.kind == kLowF32 ? 4 : 2) + 24.0 * S.n + (last64 ? 16.0 * S.n : 0.0));
         if (last64) {
             if (M.kind == kLowF32)
