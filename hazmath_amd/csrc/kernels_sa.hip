@@ -233,9 +233,11 @@ __global__ void k_rap_rows(int r0, int r1, const int *__restrict__ rows, const l
 // the rows it cannot take (more than max_cols columns, or one A row longer
 // than a stage); the per-thread kernel takes those and the fill pass skips
 // them.
-constexpr int kRapwPairs = 512;
+// 256 pairs and 256 columns keep the LDS at 12.8 KB: twelve single-wavefront
+// workgroups per CU hide the rounds' gathers from P (with 512 / 512, six).
+constexpr int kRapwPairs = 256;
 constexpr double kRapSlotBytes = 4e9;   // slots of the one-pass form: at most 4 GB
-constexpr int kRapwCols = 512;
+constexpr int kRapwCols = 256;
 constexpr int kRapwHash = 2 * kRapwCols;   // at most kRapwCols + 64 keys: probing ends
 
 __device__ __forceinline__ unsigned rapw_hash(int key) {
