@@ -359,7 +359,7 @@ __global__ __launch_bounds__(kBlock) void k_cheb_step_m(int n, int nslices, cons
 template <int kLanes, class MT>
 __device__ inline float grp_row_dot(int k0, int k1, int j, const int *__restrict__ ja, const MT *__restrict__ a,
                                     const float *__restrict__ x) {
-    constexpr int kPass = 4;
+    constexpr int kPass = 64 / kLanes;
     float acc = 0.0f;
     for (int kb = k0 + j; kb < k1; kb += kLanes * kPass) {
         float av[kPass];
