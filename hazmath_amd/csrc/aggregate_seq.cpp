@@ -43,7 +43,7 @@ int aggregate_host(const DevCSR &N, int type, int max_aggregation, int *agg_dev,
     double *a = static_cast<double *>(g_a.get(sizeof(double) * (size_t)N.nnz));
     hipStream_t st = stream();
     HZ_CHECK(hipMemcpyAsync(ia, N.ia, sizeof(int) * (n + 1), hipMemcpyDeviceToHost, st));
-    std::vector<int> v;
+    static std::vector<int> v;   // kept across calls: no fresh pages per call
     int st_code;
     if (type == HZ_VMB) {
         // VMB's first pass walks the rows downwards: the column indices come
@@ -79,9 +79,22 @@ int aggregate_host(const DevCSR &N, int type, int max_aggregation, int *agg_dev,
         RowBlocksDown arrival;
         arrival.nblocks = nb;
         arrival.lo = lo;
-        arrival.wait = [](int b) { HZ_CHECK(hipEventSynchronize(ev[b])); };
-        st_code = vmb_sweep(n, ia, ja, max_aggregation, v, *nagg, &arrival);
+        static double waited = 0.0;
+        waited = 0.0;
+        const bool tr = trace_enabled();
+        arrival.wait = [tr](int b) {
+            const double w0 = tr ? wall_ms() : 0.0;
+            HZ_CHECK(hipEventSynchronize(ev[b]));
+            if (tr) waited += wall_ms() - w0;
+        };
+        double pass_ms[3] = {0, 0, 0};
+        const double s0 = tr ? wall_ms() : 0.0;
+        st_code = vmb_sweep(n, ia, ja, max_aggregation, v, *nagg, &arrival, tr ? pass_ms : nullptr);
         HZ_CHECK(hipStreamSynchronize(st));
+        if (tr)
+            std::fprintf(stderr, "[hz]   VMB n=%d nnz=%lld: sweep %.2f ms (pass 1 %.2f, of which waiting for the graph "
+                         "%.2f; pass 2 %.2f; pass 3 %.2f)\n", n, total, wall_ms() - s0, pass_ms[0], waited,
+                         pass_ms[1], pass_ms[2]);
     } else {
         if (N.nnz) {
             HZ_CHECK(hipMemcpyAsync(ja, N.ja, sizeof(int) * N.nnz, hipMemcpyDeviceToHost, st));

@@ -7,6 +7,7 @@
 #include "aggregate_sweeps.h"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 
@@ -15,9 +16,17 @@
 namespace hz {
 
 int vmb_sweep(int n, const int *ia, const int *ja, int maxagg, std::vector<int> &v, int &nagg,
-              const RowBlocksDown *arrival) {
+              const RowBlocksDown *arrival, double *ms) {
+    const auto now = [] {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+    };
+    const double t0 = ms ? now() : 0.0;
     int left = n, na = 0;
     v.assign(n, -2);
+    // aggregate sizes as pass 1 forms them (the vertices that take the id),
+    // buffers kept across calls
+    static std::vector<int> size;
+    size.clear();
     int blk = 0, here = n;   // rows >= here have arrived
     for (int i = n - 1; i >= 0; --i) {
         while (arrival && i < here && blk < arrival->nblocks) {
@@ -33,34 +42,40 @@ int vmb_sweep(int n, const int *ia, const int *ja, int maxagg, std::vector<int> 
         for (int k = ia[i]; k < ia[i + 1] && untouched; ++k)
             if (v[ja[k]] >= HZ_UNPT) untouched = false;
         if (!untouched) continue;
-        int cnt = 1;
+        int cnt = 1, members = 1;
         v[i] = na;
         --left;
         for (int k = ia[i]; k < ia[i + 1]; ++k)
             if (ja[k] != i && cnt < maxagg) {
+                members += v[ja[k]] != na;
                 v[ja[k]] = na;
                 --left;
                 ++cnt;
             }
+        size.push_back(members);
         ++na;
     }
     nagg = na;
+    const double t1 = ms ? now() : 0.0;
     if (na < 1) return HZ_ERROR_AMG_COARSEING;
-    std::vector<int> c1(v), size(na, 0);
-    for (int i = 0; i < n; ++i)
-        if (v[i] >= 0) size[v[i]]++;
+    // pass 2 reads only pass 1's map: its own assignments are collected and
+    // written after it (each vertex is visited once, so none is read back)
+    static std::vector<std::pair<int, int>> joins;
+    joins.clear();
     for (int i = n - 1; i >= 0; --i) {
         if (v[i] >= HZ_UNPT) continue;
         for (int k = ia[i]; k < ia[i + 1]; ++k) {
-            const int a = c1[ja[k]];
+            const int a = v[ja[k]];
             if (a > HZ_UNPT && size[a] < maxagg) {
-                v[i] = a;
+                joins.emplace_back(i, a);
                 --left;
                 size[a]++;
                 break;
             }
         }
     }
+    for (const auto &j : joins) v[j.first] = j.second;
+    const double t2 = ms ? now() : 0.0;
     while (left > 0) {
         for (int i = 0; i < n; ++i) {
             if (v[i] >= HZ_UNPT) continue;
@@ -77,6 +92,11 @@ int vmb_sweep(int n, const int *ia, const int *ja, int maxagg, std::vector<int> 
         }
     }
     nagg = na;
+    if (ms) {
+        ms[0] = t1 - t0;
+        ms[1] = t2 - t1;
+        ms[2] = now() - t2;
+    }
     return HZ_SUCCESS;
 }
 

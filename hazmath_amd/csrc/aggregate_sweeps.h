@@ -19,8 +19,9 @@ struct RowBlocksDown {
 // aggregation_vmb (amg_setup_ua.c:596-727) on the strength graph (ia, ja) of n
 // rows: v[i] = aggregate of i or -1 (UNPT); returns SUCCESS.  arrival: ja
 // arrives in row blocks (null: all there).
+// ms (optional): wall milliseconds of the three passes.
 int vmb_sweep(int n, const int *ia, const int *ja, int maxagg, std::vector<int> &v, int &nagg,
-              const RowBlocksDown *arrival = nullptr);
+              const RowBlocksDown *arrival = nullptr, double *ms = nullptr);
 // aggregation_hem (amg_setup_ua.c:913-1051) on the strength graph (ia, ja, a):
 // SUCCESS, or -1017 (the reference's exit(17), its message printed) when a
 // leftover has no aggregated neighbour.
