@@ -33,15 +33,28 @@ __global__ __launch_bounds__(kBlock) void k_cheb_diag(int n, const int *__restri
                                                       const double *__restrict__ a, double *__restrict__ d,
                                                       double *__restrict__ part) {
     __shared__ double sh[kBlock / kWave];
+    constexpr int U = 16;   // entries whose loads are issued together
     double m = 0.0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         double di = 0.0, s = 0.0;
         bool found = false;
-        for (int k = ia[i]; k < ia[i + 1]; ++k) {
-            s += fabs(a[k]);
-            if (!found && ja[k] == i) {
-                di = a[k];
-                found = true;
+        const int k1 = ia[i + 1];
+        for (int kb = ia[i]; kb < k1; kb += U) {
+            int jj[U];
+            double vv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                jj[u] = kb + u < k1 ? ja[kb + u] : -1;
+                vv[u] = kb + u < k1 ? a[kb + u] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (jj[u] < 0) continue;
+                s += fabs(vv[u]);
+                if (!found && jj[u] == i) {
+                    di = vv[u];
+                    found = true;
+                }
             }
         }
         d[i] = di;
@@ -158,28 +171,43 @@ __global__ void k_cheb_step_grp(int n, const int *__restrict__ ia, const int *__
 __global__ void k_row_stats(int n, const int *__restrict__ ia, const int *__restrict__ ja,
                             const double *__restrict__ a, const int *__restrict__ agg, double *__restrict__ rs) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        constexpr int U = 16;   // entries whose loads are issued together
         double di = 0.0, s = 0.0, q = 0.0, mx = 0.0, mn = 0.0, pos = 0.0, in = 0.0;
         bool found = false;
         const int gi = agg[i];
-        for (int k = ia[i]; k < ia[i + 1]; ++k) {
-            const int j = ja[k];
-            const double v = a[k];
-            if (j == i) {
-                if (!found) {
-                    di = v;
-                    found = true;
+        const int k1 = ia[i + 1];
+        for (int kb = ia[i]; kb < k1; kb += U) {
+            int jj[U], gj[U];
+            double vv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                jj[u] = kb + u < k1 ? ja[kb + u] : -1;
+                vv[u] = kb + u < k1 ? a[kb + u] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) gj[u] = (jj[u] >= 0 && gi > HZ_UNPT) ? agg[jj[u]] : HZ_UNPT - 1;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int j = jj[u];
+                if (j < 0) continue;
+                const double v = vv[u];
+                if (j == i) {
+                    if (!found) {
+                        di = v;
+                        found = true;
+                    }
+                    continue;
                 }
-                continue;
+                const double av = fabs(v);
+                s += av;
+                q += v;
+                if (v > 0.0) pos += v;
+                if (av > 0.0) {
+                    mx = fmax(mx, av);
+                    mn = (mn == 0.0) ? av : fmin(mn, av);
+                }
+                if (gi > HZ_UNPT && gj[u] == gi) in += av;
             }
-            const double av = fabs(v);
-            s += av;
-            q += v;
-            if (v > 0.0) pos += v;
-            if (av > 0.0) {
-                mx = fmax(mx, av);
-                mn = (mn == 0.0) ? av : fmin(mn, av);
-            }
-            if (gi > HZ_UNPT && agg[j] == gi) in += av;
         }
         double *o = rs + (size_t)i * 9;
         const double ad = fabs(di);
@@ -280,7 +308,7 @@ void cheby_apply(const DevSELL &S, const ChebLevel &c, int steps, const double *
 void aggregate_features(const DevCSR &A, const int *agg, int nagg, const int *rptr, const int *rmem, int level,
                         float *feat) {
     double *rs = arena().alloc_n<double>((size_t)A.n * 9 + 1);
-    k_row_stats<<<grid_for(A.n), kBlock, 0, stream()>>>(A.n, A.ia, A.ja, A.val, agg, rs);
+    k_row_stats<<<div_up(A.n, kBlock), kBlock, 0, stream()>>>(A.n, A.ia, A.ja, A.val, agg, rs);
     k_agg_features<<<grid_for(nagg), kBlock, 0, stream()>>>(nagg, rptr, rmem, rs, level, feat);
     HZ_LAUNCH_CHECK();
 }

@@ -32,13 +32,35 @@ inline int grid_for(long n) {
 __global__ void k_sa_filter(int n, const int *__restrict__ aia, const int *__restrict__ aja,
                             const double *__restrict__ aval, const int *__restrict__ nia,
                             const int *__restrict__ nja, double *__restrict__ nval) {
+    constexpr int U = 16;   // entries whose loads are issued together
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         double sa = 0.0, sn = 0.0;
-        for (int k = aia[i]; k < aia[i + 1]; ++k)
-            if (aja[k] != i) sa += aval[k];
-        for (int k = nia[i]; k < nia[i + 1]; ++k)
-            if (nja[k] != i) sn += nval[k];
-        for (int k = nia[i]; k < nia[i + 1]; ++k)
+        const int a1 = aia[i + 1], n0 = nia[i], n1 = nia[i + 1];
+        for (int kb = aia[i]; kb < a1; kb += U) {
+            int jj[U];
+            double vv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                jj[u] = kb + u < a1 ? aja[kb + u] : -1;
+                vv[u] = kb + u < a1 ? aval[kb + u] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (jj[u] >= 0 && jj[u] != i) sa += vv[u];
+        }
+        for (int kb = n0; kb < n1; kb += U) {
+            int jj[U];
+            double vv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                jj[u] = kb + u < n1 ? nja[kb + u] : -1;
+                vv[u] = kb + u < n1 ? nval[kb + u] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (jj[u] >= 0 && jj[u] != i) sn += vv[u];
+        }
+        for (int k = n0; k < n1; ++k)
             if (nja[k] == i) nval[k] += sa - sn;
     }
 }
@@ -64,10 +86,32 @@ __device__ __forceinline__ bool first_in_row(const int *ja, const int *agg, int 
     return true;
 }
 
+// Rows of at most kRowReg entries are held in registers by k_sa_count and
+// k_sa_fill (their loads issued together); longer rows take the loops over
+// memory.  Both forms make the same decisions and sums in the same order.
+constexpr int kRowReg = 16;
+
 __global__ void k_sa_count(int n, const int *__restrict__ ia, const int *__restrict__ ja,
                            const int *__restrict__ agg, int *__restrict__ cnt) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const int b = ia[i], e = ia[i + 1];
+        if (e - b <= kRowReg) {
+            int cc[kRowReg];
+#pragma unroll
+            for (int u = 0; u < kRowReg; ++u) cc[u] = b + u < e ? ja[b + u] : -1;
+#pragma unroll
+            for (int u = 0; u < kRowReg; ++u) cc[u] = cc[u] >= 0 ? agg[cc[u]] : HZ_UNPT;
+            int c0 = 0;
+#pragma unroll
+            for (int q = 0; q < kRowReg; ++q) {
+                bool first = cc[q] > HZ_UNPT;
+#pragma unroll
+                for (int p = 0; p < q; ++p) first = first && cc[p] != cc[q];
+                c0 += first;
+            }
+            cnt[i] = c0;
+            continue;
+        }
         int c0 = 0;
         for (int q = b; q < e; ++q) {
             const int c = agg[ja[q]];
@@ -91,6 +135,35 @@ __global__ void k_sa_fill(int n, const int *__restrict__ ia, const int *__restri
         // per-aggregate damping: row i uses its own aggregate's value
         const double omega = (omega_agg && agg[i] > HZ_UNPT) ? (double)omega_agg[agg[i]] : omega_all;
         int pos = pia[i];
+        if (e - b <= kRowReg) {
+            int jj[kRowReg], cc[kRowReg];
+            double sv[kRowReg];
+#pragma unroll
+            for (int u = 0; u < kRowReg; ++u) {
+                jj[u] = b + u < e ? ja[b + u] : -1;
+                sv[u] = b + u < e ? mval[b + u] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < kRowReg; ++u) {
+                cc[u] = jj[u] >= 0 ? agg[jj[u]] : HZ_UNPT;
+                sv[u] = (jj[u] == i) ? 1 - omega * sv[u] / di : -omega * sv[u] / di;
+            }
+#pragma unroll
+            for (int q = 0; q < kRowReg; ++q) {
+                bool first = cc[q] > HZ_UNPT;
+#pragma unroll
+                for (int p = 0; p < q; ++p) first = first && cc[p] != cc[q];
+                if (!first) continue;
+                double s = 0.0;
+#pragma unroll
+                for (int p = q; p < kRowReg; ++p)
+                    if (cc[p] == cc[q]) s += sv[p] * 1.0;
+                pja[pos] = cc[q];
+                pval[pos] = s;
+                ++pos;
+            }
+            continue;
+        }
         for (int q = b; q < e; ++q) {
             const int c = agg[ja[q]];
             if (c <= HZ_UNPT || !first_in_row(ja, agg, b, q, c)) continue;
