@@ -34,6 +34,72 @@ struct PinnedStage {
     }
 };
 PinnedStage g_ia, g_ja, g_a, g_v, g_small;
+
+// VMB's second pass on the GPU, from pass 1's map in agg (-2: not yet
+// aggregated): every such vertex takes the first neighbour that is in a pass-1
+// aggregate.  The sequential pass makes exactly these choices when no
+// aggregate passes the cap with all of them (each then has room when its turn
+// comes); flag bit 1: some aggregate would, bit 2: some vertex has no such
+// neighbour (pass 3 would run) -- the host pass runs instead.
+constexpr int kVmbGpuPass2Rows = 65536;
+constexpr int kVmbBlock = 256;
+
+__global__ void k_vmb2_first(int n, const int *__restrict__ ia, const int *__restrict__ ja,
+                             const int *__restrict__ agg, int *__restrict__ cand, int *__restrict__ count,
+                             int *__restrict__ flag) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        int c = HZ_UNPT - 2;   // aggregated in pass 1
+        if (agg[i] == HZ_UNPT - 1) {
+            c = HZ_UNPT;
+            for (int k = ia[i]; k < ia[i + 1]; ++k) {
+                const int a = agg[ja[k]];
+                if (a > HZ_UNPT) {
+                    c = a;
+                    break;
+                }
+            }
+            if (c > HZ_UNPT) atomicAdd(&count[c], 1);
+            else atomicOr(flag, 2);
+        }
+        cand[i] = c;
+    }
+}
+
+__global__ void k_vmb2_caps(int na, const int *__restrict__ size, const int *__restrict__ count, int maxagg,
+                            int *__restrict__ flag) {
+    for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < na; a += gridDim.x * blockDim.x)
+        if (size[a] + count[a] > maxagg) atomicOr(flag, 1);
+}
+
+__global__ void k_vmb2_join(int n, const int *__restrict__ cand, int *__restrict__ agg) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        if (cand[i] > HZ_UNPT) agg[i] = cand[i];
+}
+
+// pass 2 of VMB on the GPU from pass 1's map v (pinned); agg_dev ends with the
+// final map when it returns true
+bool vmb_pass2_gpu(const DevCSR &N, const int *v, const std::vector<int> &size, int maxagg, int *agg_dev) {
+    const int n = N.n, na = (int)size.size();
+    hipStream_t st = stream();
+    int *dsize = arena().alloc_n<int>((size_t)na + 1);
+    int *count = arena().alloc_n<int>((size_t)na + 1);
+    int *cand = arena().alloc_n<int>((size_t)n + 1);
+    int *flag = arena().alloc_n<int>(1);
+    HZ_CHECK(hipMemcpyAsync(agg_dev, v, sizeof(int) * (size_t)n, hipMemcpyHostToDevice, st));
+    HZ_CHECK(hipMemcpyAsync(dsize, size.data(), sizeof(int) * (size_t)na, hipMemcpyHostToDevice, st));
+    HZ_CHECK(hipMemsetAsync(count, 0, sizeof(int) * (size_t)na, st));
+    HZ_CHECK(hipMemsetAsync(flag, 0, sizeof(int), st));
+    const int g = (int)std::min<long>(8192, ((long)n + kVmbBlock - 1) / kVmbBlock);
+    const int gc = (int)std::min<long>(8192, ((long)na + kVmbBlock - 1) / kVmbBlock);
+    k_vmb2_first<<<g, kVmbBlock, 0, st>>>(n, N.ia, N.ja, agg_dev, cand, count, flag);
+    k_vmb2_caps<<<gc, kVmbBlock, 0, st>>>(na, dsize, count, maxagg, flag);
+    k_vmb2_join<<<g, kVmbBlock, 0, st>>>(n, cand, agg_dev);
+    HZ_LAUNCH_CHECK();
+    int *hf = static_cast<int *>(g_small.get(4 * sizeof(int)));
+    HZ_CHECK(hipMemcpyAsync(hf, flag, sizeof(int), hipMemcpyDeviceToHost, st));
+    HZ_CHECK(hipStreamSynchronize(st));
+    return hf[0] == 0;
+}
 }  // namespace
 
 int aggregate_host(const DevCSR &N, int type, int max_aggregation, int *agg_dev, int *nagg) {
@@ -44,6 +110,7 @@ int aggregate_host(const DevCSR &N, int type, int max_aggregation, int *agg_dev,
     hipStream_t st = stream();
     HZ_CHECK(hipMemcpyAsync(ia, N.ia, sizeof(int) * (n + 1), hipMemcpyDeviceToHost, st));
     static std::vector<int> v;   // kept across calls: no fresh pages per call
+    int *pv = static_cast<int *>(g_v.get(sizeof(int) * ((size_t)n + 1)));
     int st_code;
     if (type == HZ_VMB) {
         // VMB's first pass walks the rows downwards: the column indices come
@@ -89,12 +156,30 @@ int aggregate_host(const DevCSR &N, int type, int max_aggregation, int *agg_dev,
         };
         double pass_ms[3] = {0, 0, 0};
         const double s0 = tr ? wall_ms() : 0.0;
-        st_code = vmb_sweep(n, ia, ja, max_aggregation, v, *nagg, &arrival, tr ? pass_ms : nullptr);
+        // pass 1 on the host, into the pinned map
+        static std::vector<int> size;
+        int left = 0;
+        const int na = vmb_pass1(n, ia, ja, max_aggregation, pv, size, left, &arrival);
         HZ_CHECK(hipStreamSynchronize(st));
+        const double s1 = tr ? wall_ms() : 0.0;
+        bool on_gpu = false;
+        if (na < 1) {
+            *nagg = 0;
+            st_code = HZ_ERROR_AMG_COARSEING;
+        } else {
+            st_code = HZ_SUCCESS;
+            on_gpu = n >= kVmbGpuPass2Rows && vmb_pass2_gpu(N, pv, size, max_aggregation, agg_dev);
+            *nagg = on_gpu ? na : vmb_pass23(n, ia, ja, max_aggregation, pv, size, na, left,
+                                              tr ? pass_ms : nullptr);
+        }
         if (tr)
             std::fprintf(stderr, "[hz]   VMB n=%d nnz=%lld: sweep %.2f ms (pass 1 %.2f, of which waiting for the graph "
-                         "%.2f; pass 2 %.2f; pass 3 %.2f)\n", n, total, wall_ms() - s0, pass_ms[0], waited,
-                         pass_ms[1], pass_ms[2]);
+                         "%.2f; pass 2 %s %.2f; pass 3 %.2f)\n", n, total, wall_ms() - s0, s1 - s0, waited,
+                         on_gpu ? "on the GPU" : "on the host", on_gpu ? wall_ms() - s1 : pass_ms[1], pass_ms[2]);
+        if (on_gpu) return st_code;
+        HZ_CHECK(hipMemcpyAsync(agg_dev, pv, sizeof(int) * n, hipMemcpyHostToDevice, st));
+        HZ_CHECK(hipStreamSynchronize(st));
+        return st_code;
     } else {
         if (N.nnz) {
             HZ_CHECK(hipMemcpyAsync(ja, N.ja, sizeof(int) * N.nnz, hipMemcpyDeviceToHost, st));
@@ -104,7 +189,6 @@ int aggregate_host(const DevCSR &N, int type, int max_aggregation, int *agg_dev,
         st_code = hem_sweep(n, ia, ja, a, v, *nagg);
     }
     if (st_code == -1017) return st_code;
-    int *pv = static_cast<int *>(g_v.get(sizeof(int) * (size_t)n));
     std::copy(v.begin(), v.end(), pv);
     HZ_CHECK(hipMemcpyAsync(agg_dev, pv, sizeof(int) * n, hipMemcpyHostToDevice, st));
     HZ_CHECK(hipStreamSynchronize(st));

@@ -15,18 +15,12 @@
 
 namespace hz {
 
-int vmb_sweep(int n, const int *ia, const int *ja, int maxagg, std::vector<int> &v, int &nagg,
-              const RowBlocksDown *arrival, double *ms) {
-    const auto now = [] {
-        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
-    };
-    const double t0 = ms ? now() : 0.0;
-    int left = n, na = 0;
-    v.assign(n, -2);
-    // aggregate sizes as pass 1 forms them (the vertices that take the id),
-    // buffers kept across calls
-    static std::vector<int> size;
-    size.clear();
+int vmb_pass1(int n, const int *ia, const int *ja, int maxagg, int *v, std::vector<int> &size, int &left,
+              const RowBlocksDown *arrival) {
+    left = n;
+    int na = 0;
+    std::fill(v, v + n, -2);
+    size.clear();   // aggregate sizes as the pass forms them (the vertices that take the id)
     int blk = 0, here = n;   // rows >= here have arrived
     for (int i = n - 1; i >= 0; --i) {
         while (arrival && i < here && blk < arrival->nblocks) {
@@ -55,9 +49,15 @@ int vmb_sweep(int n, const int *ia, const int *ja, int maxagg, std::vector<int> 
         size.push_back(members);
         ++na;
     }
-    nagg = na;
+    return na;
+}
+
+int vmb_pass23(int n, const int *ia, const int *ja, int maxagg, int *v, std::vector<int> &size, int na, int left,
+               double *ms) {
+    const auto now = [] {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+    };
     const double t1 = ms ? now() : 0.0;
-    if (na < 1) return HZ_ERROR_AMG_COARSEING;
     // pass 2 reads only pass 1's map: its own assignments are collected and
     // written after it (each vertex is visited once, so none is read back)
     static std::vector<std::pair<int, int>> joins;
@@ -91,12 +91,27 @@ int vmb_sweep(int n, const int *ia, const int *ja, int maxagg, std::vector<int> 
             ++na;
         }
     }
-    nagg = na;
     if (ms) {
-        ms[0] = t1 - t0;
         ms[1] = t2 - t1;
         ms[2] = now() - t2;
     }
+    return na;
+}
+
+int vmb_sweep(int n, const int *ia, const int *ja, int maxagg, std::vector<int> &v, int &nagg,
+              const RowBlocksDown *arrival, double *ms) {
+    const auto now = [] {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+    };
+    const double t0 = ms ? now() : 0.0;
+    static std::vector<int> size;
+    int left = 0;
+    v.resize(n);
+    const int na = vmb_pass1(n, ia, ja, maxagg, v.data(), size, left, arrival);
+    nagg = na;
+    if (ms) ms[0] = now() - t0;
+    if (na < 1) return HZ_ERROR_AMG_COARSEING;
+    nagg = vmb_pass23(n, ia, ja, maxagg, v.data(), size, na, left, ms);
     return HZ_SUCCESS;
 }
 

@@ -103,11 +103,13 @@ def test_rap_agg_bitexact(lib, orc, seed):
         assert np.array_equal(a, b_)
 
 
-def _hierarchies(lib, orc, libc, A, agg_type, theta, coarse_solver=32, coarse_dof=100):
+def _hierarchies(lib, orc, libc, A, agg_type, theta, coarse_solver=32, coarse_dof=100, max_aggregation=None):
     out = []
     for which in ("gpu", "oracle"):
         it, amg = capi.solver_params(lib, aggregation=agg_type, theta=theta,
                                      coarse_solver=coarse_solver, coarse_dof=coarse_dof)
+        if max_aggregation is not None:
+            amg.max_aggregation = max_aggregation
         h = capi.hz_hierarchy()
         libc.srand(1)
         if which == "gpu":
@@ -182,6 +184,31 @@ def test_hierarchy_bitexact_kuhn3d(lib, orc, libc, m, agg_type, hem_path):
         if "agg" in o:
             assert np.array_equal(g["agg"], o["agg"])
     assert rg == ro
+
+
+@pytest.mark.parametrize("problem,maxagg", [("poisson", 20), ("aniso", 20), ("poisson", 4)])
+def test_vmb_second_pass_on_the_gpu_bitexact(lib, orc, libc, problem, maxagg):
+    """VMB levels of 65,536 rows or more run the second pass (join the first
+    neighbouring pass-1 aggregate) on the GPU when no aggregate would pass
+    max_aggregation with every first choice, and the host pass otherwise
+    (max_aggregation 4: pass 1 fills its aggregates); both give the
+    reference's aggregates and coarse matrices (47^3 = 103,823 rows)."""
+    if problem == "poisson":
+        ia, ja, va = problems.poisson3d_kuhn(48)
+    else:
+        ia, ja, va = problems.aniso3d_kuhn(48, 0.7, 1e-3)
+    A = capi.HostCSR(ia, ja, va)
+    (lg, thg, _), (lo, tho, _) = _hierarchies(lib, orc, libc, A, capi.VMB, 0.04, coarse_dof=100,
+                                              max_aggregation=maxagg)
+    print(f"VMB {problem} max_aggregation {maxagg}: levels {[g['ia'].size - 1 for g in lg]}", flush=True)
+    assert len(lg) == len(lo)
+    for g, o in zip(lg, lo):
+        if "agg" in o:
+            assert g["nagg"] == o["nagg"]
+            assert np.array_equal(g["agg"], o["agg"])
+        for key in ("ia", "ja", "val"):
+            assert np.array_equal(g[key], o[key]), key
+    assert thg == tho
 
 
 @pytest.mark.parametrize("m", [64, 106])
