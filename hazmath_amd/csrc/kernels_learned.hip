@@ -338,12 +338,15 @@ namespace {
 // Lanczos on B = D^-1/2 A D^-1/2 (similar to D^-1 A, symmetric for SPD A;
 // d: first diagonal entries, rows with d <= 0 drop out).  The scalars of the
 // recurrence stay on the device (alpha[k], beta[k]); the host reads them once.
-// Every kernel runs a fixed grid of kLzGrid blocks (grid-stride loops), so
-// each reduction has at most kLzGrid partials.  16 lanes share a row of A
-// (coalesced loads) and combine their products by a butterfly: an estimate,
-// so the summation order is free.
-constexpr int kLzLanes = 16;
-constexpr int kLzGrid = 2048;
+// Every kernel runs a grid of at most kLzGrid blocks (grid-stride loops), so
+// each reduction has at most kLzGrid partials.  4 lanes share a row of A, each
+// issuing the loads of up to kLzPass of its entries (lane j: entries j, j + 4,
+// ...) before their gathers, and combine their products by a butterfly: an
+// estimate, so the summation order is free.  (16 lanes a row, one entry each,
+// on at most 2,048 blocks: 175 us a step on the 128^3 anisotropic level.)
+constexpr int kLzLanes = 4;
+constexpr int kLzPass = 4;
+constexpr int kLzGrid = 8192;
 
 // v_k = s wprev (s = 1 / *norm; written out) and y = B v_k, block partials of
 // (y, v_k); v_k's gathered entries are formed from wprev on the fly
@@ -362,9 +365,18 @@ __global__ __launch_bounds__(kBlock) void k_lz_apply(int n, const int *__restric
     for (int i = g0; i < n; i += groups) {   // trip count uniform within the lane group
         const int k0 = ia[i], k1 = ia[i + 1];
         double t = 0.0;
-        for (int k = k0 + j; k < k1; k += kLzLanes) {
-            const int c = ja[k];
-            t += a[k] * (isd[c] * (sc * wprev[c]));
+        for (int kb = k0 + j; kb < k1; kb += kLzLanes * kLzPass) {
+            int cc[kLzPass];
+            double av[kLzPass];
+#pragma unroll
+            for (int u = 0; u < kLzPass; ++u) {
+                const int k = kb + u * kLzLanes;
+                cc[u] = k < k1 ? ja[k] : -1;
+                av[u] = k < k1 ? a[k] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < kLzPass; ++u)
+                if (cc[u] >= 0) t += av[u] * (isd[cc[u]] * (sc * wprev[cc[u]]));
         }
 #pragma unroll
         for (int o = kLzLanes / 2; o > 0; o >>= 1) t += __shfl_xor(t, o, kWave);
