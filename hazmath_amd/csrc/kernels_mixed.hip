@@ -491,7 +491,19 @@ __global__ __launch_bounds__(256) void k_gemv_m(int n, const double *__restrict_
     if (row >= n) return;
     double s = 0.0;
     const double *a = Ainv + (size_t)row * n;
-    for (int k = lane; k < n; k += kWave) s += a[k] * (double)b[k];
+    constexpr int U = 16;   // a lane's loads issued together: entries lane, lane + 64, ... in that order
+    for (int kb = lane; kb < n; kb += kWave * U) {
+        double av[U], bv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = kb + u * kWave;
+            av[u] = k < n ? a[k] : 0.0;
+            bv[u] = k < n ? (double)b[k] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (kb + u * kWave < n) s += av[u] * bv[u];
+    }
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, kWave);
     if (lane == 0) x[row] = (float)s;
