@@ -182,9 +182,13 @@ def parse():
     p.add_argument("--learned-problem", choices=("aniso", "poisson"), default="aniso",
                    help="--learned on the anisotropic problem (eps=1e-3) or on the Poisson problem")
     p.add_argument("--cheb-steps", type=int, default=4,
-                   help="Chebyshev steps per pre- and post-smoothing of the SA carrier (constant and learned); "
-                        "measured on the 128^3 anisotropic problem, learned FP32: 2 -> 151.7 ms, 3 -> 141.2, "
-                        "4 -> 139.2, 5 -> 140.1, 6 -> 143.7, 8 -> 155.1")
+                   help="Chebyshev steps per pre- and post-smoothing of the SA carrier's finest level (constant "
+                        "and learned); measured on the 128^3 anisotropic problem, learned FP32 (every level the "
+                        "same): 2 -> 151.7 ms, 3 -> 141.2, 4 -> 139.2, 5 -> 140.1, 6 -> 143.7, 8 -> 155.1")
+    p.add_argument("--cheb-coarse-steps", type=int, default=2,
+                   help="Chebyshev steps per smoothing on the SA carrier's coarser levels (hz_options."
+                        "cheb_coarse_steps; 0: as --cheb-steps); 128^3 anisotropic, learned FP32 with "
+                        "--cheb-steps 4: 4 -> 96 iterations, PCG 78.3 ms; 3 -> 96, 73.6; 2 -> 96, 68.8; 1 -> 110, 73.2")
     p.add_argument("--cpu-sample-m", type=int, default=128,
                    help="cells per direction of the CPU-baseline sample")
     p.add_argument("--cpu-runs", type=int, default=5, help="CPU-baseline solves (median reported)")
@@ -448,6 +452,7 @@ def learned_comparison(a, lib, steps, warmup, kinds=("plain", "sa", "learned", "
     theta_dir = float(np.random.default_rng(5).uniform(0.0, np.pi))
     poisson = getattr(a, "learned_problem", "aniso") == "poisson"
     cheb_steps = getattr(a, "cheb_steps", 4)
+    cheb_coarse = getattr(a, "cheb_coarse_steps", 2)
     if poisson:
         ia, ja, va = problems.poisson3d_kuhn(a.m)
     else:
@@ -478,7 +483,10 @@ def learned_comparison(a, lib, steps, warmup, kinds=("plain", "sa", "learned", "
         opts = capi.hz_options()
         lib.hz_get_options(C.byref(opts))
         old = opts.vcycle_precision
+        old_coarse = opts.cheb_coarse_steps
         opts.vcycle_precision = 1 if kind == "learned_fp32" else 0
+        # the SA carrier's levels below the finest: Chebyshev(cheb_coarse) (plain: no Chebyshev)
+        opts.cheb_coarse_steps = cheb_coarse
         lib.hz_set_options(C.byref(opts))
         libc.srand(1)
         try:
@@ -497,6 +505,7 @@ def learned_comparison(a, lib, steps, warmup, kinds=("plain", "sa", "learned", "
                                               capi.fptr(weights), 400, cheb_steps)
         finally:
             opts.vcycle_precision = old
+            opts.cheb_coarse_steps = old_coarse
             lib.hz_set_options(C.byref(opts))
         info = capi.hz_solve_info()
         lib.hz_get_last_info(C.byref(info))
@@ -529,7 +538,7 @@ def learned_comparison(a, lib, steps, warmup, kinds=("plain", "sa", "learned", "
                f"theta={theta_dir:.4f}, {a.m}^3 cells")
     workload = (f"{problem}, n={n}, seeded RHS, PCG to {a.tol:g}; "
                 f"coarse_dof={a.coarse_dof}; plain = UA HEM L1 V(1,1) FP64 drop-in; sa = SA on VMB with "
-                f"Chebyshev({cheb_steps}), constant "
+                f"Chebyshev({cheb_steps}) on the finest level and Chebyshev({cheb_coarse or cheb_steps}) below, constant "
                 "parameters; learned = the same with per-aggregate damping and bound scale from "
                 "the committed network, plain method as fallback; learned_fp32 = learned with the FP32 cycle "
                 "under the FP64 PCG")

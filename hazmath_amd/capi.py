@@ -86,7 +86,8 @@ class hz_options(C.Structure):
     _fields_ = [("jacobi_weight", REAL), ("hec_rng_source", INT), ("verbose_timing", INT),
                 ("return_on_fatal", INT), ("vcycle_precision", INT), ("batch_time_kernels", INT),
                 ("time_pcg_spmv", INT), ("hem_host_max_rows", INT),
-                ("rap_wave_stage_pairs", INT), ("hem_pipeline_max_sweeps", INT), ("release_after_call", INT)]
+                ("rap_wave_stage_pairs", INT), ("hem_pipeline_max_sweeps", INT), ("release_after_call", INT),
+                ("cheb_coarse_steps", INT)]
 
 
 class hz_batch_stats(C.Structure):

@@ -409,10 +409,16 @@ struct CycleState {
     bool first_done[HZ_MAX_AMG_LVL];  // zero-iterate sweep already fused into restriction
 };
 
+// Chebyshev steps of a smoothing on level l: the caller's count, or below
+// level 0 the coarse-level count when one is set.
+int cheb_steps_at(const CycleParams &cp, int l, int steps) {
+    return (cp.cheb_coarse > 0 && l >= 1) ? cp.cheb_coarse : steps;
+}
+
 void presmooth(Hierarchy &H, int l, CycleState &cs) {
     Level &L = H.lv[l];
     const CycleParams &cp = H.cp;
-    int sweeps = cp.presmooth;
+    int sweeps = cp.smoother == HZ_SMOOTHER_CHEBY ? cheb_steps_at(cp, l, cp.presmooth) : cp.presmooth;
     if (cp.smoother == HZ_SMOOTHER_CHEBY || is_gs_family(cp.smoother)) {
         // Chebyshev from zero: its first step reads no x (nor the matrix)
         const bool cheb_from_zero = cp.smoother == HZ_SMOOTHER_CHEBY && cs.zero[l] && sweeps > 0;
@@ -465,7 +471,8 @@ void prolong_postsmooth(Hierarchy &H, int l, double alpha, const double *alpha_d
     if (cp.postsmooth <= 0 || cp.smoother == HZ_SMOOTHER_CHEBY || is_gs_family(cp.smoother)) {
         if (L.sa) csr_prolong(L.P, C.x, alpha, alpha_dev, L.x);
         else prolong_agg(L.A.n, L.agg, C.x, alpha, alpha_dev, L.x);
-        if (cp.smoother == HZ_SMOOTHER_CHEBY) cheby_apply(L.S, L.cheb, cp.postsmooth, L.b, L.x, L.xt, L.cwork, &L.A);
+        if (cp.smoother == HZ_SMOOTHER_CHEBY)
+            cheby_apply(L.S, L.cheb, cheb_steps_at(cp, l, cp.postsmooth), L.b, L.x, L.xt, L.cwork, &L.A);
         else if (is_gs_family(cp.smoother)) gs_smooth(L.gs, cp.smoother, true, cp.postsmooth, cp.relax, L.b, L.x);
         return;
     }
@@ -570,7 +577,7 @@ void presmooth_m(Hierarchy &H, int l, CycleState &cs) {
     Level &L = H.lv[l];
     const CycleParams &cp = H.cp;
     const float w = (float)cp.jacobi_w;
-    int sweeps = cp.presmooth;
+    int sweeps = cp.smoother == HZ_SMOOTHER_CHEBY ? cheb_steps_at(cp, l, cp.presmooth) : cp.presmooth;
     if (cp.smoother == HZ_SMOOTHER_CHEBY) {
         // from zero, the first step reads no x (nor the matrix)
         const bool from_zero = cs.zero[l] && sweeps > 0;
@@ -615,7 +622,8 @@ bool prolong_postsmooth_m(Hierarchy &H, int l, bool final, double *z, const doub
         else prolong_m(L.A.n, L.agg, C.xf, L.xf);
         if (cp.postsmooth <= 0) return false;
         if (cheb) {
-            cheby_apply_m(L.S, L.A, L.low, L.cheb, L.cdiagf, cp.postsmooth, L.bf, L.xf, L.xtf, L.cworkf,
+            cheby_apply_m(L.S, L.A, L.low, L.cheb, L.cdiagf, cheb_steps_at(cp, l, cp.postsmooth), L.bf, L.xf, L.xtf,
+                          L.cworkf,
                           final ? z : nullptr, r, final ? dot_out : nullptr, H.ws);
             return final;
         }

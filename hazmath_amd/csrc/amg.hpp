@@ -43,6 +43,7 @@ struct CycleParams {
     double relax = 1.0;         // SOR-type relaxation (AMG_param relaxation)
     int maxit = 1;              // cycles per preconditioner application
     int precision = 0;          // 0 FP64 cycle; 1 FP32 cycle; 2 FP32 cycle, BF16 values below level 0
+    int cheb_coarse = 0;        // Chebyshev steps per smoothing below level 0 (0: presmooth / postsmooth)
 };
 
 struct Hierarchy {

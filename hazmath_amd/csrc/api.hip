@@ -11,7 +11,7 @@
 using namespace hz;
 
 namespace {
-hz_options g_opt = {1.0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+hz_options g_opt = {1.0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 
 // The library's device memory for one top-level call: the arena is reset on
 // entry, and with hz_options.release_after_call its chunks go back to HIP when
@@ -189,6 +189,10 @@ int check_supported(const linear_itsolver_param *it, const AMG_param *amg) {
         std::printf("### ERROR: cycle type %lld is not available in this build\n", (long long)amg->cycle_type);
         return HZ_ERROR_INPUT_PAR;
     }
+    if (g_opt.cheb_coarse_steps < 0) {
+        std::printf("### ERROR: cheb_coarse_steps %lld is negative\n", (long long)g_opt.cheb_coarse_steps);
+        return HZ_ERROR_INPUT_PAR;
+    }
     if (g_opt.vcycle_precision < 0 || g_opt.vcycle_precision > 2) {
         std::printf("### ERROR: vcycle_precision %lld is not 0, 1 or 2\n", (long long)g_opt.vcycle_precision);
         return HZ_ERROR_INPUT_PAR;
@@ -301,6 +305,7 @@ INT core_solve(const DevCSR &A, const double *db, double *dx, linear_itsolver_pa
     if (st <= -1000) return fatal(st);
     if (st < 0) return st;
     H.cp.precision = g_opt.vcycle_precision;
+    H.cp.cheb_coarse = (int)g_opt.cheb_coarse_steps;
     amg_prepare_cycle(H, amg);
     H.cp.jacobi_w = g_opt.jacobi_weight;
     HZ_CHECK(hipEventRecord(e1, stream()));
@@ -915,6 +920,7 @@ precond *create_precond_amg(dCSRmat *A, AMG_param *amgparam) {
             return nullptr;
         }
         P->H.cp.precision = g_opt.vcycle_precision;
+        P->H.cp.cheb_coarse = (int)g_opt.cheb_coarse_steps;
         amg_prepare_cycle(P->H, amgparam);
         P->H.cp.jacobi_w = g_opt.jacobi_weight;
         P->n = A->row;

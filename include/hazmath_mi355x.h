@@ -345,6 +345,11 @@ typedef struct {
                               HIP before it returns (the reference frees its work arrays); 0
                               (default): the pool is kept, so a warm call performs no hipMalloc.
                               Preconditioner objects keep their own memory until hz_precond_free */
+    INT  cheb_coarse_steps;   /* Chebyshev smoother (amg.smoother 91): steps per smoothing on the levels
+                              below the finest; 0 (default): presmooth_iter / postsmooth_iter on every
+                              level.  No counterpart in the reference; measured on the learned method
+                              (128^3 anisotropic, Chebyshev(4) on the finest level): 2 keeps the
+                              iteration count and takes 12 % off the PCG */
 } hz_options;
 
 void hz_get_options(hz_options *opt);

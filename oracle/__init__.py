@@ -67,6 +67,8 @@ def load(path=None):
     lib.orc_gs_smooth_ordered.restype = C.c_int
     lib.orc_set_learned.argtypes = [ORC_OMEGA_FN, P(REAL)]
     lib.orc_set_learned.restype = None
+    lib.orc_set_cheb_coarse_steps.argtypes = [INT]
+    lib.orc_set_cheb_coarse_steps.restype = None
     if path is None:
         _LIB = lib
     return lib

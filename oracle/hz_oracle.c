@@ -1368,9 +1368,15 @@ typedef struct {
     INT maxit;
 } orc_prec_data;
 
+/* Chebyshev steps below the finest level (0: the cycle's counts); the
+ * library's hz_options.cheb_coarse_steps, no reference counterpart. */
+static INT g_orc_cheb_coarse = 0;
+void orc_set_cheb_coarse_steps(INT steps) { g_orc_cheb_coarse = steps; }
+
 static void orc_smooth(const orc_prec_data *pd, orc_level *L, INT sweeps, int post) {
     if (pd->cyc.smoother == 91) {   /* Chebyshev (build-owned): bound scale 1 or the level's set scale */
         const INT l = (INT)(L - pd->m->lv);
+        if (g_orc_cheb_coarse > 0 && l >= 1) sweeps = g_orc_cheb_coarse;
         orc_cheby_smooth(&L->A, L->b, L->x, sweeps, g_orc_cheb_scale_set ? g_orc_cheb_scale[l] : 1.0);
         return;
     }

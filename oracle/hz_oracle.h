@@ -74,6 +74,9 @@ int orc_gs_smooth_ordered(const dCSRmat *A, const REAL *b, REAL *x, INT smoother
                           REAL relax, const INT *ord, const INT *cptr, INT nblk);
 int orc_is_gs_family(INT smoother);
 void orc_cheby_smooth(const dCSRmat *A, const REAL *b, REAL *x, INT steps, REAL scale);
+/* Chebyshev steps per smoothing below the finest level in the cycle (0: the
+ * cycle's presmooth / postsmooth counts); no reference counterpart. */
+void orc_set_cheb_coarse_steps(INT steps);
 /* linear_solver_amg + amg_solve (itsolver.c:524-640, amg_solve.c:109-161) */
 INT  orc_linear_solver_amg(dCSRmat *A, dvector *b, dvector *x, AMG_param *param);
 /* create_precond_amg / precond_amg (helper.c:206-283, precond.c:164-183) */

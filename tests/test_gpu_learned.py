@@ -117,16 +117,24 @@ def test_learned_scale_floor_binds(lib, libc):
         assert info.lambda_ratio[0] == pytest.approx(full, rel=1e-12)
 
 
-def _set_precision(lib, p):
+def _set_precision(lib, p, cheb_coarse=None):
+    """Set hz_options.vcycle_precision (and cheb_coarse_steps when given);
+    returns the previous (precision, cheb_coarse_steps)."""
     opts = capi.hz_options()
     lib.hz_get_options(C.byref(opts))
-    old = opts.vcycle_precision
+    old = (opts.vcycle_precision, opts.cheb_coarse_steps)
     opts.vcycle_precision = p
+    if cheb_coarse is not None:
+        opts.cheb_coarse_steps = cheb_coarse
     lib.hz_set_options(C.byref(opts))
     return old
 
 
-def _oracle_composition(lib, orc, libc, ia, ja, va, b, theta, scales, cheb_steps=2):
+def _restore(lib, old):
+    _set_precision(lib, old[0], old[1])
+
+
+def _oracle_composition(lib, orc, libc, ia, ja, va, b, theta, scales, cheb_steps=2, cheb_coarse=0):
     """orc_krylov_amg with the learned method's settings (SA on VMB, Chebyshev
     pre/post steps, the drop-in's other parameters) and the per-aggregate
     damping of the library's feature kernel + network on each oracle level."""
@@ -144,6 +152,7 @@ def _oracle_composition(lib, orc, libc, ia, ja, va, b, theta, scales, cheb_steps
     cb = oracle.ORC_OMEGA_FN(omega_fn)
     sc = (capi.REAL * capi.MAX_AMG_LVL)(*scales)
     orc.orc_set_learned(cb, sc)
+    orc.orc_set_cheb_coarse_steps(cheb_coarse)
     try:
         it, amg = oracle.params(tol=1e-10, maxit=1000, aggregation=capi.VMB, smoother=91, coarse_dof=200,
                                 theta=0.04)
@@ -155,30 +164,34 @@ def _oracle_composition(lib, orc, libc, ia, ja, va, b, theta, scales, cheb_steps
                                C.byref(amg), C.byref(capi.orc_trace()))
     finally:
         orc.orc_set_learned(C.cast(None, oracle.ORC_OMEGA_FN), None)
+        orc.orc_set_cheb_coarse_steps(0)
     return int(r), x.arr.copy(), omegas
 
 
-@pytest.mark.parametrize("m,th", [(16, 0.3), (20, 2.2)])
+@pytest.mark.parametrize("m,th,steps,coarse", [(16, 0.3, 2, 0), (20, 2.2, 2, 0), (20, 2.2, 4, 2)])
 @pytest.mark.parametrize("precision,bar", [(0, 1e-10), (1, 1e-8)])
-def test_learned_entry_vs_oracle_composition(lib, orc, libc, m, th, precision, bar):
-    """hz_krylov_amg_learned (SA + Chebyshev(2), committed network) on a 3-D
-    anisotropic problem (eps = 1e-3) against the oracle composition: PCG
+def test_learned_entry_vs_oracle_composition(lib, orc, libc, m, th, steps, coarse, precision, bar):
+    """hz_krylov_amg_learned (SA + Chebyshev(steps), committed network) on a
+    3-D anisotropic problem (eps = 1e-3) against the oracle composition: PCG
     iterations within one, solution within 1e-10 with the FP64 cycle and 1e-8
-    with the FP32 cycle; every level's scale follows the floor rule."""
+    with the FP32 cycle; every level's scale follows the floor rule.  With
+    hz_options.cheb_coarse_steps the levels below the finest take that many
+    steps (the oracle's orc_set_cheb_coarse_steps)."""
     theta = learned.load_model()
     ia, ja, va = problems.aniso3d_kuhn(m, th, 1e-3)
     n = ia.size - 1
     b = problems.uniform_rhs(n, seed=1)
-    old = _set_precision(lib, precision)
+    old = _set_precision(lib, precision, coarse)
     try:
         libc.srand(1)
-        rg, xg, info = learned.solve_learned(lib, ia, ja, va, b, theta, budget=400)
+        rg, xg, info = learned.solve_learned(lib, ia, ja, va, b, theta, budget=400, cheb_steps=steps)
     finally:
-        _set_precision(lib, old)
+        _restore(lib, old)
     assert info.fallback_used == 0 and rg > 0
     nl = info.num_levels
     scales = [info.cheb_scale[l] if l < nl - 1 else 1.0 for l in range(capi.MAX_AMG_LVL)]
-    ro, xo, omegas = _oracle_composition(lib, orc, libc, ia, ja, va, b, theta, scales)
+    ro, xo, omegas = _oracle_composition(lib, orc, libc, ia, ja, va, b, theta, scales, cheb_steps=steps,
+                                         cheb_coarse=coarse)
     rel = np.linalg.norm(xg - xo) / np.linalg.norm(xo)
     _say(f"m={m} precision {precision}: GPU {rg} iterations, oracle composition {ro}, distance {rel:.2e}")
     assert ro > 0 and abs(rg - ro) <= 1
@@ -207,7 +220,7 @@ def test_learned_fp32_cycle_reaches_fp64_tolerance(lib, libc):
             libc.srand(1)
             r, x, info = learned.solve_learned(lib, ia, ja, va, b, theta, budget=400)
         finally:
-            _set_precision(lib, old)
+            _restore(lib, old)
         rel = np.linalg.norm(b - M @ x) / np.linalg.norm(b)
         _say(f"precision {p}: {r} iterations, fallback {info.fallback_used}, residual {rel:.2e}, "
              f"{info.total_ms:.1f} ms")
