@@ -328,10 +328,10 @@ typedef struct {
                               pipeline, -2 the full host sweep, -3 the preference lists */
     INT  rap_wave_stage_pairs; /* SA's general Galerkin product (dcsr_rap) runs one wavefront per
                               coarse row, staging the row's (R entry, A entry) pairs in LDS this
-                              many at a time; rows of more than 512 columns, or with one A row
+                              many at a time; rows of more than 256 columns, or with one A row
                               longer than a stage, run one thread per row.  All forms give the
-                              same matrix.  0: 512 (the LDS capacity; larger values are capped
-                              there), -1: every row one thread per row, -2: the default with a
+                              same matrix.  0: 256 (the stage's LDS capacity; larger values are
+                              capped there), -1: every row one thread per row, -2: the default with a
                               16-column limit */
     INT  hem_pipeline_max_sweeps; /* HEM's one-launch GPU pipeline gives up on a chunk of vertices
                               after this many sweeps (checked every 16).  > 0: the remaining
