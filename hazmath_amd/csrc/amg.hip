@@ -258,7 +258,7 @@ int amg_setup(Hierarchy &H, const DevCSR &A0, AMG_param *param, bool) {
             if (tr) { HZ_CHECK(hipStreamSynchronize(stream())); ts[1] = wall_ms(); }
             csr_transpose_gpu(L.P, L.R);
             if (tr) { HZ_CHECK(hipStreamSynchronize(stream())); ts[2] = wall_ms(); }
-            rap_general(L.R, L.A, L.P, C.A);
+            rap_general(L.R, L.A, L.P, C.A, lvl == 0);
             if (tr) {
                 HZ_CHECK(hipStreamSynchronize(stream()));
                 ts[3] = wall_ms();

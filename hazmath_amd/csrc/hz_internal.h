@@ -200,7 +200,9 @@ void sa_prolongation(const DevCSR &A, DevCSR &N, const int *agg, int nagg, doubl
 // AT = A^T with every row listing its source rows in increasing order (dcsr_trans).
 void csr_transpose_gpu(const DevCSR &A, DevCSR &AT);
 // C = R A P with the layout and summation order of dcsr_rap.
-void rap_general(const DevCSR &R, const DevCSR &A, const DevCSR &P, DevCSR &C);
+// narrow_first: try tables of half the width first (rows of the finest
+// level's coarse operator are usually narrow); the same matrix either way.
+void rap_general(const DevCSR &R, const DevCSR &A, const DevCSR &P, DevCSR &C, bool narrow_first = false);
 // hz_options.rap_wave_stage_pairs: (R entry, A entry) pairs the wavefront
 // Galerkin kernel stages in LDS at a time; 0 = its capacity, -1 = every row
 // through the one-thread-per-row kernel, -2 = the default with a 16-column

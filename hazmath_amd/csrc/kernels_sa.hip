@@ -311,11 +311,13 @@ __global__ void k_rap_rows(int r0, int r1, const int *__restrict__ rows, const l
 constexpr int kRapwPairs = 256;
 constexpr double kRapSlotBytes = 4e9;   // slots of the one-pass form: at most 4 GB
 constexpr int kRapwCols = 256;
-constexpr int kRapwHash = 2 * kRapwCols;   // at most kRapwCols + 64 keys: probing ends
+// the finest level's product first tries tables of this many columns
+constexpr int kRapwNarrowCols = 128;
 
+template <int kHash>
 __device__ __forceinline__ unsigned rapw_hash(int key) {
     unsigned h = (unsigned)key * 2654435761u;
-    return (h ^ (h >> 15)) & (kRapwHash - 1u);
+    return (h ^ (h >> 15)) & (kHash - 1u);
 }
 
 __device__ __forceinline__ int wave_incl_scan(int v, int lane) {
@@ -327,7 +329,8 @@ __device__ __forceinline__ int wave_incl_scan(int v, int lane) {
     return v;
 }
 
-template <bool kFill, bool kSlots = false>
+// kCols: the table's columns and the stage's pairs (LDS 50 kCols bytes)
+template <bool kFill, bool kSlots = false, int kCols = kRapwCols>
 __global__ __launch_bounds__(kWave) void k_rapw(int nc, const int *__restrict__ ria, const int *__restrict__ rja,
                                                 const double *__restrict__ rval, const int *__restrict__ aia,
                                                 const int *__restrict__ aja, const double *__restrict__ aval,
@@ -335,24 +338,25 @@ __global__ __launch_bounds__(kWave) void k_rapw(int nc, const int *__restrict__ 
                                                 const double *__restrict__ pval, int max_pairs, int max_cols,
                                                 int *__restrict__ rowcount, const int *__restrict__ cia,
                                                 int *__restrict__ cja, double *__restrict__ cval) {
-    __shared__ int s_pst[kRapwPairs];          // P-row start of each pair
-    __shared__ int s_off[kRapwPairs];          // its P-row length, then its first candidate
-    __shared__ double s_ra[kFill ? kRapwPairs : 1];
-    __shared__ int s_key[kRapwHash], s_pos[kRapwHash];
-    __shared__ double s_val[kFill ? kRapwHash : 1];
+    constexpr int kPairs = kCols, kHash = 2 * kCols;   // at most kCols + 64 keys: probing ends
+    __shared__ int s_pst[kPairs];              // P-row start of each pair
+    __shared__ int s_off[kPairs];          // its P-row length, then its first candidate
+    __shared__ double s_ra[kFill ? kPairs : 1];
+    __shared__ int s_key[kHash], s_pos[kHash];
+    __shared__ double s_val[kFill ? kHash : 1];
     __shared__ double s_rv[kFill ? kWave : 1];
     const int lane = threadIdx.x;
     const int ic = blockIdx.x;
     if (ic >= nc) return;   // the whole (single-wavefront) block
     if (kFill && !kSlots && rowcount[ic] < 0) return;
     // column table: the diagonal first, from 0.0
-    for (int t = lane; t < kRapwHash; t += kWave) {
+    for (int t = lane; t < kHash; t += kWave) {
         s_key[t] = -1;
         s_pos[t] = -1;
     }
     __syncthreads();
     if (lane == 0) {
-        const unsigned d = rapw_hash(ic);
+        const unsigned d = rapw_hash<kHash>(ic);
         s_key[d] = ic;
         s_pos[d] = 0;
         if (kFill) s_val[d] = 0.0;
@@ -439,11 +443,11 @@ __global__ __launch_bounds__(kWave) void k_rapw(int nc, const int *__restrict__ 
             const bool lead = has && __builtin_ctzll(mine) == lane;
             unsigned slot = 0;
             if (lead) {   // distinct columns: only the first lane of each probes
-                slot = rapw_hash(col);
+                slot = rapw_hash<kHash>(col);
                 while (true) {
                     const int prev = atomicCAS(&s_key[slot], -1, col);
                     if (prev == -1 || prev == col) break;
-                    slot = (slot + 1) & (kRapwHash - 1u);
+                    slot = (slot + 1) & (kHash - 1u);
                 }
             }
             const bool fresh = lead && s_pos[slot] < 0;
@@ -467,7 +471,7 @@ __global__ __launch_bounds__(kWave) void k_rapw(int nc, const int *__restrict__ 
     }
     __syncthreads();
     const long long o = kSlots ? (long long)ic * max_cols : (long long)cia[ic];
-    for (int t = lane; t < kRapwHash; t += kWave)
+    for (int t = lane; t < kHash; t += kWave)
         if (s_key[t] >= 0) {
             cja[o + s_pos[t]] = s_key[t];
             cval[o + s_pos[t]] = s_val[t];
@@ -631,7 +635,7 @@ void csr_transpose_gpu(const DevCSR &A, DevCSR &AT) {
 
 int g_rap_wave_pairs = 0;
 
-void rap_general(const DevCSR &R, const DevCSR &A, const DevCSR &P, DevCSR &C) {
+void rap_general(const DevCSR &R, const DevCSR &A, const DevCSR &P, DevCSR &C, bool narrow_first) {
     hipStream_t st = stream();
     const int nc = R.n;
     C.n = nc;
@@ -646,30 +650,51 @@ void rap_general(const DevCSR &R, const DevCSR &A, const DevCSR &P, DevCSR &C) {
     }
     int *rowcount = arena().alloc_n<int>((size_t)nc + 1);
     const bool wave = g_rap_wave_pairs >= 0 || g_rap_wave_pairs == -2;
-    const int max_pairs = g_rap_wave_pairs > 0 ? std::min(g_rap_wave_pairs, kRapwPairs) : kRapwPairs;
-    const int max_cols = g_rap_wave_pairs == -2 ? 16 : kRapwCols;
+    int max_pairs = g_rap_wave_pairs > 0 ? std::min(g_rap_wave_pairs, kRapwPairs) : kRapwPairs;
+    int max_cols = g_rap_wave_pairs == -2 ? 16 : kRapwCols;
     // one pass into fixed slots of max_cols per row (then compacted) while the
     // slots fit kRapSlotBytes; otherwise a count pass, then the fill
     const bool slots = wave && (double)nc * max_cols * 12.0 <= kRapSlotBytes;
     int *slotcount = nullptr, *sja = nullptr;
     double *sval = nullptr;
-    if (slots) {
+    int *list = arena().alloc_n<int>((size_t)nc + 1);
+    int *nlist = arena().alloc_n<int>(1);
+    bool narrow_done = false;
+    if (slots && narrow_first && g_rap_wave_pairs == 0) {
+        // half-size tables first (twice the workgroups per CU); when a row
+        // passes them, the whole product runs again with the full tables
+        slotcount = arena().alloc_n<int>((size_t)nc + 1);
+        sja = arena().alloc_n<int>((size_t)nc * kRapwNarrowCols);
+        sval = arena().alloc_n<double>((size_t)nc * kRapwNarrowCols);
+        k_rapw<true, true, kRapwNarrowCols><<<nc, kWave, 0, st>>>(nc, R.ia, R.ja, R.val, A.ia, A.ja, A.val, P.ia,
+                                                                  P.ja, P.val, kRapwNarrowCols, kRapwNarrowCols,
+                                                                  slotcount, nullptr, sja, sval);
+        HZ_CHECK(hipMemsetAsync(nlist, 0, sizeof(int), st));
+        k_rapw_big<<<grid_for(nc), kBlock, 0, st>>>(nc, slotcount, list, nlist);
+        HZ_LAUNCH_CHECK();
+        int over = 0;
+        HZ_CHECK(memcpy_sync(&over, nlist, sizeof(int), hipMemcpyDeviceToHost));
+        narrow_done = over == 0;
+        if (narrow_done) {
+            max_pairs = max_cols = kRapwNarrowCols;
+            HZ_CHECK(hipMemcpyAsync(rowcount, slotcount, sizeof(int) * (size_t)nc, hipMemcpyDeviceToDevice, st));
+        }
+    }
+    if (slots && !narrow_done) {
         slotcount = arena().alloc_n<int>((size_t)nc + 1);
         sja = arena().alloc_n<int>((size_t)nc * max_cols);
         sval = arena().alloc_n<double>((size_t)nc * max_cols);
         k_rapw<true, true><<<nc, kWave, 0, st>>>(nc, R.ia, R.ja, R.val, A.ia, A.ja, A.val, P.ia, P.ja, P.val,
                                                  max_pairs, max_cols, slotcount, nullptr, sja, sval);
         HZ_CHECK(hipMemcpyAsync(rowcount, slotcount, sizeof(int) * (size_t)nc, hipMemcpyDeviceToDevice, st));
-    } else if (wave) {
+    } else if (wave && !slots) {
         k_rapw<false><<<nc, kWave, 0, st>>>(nc, R.ia, R.ja, R.val, A.ia, A.ja, A.val, P.ia, P.ja, P.val, max_pairs,
                                             max_cols, rowcount, nullptr, nullptr, nullptr);
-    } else {
+    } else if (!wave) {
         HZ_CHECK(hipMemsetAsync(rowcount, 0xff, sizeof(int) * (size_t)nc, st));   // every row -1
     }
     HZ_LAUNCH_CHECK();
     // rows beyond the wavefront kernel's LDS: one thread per row, global tables
-    int *list = arena().alloc_n<int>((size_t)nc + 1);
-    int *nlist = arena().alloc_n<int>(1);
     HZ_CHECK(hipMemsetAsync(nlist, 0, sizeof(int), st));
     k_rapw_big<<<grid_for(nc), kBlock, 0, st>>>(nc, rowcount, list, nlist);
     HZ_LAUNCH_CHECK();

@@ -742,6 +742,37 @@ def test_sa_kuhn3d_hierarchy_and_solve(lib, orc, libc):
     assert np.linalg.norm(xg - xo) / np.linalg.norm(xo) < 1e-10
 
 
+@pytest.mark.parametrize("agg,wide", [(capi.VMB, False), (capi.HEC, True)])
+def test_sa_galerkin_narrow_tables_first_bitexact(lib, orc, libc, agg, wide):
+    """The default form of the finest level's general Galerkin product first
+    runs with 128-column tables and, when a row has more columns, runs again
+    with the full 256-column ones: 3-D Kuhn 16^3 with VMB aggregates (coarse
+    rows within 128 columns) and with HEC pairs (wider rows: the second run).
+    Every coarse matrix bit-identical to the oracle's."""
+    ia, ja, va = problems.poisson3d_kuhn(16)
+    A = capi.HostCSR(ia, ja, va)
+    out = []
+    for which in ("gpu", "oracle"):
+        import oracle
+        fn = (lambda **k: capi.solver_params(lib, **k)) if which == "gpu" else oracle.params
+        _, amg = _sa_param(fn, agg, 1, theta=0.04, coarse_dof=60)
+        h = capi.hz_hierarchy()
+        libc.srand(1)
+        st = (lib.hz_setup_hierarchy if which == "gpu" else orc.orc_setup_hierarchy)(A.ptr(), C.byref(amg),
+                                                                                      C.byref(h))
+        assert st == 0
+        out.append(capi.hierarchy_to_numpy(h))
+        (lib.hz_hierarchy_free if which == "gpu" else orc.orc_hierarchy_free)(C.byref(h))
+    lg, lo = out
+    assert len(lg) == len(lo) and len(lg) > 1
+    widest = int(np.diff(lg[1]["ia"]).max())
+    print(f"aggregation {agg}: widest row of the level-1 operator {widest}", flush=True)
+    assert (widest > 128) == wide
+    for g, o in zip(lg, lo):
+        for key in ("ia", "ja", "val"):
+            assert np.array_equal(g[key], o[key]), key
+
+
 @pytest.mark.parametrize("problem", ["kuhn16_hec", "p2d40_hem"])
 @pytest.mark.parametrize("form", [-1, -2, 100])
 def test_sa_galerkin_product_forms_bitexact(lib, orc, libc, problem, form):
