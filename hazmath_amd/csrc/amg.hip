@@ -184,6 +184,16 @@ int amg_setup(Hierarchy &H, const DevCSR &A0, AMG_param *param, bool) {
         if (tr) { HZ_CHECK(hipStreamSynchronize(stream())); t0 = wall_ms(); }
         DevCSR N;
         strength_graph(L.A, param->strong_coupled, N, counts, L.S.dfirst);
+        // learned SA: the Chebyshev diagonal and Gershgorin bound, and the Lanczos
+        // estimate behind the bound's floor, need only A: the estimate runs on the
+        // second stream while the host aggregates
+        const bool learned_level = param->AMG_type == HZ_SA_AMG && H.mlp_theta && !(lvl == 0 && H.omega_level0);
+        ChebLevel g1;
+        LanczosAsync lz;
+        if (learned_level) {
+            g1 = cheby_prepare(L.A, 1.0);
+            if (g1.lmax > 0.0) lz.start(L.A, g1.diag, 30);
+        }
         if (tr) { HZ_CHECK(hipStreamSynchronize(stream())); t1 = wall_ms(); }
         L.agg = arena().alloc_n<int>((size_t)n + 1);
         int nagg = 0, st = HZ_SUCCESS;
@@ -235,15 +245,13 @@ int amg_setup(Hierarchy &H, const DevCSR &A0, AMG_param *param, bool) {
                 float *om = arena().alloc_n<float>((size_t)nagg + 1);
                 double scale = learned_parameters(nagg, out, om);
                 // floor: the Chebyshev interval must reach the level's lambda_max(D^-1 A).  The Lanczos
-                // estimate (up to 30 steps) is from below, hence 1.1x; the Gershgorin bound (scale 1)
-                // always covers lambda_max, so the floor never exceeds 1 and is not needed (nor estimated)
-                // when the network asks for 1 or more.  The estimate stops once 1.1x its Ritz value
-                // reaches the Gershgorin bound: the floor is then 1 whatever the later steps give.
-                const ChebLevel g1 = cheby_prepare(L.A, 1.0);
+                // estimate (30 steps, queued before the aggregation) is from below, hence 1.1x; the
+                // Gershgorin bound (scale 1) always covers lambda_max, so the floor never exceeds 1 and is
+                // not needed (its result not read) when the network asks for 1 or more.
                 if ((int)H.cheb_scale.size() < max_levels) H.cheb_scale.assign(max_levels, 1.0);
                 if ((int)H.lambda_ratio.size() < max_levels) H.lambda_ratio.assign(max_levels, 0.0);
                 if (g1.lmax > 0.0 && scale < 1.0) {
-                    const double ratio = lanczos_lambda_dinv(L.A, g1.diag, 30, nullptr, g1.lmax / 1.1) / g1.lmax;
+                    const double ratio = lz.finish() / g1.lmax;
                     const double floor = std::min(1.0, 1.1 * ratio);
                     scale = std::min(1.5, std::max(scale, floor));
                     H.lambda_ratio[lvl] = ratio;

@@ -240,6 +240,22 @@ double learned_parameters(int nagg, const float *out, float *omega);
 // few steps and returned once it reaches stop_at (it only grows with the
 // steps), which saves the remaining steps when only "at least stop_at" matters.
 double lanczos_lambda_dinv(const DevCSR &A, const double *d, int steps, int *steps_done, double stop_at = 0.0);
+// The library's second stream (default priority).
+hipStream_t side_stream();
+// The same estimate (all `steps`, no early stop) queued on a second stream
+// (default priority), so that it runs while the host works; finish() waits
+// for it and returns the largest Ritz value, drain() only waits.
+struct LanczosAsync {
+    bool active = false;
+    double *ab = nullptr;
+    int nsteps = 0;
+    void start(const DevCSR &A, const double *d, int steps);
+    double finish();
+    void drain();
+    ~LanczosAsync() {
+        if (active) (void)hipStreamSynchronize(side_stream());
+    }
+};
 // feat [nagg][12] (FP32) of level `level`
 void aggregate_features(const DevCSR &A, const int *agg, int nagg, const int *rptr, const int *rmem, int level,
                         float *feat);

@@ -482,18 +482,29 @@ static double ritz_max(const double *ha, const double *hb, int m, int *used) {
     return hi;
 }
 
-double lanczos_lambda_dinv(const DevCSR &A, const double *d, int steps, int *steps_done, double stop_at) {
+namespace {
+// The estimate's device state: the recurrence scalars alpha[0..steps),
+// beta[0..steps) and |w0| at ab.
+struct LanczosRun {
+    double *ab = nullptr;
+    int steps = 0;
+};
+
+// Queues the start vector and `steps` Lanczos steps on st; when stop_at > 0
+// the host looks at the Ritz values every kLzCheckSteps steps and stops once
+// the largest reaches stop_at (*stopped: its value).
+LanczosRun lanczos_enqueue(const DevCSR &A, const double *d, int steps, hipStream_t st, double stop_at,
+                           double *stopped, int *steps_done) {
     const int n = A.n;
-    if (steps_done) *steps_done = 0;
-    if (n == 0 || steps <= 0) return 0.0;
-    hipStream_t st = stream();
+    LanczosRun run;
+    run.steps = steps;
     const int G = std::max(1, std::min(kLzGrid, div_up((long)n * kLzLanes, kBlock)));
     double *isd = arena().alloc_n<double>((size_t)n + 1);
     double *vb[2] = {arena().alloc_n<double>((size_t)n + 1), arena().alloc_n<double>((size_t)n + 1)};
     double *yb[2] = {arena().alloc_n<double>((size_t)n + 1), arena().alloc_n<double>((size_t)n + 1)};
     double *part = arena().alloc_n<double>((size_t)G + 1);
-    double *ab = arena().alloc_n<double>(2 * (size_t)steps + 2);   // alpha[0..steps), beta[0..steps), |w0|
-    double *alpha = ab, *beta = ab + steps, *nrm0 = ab + 2 * steps;
+    run.ab = arena().alloc_n<double>(2 * (size_t)steps + 2);   // alpha[0..steps), beta[0..steps), |w0|
+    double *alpha = run.ab, *beta = run.ab + steps, *nrm0 = run.ab + 2 * steps;
     std::vector<double> h(2 * (size_t)steps);
     const double *ha = h.data(), *hb = h.data() + steps;
     // y[1] holds the start vector: step 0 reads it as "wprev" with norm |w0|
@@ -513,21 +524,76 @@ double lanczos_lambda_dinv(const DevCSR &A, const double *d, int steps, int *ste
         // stop_at, more steps cannot bring it back below
         const int m = k + 1;
         if (stop_at > 0.0 && m < steps && m % kLzCheckSteps == 0) {
-            HZ_CHECK(memcpy_sync(h.data(), alpha, sizeof(double) * m, hipMemcpyDeviceToHost));
-            HZ_CHECK(memcpy_sync(h.data() + steps, beta, sizeof(double) * m, hipMemcpyDeviceToHost));
+            HZ_CHECK(hipMemcpyAsync(h.data(), alpha, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+            HZ_CHECK(hipMemcpyAsync(h.data() + steps, beta, sizeof(double) * m, hipMemcpyDeviceToHost, st));
+            HZ_CHECK(hipStreamSynchronize(st));
             int used = 0;
             const double theta = ritz_max(ha, hb, m, &used);
             if (theta >= stop_at) {
                 if (steps_done) *steps_done = used;
-                return theta;
+                *stopped = theta;
+                run.steps = -1;
+                return run;
             }
         }
     }
-    HZ_CHECK(memcpy_sync(h.data(), ab, sizeof(double) * 2 * steps, hipMemcpyDeviceToHost));
+    return run;
+}
+
+// The largest Ritz value of a queued run (waits for st).
+double lanczos_finish(const LanczosRun &run, hipStream_t st, int *steps_done) {
+    const int steps = run.steps;
+    std::vector<double> h(2 * (size_t)steps);
+    HZ_CHECK(hipMemcpyAsync(h.data(), run.ab, sizeof(double) * 2 * steps, hipMemcpyDeviceToHost, st));
+    HZ_CHECK(hipStreamSynchronize(st));
     int used = 0;
-    const double theta = ritz_max(ha, hb, steps, &used);
+    const double theta = ritz_max(h.data(), h.data() + steps, steps, &used);
     if (steps_done) *steps_done = used;
     return theta;
+}
+}  // namespace
+
+double lanczos_lambda_dinv(const DevCSR &A, const double *d, int steps, int *steps_done, double stop_at) {
+    if (steps_done) *steps_done = 0;
+    if (A.n == 0 || steps <= 0) return 0.0;
+    double stopped = 0.0;
+    const LanczosRun run = lanczos_enqueue(A, d, steps, stream(), stop_at, &stopped, steps_done);
+    if (run.steps < 0) return stopped;
+    return lanczos_finish(run, stream(), steps_done);
+}
+
+hipStream_t side_stream() {
+    static hipStream_t s = nullptr;
+    if (!s) HZ_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));   // default priority
+    return s;
+}
+
+void LanczosAsync::start(const DevCSR &A, const double *d, int steps) {
+    static hipEvent_t ready = nullptr;
+    if (!ready) HZ_CHECK(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    active = A.n > 0 && steps > 0;
+    if (!active) return;
+    // the side stream starts once the main stream has formed A and d
+    HZ_CHECK(hipEventRecord(ready, stream()));
+    HZ_CHECK(hipStreamWaitEvent(side_stream(), ready, 0));
+    const LanczosRun run = lanczos_enqueue(A, d, steps, side_stream(), 0.0, nullptr, nullptr);
+    ab = run.ab;
+    nsteps = run.steps;
+}
+
+double LanczosAsync::finish() {
+    if (!active) return 0.0;
+    active = false;
+    LanczosRun run;
+    run.ab = ab;
+    run.steps = nsteps;
+    return lanczos_finish(run, side_stream(), nullptr);
+}
+
+void LanczosAsync::drain() {
+    if (!active) return;
+    active = false;
+    HZ_CHECK(hipStreamSynchronize(side_stream()));
 }
 
 double learned_parameters(int nagg, const float *out, float *omega) {

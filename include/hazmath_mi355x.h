@@ -388,10 +388,8 @@ typedef struct {
     REAL   lambda_ratio[HZ_MAX_AMG_LVL]; /* learned SA: estimated lambda_max(D^-1 A) over the
                                             Gershgorin bound per level, 0 where not estimated
                                             (levels whose network scale is 1 or more: the
-                                            floor, at most 1, cannot bind there); the Lanczos
-                                            estimate stops early (every 8 steps it is looked
-                                            at) once 1.1x the ratio reaches 1, where the floor
-                                            is 1 whatever later steps give */
+                                            floor, at most 1, cannot bind there); 30 Lanczos
+                                            steps, run while the host aggregates the level */
     REAL   solve_alg_bytes;  /* algorithmic bytes of every kernel the PCG launched (each distinct
                                 input read once, each output written once; the gathered coarse
                                 corrections of the transfers not counted): over solve_ms, the
