@@ -587,13 +587,15 @@ void presmooth_m(Hierarchy &H, int l, CycleState &cs) {
     const float w = (float)cp.jacobi_w;
     int sweeps = cp.smoother == HZ_SMOOTHER_CHEBY ? cheb_steps_at(cp, l, cp.presmooth) : cp.presmooth;
     if (cp.smoother == HZ_SMOOTHER_CHEBY) {
-        // from zero, the first step reads no x (nor the matrix)
-        const bool from_zero = cs.zero[l] && sweeps > 0;
-        if (cs.zero[l] && !from_zero) fill_f(L.xf, L.A.n);
+        // from zero, the first step reads no x (nor the matrix); it may have
+        // been formed already, where b was
+        const bool formed = cs.zero[l] && cs.first_done[l] && sweeps > 0;
+        const bool from_zero = cs.zero[l] && sweeps > 0 && !formed;
+        if (cs.zero[l] && sweeps <= 0) fill_f(L.xf, L.A.n);
         cs.zero[l] = false;
         cs.first_done[l] = false;
         cheby_apply_m(L.S, L.A, L.low, L.cheb, L.cdiagf, sweeps, L.bf, L.xf, L.xtf, L.cworkf, nullptr, nullptr,
-                      nullptr, H.ws, from_zero);
+                      nullptr, H.ws, from_zero, formed);
         return;
     }
     if (cs.zero[l]) {
@@ -671,9 +673,15 @@ forward:
         presmooth_m(H, l, cs);
         // the next level's first sweep from zero rides on the restriction (UA, Jacobi / L1)
         const bool fuse = (l + 1 < nl - 1) && cp.presmooth > 0 && cp.smoother != HZ_SMOOTHER_CHEBY && !L.sa;
+        // Chebyshev below: the next level's first step from zero rides on the restriction
+        const bool cheb_fuse = L.sa && cp.smoother == HZ_SMOOTHER_CHEBY && (l + 1 < nl - 1) &&
+                               cheb_steps_at(cp, l + 1, cp.presmooth) > 0;
         if (L.sa) {
             residual_m(L.S, L.low, L.bf, L.xf, L.wf, &L.A);
-            csr_mxv_m(L.R, L.Rf, L.wf, C.bf);   // dcsr_mxv(R), mgcycle.c:302
+            if (cheb_fuse)   // dcsr_mxv(R), mgcycle.c:302
+                csr_mxv_m(L.R, L.Rf, L.wf, C.bf, C.cdiagf, cheb_first_c2(C.cheb), C.xf, C.cworkf);
+            else
+                csr_mxv_m(L.R, L.Rf, L.wf, C.bf);
         } else if (L.A.n <= fused_down_max_rows()) {
             down_fused_m(L.nagg, L.rptr, L.rmem, L.A, L.low, L.bf, L.xf, C.bf, cp.smoother, w,
                          fuse ? C.dsmf : nullptr, fuse ? C.xf : nullptr);
@@ -684,7 +692,7 @@ forward:
         }
         ++l;
         cs.zero[l] = true;
-        cs.first_done[l] = fuse;
+        cs.first_done[l] = fuse || cheb_fuse;
     }
     {
         Level &C = H.lv[nl - 1];
@@ -717,14 +725,19 @@ bool amg_precond_m(Hierarchy &H, const double *r, double *z, double *zr_out) {
         return false;
     }
     const bool first_fused = cp.presmooth > 0 && cp.smoother != HZ_SMOOTHER_CHEBY;
-    enter_m(n, r, cp.smoother, (float)cp.jacobi_w, L0.dsmf, L0.bf, first_fused ? L0.xf : nullptr);
+    // Chebyshev: its first step from zero is formed with b
+    const bool cheb_first = cp.smoother == HZ_SMOOTHER_CHEBY && cheb_steps_at(cp, 0, cp.presmooth) > 0;
+    if (cheb_first)
+        enter_m(n, r, cp.smoother, (float)cp.jacobi_w, L0.cdiagf, L0.bf, L0.xf, cheb_first_c2(L0.cheb), L0.cworkf);
+    else
+        enter_m(n, r, cp.smoother, (float)cp.jacobi_w, L0.dsmf, L0.bf, first_fused ? L0.xf : nullptr);
     CycleState cs;
     for (int i = 0; i < HZ_MAX_AMG_LVL; ++i) {
         cs.zero[i] = false;
         cs.first_done[i] = false;
     }
     cs.zero[0] = true;
-    cs.first_done[0] = first_fused;
+    cs.first_done[0] = first_fused || cheb_first;
     bool z_written = false;
     for (int it = 0; it < cp.maxit; ++it) z_written = mgcycle_m(H, cs, it == cp.maxit - 1, z, r, zr_out);
     if (!z_written) {

@@ -299,18 +299,26 @@ float *values_f32(const DevCSR &A);
 // `steps` Chebyshev steps in FP32 on the level's low-precision values (x and
 // xt swap roles each step); with out64 the last step writes x + d to out64 in
 // FP64 instead, with (out64, r64) summed into *dot_out when dot_out is set.
-// x_zero: as for cheby_apply.
+// x_zero: as for cheby_apply; first_done: x and dwork already hold the first
+// step from zero (formed where b was).
 void cheby_apply_m(const DevSELL &S, const DevCSR &A, const LowMatrix &M, const ChebLevel &c, const float *diag,
                    int steps, const float *b, float *&x, float *&xt, float *dwork, double *out64, const double *r64,
-                   double *dot_out, const ReduceWs &ws, bool x_zero = false);
+                   double *dot_out, const ReduceWs &ws, bool x_zero = false, bool first_done = false);
 // y = A x and x += P xc in FP32 with FP32 values (smoothed aggregation transfers)
-void csr_mxv_m(const DevCSR &A, const float *val, const float *x, float *y);
+void csr_mxv_m(const DevCSR &A, const float *val, const float *x, float *y, const float *diag = nullptr,
+               float c2 = 0.0f, float *xfirst = nullptr, float *dfirst = nullptr);
+// xfirst (csr_mxv_m): also the next level's first Chebyshev step from zero on y
+// (diag, c2 = cheb_first_c2 of that level; dfirst receives its d)
+float cheb_first_c2(const ChebLevel &c);
 void csr_prolong_m(const DevCSR &P, const float *val, const float *xc, float *x);
 void d2f(const double *a, float *b, long n);
 void f2d(const float *a, double *b, long n);
 void fill_f(float *x, long n);
-// b = fl(r); x = first sweep from zero when x != null
-void enter_m(int n, const double *r, int smoother, float w, const float *d, float *b, float *x);
+// b = fl(r); x = first sweep from zero when x != null; with dcheb, the
+// Chebyshev smoother's first step from zero instead (d: its diagonal, c2 its
+// first coefficient; dcheb receives the step's d)
+void enter_m(int n, const double *r, int smoother, float w, const float *d, float *b, float *x, float c2 = 0.0f,
+             float *dcheb = nullptr);
 // xout = sweep(x [+ P xc when agg != null]); with out64: writes the result in
 // FP64 to out64 instead and sum_i out64_i r64_i to dot_out.
 void sweep_m(const DevSELL &S, const LowMatrix &M, int smoother, float w, const int *agg, const float *xc,

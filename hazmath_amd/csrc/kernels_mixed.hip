@@ -73,11 +73,18 @@ __device__ __forceinline__ float zero_sweep(int smoother, float w, float b, floa
 
 // b = fl(r); optionally x = first sweep from zero
 __global__ void k_enter(int n, const double *__restrict__ r, int smoother, float w, const float *__restrict__ d,
-                        float *__restrict__ b, float *__restrict__ x) {
+                        float *__restrict__ b, float *__restrict__ x, float c2, float *__restrict__ dcheb) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const float bi = (float)r[i];
         b[i] = bi;
-        if (x) x[i] = zero_sweep(smoother, w, bi, d[i]);
+        if (dcheb) {   // Chebyshev's first step from zero (k_cheb_first_m's values)
+            const float di = d[i];
+            const float dn = 0.0f + (di != 0.0f ? c2 * bi / di : 0.0f);
+            dcheb[i] = dn;
+            x[i] = 0.0f + dn;
+        } else if (x) {
+            x[i] = zero_sweep(smoother, w, bi, d[i]);
+        }
     }
 }
 
@@ -433,7 +440,9 @@ __global__ void k_cheb_step_grp_m(int n, const int *__restrict__ ia, const int *
 // that order.
 template <int kLanes>
 __global__ void k_csr_mxv_m(int n, const int *__restrict__ ia, const int *__restrict__ ja,
-                            const float *__restrict__ a, const float *__restrict__ x, float *__restrict__ y) {
+                            const float *__restrict__ a, const float *__restrict__ x, float *__restrict__ y,
+                            const float *__restrict__ diag, float c2, float *__restrict__ xfirst,
+                            float *__restrict__ dfirst) {
     constexpr int kPass = 4;
     const long gt = (long)blockIdx.x * blockDim.x + threadIdx.x;
     const int r = (int)(gt / kLanes), j = (int)(gt % kLanes);
@@ -455,7 +464,15 @@ __global__ void k_csr_mxv_m(int n, const int *__restrict__ ia, const int *__rest
     }
 #pragma unroll
     for (int o = kLanes / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kWave);
-    if (valid && j == 0) y[r] = acc;
+    if (valid && j == 0) {
+        y[r] = acc;
+        if (xfirst) {   // the coarse level's first Chebyshev step from zero (k_cheb_first_m's values)
+            const float di = diag[r];
+            const float dn = 0.0f + (di != 0.0f ? c2 * acc / di : 0.0f);
+            dfirst[r] = dn;
+            xfirst[r] = 0.0f + dn;
+        }
+    }
 }
 
 // x += P xc in FP32 (smoothed aggregation's prolongation), one thread per row;
@@ -594,10 +611,11 @@ void fill_f(float *x, long n) {
     HZ_LAUNCH_CHECK();
 }
 
-void enter_m(int n, const double *r, int smoother, float w, const float *d, float *b, float *x) {
+void enter_m(int n, const double *r, int smoother, float w, const float *d, float *b, float *x, float c2,
+             float *dcheb) {
     if (n <= 0) return;
     count_bytes(8.0 * n + 4.0 * n + (x ? 8.0 * n : 0.0));
-    k_enter<<<grid_for(n), kBlock, 0, stream()>>>(n, r, smoother, w, d, b, x);
+    k_enter<<<grid_for(n), kBlock, 0, stream()>>>(n, r, smoother, w, d, b, x, c2, dcheb);
     HZ_LAUNCH_CHECK();
 }
 
@@ -681,7 +699,7 @@ void prolong_m(int n, const int *agg, const float *xc, float *x) {
 
 void cheby_apply_m(const DevSELL &S, const DevCSR &A, const LowMatrix &M, const ChebLevel &c, const float *diag,
                    int steps, const float *b, float *&x, float *&xt, float *dwork, double *out64, const double *r64,
-                   double *dot_out, const ReduceWs &ws, bool x_zero) {
+                   double *dot_out, const ReduceWs &ws, bool x_zero, bool first_done) {
     if (S.n == 0 || steps <= 0) return;
     const double theta = 0.5 * (c.lmax + c.lmin), delta = 0.5 * (c.lmax - c.lmin);
     const double sigma = theta / delta;
@@ -698,6 +716,7 @@ void cheby_apply_m(const DevSELL &S, const DevCSR &A, const LowMatrix &M, const 
             rho = rn;
         }
         const bool last64 = out64 && k == steps - 1;
+        if (k == 0 && first_done && !last64) continue;   // x and dwork hold the first step already
         if (k == 0 && x_zero && !last64) {
             count_bytes(16.0 * S.n);
             k_cheb_first_m<<<div_up(S.n, kBlock), kBlock, 0, stream()>>>(S.n, diag, (float)c2, b, xt, dwork);
@@ -744,12 +763,19 @@ void cheby_apply_m(const DevSELL &S, const DevCSR &A, const LowMatrix &M, const 
     }
 }
 
-void csr_mxv_m(const DevCSR &A, const float *val, const float *x, float *y) {
+void csr_mxv_m(const DevCSR &A, const float *val, const float *x, float *y, const float *diag, float c2,
+               float *xfirst, float *dfirst) {
     if (A.n <= 0) return;
-    count_bytes(matrix_bytes(A.nnz, A.n, 4) + 4.0 * A.ncol + 4.0 * A.n);
+    count_bytes(matrix_bytes(A.nnz, A.n, 4) + 4.0 * A.ncol + 4.0 * A.n + (xfirst ? 12.0 * A.n : 0.0));
     constexpr int G = 16;
-    k_csr_mxv_m<G><<<div_up((long)A.n * G, kBlock), kBlock, 0, stream()>>>(A.n, A.ia, A.ja, val, x, y);
+    k_csr_mxv_m<G><<<div_up((long)A.n * G, kBlock), kBlock, 0, stream()>>>(A.n, A.ia, A.ja, val, x, y, diag, c2,
+                                                                            xfirst, dfirst);
     HZ_LAUNCH_CHECK();
+}
+
+float cheb_first_c2(const ChebLevel &c) {
+    const double theta = 0.5 * (c.lmax + c.lmin);
+    return (float)(1.0 / theta);
 }
 
 void csr_prolong_m(const DevCSR &P, const float *val, const float *xc, float *x) {
