@@ -236,10 +236,8 @@ double learned_parameters(int nagg, const float *out, float *omega);
 // lambda_max(D^-1 A) as the largest Ritz value of `steps` Lanczos steps on
 // D^-1/2 A D^-1/2 (d: the diagonal the Chebyshev smoother divides by); an
 // estimate from below.  *steps_done (optional): steps before the Krylov space
-// became invariant.  stop_at > 0: the largest Ritz value is looked at every
-// few steps and returned once it reaches stop_at (it only grows with the
-// steps), which saves the remaining steps when only "at least stop_at" matters.
-double lanczos_lambda_dinv(const DevCSR &A, const double *d, int steps, int *steps_done, double stop_at = 0.0);
+// became invariant.
+double lanczos_lambda_dinv(const DevCSR &A, const double *d, int steps, int *steps_done);
 // The library's second stream (default priority).
 hipStream_t side_stream();
 // The same estimate (all `steps`, no early stop) queued on a second stream

@@ -436,9 +436,6 @@ __global__ __launch_bounds__(kBlock) void k_lz_start(int n, const double *__rest
     if (threadIdx.x == 0) part[blockIdx.x] = bs;
 }
 
-// An early-stopping estimate looks at its Ritz values every this many steps.
-constexpr int kLzCheckSteps = 8;
-
 // Number of eigenvalues of the symmetric tridiagonal (diag a, off-diagonal b)
 // below x (Sturm sequence).
 int sturm_below(int m, const double *a, const double *b, double x) {
@@ -490,11 +487,8 @@ struct LanczosRun {
     int steps = 0;
 };
 
-// Queues the start vector and `steps` Lanczos steps on st; when stop_at > 0
-// the host looks at the Ritz values every kLzCheckSteps steps and stops once
-// the largest reaches stop_at (*stopped: its value).
-LanczosRun lanczos_enqueue(const DevCSR &A, const double *d, int steps, hipStream_t st, double stop_at,
-                           double *stopped, int *steps_done) {
+// Queues the start vector and `steps` Lanczos steps on st.
+LanczosRun lanczos_enqueue(const DevCSR &A, const double *d, int steps, hipStream_t st) {
     const int n = A.n;
     LanczosRun run;
     run.steps = steps;
@@ -505,8 +499,6 @@ LanczosRun lanczos_enqueue(const DevCSR &A, const double *d, int steps, hipStrea
     double *part = arena().alloc_n<double>((size_t)G + 1);
     run.ab = arena().alloc_n<double>(2 * (size_t)steps + 2);   // alpha[0..steps), beta[0..steps), |w0|
     double *alpha = run.ab, *beta = run.ab + steps, *nrm0 = run.ab + 2 * steps;
-    std::vector<double> h(2 * (size_t)steps);
-    const double *ha = h.data(), *hb = h.data() + steps;
     // y[1] holds the start vector: step 0 reads it as "wprev" with norm |w0|
     k_lz_start<<<G, kBlock, 0, st>>>(n, d, isd, yb[1], part);
     k_sum_partials<<<1, kBlock, 0, st>>>(part, G, 1, nrm0);
@@ -520,22 +512,6 @@ LanczosRun lanczos_enqueue(const DevCSR &A, const double *d, int steps, hipStrea
         k_lz_orth<<<G, kBlock, 0, st>>>(n, alpha + k, k > 0 ? beta + k - 1 : nullptr, v, vprev, y, part);
         k_sum_partials<<<1, kBlock, 0, st>>>(part, G, 1, beta + k);
         HZ_LAUNCH_CHECK();
-        // Ritz values only grow with the step count: once the largest reaches
-        // stop_at, more steps cannot bring it back below
-        const int m = k + 1;
-        if (stop_at > 0.0 && m < steps && m % kLzCheckSteps == 0) {
-            HZ_CHECK(hipMemcpyAsync(h.data(), alpha, sizeof(double) * m, hipMemcpyDeviceToHost, st));
-            HZ_CHECK(hipMemcpyAsync(h.data() + steps, beta, sizeof(double) * m, hipMemcpyDeviceToHost, st));
-            HZ_CHECK(hipStreamSynchronize(st));
-            int used = 0;
-            const double theta = ritz_max(ha, hb, m, &used);
-            if (theta >= stop_at) {
-                if (steps_done) *steps_done = used;
-                *stopped = theta;
-                run.steps = -1;
-                return run;
-            }
-        }
     }
     return run;
 }
@@ -553,13 +529,10 @@ double lanczos_finish(const LanczosRun &run, hipStream_t st, int *steps_done) {
 }
 }  // namespace
 
-double lanczos_lambda_dinv(const DevCSR &A, const double *d, int steps, int *steps_done, double stop_at) {
+double lanczos_lambda_dinv(const DevCSR &A, const double *d, int steps, int *steps_done) {
     if (steps_done) *steps_done = 0;
     if (A.n == 0 || steps <= 0) return 0.0;
-    double stopped = 0.0;
-    const LanczosRun run = lanczos_enqueue(A, d, steps, stream(), stop_at, &stopped, steps_done);
-    if (run.steps < 0) return stopped;
-    return lanczos_finish(run, stream(), steps_done);
+    return lanczos_finish(lanczos_enqueue(A, d, steps, stream()), stream(), steps_done);
 }
 
 hipStream_t side_stream() {
@@ -576,7 +549,7 @@ void LanczosAsync::start(const DevCSR &A, const double *d, int steps) {
     // the side stream starts once the main stream has formed A and d
     HZ_CHECK(hipEventRecord(ready, stream()));
     HZ_CHECK(hipStreamWaitEvent(side_stream(), ready, 0));
-    const LanczosRun run = lanczos_enqueue(A, d, steps, side_stream(), 0.0, nullptr, nullptr);
+    const LanczosRun run = lanczos_enqueue(A, d, steps, side_stream());
     ab = run.ab;
     nsteps = run.steps;
 }
