@@ -32,8 +32,10 @@ int vmb_pass1(int n, const int *ia, const int *ja, int maxagg, int *v, std::vect
             --left;
             continue;
         }
+        // "every neighbour still free": scanned from the row's end, where the
+        // higher rows, swept (and aggregated) first, sit -- the same answer
         bool untouched = true;
-        for (int k = ia[i]; k < ia[i + 1] && untouched; ++k)
+        for (int k = ia[i + 1] - 1; k >= ia[i] && untouched; --k)
             if (v[ja[k]] >= HZ_UNPT) untouched = false;
         if (!untouched) continue;
         int cnt = 1, members = 1;
