@@ -393,7 +393,8 @@ void amg_prepare_cycle(Hierarchy &H, const AMG_param *param) {
                 L.wf = arena().alloc_n<float>(n);
                 L.dsmf = arena().alloc_n<float>(n);
                 d2f(L.dsm, L.dsmf, L.A.n);
-                low_values(L.S, L.A, (cp.precision == 2 && l > 0) ? kLowBF16 : kLowF32, L.low);
+                low_values(L.S, L.A, (cp.precision == 2 && l > 0) ? kLowBF16 : kLowF32, L.low,
+                           cp.smoother == HZ_SMOOTHER_CHEBY);
                 if (L.sa) {
                     L.Pf = values_f32(L.P);
                     L.Rf = values_f32(L.R);

@@ -290,8 +290,14 @@ struct LowMatrix {
     int kind = 0;
     void *sell = nullptr;
     void *csr = nullptr;
+    // the SELL column indices again as 16-bit offsets from the row (padding
+    // kColPad16), two entries of a row to a 32-bit word, slice s from word
+    // dcptr[s], when every stored entry's fits; null otherwise
+    unsigned int *dcol = nullptr;
+    long long *dcptr = nullptr;
 };
-void low_values(const DevSELL &S, const DevCSR &A, int kind, LowMatrix &out);
+constexpr short kColPad16 = -32768;
+void low_values(const DevSELL &S, const DevCSR &A, int kind, LowMatrix &out, bool col16 = false);
 // FP32 copy of a CSR matrix's values (smoothed aggregation's P and R).
 float *values_f32(const DevCSR &A);
 // `steps` Chebyshev steps in FP32 on the level's low-precision values (x and

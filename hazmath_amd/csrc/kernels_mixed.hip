@@ -10,6 +10,8 @@
 // the PCG sees the same interface as with the FP64 cycle.  The coarsest solve
 // uses the FP64 inverse (FP32 in and out, FP64 accumulation).
 #include <algorithm>
+#include <cstdlib>
+#include <vector>
 
 #include "hz_internal.h"
 
@@ -33,6 +35,31 @@ inline int grid_for(long n) {
 __device__ __forceinline__ float ldv(const float *p) { return ld_nt(p); }
 __device__ __forceinline__ float ldv(const unsigned short *p) {
     return __uint_as_float(((unsigned int)ld_nt(p)) << 16);
+}
+
+// Column indices of a round of U SELL entries (k .. k+U-1 of row r, -1 past
+// the width wd or for padding), streamed once per launch: the int32 ones, or
+// the 16-bit offsets from the row stored two entries to a 32-bit word (entry
+// 2p of the row in the low half of word p, 2p+1 in the high half; words of a
+// slice lane-major as the SELL entries are), one load per pair.
+template <int U>
+__device__ __forceinline__ void ld_cols(const int *c, int k, int wd, int, int (&cc)[U]) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) cc[u] = k + u < wd ? ld_nt(c + (k + u) * kSlice) : -1;
+}
+__device__ __forceinline__ int col_of16(unsigned int h, int r) {
+    const int d = (int)(short)(unsigned short)h;
+    return d == kColPad16 ? -1 : r + d;
+}
+template <int U>
+__device__ __forceinline__ void ld_cols(const unsigned int *c, int k, int wd, int r, int (&cc)[U]) {
+    static_assert(U % 2 == 0, "pairs");
+#pragma unroll
+    for (int u = 0; u < U; u += 2) {
+        const unsigned int w = k + u < wd ? ld_nt(c + ((k + u) / 2) * kSlice) : 0x80008000u;
+        cc[u] = col_of16(w & 0xFFFFu, r);
+        cc[u + 1] = k + u + 1 < wd ? col_of16(w >> 16, r) : -1;
+    }
 }
 
 __device__ __forceinline__ unsigned short f2bf(float f) {
@@ -72,6 +99,35 @@ __device__ __forceinline__ float zero_sweep(int smoother, float w, float b, floa
 }
 
 // b = fl(r); optionally x = first sweep from zero
+// 16-bit column offsets of a SELL matrix in the paired form ld_cols reads
+// (one wave per slice; a slice of width wd takes 64 * ceil(wd / 2) words from
+// word cptr[s]); any stored entry whose offset from its row needs more sets
+// *misfit
+__global__ __launch_bounds__(kBlock) void k_col16(int nslices, const long long *__restrict__ sptr,
+                                                  const long long *__restrict__ cptr, const int *__restrict__ col,
+                                                  unsigned int *__restrict__ d, int *__restrict__ misfit) {
+    const int s = blockIdx.x * kSlicesPerBlock + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (s >= nslices) return;
+    const int r = s * kSlice + lane;
+    const long long base = sptr[s];
+    const int wd = (int)((sptr[s + 1] - base) / kSlice);
+    bool bad = false;
+    for (int k = 0; k < wd; k += 2) {
+        unsigned int w = 0;
+        for (int h = 0; h < 2; ++h) {
+            const int c = k + h < wd ? col[base + (long long)(k + h) * kSlice + lane] : -1;
+            const int off = c - r;
+            const bool fits = off >= -32767 && off <= 32767;
+            bad = bad || (c >= 0 && !fits);
+            const unsigned int half = c < 0 ? 0x8000u : (fits ? (unsigned int)(unsigned short)(short)off : 0u);
+            w |= half << (16 * h);
+        }
+        d[cptr[s] + (long long)(k / 2) * kSlice + lane] = w;
+    }
+    if (bad) *misfit = 1;
+}
+
 __global__ void k_enter(int n, const double *__restrict__ r, int smoother, float w, const float *__restrict__ d,
                         float *__restrict__ b, float *__restrict__ x, float c2, float *__restrict__ dcheb) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -192,9 +248,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep_m(int n, int nslices, const lo
     }
 }
 
-template <class MT>
+template <class MT, class CT>
 __global__ __launch_bounds__(kBlock) void k_residual_m(int n, int nslices, const long long *__restrict__ sptr,
-                                                       const int *__restrict__ col, const MT *__restrict__ val,
+                                                       const long long *__restrict__ cptr,
+                                                       const CT *__restrict__ col, const MT *__restrict__ val,
                                                        const float *__restrict__ b, const float *__restrict__ x,
                                                        float *__restrict__ w) {
     const int s = xcd_block_id() * kSlicesPerBlock + threadIdx.x / kWave;
@@ -203,19 +260,17 @@ __global__ __launch_bounds__(kBlock) void k_residual_m(int n, int nslices, const
     const int r = s * kSlice + lane;
     const long long base = sptr[s];
     const int wd = (int)((sptr[s + 1] - base) / kSlice);
-    const int *c = col + base + lane;
+    // cptr: word offsets of the slices in the paired 16-bit form
+    const CT *c = col + (cptr ? cptr[s] : base) + lane;
     const MT *v = val + base + lane;
     float acc = 0.0f;
     constexpr int U = 16;   // rounds of 16 masked past the slice width
     for (int k = 0; k < wd; k += U) {
         int cc[U];
         float vv[U], xx[U];
+        ld_cols<U>(c, k, wd, r, cc);
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const bool ok = k + u < wd;
-            cc[u] = ok ? ld_nt(c + (k + u) * kSlice) : -1;
-            vv[u] = ok ? ldv(v + (k + u) * kSlice) : 0.0f;
-        }
+        for (int u = 0; u < U; ++u) vv[u] = k + u < wd ? ldv(v + (k + u) * kSlice) : 0.0f;
 #pragma unroll
         for (int u = 0; u < U; ++u) xx[u] = cc[u] >= 0 ? x[cc[u]] : 0.0f;
 #pragma unroll
@@ -301,9 +356,10 @@ __global__ void k_prolong_m(int n, const int *__restrict__ agg, const float *__r
 // One Chebyshev step (the FP64 k_cheb_step's recurrence) in FP32:
 // dnew = c1 d + c2 (b - A x) / diag, x + dnew to xout, or on the last step of
 // the cycle to out64 (FP64) with block partials of sum_i out64_i r64_i.
-template <bool kOut64, class MT>
+template <bool kOut64, class MT, class CT = int>
 __global__ __launch_bounds__(kBlock) void k_cheb_step_m(int n, int nslices, const long long *__restrict__ sptr,
-                                                        const int *__restrict__ col, const MT *__restrict__ val,
+                                                        const long long *__restrict__ cptr,
+                                                        const CT *__restrict__ col, const MT *__restrict__ val,
                                                         const float *__restrict__ diag, float c1, float c2,
                                                         const float *__restrict__ b, const float *__restrict__ x,
                                                         float *__restrict__ xout, float *__restrict__ d,
@@ -318,21 +374,23 @@ __global__ __launch_bounds__(kBlock) void k_cheb_step_m(int n, int nslices, cons
     if (s < nslices) {
         const long long base = sptr[s];
         const int wd = (int)((sptr[s + 1] - base) / kSlice);
-        const int *c = col + base + lane;
+        const CT *c = col + (cptr ? cptr[s] : base) + lane;
         const MT *v = val + base + lane;
         float t = (r < n) ? b[r] : 0.0f;
         constexpr int U = 16;   // rounds of 16 masked past the slice width
         for (int k = 0; k < wd; k += U) {
             int cc[U];
             float vv[U], xx[U];
+            // the last step's column loads stay in the last-level cache for
+            // the PCG SpMV that follows (it shares them)
+            if constexpr (kOut64) {
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const bool ok = k + u < wd;
-                // the last step's column loads stay in the last-level cache
-                // for the PCG SpMV that follows (it shares them)
-                cc[u] = ok ? (kOut64 ? c[(k + u) * kSlice] : ld_nt(c + (k + u) * kSlice)) : -1;
-                vv[u] = ok ? ldv(v + (k + u) * kSlice) : 0.0f;
+                for (int u = 0; u < U; ++u) cc[u] = k + u < wd ? (int)c[(k + u) * kSlice] : -1;
+            } else {
+                ld_cols<U>(c, k, wd, r, cc);
             }
+#pragma unroll
+            for (int u = 0; u < U; ++u) vv[u] = k + u < wd ? ldv(v + (k + u) * kSlice) : 0.0f;
 #pragma unroll
             for (int u = 0; u < U; ++u) xx[u] = cc[u] >= 0 ? x[cc[u]] : 0.0f;
 #pragma unroll
@@ -570,7 +628,14 @@ static bool grp_level(const DevSELL &S, const DevCSR *A) {
     return A && A->ia && A->n == S.n && S.n <= 65536 && (long long)A->nnz >= 16LL * A->n;
 }
 
-void low_values(const DevSELL &S, const DevCSR &A, int kind, LowMatrix &out) {
+// matrix stream of a cycle kernel on the low-precision values: 2-byte column
+// offsets where the level has them
+static double low_matrix_bytes(const DevSELL &S, const LowMatrix &M) {
+    const int vb = M.kind == kLowF32 ? 4 : 2;
+    return M.dcol ? (double)S.nnz * (vb + 2) + 4.0 * (S.n + 1) : matrix_bytes(S.nnz, S.n, vb);
+}
+
+void low_values(const DevSELL &S, const DevCSR &A, int kind, LowMatrix &out, bool col16) {
     out.kind = kind;
     const long long ne = S.nent, nz = A.nnz;
     const int gs = (int)std::max(1LL, std::min(8192LL, (ne + kBlock - 1) / kBlock));
@@ -591,6 +656,34 @@ void low_values(const DevSELL &S, const DevCSR &A, int kind, LowMatrix &out) {
         out.csr = c;
     }
     HZ_LAUNCH_CHECK();
+    // col16: the Chebyshev steps and residuals of the cycle read 2-byte column
+    // offsets instead of the int32 columns where every offset fits (learned
+    // FP32 solve at 128^3 anisotropic 67.3 -> 62.0 ms; HZ_COL16=0 keeps the
+    // int32 ones).  The Jacobi / L1 cycle reads its level-0 columns from the
+    // last-level cache and its smaller levels through the fused restriction:
+    // building them there cost 0.8 ms of setup and saved nothing.
+    out.dcol = nullptr;
+    out.dcptr = nullptr;
+    const char *e = std::getenv("HZ_COL16");
+    if (!col16 || (e && std::atoi(e) == 0) || ne == 0 || grp_level(S, &A)) return;
+    std::vector<long long> sp(S.nslices + 1), cp(S.nslices + 1, 0);
+    HZ_CHECK(hipMemcpyAsync(sp.data(), S.sptr, sp.size() * sizeof(long long), hipMemcpyDeviceToHost, stream()));
+    HZ_CHECK(hipStreamSynchronize(stream()));
+    for (int i = 0; i < S.nslices; ++i) cp[i + 1] = cp[i] + (long long)kSlice * (((sp[i + 1] - sp[i]) / kSlice + 1) / 2);
+    long long *dcp = arena().alloc_n<long long>(cp.size());
+    unsigned int *d = arena().alloc_n<unsigned int>((size_t)cp.back() + 1);
+    int *misfit = arena().alloc_n<int>(1);
+    HZ_CHECK(hipMemcpyAsync(dcp, cp.data(), cp.size() * sizeof(long long), hipMemcpyHostToDevice, stream()));
+    HZ_CHECK(hipMemsetAsync(misfit, 0, sizeof(int), stream()));
+    k_col16<<<div_up(S.nslices, kSlicesPerBlock), kBlock, 0, stream()>>>(S.nslices, S.sptr, dcp, S.col, d, misfit);
+    HZ_LAUNCH_CHECK();
+    int h = 1;
+    HZ_CHECK(hipMemcpyAsync(&h, misfit, sizeof(int), hipMemcpyDeviceToHost, stream()));
+    HZ_CHECK(hipStreamSynchronize(stream()));
+    if (h == 0) {
+        out.dcptr = dcp;
+        out.dcol = d;
+    }
 }
 
 void d2f(const double *a, float *b, long n) {
@@ -647,7 +740,8 @@ void sweep_m(const DevSELL &S, const LowMatrix &M, int smoother, float w, const 
 
 void residual_m(const DevSELL &S, const LowMatrix &M, const float *b, const float *x, float *w, const DevCSR *A) {
     if (S.n == 0) return;
-    count_bytes(matrix_bytes(S.nnz, S.n, M.kind == kLowF32 ? 4 : 2) + 12.0 * S.n);
+    const unsigned int *dcol = M.dcol;
+    count_bytes(low_matrix_bytes(S, M) + 12.0 * S.n);
     const int grid = div_up(S.nslices, kSlicesPerBlock);
     if (grp_level(S, A)) {
         constexpr int G = 16;
@@ -658,12 +752,23 @@ void residual_m(const DevSELL &S, const LowMatrix &M, const float *b, const floa
         else
             k_residual_grp_m<G, unsigned short><<<g, kBlock, 0, stream()>>>(
                 S.n, A->ia, A->ja, static_cast<const unsigned short *>(M.csr), b, x, w);
-    } else if (M.kind == kLowF32)
-        k_residual_m<float><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col,
-                                                           static_cast<const float *>(M.sell), b, x, w);
-    else
-        k_residual_m<unsigned short><<<grid, kBlock, 0, stream()>>>(
-            S.n, S.nslices, S.sptr, S.col, static_cast<const unsigned short *>(M.sell), b, x, w);
+    } else if (M.kind == kLowF32) {
+        const float *v = static_cast<const float *>(M.sell);
+        if (dcol)
+            k_residual_m<float, unsigned int><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, M.dcptr, dcol, v,
+                                                                             b, x, w);
+        else
+            k_residual_m<float, int><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, nullptr, S.col, v, b, x,
+                                                                    w);
+    } else {
+        const unsigned short *v = static_cast<const unsigned short *>(M.sell);
+        if (dcol)
+            k_residual_m<unsigned short, unsigned int><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr,
+                                                                                      M.dcptr, dcol, v, b, x, w);
+        else
+            k_residual_m<unsigned short, int><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, nullptr, S.col,
+                                                                              v, b, x, w);
+    }
     HZ_LAUNCH_CHECK();
 }
 
@@ -724,15 +829,21 @@ void cheby_apply_m(const DevSELL &S, const DevCSR &A, const LowMatrix &M, const 
             std::swap(x, xt);
             continue;
         }
-        count_bytes(matrix_bytes(S.nnz, S.n, M.kind == kLowF32 ? 4 : 2) + 24.0 * S.n + (last64 ? 16.0 * S.n : 0.0));
+        // the last step reads the int32 columns (the PCG's product that follows
+        // shares them).  The first step to read the matrix after the PCG's
+        // product took 42.4 us on the int32 columns that product leaves in the
+        // last-level cache, 40.0 us on the offsets (128^3 level 0).
+        const unsigned int *dcol = last64 ? nullptr : M.dcol;
+        count_bytes((dcol ? low_matrix_bytes(S, M) : matrix_bytes(S.nnz, S.n, M.kind == kLowF32 ? 4 : 2)) +
+                    24.0 * S.n + (last64 ? 16.0 * S.n : 0.0));
         if (last64) {
             if (M.kind == kLowF32)
                 k_cheb_step_m<true, float><<<grid, kBlock, 0, stream()>>>(
-                    S.n, S.nslices, S.sptr, S.col, static_cast<const float *>(M.sell), diag, (float)c1, (float)c2, b,
+                    S.n, S.nslices, S.sptr, nullptr, S.col, static_cast<const float *>(M.sell), diag, (float)c1, (float)c2, b,
                     x, nullptr, dwork, out64, r64, ws.partials);
             else
                 k_cheb_step_m<true, unsigned short><<<grid, kBlock, 0, stream()>>>(
-                    S.n, S.nslices, S.sptr, S.col, static_cast<const unsigned short *>(M.sell), diag, (float)c1,
+                    S.n, S.nslices, S.sptr, nullptr, S.col, static_cast<const unsigned short *>(M.sell), diag, (float)c1,
                     (float)c2, b, x, nullptr, dwork, out64, r64, ws.partials);
             HZ_LAUNCH_CHECK();
             if (dot_out) {
@@ -749,13 +860,21 @@ void cheby_apply_m(const DevSELL &S, const DevCSR &A, const LowMatrix &M, const 
                 k_cheb_step_grp_m<G, unsigned short><<<div_up((long)S.n * G, kBlock), kBlock, 0, stream()>>>(
                     S.n, A.ia, A.ja, static_cast<const unsigned short *>(M.csr), diag, (float)c1, (float)c2, b, x, xt,
                     dwork);
+        } else if (M.kind == kLowF32 && dcol) {
+            k_cheb_step_m<false, float, unsigned int><<<grid, kBlock, 0, stream()>>>(
+                S.n, S.nslices, S.sptr, M.dcptr, dcol, static_cast<const float *>(M.sell), diag, (float)c1, (float)c2, b, x,
+                xt, dwork, nullptr, nullptr, nullptr);
         } else if (M.kind == kLowF32) {
             k_cheb_step_m<false, float><<<grid, kBlock, 0, stream()>>>(
-                S.n, S.nslices, S.sptr, S.col, static_cast<const float *>(M.sell), diag, (float)c1, (float)c2, b, x,
+                S.n, S.nslices, S.sptr, nullptr, S.col, static_cast<const float *>(M.sell), diag, (float)c1, (float)c2, b, x,
                 xt, dwork, nullptr, nullptr, nullptr);
+        } else if (dcol) {
+            k_cheb_step_m<false, unsigned short, unsigned int><<<grid, kBlock, 0, stream()>>>(
+                S.n, S.nslices, S.sptr, M.dcptr, dcol, static_cast<const unsigned short *>(M.sell), diag, (float)c1,
+                (float)c2, b, x, xt, dwork, nullptr, nullptr, nullptr);
         } else {
             k_cheb_step_m<false, unsigned short><<<grid, kBlock, 0, stream()>>>(
-                S.n, S.nslices, S.sptr, S.col, static_cast<const unsigned short *>(M.sell), diag, (float)c1,
+                S.n, S.nslices, S.sptr, nullptr, S.col, static_cast<const unsigned short *>(M.sell), diag, (float)c1,
                 (float)c2, b, x, xt, dwork, nullptr, nullptr, nullptr);
         }
         HZ_LAUNCH_CHECK();

@@ -230,3 +230,40 @@ def test_learned_fp32_cycle_reaches_fp64_tolerance(lib, libc):
     assert res[1][0] <= res[0][0] + 3 and res[2][0] <= res[0][0] + 6
     x64 = res[0][1]
     assert np.linalg.norm(res[1][1] - x64) / np.linalg.norm(x64) < 1e-8
+
+
+@pytest.mark.parametrize("precision,permute", [(1, False), (2, False), (1, True)])
+def test_fp32_cycle_16bit_column_offsets_bitexact(lib, libc, monkeypatch, precision, permute):
+    """The FP32 cycle's SELL Chebyshev steps and residuals read 16-bit column
+    offsets from the row where every stored entry's fits (padding -32768): the
+    learned mixed-precision solve (FP32 values, and BF16 below level 0) is
+    bit-identical with them switched off (HZ_COL16=0).  A random symmetric
+    permutation of a 59,319-row problem has offsets beyond the 16-bit range,
+    so its levels keep the int32 columns, with the same result."""
+    theta = learned.load_model()
+    ia, ja, va = problems.aniso3d_kuhn(40 if permute else 32, 1.1, 1e-3)
+    n = ia.size - 1
+    M = sp.csr_matrix((va, ja, ia), shape=(n, n))
+    if permute:
+        perm = np.random.default_rng(7).permutation(n)
+        M = M[perm][:, perm].tocsr()
+        M.sort_indices()
+        ia, ja, va = M.indptr.astype(ia.dtype), M.indices.astype(ja.dtype), M.data.astype(va.dtype)
+        rows = np.repeat(np.arange(n), np.diff(ia))
+        assert np.abs(ja.astype(np.int64) - rows).max() > 32767
+    b = problems.uniform_rhs(n, seed=1)
+    out = {}
+    for flag in ("0", "1"):
+        monkeypatch.setenv("HZ_COL16", flag)
+        old = _set_precision(lib, precision, cheb_coarse=2)
+        try:
+            libc.srand(1)
+            r, x, info = learned.solve_learned(lib, ia, ja, va, b, theta, budget=400, cheb_steps=4)
+        finally:
+            _restore(lib, old)
+        rel = np.linalg.norm(b - M @ x) / np.linalg.norm(b)
+        _say(f"HZ_COL16={flag} precision {precision} permuted {permute}: {r} iterations, residual {rel:.2e}")
+        assert info.fallback_used == 0 and rel < 1e-10
+        out[flag] = (r, x.copy())
+    assert out["0"][0] == out["1"][0]
+    assert np.array_equal(out["0"][1], out["1"][1])
