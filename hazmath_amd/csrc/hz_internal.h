@@ -292,9 +292,8 @@ struct LowMatrix {
     void *csr = nullptr;
     // the SELL column indices again as 16-bit offsets from the row (padding
     // kColPad16), two entries of a row to a 32-bit word, slice s from word
-    // dcptr[s], when every stored entry's fits; null otherwise
+    // sptr[s] / 2 + 32 s, when every stored entry's fits; null otherwise
     unsigned int *dcol = nullptr;
-    long long *dcptr = nullptr;
 };
 constexpr short kColPad16 = -32768;
 void low_values(const DevSELL &S, const DevCSR &A, int kind, LowMatrix &out, bool col16 = false);

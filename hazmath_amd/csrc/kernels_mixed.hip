@@ -11,7 +11,7 @@
 // uses the FP64 inverse (FP32 in and out, FP64 accumulation).
 #include <algorithm>
 #include <cstdlib>
-#include <vector>
+#include <type_traits>
 
 #include "hz_internal.h"
 
@@ -41,7 +41,14 @@ __device__ __forceinline__ float ldv(const unsigned short *p) {
 // the width wd or for padding), streamed once per launch: the int32 ones, or
 // the 16-bit offsets from the row stored two entries to a 32-bit word (entry
 // 2p of the row in the low half of word p, 2p+1 in the high half; words of a
-// slice lane-major as the SELL entries are), one load per pair.
+// slice lane-major as the SELL entries are), one load per pair.  Slice s's
+// words start at word sptr[s] / 2 + 32 s (its entries take 64 wd, its words
+// 64 ceil(wd / 2) <= 32 wd + 32): no offsets of their own.
+template <class CT>
+__device__ __forceinline__ long long col_base(long long base, int s) {
+    if constexpr (std::is_same_v<CT, unsigned int>) return base / 2 + 32LL * s;   // paired words
+    else return base;
+}
 template <int U>
 __device__ __forceinline__ void ld_cols(const int *c, int k, int wd, int, int (&cc)[U]) {
 #pragma unroll
@@ -100,12 +107,11 @@ __device__ __forceinline__ float zero_sweep(int smoother, float w, float b, floa
 
 // b = fl(r); optionally x = first sweep from zero
 // 16-bit column offsets of a SELL matrix in the paired form ld_cols reads
-// (one wave per slice; a slice of width wd takes 64 * ceil(wd / 2) words from
-// word cptr[s]); any stored entry whose offset from its row needs more sets
-// *misfit
+// (one wave per slice); any stored entry whose offset from its row needs more
+// sets *misfit
 __global__ __launch_bounds__(kBlock) void k_col16(int nslices, const long long *__restrict__ sptr,
-                                                  const long long *__restrict__ cptr, const int *__restrict__ col,
-                                                  unsigned int *__restrict__ d, int *__restrict__ misfit) {
+                                                  const int *__restrict__ col, unsigned int *__restrict__ d,
+                                                  int *__restrict__ misfit) {
     const int s = blockIdx.x * kSlicesPerBlock + threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
     if (s >= nslices) return;
@@ -123,7 +129,7 @@ __global__ __launch_bounds__(kBlock) void k_col16(int nslices, const long long *
             const unsigned int half = c < 0 ? 0x8000u : (fits ? (unsigned int)(unsigned short)(short)off : 0u);
             w |= half << (16 * h);
         }
-        d[cptr[s] + (long long)(k / 2) * kSlice + lane] = w;
+        d[col_base<unsigned int>(base, s) + (long long)(k / 2) * kSlice + lane] = w;
     }
     if (bad) *misfit = 1;
 }
@@ -250,7 +256,6 @@ __global__ __launch_bounds__(kBlock) void k_sweep_m(int n, int nslices, const lo
 
 template <class MT, class CT>
 __global__ __launch_bounds__(kBlock) void k_residual_m(int n, int nslices, const long long *__restrict__ sptr,
-                                                       const long long *__restrict__ cptr,
                                                        const CT *__restrict__ col, const MT *__restrict__ val,
                                                        const float *__restrict__ b, const float *__restrict__ x,
                                                        float *__restrict__ w) {
@@ -260,8 +265,7 @@ __global__ __launch_bounds__(kBlock) void k_residual_m(int n, int nslices, const
     const int r = s * kSlice + lane;
     const long long base = sptr[s];
     const int wd = (int)((sptr[s + 1] - base) / kSlice);
-    // cptr: word offsets of the slices in the paired 16-bit form
-    const CT *c = col + (cptr ? cptr[s] : base) + lane;
+    const CT *c = col + col_base<CT>(base, s) + lane;
     const MT *v = val + base + lane;
     float acc = 0.0f;
     constexpr int U = 16;   // rounds of 16 masked past the slice width
@@ -358,7 +362,6 @@ __global__ void k_prolong_m(int n, const int *__restrict__ agg, const float *__r
 // the cycle to out64 (FP64) with block partials of sum_i out64_i r64_i.
 template <bool kOut64, class MT, class CT = int>
 __global__ __launch_bounds__(kBlock) void k_cheb_step_m(int n, int nslices, const long long *__restrict__ sptr,
-                                                        const long long *__restrict__ cptr,
                                                         const CT *__restrict__ col, const MT *__restrict__ val,
                                                         const float *__restrict__ diag, float c1, float c2,
                                                         const float *__restrict__ b, const float *__restrict__ x,
@@ -374,7 +377,7 @@ __global__ __launch_bounds__(kBlock) void k_cheb_step_m(int n, int nslices, cons
     if (s < nslices) {
         const long long base = sptr[s];
         const int wd = (int)((sptr[s + 1] - base) / kSlice);
-        const CT *c = col + (cptr ? cptr[s] : base) + lane;
+        const CT *c = col + col_base<CT>(base, s) + lane;
         const MT *v = val + base + lane;
         float t = (r < n) ? b[r] : 0.0f;
         constexpr int U = 16;   // rounds of 16 masked past the slice width
@@ -663,27 +666,17 @@ void low_values(const DevSELL &S, const DevCSR &A, int kind, LowMatrix &out, boo
     // last-level cache and its smaller levels through the fused restriction:
     // building them there cost 0.8 ms of setup and saved nothing.
     out.dcol = nullptr;
-    out.dcptr = nullptr;
     const char *e = std::getenv("HZ_COL16");
     if (!col16 || (e && std::atoi(e) == 0) || ne == 0 || grp_level(S, &A)) return;
-    std::vector<long long> sp(S.nslices + 1), cp(S.nslices + 1, 0);
-    HZ_CHECK(hipMemcpyAsync(sp.data(), S.sptr, sp.size() * sizeof(long long), hipMemcpyDeviceToHost, stream()));
-    HZ_CHECK(hipStreamSynchronize(stream()));
-    for (int i = 0; i < S.nslices; ++i) cp[i + 1] = cp[i] + (long long)kSlice * (((sp[i + 1] - sp[i]) / kSlice + 1) / 2);
-    long long *dcp = arena().alloc_n<long long>(cp.size());
-    unsigned int *d = arena().alloc_n<unsigned int>((size_t)cp.back() + 1);
+    unsigned int *d = arena().alloc_n<unsigned int>((size_t)(S.nent / 2 + 32LL * S.nslices) + 1);
     int *misfit = arena().alloc_n<int>(1);
-    HZ_CHECK(hipMemcpyAsync(dcp, cp.data(), cp.size() * sizeof(long long), hipMemcpyHostToDevice, stream()));
     HZ_CHECK(hipMemsetAsync(misfit, 0, sizeof(int), stream()));
-    k_col16<<<div_up(S.nslices, kSlicesPerBlock), kBlock, 0, stream()>>>(S.nslices, S.sptr, dcp, S.col, d, misfit);
+    k_col16<<<div_up(S.nslices, kSlicesPerBlock), kBlock, 0, stream()>>>(S.nslices, S.sptr, S.col, d, misfit);
     HZ_LAUNCH_CHECK();
     int h = 1;
     HZ_CHECK(hipMemcpyAsync(&h, misfit, sizeof(int), hipMemcpyDeviceToHost, stream()));
     HZ_CHECK(hipStreamSynchronize(stream()));
-    if (h == 0) {
-        out.dcptr = dcp;
-        out.dcol = d;
-    }
+    if (h == 0) out.dcol = d;
 }
 
 void d2f(const double *a, float *b, long n) {
@@ -755,18 +748,18 @@ void residual_m(const DevSELL &S, const LowMatrix &M, const float *b, const floa
     } else if (M.kind == kLowF32) {
         const float *v = static_cast<const float *>(M.sell);
         if (dcol)
-            k_residual_m<float, unsigned int><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, M.dcptr, dcol, v,
+            k_residual_m<float, unsigned int><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, dcol, v,
                                                                              b, x, w);
         else
-            k_residual_m<float, int><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, nullptr, S.col, v, b, x,
+            k_residual_m<float, int><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col, v, b, x,
                                                                     w);
     } else {
         const unsigned short *v = static_cast<const unsigned short *>(M.sell);
         if (dcol)
-            k_residual_m<unsigned short, unsigned int><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr,
-                                                                                      M.dcptr, dcol, v, b, x, w);
+            k_residual_m<unsigned short, unsigned int><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, dcol, v,
+                                                                                      b, x, w);
         else
-            k_residual_m<unsigned short, int><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, nullptr, S.col,
+            k_residual_m<unsigned short, int><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col,
                                                                               v, b, x, w);
     }
     HZ_LAUNCH_CHECK();
@@ -839,11 +832,11 @@ void cheby_apply_m(const DevSELL &S, const DevCSR &A, const LowMatrix &M, const 
         if (last64) {
             if (M.kind == kLowF32)
                 k_cheb_step_m<true, float><<<grid, kBlock, 0, stream()>>>(
-                    S.n, S.nslices, S.sptr, nullptr, S.col, static_cast<const float *>(M.sell), diag, (float)c1, (float)c2, b,
+                    S.n, S.nslices, S.sptr, S.col, static_cast<const float *>(M.sell), diag, (float)c1, (float)c2, b,
                     x, nullptr, dwork, out64, r64, ws.partials);
             else
                 k_cheb_step_m<true, unsigned short><<<grid, kBlock, 0, stream()>>>(
-                    S.n, S.nslices, S.sptr, nullptr, S.col, static_cast<const unsigned short *>(M.sell), diag, (float)c1,
+                    S.n, S.nslices, S.sptr, S.col, static_cast<const unsigned short *>(M.sell), diag, (float)c1,
                     (float)c2, b, x, nullptr, dwork, out64, r64, ws.partials);
             HZ_LAUNCH_CHECK();
             if (dot_out) {
@@ -862,19 +855,19 @@ void cheby_apply_m(const DevSELL &S, const DevCSR &A, const LowMatrix &M, const 
                     dwork);
         } else if (M.kind == kLowF32 && dcol) {
             k_cheb_step_m<false, float, unsigned int><<<grid, kBlock, 0, stream()>>>(
-                S.n, S.nslices, S.sptr, M.dcptr, dcol, static_cast<const float *>(M.sell), diag, (float)c1, (float)c2, b, x,
+                S.n, S.nslices, S.sptr, dcol, static_cast<const float *>(M.sell), diag, (float)c1, (float)c2, b, x,
                 xt, dwork, nullptr, nullptr, nullptr);
         } else if (M.kind == kLowF32) {
             k_cheb_step_m<false, float><<<grid, kBlock, 0, stream()>>>(
-                S.n, S.nslices, S.sptr, nullptr, S.col, static_cast<const float *>(M.sell), diag, (float)c1, (float)c2, b, x,
+                S.n, S.nslices, S.sptr, S.col, static_cast<const float *>(M.sell), diag, (float)c1, (float)c2, b, x,
                 xt, dwork, nullptr, nullptr, nullptr);
         } else if (dcol) {
             k_cheb_step_m<false, unsigned short, unsigned int><<<grid, kBlock, 0, stream()>>>(
-                S.n, S.nslices, S.sptr, M.dcptr, dcol, static_cast<const unsigned short *>(M.sell), diag, (float)c1,
+                S.n, S.nslices, S.sptr, dcol, static_cast<const unsigned short *>(M.sell), diag, (float)c1,
                 (float)c2, b, x, xt, dwork, nullptr, nullptr, nullptr);
         } else {
             k_cheb_step_m<false, unsigned short><<<grid, kBlock, 0, stream()>>>(
-                S.n, S.nslices, S.sptr, nullptr, S.col, static_cast<const unsigned short *>(M.sell), diag, (float)c1,
+                S.n, S.nslices, S.sptr, S.col, static_cast<const unsigned short *>(M.sell), diag, (float)c1,
                 (float)c2, b, x, xt, dwork, nullptr, nullptr, nullptr);
         }
         HZ_LAUNCH_CHECK();
