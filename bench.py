@@ -705,13 +705,22 @@ def main():
                                       a.kernel_reps, C.byref(kt))
         if rc != 0:
             raise RuntimeError("hz_time_fine_kernels failed")
-    # algorithmic bytes: 12 nnz (val + col) + 4 (n+1) (row pointers) + 8 n (x) + 8 n (y)
-    bytes_spmv = 12 * nnz + 4 * (n + 1) + 16 * n
+    # algorithmic bytes: 12 nnz (val + col) + 4 (n+1) (row pointers) + 8 n (x) + 8 n (y).
+    # The solve's fine-level kernels of an FP64 Jacobi / L1 cycle read the columns
+    # as 16-bit offsets from the row, two to a 32-bit word (10 bytes an entry), on
+    # levels of 65,536 rows or more whose offsets all fit (the library's rule;
+    # HZ_COL16=0 keeps the int32 columns); the back-to-back kernels read int32.
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(ia))
+    paired = (a.precision == "fp64" and a.smoother in ("l1", "jacobi") and n >= 65536
+              and os.environ.get("HZ_COL16", "1") != "0"
+              and int(np.abs(ja.astype(np.int64) - rows).max()) <= 32767)
+    eb = 10 if paired else 12
+    bytes_spmv = eb * nnz + 4 * (n + 1) + 16 * n
     bytes_jacobi = 12 * nnz + 4 * (n + 1) + 24 * n
     # residual w = b - A x: matrix, x and b read, w written; last sweep: matrix, b (= r) and x
     # read, x' written ((x', r) needs no other array)
-    bytes_res = 12 * nnz + 4 * (n + 1) + 24 * n
-    bytes_sweep_dot = 12 * nnz + 4 * (n + 1) + 24 * n
+    bytes_res = eb * nnz + 4 * (n + 1) + 24 * n
+    bytes_sweep_dot = eb * nnz + 4 * (n + 1) + 24 * n
     spmv_gbs = bytes_spmv / (pcg_spmv_ms * 1e-3) / 1e9
     jac_gbs = bytes_jacobi / (kt.jacobi_ms * 1e-3) / 1e9 if kt.jacobi_ms > 0 else None
 
@@ -792,7 +801,9 @@ def main():
         "spmv_gbs": spmv_gbs,
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_spmv<true, true, 1>: SELL-64 SpMV t=Ap with (t,p) partials, fine level, in the PCG",
+            "kernel": ("k_spmv<true, true, 1, unsigned int>: SELL-64 SpMV t=Ap with (t,p) partials, columns as "
+                       "paired 16-bit offsets (10 bytes an entry)" if paired else
+                       "k_spmv<true, true, 1>: SELL-64 SpMV t=Ap with (t,p) partials") + ", fine level, in the PCG",
             "achieved": spmv_gbs,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -818,7 +829,8 @@ def main():
         },
         "kernels": {
             "back_to_back_spmv_ms": kt.spmv_ms,
-            "back_to_back_spmv_gbs": bytes_spmv / (kt.spmv_ms * 1e-3) / 1e9 if kt.spmv_ms > 0 else None,
+            "back_to_back_spmv_gbs": (12 * nnz + 4 * (n + 1) + 16 * n) / (kt.spmv_ms * 1e-3) / 1e9
+            if kt.spmv_ms > 0 else None,
             "spmv_dot_ms": kt.spmv_dot_ms, "residual_ms": kt.residual_ms,
             "jacobi_ms": kt.jacobi_ms, "jacobi_gbs": jac_gbs, "sell_entries": kt.sell_entries,
         },

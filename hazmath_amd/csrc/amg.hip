@@ -407,6 +407,24 @@ void amg_prepare_cycle(Hierarchy &H, const AMG_param *param) {
             }
         }
     }
+    // the FP64 Jacobi / L1 cycle and the PCG's product read 16-bit column
+    // offsets on levels of 65,536 rows or more where every offset fits: one
+    // conversion a level, one wait for all of them
+    if (cp.precision == 0 && (cp.smoother == HZ_SMOOTHER_L1DIAG || cp.smoother == HZ_SMOOTHER_JACOBI) &&
+        col16_enabled()) {
+        std::vector<int> lv;
+        for (int l = 0; l < H.nl - 1; ++l)
+            if (H.lv[l].S.n >= 65536) lv.push_back(l);
+        if (!lv.empty()) {
+            int *misfit = arena().alloc_n<int>(lv.size());
+            std::vector<unsigned int *> d(lv.size());
+            for (size_t i = 0; i < lv.size(); ++i) d[i] = col16_launch(H.lv[lv[i]].S, misfit + i);
+            std::vector<int> h(lv.size(), 1);
+            HZ_CHECK(hipMemcpyAsync(h.data(), misfit, sizeof(int) * lv.size(), hipMemcpyDeviceToHost, stream()));
+            HZ_CHECK(hipStreamSynchronize(stream()));
+            for (size_t i = 0; i < lv.size(); ++i) H.lv[lv[i]].S.pcol = h[i] == 0 ? d[i] : nullptr;
+        }
+    }
     (void)n0;
 }
 

@@ -1,6 +1,7 @@
 // hz_common.h -- shared host/device helpers of the MI355X UA-AMG-PCG library.
 #pragma once
 
+#include <type_traits>
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
@@ -152,6 +153,43 @@ __device__ __forceinline__ double final_max(const double *partials, int n, doubl
 template <class T>
 __device__ __forceinline__ T ld_nt(const T *p) {
     return __builtin_nontemporal_load(p);
+}
+
+// 16-bit column offsets from the row, two entries of a row to a 32-bit word:
+// entry 2p of the row in the low half of word p, 2p+1 in the high half, words
+// of a SELL slice lane-major as its entries are, padding kColPad16.  Slice s's
+// words start at word sptr[s] / 2 + 32 s (its entries take 64 wd, its words
+// 64 ceil(wd / 2) <= 32 wd + 32), so no offsets of their own are needed.
+constexpr int kColPad16 = -32768;
+template <class CT>
+__device__ __forceinline__ long long col_base(long long base, int s) {
+    if constexpr (std::is_same_v<CT, unsigned int>) return base / 2 + 32LL * s;   // paired words
+    else return base;
+}
+__device__ __forceinline__ int col_of16(unsigned int h, int r) {
+    const int d = (int)(short)(unsigned short)h;
+    return d == kColPad16 ? -1 : r + d;
+}
+// Column indices of a round of U SELL entries k .. k+U-1 of row r (-1 past the
+// width wd or for padding); entries from kKeepFrom on use ordinary loads (their
+// lines stay in the last-level cache), the others non-temporal ones.  int32
+// columns: a load per entry; paired offsets: a load per pair.
+template <int U, int kKeepFrom = U>
+__device__ __forceinline__ void ld_cols(const int *c, int k, int wd, int, int (&cc)[U]) {
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+        cc[u] = k + u < wd ? (u >= kKeepFrom ? c[(k + u) * 64] : ld_nt(c + (k + u) * 64)) : -1;
+}
+template <int U, int kKeepFrom = U>
+__device__ __forceinline__ void ld_cols(const unsigned int *c, int k, int wd, int r, int (&cc)[U]) {
+    static_assert(U % 2 == 0 && kKeepFrom % 2 == 0, "pairs");
+#pragma unroll
+    for (int u = 0; u < U; u += 2) {
+        const unsigned int *p = c + ((k + u) / 2) * 64;
+        const unsigned int w = k + u < wd ? (u >= kKeepFrom ? *p : ld_nt(p)) : 0x80008000u;
+        cc[u] = col_of16(w & 0xFFFFu, r);
+        cc[u + 1] = k + u + 1 < wd ? col_of16(w >> 16, r) : -1;
+    }
 }
 
 // XCD-aware block index: MI355X dispatches consecutive workgroups round-robin

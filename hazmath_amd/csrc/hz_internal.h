@@ -27,6 +27,9 @@ struct DevSELL {
     double *dl1 = nullptr;     // row l1 norms (L1DIAG diagonal), formed by sell_from_csr
     double *djac = nullptr;    // last stored diagonal entry of each row (Jacobi diagonal)
     double *dfirst = nullptr;  // first stored diagonal entry of each row (0.0 if none)
+    // the columns again as paired 16-bit offsets (col_base), when the FP64
+    // cycle's level has them; null otherwise
+    unsigned int *pcol = nullptr;
 };
 
 // Device scalars of the PCG (one 256-byte block; host copies it back once per
@@ -295,8 +298,12 @@ struct LowMatrix {
     // sptr[s] / 2 + 32 s, when every stored entry's fits; null otherwise
     unsigned int *dcol = nullptr;
 };
-constexpr short kColPad16 = -32768;
 void low_values(const DevSELL &S, const DevCSR &A, int kind, LowMatrix &out, bool col16 = false);
+// queue the conversion of S's columns into paired 16-bit offsets (col_base);
+// the device int *misfit is cleared first and set when some offset does not fit
+unsigned int *col16_launch(const DevSELL &S, int *misfit);
+// HZ_COL16=0 switches the 16-bit column offsets off
+bool col16_enabled();
 // FP32 copy of a CSR matrix's values (smoothed aggregation's P and R).
 float *values_f32(const DevCSR &A);
 // `steps` Chebyshev steps in FP32 on the level's low-precision values (x and

@@ -11,7 +11,6 @@
 // uses the FP64 inverse (FP32 in and out, FP64 accumulation).
 #include <algorithm>
 #include <cstdlib>
-#include <type_traits>
 
 #include "hz_internal.h"
 
@@ -35,38 +34,6 @@ inline int grid_for(long n) {
 __device__ __forceinline__ float ldv(const float *p) { return ld_nt(p); }
 __device__ __forceinline__ float ldv(const unsigned short *p) {
     return __uint_as_float(((unsigned int)ld_nt(p)) << 16);
-}
-
-// Column indices of a round of U SELL entries (k .. k+U-1 of row r, -1 past
-// the width wd or for padding), streamed once per launch: the int32 ones, or
-// the 16-bit offsets from the row stored two entries to a 32-bit word (entry
-// 2p of the row in the low half of word p, 2p+1 in the high half; words of a
-// slice lane-major as the SELL entries are), one load per pair.  Slice s's
-// words start at word sptr[s] / 2 + 32 s (its entries take 64 wd, its words
-// 64 ceil(wd / 2) <= 32 wd + 32): no offsets of their own.
-template <class CT>
-__device__ __forceinline__ long long col_base(long long base, int s) {
-    if constexpr (std::is_same_v<CT, unsigned int>) return base / 2 + 32LL * s;   // paired words
-    else return base;
-}
-template <int U>
-__device__ __forceinline__ void ld_cols(const int *c, int k, int wd, int, int (&cc)[U]) {
-#pragma unroll
-    for (int u = 0; u < U; ++u) cc[u] = k + u < wd ? ld_nt(c + (k + u) * kSlice) : -1;
-}
-__device__ __forceinline__ int col_of16(unsigned int h, int r) {
-    const int d = (int)(short)(unsigned short)h;
-    return d == kColPad16 ? -1 : r + d;
-}
-template <int U>
-__device__ __forceinline__ void ld_cols(const unsigned int *c, int k, int wd, int r, int (&cc)[U]) {
-    static_assert(U % 2 == 0, "pairs");
-#pragma unroll
-    for (int u = 0; u < U; u += 2) {
-        const unsigned int w = k + u < wd ? ld_nt(c + ((k + u) / 2) * kSlice) : 0x80008000u;
-        cc[u] = col_of16(w & 0xFFFFu, r);
-        cc[u + 1] = k + u + 1 < wd ? col_of16(w >> 16, r) : -1;
-    }
 }
 
 __device__ __forceinline__ unsigned short f2bf(float f) {
@@ -666,17 +633,26 @@ void low_values(const DevSELL &S, const DevCSR &A, int kind, LowMatrix &out, boo
     // last-level cache and its smaller levels through the fused restriction:
     // building them there cost 0.8 ms of setup and saved nothing.
     out.dcol = nullptr;
-    const char *e = std::getenv("HZ_COL16");
-    if (!col16 || (e && std::atoi(e) == 0) || ne == 0 || grp_level(S, &A)) return;
-    unsigned int *d = arena().alloc_n<unsigned int>((size_t)(S.nent / 2 + 32LL * S.nslices) + 1);
+    if (!col16 || !col16_enabled() || ne == 0 || grp_level(S, &A)) return;
     int *misfit = arena().alloc_n<int>(1);
-    HZ_CHECK(hipMemsetAsync(misfit, 0, sizeof(int), stream()));
-    k_col16<<<div_up(S.nslices, kSlicesPerBlock), kBlock, 0, stream()>>>(S.nslices, S.sptr, S.col, d, misfit);
-    HZ_LAUNCH_CHECK();
+    unsigned int *d = col16_launch(S, misfit);
     int h = 1;
     HZ_CHECK(hipMemcpyAsync(&h, misfit, sizeof(int), hipMemcpyDeviceToHost, stream()));
     HZ_CHECK(hipStreamSynchronize(stream()));
     if (h == 0) out.dcol = d;
+}
+
+bool col16_enabled() {
+    const char *e = std::getenv("HZ_COL16");
+    return !(e && std::atoi(e) == 0);
+}
+
+unsigned int *col16_launch(const DevSELL &S, int *misfit) {
+    unsigned int *d = arena().alloc_n<unsigned int>((size_t)(S.nent / 2 + 32LL * S.nslices) + 1);
+    HZ_CHECK(hipMemsetAsync(misfit, 0, sizeof(int), stream()));
+    k_col16<<<div_up(S.nslices, kSlicesPerBlock), kBlock, 0, stream()>>>(S.nslices, S.sptr, S.col, d, misfit);
+    HZ_LAUNCH_CHECK();
+    return d;
 }
 
 void d2f(const double *a, float *b, long n) {

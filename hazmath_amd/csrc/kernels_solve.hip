@@ -107,10 +107,12 @@ __global__ __launch_bounds__(kBlock) void k_sell_fill(int n, const int *__restri
 // kKeep 1: the second half of each round uses ordinary loads, whose lines
 // stay in the last-level cache for the next kernel that streams the same
 // matrix; kKeep 2: every load is ordinary; kKeep 4: the column indices only.
-template <bool kNtTail, bool kWide = false, int kKeep = 0>
-__device__ __forceinline__ double sell_row_dot(const int *c, const double *v, int w, const double *x) {
+template <bool kNtTail, bool kWide = false, int kKeep = 0, class CT = int>
+__device__ __forceinline__ double sell_row_dot(const CT *c, const double *v, int w, const double *x, int r) {
     // kWide: rounds of 16 masked past the slice width (a 15-entry row in one round)
     constexpr int U = kWide ? 16 : 8;
+    static_assert(kWide || std::is_same_v<CT, int>, "paired offsets: masked rounds only");
+    constexpr int kKeepFrom = (kKeep == 2 || kKeep == 4) ? 0 : (kKeep == 1 ? U / 2 : U);
     double acc = 0.0;
     int k = 0;
     for (; kWide ? (k < w) : (k + U <= w); k += U) {
@@ -120,10 +122,9 @@ __device__ __forceinline__ double sell_row_dot(const int *c, const double *v, in
         for (int u = 0; u < U; ++u) {
             const bool ok = !kWide || k + u < w;
             const bool keep = kKeep == 2 || (kKeep == 1 && u >= U / 2);
-            const bool keep_col = keep || kKeep == 4;
-            cc[u] = ok ? (keep_col ? c[(k + u) * kSlice] : ld_nt(c + (k + u) * kSlice)) : -1;
             vv[u] = ok ? (keep ? v[(k + u) * kSlice] : ld_nt(v + (k + u) * kSlice)) : 0.0;
         }
+        ld_cols<U, kKeepFrom>(c, k, w, r, cc);
 #pragma unroll
         for (int u = 0; u < U; ++u) xx[u] = cc[u] >= 0 ? x[cc[u]] : 0.0;
 #pragma unroll
@@ -138,9 +139,9 @@ __device__ __forceinline__ double sell_row_dot(const int *c, const double *v, in
 }
 
 // y = A x (row sums from 0.0 in CSR order); optional deterministic (y, x).
-template <bool kDot, bool kWide = false, int kKeep = 0>
+template <bool kDot, bool kWide = false, int kKeep = 0, class CT = int>
 __global__ __launch_bounds__(kBlock) void k_spmv(int n, int nslices, const long long *__restrict__ sptr,
-                                                 const int *__restrict__ col,
+                                                 const CT *__restrict__ col,
                                                  const double *__restrict__ val,
                                                  const double *__restrict__ x, double *__restrict__ y,
                                                  double *__restrict__ partials) {
@@ -154,9 +155,9 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n, int nslices, const long 
     if (s < nslices) {
         const long long base = sptr[s];
         const int w = (int)((sptr[s + 1] - base) / kSlice);
-        const int *c = col + base + lane;
+        const CT *c = col + col_base<CT>(base, s) + lane;
         const double *v = val + base + lane;
-        acc = sell_row_dot<true, kWide, kKeep>(c, v, w, x);
+        acc = sell_row_dot<true, kWide, kKeep>(c, v, w, x, r);
         if (r < n) {
             y[r] = acc;
             if (kDot) prod = acc * x[r];
@@ -169,9 +170,9 @@ __global__ __launch_bounds__(kBlock) void k_spmv(int n, int nslices, const long 
 }
 
 // w = b - A x  (array_cp + dcsr_aAxpy(-1.0): row sum t from 0.0, then b - t)
-template <bool kWide = false, int kKeep = 0>
+template <bool kWide = false, int kKeep = 0, class CT = int>
 __global__ __launch_bounds__(kBlock) void k_residual(int n, int nslices, const long long *__restrict__ sptr,
-                                                     const int *__restrict__ col,
+                                                     const CT *__restrict__ col,
                                                      const double *__restrict__ val,
                                                      const double *__restrict__ b,
                                                      const double *__restrict__ x,
@@ -182,9 +183,9 @@ __global__ __launch_bounds__(kBlock) void k_residual(int n, int nslices, const l
     const int r = s * kSlice + lane;
     const long long base = sptr[s];
     const int wd = (int)((sptr[s + 1] - base) / kSlice);
-    const int *c = col + base + lane;
+    const CT *c = col + col_base<CT>(base, s) + lane;
     const double *v = val + base + lane;
-    const double acc = sell_row_dot<false, kWide, kKeep>(c, v, wd, x);
+    const double acc = sell_row_dot<false, kWide, kKeep>(c, v, wd, x, r);
     if (r < n) w[r] = b[r] - acc;
 }
 
@@ -207,9 +208,9 @@ __device__ __forceinline__ double prolong_value(double xj, int g, double xcg, do
 // L1DIAG (smoother.c:324-381): t = b - sum_j a_ij x_j, d = sum_j |a_ij|; x_i += t/d.
 // Entries are consumed four at a time with all their loads issued first, so a
 // row costs a few dependent memory round trips instead of one per entry.
-template <int kSmoother, bool kProlong, bool kDot, bool kWide = false>
+template <int kSmoother, bool kProlong, bool kDot, bool kWide = false, class CT = int>
 __global__ __launch_bounds__(kBlock) void k_sweep(int n, int nslices, const long long *__restrict__ sptr,
-                                                  const int *__restrict__ col,
+                                                  const CT *__restrict__ col,
                                                   const double *__restrict__ val, double wgt,
                                                   const int *__restrict__ agg,
                                                   const double *__restrict__ xc, double alpha,
@@ -228,7 +229,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(int n, int nslices, const long
     if (s < nslices) {
         const long long base = sptr[s];
         const int wd = (int)((sptr[s + 1] - base) / kSlice);
-        const int *c = col + base + lane;
+        const CT *c = col + col_base<CT>(base, s) + lane;
         const double *v = val + base + lane;
         double t = (r < n) ? b[r] : 0.0;
         double d = 0.0;
@@ -238,6 +239,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep(int n, int nslices, const long
         // 15-entry row takes one round; otherwise 8 wide plus a tail.
         constexpr bool kMask = kProlong || kWide;
         constexpr int U = kMask ? 16 : 8;
+        static_assert(kMask || std::is_same_v<CT, int>, "paired offsets: masked rounds only");
         int k = 0;
         for (; kMask ? (k < wd) : (k + U <= wd); k += U) {
             int cc[U];
@@ -249,9 +251,9 @@ __global__ __launch_bounds__(kBlock) void k_sweep(int n, int nslices, const long
                 // second half of each round uses ordinary loads, which stay in the
                 // last-level cache for that SpMV
                 const bool keep = kWide && kDot && u >= 8;
-                cc[u] = ok ? (keep ? c[(k + u) * kSlice] : ld_nt(c + (k + u) * kSlice)) : -1;
                 vv[u] = ok ? (keep ? v[(k + u) * kSlice] : ld_nt(v + (k + u) * kSlice)) : 0.0;
             }
+            ld_cols<U, (kWide && kDot) ? 8 : U>(c, k, wd, r, cc);
 #pragma unroll
             for (int u = 0; u < U; ++u) xx[u] = cc[u] >= 0 ? x[cc[u]] : 0.0;
             if (kProlong) {
@@ -667,17 +669,29 @@ void sell_from_csr(const DevCSR &A, DevSELL &S) {
     }
 }
 
+// matrix stream of an FP64 SELL launch: 2-byte columns where it reads the
+// paired offsets
+static double sell_bytes(const DevSELL &S, bool paired) {
+    return paired ? (double)S.nnz * 10 + 4.0 * (S.n + 1) : matrix_bytes(S.nnz, S.n);
+}
+
 void spmv(const DevSELL &S, const double *x, double *y, double *dot_out, const ReduceWs &ws,
           LaunchTimer *timer, bool cols_only) {
     if (S.n == 0) return;
-    count_bytes(matrix_bytes(S.nnz, S.n) + 16.0 * S.n);
+    // the paired offsets in the PCG's product, unless it leaves the int32
+    // columns in the last-level cache for an FP32 cycle
+    const bool paired = dot_out && S.pcol && !cols_only;
+    count_bytes(sell_bytes(S, paired) + 16.0 * S.n);
     const int grid = div_up(S.nslices, kSlicesPerBlock);
     if (dot_out) {
         if (timer) timer->begin();
         // the PCG's SpMV: rounds of 16, the second half of each with ordinary
         // loads that stay in the last-level cache for the cycle's residual
         // (ahead of an FP32 cycle: every column index, no value)
-        if (cols_only)
+        if (paired)
+            k_spmv<true, true, 1, unsigned int><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.pcol,
+                                                                                  S.val, x, y, ws.partials);
+        else if (cols_only)
             k_spmv<true, true, 4><<<grid, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col, S.val, x, y,
                                                                     ws.partials);
         else
@@ -695,18 +709,22 @@ void spmv(const DevSELL &S, const double *x, double *y, double *dot_out, const R
 
 void residual(const DevSELL &S, const double *b, const double *x, double *w, LaunchTimer *timer) {
     if (S.n == 0) return;
-    count_bytes(matrix_bytes(S.nnz, S.n) + 24.0 * S.n);
+    count_bytes(sell_bytes(S, S.pcol) + 24.0 * S.n);
     // rounds of 16 masked loads: a 15-entry row in one round of loads
     if (timer) timer->begin();
+    const int g = div_up(S.nslices, kSlicesPerBlock);
     // a matrix that fits the 256 MB last-level cache with room to spare is
     // loaded with ordinary loads: the same level's post-sweep, after the
     // coarser levels' work, still finds part of it there
-    if (matrix_bytes(S.nnz, S.n) <= kCacheResidentBytes)
-        k_residual<true, 2><<<div_up(S.nslices, kSlicesPerBlock), kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr,
-                                                                                         S.col, S.val, b, x, w);
+    const bool resident = matrix_bytes(S.nnz, S.n) <= kCacheResidentBytes;
+    if (S.pcol && resident)
+        k_residual<true, 2, unsigned int><<<g, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.pcol, S.val, b, x, w);
+    else if (S.pcol)
+        k_residual<true, 0, unsigned int><<<g, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.pcol, S.val, b, x, w);
+    else if (resident)
+        k_residual<true, 2><<<g, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col, S.val, b, x, w);
     else
-        k_residual<true><<<div_up(S.nslices, kSlicesPerBlock), kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr,
-                                                                                     S.col, S.val, b, x, w);
+        k_residual<true><<<g, kBlock, 0, stream()>>>(S.n, S.nslices, S.sptr, S.col, S.val, b, x, w);
     if (timer) timer->end();
     HZ_LAUNCH_CHECK();
 }
@@ -716,10 +734,18 @@ template <bool kProlong, bool kDot>
 void launch_sweep(const DevSELL &S, int smoother, double w, const int *agg, const double *xc,
                   double alpha, const double *alpha_dev, const double *b, const double *x, double *xout,
                   double *partials) {
-    count_bytes(matrix_bytes(S.nnz, S.n) + 24.0 * S.n + (kProlong ? 4.0 * S.n : 0.0));
     const int grid = div_up(S.nslices, kSlicesPerBlock);
     const bool wide = !kProlong;   // rounds of 16 masked loads (the prolongation form has them anyway)
-    if (smoother == HZ_SMOOTHER_L1DIAG) {
+    const bool paired = wide && S.pcol;
+    count_bytes(sell_bytes(S, paired) + 24.0 * S.n + (kProlong ? 4.0 * S.n : 0.0));
+    if (paired) {
+        if (smoother == HZ_SMOOTHER_L1DIAG)
+            k_sweep<HZ_SMOOTHER_L1DIAG, kProlong, kDot, true, unsigned int><<<grid, kBlock, 0, stream()>>>(
+                S.n, S.nslices, S.sptr, S.pcol, S.val, w, agg, xc, alpha, alpha_dev, b, x, xout, partials);
+        else
+            k_sweep<HZ_SMOOTHER_JACOBI, kProlong, kDot, true, unsigned int><<<grid, kBlock, 0, stream()>>>(
+                S.n, S.nslices, S.sptr, S.pcol, S.val, w, agg, xc, alpha, alpha_dev, b, x, xout, partials);
+    } else if (smoother == HZ_SMOOTHER_L1DIAG) {
         if (wide)
             k_sweep<HZ_SMOOTHER_L1DIAG, kProlong, kDot, true><<<grid, kBlock, 0, stream()>>>(
                 S.n, S.nslices, S.sptr, S.col, S.val, w, agg, xc, alpha, alpha_dev, b, x, xout, partials);

@@ -1362,3 +1362,42 @@ def test_batch_tuner_cache_file_round_trip(lib, tmp_path):
     assert runs[1][0] == 0 and runs[1][1] == runs[0][0] + runs[0][1]
     assert np.array_equal(runs[0][2], runs[1][2])
     assert all(np.array_equal(a, b) for a, b in zip(runs[0][3], runs[1][3]))
+
+
+@pytest.mark.parametrize("smoother,permute", [("l1", False), ("jacobi", False), ("l1", True)])
+def test_fp64_cycle_16bit_column_offsets_bitexact(lib, libc, monkeypatch, smoother, permute):
+    """On levels of 65,536 rows or more the FP64 Jacobi / L1 cycle and the
+    PCG's product read the columns as 16-bit offsets from the row, two to a
+    32-bit word, where every offset fits: the 103,823-row Poisson solve is
+    bit-identical (iterations and every solution entry) with them switched off
+    (HZ_COL16=0).  A random symmetric permutation has offsets beyond the
+    16-bit range, so its levels keep the int32 columns, with the same result."""
+    import scipy.sparse as sp
+    ia, ja, va = problems.poisson3d_kuhn(48)
+    n = ia.size - 1
+    M = sp.csr_matrix((va, ja, ia), shape=(n, n))
+    if permute:
+        perm = np.random.default_rng(11).permutation(n)
+        M = M[perm][:, perm].tocsr()
+        M.sort_indices()
+        ia, ja, va = M.indptr.astype(ia.dtype), M.indices.astype(ja.dtype), M.data.astype(va.dtype)
+        rows = np.repeat(np.arange(n), np.diff(ia))
+        assert np.abs(ja.astype(np.int64) - rows).max() > 32767
+    b = problems.uniform_rhs(n, seed=4)
+    A = capi.HostCSR(ia, ja, va)
+    sm = capi.SMOOTHER_L1DIAG if smoother == "l1" else capi.SMOOTHER_JACOBI
+    kw = dict(tol=1e-10, maxit=500, aggregation=capi.HEM, smoother=sm, coarse_dof=1000, theta=0.04)
+    out = {}
+    for flag in ("0", "1"):
+        monkeypatch.setenv("HZ_COL16", flag)
+        x = capi.HostVec(np.zeros(n))
+        it, amg = capi.solver_params(lib, **kw)
+        libc.srand(1)
+        r = lib.linear_solver_dcsr_krylov_amg(A.ptr(), capi.HostVec(b).ptr(), x.ptr(), C.byref(it), C.byref(amg))
+        rel = np.linalg.norm(b - M @ x.arr) / np.linalg.norm(b)
+        print(f"[col16] HZ_COL16={flag} {smoother} permuted {permute}: {r} iterations, residual {rel:.2e}",
+              flush=True)
+        assert r > 0 and rel < 1e-10
+        out[flag] = (r, x.arr.copy())
+    assert out["0"][0] == out["1"][0]
+    assert np.array_equal(out["0"][1], out["1"][1])
