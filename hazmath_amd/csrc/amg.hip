@@ -395,6 +395,9 @@ void amg_prepare_cycle(Hierarchy &H, const AMG_param *param) {
                 d2f(L.dsm, L.dsmf, L.A.n);
                 low_values(L.S, L.A, (cp.precision == 2 && l > 0) ? kLowBF16 : kLowF32, L.low,
                            cp.smoother == HZ_SMOOTHER_CHEBY);
+                // level 0: the PCG's product reads the same offsets (the
+                // cycle's last step leaves them in the last-level cache)
+                if (l == 0 && cp.smoother == HZ_SMOOTHER_CHEBY) L.S.pcol = L.low.dcol;
                 if (L.sa) {
                     L.Pf = values_f32(L.P);
                     L.Rf = values_f32(L.R);
@@ -1002,7 +1005,7 @@ PcgResult pcg_solve(Hierarchy &H, const DevSELL &A, const double *b, double *u, 
     // before the host has read that iteration's scalars (they do not touch u or
     // r); the rare restart / convergence paths below redo or drop that work.
     while (iter++ < MaxIt) {
-        spmv(A, p, t, &H.sc->temp2, H.ws, H.spmv_timer, H.cp.precision > 0 && H.nl > 1);
+        spmv(A, p, t, &H.sc->temp2, H.ws, H.spmv_timer, H.cp.precision > 0 && H.nl > 1 && !A.pcol);
         pcg_update(m, H.sc, p, t, u, r, alpha_hist, iter, H.ws, fuse_first ? &first : nullptr);
         fetch_async(H);
         if (speculate) {

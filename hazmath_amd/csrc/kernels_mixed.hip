@@ -353,12 +353,7 @@ __global__ __launch_bounds__(kBlock) void k_cheb_step_m(int n, int nslices, cons
             float vv[U], xx[U];
             // the last step's column loads stay in the last-level cache for
             // the PCG SpMV that follows (it shares them)
-            if constexpr (kOut64) {
-#pragma unroll
-                for (int u = 0; u < U; ++u) cc[u] = k + u < wd ? (int)c[(k + u) * kSlice] : -1;
-            } else {
-                ld_cols<U>(c, k, wd, r, cc);
-            }
+            ld_cols<U, kOut64 ? 0 : U>(c, k, wd, r, cc);
 #pragma unroll
             for (int u = 0; u < U; ++u) vv[u] = k + u < wd ? ldv(v + (k + u) * kSlice) : 0.0f;
 #pragma unroll
@@ -798,13 +793,30 @@ void cheby_apply_m(const DevSELL &S, const DevCSR &A, const LowMatrix &M, const 
             std::swap(x, xt);
             continue;
         }
-        // the last step reads the int32 columns (the PCG's product that follows
-        // shares them).  The first step to read the matrix after the PCG's
+        // the last step reads the columns the PCG's product that follows reads
+        // (it finds them in the last-level cache): the offsets where the level
+        // gave them to the FP64 matrix as well, else the int32 ones.  The first step to read the matrix after the PCG's
         // product took 42.4 us on the int32 columns that product leaves in the
         // last-level cache, 40.0 us on the offsets (128^3 level 0).
-        const unsigned int *dcol = last64 ? nullptr : M.dcol;
+        const unsigned int *dcol = (last64 && !S.pcol) ? nullptr : M.dcol;
         count_bytes((dcol ? low_matrix_bytes(S, M) : matrix_bytes(S.nnz, S.n, M.kind == kLowF32 ? 4 : 2)) +
                     24.0 * S.n + (last64 ? 16.0 * S.n : 0.0));
+        if (last64 && dcol) {
+            if (M.kind == kLowF32)
+                k_cheb_step_m<true, float, unsigned int><<<grid, kBlock, 0, stream()>>>(
+                    S.n, S.nslices, S.sptr, dcol, static_cast<const float *>(M.sell), diag, (float)c1, (float)c2, b,
+                    x, nullptr, dwork, out64, r64, ws.partials);
+            else
+                k_cheb_step_m<true, unsigned short, unsigned int><<<grid, kBlock, 0, stream()>>>(
+                    S.n, S.nslices, S.sptr, dcol, static_cast<const unsigned short *>(M.sell), diag, (float)c1,
+                    (float)c2, b, x, nullptr, dwork, out64, r64, ws.partials);
+            HZ_LAUNCH_CHECK();
+            if (dot_out) {
+                k_final_sum_m<<<1, kFinBlock, 0, stream()>>>(ws.partials, grid, dot_out);
+                HZ_LAUNCH_CHECK();
+            }
+            return;
+        }
         if (last64) {
             if (M.kind == kLowF32)
                 k_cheb_step_m<true, float><<<grid, kBlock, 0, stream()>>>(
